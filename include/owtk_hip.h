@@ -1,0 +1,332 @@
+/*
+ * owtk_hip.h - C ABI of libowtk_hip.so, the gfx950 (MI355X) execution policy
+ * for the OWT-Krylov block-CSR solve path.
+ *
+ * Conventions
+ *   - Every function returns int: OWTK_OK (0) or a negative owtk_status.
+ *     owtk_last_error() returns the message of the calling thread's last
+ *     failure.  The header-only C++ layer (owtk/hip.hpp) turns
+ *     OWTK_ERROR_INVALID_ARGUMENT into std::invalid_argument and
+ *     OWTK_ERROR_ZERO_PIVOT into std::runtime_error("zero pivot in ILU(0)"),
+ *     the exception types the reference throws (block_csr.hpp:281-300,
+ *     preconditioner.hpp:451,480).
+ *   - Host arrays passed to *_create / *_update_values are borrowed for the
+ *     duration of the call only; the library keeps device copies.
+ *   - Device vectors are plain device pointers in the reference BlockVector
+ *     layout: [local node][component], owned nodes first, then ghosts
+ *     (core.hpp:193-196).  Vector element type follows the operation's dtype:
+ *     OWTK_F64 -> double, OWTK_F32 and OWTK_BF16 (matrix storage) -> float.
+ *   - Indices are int32 (row offsets and columns).
+ *   - Objects are bound to the context that created them.  All work of a
+ *     context runs on its one HIP stream, in call order.  Calls on one
+ *     context are not reentrant.
+ */
+#ifndef OWTK_HIP_H
+#define OWTK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status -- */
+typedef enum owtk_status {
+    OWTK_OK = 0,
+    OWTK_ERROR_INVALID_ARGUMENT = -1, /* std::invalid_argument (layout/storage) */
+    OWTK_ERROR_ZERO_PIVOT = -2,       /* std::runtime_error("zero pivot in ILU(0)") */
+    OWTK_ERROR_HIP = -3,              /* HIP runtime failure */
+    OWTK_ERROR_NO_DEVICE = -4,        /* no usable gfx950 device */
+    OWTK_ERROR_OUT_OF_MEMORY = -5,
+    OWTK_ERROR_UNSUPPORTED = -6,      /* option outside the implemented path */
+    OWTK_ERROR_RUNTIME = -7           /* other std::runtime_error */
+} owtk_status;
+
+typedef enum owtk_dtype {
+    OWTK_F64 = 0,
+    OWTK_F32 = 1,
+    OWTK_BF16 = 2
+} owtk_dtype;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* owtk_last_error(void);
+/* Library version string. */
+const char* owtk_version(void);
+
+/* --------------------------------------------------------------- context -- */
+typedef struct owtk_context_s* owtk_context;
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int owtk_device_count(int* count);
+/* Create a context (one HIP stream) on `device`.  Replaces the per-call data
+ * mapping of OpenMPTargetExecutionPolicy (execution.hpp:36-90) with
+ * device-resident state. */
+int owtk_context_create(int device, owtk_context* out);
+int owtk_context_destroy(owtk_context ctx);
+int owtk_context_synchronize(owtk_context ctx);
+/* Name of the policy ("hip-gfx950"), cf. HostExecutionPolicy::name()
+ * (execution.hpp:26). */
+const char* owtk_policy_name(void);
+
+/* ---------------------------------------------------------- device memory -- */
+int owtk_malloc(owtk_context ctx, size_t bytes, void** device_pointer);
+int owtk_free(owtk_context ctx, void* device_pointer);
+int owtk_memcpy_htod(owtk_context ctx, void* device_dst, const void* host_src, size_t bytes);
+int owtk_memcpy_dtoh(owtk_context ctx, void* host_dst, const void* device_src, size_t bytes);
+int owtk_memcpy_dtod(owtk_context ctx, void* device_dst, const void* device_src, size_t bytes);
+/* Zero `bytes` bytes with a fill kernel (safe inside captured graphs). */
+int owtk_fill_zero(owtk_context ctx, void* device_pointer, size_t bytes);
+/* Element conversion between vector types (double <-> float). */
+int owtk_convert(owtk_context ctx, size_t count, owtk_dtype dst_type, void* dst,
+                 owtk_dtype src_type, const void* src);
+
+/* -------------------------------------------------------------- matrices -- */
+typedef struct owtk_matrix_s* owtk_matrix;
+
+typedef enum owtk_matrix_kind {
+    OWTK_MATRIX_CSR = 0,   /* BlockCsrMatrix: component-diagonal [entry][comp] (block_csr.hpp:17-21) */
+    OWTK_MATRIX_SPLIT = 1, /* SplitBlockCsrMatrixView: diag [node][comp] + edges [edge][comp] (split_block_csr.hpp:17-28) */
+    OWTK_MATRIX_BSR = 2    /* DenseBlockCsrMatrix: row-major bs x bs blocks (dense_block_csr.hpp:15-21) */
+} owtk_matrix_kind;
+
+typedef struct owtk_matrix_info {
+    int32_t kind;
+    int32_t block_size;
+    int64_t owned_nodes;
+    int64_t ghost_nodes;
+    int64_t entries;            /* stored block entries (split: off-diagonal only) */
+    int64_t device_bytes;       /* device bytes held by the matrix (all storages) */
+    int64_t spmv_bytes_f64;     /* algorithmic HBM bytes of one FP64 SpMV */
+    int64_t spmv_bytes_f32;     /* ... with FP32 values, FP32 vectors */
+    int64_t spmv_bytes_bf16;    /* ... with BF16 values, FP32 vectors */
+    int32_t has_f32;            /* low-precision shadows present */
+    int32_t has_bf16;
+} owtk_matrix_info;
+
+/* BlockCsrMatrix(owned, ghost, block, offsets, columns, values)
+ * (block_csr.hpp:27-43).  values: entries * block doubles. */
+int owtk_matrix_create_csr(owtk_context ctx, int64_t owned_nodes, int64_t ghost_nodes,
+                           int64_t block_size, const int32_t* row_offsets,
+                           const int32_t* columns, const double* values,
+                           owtk_matrix* out);
+/* SplitBlockCsrMatrixView(owned, ghost, block, offsets, columns, diagonal,
+ * off_diagonal) (split_block_csr.hpp:32-49). */
+int owtk_matrix_create_split(owtk_context ctx, int64_t owned_nodes, int64_t ghost_nodes,
+                             int64_t block_size, const int32_t* off_diagonal_row_offsets,
+                             const int32_t* off_diagonal_columns, const double* diagonal,
+                             const double* off_diagonal, owtk_matrix* out);
+/* DenseBlockCsrMatrix(owned, ghost, block, offsets, columns, values)
+ * (dense_block_csr.hpp:26-42).  values: entries * block * block doubles. */
+int owtk_matrix_create_bsr(owtk_context ctx, int64_t owned_nodes, int64_t ghost_nodes,
+                           int64_t block_size, const int32_t* row_offsets,
+                           const int32_t* columns, const double* values,
+                           owtk_matrix* out);
+/* Numeric update with an unchanged structure.  The reference split view
+ * borrows its value arrays, so an application's edit is visible at the next
+ * apply (split_block_csr.hpp:25-28); on the device that visibility is this
+ * explicit call.  CSR/BSR: `values` only (second pointer NULL).  Split:
+ * `values` = diagonal, `off_diagonal` = edges.  Low-precision shadows are
+ * refreshed too. */
+int owtk_matrix_update_values(owtk_matrix m, const double* values, const double* off_diagonal);
+/* Add (or refresh) an FP32 or BF16 copy of the values, used by dtype-selected
+ * SpMVs and by the mixed-precision solvers. */
+int owtk_matrix_add_storage(owtk_matrix m, owtk_dtype dtype);
+int owtk_matrix_info_get(owtk_matrix m, owtk_matrix_info* info);
+int owtk_matrix_destroy(owtk_matrix m);
+
+/* y = A x (apply / apply_rows over all owned rows: block_csr.hpp:192-215,
+ * split_block_csr.hpp:89-119, dense_block_csr.hpp:173-205).  dtype selects the
+ * value storage; x has (owned+ghost)*block elements, y owned*block. */
+int owtk_spmv(owtk_matrix m, owtk_dtype dtype, const void* x, void* y);
+/* r = b - A x (detail::true_residual, krylov_solvers.hpp:62-74), FP64. */
+int owtk_residual(owtk_matrix m, const double* b, const double* x, double* r);
+
+/* -------------------------------------------------------- preconditioners -- */
+typedef struct owtk_precond_s* owtk_precond;
+
+typedef enum owtk_precond_kind {
+    OWTK_PC_IDENTITY = 0, /* IdentityPreconditioner (preconditioner.hpp:17-23) */
+    OWTK_PC_JACOBI = 1,   /* JacobiPreconditioner (preconditioner.hpp:25-117) */
+    OWTK_PC_SSOR = 2,     /* LocalSsorPreconditioner / SplitLocalSsorPreconditioner
+                             (preconditioner.hpp:124-247, split_block_csr.hpp:204-332);
+                             dense BSR: block SSOR (no reference counterpart) */
+    OWTK_PC_ILU = 3,      /* IluLevelPreconditioner / Ilu0Preconditioner
+                             (preconditioner.hpp:253-508) */
+    OWTK_PC_LEARNED = 4   /* vertex-block graph network (no reference counterpart) */
+} owtk_precond_kind;
+
+typedef struct owtk_precond_params {
+    int32_t kind;             /* owtk_precond_kind */
+    int32_t storage;          /* owtk_dtype of the preconditioner's values */
+    double omega;             /* SSOR relaxation, (0, 2) */
+    double minimum_diagonal;  /* Jacobi singular-diagonal bound; <= 0: 64*eps */
+    int64_t block_nodes;      /* block-Jacobi partition: consecutive node ranges
+                                 of this size; 0 = one block over all owned nodes */
+    int32_t fill_level;       /* ILU(k) level (0 = ILU(0)) */
+    int32_t reserved;
+    void* learned_model;      /* owtk_gnn for OWTK_PC_LEARNED */
+} owtk_precond_params;
+
+void owtk_precond_params_default(owtk_precond_params* params, int32_t kind);
+int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_precond* out);
+/* Numeric refresh from the matrix's current device values, reusing all
+ * symbolic work (update_values: preconditioner.hpp:55-71, 142-174, 269-311). */
+int owtk_precond_update_values(owtk_precond pc);
+/* z = M^{-1} r; in == out is allowed (preconditioner.hpp:332-333).  Vector
+ * type: double for F64 storage, float otherwise. */
+int owtk_precond_apply(owtk_precond pc, const void* in, void* out);
+int owtk_precond_numeric_updates(owtk_precond pc, uint64_t* count);
+typedef struct owtk_precond_info {
+    int32_t kind;
+    int32_t storage;
+    int64_t blocks;
+    int64_t forward_levels;   /* longest forward level schedule over all blocks */
+    int64_t backward_levels;
+    int64_t factor_entries;   /* stored factor entries (ILU) */
+    int64_t apply_bytes;      /* algorithmic HBM bytes of one apply */
+} owtk_precond_info;
+int owtk_precond_info_get(owtk_precond pc, owtk_precond_info* info);
+/* Copy the ILU factors to the host in the reference layout (factor_row_offsets,
+ * factor_columns, factor_values, inverse_diagonal: preconditioner.hpp:365-370).
+ * Pass NULL arrays to query sizes through *entries. */
+int owtk_precond_ilu_factors(owtk_precond pc, int64_t* entries, int64_t* row_offsets,
+                             int64_t* columns, double* values, double* inverse_diagonal);
+int owtk_precond_destroy(owtk_precond pc);
+
+/* ---------------------------------------------------------------- solvers -- */
+/* Mirrors SolverOptions<T> (core.hpp:88-111); enum values mirror
+ * GmresOrthogonalization (core.hpp:32-39).  The application callback
+ * convergence_test is not part of the device path. */
+typedef struct owtk_solver_options {
+    double relative_tolerance;
+    double absolute_tolerance;
+    double breakdown_tolerance;
+    uint64_t maximum_iterations;
+    uint64_t convergence_check_interval;
+    uint64_t restart;
+    uint64_t residual_replacement_interval;
+    int32_t gmres_orthogonalization; /* 0 iterated CGS, 1 adaptive CGS, 2 one-sync CGS */
+    int32_t collect_timings;
+} owtk_solver_options;
+
+/* Mirrors SolverResult<T> (core.hpp:118-146); status and breakdown_reason use
+ * the numeric values of SolverStatus (core.hpp:24-30) and BreakdownReason
+ * (core.hpp:41-52). */
+typedef struct owtk_solver_result {
+    int32_t status;
+    int32_t breakdown_reason;
+    uint64_t iterations;
+    uint64_t operator_applications;
+    uint64_t preconditioner_applications;
+    uint64_t global_reductions;
+    uint64_t reorthogonalizations;
+    uint64_t arnoldi_norm_verifications;
+    double initial_residual_norm;
+    double recursive_residual_norm;
+    double true_residual_norm;
+    double relative_residual_norm;
+    double solve_seconds;
+    /* iterative refinement (owtk_fgmres_refined) only */
+    uint64_t refinement_steps;
+    uint64_t inner_iterations;
+} owtk_solver_result;
+
+/* Defaults of SolverOptions<double>. */
+void owtk_solver_options_default(owtk_solver_options* options);
+
+/* bicgstab(op, b, x, options, pc) (krylov_solvers.hpp:542-745) with every
+ * vector resident on the device.  pc == NULL means IdentityPreconditioner. */
+int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                  const owtk_solver_options* options, owtk_solver_result* result);
+/* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident. */
+int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                const owtk_solver_options* options, owtk_solver_result* result);
+
+/* Mixed precision: FP64 iterative refinement around an inner FGMRES whose
+ * matrix and preconditioner values are stored in `inner_storage` (F32/BF16)
+ * with FP32 vectors and arithmetic; outer residual, update, and all inner
+ * products and norms in FP64.  No reference counterpart. */
+typedef struct owtk_refinement_options {
+    owtk_solver_options outer;   /* relative/absolute tolerance and limits of the FP64 loop */
+    int32_t inner_storage;       /* OWTK_F32 or OWTK_BF16 */
+    int32_t inner_restart;       /* inner FGMRES(m) */
+    double inner_relative_tolerance;
+    uint64_t inner_maximum_iterations;
+    uint64_t maximum_refinements;
+} owtk_refinement_options;
+void owtk_refinement_options_default(owtk_refinement_options* options);
+int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, double* x,
+                        const owtk_refinement_options* options, owtk_solver_result* result);
+
+/* --------------------------------------------------------- vector kernels -- */
+/* dot with FP64 accumulation (SerialReduction::dot, reduction.hpp:114-145). */
+int owtk_dot(owtk_context ctx, owtk_dtype dtype, size_t n, const void* a, const void* b,
+             double* out);
+/* overflow-safe 2-norm (detail::scaled_norm, reduction.hpp:64-89). */
+int owtk_norm(owtk_context ctx, owtk_dtype dtype, size_t n, const void* a, double* out);
+/* y += alpha x (simd::axpy, simd.hpp:110-202). */
+int owtk_axpy(owtk_context ctx, size_t n, double alpha, const double* x, double* y);
+/* y += alpha x, then out = <w, y> in the same pass. */
+int owtk_axpy_dot(owtk_context ctx, size_t n, double alpha, const double* x, double* y,
+                  const double* w, double* out);
+/* out[j] = <w, v_j> for j < k, out[k] = <w, w>: the batched CGS local dots
+ * (krylov_solvers.hpp:273-282). vs: host array of k device pointers. */
+int owtk_multidot(owtk_context ctx, size_t n, size_t k, const double* const* vs,
+                  const double* w, double* out);
+
+/* ------------------------------------------- synthetic grids (host only) -- */
+typedef enum owtk_ordering {
+    OWTK_ORDER_IDENTITY = 0,
+    OWTK_ORDER_RCM = 1,      /* reverse_cuthill_mckee_order (ordering.hpp:138-180) */
+    OWTK_ORDER_HILBERT = 2   /* hilbert_order (ordering.hpp:231-261) */
+} owtk_ordering;
+
+typedef enum owtk_coefficient_law {
+    OWTK_LAW_UNIFORM = 0,     /* -U(0.1,1)(1+0.3U(-1,1)) per (edge, component) */
+    OWTK_LAW_LOGNORMAL_FIELD = 1, /* times exp(sigma (g_u+g_v)/2), g a correlated unit Gaussian field */
+    OWTK_LAW_EDGE_CONTRAST = 2    /* times exp(sigma z_e), z_e iid N(0,1) per undirected edge */
+} owtk_coefficient_law;
+
+typedef struct owtk_grid_spec {
+    int32_t nx, ny;          /* nodes per direction; quads split along (i,j)-(i+1,j+1) */
+    int32_t block_size;
+    int32_t ordering;        /* owtk_ordering applied after the seeded permutation */
+    uint64_t seed;
+    int32_t permute;         /* seeded random node permutation before ordering */
+    int32_t coefficient_law; /* owtk_coefficient_law */
+    double jitter;           /* node jitter, fraction of h (0.2) */
+    double diagonal_shift;   /* diagonal = row |off-diagonal| sum + shift (0.05) */
+    double field_sigma;      /* lognormal sigma */
+    double field_correlation;/* correlation length in units of h */
+} owtk_grid_spec;
+
+void owtk_grid_spec_default(owtk_grid_spec* spec, int32_t nx, int32_t ny, int32_t block_size,
+                            uint64_t seed);
+/* Node and off-diagonal entry counts of the triangulated grid. */
+int owtk_grid_sizes(const owtk_grid_spec* spec, int64_t* nodes, int64_t* off_diagonal_entries);
+/* Split layout (also the component-diagonal CSR minus its diagonal).  Any
+ * output pointer except offsets/columns/diagonal/off_diagonal may be NULL.
+ * coordinates: 2 doubles per node; final_to_natural: natural id j*nx+i. */
+int owtk_grid_generate_split(const owtk_grid_spec* spec, int32_t* row_offsets,
+                             int32_t* columns, double* diagonal, double* off_diagonal,
+                             double* rhs, double* coordinates, int64_t* final_to_natural);
+/* Dense BSR: diagonal blocks N(0,1)+6I, off-diagonal blocks N(0,0.15)*U(0.1,1).
+ * offsets/columns include the diagonal entry of each row (sorted columns). */
+int owtk_grid_generate_bsr(const owtk_grid_spec* spec, int32_t* row_offsets,
+                           int32_t* columns, double* values, double* rhs);
+/* Merge a split layout into component-diagonal CSR with sorted columns. */
+int owtk_split_to_csr(int64_t owned_nodes, int64_t block_size, const int32_t* offsets,
+                      const int32_t* columns, const double* diagonal,
+                      const double* off_diagonal, int32_t* csr_offsets,
+                      int32_t* csr_columns, double* csr_values);
+/* Orderings (new_to_old).  rcm: graph of a CSR over owned nodes. */
+int owtk_order_rcm(int64_t n, const int32_t* row_offsets, const int32_t* columns,
+                   int64_t* new_to_old);
+int owtk_order_hilbert(int64_t n, const double* coordinates, int64_t* new_to_old);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OWTK_HIP_H */

@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports
+it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+LIB = pathlib.Path(__file__).resolve().parent / "liboracle.so"
+
+KIND_CSR, KIND_SPLIT, KIND_BSR = 0, 1, 2
+
+
+class Matrix(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("block_size", C.c_int32), ("owned_nodes", C.c_int64),
+                ("ghost_nodes", C.c_int64), ("offsets", C.c_void_p), ("columns", C.c_void_p),
+                ("values", C.c_void_p), ("diagonal", C.c_void_p)]
+
+
+class PcParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("fill_level", C.c_int32), ("omega", C.c_double),
+                ("minimum_diagonal", C.c_double), ("block_nodes", C.c_int64)]
+
+
+class Options(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_double), ("absolute_tolerance", C.c_double),
+                ("breakdown_tolerance", C.c_double), ("maximum_iterations", C.c_uint64),
+                ("convergence_check_interval", C.c_uint64), ("restart", C.c_uint64),
+                ("residual_replacement_interval", C.c_uint64), ("gmres_orthogonalization", C.c_int32),
+                ("collect_timings", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("breakdown_reason", C.c_int32), ("iterations", C.c_uint64),
+                ("operator_applications", C.c_uint64), ("preconditioner_applications", C.c_uint64),
+                ("global_reductions", C.c_uint64), ("reorthogonalizations", C.c_uint64),
+                ("arnoldi_norm_verifications", C.c_uint64), ("initial_residual_norm", C.c_double),
+                ("recursive_residual_norm", C.c_double), ("true_residual_norm", C.c_double),
+                ("relative_residual_norm", C.c_double), ("solve_seconds", C.c_double),
+                ("refinement_steps", C.c_uint64), ("inner_iterations", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise OSError(f"{LIB} missing: run `python oracle/build.py`")
+        L = C.CDLL(str(LIB))
+        P = C.c_void_p
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_spmv.argtypes = [C.POINTER(Matrix), P, P]
+        L.oracle_spmv_f32.argtypes = [C.POINTER(Matrix), P, P, P, P]
+        L.oracle_dot.restype = C.c_double
+        L.oracle_dot.argtypes = [C.c_int64, P, P]
+        L.oracle_norm.restype = C.c_double
+        L.oracle_norm.argtypes = [C.c_int64, P]
+        L.oracle_pc_create.argtypes = [C.POINTER(Matrix), C.POINTER(PcParams), C.POINTER(P)]
+        L.oracle_pc_update.argtypes = [P]
+        L.oracle_pc_apply.argtypes = [P, P, P]
+        L.oracle_pc_destroy.argtypes = [P]
+        L.oracle_pc_destroy.restype = None
+        L.oracle_ilu_factors.argtypes = [P, C.POINTER(C.c_int64), P, P, P, P]
+        L.oracle_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        L.oracle_fgmres.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"oracle error {code}: {msg}")
+        self.code = code
+
+
+def _check(code):
+    if code != 0:
+        raise OracleError(code, lib().oracle_last_error().decode())
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class OMatrix:
+    """Oracle view of a block-CSR matrix (keeps its numpy arrays alive)."""
+
+    def __init__(self, kind, owned, ghost, block, offsets, columns, values, diagonal=None):
+        self.kind, self.owned, self.ghost, self.block = kind, owned, ghost, block
+        self.offsets = np.ascontiguousarray(offsets, np.int32)
+        self.columns = np.ascontiguousarray(columns, np.int32)
+        self.values = np.ascontiguousarray(values, np.float64)
+        self.diagonal = None if diagonal is None else np.ascontiguousarray(diagonal, np.float64)
+        self.s = Matrix(kind, block, owned, ghost, self.offsets.ctypes.data, self.columns.ctypes.data,
+                        self.values.ctypes.data, 0 if self.diagonal is None else self.diagonal.ctypes.data)
+
+    @classmethod
+    def split(cls, owned, ghost, block, offsets, columns, diagonal, off_diagonal):
+        return cls(KIND_SPLIT, owned, ghost, block, offsets, columns, off_diagonal, diagonal)
+
+    @classmethod
+    def csr(cls, owned, ghost, block, offsets, columns, values):
+        return cls(KIND_CSR, owned, ghost, block, offsets, columns, values)
+
+    @classmethod
+    def bsr(cls, owned, ghost, block, offsets, columns, values):
+        return cls(KIND_BSR, owned, ghost, block, offsets, columns, values)
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.owned * self.block)
+        _check(lib().oracle_spmv(C.byref(self.s), _p(x), _p(y)))
+        return y
+
+    def spmv_f32(self, values32, diag32, x32):
+        v = np.ascontiguousarray(values32, np.float32)
+        d = None if diag32 is None else np.ascontiguousarray(diag32, np.float32)
+        x = np.ascontiguousarray(x32, np.float32)
+        y = np.zeros(self.owned * self.block, np.float32)
+        _check(lib().oracle_spmv_f32(C.byref(self.s), _p(v), _p(d), _p(x), _p(y)))
+        return y
+
+
+def dot(a, b) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().oracle_dot(a.size, _p(a), _p(b))
+
+
+def norm(a) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    return lib().oracle_norm(a.size, _p(a))
+
+
+class OPreconditioner:
+    IDENTITY, JACOBI, SSOR, ILU = 0, 1, 2, 3
+
+    def __init__(self, matrix: OMatrix, kind, omega=1.0, block_nodes=0, fill_level=0, minimum_diagonal=0.0):
+        self.matrix = matrix
+        p = PcParams(kind, fill_level, omega, minimum_diagonal, block_nodes)
+        self.h = C.c_void_p()
+        _check(lib().oracle_pc_create(C.byref(matrix.s), C.byref(p), C.byref(self.h)))
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        out = np.zeros(self.matrix.owned * self.matrix.block)
+        _check(lib().oracle_pc_apply(self.h, _p(r), _p(out)))
+        return out
+
+    def apply_inplace(self, r):
+        a = np.array(r, np.float64)
+        _check(lib().oracle_pc_apply(self.h, _p(a), _p(a)))
+        return a
+
+    def update(self):
+        _check(lib().oracle_pc_update(self.h))
+
+    def ilu_factors(self):
+        n = C.c_int64()
+        _check(lib().oracle_ilu_factors(self.h, C.byref(n), None, None, None, None))
+        rows, bs = self.matrix.owned, self.matrix.block
+        off = np.zeros(rows + 1, np.int64)
+        col = np.zeros(n.value, np.int64)
+        val = np.zeros(n.value * bs)
+        inv = np.zeros(rows * bs)
+        _check(lib().oracle_ilu_factors(self.h, C.byref(n), _p(off), _p(col), _p(val), _p(inv)))
+        return off, col, val, inv
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().oracle_pc_destroy(self.h)
+        except Exception:
+            pass
+
+
+def options(**kw) -> Options:
+    o = Options(1e-8, 0.0, 64 * np.finfo(np.float64).eps, 1000, 10, 30, 50, 0, 0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def bicgstab(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    r = Result()
+    _check(lib().oracle_bicgstab(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o), C.byref(r)))
+    return x, r.as_dict()
+
+
+def fgmres(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    r = Result()
+    _check(lib().oracle_fgmres(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o), C.byref(r)))
+    return x, r.as_dict()

@@ -1,0 +1,1013 @@
+// ORACLE - TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of the OWT-Krylov block-CSR solve path, used by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+// It is never linked into, loaded by or called from the product library.
+//
+// Each function cites the reference file:line it restates.  Operation order
+// per output element follows the reference; wherever the reference writes a
+// multiply-add statement (out += a*b, y -= a*b, ...) this file uses an
+// explicit std::fma, which is what GCC emits for those statements when FMA is
+// available (-O3 -march=native, default -ffp-contract=fast).  Built with
+// -ffp-contract=off so that no other contraction happens.
+//
+// Pinning (see tests/test_oracle.py): SpMV by the reference's closed-form
+// known answers (test_core.cpp:31-58, 176-195, 233-279) and a long-double
+// dense matvec; Jacobi/SSOR by closed forms (test_core.cpp:434-437,
+// review_cases.cpp:95-113, review_contracts.cpp:29-96); ILU(k) by a dense
+// dot-product Doolittle on the retained pattern and LU basis columns
+// (test_bicgstab_ilu_audit.cpp:14-91); BiCGSTAB by a long-double dense
+// recurrence (ilu_audit_reference.hpp:119-156) and the exact-preconditioner
+// one-iteration case; FGMRES by orthonormality/Arnoldi invariants and a
+// direct dense solve; norms by the extreme-magnitude cases
+// (test_foundation.cpp:71-107).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+
+// ------------------------------------------------------------------ matrix --
+enum Kind { kCsr = 0, kSplit = 1, kBsr = 2 };
+
+struct Matrix {
+    int kind;
+    std::int64_t owned, ghost;
+    int bs;
+    const std::int32_t* off;
+    const std::int32_t* col;
+    const double* val;   // csr: entries*bs; split: off-diagonal; bsr: entries*bs*bs
+    const double* diag;  // split only
+    std::int64_t local() const { return owned + ghost; }
+};
+
+// BlockCsrMatrix::apply_rows (block_csr.hpp:198-215)
+void spmv_csr(const Matrix& a, const double* x, double* y)
+{
+    const int bs = a.bs;
+    for (std::int64_t row = 0; row < a.owned; ++row) {
+        double* out = y + row * bs;
+        std::fill(out, out + bs, 0.0);
+        for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e) {
+            const double* in = x + static_cast<std::int64_t>(a.col[e]) * bs;
+            const double* coef = a.val + static_cast<std::int64_t>(e) * bs;
+            for (int c = 0; c < bs; ++c) out[c] = std::fma(coef[c], in[c], out[c]);
+        }
+    }
+}
+
+// SplitBlockCsrMatrixView::apply_rows (split_block_csr.hpp:95-119)
+void spmv_split(const Matrix& a, const double* x, double* y)
+{
+    const int bs = a.bs;
+    for (std::int64_t row = 0; row < a.owned; ++row) {
+        double* out = y + row * bs;
+        for (int c = 0; c < bs; ++c) out[c] = a.diag[row * bs + c] * x[row * bs + c];
+        for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e) {
+            const double* in = x + static_cast<std::int64_t>(a.col[e]) * bs;
+            const double* coef = a.val + static_cast<std::int64_t>(e) * bs;
+            for (int c = 0; c < bs; ++c) out[c] = std::fma(coef[c], in[c], out[c]);
+        }
+    }
+}
+
+// DenseBlockCsrMatrix::apply_rows (dense_block_csr.hpp:179-205)
+void spmv_bsr(const Matrix& a, const double* x, double* y)
+{
+    const int bs = a.bs;
+    for (std::int64_t row = 0; row < a.owned; ++row) {
+        double* dst = y + row * bs;
+        std::fill(dst, dst + bs, 0.0);
+        for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e) {
+            const double* src = x + static_cast<std::int64_t>(a.col[e]) * bs;
+            const double* blk = a.val + static_cast<std::int64_t>(e) * bs * bs;
+            for (int r = 0; r < bs; ++r) {
+                double t = 0.0;
+                for (int c = 0; c < bs; ++c) t = std::fma(blk[r * bs + c], src[c], t);
+                dst[r] += t;
+            }
+        }
+    }
+}
+
+void apply(const Matrix& a, const double* x, double* y)
+{
+    if (a.kind == kCsr) spmv_csr(a, x, y);
+    else if (a.kind == kSplit) spmv_split(a, x, y);
+    else spmv_bsr(a, x, y);
+}
+
+// Merged component-diagonal CSR (diagonal in sorted position) of a split or
+// csr matrix, optionally restricted to the block-diagonal of `block_nodes`
+// consecutive node ranges (the rank-local restriction the reference applies to
+// non-owned columns, preconditioner.hpp:389-391).  Ghost columns dropped.
+struct Csr {
+    std::int64_t n = 0;
+    int bs = 1;
+    std::vector<std::int32_t> off, col;
+    std::vector<double> val;
+};
+
+Csr local_csr(const Matrix& a, std::int64_t block_nodes, bool sorted = true)
+{
+    Csr out;
+    out.n = a.owned;
+    out.bs = a.bs;
+    out.off.push_back(0);
+    const int bs = a.bs;
+    auto same_block = [&](std::int64_t r, std::int64_t c) {
+        if (c >= a.owned) return false;
+        if (block_nodes <= 0) return true;
+        return r / block_nodes == c / block_nodes;
+    };
+    for (std::int64_t row = 0; row < a.owned; ++row) {
+        std::vector<std::pair<std::int32_t, std::vector<double>>> entries;
+        if (a.kind == kSplit) {
+            entries.push_back({static_cast<std::int32_t>(row),
+                               std::vector<double>(a.diag + row * bs, a.diag + row * bs + bs)});
+        }
+        for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e) {
+            if (!same_block(row, a.col[e])) continue;
+            const double* v = a.val + static_cast<std::int64_t>(e) * bs;
+            entries.push_back({a.col[e], std::vector<double>(v, v + bs)});
+        }
+        if (sorted)
+            std::stable_sort(entries.begin(), entries.end(),
+                             [](const auto& l, const auto& r) { return l.first < r.first; });
+        for (auto& [c, v] : entries) {
+            out.col.push_back(c);
+            out.val.insert(out.val.end(), v.begin(), v.end());
+        }
+        out.off.push_back(static_cast<std::int32_t>(out.col.size()));
+    }
+    return out;
+}
+
+// -------------------------------------------------------------- reductions --
+// SerialReduction::dot (reduction.hpp:114-145): eight lane accumulators.
+double dot(std::int64_t n, const double* a, const double* b)
+{
+    double partial[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::int64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+        for (int lane = 0; lane < 8; ++lane) partial[lane] = std::fma(a[i + lane], b[i + lane], partial[lane]);
+    double sum = (partial[0] + partial[1]) + (partial[2] + partial[3]) + (partial[4] + partial[5])
+        + (partial[6] + partial[7]);
+    for (; i < n; ++i) sum = std::fma(a[i], b[i], sum);
+    return sum;
+}
+
+// detail::scaled_norm (reduction.hpp:64-89)
+double norm(std::int64_t n, const double* v)
+{
+    double scale = 0;
+    for (std::int64_t i = 0; i < n; ++i) {
+        const double m = std::abs(v[i]);
+        if (std::isnan(m)) return std::numeric_limits<double>::quiet_NaN();
+        scale = std::max(scale, m);
+    }
+    if (scale == 0.0 || std::isinf(scale)) return scale;
+    double sum = 0, correction = 0;
+    for (std::int64_t i = 0; i < n; ++i) {
+        const double ratio = v[i] / scale;
+        const double term = ratio * ratio;
+        const double updated = sum + term;
+        correction += sum >= term ? (sum - updated) + term : (term - updated) + sum;
+        sum = updated;
+    }
+    return scale * std::sqrt(sum + correction);
+}
+
+// --------------------------------------------------------- preconditioners --
+struct Pc {
+    virtual ~Pc() = default;
+    virtual void apply(const double* in, double* out) = 0;
+    virtual void update() {}
+    std::uint64_t updates = 0;
+};
+
+struct Identity : Pc {
+    std::int64_t n;
+    explicit Identity(std::int64_t size) : n(size) {}
+    void apply(const double* in, double* out) override { std::copy(in, in + n, out); }  // preconditioner.hpp:17-23
+};
+
+// JacobiPreconditioner (preconditioner.hpp:25-117)
+struct Jacobi : Pc {
+    const Matrix* a;
+    double minimum;
+    std::vector<double> inv;
+    Jacobi(const Matrix* m, double min_diag) : a(m), minimum(min_diag) { update(); }
+    void update() override
+    {
+        const int bs = a->bs;
+        inv.assign(a->owned * bs, 0.0);
+        std::vector<double> d(a->owned * bs, 0.0);
+        if (a->kind == kSplit) {
+            std::copy(a->diag, a->diag + a->owned * bs, d.begin());
+        } else if (a->kind == kCsr) {
+            for (std::int64_t row = 0; row < a->owned; ++row) {
+                bool found = false;
+                for (std::int32_t e = a->off[row]; e < a->off[row + 1]; ++e) {
+                    if (a->col[e] == row) {
+                        std::copy_n(a->val + static_cast<std::int64_t>(e) * bs, bs, d.begin() + row * bs);
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found) throw std::invalid_argument("BlockCsrMatrix row has no diagonal");
+            }
+        } else {
+            throw std::invalid_argument("dense-block CSR has no component diagonal");
+        }
+        for (std::size_t i = 0; i < d.size(); ++i) {
+            if (!std::isfinite(d[i]) || std::abs(d[i]) <= minimum)
+                throw std::invalid_argument("Jacobi preconditioner has a singular diagonal");
+            inv[i] = 1.0 / d[i];
+        }
+        ++updates;
+    }
+    void apply(const double* in, double* out) override
+    {
+        for (std::size_t i = 0; i < inv.size(); ++i) out[i] = inv[i] * in[i];
+    }
+};
+
+// LocalSsorPreconditioner (preconditioner.hpp:124-247) on the merged local CSR
+// (optionally block-restricted), and SplitLocalSsorPreconditioner
+// (split_block_csr.hpp:204-332) which skips the output clear and the omega=1
+// scaling; both give the same values.
+struct Ssor : Pc {
+    const Matrix* a;
+    double omega;
+    std::int64_t block_nodes;
+    Csr c;
+    std::vector<double> diag, work;
+    Ssor(const Matrix* m, double w, std::int64_t blocks) : a(m), omega(w), block_nodes(blocks)
+    {
+        if (!(w > 0 && w < 2)) throw std::invalid_argument("SSOR omega must be in (0,2)");
+        update();
+    }
+    void update() override
+    {
+        // the sweeps visit each row's entries in the matrix's own CSR order
+        c = local_csr(*a, block_nodes, false);
+        const int bs = a->bs;
+        diag.assign(a->owned * bs, 0.0);
+        work.assign(a->owned * bs, 0.0);
+        const double tol = 64 * std::numeric_limits<double>::epsilon();
+        for (std::int64_t row = 0; row < c.n; ++row) {
+            bool found = false;
+            for (std::int32_t e = c.off[row]; e < c.off[row + 1]; ++e) {
+                if (c.col[e] != row) continue;
+                for (int k = 0; k < bs; ++k) {
+                    const double v = c.val[static_cast<std::int64_t>(e) * bs + k];
+                    if (!std::isfinite(v) || std::abs(v) <= tol)
+                        throw std::invalid_argument("SSOR preconditioner has a singular diagonal");
+                    diag[row * bs + k] = v;
+                }
+                found = true;
+                break;
+            }
+            if (!found) throw std::invalid_argument("SSOR preconditioner row has no diagonal");
+        }
+        ++updates;
+    }
+    void apply(const double* in, double* out) override
+    {
+        const int bs = a->bs;
+        const std::int64_t n = c.n;
+        for (std::int64_t row = 0; row < n; ++row) {
+            double* y = work.data() + row * bs;
+            std::copy_n(in + row * bs, bs, y);
+            for (std::int32_t e = c.off[row]; e < c.off[row + 1]; ++e) {
+                const std::int64_t column = c.col[e];
+                if (column >= row) continue;
+                const double* coef = c.val.data() + static_cast<std::int64_t>(e) * bs;
+                const double* prev = work.data() + column * bs;
+                for (int k = 0; k < bs; ++k) y[k] = std::fma(-(omega * coef[k]), prev[k], y[k]);
+            }
+            for (int k = 0; k < bs; ++k) y[k] /= diag[row * bs + k];
+        }
+        std::vector<double> z(n * bs);
+        for (std::int64_t row = n; row-- > 0;) {
+            double* zr = z.data() + row * bs;
+            const double* y = work.data() + row * bs;
+            for (int k = 0; k < bs; ++k) zr[k] = diag[row * bs + k] * y[k];
+            for (std::int32_t e = c.off[row]; e < c.off[row + 1]; ++e) {
+                const std::int64_t column = c.col[e];
+                if (column <= row) continue;
+                const double* coef = c.val.data() + static_cast<std::int64_t>(e) * bs;
+                const double* prev = z.data() + column * bs;
+                for (int k = 0; k < bs; ++k) zr[k] = std::fma(-(omega * coef[k]), prev[k], zr[k]);
+            }
+            for (int k = 0; k < bs; ++k) zr[k] /= diag[row * bs + k];
+        }
+        const double s = omega * (2.0 - omega);
+        if (omega != 1.0)
+            for (std::int64_t i = 0; i < n * bs; ++i) z[i] *= s;
+        std::copy(z.begin(), z.end(), out);
+    }
+};
+
+// IluLevelPreconditioner (preconditioner.hpp:253-500) on the merged local CSR,
+// optionally block-restricted (block-Jacobi ILU).
+struct Ilu : Pc {
+    const Matrix* a;
+    std::int64_t block_nodes;
+    std::size_t fill;
+    std::int64_t n = 0;
+    int bs = 1;
+    static constexpr std::size_t no_source = std::numeric_limits<std::size_t>::max();
+    std::vector<std::size_t> row_offsets, columns, diagonal_positions, source_positions;
+    std::vector<double> values, inverse_diagonal;
+
+    Ilu(const Matrix* m, std::int64_t blocks, int fill_level) : a(m), block_nodes(blocks), fill(fill_level)
+    {
+        const Csr c = local_csr(*a, block_nodes);
+        n = c.n;
+        bs = c.bs;
+        build_pattern(c);
+        factorize();
+        updates = 1;
+    }
+    // build_local_pattern (preconditioner.hpp:373-427)
+    void build_pattern(const Csr& c)
+    {
+        row_offsets.push_back(0);
+        diagonal_positions.resize(n);
+        std::vector<std::size_t> levels;
+        for (std::int64_t row = 0; row < n; ++row) {
+            bool diagonal_found = false;
+            std::map<std::size_t, std::pair<std::size_t, std::size_t>> entries;
+            for (std::int32_t s = c.off[row]; s < c.off[row + 1]; ++s) {
+                const std::size_t column = c.col[s];
+                if (!entries.emplace(column, std::pair{std::size_t(0), static_cast<std::size_t>(s)}).second)
+                    throw std::invalid_argument("ILU requires unique matrix columns");
+            }
+            for (auto it = entries.begin(); it != entries.end() && it->first < static_cast<std::size_t>(row); ++it) {
+                const auto pivot = it->first, lower_level = it->second.first;
+                if (lower_level >= fill) continue;
+                for (auto upper = diagonal_positions[pivot] + 1; upper < row_offsets[pivot + 1]; ++upper) {
+                    if (levels[upper] > fill - lower_level - 1) continue;
+                    const auto level = lower_level + levels[upper] + 1;
+                    auto [target, inserted] = entries.emplace(columns[upper], std::pair{level, no_source});
+                    if (!inserted) target->second.first = std::min(target->second.first, level);
+                }
+            }
+            for (const auto& [column, entry] : entries) {
+                const auto [level, source] = entry;
+                const auto destination = columns.size();
+                columns.push_back(column);
+                levels.push_back(level);
+                source_positions.push_back(source);
+                if (source == no_source) values.insert(values.end(), bs, 0.0);
+                else values.insert(values.end(), c.val.begin() + source * bs, c.val.begin() + source * bs + bs);
+                if (column == static_cast<std::size_t>(row)) {
+                    diagonal_positions[row] = destination;
+                    diagonal_found = true;
+                }
+            }
+            if (!diagonal_found) throw std::invalid_argument("ILU(0) row has no diagonal");
+            row_offsets.push_back(columns.size());
+        }
+    }
+    std::size_t find_entry(std::size_t row, std::size_t column) const
+    {
+        for (std::size_t e = row_offsets[row]; e < row_offsets[row + 1]; ++e)
+            if (columns[e] == column) return e;
+        return values.size();
+    }
+    // factorize (preconditioner.hpp:439-485)
+    void factorize()
+    {
+        const double tol = 64 * std::numeric_limits<double>::epsilon();
+        for (std::int64_t row = 0; row < n; ++row) {
+            for (std::size_t lower = row_offsets[row]; lower < diagonal_positions[row]; ++lower) {
+                const std::size_t pivot_row = columns[lower];
+                double* lv = values.data() + lower * bs;
+                const double* pivot = values.data() + diagonal_positions[pivot_row] * bs;
+                for (int k = 0; k < bs; ++k) {
+                    if (std::abs(pivot[k]) <= tol) throw std::runtime_error("zero pivot in ILU(0)");
+                    lv[k] /= pivot[k];
+                }
+                for (std::size_t upper = diagonal_positions[pivot_row] + 1; upper < row_offsets[pivot_row + 1];
+                     ++upper) {
+                    const std::size_t target = find_entry(row, columns[upper]);
+                    if (target == values.size()) continue;
+                    double* tv = values.data() + target * bs;
+                    const double* uv = values.data() + upper * bs;
+                    for (int k = 0; k < bs; ++k) tv[k] = std::fma(-lv[k], uv[k], tv[k]);
+                }
+            }
+        }
+        inverse_diagonal.assign(n * bs, 0.0);
+        for (std::int64_t row = 0; row < n; ++row) {
+            const double* d = values.data() + diagonal_positions[row] * bs;
+            for (int k = 0; k < bs; ++k) {
+                if (!std::isfinite(d[k]) || std::abs(d[k]) <= tol) throw std::runtime_error("zero pivot in ILU(0)");
+                inverse_diagonal[row * bs + k] = 1.0 / d[k];
+            }
+        }
+    }
+    // update_values (preconditioner.hpp:269-311): re-gather through the
+    // recorded source positions of the (block-)restricted matrix.
+    void update() override
+    {
+        const Csr c = local_csr(*a, block_nodes);
+        for (std::size_t d = 0; d < columns.size(); ++d) {
+            const std::size_t s = source_positions[d];
+            if (s == no_source) std::fill_n(values.begin() + d * bs, bs, 0.0);
+            else std::copy_n(c.val.begin() + s * bs, bs, values.begin() + d * bs);
+        }
+        factorize();
+        ++updates;
+    }
+    // apply (preconditioner.hpp:320-363); in == out allowed.
+    void apply(const double* in, double* out) override
+    {
+        for (std::int64_t row = 0; row < n; ++row) {
+            double* y = out + row * bs;
+            if (in != out) std::copy_n(in + row * bs, bs, y);
+            for (std::size_t e = row_offsets[row]; e < diagonal_positions[row]; ++e) {
+                const double* f = values.data() + e * bs;
+                const double* prev = out + columns[e] * bs;
+                for (int k = 0; k < bs; ++k) y[k] = std::fma(-f[k], prev[k], y[k]);
+            }
+        }
+        for (std::int64_t row = n; row-- > 0;) {
+            double* x = out + row * bs;
+            for (std::size_t e = diagonal_positions[row] + 1; e < row_offsets[row + 1]; ++e) {
+                const double* f = values.data() + e * bs;
+                const double* prev = out + columns[e] * bs;
+                for (int k = 0; k < bs; ++k) x[k] = std::fma(-f[k], prev[k], x[k]);
+            }
+            const double* inv = inverse_diagonal.data() + row * bs;
+            for (int k = 0; k < bs; ++k) x[k] *= inv[k];
+        }
+    }
+};
+
+// ----------------------------------------------------------------- solvers --
+enum Status { converged = 0, maximum_iterations = 1, breakdown = 2, diverged = 3, invalid_input = 4 };
+enum Reason {
+    none = 0,
+    arnoldi_invariant_subspace = 1,
+    singular_projected_system = 2,
+    biorthogonality_loss = 3,
+    alpha_denominator = 4,
+    omega_denominator = 5,
+    omega_zero = 6,
+    non_finite_scalar = 9,
+};
+
+} // namespace
+
+extern "C" {
+
+typedef struct oracle_options {
+    double relative_tolerance, absolute_tolerance, breakdown_tolerance;
+    std::uint64_t maximum_iterations, convergence_check_interval, restart, residual_replacement_interval;
+    std::int32_t gmres_orthogonalization, collect_timings;
+} oracle_options;
+
+typedef struct oracle_result {
+    std::int32_t status, breakdown_reason;
+    std::uint64_t iterations, operator_applications, preconditioner_applications, global_reductions,
+        reorthogonalizations, arnoldi_norm_verifications;
+    double initial_residual_norm, recursive_residual_norm, true_residual_norm, relative_residual_norm;
+    double solve_seconds;
+    std::uint64_t refinement_steps, inner_iterations;
+} oracle_result;
+
+} // extern "C"
+
+namespace {
+
+struct Solve {
+    const Matrix& a;
+    Pc& pc;
+    oracle_result& r;
+    std::int64_t n;  // owned size
+    std::int64_t local;
+    Solve(const Matrix& m, Pc& p, oracle_result& res) : a(m), pc(p), r(res), n(m.owned * m.bs), local(m.local() * m.bs) {}
+    void op(const double* x, double* y)
+    {
+        apply(a, x, y);
+        ++r.operator_applications;
+    }
+    void prec(const double* in, double* out)
+    {
+        pc.apply(in, out);
+        ++r.preconditioner_applications;
+    }
+    double d(const double* x, const double* y)
+    {
+        ++r.global_reductions;
+        return dot(n, x, y);
+    }
+    double nrm(const double* x)
+    {
+        ++r.global_reductions;
+        return norm(n, x);
+    }
+    // detail::true_residual (krylov_solvers.hpp:62-74)
+    void true_residual(const double* b, const double* x, double* res, double* work)
+    {
+        op(x, work);
+        for (std::int64_t i = 0; i < n; ++i) res[i] = b[i] - work[i];
+    }
+    void set_residual(double recursive, double truen, double rhs_norm)
+    {
+        r.recursive_residual_norm = recursive;
+        r.true_residual_norm = truen;
+        r.relative_residual_norm = truen / (rhs_norm > 0 ? rhs_norm : 1.0);
+    }
+};
+
+bool valid_options(const oracle_options& o)
+{
+    return std::isfinite(o.relative_tolerance) && std::isfinite(o.absolute_tolerance)
+        && std::isfinite(o.breakdown_tolerance) && o.relative_tolerance >= 0 && o.absolute_tolerance >= 0
+        && o.breakdown_tolerance > 0 && o.maximum_iterations > 0 && o.convergence_check_interval > 0;
+}
+
+// convergence_threshold (core.hpp:549-560)
+double threshold_of(double rhs_norm, const oracle_options& o)
+{
+    const double rel = o.relative_tolerance * (rhs_norm > 0 ? rhs_norm : 1.0);
+    if (!std::isfinite(rhs_norm) || !std::isfinite(rel)) return std::numeric_limits<double>::quiet_NaN();
+    return std::max(o.absolute_tolerance, rel);
+}
+
+void init_result(oracle_result& r)
+{
+    std::memset(&r, 0, sizeof(r));
+    r.status = invalid_input;
+    r.initial_residual_norm = r.recursive_residual_norm = r.true_residual_norm = r.relative_residual_norm =
+        std::numeric_limits<double>::quiet_NaN();
+}
+
+// bicgstab (krylov_solvers.hpp:542-745)
+void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
+{
+    init_result(r);
+    if (!valid_options(o)) return;
+    Solve s(a, pc, r);
+    const std::int64_t L = s.local;
+    std::vector<double> residual(L), shadow(L), search(L), psearch(L), osearch(L), inter(L), pinter(L), ointer(L),
+        work(L);
+    auto mark = [&](Reason reason) {
+        r.status = breakdown;
+        r.breakdown_reason = reason;
+    };
+    s.true_residual(b, x, residual.data(), work.data());
+    std::copy_n(residual.begin(), s.n, shadow.begin());
+    const double rhs_norm = s.nrm(b);
+    const double initial = s.nrm(residual.data());
+    const double thr = threshold_of(rhs_norm, o);
+    const double tiny = 1024 * std::numeric_limits<double>::min();
+    r.initial_residual_norm = initial;
+    s.set_residual(initial, initial, rhs_norm);
+    if (!std::isfinite(initial) || !std::isfinite(thr)) {
+        mark(non_finite_scalar);
+        return;
+    }
+    if (initial <= thr) {
+        r.status = converged;
+        return;
+    }
+    double rho_prev = 1, alpha = 1, omega = 1;
+    bool first = true;
+    auto restart = [&] {
+        std::copy_n(residual.begin(), s.n, shadow.begin());
+        std::fill_n(search.begin(), s.n, 0.0);
+        std::fill_n(osearch.begin(), s.n, 0.0);
+        rho_prev = alpha = omega = 1;
+        first = true;
+    };
+    for (std::uint64_t it = 1; it <= o.maximum_iterations; ++it) {
+        r.iterations = it;
+        const double rho = s.d(shadow.data(), residual.data());
+        if (!std::isfinite(rho) || std::abs(rho) <= tiny) {
+            mark(std::isfinite(rho) ? biorthogonality_loss : non_finite_scalar);
+            return;
+        }
+        if (first) {
+            std::copy_n(residual.begin(), s.n, search.begin());
+            first = false;
+        } else {
+            if (std::abs(omega) <= tiny) {
+                mark(omega_zero);
+                return;
+            }
+            const double beta = (rho / rho_prev) * (alpha / omega);
+            for (std::int64_t i = 0; i < s.n; ++i)
+                search[i] = std::fma(beta, std::fma(-omega, osearch[i], search[i]), residual[i]);
+        }
+        s.prec(search.data(), psearch.data());
+        s.op(psearch.data(), osearch.data());
+        const double sv = s.d(shadow.data(), osearch.data());
+        if (!std::isfinite(sv) || std::abs(sv) <= tiny) {
+            mark(std::isfinite(sv) ? alpha_denominator : non_finite_scalar);
+            return;
+        }
+        alpha = rho / sv;
+        for (std::int64_t i = 0; i < s.n; ++i) inter[i] = std::fma(-alpha, osearch[i], residual[i]);
+        const double inorm = s.nrm(inter.data());
+        if (inorm <= thr) {
+            for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(alpha, psearch[i], x[i]);
+            s.true_residual(b, x, residual.data(), work.data());
+            const double tn = s.nrm(residual.data());
+            s.set_residual(inorm, tn, rhs_norm);
+            if (tn <= thr) {
+                r.status = converged;
+                return;
+            }
+            restart();
+            continue;
+        }
+        s.prec(inter.data(), pinter.data());
+        s.op(pinter.data(), ointer.data());
+        const double num = s.d(ointer.data(), inter.data());
+        const double den = s.d(ointer.data(), ointer.data());
+        if (!std::isfinite(den) || std::abs(den) <= tiny) {
+            mark(std::isfinite(den) ? omega_denominator : non_finite_scalar);
+            return;
+        }
+        omega = num / den;
+        const double angle = std::abs(num) / std::sqrt(den) / inorm;
+        if (!std::isfinite(omega) || omega == 0.0 || angle <= o.breakdown_tolerance) {
+            mark(std::isfinite(omega) ? omega_zero : non_finite_scalar);
+            return;
+        }
+        for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(alpha, psearch[i], x[i]);
+        for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(omega, pinter[i], x[i]);
+        for (std::int64_t i = 0; i < s.n; ++i) residual[i] = std::fma(-omega, ointer[i], inter[i]);
+        const bool replace = o.residual_replacement_interval > 0 && it % o.residual_replacement_interval == 0;
+        const bool check = it % o.convergence_check_interval == 0;
+        if (replace || check) {
+            const double rn = s.nrm(residual.data());
+            if (replace || rn <= thr) {
+                s.true_residual(b, x, residual.data(), work.data());
+                const double tn = s.nrm(residual.data());
+                s.set_residual(rn, tn, rhs_norm);
+                if (tn <= thr) {
+                    r.status = converged;
+                    return;
+                }
+                restart();
+                continue;
+            } else {
+                r.recursive_residual_norm = rn;
+            }
+        }
+        rho_prev = rho;
+    }
+    s.true_residual(b, x, residual.data(), work.data());
+    const double tn = s.nrm(residual.data());
+    s.set_residual(r.recursive_residual_norm, tn, rhs_norm);
+    r.status = tn <= thr ? converged : maximum_iterations;
+    if (!std::isfinite(tn)) mark(non_finite_scalar);
+}
+
+// gmres_update (krylov_solvers.hpp:23-40)
+void gmres_update(std::int64_t n, const double* base, const std::vector<std::vector<double>>& dirs,
+                  const std::vector<double>& coef, std::size_t count, double* out)
+{
+    for (std::int64_t i = 0; i < n; ++i) {
+        double sum = 0, comp = 0;
+        for (std::size_t j = 0; j < count; ++j) {
+            const double term = coef[j] * dirs[j][i];
+            const double updated = sum + term;
+            comp += std::abs(sum) >= std::abs(term) ? (sum - updated) + term : (term - updated) + sum;
+            sum = updated;
+        }
+        out[i] = base[i] + (sum + comp);
+    }
+}
+
+// fgmres (krylov_solvers.hpp:159-486)
+void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
+{
+    init_result(r);
+    if (!valid_options(o) || o.restart == 0) return;
+    Solve s(a, pc, r);
+    const std::size_t m = o.restart;
+    const std::int64_t L = s.local;
+    std::vector<double> residual(L), work(L);
+    std::vector<std::vector<double>> V(m + 1, std::vector<double>(L)), Z(m, std::vector<double>(L));
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), glob(m + 2), coef(m + 1);
+    auto h = [&](std::size_t row, std::size_t col) -> double& { return H[col * (m + 1) + row]; };
+    auto mark = [&](Reason reason) {
+        r.status = breakdown;
+        r.breakdown_reason = reason;
+    };
+    s.true_residual(b, x, residual.data(), work.data());
+    const double rhs_norm = s.nrm(b);
+    const double initial = s.nrm(residual.data());
+    const double thr = threshold_of(rhs_norm, o);
+    r.initial_residual_norm = initial;
+    s.set_residual(initial, initial, rhs_norm);
+    if (!std::isfinite(initial) || !std::isfinite(thr)) {
+        mark(non_finite_scalar);
+        return;
+    }
+    if (initial <= thr) {
+        r.status = converged;
+        return;
+    }
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double dmin = std::numeric_limits<double>::min();
+    while (r.iterations < o.maximum_iterations) {
+        const double beta = s.nrm(residual.data());
+        for (std::int64_t i = 0; i < s.n; ++i) V[0][i] = residual[i] / beta;
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        std::size_t cols = 0;
+        bool abreak = false;
+        for (; cols < m && r.iterations < o.maximum_iterations; ++cols, ++r.iterations) {
+            s.prec(V[cols].data(), Z[cols].data());
+            s.op(Z[cols].data(), work.data());
+            const std::size_t count = cols + 1;
+            for (std::size_t row = 0; row < count; ++row) glob[row] = dot(s.n, work.data(), V[row].data());
+            glob[count] = dot(s.n, work.data(), work.data());
+            ++r.global_reductions;
+            const double original = glob[count];
+            double projected = original;
+            for (std::size_t row = 0; row < count; ++row) {
+                h(row, cols) = glob[row];
+                for (std::int64_t i = 0; i < s.n; ++i) work[i] = std::fma(-h(row, cols), V[row][i], work[i]);
+                projected -= h(row, cols) * h(row, cols);
+            }
+            const double nonneg = std::max(projected, 0.0);
+            const bool forced = o.gmres_orthogonalization == 0;
+            const bool adaptive = o.gmres_orthogonalization == 1 && (projected <= 0 || nonneg < 0.5 * original);
+            double nsq = nonneg;
+            if (forced || adaptive) {
+                for (std::size_t row = 0; row < count; ++row) glob[row] = dot(s.n, work.data(), V[row].data());
+                glob[count] = dot(s.n, work.data(), work.data());
+                ++r.global_reductions;
+                ++r.reorthogonalizations;
+                nsq = glob[count];
+                for (std::size_t row = 0; row < count; ++row) {
+                    const double corr = glob[row];
+                    h(row, cols) += corr;
+                    for (std::int64_t i = 0; i < s.n; ++i) work[i] = std::fma(-corr, V[row][i], work[i]);
+                    nsq -= corr * corr;
+                }
+            }
+            if (original < dmin || !std::isfinite(nsq) || nsq <= eps * original || (nsq > 0 && nsq < dmin)) {
+                h(cols + 1, cols) = s.nrm(work.data());
+                ++r.arnoldi_norm_verifications;
+            } else {
+                h(cols + 1, cols) = std::sqrt(std::max(nsq, 0.0));
+            }
+            if (!std::isfinite(h(cols + 1, cols))) {
+                mark(non_finite_scalar);
+                return;
+            }
+            double cscale = 0;
+            for (std::size_t row = 0; row <= cols + 1; ++row) cscale = std::max(cscale, std::abs(h(row, cols)));
+            if (h(cols + 1, cols) > o.breakdown_tolerance * cscale) {
+                const double inv = 1.0 / h(cols + 1, cols);
+                for (std::int64_t i = 0; i < s.n; ++i) V[cols + 1][i] = work[i] * inv;
+            } else {
+                abreak = true;
+            }
+            for (std::size_t row = 0; row < cols; ++row) {
+                const double first = cs[row] * h(row, cols) + sn[row] * h(row + 1, cols);
+                h(row + 1, cols) = -sn[row] * h(row, cols) + cs[row] * h(row + 1, cols);
+                h(row, cols) = first;
+            }
+            const double den = std::hypot(h(cols, cols), h(cols + 1, cols));
+            if (!std::isfinite(den)) {
+                mark(non_finite_scalar);
+                return;
+            }
+            if (den == 0) {
+                abreak = true;
+                cs[cols] = 1;
+                sn[cols] = 0;
+            } else {
+                cs[cols] = h(cols, cols) / den;
+                sn[cols] = h(cols + 1, cols) / den;
+            }
+            h(cols, cols) = cs[cols] * h(cols, cols) + sn[cols] * h(cols + 1, cols);
+            h(cols + 1, cols) = 0;
+            g[cols + 1] = -sn[cols] * g[cols];
+            g[cols] *= cs[cols];
+            const double est = std::abs(g[cols + 1]);
+            r.recursive_residual_norm = est;
+            if (est <= thr || abreak) {
+                ++cols;
+                ++r.iterations;
+                break;
+            }
+        }
+        std::fill(coef.begin(), coef.end(), 0.0);
+        for (std::size_t rev = cols; rev-- > 0;) {
+            double value = g[rev];
+            for (std::size_t c = rev + 1; c < cols; ++c) value -= h(rev, c) * coef[c];
+            double cscale = 0;
+            for (std::size_t row = 0; row <= rev; ++row) cscale = std::max(cscale, std::abs(h(row, rev)));
+            if (std::abs(h(rev, rev)) <= o.breakdown_tolerance * cscale) {
+                mark(singular_projected_system);
+                return;
+            }
+            coef[rev] = value / h(rev, rev);
+        }
+        gmres_update(s.n, x, Z, coef, cols, x);
+        s.true_residual(b, x, residual.data(), work.data());
+        const double tn = s.nrm(residual.data());
+        s.set_residual(r.recursive_residual_norm, tn, rhs_norm);
+        if (!std::isfinite(tn)) {
+            mark(non_finite_scalar);
+            return;
+        }
+        if (tn <= thr) {
+            r.status = converged;
+            return;
+        }
+        if (abreak && !(tn < beta)) {
+            mark(arnoldi_invariant_subspace);
+            return;
+        }
+    }
+    r.status = maximum_iterations;
+}
+
+template<class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        g_error.clear();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return -1;
+    } catch (const std::runtime_error& e) {
+        g_error = e.what();
+        return std::string(e.what()).find("zero pivot") != std::string::npos ? -2 : -7;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -7;
+    }
+}
+
+} // namespace
+
+// ------------------------------------------------------------------- C ABI --
+extern "C" {
+
+typedef struct oracle_matrix {
+    std::int32_t kind;
+    std::int32_t block_size;
+    std::int64_t owned_nodes, ghost_nodes;
+    const std::int32_t* offsets;
+    const std::int32_t* columns;
+    const double* values;
+    const double* diagonal;
+} oracle_matrix;
+
+static Matrix to_matrix(const oracle_matrix* m)
+{
+    if (!m || m->owned_nodes <= 0 || m->block_size <= 0) throw std::invalid_argument("invalid matrix");
+    return Matrix{m->kind, m->owned_nodes, m->ghost_nodes, m->block_size, m->offsets, m->columns, m->values, m->diagonal};
+}
+
+const char* oracle_last_error(void) { return g_error.c_str(); }
+
+int oracle_spmv(const oracle_matrix* m, const double* x, double* y)
+{
+    return guard([&] { apply(to_matrix(m), x, y); });
+}
+
+// Low-precision SpMV: values (and split diagonal) already rounded to the
+// storage type and given as float, vectors float, float fma accumulation.
+int oracle_spmv_f32(const oracle_matrix* m, const float* values, const float* diagonal, const float* x, float* y)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        const int bs = a.bs;
+        for (std::int64_t row = 0; row < a.owned; ++row) {
+            if (a.kind == kBsr) {
+                float acc[64] = {0};
+                for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e)
+                    for (int r = 0; r < bs; ++r) {
+                        float t = 0.0f;
+                        for (int c = 0; c < bs; ++c)
+                            t = std::fma(values[(static_cast<std::int64_t>(e) * bs + r) * bs + c],
+                                         x[static_cast<std::int64_t>(a.col[e]) * bs + c], t);
+                        acc[r] += t;
+                    }
+                for (int r = 0; r < bs; ++r) y[row * bs + r] = acc[r];
+                continue;
+            }
+            for (int c = 0; c < bs; ++c) {
+                float acc = a.kind == kSplit ? diagonal[row * bs + c] * x[row * bs + c] : 0.0f;
+                for (std::int32_t e = a.off[row]; e < a.off[row + 1]; ++e)
+                    acc = std::fma(values[static_cast<std::int64_t>(e) * bs + c],
+                                   x[static_cast<std::int64_t>(a.col[e]) * bs + c], acc);
+                y[row * bs + c] = acc;
+            }
+        }
+    });
+}
+
+double oracle_dot(std::int64_t n, const double* a, const double* b) { return dot(n, a, b); }
+double oracle_norm(std::int64_t n, const double* a) { return norm(n, a); }
+
+typedef struct oracle_pc_params {
+    std::int32_t kind;     // 0 identity, 1 jacobi, 2 ssor, 3 ilu
+    std::int32_t fill_level;
+    double omega;
+    double minimum_diagonal;
+    std::int64_t block_nodes;
+} oracle_pc_params;
+
+struct OraclePcHandle {
+    Matrix matrix;
+    std::unique_ptr<Pc> pc;
+};
+
+int oracle_pc_create(const oracle_matrix* m, const oracle_pc_params* p, void** out)
+{
+    return guard([&] {
+        if (!p || !out) throw std::invalid_argument("null argument");
+        auto h = std::make_unique<OraclePcHandle>();
+        h->matrix = to_matrix(m);
+        const double mind = p->minimum_diagonal > 0 ? p->minimum_diagonal : 64 * std::numeric_limits<double>::epsilon();
+        switch (p->kind) {
+        case 0: h->pc = std::make_unique<Identity>(h->matrix.owned * h->matrix.bs); break;
+        case 1: h->pc = std::make_unique<Jacobi>(&h->matrix, mind); break;
+        case 2: h->pc = std::make_unique<Ssor>(&h->matrix, p->omega, p->block_nodes); break;
+        case 3: h->pc = std::make_unique<Ilu>(&h->matrix, p->block_nodes, p->fill_level); break;
+        default: throw std::invalid_argument("unknown preconditioner");
+        }
+        *out = h.release();
+    });
+}
+
+int oracle_pc_update(void* handle)
+{
+    return guard([&] { static_cast<OraclePcHandle*>(handle)->pc->update(); });
+}
+
+int oracle_pc_apply(void* handle, const double* in, double* out)
+{
+    return guard([&] { static_cast<OraclePcHandle*>(handle)->pc->apply(in, out); });
+}
+
+void oracle_pc_destroy(void* handle) { delete static_cast<OraclePcHandle*>(handle); }
+
+// ILU factors in the reference's accessor layout (preconditioner.hpp:365-370).
+int oracle_ilu_factors(void* handle, std::int64_t* entries, std::int64_t* offsets, std::int64_t* columns,
+                       double* values, double* inverse_diagonal)
+{
+    return guard([&] {
+        auto* ilu = dynamic_cast<Ilu*>(static_cast<OraclePcHandle*>(handle)->pc.get());
+        if (!ilu) throw std::invalid_argument("not an ILU preconditioner");
+        *entries = static_cast<std::int64_t>(ilu->columns.size());
+        if (offsets) std::copy(ilu->row_offsets.begin(), ilu->row_offsets.end(), offsets);
+        if (columns) std::copy(ilu->columns.begin(), ilu->columns.end(), columns);
+        if (values) std::copy(ilu->values.begin(), ilu->values.end(), values);
+        if (inverse_diagonal) std::copy(ilu->inverse_diagonal.begin(), ilu->inverse_diagonal.end(), inverse_diagonal);
+    });
+}
+
+int oracle_bicgstab(const oracle_matrix* m, void* pc_handle, const double* b, double* x, const oracle_options* o,
+                    oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        bicgstab(a, pc, b, x, *o, *r);
+    });
+}
+
+int oracle_fgmres(const oracle_matrix* m, void* pc_handle, const double* b, double* x, const oracle_options* o,
+                  oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        fgmres(a, pc, b, x, *o, *r);
+    });
+}
+
+} // extern "C"

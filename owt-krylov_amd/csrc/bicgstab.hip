@@ -1,0 +1,449 @@
+// Device-resident BiCGSTAB (bicgstab, krylov_solvers.hpp:542-745).
+//
+// Every vector and scalar of the recurrence stays on the GPU.  The seven
+// launches of an iteration carry their reductions in the same pass:
+//   p   = r + beta (p - omega v)                      (beta from the state)
+//   z   = M^-1 p
+//   v   = A z            + <shadow, v>   -> alpha
+//   s   = r - alpha v    + ||s||         -> early-exit test
+//   y   = M^-1 s
+//   t   = A y            + <t, s>, <t,t> -> omega
+//   x  += alpha z; x += omega y; r = s - omega t + <shadow, r>, ||r||
+//                                        -> next rho, cadence test
+// Scalar logic (breakdown guards, cadence, counters) runs in the finalizer of
+// the kernel's last block.  Once a stop condition is recorded every later
+// launch returns immediately, so the device state equals the reference's at
+// the point it would have returned.  The host synchronises only at the
+// reference's convergence-check/replacement cadence (every
+// convergence_check_interval iterations by default) and on stops.
+#include "precond.hpp"
+#include "solver_common.hpp"
+#include "kernels/blas1.hpp"
+#include "kernels/spmv.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <limits>
+
+using namespace owtk;
+
+namespace owtk {
+namespace {
+
+using dev::kBlock;
+using dev::ReduceScratch;
+
+enum Stop : int { stop_none = 0, stop_breakdown = 1, stop_early = 2, stop_cadence = 3, stop_max = 4 };
+
+struct BicgState {
+    double rho, rho_prev, alpha, omega, beta;
+    double threshold, tiny, breakdown_tolerance;
+    double snorm, rnorm, recursive;
+    double norm_out;
+    long long iteration, max_iterations, check_interval, replace_interval;
+    int first;
+    int stop;
+    int stage;
+    int reason;
+    long long stop_iteration;
+};
+
+__device__ void fin_rho(BicgState* st, double rho, long long it)
+{
+    if (!__builtin_isfinite(rho) || __builtin_fabs(rho) <= st->tiny) {
+        st->stop = stop_breakdown;
+        st->reason = __builtin_isfinite(rho) ? reason_biorthogonality_loss : reason_non_finite_scalar;
+        st->stage = 0;
+        st->stop_iteration = it;
+        return;
+    }
+    if (!st->first) {
+        if (__builtin_fabs(st->omega) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = reason_omega_zero;
+            st->stage = 0;
+            st->stop_iteration = it;
+            return;
+        }
+        st->beta = (rho / st->rho_prev) * (st->alpha / st->omega);
+    }
+    st->rho = rho;
+}
+
+// <shadow, r> at (re)start
+__global__ __launch_bounds__(256) void k_rho(const double* __restrict__ shadow, const double* __restrict__ r,
+                                             std::int64_t n, BicgState* st, ReduceScratch scratch)
+{
+    double red[1] = {0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        red[0] = __builtin_fma(shadow[i], r[i], red[0]);
+    dev::grid_reduce<1>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x == 0) fin_rho(st, t[0], st->iteration);
+    });
+}
+
+// p = r (first) or p = r + beta (p - omega v)
+__global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const double* __restrict__ r,
+                                                  const double* __restrict__ v, std::int64_t n, const BicgState* st)
+{
+    if (st->stop) return;
+    const bool first = st->first != 0;
+    const double beta = st->beta, omega = st->omega;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        p[i] = first ? r[i] : __builtin_fma(beta, __builtin_fma(-omega, v[i], p[i]), r[i]);
+}
+
+// s = r - alpha v, ||s||
+__global__ __launch_bounds__(256) void k_s_update(double* __restrict__ s, const double* __restrict__ r,
+                                                  const double* __restrict__ v, std::int64_t n, BicgState* st,
+                                                  ReduceScratch scratch)
+{
+    if (st->stop) return;
+    const double alpha = st->alpha;
+    double red[2] = {0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double si = __builtin_fma(-alpha, v[i], r[i]);
+        s[i] = si;
+        red[0] = __builtin_fma(si, si, red[0]);
+        red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(si));
+    }
+    dev::grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+        const double nrm = dev::block_norm(t[0], t[1], s, n);
+        if (threadIdx.x == 0) {
+            st->snorm = nrm;
+            if (nrm <= st->threshold) {
+                st->stop = stop_early;
+                st->stop_iteration = st->iteration;
+            }
+        }
+    });
+}
+
+// x += alpha z; x += omega y; r = s - omega t; <shadow, r>, ||r||
+__global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, double* __restrict__ r,
+                                                   const double* __restrict__ z, const double* __restrict__ y,
+                                                   const double* __restrict__ s, const double* __restrict__ t,
+                                                   const double* __restrict__ shadow, std::int64_t n, BicgState* st,
+                                                   ReduceScratch scratch)
+{
+    if (st->stop) return;
+    const double alpha = st->alpha, omega = st->omega;
+    double red[3] = {0.0, 0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double xi = __builtin_fma(alpha, z[i], x[i]);
+        xi = __builtin_fma(omega, y[i], xi);
+        x[i] = xi;
+        const double ri = __builtin_fma(-omega, t[i], s[i]);
+        r[i] = ri;
+        red[0] = __builtin_fma(shadow[i], ri, red[0]);
+        red[1] = __builtin_fma(ri, ri, red[1]);
+        red[2] = dev::combine<dev::Combine::max>(red[2], __builtin_fabs(ri));
+    }
+    dev::grid_reduce<3>(red, 4, scratch, [&](const double* tot) {
+        const long long it = st->iteration;
+        const bool replace = st->replace_interval > 0 && it % st->replace_interval == 0;
+        const bool check = it % st->check_interval == 0;
+        double rn = 0.0;
+        if (replace || check) rn = dev::block_norm(tot[1], tot[2], r, n);
+        if (threadIdx.x != 0) return;
+        if (replace || check) {
+            st->rnorm = rn;
+            if (replace || rn <= st->threshold) {
+                st->stop = stop_cadence;
+                st->stop_iteration = it;
+                return;
+            }
+            st->recursive = rn;
+        }
+        st->rho_prev = st->rho;
+        if (it >= st->max_iterations) {
+            st->stop = stop_max;
+            st->stop_iteration = it;
+            return;
+        }
+        st->iteration = it + 1;
+        fin_rho(st, tot[0], it + 1);
+    });
+}
+
+// v = A z epilogue: <shadow, v> -> alpha
+struct FinAlpha {
+    BicgState* st;
+    __device__ void operator()(double sv) const
+    {
+        if (!__builtin_isfinite(sv) || __builtin_fabs(sv) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(sv) ? reason_alpha_denominator : reason_non_finite_scalar;
+            st->stage = 1;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        st->alpha = st->rho / sv;
+        st->first = 0;
+    }
+};
+
+// t = A y epilogue: <t, s>, <t, t> -> omega
+struct FinOmega {
+    BicgState* st;
+    __device__ void operator()(double num, double den) const
+    {
+        if (!__builtin_isfinite(den) || __builtin_fabs(den) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(den) ? reason_omega_denominator : reason_non_finite_scalar;
+            st->stage = 3;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        const double omega = num / den;
+        st->omega = omega;
+        const double angle = __builtin_fabs(num) / __builtin_sqrt(den) / st->snorm;
+        if (!__builtin_isfinite(omega) || omega == 0.0 || angle <= st->breakdown_tolerance) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(omega) ? reason_omega_zero : reason_non_finite_scalar;
+            st->stage = 3;
+            st->stop_iteration = st->iteration;
+        }
+    }
+};
+
+struct Workspace {
+    DeviceArray<double> vecs;  // 8 owned-size vectors (+ ghosts for SpMV inputs)
+    DeviceArray<BicgState> state;
+    std::int64_t local = 0;
+    double* v(int k) const { return vecs.get() + static_cast<std::int64_t>(k) * local; }
+};
+
+} // namespace
+
+int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* o,
+                 owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    init_result(res);
+    if (!valid_options(o)) return OWTK_OK;
+    if (pc && pc->m->owned * pc->m->bs != m->owned * m->bs)
+        throw std::invalid_argument("preconditioner layout mismatch");
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
+        throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const auto scratch = ctx->scratch();
+    const std::int64_t n = m->owned * m->bs;
+    const std::int64_t L = m->local() * m->bs;
+    Workspace w;
+    w.local = L;
+    w.vecs.allocate(static_cast<std::size_t>(9 * L));
+    kern::fill_zero_bytes(st, w.vecs.get(), w.vecs.bytes());
+    w.state.allocate(1);
+    double* r = w.v(0);
+    double* shadow = w.v(1);
+    double* p = w.v(2);
+    double* z = w.v(3);
+    double* v = w.v(4);
+    double* s = w.v(5);
+    double* yv = w.v(6);
+    double* t = w.v(7);
+    BicgState* dst = w.state.get();
+    auto* hst = static_cast<BicgState*>(ctx->pinned.ptr);
+    ctx->pinned.ensure(sizeof(BicgState) + 64);
+    hst = static_cast<BicgState*>(ctx->pinned.ptr);
+    std::uint64_t ops = 0, pcs = 0, reds = 0;
+
+    auto true_residual = [&](double* norm_slot) {
+        dev::EpiResidual e;
+        e.y = r;
+        e.b = b;
+        e.n = n;
+        e.norm_out = norm_slot;
+        spmv_f64(m, x, e);
+        ++ops;
+        ++reds;
+    };
+    auto read_state = [&] {
+        OWTK_HIP(hipMemcpyAsync(hst, dst, sizeof(BicgState), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+    };
+    auto write_state = [&] { OWTK_HIP(hipMemcpyAsync(dst, hst, sizeof(BicgState), hipMemcpyHostToDevice, st)); };
+
+    // r = b - A x; shadow = r; ||b||, ||r||
+    std::memset(hst, 0, sizeof(BicgState));
+    write_state();
+    double* d_norms = ctx->scalars.get();
+    true_residual(&d_norms[1]);
+    OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    kern::norm_to(st, scratch, b, n, &d_norms[0]);
+    ++reds;
+    double norms[2];
+    OWTK_HIP(hipMemcpyAsync(norms, d_norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double rhs_norm = norms[0], initial = norms[1];
+    const double thr = threshold_of(rhs_norm, o);
+    res->initial_residual_norm = initial;
+    set_residual(res, initial, initial, rhs_norm);
+    auto finish = [&](int status) {
+        res->status = status;
+        res->operator_applications = ops;
+        res->preconditioner_applications = pcs;
+        res->global_reductions = reds;
+        if (o->collect_timings)
+            res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return OWTK_OK;
+    };
+    if (!std::isfinite(initial) || !std::isfinite(thr)) {
+        res->breakdown_reason = reason_non_finite_scalar;
+        return finish(status_breakdown);
+    }
+    if (initial <= thr) return finish(status_converged);
+
+    auto restart_state = [&](long long it) {
+        hst->rho_prev = 1;
+        hst->alpha = 1;
+        hst->omega = 1;
+        hst->first = 1;
+        hst->stop = stop_none;
+        hst->iteration = it;
+        write_state();
+        OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        kern::fill_zero_bytes(st, p, n * sizeof(double));
+        kern::fill_zero_bytes(st, v, n * sizeof(double));
+        k_rho<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+        OWTK_LAUNCH_CHECK();
+    };
+    std::memset(hst, 0, sizeof(BicgState));
+    hst->threshold = thr;
+    hst->tiny = 1024 * std::numeric_limits<double>::min();
+    hst->breakdown_tolerance = o->breakdown_tolerance;
+    hst->recursive = initial;
+    hst->max_iterations = static_cast<long long>(o->maximum_iterations);
+    hst->check_interval = static_cast<long long>(o->convergence_check_interval);
+    hst->replace_interval = static_cast<long long>(o->residual_replacement_interval);
+    restart_state(1);
+
+    const int* stop = &dst->stop;
+    const unsigned int g = dev::stream_grid(n, 4);
+    auto enqueue_iteration = [&] {
+        k_p_update<<<g, kBlock, 0, st>>>(p, r, v, n, dst);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, p, z, n, st, stop);
+        {
+            dev::EpiDot1<FinAlpha> e;
+            e.y = v;
+            e.w = shadow;
+            e.stop = stop;
+            e.fin = FinAlpha{dst};
+            spmv_f64(m, z, e);
+        }
+        k_s_update<<<g, kBlock, 0, st>>>(s, r, v, n, dst, scratch);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, s, yv, n, st, stop);
+        {
+            dev::EpiDot2<FinOmega> e;
+            e.y = t;
+            e.w = s;
+            e.stop = stop;
+            e.fin = FinOmega{dst};
+            spmv_f64(m, yv, e);
+        }
+        k_xr_update<<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+        OWTK_LAUNCH_CHECK();
+    };
+    const long long maxit = static_cast<long long>(o->maximum_iterations);
+    const long long check = static_cast<long long>(o->convergence_check_interval);
+    const long long replace = static_cast<long long>(o->residual_replacement_interval);
+    auto is_cadence = [&](long long it) { return (replace > 0 && it % replace == 0) || it % check == 0; };
+    auto count_full = [&](long long a, long long b_incl) {
+        for (long long k = a; k <= b_incl; ++k) {
+            pcs += 2;
+            ops += 2;
+            reds += 5 + (is_cadence(k) ? 1 : 0);
+        }
+    };
+
+    long long it = 1;
+    while (true) {
+        long long end = it;
+        while (end < maxit && !is_cadence(end)) ++end;
+        for (long long k = it; k <= end; ++k) enqueue_iteration();
+        read_state();
+        const int code = hst->stop;
+        const long long j = hst->stop_iteration;
+        if (code == stop_none) {
+            // cadence iteration passed without a stop
+            count_full(it, end);
+            res->iterations = static_cast<std::uint64_t>(end);
+            res->recursive_residual_norm = hst->recursive;
+            it = end + 1;
+            continue;
+        }
+        if (code == stop_breakdown) {
+            count_full(it, j - 1);
+            if (hst->stage == 0) reds += 1;
+            else if (hst->stage == 1) { pcs += 1; ops += 1; reds += 2; }
+            else { pcs += 2; ops += 2; reds += 5; }
+            res->iterations = static_cast<std::uint64_t>(j);
+            res->breakdown_reason = hst->reason;
+            res->recursive_residual_norm = hst->recursive;
+            return finish(status_breakdown);
+        }
+        if (code == stop_max) {
+            count_full(it, j);
+            res->iterations = static_cast<std::uint64_t>(j);
+            res->recursive_residual_norm = hst->recursive;
+            break;
+        }
+        // early (s small) or cadence: x/true residual then converge or restart
+        double rec;
+        if (code == stop_early) {
+            count_full(it, j - 1);
+            pcs += 1;
+            ops += 1;
+            reds += 3;
+            kern::axpy(st, n, hst->alpha, z, x);
+            rec = hst->snorm;
+        } else {
+            count_full(it, j);
+            rec = hst->rnorm;
+        }
+        res->iterations = static_cast<std::uint64_t>(j);
+        true_residual(&d_norms[2]);
+        double tn;
+        OWTK_HIP(hipMemcpyAsync(&norms[0], &d_norms[2], sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        tn = norms[0];
+        set_residual(res, rec, tn, rhs_norm);
+        hst->recursive = rec;
+        if (tn <= thr) return finish(status_converged);
+        it = j + 1;
+        if (it > maxit) break;
+        restart_state(it);
+    }
+    // final true residual (krylov_solvers.hpp:735-744)
+    true_residual(&d_norms[3]);
+    OWTK_HIP(hipMemcpyAsync(&norms[0], &d_norms[3], sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double tn = norms[0];
+    set_residual(res, res->recursive_residual_norm, tn, rhs_norm);
+    int status = tn <= thr ? status_converged : status_maximum_iterations;
+    if (!std::isfinite(tn)) {
+        status = status_breakdown;
+        res->breakdown_reason = reason_non_finite_scalar;
+    }
+    return finish(status);
+}
+
+} // namespace owtk
+
+extern "C" int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                             const owtk_solver_options* options, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        bicgstab_run(m, pc, b, x, options, result);
+    });
+}

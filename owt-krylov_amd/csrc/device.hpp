@@ -1,0 +1,227 @@
+// Device utilities for gfx950 (wave64): block reductions with a deterministic
+// "last block finishes" step, number formats, and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace owtk::dev {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // threads per block for streaming kernels
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxReduceValues = 64; // values one reduction kernel may carry
+constexpr int kMaxReduceBlocks = 16384;
+
+// ------------------------------------------------------------ number types --
+struct bf16 {
+    std::uint16_t bits;
+};
+
+__host__ __device__ inline float bf16_to_float(std::uint16_t b)
+{
+    union {
+        std::uint32_t u;
+        float f;
+    } v;
+    v.u = static_cast<std::uint32_t>(b) << 16;
+    return v.f;
+}
+
+// Round to nearest even; NaN stays NaN.
+__host__ __device__ inline std::uint16_t float_to_bf16(float f)
+{
+    union {
+        std::uint32_t u;
+        float f;
+    } v;
+    v.f = f;
+    if ((v.u & 0x7fffffffU) > 0x7f800000U) return static_cast<std::uint16_t>((v.u >> 16) | 0x40U);
+    const std::uint32_t rounding = 0x7fffU + ((v.u >> 16) & 1U);
+    return static_cast<std::uint16_t>((v.u + rounding) >> 16);
+}
+
+template<class V> __device__ inline float load_as_float(const V* p, std::int64_t i);
+template<> __device__ inline float load_as_float<float>(const float* p, std::int64_t i) { return p[i]; }
+template<> __device__ inline float load_as_float<std::uint16_t>(const std::uint16_t* p, std::int64_t i)
+{
+    return bf16_to_float(p[i]);
+}
+
+__device__ inline double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// ------------------------------------------------------------- reductions --
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v = fmax(v, __shfl_xor(v, offset, kWave));
+    return v;
+}
+
+// How a reduced value is combined: sum, or max of magnitudes (NaN-aware: a NaN
+// anywhere yields NaN).
+enum class Combine { sum, max };
+
+template<Combine C>
+__device__ inline double combine(double a, double b)
+{
+    if constexpr (C == Combine::sum) {
+        return a + b;
+    } else {
+        if (a != a || b != b) return __builtin_nan("");
+        return a > b ? a : b;
+    }
+}
+
+template<Combine C>
+__device__ inline double wave_combine(double v)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v = combine<C>(v, __shfl_xor(v, offset, kWave));
+    return v;
+}
+
+// Reduce v[0..K) over the block; thread 0 gets the block totals in v.
+// Bit k of max_mask set: value k is a max of magnitudes, else a sum.
+template<int K>
+__device__ inline void block_reduce(double (&v)[K], std::uint64_t max_mask)
+{
+    __shared__ double lds[kWavesPerBlock][K > 0 ? K : 1];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = ((max_mask >> k) & 1) ? wave_combine<Combine::max>(v[k])
+                                     : wave_combine<Combine::sum>(v[k]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double acc = lds[0][k];
+            for (int w = 1; w < kWavesPerBlock; ++w)
+                acc = ((max_mask >> k) & 1) ? combine<Combine::max>(acc, lds[w][k])
+                                            : combine<Combine::sum>(acc, lds[w][k]);
+            v[k] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+// Scratch of one context for cross-block reductions.
+struct ReduceScratch {
+    double* partials;      // [kMaxReduceValues][kMaxReduceBlocks]
+    unsigned int* ticket;  // one counter, returned to zero by the last block
+};
+
+// Every block calls this with its per-thread values.  The block partials go to
+// scratch; the last block to arrive sums all partials in a fixed order (so the
+// result is deterministic for a fixed grid) and then calls fin(totals) with
+// ALL threads of that block (totals in LDS, identical on every thread), so a
+// finalizer may run block-cooperative work; single-thread updates inside fin
+// must be guarded by threadIdx.x == 0.  Only the last block returns from here
+// past the early exit.
+template<int K, class Fin>
+__device__ inline void grid_reduce(double (&v)[K], std::uint64_t max_mask, ReduceScratch scratch,
+                                   Fin&& fin)
+{
+    __shared__ bool is_last;
+    __shared__ double totals[K > 0 ? K : 1];
+    block_reduce<K>(v, max_mask);
+    const int nblocks = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) scratch.partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
+        __threadfence();
+        const unsigned int arrived = atomicAdd(scratch.ticket, 1U);
+        is_last = arrived == static_cast<unsigned int>(nblocks - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool is_max = (max_mask >> k) & 1;
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            const double p = scratch.partials[k * kMaxReduceBlocks + b];
+            a = is_max ? combine<Combine::max>(a, p) : combine<Combine::sum>(a, p);
+        }
+        acc[k] = a;
+    }
+    block_reduce<K>(acc, max_mask);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) totals[k] = acc[k];
+        *scratch.ticket = 0U;
+    }
+    __syncthreads();
+    fin(static_cast<const double*>(totals));
+}
+
+// Sum over the block, result on every thread.
+__device__ inline double block_sum_all(double v)
+{
+    __shared__ double lds[kWavesPerBlock];
+    __shared__ double total;
+    v = wave_combine<Combine::sum>(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = lds[0];
+        for (int w = 1; w < kWavesPerBlock; ++w) acc += lds[w];
+        total = acc;
+    }
+    __syncthreads();
+    return total;
+}
+
+// 2-norm from a fused (sum of squares, max magnitude) pair, overflow and
+// underflow safe with the semantics of detail::scaled_norm
+// (reduction.hpp:64-89): NaN anywhere -> NaN, otherwise an infinite entry ->
+// inf, all zero -> 0.  When the plain sum of squares over- or underflowed,
+// the calling block re-reads v once and sums (v_i / max)^2.  Call with all
+// threads of one block; every thread gets the result.
+constexpr double kSafeSumOfSquares = 0x1.0p-960;
+
+template<class X>
+__device__ inline double block_norm(double sum_squares, double max_abs, const X* v, std::int64_t n)
+{
+    if (max_abs != max_abs) return __builtin_nan("");
+    if (max_abs == 0.0) return 0.0;
+    if (__builtin_isinf(max_abs)) return max_abs;
+    if (__builtin_isfinite(sum_squares) && sum_squares >= kSafeSumOfSquares) return __builtin_sqrt(sum_squares);
+    double acc = 0.0;
+    for (std::int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double r = static_cast<double>(v[i]) / max_abs;
+        acc = __builtin_fma(r, r, acc);
+    }
+    return max_abs * __builtin_sqrt(block_sum_all(acc));
+}
+
+// Grid for streaming kernels over n items with `per_thread` items per thread.
+inline unsigned int stream_grid(std::int64_t n, int per_thread = 1)
+{
+    std::int64_t blocks = (n + static_cast<std::int64_t>(kBlock) * per_thread - 1)
+        / (static_cast<std::int64_t>(kBlock) * per_thread);
+    if (blocks < 1) blocks = 1;
+    if (blocks > kMaxReduceBlocks) blocks = kMaxReduceBlocks;
+    return static_cast<unsigned int>(blocks);
+}
+
+} // namespace owtk::dev

@@ -1,0 +1,755 @@
+// Device-resident flexible restarted GMRES (fgmres, krylov_solvers.hpp:159-486)
+// and the mixed-precision iterative-refinement driver built on it.
+//
+// One Arnoldi column is five launches:
+//   z_k = M^-1 v_k ; w = A z_k
+//   A: <w, v_0..v_k>, <w, w>                  -> h(:,k), projected norm
+//   B: w -= sum_j h_j v_j  (+ second CGS pass dots when re-orthogonalising)
+//   C: w -= sum_j c_j v_j, then v_{k+1} = w / h_{k+1,k} (or the verified-norm
+//      path of the reference), Givens rotation, residual estimate
+//   D: scale after a verified norm (no-op otherwise)
+// Hessenberg matrix, rotations and flags live in a device state; the host
+// reads it once per restart cycle for the back-substitution.  The engine is a
+// template on the vector type: double for the FP64 solver, float for the inner
+// solve of the refinement driver (inner products and the Hessenberg stay FP64).
+#include "precond.hpp"
+#include "solver_common.hpp"
+#include "kernels/blas1.hpp"
+#include "kernels/spmv.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <vector>
+
+using namespace owtk;
+
+namespace owtk {
+
+namespace {
+
+using dev::kBlock;
+using dev::ReduceScratch;
+
+constexpr int kMaxRestart = 60;
+constexpr int kLd = kMaxRestart + 1;
+
+struct FgState {
+    double H[kLd * kMaxRestart];
+    double cs[kMaxRestart], sn[kMaxRestart], g[kLd];
+    double corr[kLd];
+    double original, projected, nsq, inv_h, threshold, breakdown_tolerance, recursive;
+    int mode;
+    int halt;
+    int stop;
+    int reason;
+    int cycle_done;
+    int abreak;
+    int columns;
+    int reorth_count, verif_count;
+    int do_reorth, verify, direct, need_scale;
+};
+
+__device__ inline double& Hd(FgState* s, int row, int col) { return s->H[col * kLd + row]; }
+
+// Post-process h(k+1,k): finiteness, breakdown test, reciprocal
+// (krylov_solvers.hpp:346-359).
+__device__ void settle_h(FgState* s, int k, double h, bool direct)
+{
+    Hd(s, k + 1, k) = h;
+    if (!__builtin_isfinite(h)) {
+        s->stop = 1;
+        s->reason = reason_non_finite_scalar;
+        s->halt = 1;
+        return;
+    }
+    double scale = 0.0;
+    for (int row = 0; row <= k + 1; ++row) scale = fmax(scale, __builtin_fabs(Hd(s, row, k)));
+    if (h > s->breakdown_tolerance * scale) {
+        s->inv_h = 1.0 / h;
+        s->direct = direct ? 1 : 0;
+        s->need_scale = direct ? 0 : 1;
+    } else {
+        s->abreak = 1;
+        s->direct = 0;
+        s->need_scale = 0;
+    }
+}
+
+// Norm decision after the projections (krylov_solvers.hpp:332-345).
+__device__ void decide(FgState* s, int k)
+{
+    const double dmin = 2.2250738585072014e-308;
+    const double eps = 2.220446049250313e-16;
+    const double nsq = s->nsq;
+    if (s->original < dmin || !__builtin_isfinite(nsq) || nsq <= eps * s->original || (nsq > 0 && nsq < dmin)) {
+        s->verify = 1;
+        s->direct = 0;
+        return;
+    }
+    s->verify = 0;
+    settle_h(s, k, __builtin_sqrt(fmax(nsq, 0.0)), true);
+}
+
+template<class T>
+__global__ __launch_bounds__(256) void k_fg_v0(T* __restrict__ v0, const T* __restrict__ r, std::int64_t n, double beta)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        v0[i] = static_cast<T>(static_cast<double>(r[i]) / beta);
+}
+
+// Pass A: h_j = <w, v_j>, j <= k, and <w, w>.
+template<int K, class T>
+__global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std::int64_t ld, int count,
+                                                   const T* __restrict__ w, std::int64_t n, FgState* s, int k,
+                                                   ReduceScratch scratch)
+{
+    if (s->halt) return;
+    double red[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) red[j] = 0.0;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double wi = static_cast<double>(w[i]);
+#pragma unroll
+        for (int j = 0; j < K - 1; ++j)
+            if (j < count) red[j] = __builtin_fma(wi, static_cast<double>(V[j * ld + i]), red[j]);
+        red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
+    }
+    dev::grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x != 0) return;
+        s->original = t[K - 1];
+        double projected = s->original;
+        for (int j = 0; j < count; ++j) {
+            Hd(s, j, k) = t[j];
+            projected -= t[j] * t[j];
+        }
+        s->projected = projected;
+        const double nonneg = fmax(projected, 0.0);
+        const bool forced = s->mode == 0;
+        const bool adaptive = s->mode == 1 && (projected <= 0.0 || nonneg < 0.5 * s->original);
+        s->do_reorth = (forced || adaptive) ? 1 : 0;
+        s->nsq = nonneg;
+        s->verify = 0;
+        s->direct = 0;
+        s->need_scale = 0;
+        if (!s->do_reorth) decide(s, k);
+    });
+}
+
+// Pass B: w -= sum h_j v_j; with re-orthogonalisation also c_j = <w, v_j>, <w, w>.
+template<int K, class T>
+__global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std::int64_t ld, int count,
+                                                   T* __restrict__ w, std::int64_t n, FgState* s, int k,
+                                                   ReduceScratch scratch)
+{
+    if (s->halt) return;
+    __shared__ double h[K];
+    if (threadIdx.x < count) h[threadIdx.x] = Hd(s, threadIdx.x, k);
+    __syncthreads();
+    const bool reorth = s->do_reorth != 0;
+    double red[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) red[j] = 0.0;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double wi = static_cast<double>(w[i]);
+        for (int j = 0; j < count; ++j) wi = __builtin_fma(-h[j], static_cast<double>(V[j * ld + i]), wi);
+        const T wt = static_cast<T>(wi);
+        w[i] = wt;
+        if (reorth) {
+            const double wr = static_cast<double>(wt);
+#pragma unroll
+            for (int j = 0; j < K - 1; ++j)
+                if (j < count) red[j] = __builtin_fma(wr, static_cast<double>(V[j * ld + i]), red[j]);
+            red[K - 1] = __builtin_fma(wr, wr, red[K - 1]);
+        }
+    }
+    dev::grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x != 0 || !s->do_reorth) return;
+        double nsq = t[K - 1];
+        for (int j = 0; j < count; ++j) {
+            s->corr[j] = t[j];
+            Hd(s, j, k) += t[j];
+            nsq -= t[j] * t[j];
+        }
+        s->nsq = nsq;
+        s->reorth_count += 1;
+        decide(s, k);
+    });
+}
+
+// Pass C: second projection, then v_{k+1} (direct) or the verified norm; then
+// the Givens rotation and residual estimate (krylov_solvers.hpp:369-433).
+template<int K, class T>
+__global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std::int64_t ld, int count,
+                                                   T* __restrict__ w, T* __restrict__ vnext, std::int64_t n,
+                                                   FgState* s, int k, ReduceScratch scratch)
+{
+    if (s->halt) return;
+    __shared__ double c[K];
+    const bool reorth = s->do_reorth != 0;
+    if (threadIdx.x < count) c[threadIdx.x] = reorth ? s->corr[threadIdx.x] : 0.0;
+    __syncthreads();
+    const bool direct = s->direct != 0;
+    const bool verify = s->verify != 0;
+    const double inv = s->inv_h;
+    double red[2] = {0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double wi = static_cast<double>(w[i]);
+        if (reorth)
+            for (int j = 0; j < count; ++j) wi = __builtin_fma(-c[j], static_cast<double>(V[j * ld + i]), wi);
+        const T wt = static_cast<T>(wi);
+        if (direct) {
+            vnext[i] = static_cast<T>(static_cast<double>(wt) * inv);
+        } else {
+            w[i] = wt;
+            if (verify) {
+                const double wr = static_cast<double>(wt);
+                red[0] = __builtin_fma(wr, wr, red[0]);
+                red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(wr));
+            }
+        }
+    }
+    dev::grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+        double verified = 0.0;
+        if (s->verify) verified = dev::block_norm(t[0], t[1], w, n);
+        if (threadIdx.x != 0) return;
+        if (s->verify) {
+            s->verif_count += 1;
+            settle_h(s, k, verified, false);
+            if (s->stop) return;
+        }
+        for (int row = 0; row < k; ++row) {
+            const double first = s->cs[row] * Hd(s, row, k) + s->sn[row] * Hd(s, row + 1, k);
+            Hd(s, row + 1, k) = -s->sn[row] * Hd(s, row, k) + s->cs[row] * Hd(s, row + 1, k);
+            Hd(s, row, k) = first;
+        }
+        const double den = hypot(Hd(s, k, k), Hd(s, k + 1, k));
+        if (!__builtin_isfinite(den)) {
+            s->stop = 1;
+            s->reason = reason_non_finite_scalar;
+            s->halt = 1;
+            return;
+        }
+        if (den == 0.0) {
+            s->abreak = 1;
+            s->cs[k] = 1.0;
+            s->sn[k] = 0.0;
+        } else {
+            s->cs[k] = Hd(s, k, k) / den;
+            s->sn[k] = Hd(s, k + 1, k) / den;
+        }
+        Hd(s, k, k) = s->cs[k] * Hd(s, k, k) + s->sn[k] * Hd(s, k + 1, k);
+        Hd(s, k + 1, k) = 0.0;
+        s->g[k + 1] = -s->sn[k] * s->g[k];
+        s->g[k] *= s->cs[k];
+        const double est = __builtin_fabs(s->g[k + 1]);
+        s->recursive = est;
+        s->columns = k + 1;
+        if (est <= s->threshold || s->abreak) {
+            s->cycle_done = 1;
+            s->halt = 1;
+        }
+    });
+}
+
+// Pass D: v_{k+1} = w / h after a verified norm.
+template<class T>
+__global__ __launch_bounds__(256) void k_fg_pass_d(const T* __restrict__ w, T* __restrict__ vnext, std::int64_t n,
+                                                   const FgState* s)
+{
+    if (s->stop || !s->need_scale) return;
+    const double inv = s->inv_h;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        vnext[i] = static_cast<T>(static_cast<double>(w[i]) * inv);
+}
+
+struct Coefficients {
+    double y[kMaxRestart];
+};
+
+// x = base + (sum_j y_j Z_j with Neumaier compensation) (detail::gmres_update,
+// krylov_solvers.hpp:23-40).
+template<class T>
+__global__ __launch_bounds__(256) void k_fg_update(T* __restrict__ x, const T* __restrict__ Z, std::int64_t ld,
+                                                   int count, Coefficients coef, std::int64_t n)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double sum = 0.0, comp = 0.0;
+        for (int j = 0; j < count; ++j) {
+            const double term = coef.y[j] * static_cast<double>(Z[j * ld + i]);
+            const double updated = sum + term;
+            comp += __builtin_fabs(sum) >= __builtin_fabs(term) ? (sum - updated) + term : (term - updated) + sum;
+            sum = updated;
+        }
+        x[i] = static_cast<T>(static_cast<double>(x[i]) + (sum + comp));
+    }
+}
+
+// r = b - A x with its norm, for either vector type.
+template<class X>
+struct EpiResidualT : dev::EpiNone {
+    static constexpr int K = 2;
+    static constexpr std::uint64_t kMaxMask = 2;
+    X* y;
+    const X* b;
+    std::int64_t n;
+    double* norm_out;
+    template<class A>
+    __device__ void elem(std::int64_t i, A ax, double* red) const
+    {
+        const X r = static_cast<X>(b[i] - static_cast<X>(ax));
+        y[i] = r;
+        const double rd = static_cast<double>(r);
+        red[0] = __builtin_fma(rd, rd, red[0]);
+        red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(rd));
+    }
+    __device__ void finalize(const double* t) const
+    {
+        const double nrm = dev::block_norm(t[0], t[1], y, n);
+        if (threadIdx.x == 0 && norm_out) *norm_out = nrm;
+    }
+};
+
+__global__ void k_stop_copy64(const double* __restrict__ in, double* __restrict__ out, std::int64_t n, const int* stop)
+{
+    if (stop && *stop) return;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) out[i] = in[i];
+}
+
+__global__ void k_stop_copy32(const float* __restrict__ in, float* __restrict__ out, std::int64_t n, const int* stop)
+{
+    if (stop && *stop) return;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) out[i] = in[i];
+}
+
+__global__ void k_add_f32_to_f64(double* __restrict__ x, const float* __restrict__ d, std::int64_t n)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        x[i] = x[i] + static_cast<double>(d[i]);
+}
+
+struct Counters {
+    std::uint64_t ops = 0, pcs = 0, reds = 0, reorth = 0, verif = 0;
+};
+
+// The FGMRES algorithm over vectors of type T.  op(x, y, stop) computes y = A x;
+// residual(b, x, r, norm_slot) computes r = b - A x and its norm; pc(in, out,
+// stop) applies the preconditioner.
+template<class T>
+struct FgEngine {
+    owtk_context ctx;
+    std::int64_t n = 0;       // owned size
+    std::int64_t local = 0;   // local size (SpMV inputs)
+    std::function<void(const T*, T*, const int*)> op;
+    std::function<void(const T*, const T*, T*, double*)> residual;
+    std::function<void(const T*, T*, const int*)> pc;
+
+    int run(const T* b, T* x, const owtk_solver_options* o, owtk_solver_result* res, Counters& cnt)
+    {
+        init_result(res);
+        if (!valid_options(o) || o->restart == 0) return OWTK_OK;
+        if (o->restart > static_cast<std::uint64_t>(kMaxRestart))
+            throw unsupported_error("restart above 60 is not supported on the device path");
+        const hipStream_t st = ctx->stream;
+        const auto scratch = ctx->scratch();
+        const int m = static_cast<int>(o->restart);
+        const std::int64_t ld = local;
+        DeviceArray<T> vecs(static_cast<std::size_t>((2 * m + 3) * ld));
+        kern::fill_zero_bytes(st, vecs.get(), vecs.bytes());
+        T* r = vecs.get();
+        T* w = r + ld;
+        T* V = w + ld;                  // m+1 vectors
+        T* Z = V + static_cast<std::int64_t>(m + 1) * ld;  // m vectors
+        DeviceArray<FgState> dstate(1);
+        ctx->pinned.ensure(sizeof(FgState) + 256);
+        auto* hs = static_cast<FgState*>(ctx->pinned.ptr);
+        double* d_norm = ctx->scalars.get() + 16;
+        const int* halt = &dstate.get()->halt;
+        auto read_norm = [&]() {
+            double v;
+            OWTK_HIP(hipMemcpyAsync(&v, d_norm, sizeof(double), hipMemcpyDeviceToHost, st));
+            ctx->sync();
+            return v;
+        };
+        auto mark = [&](int reason) {
+            res->status = status_breakdown;
+            res->breakdown_reason = reason;
+        };
+        auto finish = [&](int status) {
+            if (status >= 0) res->status = status;
+            res->operator_applications = cnt.ops;
+            res->preconditioner_applications = cnt.pcs;
+            res->global_reductions = cnt.reds;
+            res->reorthogonalizations = cnt.reorth;
+            res->arnoldi_norm_verifications = cnt.verif;
+            return OWTK_OK;
+        };
+        residual(b, x, r, d_norm);
+        ++cnt.ops;
+        const double initial = read_norm();
+        if constexpr (sizeof(T) == 8) kern::norm_to(st, scratch, reinterpret_cast<const double*>(b), n, d_norm);
+        else kern::norm_to(st, scratch, reinterpret_cast<const float*>(b), n, d_norm);
+        const double rhs_norm = read_norm();
+        cnt.reds += 2;
+        const double thr = threshold_of(rhs_norm, o);
+        res->initial_residual_norm = initial;
+        set_residual(res, initial, initial, rhs_norm);
+        if (!std::isfinite(initial) || !std::isfinite(thr)) {
+            mark(reason_non_finite_scalar);
+            return finish(-1);
+        }
+        if (initial <= thr) return finish(status_converged);
+        double beta = initial;
+        const unsigned int g = dev::stream_grid(n, 4);
+        std::int64_t red_blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
+        if (red_blocks > 1024) red_blocks = 1024;
+        if (red_blocks < 1) red_blocks = 1;
+        const unsigned int gr = static_cast<unsigned int>(red_blocks);
+        while (res->iterations < o->maximum_iterations) {
+            // beta = ||r|| (the reference re-reduces it; same vector, same value)
+            ++cnt.reds;
+            k_fg_v0<T><<<g, kBlock, 0, st>>>(V, r, n, beta);
+            OWTK_LAUNCH_CHECK();
+            std::memset(hs, 0, sizeof(FgState));
+            hs->g[0] = beta;
+            hs->threshold = thr;
+            hs->breakdown_tolerance = o->breakdown_tolerance;
+            hs->mode = o->gmres_orthogonalization;
+            hs->recursive = res->recursive_residual_norm;
+            OWTK_HIP(hipMemcpyAsync(dstate.get(), hs, sizeof(FgState), hipMemcpyHostToDevice, st));
+            const std::uint64_t left = o->maximum_iterations - res->iterations;
+            const int cols = static_cast<int>(std::min<std::uint64_t>(static_cast<std::uint64_t>(m), left));
+            for (int k = 0; k < cols; ++k) {
+                T* vk = V + static_cast<std::int64_t>(k) * ld;
+                T* zk = Z + static_cast<std::int64_t>(k) * ld;
+                T* vnext = V + static_cast<std::int64_t>(k + 1) * ld;
+                pc(vk, zk, halt);
+                op(zk, w, halt);
+                launch_passes(k, V, ld, w, vnext, dstate.get(), gr, st, scratch);
+            }
+            OWTK_HIP(hipMemcpyAsync(hs, dstate.get(), sizeof(FgState), hipMemcpyDeviceToHost, st));
+            ctx->sync();
+            const int done = hs->columns;
+            // columns whose launches ran: all up to `done`, and on a hard stop
+            // the failing column too
+            const int ran = hs->stop ? std::max(done + 1, 1) : done;
+            cnt.pcs += ran;
+            cnt.ops += ran;
+            cnt.reds += ran + hs->reorth_count + hs->verif_count;
+            cnt.reorth += hs->reorth_count;
+            cnt.verif += hs->verif_count;
+            res->iterations += static_cast<std::uint64_t>(hs->stop ? done : done);
+            res->recursive_residual_norm = hs->recursive;
+            if (hs->stop) {
+                mark(hs->reason);
+                return finish(-1);
+            }
+            const bool abreak = hs->abreak != 0;
+            // back-substitution (krylov_solvers.hpp:439-457)
+            Coefficients coef{};
+            auto h = [&](int row, int col) { return hs->H[col * kLd + row]; };
+            for (int rev = done; rev-- > 0;) {
+                double value = hs->g[rev];
+                for (int c = rev + 1; c < done; ++c) value -= h(rev, c) * coef.y[c];
+                double scale = 0;
+                for (int row = 0; row <= rev; ++row) scale = std::max(scale, std::abs(h(row, rev)));
+                if (std::abs(h(rev, rev)) <= o->breakdown_tolerance * scale) {
+                    mark(reason_singular_projected_system);
+                    return finish(-1);
+                }
+                coef.y[rev] = value / h(rev, rev);
+            }
+            k_fg_update<T><<<g, kBlock, 0, st>>>(x, Z, ld, done, coef, n);
+            OWTK_LAUNCH_CHECK();
+            residual(b, x, r, d_norm);
+            ++cnt.ops;
+            ++cnt.reds;
+            const double tn = read_norm();
+            set_residual(res, res->recursive_residual_norm, tn, rhs_norm);
+            if (!std::isfinite(tn)) {
+                mark(reason_non_finite_scalar);
+                return finish(-1);
+            }
+            if (tn <= thr) return finish(status_converged);
+            if (abreak && !(tn < beta)) {
+                mark(reason_arnoldi_invariant_subspace);
+                return finish(-1);
+            }
+            beta = tn;
+        }
+        return finish(status_maximum_iterations);
+    }
+
+    template<int K>
+    void passes(int k, T* V, std::int64_t ld, T* w, T* vnext, FgState* s, unsigned int gr, hipStream_t st,
+                ReduceScratch scratch)
+    {
+        const int count = k + 1;
+        k_fg_pass_a<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        k_fg_pass_b<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        k_fg_pass_c<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
+        k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);
+        OWTK_LAUNCH_CHECK();
+    }
+
+    void launch_passes(int k, T* V, std::int64_t ld, T* w, T* vnext, FgState* s, unsigned int gr, hipStream_t st,
+                       ReduceScratch scratch)
+    {
+        const int need = k + 2;
+        if (need <= 4) passes<4>(k, V, ld, w, vnext, s, gr, st, scratch);
+        else if (need <= 8) passes<8>(k, V, ld, w, vnext, s, gr, st, scratch);
+        else if (need <= 16) passes<16>(k, V, ld, w, vnext, s, gr, st, scratch);
+        else if (need <= 32) passes<32>(k, V, ld, w, vnext, s, gr, st, scratch);
+        else passes<62>(k, V, ld, w, vnext, s, gr, st, scratch);
+    }
+};
+
+FgEngine<double> fp64_engine(owtk_matrix m, owtk_precond pc)
+{
+    FgEngine<double> e;
+    e.ctx = m->ctx;
+    e.n = m->owned * m->bs;
+    e.local = m->local() * m->bs;
+    const std::int64_t n = e.n;
+    const hipStream_t st = m->ctx->stream;
+    e.op = [m](const double* x, double* y, const int* stop) {
+        dev::EpiStore<double> ep;
+        ep.y = y;
+        ep.stop = stop;
+        spmv_f64(m, x, ep);
+    };
+    e.residual = [m, n](const double* b, const double* x, double* r, double* norm_slot) {
+        EpiResidualT<double> ep;
+        ep.y = r;
+        ep.b = b;
+        ep.n = n;
+        ep.norm_out = norm_slot;
+        spmv_f64(m, x, ep);
+    };
+    e.pc = [pc, n, st](const double* in, double* out, const int* stop) { apply_pc_f64(pc, in, out, n, st, stop); };
+    return e;
+}
+
+FgEngine<float> low_engine(owtk_matrix m, owtk_precond pc, owtk_dtype dtype)
+{
+    FgEngine<float> e;
+    e.ctx = m->ctx;
+    e.n = m->owned * m->bs;
+    e.local = m->local() * m->bs;
+    const std::int64_t n = e.n;
+    const hipStream_t st = m->ctx->stream;
+    e.op = [m, dtype](const float* x, float* y, const int* stop) {
+        dev::EpiStore<float> ep;
+        ep.y = y;
+        ep.stop = stop;
+        spmv_low(m, dtype, x, ep);
+    };
+    e.residual = [m, n, dtype](const float* b, const float* x, float* r, double* norm_slot) {
+        EpiResidualT<float> ep;
+        ep.y = r;
+        ep.b = b;
+        ep.n = n;
+        ep.norm_out = norm_slot;
+        spmv_low(m, dtype, x, ep);
+    };
+    e.pc = [pc, n, st](const float* in, float* out, const int* stop) { apply_pc_f32(pc, in, out, n, st, stop); };
+    return e;
+}
+
+} // namespace
+
+void copy_stop_f64(hipStream_t st, const double* in, double* out, std::int64_t n, const int* stop)
+{
+    if (in == out) return;
+    k_stop_copy64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(in, out, n, stop);
+    OWTK_LAUNCH_CHECK();
+}
+
+void copy_stop_f32(hipStream_t st, const float* in, float* out, std::int64_t n, const int* stop)
+{
+    if (in == out) return;
+    k_stop_copy32<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(in, out, n, stop);
+    OWTK_LAUNCH_CHECK();
+}
+
+int fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* o,
+               owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
+        throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
+    auto e = fp64_engine(m, pc);
+    Counters cnt;
+    e.run(b, x, o, res, cnt);
+    if (o->collect_timings)
+        res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return OWTK_OK;
+}
+
+// FP64 iterative refinement around an inner low-precision FGMRES.
+int fgmres_refined_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_refinement_options* ro,
+                       owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    init_result(res);
+    const owtk_solver_options* o = &ro->outer;
+    if (!valid_options(o)) return OWTK_OK;
+    const owtk_dtype dtype = static_cast<owtk_dtype>(ro->inner_storage);
+    if (dtype != OWTK_F32 && dtype != OWTK_BF16) throw std::invalid_argument("inner storage must be F32 or BF16");
+    if ((dtype == OWTK_F32 && !m->v32.get()) || (dtype == OWTK_BF16 && !m->v16.get())) {
+        const int rc = owtk_matrix_add_storage(m, dtype);
+        if (rc != OWTK_OK) throw std::runtime_error("could not add low-precision matrix storage");
+    }
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage == OWTK_F64)
+        throw std::invalid_argument("inner preconditioner needs F32 or BF16 storage");
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const auto scratch = ctx->scratch();
+    const std::int64_t n = m->owned * m->bs;
+    const std::int64_t L = m->local() * m->bs;
+    DeviceArray<double> r(static_cast<std::size_t>(L));
+    DeviceArray<float> rf(static_cast<std::size_t>(L)), d(static_cast<std::size_t>(L));
+    kern::fill_zero_bytes(st, rf.get(), rf.bytes());
+    double* d_norm = ctx->scalars.get() + 24;
+    auto read = [&]() {
+        double v;
+        OWTK_HIP(hipMemcpyAsync(&v, d_norm, sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        return v;
+    };
+    Counters cnt;
+    auto outer_residual = [&]() {
+        dev::EpiResidual e;
+        e.y = r.get();
+        e.b = b;
+        e.n = n;
+        e.norm_out = d_norm;
+        spmv_f64(m, x, e);
+        ++cnt.ops;
+        ++cnt.reds;
+        return read();
+    };
+    kern::norm_to(st, scratch, b, n, d_norm);
+    ++cnt.reds;
+    const double rhs_norm = read();
+    const double thr = threshold_of(rhs_norm, o);
+    double rn = outer_residual();
+    res->initial_residual_norm = rn;
+    set_residual(res, rn, rn, rhs_norm);
+    auto finish = [&](int status) {
+        res->status = status;
+        res->operator_applications = cnt.ops;
+        res->preconditioner_applications = cnt.pcs;
+        res->global_reductions = cnt.reds;
+        res->reorthogonalizations = cnt.reorth;
+        res->arnoldi_norm_verifications = cnt.verif;
+        if (o->collect_timings)
+            res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return OWTK_OK;
+    };
+    if (!std::isfinite(rn) || !std::isfinite(thr)) {
+        res->breakdown_reason = reason_non_finite_scalar;
+        return finish(status_breakdown);
+    }
+    auto inner = low_engine(m, pc, dtype);
+    owtk_solver_options io = *o;
+    io.relative_tolerance = ro->inner_relative_tolerance;
+    io.absolute_tolerance = 0.0;
+    io.restart = static_cast<std::uint64_t>(ro->inner_restart);
+    io.maximum_iterations = ro->inner_maximum_iterations;
+    std::uint64_t steps = 0;
+    while (!(rn <= thr)) {
+        if (steps >= ro->maximum_refinements || res->inner_iterations >= o->maximum_iterations)
+            return finish(status_maximum_iterations);
+        kern::convert(st, rf.get(), r.get(), static_cast<std::size_t>(n));
+        kern::fill_zero_bytes(st, d.get(), d.bytes());
+        owtk_solver_result ir;
+        inner.run(rf.get(), d.get(), &io, &ir, cnt);
+        res->inner_iterations += ir.iterations;
+        res->iterations = res->inner_iterations;
+        ++steps;
+        res->refinement_steps = steps;
+        if (ir.status == status_breakdown && ir.iterations == 0) {
+            res->breakdown_reason = ir.breakdown_reason;
+            return finish(status_breakdown);
+        }
+        k_add_f32_to_f64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(x, d.get(), n);
+        OWTK_LAUNCH_CHECK();
+        const double previous = rn;
+        rn = outer_residual();
+        res->recursive_residual_norm = ir.true_residual_norm;
+        set_residual(res, ir.true_residual_norm, rn, rhs_norm);
+        if (!std::isfinite(rn)) {
+            res->breakdown_reason = reason_non_finite_scalar;
+            return finish(status_breakdown);
+        }
+        if (!(rn < previous)) {
+            // no progress: the low-precision solve cannot reduce this residual further
+            res->breakdown_reason = reason_none;
+            return finish(status_diverged);
+        }
+    }
+    return finish(status_converged);
+}
+
+} // namespace owtk
+
+extern "C" {
+
+void owtk_solver_options_default(owtk_solver_options* o)
+{
+    if (!o) return;
+    o->relative_tolerance = 1e-8;
+    o->absolute_tolerance = 0.0;
+    o->breakdown_tolerance = 64.0 * std::numeric_limits<double>::epsilon();
+    o->maximum_iterations = 1000;
+    o->convergence_check_interval = 10;
+    o->restart = 30;
+    o->residual_replacement_interval = 50;
+    o->gmres_orthogonalization = 0;
+    o->collect_timings = 0;
+}
+
+void owtk_refinement_options_default(owtk_refinement_options* o)
+{
+    if (!o) return;
+    owtk_solver_options_default(&o->outer);
+    o->outer.maximum_iterations = 100000;
+    o->inner_storage = OWTK_F32;
+    o->inner_restart = 30;
+    o->inner_relative_tolerance = 1e-4;
+    o->inner_maximum_iterations = 300;
+    o->maximum_refinements = 50;
+}
+
+int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* options,
+                owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        fgmres_run(m, pc, b, x, options, result);
+    });
+}
+
+int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, double* x,
+                        const owtk_refinement_options* options, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        fgmres_refined_run(m, inner_pc, b, x, options, result);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,171 @@
+// Objects behind the opaque C handles and the host-side launch interfaces
+// shared between the translation units of libowtk_hip.
+#pragma once
+
+#include "device.hpp"
+#include "owtk_hip_api.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace owtk {
+
+[[noreturn]] void throw_hip(hipError_t err, const char* what);
+
+#define OWTK_HIP(call)                                                  \
+    do {                                                                \
+        hipError_t owtk_hip_status_ = (call);                           \
+        if (owtk_hip_status_ != hipSuccess) ::owtk::throw_hip(owtk_hip_status_, #call); \
+    } while (0)
+
+#define OWTK_LAUNCH_CHECK() OWTK_HIP(hipGetLastError())
+
+// Owning device allocation.
+template<class T>
+struct DeviceArray {
+    T* ptr = nullptr;
+    std::size_t count = 0;
+
+    DeviceArray() = default;
+    explicit DeviceArray(std::size_t n) { allocate(n); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : ptr(o.ptr), count(o.count)
+    {
+        o.ptr = nullptr;
+        o.count = 0;
+    }
+    DeviceArray& operator=(DeviceArray&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            count = o.count;
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        return *this;
+    }
+    ~DeviceArray() { release(); }
+
+    void allocate(std::size_t n)
+    {
+        release();
+        if (n == 0) return;
+        void* p = nullptr;
+        const hipError_t err = hipMalloc(&p, n * sizeof(T));
+        if (err == hipErrorOutOfMemory) throw std::bad_alloc();
+        if (err != hipSuccess) throw_hip(err, "hipMalloc");
+        ptr = static_cast<T*>(p);
+        count = n;
+    }
+    void release() noexcept
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    std::size_t bytes() const { return count * sizeof(T); }
+    T* get() const { return ptr; }
+};
+
+// Pinned host staging for scalar read-back.
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    std::size_t bytes = 0;
+    void ensure(std::size_t n);
+    ~PinnedBuffer();
+};
+
+} // namespace owtk
+
+struct owtk_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    owtk::DeviceArray<double> partials;      // [kMaxReduceValues][kMaxReduceBlocks]
+    owtk::DeviceArray<unsigned int> ticket;  // grid_reduce arrival counter
+    owtk::DeviceArray<double> scalars;       // small device scalar results
+    owtk::PinnedBuffer pinned;
+
+    owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
+    void activate() const;  // hipSetDevice(device)
+    void sync() const;
+};
+
+namespace owtk {
+
+// Device layout of one value storage of a matrix.
+//
+// Small blocks (block size <= 4): SELL-64 ("sliced ELLPACK", slice height = one
+// wave).  Slice s holds rows 64s..64s+63; its width w_s is the longest row of
+// the slice; slot (s, k, lane) lives at slice_ptr[s] + 64k + lane, so the k-th
+// entry of 64 consecutive rows is one contiguous 64-wide load.  Component-
+// diagonal values sit at [slot & ~63] * B + 64c + (slot & 63) (per entry k and
+// component c a contiguous 64-lane run); dense blocks at slot * B*B (each
+// lane's block contiguous).  Row entries keep the CSR order, so the
+// per-row accumulation order is the reference's.
+//
+// Larger component-diagonal blocks keep the reference's CSR [entry][comp]
+// layout (lanes run over components, which is already contiguous).
+enum class Layout { sell, csr };
+
+struct MatrixStructure {
+    Layout layout = Layout::sell;
+    std::int64_t rows = 0;
+    std::int64_t slices = 0;
+    std::int64_t slots = 0;            // SELL slots incl. padding (or CSR entries)
+    int max_width = 0;
+    DeviceArray<std::int32_t> slice_ptr;
+    DeviceArray<std::int32_t> row_len;   // SELL: entries per row
+    DeviceArray<std::int32_t> columns;   // per slot (or per CSR entry)
+    DeviceArray<std::int32_t> offsets;   // CSR layout only
+    std::vector<std::int64_t> slot_of_entry; // host: CSR entry -> slot
+};
+
+} // namespace owtk
+
+struct owtk_matrix_s {
+    owtk_context ctx = nullptr;
+    int kind = OWTK_MATRIX_SPLIT;
+    int bs = 1;
+    std::int64_t owned = 0;
+    std::int64_t ghost = 0;
+    std::int64_t entries = 0;   // stored block entries (split: off-diagonal)
+    // host copy of the structure (symbolic analyses of preconditioners)
+    std::vector<std::int32_t> h_offsets;
+    std::vector<std::int32_t> h_columns;
+    owtk::MatrixStructure s;
+    // values in slot layout (value_count per storage)
+    std::int64_t value_count = 0;
+    owtk::DeviceArray<double> v64;
+    owtk::DeviceArray<float> v32;
+    owtk::DeviceArray<std::uint16_t> v16;
+    // split diagonal, [slice][comp][lane] (SELL) or [row][comp] (CSR layout)
+    std::int64_t diag_count = 0;
+    owtk::DeviceArray<double> d64;
+    owtk::DeviceArray<float> d32;
+    owtk::DeviceArray<std::uint16_t> d16;
+    std::uint64_t value_epoch = 0; // bumped by update_values
+
+    std::int64_t local() const { return owned + ghost; }
+    bool dense() const { return kind == OWTK_MATRIX_BSR; }
+    bool has_diag_array() const { return kind == OWTK_MATRIX_SPLIT; }
+    // index of (slot, component c) in the value array (component-diagonal)
+    std::int64_t value_index(std::int64_t slot, int c) const
+    {
+        if (s.layout == owtk::Layout::csr) return slot * bs + c;
+        return (slot & ~std::int64_t(63)) * bs + 64 * c + (slot & 63);
+    }
+    std::int64_t diag_index(std::int64_t row, int c) const
+    {
+        if (s.layout == owtk::Layout::csr) return row * bs + c;
+        return (row & ~std::int64_t(63)) * bs + 64 * c + (row & 63);
+    }
+};
+

@@ -1,0 +1,195 @@
+// Streaming vector kernels: fills, conversions, dot/norm with FP64
+// accumulation, axpy, axpy+dot and the batched multi-dot of classical
+// Gram-Schmidt.
+#pragma once
+
+#include "../device.hpp"
+#include "../internal.hpp"
+
+namespace owtk::kern {
+inline namespace {  // kernels get internal linkage in every including TU
+
+using namespace owtk::dev;
+
+constexpr int kMaxMultidot = 62;
+
+__global__ __launch_bounds__(256) void k_fill_zero16(uint4* p, std::int64_t n)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero1(unsigned char* p, std::int64_t n)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = 0;
+}
+
+inline void fill_zero_bytes(hipStream_t st, void* p, std::size_t bytes)
+{
+    if (bytes == 0) return;
+    const auto addr = reinterpret_cast<std::uintptr_t>(p);
+    if (addr % 16 == 0 && bytes % 16 == 0) {
+        const std::int64_t n = static_cast<std::int64_t>(bytes / 16);
+        k_fill_zero16<<<stream_grid(n, 4), kBlock, 0, st>>>(static_cast<uint4*>(p), n);
+    } else {
+        const std::int64_t n = static_cast<std::int64_t>(bytes);
+        k_fill_zero1<<<stream_grid(n, 4), kBlock, 0, st>>>(static_cast<unsigned char*>(p), n);
+    }
+    OWTK_LAUNCH_CHECK();
+}
+
+template<class D, class S>
+__global__ __launch_bounds__(256) void k_convert(D* dst, const S* src, std::int64_t n)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = static_cast<D>(src[i]);
+}
+
+template<class D, class S>
+inline void convert(hipStream_t st, D* dst, const S* src, std::size_t n)
+{
+    k_convert<D, S><<<stream_grid(static_cast<std::int64_t>(n), 4), kBlock, 0, st>>>(dst, src, static_cast<std::int64_t>(n));
+    OWTK_LAUNCH_CHECK();
+}
+
+// FP64 -> BF16 (round to nearest even) and FP32 copies of a value array.
+__global__ __launch_bounds__(256) void k_to_bf16(std::uint16_t* dst, const double* src, std::int64_t n)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = float_to_bf16(static_cast<float>(src[i]));
+}
+
+template<class X>
+__global__ __launch_bounds__(256) void k_dot(const X* __restrict__ a, const X* __restrict__ b, std::int64_t n,
+                                             ReduceScratch scratch, double* out)
+{
+    double red[1] = {0.0};
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        red[0] = __builtin_fma(static_cast<double>(a[i]), static_cast<double>(b[i]), red[0]);
+    grid_reduce<1>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x == 0) *out = t[0];
+    });
+}
+
+template<class X>
+inline void dot_to(hipStream_t st, ReduceScratch scratch, const X* a, const X* b, std::int64_t n, double* out)
+{
+    k_dot<X><<<stream_grid(n, 4), kBlock, 0, st>>>(a, b, n, scratch, out);
+    OWTK_LAUNCH_CHECK();
+}
+
+template<class X>
+__global__ __launch_bounds__(256) void k_norm(const X* __restrict__ a, std::int64_t n, ReduceScratch scratch,
+                                              double* out)
+{
+    double red[2] = {0.0, 0.0};
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = static_cast<double>(a[i]);
+        red[0] = __builtin_fma(v, v, red[0]);
+        red[1] = combine<Combine::max>(red[1], __builtin_fabs(v));
+    }
+    grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+        const double nrm = block_norm(t[0], t[1], a, n);
+        if (threadIdx.x == 0) *out = nrm;
+    });
+}
+
+template<class X>
+inline void norm_to(hipStream_t st, ReduceScratch scratch, const X* a, std::int64_t n, double* out)
+{
+    k_norm<X><<<stream_grid(n, 4), kBlock, 0, st>>>(a, n, scratch, out);
+    OWTK_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_axpy(std::int64_t n, double alpha, const double* __restrict__ x,
+                                              double* __restrict__ y)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        y[i] = __builtin_fma(alpha, x[i], y[i]);
+}
+
+inline void axpy(hipStream_t st, std::int64_t n, double alpha, const double* x, double* y)
+{
+    k_axpy<<<stream_grid(n, 4), kBlock, 0, st>>>(n, alpha, x, y);
+    OWTK_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_axpy_dot(std::int64_t n, double alpha, const double* __restrict__ x,
+                                                  double* __restrict__ y, const double* __restrict__ w,
+                                                  ReduceScratch scratch, double* out)
+{
+    double red[1] = {0.0};
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = __builtin_fma(alpha, x[i], y[i]);
+        y[i] = v;
+        red[0] = __builtin_fma(w[i], v, red[0]);
+    }
+    grid_reduce<1>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x == 0) *out = t[0];
+    });
+}
+
+inline void axpy_dot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, double alpha, const double* x,
+                        double* y, const double* w, double* out)
+{
+    k_axpy_dot<<<stream_grid(n, 4), kBlock, 0, st>>>(n, alpha, x, y, w, scratch, out);
+    OWTK_LAUNCH_CHECK();
+}
+
+struct BasisPointers {
+    const double* v[kMaxMultidot];
+};
+
+// out[j] = <w, v_j> (j < k), out[k] = <w, w>; K = reduced-value bucket >= k+1.
+template<int K>
+__global__ __launch_bounds__(256) void k_multidot(std::int64_t n, int k, BasisPointers basis,
+                                                  const double* __restrict__ w, ReduceScratch scratch,
+                                                  double* out)
+{
+    double red[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) red[j] = 0.0;
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < K - 1; ++j)
+            if (j < k) red[j] = __builtin_fma(wi, basis.v[j][i], red[j]);
+        red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
+    }
+    grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+        if (threadIdx.x == 0) {
+            for (int j = 0; j < k; ++j) out[j] = t[j];
+            out[k] = t[K - 1];
+        }
+    });
+}
+
+inline void multidot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, int k, const double* const* vs,
+                        const double* w, double* out)
+{
+    BasisPointers bp{};
+    for (int j = 0; j < k; ++j) bp.v[j] = vs[j];
+    std::int64_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    const unsigned int g = static_cast<unsigned int>(blocks);
+    if (k + 1 <= 4) k_multidot<4><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    else if (k + 1 <= 8) k_multidot<8><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    else if (k + 1 <= 16) k_multidot<16><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    else if (k + 1 <= 32) k_multidot<32><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    else k_multidot<63><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    OWTK_LAUNCH_CHECK();
+}
+
+} // namespace
+} // namespace owtk::kern

@@ -1,0 +1,359 @@
+// SpMV kernels for the three block-CSR forms, with fused epilogues.
+//
+// Accumulation order per output element is the reference's:
+//   component-diagonal CSR: out = 0, then out = fma(a_e, x_col(e), out) over the
+//     row's entries in CSR order (block_csr.hpp:198-215);
+//   split: out = d * x_row, then the same fma chain over the edges
+//     (split_block_csr.hpp:95-119);
+//   dense BSR: per block row t = fma chain over the block's columns starting
+//     at 0, then out += t (dense_block_csr.hpp:179-205).
+// The build compiles with -ffp-contract=off and every fused multiply-add is an
+// explicit fma, so these kernels and the CPU oracle round identically.
+//
+// An epilogue E receives every finished output element and may fold it into
+// up to E::K reduced values; the last block then runs E::finalize.
+#pragma once
+
+#include "../device.hpp"
+
+namespace owtk::dev {
+
+template<class X> struct acc_type { using type = double; };
+template<> struct acc_type<float> { using type = float; };
+
+template<class V> __device__ inline auto load_value(const V* p, std::int64_t i)
+{
+    if constexpr (sizeof(V) == 2) return bf16_to_float(reinterpret_cast<const std::uint16_t*>(p)[i]);
+    else return p[i];
+}
+
+// ----------------------------------------------------------------- epilogues
+struct EpiNone {
+    static constexpr int K = 0;
+    static constexpr std::uint64_t kMaxMask = 0;
+    const int* stop = nullptr;
+    __device__ bool stopped() const { return stop != nullptr && *stop != 0; }
+    __device__ void finalize(const double*) const {}
+};
+
+template<class X>
+struct EpiStore : EpiNone {
+    X* y;
+    template<class A>
+    __device__ void elem(std::int64_t i, A value, double*) const { y[i] = static_cast<X>(value); }
+};
+
+// y = b - A x, reduced: sum y^2, max |y|; norm written by finalize.
+struct EpiResidual : EpiNone {
+    static constexpr int K = 2;
+    static constexpr std::uint64_t kMaxMask = 2;
+    double* y;
+    const double* b;
+    std::int64_t n;       // owned size
+    double* norm_out;     // may be null
+    __device__ void elem(std::int64_t i, double ax, double* red) const
+    {
+        const double r = b[i] - ax;
+        y[i] = r;
+        red[0] = __builtin_fma(r, r, red[0]);
+        red[1] = combine<Combine::max>(red[1], __builtin_fabs(r));
+    }
+    __device__ void finalize(const double* t) const
+    {
+        const double nrm = block_norm(t[0], t[1], y, n);
+        if (threadIdx.x == 0 && norm_out) *norm_out = nrm;
+    }
+};
+
+// y = A x, reduced <w, y>; fin(total) on thread 0 of the last block.
+template<class Fin>
+struct EpiDot1 : EpiNone {
+    static constexpr int K = 1;
+    static constexpr std::uint64_t kMaxMask = 0;
+    double* y;
+    const double* w;
+    Fin fin;
+    __device__ void elem(std::int64_t i, double v, double* red) const
+    {
+        y[i] = v;
+        red[0] = __builtin_fma(w[i], v, red[0]);
+    }
+    __device__ void finalize(const double* t) const
+    {
+        if (threadIdx.x == 0) fin(t[0]);
+    }
+};
+
+// y = A x, reduced <y, w>, <y, y>; fin(t0, t1) on thread 0 of the last block.
+template<class Fin>
+struct EpiDot2 : EpiNone {
+    static constexpr int K = 2;
+    static constexpr std::uint64_t kMaxMask = 0;
+    double* y;
+    const double* w;
+    Fin fin;
+    __device__ void elem(std::int64_t i, double v, double* red) const
+    {
+        y[i] = v;
+        red[0] = __builtin_fma(v, w[i], red[0]);
+        red[1] = __builtin_fma(v, v, red[1]);
+    }
+    __device__ void finalize(const double* t) const
+    {
+        if (threadIdx.x == 0) fin(t[0], t[1]);
+    }
+};
+
+} // namespace owtk::dev
+
+namespace owtk::kern {
+
+using namespace owtk::dev;
+
+// Physical block -> logical row block.  Blocks are dealt round-robin over the
+// 8 XCDs; giving each XCD a contiguous eighth of the rows keeps the x-gather
+// working set of an XCD inside its own L2.
+__device__ inline unsigned int xcd_block(unsigned int b, unsigned int n)
+{
+    const unsigned int per = n / 8;
+    if (per == 0 || n % 8 != 0) return b;
+    return (b % 8) * per + b / 8;
+}
+
+// Component-diagonal (and split) SpMV, SELL-64 layout, one lane per row.
+template<int BS, class V, class X, bool DIAG, class E>
+__global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restrict__ slice_ptr,
+                                                    const std::int32_t* __restrict__ row_len,
+                                                    const std::int32_t* __restrict__ cols,
+                                                    const V* __restrict__ vals,
+                                                    const V* __restrict__ diag,
+                                                    const X* __restrict__ x, std::int64_t rows,
+                                                    E e, ReduceScratch scratch)
+{
+    using A = typename acc_type<X>::type;
+    if (e.stopped()) return;
+    double red[E::K > 0 ? E::K : 1];
+#pragma unroll
+    for (int k = 0; k < (E::K > 0 ? E::K : 1); ++k) red[k] = 0.0;
+    const std::int64_t row = static_cast<std::int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    if (row < rows) {
+        const int lane = static_cast<int>(row & 63);
+        const std::int64_t slice = row >> 6;
+        A acc[BS];
+        if constexpr (DIAG) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                acc[c] = static_cast<A>(load_value(diag, slice * 64 * BS + c * 64 + lane))
+                    * static_cast<A>(x[row * BS + c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc[c] = A(0);
+        }
+        const std::int64_t base = slice_ptr[slice];
+        const int len = row_len[row];
+        for (int k = 0; k < len; ++k) {
+            const std::int64_t run = base + 64 * k;
+            const std::int64_t col = cols[run + lane];
+            const V* vk = vals + run * BS + lane;
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                acc[c] = dfma(static_cast<A>(load_value(vk, 64 * c)), static_cast<A>(x[col * BS + c]), acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < BS; ++c) e.elem(row * BS + c, acc[c], red);
+    }
+    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+}
+
+// Component-diagonal (and split) SpMV for any block size, reference CSR
+// layout: one wave per row, lanes over components.
+template<class V, class X, bool DIAG, class E>
+__global__ __launch_bounds__(256) void spmv_csr_cd(const std::int32_t* __restrict__ offsets,
+                                                   const std::int32_t* __restrict__ cols,
+                                                   const V* __restrict__ vals,
+                                                   const V* __restrict__ diag,
+                                                   const X* __restrict__ x, std::int64_t rows,
+                                                   int bs, E e, ReduceScratch scratch)
+{
+    using A = typename acc_type<X>::type;
+    if (e.stopped()) return;
+    double red[E::K > 0 ? E::K : 1];
+#pragma unroll
+    for (int k = 0; k < (E::K > 0 ? E::K : 1); ++k) red[k] = 0.0;
+    const int lane = threadIdx.x & 63;
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * kWavesPerBlock;
+    for (std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / 64;
+         row < rows; row += waves) {
+        const std::int32_t begin = offsets[row];
+        const std::int32_t end = offsets[row + 1];
+        for (int c = lane; c < bs; c += 64) {
+            A acc = A(0);
+            if constexpr (DIAG)
+                acc = static_cast<A>(load_value(diag, row * bs + c)) * static_cast<A>(x[row * bs + c]);
+            for (std::int32_t en = begin; en < end; ++en) {
+                const std::int64_t col = cols[en];
+                acc = dfma(static_cast<A>(load_value(vals, static_cast<std::int64_t>(en) * bs + c)),
+                           static_cast<A>(x[col * bs + c]), acc);
+            }
+            e.elem(row * bs + c, acc, red);
+        }
+    }
+    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+}
+
+// Dense BSR, SELL-64 layout, one lane per row, BS x BS blocks.
+template<int BS, class V, class X, class E>
+__global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restrict__ slice_ptr,
+                                                     const std::int32_t* __restrict__ row_len,
+                                                     const std::int32_t* __restrict__ cols,
+                                                     const V* __restrict__ vals,
+                                                     const X* __restrict__ x, std::int64_t rows,
+                                                     E e, ReduceScratch scratch)
+{
+    using A = typename acc_type<X>::type;
+    if (e.stopped()) return;
+    double red[E::K > 0 ? E::K : 1];
+#pragma unroll
+    for (int k = 0; k < (E::K > 0 ? E::K : 1); ++k) red[k] = 0.0;
+    const std::int64_t row = static_cast<std::int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    if (row < rows) {
+        const int lane = static_cast<int>(row & 63);
+        const std::int64_t slice = row >> 6;
+        A acc[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) acc[r] = A(0);
+        const std::int64_t base = slice_ptr[slice];
+        const int len = row_len[row];
+        for (int k = 0; k < len; ++k) {
+            const std::int64_t slot = base + 64 * k + lane;
+            const std::int64_t col = cols[slot];
+            const V* blk = vals + slot * (BS * BS);
+            A xc[BS];
+#pragma unroll
+            for (int c = 0; c < BS; ++c) xc[c] = static_cast<A>(x[col * BS + c]);
+#pragma unroll
+            for (int r = 0; r < BS; ++r) {
+                A t = A(0);
+#pragma unroll
+                for (int c = 0; c < BS; ++c) t = dfma(static_cast<A>(load_value(blk, r * BS + c)), xc[c], t);
+                acc[r] = acc[r] + t;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < BS; ++r) e.elem(row * BS + r, acc[r], red);
+    }
+    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+}
+
+// Dense BSR for any block size, reference CSR layout: one thread per
+// (row, block row).
+template<class V, class X, class E>
+__global__ __launch_bounds__(256) void spmv_csr_bsr(const std::int32_t* __restrict__ offsets,
+                                                    const std::int32_t* __restrict__ cols,
+                                                    const V* __restrict__ vals,
+                                                    const X* __restrict__ x, std::int64_t rows,
+                                                    int bs, E e, ReduceScratch scratch)
+{
+    using A = typename acc_type<X>::type;
+    if (e.stopped()) return;
+    double red[E::K > 0 ? E::K : 1];
+#pragma unroll
+    for (int k = 0; k < (E::K > 0 ? E::K : 1); ++k) red[k] = 0.0;
+    const std::int64_t total = rows * bs;
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * kBlock;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
+        const std::int64_t row = i / bs;
+        const int r = static_cast<int>(i % bs);
+        A acc = A(0);
+        for (std::int32_t en = offsets[row]; en < offsets[row + 1]; ++en) {
+            const std::int64_t col = cols[en];
+            const std::int64_t blk = static_cast<std::int64_t>(en) * bs * bs + static_cast<std::int64_t>(r) * bs;
+            A t = A(0);
+            for (int c = 0; c < bs; ++c)
+                t = dfma(static_cast<A>(load_value(vals, blk + c)), static_cast<A>(x[col * bs + c]), t);
+            acc = acc + t;
+        }
+        e.elem(i, acc, red);
+    }
+    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+}
+
+} // namespace owtk::kern
+
+namespace owtk {
+
+// Launch the SpMV of m with value storage `dtype` and epilogue e.  V/X are
+// deduced from dtype by the caller through the typed entry below.
+template<class V, class X, class E>
+void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
+{
+    using namespace owtk::kern;
+    const hipStream_t st = m->ctx->stream;
+    const auto scratch = m->ctx->scratch();
+    const auto& s = m->s;
+    const std::int64_t rows = m->owned;
+    if (s.layout == Layout::sell) {
+        const std::int64_t blocks = (rows + kBlock - 1) / kBlock;
+        if (E::K > 0 && blocks > kMaxReduceBlocks)
+            throw unsupported_error("fused SpMV reductions support at most 4,194,304 rows");
+        if (blocks > 0x7fffffff) throw std::invalid_argument("too many rows");
+        const unsigned int grid = static_cast<unsigned int>(blocks);
+        if (m->dense()) {
+            switch (m->bs) {
+            case 2: spmv_sell_bsr<2, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            case 3: spmv_sell_bsr<3, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            case 4: spmv_sell_bsr<4, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            default: spmv_sell_bsr<1, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            }
+        } else if (m->has_diag_array()) {
+            switch (m->bs) {
+            case 1: spmv_sell_cd<1, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            case 2: spmv_sell_cd<2, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            case 3: spmv_sell_cd<3, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            default: spmv_sell_cd<4, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            }
+        } else {
+            switch (m->bs) {
+            case 1: spmv_sell_cd<1, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            case 2: spmv_sell_cd<2, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            case 3: spmv_sell_cd<3, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            default: spmv_sell_cd<4, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
+            }
+        }
+    } else {
+        if (m->dense()) {
+            const unsigned int grid = stream_grid(rows * m->bs);
+            spmv_csr_bsr<V, X, E><<<grid, kBlock, 0, st>>>(s.offsets.get(), s.columns.get(), vals, x, rows, m->bs, e, scratch);
+        } else {
+            std::int64_t blocks = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
+            if (blocks > kMaxReduceBlocks) blocks = kMaxReduceBlocks;
+            if (m->has_diag_array())
+                spmv_csr_cd<V, X, true, E><<<static_cast<unsigned int>(blocks), kBlock, 0, st>>>(s.offsets.get(), s.columns.get(), vals, diag, x, rows, m->bs, e, scratch);
+            else
+                spmv_csr_cd<V, X, false, E><<<static_cast<unsigned int>(blocks), kBlock, 0, st>>>(s.offsets.get(), s.columns.get(), vals, diag, x, rows, m->bs, e, scratch);
+        }
+    }
+    OWTK_LAUNCH_CHECK();
+}
+
+// FP64 SpMV with an epilogue (the solver path).
+template<class E>
+void spmv_f64(owtk_matrix m, const double* x, E e)
+{
+    spmv_dispatch<double, double, E>(m, m->v64.get(), m->d64.get(), x, e);
+}
+
+// Low-precision SpMV (FP32 or BF16 values, FP32 vectors).
+template<class E>
+void spmv_low(owtk_matrix m, owtk_dtype dtype, const float* x, E e)
+{
+    if (dtype == OWTK_F32) {
+        if (!m->v32.get()) throw std::invalid_argument("matrix has no FP32 storage");
+        spmv_dispatch<float, float, E>(m, m->v32.get(), m->d32.get(), x, e);
+    } else {
+        if (!m->v16.get()) throw std::invalid_argument("matrix has no BF16 storage");
+        spmv_dispatch<std::uint16_t, float, E>(m, m->v16.get(), m->d16.get(), x, e);
+    }
+}
+
+} // namespace owtk

@@ -1,0 +1,1066 @@
+// Jacobi, SSOR and ILU(k) preconditioners on the device.
+//
+// Symbolic work (block partition, ILU fill pattern, update lists, level
+// schedules) runs once on the host at creation; numeric updates (value
+// gather, ILU factorization, inverse diagonals) and every apply run on the
+// GPU.  This is the reference's "symbolic once, numeric many" contract
+// (IluLevelPreconditioner::update_values, preconditioner.hpp:269-311).
+#include "precond.hpp"
+#include "kernels/blas1.hpp"
+#include "host/host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <numeric>
+
+using namespace owtk;
+
+namespace owtk {
+namespace {
+
+using dev::kBlock;
+
+constexpr double kTol = 64.0 * std::numeric_limits<double>::epsilon();
+
+enum Flag : int { flag_ok = 0, flag_zero_pivot = 1, flag_singular_diag = 2, flag_no_diag = 3 };
+
+// ------------------------------------------------------------------ kernels --
+
+// Gather matrix values through encoded sources: src >= 0 -> values[src + c*stride],
+// src <= -2 -> diag[-2-src + c*stride], src == -1 -> 0.  out is [c][count].
+__global__ __launch_bounds__(256) void k_gather(double* __restrict__ out, const std::int64_t* __restrict__ src,
+                                                std::int64_t count, int bs, std::int64_t stride,
+                                                const double* __restrict__ values, const double* __restrict__ diag,
+                                                double scale)
+{
+    const std::int64_t total = count * bs;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t c = t / count;
+        const std::int64_t e = t - c * count;
+        const std::int64_t s = src[e];
+        double v = 0.0;
+        if (s >= 0) v = values[s + c * stride];
+        else if (s <= -2) v = diag[(-2 - s) + c * stride];
+        out[t] = scale == 1.0 ? v : scale * v;
+    }
+}
+
+// Gather from a [c][nsrc] array by index (index < 0 -> 0).
+__global__ __launch_bounds__(256) void k_gather_cm(double* __restrict__ out, const std::int64_t* __restrict__ src,
+                                                   std::int64_t count, int bs, const double* __restrict__ in,
+                                                   std::int64_t nsrc)
+{
+    const std::int64_t total = count * bs;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t c = t / count;
+        const std::int64_t e = t - c * count;
+        const std::int64_t s = src[e];
+        out[t] = s >= 0 ? in[c * nsrc + s] : 0.0;
+    }
+}
+
+// SSOR diagonal check (LocalSsorPreconditioner::update_values,
+// preconditioner.hpp:142-174): finite and |d| > 64 eps.
+__global__ __launch_bounds__(256) void k_check_diag(const double* __restrict__ d, std::int64_t n, double tol, int* flag)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double v = d[i];
+        if (!(__builtin_isfinite(v) && __builtin_fabs(v) > tol)) atomicExch(flag, static_cast<int>(flag_singular_diag));
+    }
+}
+
+// Jacobi inverse (JacobiPreconditioner::update_values, preconditioner.hpp:55-71).
+__global__ __launch_bounds__(256) void k_jacobi_inverse(double* __restrict__ inv, const std::int64_t* __restrict__ src,
+                                                        std::int64_t rows, int bs, std::int64_t stride,
+                                                        const double* __restrict__ values,
+                                                        const double* __restrict__ diag, double minimum, int* flag)
+{
+    const std::int64_t total = rows * bs;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t row = t / bs;
+        const std::int64_t c = t - row * bs;
+        const std::int64_t s = src[row];
+        const double v = s >= 0 ? values[s + c * stride] : diag[(-2 - s) + c * stride];
+        if (!(__builtin_isfinite(v) && __builtin_fabs(v) > minimum)) atomicExch(flag, static_cast<int>(flag_singular_diag));
+        inv[t] = 1.0 / v;
+    }
+}
+
+template<class X, class V>
+__global__ __launch_bounds__(256) void k_jacobi_apply(const V* __restrict__ inv, const X* __restrict__ in,
+                                                      X* __restrict__ out, std::int64_t n, const int* stop)
+{
+    if (stop && *stop) return;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        out[i] = static_cast<X>(inv[i]) * in[i];
+}
+
+// Dense block-Jacobi: invert each bs x bs diagonal block by Gauss-Jordan with
+// partial pivoting (thread per row).  No reference counterpart: the dense
+// BSR form has no diagonal() (dense_block_csr.hpp).
+template<int BS>
+__global__ __launch_bounds__(256) void k_block_inverse(double* __restrict__ inv, const std::int64_t* __restrict__ slot,
+                                                       std::int64_t rows, const double* __restrict__ values,
+                                                       int* flag)
+{
+    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    double a[BS][BS], b[BS][BS];
+    const double* blk = values + slot[row] * (BS * BS);
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            a[r][c] = blk[r * BS + c];
+            b[r][c] = r == c ? 1.0 : 0.0;
+        }
+    for (int k = 0; k < BS; ++k) {
+        int p = k;
+        for (int r = k + 1; r < BS; ++r)
+            if (__builtin_fabs(a[r][k]) > __builtin_fabs(a[p][k])) p = r;
+        if (!(__builtin_fabs(a[p][k]) > 0.0) || !__builtin_isfinite(a[p][k])) {
+            atomicExch(flag, static_cast<int>(flag_singular_diag));
+            return;
+        }
+        if (p != k)
+            for (int c = 0; c < BS; ++c) {
+                double t = a[k][c]; a[k][c] = a[p][c]; a[p][c] = t;
+                t = b[k][c]; b[k][c] = b[p][c]; b[p][c] = t;
+            }
+        const double inv_p = 1.0 / a[k][k];
+        for (int c = 0; c < BS; ++c) {
+            a[k][c] *= inv_p;
+            b[k][c] *= inv_p;
+        }
+        for (int r = 0; r < BS; ++r) {
+            if (r == k) continue;
+            const double f = a[r][k];
+            for (int c = 0; c < BS; ++c) {
+                a[r][c] = __builtin_fma(-f, a[k][c], a[r][c]);
+                b[r][c] = __builtin_fma(-f, b[k][c], b[r][c]);
+            }
+        }
+    }
+    double* out = inv + row * (BS * BS);
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) out[r * BS + c] = b[r][c];
+}
+
+template<int BS, class X, class V>
+__global__ __launch_bounds__(256) void k_block_jacobi_apply(const V* __restrict__ inv, const X* __restrict__ in,
+                                                            X* __restrict__ out, std::int64_t rows, const int* stop)
+{
+    if (stop && *stop) return;
+    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    X r[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) r[c] = in[row * BS + c];
+    const V* blk = inv + row * (BS * BS);
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+        X t = X(0);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) t = dev::dfma(static_cast<X>(blk[i * BS + c]), r[c], t);
+        out[row * BS + i] = t;
+    }
+}
+
+struct TriArgs {
+    const std::int32_t* row0;
+    const std::int32_t* nrows;
+    const std::int32_t* flev;
+    const std::int32_t* blev;
+    const std::int32_t* flev_pos;
+    const std::int32_t* blev_pos;
+    const std::int32_t* frow;
+    const std::int32_t* brow;
+    const std::int64_t* fptr;
+    const std::int64_t* bptr;
+    const std::int32_t* fcol;
+    const std::int32_t* bcol;
+    std::int64_t fnnz, bnnz, rows;
+    int bs;
+};
+
+// ILU(k) numeric factorization (IluLevelPreconditioner::factorize,
+// preconditioner.hpp:439-485), one workgroup per (block, component), rows in
+// forward level order.  F is the FP64 factor, [c][entry].
+__global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int64_t* __restrict__ offsets,
+                                                    const std::int64_t* __restrict__ diag_pos,
+                                                    const std::int64_t* __restrict__ upd_ptr,
+                                                    const std::int64_t* __restrict__ upd_target,
+                                                    const std::int64_t* __restrict__ upd_upper,
+                                                    const std::int64_t* __restrict__ columns_global,
+                                                    double* __restrict__ F, std::int64_t nnz,
+                                                    double* __restrict__ inv, int* flag)
+{
+    const int c = blockIdx.x % a.bs;
+    const int b = blockIdx.x / a.bs;
+    const std::int64_t r0 = a.row0[b];
+    double* Fc = F + static_cast<std::int64_t>(c) * nnz;
+    const int l_begin = a.flev[b], l_end = a.flev[b + 1] - 1;
+    for (int l = l_begin; l < l_end; ++l) {
+        const int p0 = a.flev_pos[l], p1 = a.flev_pos[l + 1];
+        for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+            const std::int64_t row = r0 + a.frow[r0 + p];
+            const std::int64_t dpos = diag_pos[row];
+            for (std::int64_t e = offsets[row]; e < dpos; ++e) {
+                const std::int64_t k = columns_global[e];
+                const double pivot = Fc[diag_pos[k]];
+                if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+                const double lv = Fc[e] / pivot;
+                Fc[e] = lv;
+                for (std::int64_t u = upd_ptr[e]; u < upd_ptr[e + 1]; ++u) {
+                    const std::int64_t t = upd_target[u];
+                    Fc[t] = __builtin_fma(-lv, Fc[upd_upper[u]], Fc[t]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const std::int64_t n = a.nrows[b];
+    for (std::int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const std::int64_t row = r0 + i;
+        const double d = Fc[diag_pos[row]];
+        if (!(__builtin_isfinite(d) && __builtin_fabs(d) > kTol)) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+        inv[static_cast<std::int64_t>(c) * a.rows + row] = 1.0 / d;
+    }
+}
+
+enum class TriMode { ilu, ssor };
+
+template<class V> __device__ inline auto vload(const V* p, std::int64_t i)
+{
+    if constexpr (sizeof(V) == 2) return dev::bf16_to_float(p[i]);
+    else return p[i];
+}
+
+// Level-scheduled triangular apply, one workgroup per (block, component).
+//  ILU : forward y_i = r_i - sum_j<i l_ij y_j; backward x_i = (y_i - sum_j>i u_ij x_j) * inv_i
+//        (IluLevelPreconditioner::apply, preconditioner.hpp:320-363)
+//  SSOR: forward y_i = (r_i - sum_j<i (w a_ij) y_j) / d_i; backward
+//        z_i = (d_i y_i - sum_j>i (w a_ij) z_j) / d_i, then * w(2-w) if w != 1
+//        (preconditioner.hpp:181-239, split_block_csr.hpp:243-309)
+// With LDS the block's component vector lives in LDS; otherwise it is kept in
+// `out` (strided by bs).
+template<TriMode MODE, class X, class V, bool LDS>
+__global__ __launch_bounds__(256) void k_tri_apply(TriArgs a, const V* __restrict__ fval, const V* __restrict__ bval,
+                                                   const X* __restrict__ dval, const X* in, X* out, X scale,
+                                                   const int* stop)
+{
+    extern __shared__ unsigned char smem_raw[];
+    if (stop && *stop) return;
+    const int bs = a.bs;
+    const int c = blockIdx.x % bs;
+    const int b = blockIdx.x / bs;
+    const std::int64_t r0 = a.row0[b];
+    const std::int64_t nb = a.nrows[b];
+    X* lds = reinterpret_cast<X*>(smem_raw);
+    auto Y = [&](std::int64_t i) -> X& {
+        if constexpr (LDS) return lds[i];
+        else return out[(r0 + i) * bs + c];
+    };
+    if constexpr (LDS) {
+        for (std::int64_t i = threadIdx.x; i < nb; i += blockDim.x) lds[i] = in[(r0 + i) * bs + c];
+    } else {
+        if (in != out)
+            for (std::int64_t i = threadIdx.x; i < nb; i += blockDim.x) out[(r0 + i) * bs + c] = in[(r0 + i) * bs + c];
+    }
+    __syncthreads();
+    const V* fv = fval + static_cast<std::int64_t>(c) * a.fnnz;
+    const V* bv = bval + static_cast<std::int64_t>(c) * a.bnnz;
+    const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
+    for (int l = a.flev[b]; l < a.flev[b + 1] - 1; ++l) {
+        const int p0 = a.flev_pos[l], p1 = a.flev_pos[l + 1];
+        for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+            const std::int64_t gp = r0 + p;
+            const std::int32_t i = a.frow[gp];
+            X acc = Y(i);
+            for (std::int64_t e = a.fptr[gp]; e < a.fptr[gp + 1]; ++e)
+                acc = dev::dfma(-static_cast<X>(vload(fv, e)), Y(a.fcol[e]), acc);
+            if constexpr (MODE == TriMode::ssor) acc = acc / dv[i];
+            Y(i) = acc;
+        }
+        __syncthreads();
+    }
+    for (int l = a.blev[b]; l < a.blev[b + 1] - 1; ++l) {
+        const int p0 = a.blev_pos[l], p1 = a.blev_pos[l + 1];
+        for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+            const std::int64_t gp = r0 + p;
+            const std::int32_t i = a.brow[gp];
+            X acc;
+            if constexpr (MODE == TriMode::ssor) acc = dv[i] * Y(i);
+            else acc = Y(i);
+            for (std::int64_t e = a.bptr[gp]; e < a.bptr[gp + 1]; ++e)
+                acc = dev::dfma(-static_cast<X>(vload(bv, e)), Y(a.bcol[e]), acc);
+            if constexpr (MODE == TriMode::ssor) acc = acc / dv[i];
+            else acc = acc * dv[i];
+            Y(i) = acc;
+        }
+        __syncthreads();
+    }
+    if constexpr (LDS) {
+        for (std::int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+            X v = lds[i];
+            if constexpr (MODE == TriMode::ssor)
+                if (scale != X(1)) v = v * scale;
+            out[(r0 + i) * bs + c] = v;
+        }
+    } else if constexpr (MODE == TriMode::ssor) {
+        if (scale != X(1))
+            for (std::int64_t i = threadIdx.x; i < nb; i += blockDim.x) out[(r0 + i) * bs + c] *= scale;
+    }
+}
+
+// ---------------------------------------------------------------- symbolic --
+
+// Encoded gather source of matrix value (row, CSR entry e): >= 0 index of
+// component 0 in the value array; <= -2: diagonal array.
+std::int64_t entry_source(owtk_matrix m, std::int64_t e) { return m->value_index(m->s.slot_of_entry[e], 0); }
+std::int64_t diag_source(owtk_matrix m, std::int64_t row) { return -2 - m->diag_index(row, 0); }
+std::int64_t comp_stride(owtk_matrix m) { return m->s.layout == Layout::sell ? 64 : 1; }
+
+// Block-restricted local rows in the original CSR order (ghost columns and
+// cross-block columns dropped), with each row's diagonal source.
+struct LocalRows {
+    std::vector<std::int64_t> off;
+    std::vector<std::int64_t> col;
+    std::vector<std::int64_t> src;
+    std::vector<std::int64_t> diag_src; // -1 when the row has no diagonal
+};
+
+LocalRows local_rows(owtk_matrix m, std::int64_t block_nodes)
+{
+    LocalRows L;
+    const std::int64_t n = m->owned;
+    L.off.assign(n + 1, 0);
+    L.diag_src.assign(n, -1);
+    L.col.reserve(m->h_columns.size() + (m->has_diag_array() ? n : 0));
+    L.src.reserve(L.col.capacity());
+    for (std::int64_t row = 0; row < n; ++row) {
+        const std::int64_t blk = block_nodes > 0 ? row / block_nodes : 0;
+        if (m->has_diag_array()) L.diag_src[row] = diag_source(m, row);
+        for (std::int32_t e = m->h_offsets[row]; e < m->h_offsets[row + 1]; ++e) {
+            const std::int64_t c = m->h_columns[e];
+            if (c >= n) continue;
+            if (block_nodes > 0 && c / block_nodes != blk) continue;
+            if (!m->has_diag_array() && c == row && L.diag_src[row] == -1) L.diag_src[row] = entry_source(m, e);
+            L.col.push_back(c);
+            L.src.push_back(entry_source(m, e));
+        }
+        if (m->has_diag_array()) {
+            // the split diagonal is a separate array; keep it out of the edge list
+        }
+        L.off[row + 1] = static_cast<std::int64_t>(L.col.size());
+    }
+    return L;
+}
+
+template<class T>
+void upload(DeviceArray<T>& d, const std::vector<T>& h)
+{
+    d.allocate(std::max<std::size_t>(h.size(), 1));
+    if (!h.empty()) OWTK_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// Build the level schedule of one triangular pair.  lower(i)/upper(i) list
+// the block-local entries of row i (global index space) in apply order.
+struct TriHost {
+    // per row: forward entries (col, src) and backward entries
+    std::vector<std::int64_t> f_off, f_col, f_src, b_off, b_col, b_src;
+};
+
+void build_schedule(owtk_precond pc, const TriHost& t)
+{
+    auto& tri = pc->tri;
+    const std::int64_t n = pc->m->owned;
+    const std::int64_t bn = pc->block_nodes > 0 ? pc->block_nodes : n;
+    tri.rows = n;
+    tri.bs = pc->m->bs;
+    tri.blocks = (n + bn - 1) / bn;
+    tri.h_row0.assign(tri.blocks, 0);
+    tri.h_nrows.assign(tri.blocks, 0);
+    tri.h_flev.assign(tri.blocks + 1, 0);
+    tri.h_blev.assign(tri.blocks + 1, 0);
+    tri.h_flev_pos.clear();
+    tri.h_blev_pos.clear();
+    std::vector<std::int32_t> frow(n), brow(n);
+    std::vector<std::int64_t> fptr(n + 1, 0), bptr(n + 1, 0);
+    std::vector<std::int32_t> fcol, bcol;
+    std::vector<std::int64_t> fsrc, bsrc;
+    fcol.reserve(t.f_col.size());
+    bcol.reserve(t.b_col.size());
+    std::vector<std::int32_t> level(n, 0);
+    tri.max_block_rows = tri.max_flevels = tri.max_blevels = tri.max_level_width = 0;
+    for (std::int64_t b = 0; b < tri.blocks; ++b) {
+        const std::int64_t r0 = b * bn;
+        const std::int64_t nb = std::min(bn, n - r0);
+        tri.h_row0[b] = static_cast<std::int32_t>(r0);
+        tri.h_nrows[b] = static_cast<std::int32_t>(nb);
+        tri.max_block_rows = std::max(tri.max_block_rows, nb);
+        for (int dir = 0; dir < 2; ++dir) {
+            const auto& off = dir == 0 ? t.f_off : t.b_off;
+            const auto& col = dir == 0 ? t.f_col : t.b_col;
+            const auto& src = dir == 0 ? t.f_src : t.b_src;
+            int depth = 0;
+            if (dir == 0) {
+                for (std::int64_t i = r0; i < r0 + nb; ++i) {
+                    int lv = 0;
+                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) lv = std::max(lv, level[col[e]] + 1);
+                    level[i] = lv;
+                    depth = std::max(depth, lv + 1);
+                }
+            } else {
+                for (std::int64_t i = r0 + nb; i-- > r0;) {
+                    int lv = 0;
+                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) lv = std::max(lv, level[col[e]] + 1);
+                    level[i] = lv;
+                    depth = std::max(depth, lv + 1);
+                }
+            }
+            // counting sort by level, stable in row order (forward) / reverse row order (backward)
+            std::vector<std::int64_t> count(depth + 1, 0);
+            for (std::int64_t i = r0; i < r0 + nb; ++i) ++count[level[i] + 1];
+            for (int l = 0; l < depth; ++l) count[l + 1] += count[l];
+            auto& lev_pos = dir == 0 ? tri.h_flev_pos : tri.h_blev_pos;
+            auto& lev = dir == 0 ? tri.h_flev : tri.h_blev;
+            lev[b] = static_cast<std::int32_t>(lev_pos.size());
+            for (int l = 0; l <= depth; ++l) lev_pos.push_back(static_cast<std::int32_t>(count[l]));
+            for (int l = 0; l < depth; ++l) tri.max_level_width = std::max(tri.max_level_width, count[l + 1] - count[l]);
+            lev[b + 1] = static_cast<std::int32_t>(lev_pos.size());
+            if (dir == 0) tri.max_flevels = std::max<std::int64_t>(tri.max_flevels, depth);
+            else tri.max_blevels = std::max<std::int64_t>(tri.max_blevels, depth);
+            std::vector<std::int64_t> next(count.begin(), count.end() - 1);
+            auto& rowmap = dir == 0 ? frow : brow;
+            auto place = [&](std::int64_t i) { rowmap[r0 + next[level[i]]++] = static_cast<std::int32_t>(i - r0); };
+            if (dir == 0)
+                for (std::int64_t i = r0; i < r0 + nb; ++i) place(i);
+            else
+                for (std::int64_t i = r0 + nb; i-- > r0;) place(i);
+            auto& ptr = dir == 0 ? fptr : bptr;
+            auto& ocol = dir == 0 ? fcol : bcol;
+            auto& osrc = dir == 0 ? fsrc : bsrc;
+            for (std::int64_t p = 0; p < nb; ++p) {
+                const std::int64_t i = r0 + rowmap[r0 + p];
+                for (std::int64_t e = off[i]; e < off[i + 1]; ++e) {
+                    ocol.push_back(static_cast<std::int32_t>(col[e] - r0));
+                    osrc.push_back(src[e]);
+                }
+                ptr[r0 + p + 1] = static_cast<std::int64_t>(ocol.size());
+            }
+            // blocks are consecutive, so ptr[r0] is the previous block's end
+        }
+    }
+    // fix: make ptr monotone across block starts
+    tri.fnnz = static_cast<std::int64_t>(fcol.size());
+    tri.bnnz = static_cast<std::int64_t>(bcol.size());
+    upload(tri.row0, tri.h_row0);
+    upload(tri.nrows, tri.h_nrows);
+    upload(tri.flev, tri.h_flev);
+    upload(tri.blev, tri.h_blev);
+    upload(tri.flev_pos, tri.h_flev_pos);
+    upload(tri.blev_pos, tri.h_blev_pos);
+    upload(tri.frow, frow);
+    upload(tri.brow, brow);
+    upload(tri.fptr, fptr);
+    upload(tri.bptr, bptr);
+    upload(tri.fcol, fcol);
+    upload(tri.bcol, bcol);
+    upload(tri.fsrc, fsrc);
+    upload(tri.bsrc, bsrc);
+}
+
+TriArgs tri_args(const TriSchedule& t)
+{
+    return TriArgs{t.row0.get(), t.nrows.get(), t.flev.get(), t.blev.get(), t.flev_pos.get(), t.blev_pos.get(),
+                   t.frow.get(), t.brow.get(), t.fptr.get(), t.bptr.get(), t.fcol.get(), t.bcol.get(),
+                   t.fnnz, t.bnnz, t.rows, t.bs};
+}
+
+std::size_t lds_limit(owtk_precond pc)
+{
+    static thread_local int cached_device = -1;
+    static thread_local std::size_t cached = 0;
+    if (cached_device != pc->m->ctx->device) {
+        hipDeviceProp_t prop;
+        OWTK_HIP(hipGetDeviceProperties(&prop, pc->m->ctx->device));
+        cached = prop.sharedMemPerBlock;
+        cached_device = pc->m->ctx->device;
+    }
+    return cached;
+}
+
+void read_flag(owtk_precond pc, int* value)
+{
+    OWTK_HIP(hipMemcpyAsync(value, pc->flag.get(), sizeof(int), hipMemcpyDeviceToHost, pc->m->ctx->stream));
+    pc->m->ctx->sync();
+}
+
+void clear_flag(owtk_precond pc) { kern::fill_zero_bytes(pc->m->ctx->stream, pc->flag.get(), sizeof(int) * 4); }
+
+unsigned int grid_for(std::int64_t n) { return dev::stream_grid(n, 4); }
+
+// Value arrays in the preconditioner's storage precision.
+void refresh_storage(owtk_precond pc)
+{
+    auto& t = pc->tri;
+    const hipStream_t st = pc->m->ctx->stream;
+    const int bs = pc->m->bs;
+    if (pc->storage == OWTK_F32 || pc->storage == OWTK_BF16) {
+        const std::int64_t nd = t.rows * bs;
+        if (!t.dval32.get()) t.dval32.allocate(std::max<std::int64_t>(nd, 1));
+        kern::convert(st, t.dval32.get(), t.dval64.get(), static_cast<std::size_t>(nd));
+    }
+    if (pc->storage == OWTK_F32) {
+        if (!t.fval32.get()) t.fval32.allocate(std::max<std::int64_t>(t.fnnz * bs, 1));
+        if (!t.bval32.get()) t.bval32.allocate(std::max<std::int64_t>(t.bnnz * bs, 1));
+        if (t.fnnz) kern::convert(st, t.fval32.get(), t.fval64.get(), static_cast<std::size_t>(t.fnnz * bs));
+        if (t.bnnz) kern::convert(st, t.bval32.get(), t.bval64.get(), static_cast<std::size_t>(t.bnnz * bs));
+    } else if (pc->storage == OWTK_BF16) {
+        if (!t.fval16.get()) t.fval16.allocate(std::max<std::int64_t>(t.fnnz * bs, 1));
+        if (!t.bval16.get()) t.bval16.allocate(std::max<std::int64_t>(t.bnnz * bs, 1));
+        if (t.fnnz)
+            kern::k_to_bf16<<<grid_for(t.fnnz * bs), kBlock, 0, st>>>(t.fval16.get(), t.fval64.get(), t.fnnz * bs);
+        if (t.bnnz)
+            kern::k_to_bf16<<<grid_for(t.bnnz * bs), kBlock, 0, st>>>(t.bval16.get(), t.bval64.get(), t.bnnz * bs);
+        OWTK_LAUNCH_CHECK();
+    }
+}
+
+// ------------------------------------------------------------------- Jacobi --
+void jacobi_setup(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    if (m->dense()) {
+        if (m->bs > 8 || m->s.layout != Layout::sell)
+            throw unsupported_error("dense block-Jacobi supports block sizes up to 4");
+        std::vector<std::int64_t> slot(m->owned);
+        for (std::int64_t row = 0; row < m->owned; ++row) {
+            std::int64_t s = -1;
+            for (std::int32_t e = m->h_offsets[row]; e < m->h_offsets[row + 1]; ++e)
+                if (m->h_columns[e] == row) {
+                    s = m->s.slot_of_entry[e];
+                    break;
+                }
+            if (s < 0) throw std::invalid_argument("dense-block CSR row has no diagonal block");
+            slot[row] = s;
+        }
+        upload(pc->tri.dsrc, slot);
+        pc->inv64.allocate(static_cast<std::size_t>(m->owned) * m->bs * m->bs);
+        return;
+    }
+    std::vector<std::int64_t> src(m->owned);
+    for (std::int64_t row = 0; row < m->owned; ++row) {
+        if (m->has_diag_array()) {
+            src[row] = diag_source(m, row);
+            continue;
+        }
+        std::int64_t s = std::numeric_limits<std::int64_t>::min();
+        for (std::int32_t e = m->h_offsets[row]; e < m->h_offsets[row + 1]; ++e)
+            if (m->h_columns[e] == row) {
+                s = entry_source(m, e);
+                break;
+            }
+        if (s == std::numeric_limits<std::int64_t>::min())
+            throw std::invalid_argument("BlockCsrMatrix row has no diagonal");
+        src[row] = s;
+    }
+    upload(pc->tri.dsrc, src);
+    pc->inv64.allocate(static_cast<std::size_t>(m->owned) * m->bs);
+}
+
+void jacobi_numeric(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    const hipStream_t st = m->ctx->stream;
+    clear_flag(pc);
+    if (m->dense()) {
+        const unsigned int g = static_cast<unsigned int>((m->owned + 255) / 256);
+        switch (m->bs) {
+        case 1: k_block_inverse<1><<<g, kBlock, 0, st>>>(pc->inv64.get(), pc->tri.dsrc.get(), m->owned, m->v64.get(), pc->flag.get()); break;
+        case 2: k_block_inverse<2><<<g, kBlock, 0, st>>>(pc->inv64.get(), pc->tri.dsrc.get(), m->owned, m->v64.get(), pc->flag.get()); break;
+        case 3: k_block_inverse<3><<<g, kBlock, 0, st>>>(pc->inv64.get(), pc->tri.dsrc.get(), m->owned, m->v64.get(), pc->flag.get()); break;
+        default: k_block_inverse<4><<<g, kBlock, 0, st>>>(pc->inv64.get(), pc->tri.dsrc.get(), m->owned, m->v64.get(), pc->flag.get()); break;
+        }
+        OWTK_LAUNCH_CHECK();
+    } else {
+        const double minimum = pc->minimum_diagonal > 0 ? pc->minimum_diagonal : kTol;
+        k_jacobi_inverse<<<grid_for(m->owned * m->bs), kBlock, 0, st>>>(pc->inv64.get(), pc->tri.dsrc.get(),
+                                                                       m->owned, m->bs, comp_stride(m),
+                                                                       m->v64.get(), m->d64.get(), minimum,
+                                                                       pc->flag.get());
+        OWTK_LAUNCH_CHECK();
+    }
+    int flag = 0;
+    read_flag(pc, &flag);
+    if (flag != flag_ok) {
+        if (m->dense()) throw std::invalid_argument("block-Jacobi preconditioner has a singular diagonal block");
+        throw std::invalid_argument("Jacobi preconditioner has a singular diagonal");
+    }
+    if (pc->storage != OWTK_F64) {
+        if (!pc->inv32.get()) pc->inv32.allocate(pc->inv64.count);
+        kern::convert(st, pc->inv32.get(), pc->inv64.get(), pc->inv64.count);
+    }
+}
+
+// --------------------------------------------------------------------- SSOR --
+void ssor_setup(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    if (m->dense()) throw unsupported_error("dense-block SSOR is not available for this block size");
+    if (!(pc->omega > 0.0 && pc->omega < 2.0)) throw std::invalid_argument("SSOR omega must be in (0,2)");
+    const LocalRows L = local_rows(m, pc->block_nodes);
+    TriHost t;
+    const std::int64_t n = m->owned;
+    t.f_off.assign(n + 1, 0);
+    t.b_off.assign(n + 1, 0);
+    std::vector<std::int64_t> dsrc(n);
+    for (std::int64_t row = 0; row < n; ++row) {
+        if (L.diag_src[row] == -1) throw std::invalid_argument("SSOR preconditioner row has no diagonal");
+        dsrc[row] = L.diag_src[row];
+        for (std::int64_t e = L.off[row]; e < L.off[row + 1]; ++e) {
+            if (L.col[e] < row) {
+                t.f_col.push_back(L.col[e]);
+                t.f_src.push_back(L.src[e]);
+            } else if (L.col[e] > row) {
+                t.b_col.push_back(L.col[e]);
+                t.b_src.push_back(L.src[e]);
+            }
+        }
+        t.f_off[row + 1] = static_cast<std::int64_t>(t.f_col.size());
+        t.b_off[row + 1] = static_cast<std::int64_t>(t.b_col.size());
+    }
+    build_schedule(pc, t);
+    upload(pc->tri.dsrc, dsrc);
+    const int bs = m->bs;
+    pc->tri.fval64.allocate(std::max<std::int64_t>(pc->tri.fnnz * bs, 1));
+    pc->tri.bval64.allocate(std::max<std::int64_t>(pc->tri.bnnz * bs, 1));
+    pc->tri.dval64.allocate(std::max<std::int64_t>(n * bs, 1));
+}
+
+void ssor_numeric(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    auto& t = pc->tri;
+    const hipStream_t st = m->ctx->stream;
+    const int bs = m->bs;
+    const std::int64_t stride = comp_stride(m);
+    clear_flag(pc);
+    if (t.fnnz)
+        k_gather<<<grid_for(t.fnnz * bs), kBlock, 0, st>>>(t.fval64.get(), t.fsrc.get(), t.fnnz, bs, stride,
+                                                           m->v64.get(), m->d64.get(), pc->omega);
+    if (t.bnnz)
+        k_gather<<<grid_for(t.bnnz * bs), kBlock, 0, st>>>(t.bval64.get(), t.bsrc.get(), t.bnnz, bs, stride,
+                                                           m->v64.get(), m->d64.get(), pc->omega);
+    k_gather<<<grid_for(t.rows * bs), kBlock, 0, st>>>(t.dval64.get(), t.dsrc.get(), t.rows, bs, stride,
+                                                       m->v64.get(), m->d64.get(), 1.0);
+    k_check_diag<<<grid_for(t.rows * bs), kBlock, 0, st>>>(t.dval64.get(), t.rows * bs, kTol, pc->flag.get());
+    OWTK_LAUNCH_CHECK();
+    int flag = 0;
+    read_flag(pc, &flag);
+    if (flag != flag_ok) throw std::invalid_argument("SSOR preconditioner has a singular diagonal");
+    refresh_storage(pc);
+}
+
+// ---------------------------------------------------------------------- ILU --
+void ilu_setup(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    if (m->dense()) throw unsupported_error("ILU is defined for component-diagonal matrices");
+    const LocalRows L = local_rows(m, pc->block_nodes);
+    const std::int64_t n = m->owned;
+    const std::size_t fill = static_cast<std::size_t>(std::max(0, pc->fill_level));
+    auto& f = pc->ilu;
+    f.h_offsets.assign(1, 0);
+    f.h_columns.clear();
+    f.h_diag.assign(n, 0);
+    std::vector<std::int64_t> src;
+    std::vector<std::size_t> levels;
+    constexpr std::int64_t no_source = -1;
+    // build_local_pattern (preconditioner.hpp:373-427), block-restricted
+    for (std::int64_t row = 0; row < n; ++row) {
+        std::map<std::int64_t, std::pair<std::size_t, std::int64_t>> entries;
+        if (m->has_diag_array()) entries.emplace(row, std::pair{std::size_t(0), L.diag_src[row]});
+        for (std::int64_t e = L.off[row]; e < L.off[row + 1]; ++e) {
+            if (!entries.emplace(L.col[e], std::pair{std::size_t(0), L.src[e]}).second)
+                throw std::invalid_argument("ILU requires unique matrix columns");
+        }
+        for (auto it = entries.begin(); it != entries.end() && it->first < row; ++it) {
+            const std::int64_t pivot = it->first;
+            const std::size_t lower_level = it->second.first;
+            if (lower_level >= fill) continue;
+            for (std::int64_t upper = f.h_diag[pivot] + 1; upper < f.h_offsets[pivot + 1]; ++upper) {
+                if (levels[upper] > fill - lower_level - 1) continue;
+                const std::size_t level = lower_level + levels[upper] + 1;
+                auto [target, inserted] = entries.emplace(f.h_columns[upper], std::pair{level, no_source});
+                if (!inserted) target->second.first = std::min(target->second.first, level);
+            }
+        }
+        bool diagonal_found = false;
+        for (const auto& [column, entry] : entries) {
+            const std::int64_t destination = static_cast<std::int64_t>(f.h_columns.size());
+            f.h_columns.push_back(column);
+            levels.push_back(entry.first);
+            src.push_back(entry.second);
+            if (column == row) {
+                f.h_diag[row] = destination;
+                diagonal_found = true;
+            }
+        }
+        if (!diagonal_found) throw std::invalid_argument("ILU(0) row has no diagonal");
+        f.h_offsets.push_back(static_cast<std::int64_t>(f.h_columns.size()));
+    }
+    f.nnz = static_cast<std::int64_t>(f.h_columns.size());
+    // update lists: for each lower entry (i,k), targets (i,j) of the pivot row's upper entries (k,j)
+    std::vector<std::int64_t> upd_ptr(f.nnz + 1, 0), upd_target, upd_upper;
+    for (std::int64_t row = 0; row < n; ++row) {
+        for (std::int64_t e = f.h_offsets[row]; e < f.h_offsets[row + 1]; ++e) {
+            if (e < f.h_diag[row]) {
+                const std::int64_t k = f.h_columns[e];
+                for (std::int64_t u = f.h_diag[k] + 1; u < f.h_offsets[k + 1]; ++u) {
+                    const std::int64_t j = f.h_columns[u];
+                    // find (row, j)
+                    const auto begin = f.h_columns.begin() + f.h_offsets[row];
+                    const auto end = f.h_columns.begin() + f.h_offsets[row + 1];
+                    const auto it = std::lower_bound(begin, end, j);
+                    if (it != end && *it == j) {
+                        upd_target.push_back(static_cast<std::int64_t>(it - f.h_columns.begin()));
+                        upd_upper.push_back(u);
+                    }
+                }
+            }
+            upd_ptr[e + 1] = static_cast<std::int64_t>(upd_target.size());
+        }
+    }
+    upload(f.offsets, f.h_offsets);
+    upload(f.diag_pos, f.h_diag);
+    upload(f.src, src);
+    upload(f.upd_ptr, upd_ptr);
+    upload(f.upd_target, upd_target);
+    upload(f.upd_upper, upd_upper);
+    upload(f.columns, f.h_columns);
+    f.values.allocate(std::max<std::int64_t>(f.nnz * m->bs, 1));
+    // apply schedule: lower entries before the diagonal, upper after, factor order
+    TriHost t;
+    t.f_off.assign(n + 1, 0);
+    t.b_off.assign(n + 1, 0);
+    for (std::int64_t row = 0; row < n; ++row) {
+        for (std::int64_t e = f.h_offsets[row]; e < f.h_diag[row]; ++e) {
+            t.f_col.push_back(f.h_columns[e]);
+            t.f_src.push_back(e);
+        }
+        for (std::int64_t e = f.h_diag[row] + 1; e < f.h_offsets[row + 1]; ++e) {
+            t.b_col.push_back(f.h_columns[e]);
+            t.b_src.push_back(e);
+        }
+        t.f_off[row + 1] = static_cast<std::int64_t>(t.f_col.size());
+        t.b_off[row + 1] = static_cast<std::int64_t>(t.b_col.size());
+    }
+    build_schedule(pc, t);
+    const int bs = m->bs;
+    pc->tri.fval64.allocate(std::max<std::int64_t>(pc->tri.fnnz * bs, 1));
+    pc->tri.bval64.allocate(std::max<std::int64_t>(pc->tri.bnnz * bs, 1));
+    pc->tri.dval64.allocate(std::max<std::int64_t>(n * bs, 1));
+}
+
+void ilu_numeric(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    auto& f = pc->ilu;
+    auto& t = pc->tri;
+    const hipStream_t st = m->ctx->stream;
+    const int bs = m->bs;
+    clear_flag(pc);
+    k_gather<<<grid_for(f.nnz * bs), kBlock, 0, st>>>(f.values.get(), f.src.get(), f.nnz, bs, comp_stride(m),
+                                                      m->v64.get(), m->d64.get(), 1.0);
+    OWTK_LAUNCH_CHECK();
+    const unsigned int groups = static_cast<unsigned int>(t.blocks * bs);
+    k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), f.offsets.get(), f.diag_pos.get(), f.upd_ptr.get(),
+                                        f.upd_target.get(), f.upd_upper.get(), f.columns.get(), f.values.get(),
+                                        f.nnz, t.dval64.get(), pc->flag.get());
+    OWTK_LAUNCH_CHECK();
+    int flag = 0;
+    read_flag(pc, &flag);
+    if (flag != flag_ok) throw zero_pivot_error("zero pivot in ILU(0)");
+    if (t.fnnz)
+        k_gather_cm<<<grid_for(t.fnnz * bs), kBlock, 0, st>>>(t.fval64.get(), t.fsrc.get(), t.fnnz, bs,
+                                                              f.values.get(), f.nnz);
+    if (t.bnnz)
+        k_gather_cm<<<grid_for(t.bnnz * bs), kBlock, 0, st>>>(t.bval64.get(), t.bsrc.get(), t.bnnz, bs,
+                                                              f.values.get(), f.nnz);
+    OWTK_LAUNCH_CHECK();
+    refresh_storage(pc);
+}
+
+// -------------------------------------------------------------------- apply --
+template<TriMode MODE, class X, class V>
+void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X* in, X* out, const int* stop)
+{
+    const auto& t = pc->tri;
+    const unsigned int groups = static_cast<unsigned int>(t.blocks * t.bs);
+    const std::size_t need = static_cast<std::size_t>(t.max_block_rows) * sizeof(X);
+    const int threads = t.max_level_width > 128 ? 256 : (t.max_level_width > 64 ? 128 : 64);
+    const X scale = static_cast<X>(pc->omega * (2.0 - pc->omega));
+    const hipStream_t st = pc->m->ctx->stream;
+    if (need <= lds_limit(pc)) {
+        k_tri_apply<MODE, X, V, true><<<groups, threads, need, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
+    } else {
+        k_tri_apply<MODE, X, V, false><<<groups, threads, 0, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
+    }
+    OWTK_LAUNCH_CHECK();
+}
+
+} // namespace
+
+void precond_apply_f64(owtk_precond pc, const double* in, double* out, const int* stop)
+{
+    owtk_matrix m = pc->m;
+    const hipStream_t st = m->ctx->stream;
+    const std::int64_t n = pc->n();
+    switch (pc->kind) {
+    case OWTK_PC_IDENTITY:
+        if (in != out) OWTK_HIP(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        return;
+    case OWTK_PC_JACOBI:
+        if (m->dense()) {
+            const unsigned int g = static_cast<unsigned int>((m->owned + 255) / 256);
+            switch (m->bs) {
+            case 1: k_block_jacobi_apply<1, double, double><<<g, kBlock, 0, st>>>(pc->inv64.get(), in, out, m->owned, stop); break;
+            case 2: k_block_jacobi_apply<2, double, double><<<g, kBlock, 0, st>>>(pc->inv64.get(), in, out, m->owned, stop); break;
+            case 3: k_block_jacobi_apply<3, double, double><<<g, kBlock, 0, st>>>(pc->inv64.get(), in, out, m->owned, stop); break;
+            default: k_block_jacobi_apply<4, double, double><<<g, kBlock, 0, st>>>(pc->inv64.get(), in, out, m->owned, stop); break;
+            }
+        } else {
+            k_jacobi_apply<double, double><<<grid_for(n), kBlock, 0, st>>>(pc->inv64.get(), in, out, n, stop);
+        }
+        OWTK_LAUNCH_CHECK();
+        return;
+    case OWTK_PC_SSOR:
+        tri_launch<TriMode::ssor, double, double>(pc, pc->tri.fval64.get(), pc->tri.bval64.get(),
+                                                  pc->tri.dval64.get(), in, out, stop);
+        return;
+    case OWTK_PC_ILU:
+        tri_launch<TriMode::ilu, double, double>(pc, pc->tri.fval64.get(), pc->tri.bval64.get(),
+                                                 pc->tri.dval64.get(), in, out, stop);
+        return;
+    default:
+        throw unsupported_error("preconditioner kind not available on the FP64 path");
+    }
+}
+
+void precond_apply_f32(owtk_precond pc, const float* in, float* out, const int* stop)
+{
+    owtk_matrix m = pc->m;
+    const hipStream_t st = m->ctx->stream;
+    const std::int64_t n = pc->n();
+    if (pc->kind != OWTK_PC_IDENTITY && pc->storage == OWTK_F64)
+        throw std::invalid_argument("FP32 apply needs F32 or BF16 preconditioner storage");
+    switch (pc->kind) {
+    case OWTK_PC_IDENTITY:
+        if (in != out) OWTK_HIP(hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        return;
+    case OWTK_PC_JACOBI:
+        if (m->dense()) {
+            const unsigned int g = static_cast<unsigned int>((m->owned + 255) / 256);
+            switch (m->bs) {
+            case 1: k_block_jacobi_apply<1, float, float><<<g, kBlock, 0, st>>>(pc->inv32.get(), in, out, m->owned, stop); break;
+            case 2: k_block_jacobi_apply<2, float, float><<<g, kBlock, 0, st>>>(pc->inv32.get(), in, out, m->owned, stop); break;
+            case 3: k_block_jacobi_apply<3, float, float><<<g, kBlock, 0, st>>>(pc->inv32.get(), in, out, m->owned, stop); break;
+            default: k_block_jacobi_apply<4, float, float><<<g, kBlock, 0, st>>>(pc->inv32.get(), in, out, m->owned, stop); break;
+            }
+        } else {
+            k_jacobi_apply<float, float><<<grid_for(n), kBlock, 0, st>>>(pc->inv32.get(), in, out, n, stop);
+        }
+        OWTK_LAUNCH_CHECK();
+        return;
+    case OWTK_PC_SSOR:
+    case OWTK_PC_ILU: {
+        const auto& t = pc->tri;
+        const bool ilu = pc->kind == OWTK_PC_ILU;
+        if (pc->storage == OWTK_F32) {
+            if (ilu) tri_launch<TriMode::ilu, float, float>(pc, t.fval32.get(), t.bval32.get(), t.dval32.get(), in, out, stop);
+            else tri_launch<TriMode::ssor, float, float>(pc, t.fval32.get(), t.bval32.get(), t.dval32.get(), in, out, stop);
+        } else {
+            if (ilu) tri_launch<TriMode::ilu, float, std::uint16_t>(pc, t.fval16.get(), t.bval16.get(), t.dval32.get(), in, out, stop);
+            else tri_launch<TriMode::ssor, float, std::uint16_t>(pc, t.fval16.get(), t.bval16.get(), t.dval32.get(), in, out, stop);
+        }
+        return;
+    }
+    default:
+        throw unsupported_error("preconditioner kind not available on the FP32 path");
+    }
+}
+
+std::int64_t precond_apply_bytes(owtk_precond pc)
+{
+    const std::int64_t n = pc->n();
+    const int sv = pc->storage == OWTK_F64 ? 8 : (pc->storage == OWTK_F32 ? 4 : 2);
+    const int sx = pc->storage == OWTK_F64 ? 8 : 4;
+    switch (pc->kind) {
+    case OWTK_PC_IDENTITY: return 2 * n * sx;
+    case OWTK_PC_JACOBI:
+        return pc->m->dense() ? n * pc->m->bs * sv + 2 * n * sx : n * sv + 2 * n * sx;
+    default: {
+        const auto& t = pc->tri;
+        const int bs = pc->m->bs;
+        // factor values + block-local columns + row positions/pointers + diagonal + in/out
+        return (t.fnnz + t.bnnz) * (bs * sv + 4) + 2 * t.rows * (4 + 8) + n * sx + 2 * n * sx;
+    }
+    }
+}
+
+} // namespace owtk
+
+extern "C" {
+
+void owtk_precond_params_default(owtk_precond_params* p, int32_t kind)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->kind = kind;
+    p->storage = OWTK_F64;
+    p->omega = 1.0;
+    p->minimum_diagonal = 0.0;
+    p->block_nodes = 0;
+    p->fill_level = 0;
+    p->learned_model = nullptr;
+}
+
+int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_precond* out)
+{
+    return guarded([&] {
+        if (!m || !params || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        m->ctx->activate();
+        auto pc = std::make_unique<owtk_precond_s>();
+        pc->m = m;
+        pc->kind = params->kind;
+        pc->storage = params->storage;
+        pc->omega = params->omega;
+        pc->minimum_diagonal = params->minimum_diagonal;
+        pc->block_nodes = params->block_nodes;
+        pc->fill_level = params->fill_level;
+        if (pc->storage < OWTK_F64 || pc->storage > OWTK_BF16) throw std::invalid_argument("unknown storage type");
+        if (pc->block_nodes < 0) throw std::invalid_argument("negative block size");
+        pc->flag.allocate(4);
+        switch (pc->kind) {
+        case OWTK_PC_IDENTITY: break;
+        case OWTK_PC_JACOBI:
+            if (pc->storage == OWTK_BF16) pc->storage = OWTK_F32;
+            jacobi_setup(pc.get());
+            jacobi_numeric(pc.get());
+            break;
+        case OWTK_PC_SSOR:
+            ssor_setup(pc.get());
+            ssor_numeric(pc.get());
+            break;
+        case OWTK_PC_ILU:
+            ilu_setup(pc.get());
+            ilu_numeric(pc.get());
+            break;
+        default: throw unsupported_error("preconditioner kind not available");
+        }
+        pc->updates = 1;
+        m->ctx->sync();
+        *out = pc.release();
+    });
+}
+
+int owtk_precond_update_values(owtk_precond pc)
+{
+    return guarded([&] {
+        if (!pc) throw std::invalid_argument("null preconditioner");
+        pc->m->ctx->activate();
+        switch (pc->kind) {
+        case OWTK_PC_JACOBI: jacobi_numeric(pc); break;
+        case OWTK_PC_SSOR: ssor_numeric(pc); break;
+        case OWTK_PC_ILU: ilu_numeric(pc); break;
+        default: break;
+        }
+        ++pc->updates;
+        pc->m->ctx->sync();
+    });
+}
+
+int owtk_precond_apply(owtk_precond pc, const void* in, void* out)
+{
+    return guarded([&] {
+        if (!pc) throw std::invalid_argument("null preconditioner");
+        if (!in || !out) throw std::invalid_argument("null vector");
+        pc->m->ctx->activate();
+        if (pc->storage == OWTK_F64) precond_apply_f64(pc, static_cast<const double*>(in), static_cast<double*>(out), nullptr);
+        else precond_apply_f32(pc, static_cast<const float*>(in), static_cast<float*>(out), nullptr);
+    });
+}
+
+int owtk_precond_numeric_updates(owtk_precond pc, uint64_t* count)
+{
+    return guarded([&] {
+        if (!pc || !count) throw std::invalid_argument("null argument");
+        *count = pc->updates;
+    });
+}
+
+int owtk_precond_info_get(owtk_precond pc, owtk_precond_info* info)
+{
+    return guarded([&] {
+        if (!pc || !info) throw std::invalid_argument("null argument");
+        info->kind = pc->kind;
+        info->storage = pc->storage;
+        info->blocks = pc->tri.blocks;
+        info->forward_levels = pc->tri.max_flevels;
+        info->backward_levels = pc->tri.max_blevels;
+        info->factor_entries = pc->ilu.nnz;
+        info->apply_bytes = precond_apply_bytes(pc);
+    });
+}
+
+int owtk_precond_ilu_factors(owtk_precond pc, int64_t* entries, int64_t* row_offsets, int64_t* columns,
+                             double* values, double* inverse_diagonal)
+{
+    return guarded([&] {
+        if (!pc || !entries) throw std::invalid_argument("null argument");
+        if (pc->kind != OWTK_PC_ILU) throw std::invalid_argument("not an ILU preconditioner");
+        pc->m->ctx->activate();
+        const auto& f = pc->ilu;
+        *entries = f.nnz;
+        if (row_offsets) std::copy(f.h_offsets.begin(), f.h_offsets.end(), row_offsets);
+        if (columns) std::copy(f.h_columns.begin(), f.h_columns.end(), columns);
+        const int bs = pc->m->bs;
+        if (values) {
+            std::vector<double> cm(static_cast<std::size_t>(f.nnz * bs));
+            OWTK_HIP(hipMemcpy(cm.data(), f.values.get(), cm.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (std::int64_t e = 0; e < f.nnz; ++e)
+                for (int c = 0; c < bs; ++c) values[e * bs + c] = cm[c * f.nnz + e];
+        }
+        if (inverse_diagonal) {
+            const std::int64_t n = pc->m->owned;
+            std::vector<double> cm(static_cast<std::size_t>(n * bs));
+            OWTK_HIP(hipMemcpy(cm.data(), pc->tri.dval64.get(), cm.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (std::int64_t r = 0; r < n; ++r)
+                for (int c = 0; c < bs; ++c) inverse_diagonal[r * bs + c] = cm[c * n + r];
+        }
+    });
+}
+
+int owtk_precond_destroy(owtk_precond pc)
+{
+    return guarded([&] {
+        if (!pc) return;
+        pc->m->ctx->activate();
+        pc->m->ctx->sync();
+        delete pc;
+    });
+}
+
+} // extern "C"

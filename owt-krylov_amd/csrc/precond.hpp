@@ -1,0 +1,90 @@
+// Preconditioner objects shared by the C ABI and the device solvers.
+#pragma once
+
+#include "internal.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace owtk {
+
+// Level-scheduled block triangular system (one per preconditioner).
+//
+// The owned rows are cut into blocks of consecutive rows (block-Jacobi
+// partition; one block when block_nodes == 0).  Inside a block, the forward
+// sweep visits rows level by level, where a row's level is one more than the
+// deepest lower-neighbour level; the backward sweep does the same from the
+// bottom.  Rows within one level are independent, so one workgroup per
+// (block, component) runs a level in parallel and separates levels with a
+// barrier.  The per-row accumulation keeps the reference's entry order.
+struct TriSchedule {
+    std::int64_t blocks = 0;
+    std::int64_t rows = 0;
+    int bs = 1;
+    // per block
+    std::vector<std::int32_t> h_row0, h_nrows, h_flev, h_blev; // h_flev/h_blev: first level index of block
+    DeviceArray<std::int32_t> row0, nrows, flev, blev;         // flev/blev size blocks+1
+    // level -> first position (size levels+1), positions are block-local
+    DeviceArray<std::int32_t> flev_pos, blev_pos;
+    std::vector<std::int32_t> h_flev_pos, h_blev_pos;
+    // position (global, level order) -> block-local row
+    DeviceArray<std::int32_t> frow, brow;
+    // entry pointers per position (global), block-local columns
+    DeviceArray<std::int64_t> fptr, bptr;
+    DeviceArray<std::int32_t> fcol, bcol;
+    std::int64_t fnnz = 0, bnnz = 0;
+    std::int64_t max_block_rows = 0;
+    std::int64_t max_flevels = 0, max_blevels = 0;
+    std::int64_t max_level_width = 0;
+    // values, component-major: [c][entry]
+    DeviceArray<double> fval64, bval64, dval64; // dval: ILU inverse diagonal / SSOR diagonal, [c][row]
+    DeviceArray<float> fval32, bval32, dval32;
+    DeviceArray<std::uint16_t> fval16, bval16;
+    // gather sources of the apply values (numeric update)
+    DeviceArray<std::int64_t> fsrc, bsrc, dsrc;
+};
+
+struct IluFactor {
+    // factor pattern, reference accessor layout (global rows, sorted columns)
+    std::vector<std::int64_t> h_offsets, h_columns, h_diag;
+    std::int64_t nnz = 0;
+    DeviceArray<std::int64_t> diag_pos;      // factor entry of each row's diagonal
+    DeviceArray<std::int64_t> src;           // matrix gather source per factor entry (-1: fill)
+    DeviceArray<std::int64_t> columns;       // global column per factor entry
+    DeviceArray<std::int64_t> upd_ptr;       // per factor entry (lower entries): update list range
+    DeviceArray<std::int64_t> upd_target, upd_upper;
+    DeviceArray<std::int64_t> offsets;       // factor row offsets (global)
+    DeviceArray<double> values;              // [c][entry] FP64 factor
+};
+
+} // namespace owtk
+
+struct owtk_precond_s {
+    owtk_matrix m = nullptr;
+    int kind = OWTK_PC_IDENTITY;
+    int storage = OWTK_F64;
+    double omega = 1.0;
+    double minimum_diagonal = 0.0;
+    std::int64_t block_nodes = 0;
+    int fill_level = 0;
+    std::uint64_t updates = 0;
+    // Jacobi: inverse diagonal (component-diagonal) or inverse blocks (dense)
+    owtk::DeviceArray<double> inv64;
+    owtk::DeviceArray<float> inv32;
+    owtk::TriSchedule tri;
+    owtk::IluFactor ilu;
+    owtk::DeviceArray<int> flag;   // numeric failure flag (zero pivot / singular)
+    void* learned = nullptr;       // owtk_gnn model for OWTK_PC_LEARNED
+    std::int64_t n() const { return m->owned * m->bs; }
+};
+
+namespace owtk {
+
+// Apply z = M^{-1} r for the solver path.  `stop` (device, may be null) turns
+// the launches into no-ops once a solver has stopped.
+void precond_apply_f64(owtk_precond pc, const double* in, double* out, const int* stop);
+void precond_apply_f32(owtk_precond pc, const float* in, float* out, const int* stop);
+// Bytes one apply moves (algorithmic).
+std::int64_t precond_apply_bytes(owtk_precond pc);
+
+} // namespace owtk

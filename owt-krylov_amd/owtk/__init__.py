@@ -1,0 +1,422 @@
+"""owtk - MI355X (gfx950) execution policy for the OWT-Krylov block-CSR solve path.
+
+Thin Python host layer over the C ABI of libowtk_hip.so (include/owtk_hip.h):
+device context and buffers, the three block-CSR matrix forms, preconditioners
+and device-resident solvers, plus the synthetic-grid generators.  The compute
+path is the HIP library; nothing here computes on the CPU besides set-up.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_UNIFORM, MATRIX_BSR,
+                   MATRIX_CSR, MATRIX_SPLIT, ORDER_HILBERT, ORDER_IDENTITY, ORDER_RCM, PC_IDENTITY, PC_ILU,
+                   PC_JACOBI, PC_LEARNED, PC_SSOR, InvalidArgumentError, NoDeviceError, OwtkError,
+                   UnsupportedError, ZeroPivotError, check, lib)
+
+__all__ = [
+    "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "fgmres",
+    "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "device_count",
+    "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
+    "UnsupportedError", "lib", "STATUS", "REASON",
+]
+
+STATUS = ["converged", "maximum_iterations", "breakdown", "diverged", "invalid_input"]
+REASON = ["none", "arnoldi_invariant_subspace", "singular_projected_system", "biorthogonality_loss",
+          "alpha_denominator", "omega_denominator", "omega_zero", "idr_shadow_space", "idr_small_system",
+          "non_finite_scalar"]
+
+_NP = {F64: np.float64, F32: np.float32, BF16: np.float32}
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().owtk_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """One GPU, one HIP stream (owtk_context)."""
+
+    def __init__(self, device: int = 0):
+        self.handle = C.c_void_p()
+        check(lib().owtk_context_create(device, C.byref(self.handle)))
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_context_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self) -> None:
+        check(lib().owtk_context_synchronize(self.handle))
+
+    def empty(self, count: int, dtype=np.float64) -> "DeviceBuffer":
+        return DeviceBuffer(self, count, dtype)
+
+    def upload(self, array: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(array)
+        buf = DeviceBuffer(self, a.size, a.dtype)
+        buf.upload(a)
+        return buf
+
+    def dot(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int | None = None) -> float:
+        out = C.c_double()
+        n = a.count if count is None else count
+        check(lib().owtk_dot(self.handle, a.owtk_dtype, n, a.ptr, b.ptr, C.byref(out)))
+        return out.value
+
+    def norm(self, a: "DeviceBuffer", count: int | None = None) -> float:
+        out = C.c_double()
+        n = a.count if count is None else count
+        check(lib().owtk_norm(self.handle, a.owtk_dtype, n, a.ptr, C.byref(out)))
+        return out.value
+
+    def axpy(self, alpha: float, x: "DeviceBuffer", y: "DeviceBuffer") -> None:
+        check(lib().owtk_axpy(self.handle, x.count, float(alpha), x.ptr, y.ptr))
+
+    def axpy_dot(self, alpha: float, x: "DeviceBuffer", y: "DeviceBuffer", w: "DeviceBuffer") -> float:
+        out = C.c_double()
+        check(lib().owtk_axpy_dot(self.handle, x.count, float(alpha), x.ptr, y.ptr, w.ptr, C.byref(out)))
+        return out.value
+
+    def multidot(self, basis: list["DeviceBuffer"], w: "DeviceBuffer") -> np.ndarray:
+        k = len(basis)
+        ptrs = (C.c_void_p * max(k, 1))(*[b.ptr for b in basis])
+        out = np.zeros(k + 1)
+        check(lib().owtk_multidot(self.handle, w.count, k, ptrs, w.ptr, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+
+class DeviceBuffer:
+    """Device array owned by a context."""
+
+    def __init__(self, ctx: Context, count: int, dtype=np.float64):
+        self.ctx = ctx
+        self.count = int(count)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.count * self.dtype.itemsize
+        self._ptr = C.c_void_p()
+        check(lib().owtk_malloc(ctx.handle, max(self.nbytes, 1), C.byref(self._ptr)))
+        if self.nbytes:
+            check(lib().owtk_fill_zero(ctx.handle, self._ptr, self.nbytes))
+
+    @property
+    def ptr(self) -> C.c_void_p:
+        return self._ptr
+
+    @property
+    def owtk_dtype(self) -> int:
+        return F64 if self.dtype == np.float64 else F32
+
+    def upload(self, array: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(array, dtype=self.dtype)
+        if a.size != self.count:
+            raise ValueError(f"size mismatch: {a.size} != {self.count}")
+        check(lib().owtk_memcpy_htod(self.ctx.handle, self._ptr, _ptr(a), self.nbytes))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        check(lib().owtk_memcpy_dtoh(self.ctx.handle, _ptr(out), self._ptr, self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        check(lib().owtk_fill_zero(self.ctx.handle, self._ptr, self.nbytes))
+
+    def free(self) -> None:
+        if self._ptr:
+            check(lib().owtk_free(self.ctx.handle, self._ptr))
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._ptr and self.ctx.handle:
+                lib().owtk_free(self.ctx.handle, self._ptr)
+        except Exception:
+            pass
+
+
+class Matrix:
+    """Device copy of one of the reference's three block-CSR forms."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, kind: int, owned: int, ghost: int, block: int,
+                 keep: tuple):
+        self.ctx, self.handle, self.kind = ctx, handle, kind
+        self.owned, self.ghost, self.block = owned, ghost, block
+        self._keep = keep
+
+    @classmethod
+    def split(cls, ctx: Context, owned: int, ghost: int, block: int, offsets, columns, diagonal, off_diagonal):
+        o = np.ascontiguousarray(offsets, np.int32)
+        c = np.ascontiguousarray(columns, np.int32)
+        d = np.ascontiguousarray(diagonal, np.float64)
+        v = np.ascontiguousarray(off_diagonal, np.float64)
+        h = C.c_void_p()
+        check(lib().owtk_matrix_create_split(ctx.handle, owned, ghost, block, _ptr(o), _ptr(c), _ptr(d), _ptr(v),
+                                             C.byref(h)))
+        return cls(ctx, h, MATRIX_SPLIT, owned, ghost, block, (o, c))
+
+    @classmethod
+    def csr(cls, ctx: Context, owned: int, ghost: int, block: int, offsets, columns, values):
+        o = np.ascontiguousarray(offsets, np.int32)
+        c = np.ascontiguousarray(columns, np.int32)
+        v = np.ascontiguousarray(values, np.float64)
+        h = C.c_void_p()
+        check(lib().owtk_matrix_create_csr(ctx.handle, owned, ghost, block, _ptr(o), _ptr(c), _ptr(v), C.byref(h)))
+        return cls(ctx, h, MATRIX_CSR, owned, ghost, block, (o, c))
+
+    @classmethod
+    def bsr(cls, ctx: Context, owned: int, ghost: int, block: int, offsets, columns, values):
+        o = np.ascontiguousarray(offsets, np.int32)
+        c = np.ascontiguousarray(columns, np.int32)
+        v = np.ascontiguousarray(values, np.float64)
+        h = C.c_void_p()
+        check(lib().owtk_matrix_create_bsr(ctx.handle, owned, ghost, block, _ptr(o), _ptr(c), _ptr(v), C.byref(h)))
+        return cls(ctx, h, MATRIX_BSR, owned, ghost, block, (o, c))
+
+    @property
+    def owned_size(self) -> int:
+        return self.owned * self.block
+
+    @property
+    def local_size(self) -> int:
+        return (self.owned + self.ghost) * self.block
+
+    def info(self) -> _lib.MatrixInfo:
+        info = _lib.MatrixInfo()
+        check(lib().owtk_matrix_info_get(self.handle, C.byref(info)))
+        return info
+
+    def add_storage(self, dtype: int) -> None:
+        check(lib().owtk_matrix_add_storage(self.handle, dtype))
+
+    def update_values(self, values, off_diagonal=None) -> None:
+        v = np.ascontiguousarray(values, np.float64)
+        if off_diagonal is None:
+            check(lib().owtk_matrix_update_values(self.handle, _ptr(v), None))
+        else:
+            e = np.ascontiguousarray(off_diagonal, np.float64)
+            check(lib().owtk_matrix_update_values(self.handle, _ptr(v), _ptr(e)))
+
+    def spmv(self, x: DeviceBuffer, y: DeviceBuffer, dtype: int = F64) -> None:
+        check(lib().owtk_spmv(self.handle, dtype, x.ptr, y.ptr))
+
+    def residual(self, b: DeviceBuffer, x: DeviceBuffer, r: DeviceBuffer) -> None:
+        check(lib().owtk_residual(self.handle, b.ptr, x.ptr, r.ptr))
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_matrix_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().owtk_matrix_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Preconditioner:
+    def __init__(self, matrix: Matrix, kind: int, *, storage: int = F64, omega: float = 1.0,
+                 minimum_diagonal: float = 0.0, block_nodes: int = 0, fill_level: int = 0, model=None):
+        self.matrix = matrix
+        p = _lib.PrecondParams()
+        lib().owtk_precond_params_default(C.byref(p), kind)
+        p.storage, p.omega, p.minimum_diagonal = storage, omega, minimum_diagonal
+        p.block_nodes, p.fill_level = block_nodes, fill_level
+        if model is not None:
+            p.learned_model = model.handle
+        self._model = model
+        self.handle = C.c_void_p()
+        check(lib().owtk_precond_create(matrix.handle, C.byref(p), C.byref(self.handle)))
+        self.kind, self.storage = kind, storage
+
+    def apply(self, x: DeviceBuffer, y: DeviceBuffer) -> None:
+        check(lib().owtk_precond_apply(self.handle, x.ptr, y.ptr))
+
+    def update_values(self) -> None:
+        check(lib().owtk_precond_update_values(self.handle))
+
+    def numeric_updates(self) -> int:
+        n = C.c_uint64()
+        check(lib().owtk_precond_numeric_updates(self.handle, C.byref(n)))
+        return n.value
+
+    def info(self) -> _lib.PrecondInfo:
+        info = _lib.PrecondInfo()
+        check(lib().owtk_precond_info_get(self.handle, C.byref(info)))
+        return info
+
+    def ilu_factors(self):
+        n = C.c_int64()
+        check(lib().owtk_precond_ilu_factors(self.handle, C.byref(n), None, None, None, None))
+        rows = self.matrix.owned
+        bs = self.matrix.block
+        off = np.zeros(rows + 1, np.int64)
+        col = np.zeros(n.value, np.int64)
+        val = np.zeros(n.value * bs)
+        inv = np.zeros(rows * bs)
+        check(lib().owtk_precond_ilu_factors(self.handle, C.byref(n), _ptr(off), _ptr(col), _ptr(val), _ptr(inv)))
+        return off, col, val, inv
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_precond_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().owtk_precond_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def SolverOptions(**kw) -> _lib.SolverOptions:
+    o = _lib.SolverOptions()
+    lib().owtk_solver_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _solve(fn, matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options) -> dict:
+    opts = options if options is not None else SolverOptions()
+    res = _lib.SolverResult()
+    check(fn(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr, C.byref(opts), C.byref(res)))
+    d = res.as_dict()
+    d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+    d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
+    return d
+
+
+def bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
+    return _solve(lib().owtk_bicgstab, matrix, pc, b, x, options)
+
+
+def fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
+    return _solve(lib().owtk_fgmres, matrix, pc, b, x, options)
+
+
+def refinement_options(**kw) -> _lib.RefinementOptions:
+    o = _lib.RefinementOptions()
+    lib().owtk_refinement_options_default(C.byref(o))
+    for k, v in kw.items():
+        if k in ("relative_tolerance", "absolute_tolerance", "maximum_iterations"):
+            setattr(o.outer, k, v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def fgmres_refined(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
+                   options=None) -> dict:
+    opts = options if options is not None else refinement_options()
+    res = _lib.SolverResult()
+    check(lib().owtk_fgmres_refined(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
+                                    C.byref(opts), C.byref(res)))
+    d = res.as_dict()
+    d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+    d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
+    return d
+
+
+# ----------------------------------------------------------------- generators
+@dataclass
+class SplitSystem:
+    nodes: int
+    block: int
+    offsets: np.ndarray
+    columns: np.ndarray
+    diagonal: np.ndarray
+    off_diagonal: np.ndarray
+    rhs: np.ndarray
+    coordinates: np.ndarray
+    final_to_natural: np.ndarray
+
+    def csr(self):
+        return split_to_csr(self.nodes, self.block, self.offsets, self.columns, self.diagonal, self.off_diagonal)
+
+
+def grid_spec(nx: int, ny: int, block: int, seed: int = 0, ordering: int = ORDER_RCM, permute: bool = True,
+              law: int = LAW_UNIFORM, sigma: float = 0.0, correlation: float = 16.0, jitter: float = 0.2,
+              shift: float = 0.05) -> _lib.GridSpec:
+    s = _lib.GridSpec()
+    lib().owtk_grid_spec_default(C.byref(s), nx, ny, block, seed)
+    s.ordering, s.permute, s.coefficient_law = ordering, int(permute), law
+    s.field_sigma, s.field_correlation, s.jitter, s.diagonal_shift = sigma, correlation, jitter, shift
+    return s
+
+
+def generate_split(nx: int, ny: int, block: int, seed: int = 0, **kw) -> SplitSystem:
+    spec = grid_spec(nx, ny, block, seed, **kw)
+    n, o = C.c_int64(), C.c_int64()
+    check(lib().owtk_grid_sizes(C.byref(spec), C.byref(n), C.byref(o)))
+    n, o = n.value, o.value
+    offsets = np.zeros(n + 1, np.int32)
+    columns = np.zeros(o, np.int32)
+    diag = np.zeros(n * block)
+    off = np.zeros(o * block)
+    rhs = np.zeros(n * block)
+    coords = np.zeros(2 * n)
+    f2n = np.zeros(n, np.int64)
+    check(lib().owtk_grid_generate_split(C.byref(spec), _ptr(offsets), _ptr(columns), _ptr(diag), _ptr(off),
+                                         _ptr(rhs), _ptr(coords), _ptr(f2n)))
+    return SplitSystem(n, block, offsets, columns, diag, off, rhs, coords.reshape(n, 2), f2n)
+
+
+def generate_bsr(nx: int, ny: int, block: int, seed: int = 0, **kw):
+    spec = grid_spec(nx, ny, block, seed, **kw)
+    n, o = C.c_int64(), C.c_int64()
+    check(lib().owtk_grid_sizes(C.byref(spec), C.byref(n), C.byref(o)))
+    n, o = n.value, o.value
+    offsets = np.zeros(n + 1, np.int32)
+    columns = np.zeros(o + n, np.int32)
+    values = np.zeros((o + n) * block * block)
+    rhs = np.zeros(n * block)
+    check(lib().owtk_grid_generate_bsr(C.byref(spec), _ptr(offsets), _ptr(columns), _ptr(values), _ptr(rhs)))
+    return n, offsets, columns, values, rhs
+
+
+def split_to_csr(nodes: int, block: int, offsets, columns, diagonal, off_diagonal):
+    o = np.ascontiguousarray(offsets, np.int32)
+    c = np.ascontiguousarray(columns, np.int32)
+    d = np.ascontiguousarray(diagonal, np.float64)
+    v = np.ascontiguousarray(off_diagonal, np.float64)
+    entries = int(o[nodes]) + nodes
+    co = np.zeros(nodes + 1, np.int32)
+    cc = np.zeros(entries, np.int32)
+    cv = np.zeros(entries * block)
+    check(lib().owtk_split_to_csr(nodes, block, _ptr(o), _ptr(c), _ptr(d), _ptr(v), _ptr(co), _ptr(cc), _ptr(cv)))
+    return co, cc, cv
+
+
+def order_rcm(n: int, offsets, columns) -> np.ndarray:
+    o = np.ascontiguousarray(offsets, np.int32)
+    c = np.ascontiguousarray(columns, np.int32)
+    out = np.zeros(n, np.int64)
+    check(lib().owtk_order_rcm(n, _ptr(o), _ptr(c), _ptr(out)))
+    return out
+
+
+def order_hilbert(coordinates) -> np.ndarray:
+    xy = np.ascontiguousarray(coordinates, np.float64).reshape(-1)
+    n = xy.size // 2
+    out = np.zeros(n, np.int64)
+    check(lib().owtk_order_hilbert(n, _ptr(xy), _ptr(out)))
+    return out

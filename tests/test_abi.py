@@ -1,0 +1,52 @@
+"""The C ABI library loads without a GPU and exports every declared symbol."""
+import ctypes
+import pathlib
+import re
+
+import owtk
+from owtk import _lib
+
+HEADER = pathlib.Path(__file__).resolve().parent.parent / "include" / "owtk_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(owtk_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert len(names) > 40
+    assert "owtk_bicgstab" in names and "owtk_spmv" in names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_functions()) == set(_lib.PROTOTYPES)
+
+
+def test_no_device_is_reported_cleanly():
+    if owtk.device_count() == 0:
+        try:
+            owtk.Context(0)
+        except owtk.NoDeviceError:
+            pass
+        else:
+            raise AssertionError("context creation without a device must fail")
+
+
+def test_options_defaults_mirror_reference():
+    # SolverOptions<double> defaults (core.hpp:88-111)
+    o = owtk.SolverOptions()
+    assert o.relative_tolerance == 1e-8 and o.absolute_tolerance == 0.0
+    assert o.maximum_iterations == 1000 and o.convergence_check_interval == 10
+    assert o.restart == 30 and o.residual_replacement_interval == 50
+    assert o.gmres_orthogonalization == 0
+    assert o.breakdown_tolerance == 64 * 2.220446049250313e-16

@@ -1,0 +1,143 @@
+"""Device-resident solvers against the oracle and long-double recurrences."""
+import numpy as np
+import pytest
+
+import owtk
+import refcases as rc
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def solve(ctx, fn, m, pc, b, x0=None, **kw):
+    bd = ctx.upload(b)
+    xd = ctx.upload(np.zeros(m.local_size) if x0 is None else np.asarray(x0, float))
+    res = fn(m, pc, bd, xd, owtk.SolverOptions(**kw))
+    return xd.download(), res
+
+
+@pytest.mark.parametrize("use_ilu", [False, True])
+def test_bicgstab_recurrence_audit(ctx, use_ilu):
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)
+    pattern = [[a[i, j] != 0 for j in range(12)] for i in range(12)]
+    lower, upper = rc.doolittle(a, pattern)
+    inverse = (lambda r: rc.lu_solve(lower, upper, r)) if use_ilu else (lambda r: r.copy())
+    history = rc.dense_bicgstab(a, rhs, x0, inverse, 4)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU) if use_ilu else None
+    for count in range(1, 5):
+        x, res = solve(ctx, owtk.bicgstab, m, pc, rhs, x0, relative_tolerance=0.0, maximum_iterations=count,
+                       convergence_check_interval=1, residual_replacement_interval=0)
+        assert res["status_name"] == "maximum_iterations" and res["iterations"] == count
+        assert np.max(np.abs(x - history[count - 1].astype(float))) < 2e-11
+        assert res["preconditioner_applications"] == 2 * count
+        assert res["global_reductions"] == 2 + 6 * count + 1
+
+
+def test_bicgstab_exact_preconditioner(ctx):
+    m = owtk.Matrix.csr(ctx, 3, 0, 1, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2], [4.0, -1, -2, 5, -1, -1, 3])
+    om = orc.OMatrix.csr(3, 0, 1, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2], [4.0, -1, -2, 5, -1, -1, 3])
+    rhs = om.spmv([1.0, 2.0, 3.0])
+    pc = owtk.Preconditioner(m, owtk.PC_ILU)
+    x, res = solve(ctx, owtk.bicgstab, m, pc, rhs, relative_tolerance=32 * EPS, convergence_check_interval=1)
+    assert res["status_name"] == "converged" and res["iterations"] == 1
+    assert res["preconditioner_applications"] == 1
+    assert np.max(np.abs(x - [1, 2, 3])) < 128 * EPS
+
+
+def test_bicgstab_breakdown_zero_operator(ctx):
+    m = owtk.Matrix.csr(ctx, 1, 0, 1, [0, 1], [0], [0.0])
+    _, res = solve(ctx, owtk.bicgstab, m, None, np.array([1.0]), convergence_check_interval=1)
+    assert res["status_name"] == "breakdown" and res["reason_name"] == "alpha_denominator"
+
+
+def test_bicgstab_nonfinite_rhs(ctx):
+    m = owtk.Matrix.csr(ctx, 1, 0, 2, [0, 1], [0], [1.0, 1.0])
+    _, res = solve(ctx, owtk.bicgstab, m, None, np.array([np.inf, 1.0]))
+    assert res["status_name"] == "breakdown" and res["reason_name"] == "non_finite_scalar"
+
+
+def test_bicgstab_c0_jacobi_matches_oracle(ctx, c0):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    xo, ro = orc.bicgstab(om, orc.OPreconditioner(om, 1), s.rhs)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    x, res = solve(ctx, owtk.bicgstab, m, owtk.Preconditioner(m, owtk.PC_JACOBI), s.rhs)
+    print("oracle", ro["iterations"], ro["relative_residual_norm"], "gpu", res["iterations"],
+          res["relative_residual_norm"])
+    assert res["status_name"] == "converged" and ro["status"] == 0
+    assert abs(res["iterations"] - ro["iterations"]) <= 1
+    true_rel = np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs)
+    assert true_rel <= 1e-8
+    assert abs(res["relative_residual_norm"] - true_rel) <= 1e-3 * true_rel
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-7
+    if res["iterations"] == ro["iterations"]:
+        for key in ("operator_applications", "preconditioner_applications", "global_reductions"):
+            assert res[key] == ro[key], key
+
+
+@pytest.mark.parametrize("pc_kind", ["ilu", "ssor"])
+def test_bicgstab_c0_preconditioned_matches_oracle(ctx, c0, pc_kind):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    if pc_kind == "ilu":
+        opc, pc = orc.OPreconditioner(om, 3, block_nodes=512), owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512)
+    else:
+        opc, pc = orc.OPreconditioner(om, 2), owtk.Preconditioner(m, owtk.PC_SSOR)
+    xo, ro = orc.bicgstab(om, opc, s.rhs)
+    x, res = solve(ctx, owtk.bicgstab, m, pc, s.rhs)
+    assert res["status_name"] == "converged"
+    assert abs(res["iterations"] - ro["iterations"]) <= 1
+    assert np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs) <= 1e-8
+
+
+def test_fgmres_direct_solve(ctx):
+    a, offsets, columns, values, rhs, _ = rc.recurrence_matrix()
+    m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)
+    exact = np.linalg.solve(a.astype(float), rhs)
+    for mode in (0, 1, 2):
+        x, res = solve(ctx, owtk.fgmres, m, None, rhs, relative_tolerance=1e-13, restart=12,
+                       gmres_orthogonalization=mode)
+        assert res["status_name"] == "converged" and res["iterations"] <= 12
+        assert np.max(np.abs(x - exact)) < 1e-11
+
+
+@pytest.mark.parametrize("pc_kind", ["jacobi", "ilu"])
+def test_fgmres_c0_matches_oracle(ctx, c0, pc_kind):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    if pc_kind == "ilu":
+        opc, pc = orc.OPreconditioner(om, 3, block_nodes=512), owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512)
+    else:
+        opc, pc = orc.OPreconditioner(om, 1), owtk.Preconditioner(m, owtk.PC_JACOBI)
+    xo, ro = orc.fgmres(om, opc, s.rhs)
+    x, res = solve(ctx, owtk.fgmres, m, pc, s.rhs)
+    print("oracle", ro["iterations"], "gpu", res["iterations"])
+    assert res["status_name"] == "converged" and ro["status"] == 0
+    assert abs(res["iterations"] - ro["iterations"]) <= 1
+    assert np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs) <= 1e-8
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-7
+    if res["iterations"] == ro["iterations"]:
+        for key in ("operator_applications", "preconditioner_applications", "global_reductions",
+                    "reorthogonalizations"):
+            assert res[key] == ro[key], key
+
+
+@pytest.mark.parametrize("storage", [owtk.F32, owtk.BF16])
+def test_refined_fgmres_reaches_fp64_accuracy(ctx, c0, storage):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    xo, _ = orc.bicgstab(om, orc.OPreconditioner(om, 3), s.rhs, relative_tolerance=1e-13)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    m.add_storage(storage)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512, storage=storage)
+    bd, xd = ctx.upload(s.rhs), ctx.upload(np.zeros(m.local_size))
+    res = owtk.fgmres_refined(m, pc, bd, xd, owtk.refinement_options(inner_storage=storage))
+    x = xd.download()
+    true_rel = np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs)
+    print("refined", res["refinement_steps"], res["inner_iterations"], true_rel)
+    assert res["status_name"] == "converged" and true_rel <= 1e-8
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6

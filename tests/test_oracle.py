@@ -1,0 +1,240 @@
+"""Pin the CPU oracle against the reference's known answers, long-double dense
+references and closed forms (no GPU)."""
+import numpy as np
+import pytest
+
+import refcases as rc
+from oracle import oracle as orc
+
+EPS = np.finfo(np.float64).eps
+
+
+def test_spmv_csr_known_answer():
+    o, c, v, x, exp = rc.core_csr_case()
+    m = orc.OMatrix.csr(2, 0, 3, o, c, v)
+    assert m.spmv(x).tolist() == exp
+
+
+def test_spmv_bsr_known_answer():
+    o, c, v, x, exp = rc.core_bsr_case()
+    m = orc.OMatrix.bsr(2, 0, 2, o, c, v)
+    assert m.spmv(x).tolist() == exp
+
+
+def test_spmv_split_known_answer_and_update():
+    o, c, d, e, x, exp = rc.core_split_case()
+    d = np.array(d, float)
+    e = np.array(e, float)
+    m = orc.OMatrix.split(2, 1, 2, o, c, d, e)
+    assert m.spmv(x).tolist() == exp
+    d[0], e[0] = 8, -2
+    m2 = orc.OMatrix.split(2, 1, 2, o, c, d, e)
+    assert m2.spmv(x)[0] == 2
+
+
+def test_spmv_split_matches_long_double_dense(c0):
+    s = c0
+    x = np.random.default_rng(1).uniform(-1, 1, s.nodes * 3)
+    y = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal).spmv(x)
+    for comp in range(3):
+        a = rc.dense_from_split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal, comp)
+        exact = a @ x[comp::3].astype(rc.LD)
+        assert np.max(np.abs(y[comp::3] - exact.astype(float))) <= 8 * EPS * np.max(np.abs(exact.astype(float)))
+
+
+def test_csr_and_split_agree_bitwise(c0):
+    s = c0
+    co, cc, cv = s.csr()
+    x = np.random.default_rng(2).uniform(-1, 1, s.nodes * 3)
+    y_csr = orc.OMatrix.csr(s.nodes, 0, 3, co, cc, cv).spmv(x)
+    y_split = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal).spmv(x)
+    # different accumulation start (split: diag*x first, csr: sorted order) -> close, not identical
+    np.testing.assert_allclose(y_csr, y_split, rtol=1e-14, atol=1e-14)
+
+
+def test_norm_extremes():
+    # test_foundation.cpp:71-107
+    tiny = np.finfo(np.float64).tiny
+    for mag in (1.0, np.finfo(np.float64).max / 8, tiny * 4, 5e-324 * 4):
+        v = np.array([3 * mag, -4 * mag])
+        assert abs(orc.norm(v) - 5 * mag) <= 4 * EPS * 5 * mag
+    assert orc.norm(np.zeros(2)) == 0.0
+    assert np.isinf(orc.norm(np.full(2, np.finfo(np.float64).max)))
+    assert np.isinf(orc.norm(np.array([np.inf, 0.0])))
+    assert np.isnan(orc.norm(np.array([np.inf, np.nan])))
+    assert orc.norm(np.array([3.0, 4.0])) == 5.0
+
+
+def test_dot_matches_exact_sum():
+    rng = np.random.default_rng(3)
+    a, b = rng.uniform(-1, 1, 1001), rng.uniform(-1, 1, 1001)
+    exact = float(np.sum(a.astype(rc.LD) * b.astype(rc.LD)))
+    assert abs(orc.dot(a, b) - exact) <= 64 * EPS * np.sum(np.abs(a * b))
+
+
+def test_jacobi_and_ilu_updates_known_answer():
+    # test_core.cpp:403-442 after the numeric update (values 5, -1, -1, 4)
+    m = orc.OMatrix.csr(2, 0, 1, [0, 2, 4], [0, 1, 0, 1], [5.0, -1, -1, 4.0])
+    rhs = m.spmv([2.0, 3.0])
+    jac = orc.OPreconditioner(m, orc.OPreconditioner.JACOBI)
+    assert np.allclose(jac.apply(rhs), rhs / np.array([5.0, 4.0]), rtol=0, atol=1e-13)
+    ilu = orc.OPreconditioner(m, orc.OPreconditioner.ILU)
+    assert np.allclose(ilu.apply(rhs), [2.0, 3.0], rtol=0, atol=1e-13)
+
+
+def test_ssor_closed_form_review_case():
+    # review_cases.cpp:95-113, omega = 0.5
+    m = orc.OMatrix.csr(2, 0, 1, [0, 2, 4], [0, 1, 0, 1], [4.0, 1, 2, 3.0])
+    w = 0.5
+    out = orc.OPreconditioner(m, orc.OPreconditioner.SSOR, omega=w).apply([1.0, 1.0])
+    y0 = 1.0 / 4
+    y1 = (1 - w * 2 * y0) / 3
+    z0 = (4 * y0 - w * y1) / 4
+    assert abs(out[0] - w * (2 - w) * z0) < 1e-14 and abs(out[1] - w * (2 - w) * y1) < 1e-14
+
+
+@pytest.mark.parametrize("block", [1, 5, 1296])
+@pytest.mark.parametrize("omega", [0.5, 1.0, 1.5])
+def test_ssor_native_and_split_closed_form(block, omega):
+    # review_contracts.cpp:29-96: ghost couplings ignored, both layouts agree
+    split, csr, rhs, expected = rc.ssor_review_case(block, omega)
+    ms = orc.OMatrix.split(2, 1, block, split["offsets"], split["columns"], split["diagonal"], split["off_diagonal"])
+    mc = orc.OMatrix.csr(2, 1, block, csr["offsets"], csr["columns"], csr["values"])
+    for m in (ms, mc):
+        pc = orc.OPreconditioner(m, orc.OPreconditioner.SSOR, omega=omega)
+        out = pc.apply(rhs)
+        assert np.all(np.abs(out / expected - 1) < 64 * EPS)
+        assert np.array_equal(pc.apply_inplace(rhs[: 2 * block]), out)
+
+
+@pytest.mark.parametrize("fill", [0, 1, 2])
+def test_ilu_k_against_dense_doolittle(fill):
+    # test_bicgstab_ilu_audit.cpp:14-91
+    n, comps, offsets, columns, values = rc.ilu_audit_matrix()
+    for update in range(2):
+        vals = values.copy()
+        if update:
+            vals = vals * np.array([1.1 + 0.02 * (e % 3) for e in range(vals.size)])
+        m = orc.OMatrix.csr(n, 1, comps, offsets, columns, vals)
+        pc = orc.OPreconditioner(m, orc.OPreconditioner.ILU, fill_level=fill)
+        off, col, fval, inv = pc.ilu_factors()
+        for c in range(comps):
+            dense = [[0.0] * n for _ in range(n)]
+            pattern = [[False] * n for _ in range(n)]
+            for row in range(n):
+                for e in range(offsets[row], offsets[row + 1]):
+                    if columns[e] < n:
+                        dense[row][columns[e]] = vals[e * comps + c]
+                        pattern[row][columns[e]] = True
+            pat = rc.dense_symbolic(pattern, fill)
+            for i in range(n):
+                expected_cols = [j for j in range(n) if pat[i][j]]
+                assert col[off[i]:off[i + 1]].tolist() == expected_cols
+            lower, upper = rc.doolittle(dense, pat)
+            prod = lower @ upper
+            if fill == 0:
+                assert abs(prod[1, 2]) > 0.001
+            else:
+                assert abs(prod[1, 2]) < 1e-16
+            for i in range(n):
+                for j in range(n):
+                    if pat[i][j]:
+                        assert abs(prod[i, j] - rc.LD(dense[i][j])) < 1e-15
+            # factors equal the dense Doolittle factors
+            for i in range(n):
+                for e in range(off[i], off[i + 1]):
+                    j = col[e]
+                    ref = lower[i, j] if j < i else upper[i, j]
+                    assert abs(fval[e * comps + c] - float(ref)) <= 1e-14 * max(1.0, abs(float(ref)))
+            # inverse on every LU basis column
+            for basis in range(n):
+                rhs = np.zeros(n * comps)
+                for i in range(n):
+                    rhs[i * comps + c] = float(prod[i, basis])
+                out = pc.apply(rhs)
+                exp = np.zeros(n * comps)
+                exp[basis * comps + c] = 1.0
+                assert np.max(np.abs(out - exp)) < 32 * EPS * 4
+
+
+def test_ilu_zero_pivots_rejected():
+    # review_contracts.cpp:152-177, review_cases.cpp:127-139
+    for pivot in (0.0, np.nan, np.inf):
+        m = orc.OMatrix.csr(1, 0, 1, [0, 1], [0], [pivot])
+        with pytest.raises(orc.OracleError) as err:
+            orc.OPreconditioner(m, orc.OPreconditioner.ILU)
+        assert err.value.code == -2
+    m = orc.OMatrix.csr(2, 0, 1, [0, 2, 4], [0, 1, 0, 1], [1.0, 1, 1, 1])
+    with pytest.raises(orc.OracleError):
+        orc.OPreconditioner(m, orc.OPreconditioner.ILU)
+
+
+@pytest.mark.parametrize("use_ilu", [False, True])
+def test_bicgstab_recurrence_audit(use_ilu):
+    # test_bicgstab_ilu_audit.cpp:104-183: 1-4 iterations against a long-double recurrence
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = orc.OMatrix.csr(12, 0, 1, offsets, columns, values)
+    pattern = [[a[i, j] != 0 for j in range(12)] for i in range(12)]
+    lower, upper = rc.doolittle(a, pattern)
+    inverse = (lambda r: rc.lu_solve(lower, upper, r)) if use_ilu else (lambda r: r.copy())
+    history = rc.dense_bicgstab(a, rhs, x0, inverse, 4)
+    pc = orc.OPreconditioner(m, orc.OPreconditioner.ILU) if use_ilu else None
+    for count in range(1, 5):
+        x, res = orc.bicgstab(m, pc, rhs, x0, relative_tolerance=0.0, maximum_iterations=count,
+                              convergence_check_interval=1, residual_replacement_interval=0)
+        assert res["status"] == 1
+        assert np.max(np.abs(x - history[count - 1].astype(float))) < 2e-11
+
+
+def test_bicgstab_exact_preconditioner_one_iteration():
+    # test_bicgstab_ilu_audit.cpp:185-200
+    m = orc.OMatrix.csr(3, 0, 1, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2], [4.0, -1, -2, 5, -1, -1, 3])
+    rhs = m.spmv([1.0, 2.0, 3.0])
+    pc = orc.OPreconditioner(m, orc.OPreconditioner.ILU)
+    x, res = orc.bicgstab(m, pc, rhs, relative_tolerance=32 * EPS, convergence_check_interval=1)
+    assert res["status"] == 0 and res["iterations"] == 1 and res["preconditioner_applications"] == 1
+    assert np.max(np.abs(x - [1, 2, 3])) < 32 * EPS * 4
+
+
+def test_bicgstab_zero_operator_breakdown():
+    # test_core.cpp:444-469
+    m = orc.OMatrix.csr(1, 0, 1, [0, 1], [0], [0.0])
+    _, res = orc.bicgstab(m, None, [1.0], convergence_check_interval=1)
+    assert res["status"] == 2 and res["breakdown_reason"] == 4
+
+
+def test_fgmres_equals_direct_solve():
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = orc.OMatrix.csr(12, 0, 1, offsets, columns, values)
+    exact = np.linalg.solve(a.astype(float), rhs)
+    for mode in (0, 1, 2):
+        x, res = orc.fgmres(m, None, rhs, relative_tolerance=1e-13, restart=12, gmres_orthogonalization=mode)
+        assert res["status"] == 0 and res["iterations"] <= 12
+        assert np.max(np.abs(x - exact)) < 1e-11
+
+
+def test_solvers_on_block_system():
+    # test_core.cpp:471-560: 3 nodes x 2 components, Jacobi, rtol 1e-12
+    m = orc.OMatrix.csr(3, 0, 2, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2],
+                        [4, 5, -1, -1, -1, -1, 4, 5, -1, -1, -1, -1, 3, 4])
+    exact = np.arange(1, 7, dtype=float)
+    rhs = m.spmv(exact)
+    jac = orc.OPreconditioner(m, orc.OPreconditioner.JACOBI)
+    for solver in (orc.fgmres, orc.bicgstab):
+        x, res = solver(m, jac, rhs, relative_tolerance=1e-12, convergence_check_interval=1,
+                        residual_replacement_interval=0, maximum_iterations=100, restart=8)
+        assert res["status"] == 0
+        assert np.max(np.abs(x - exact)) < 1e-10
+    x, res = orc.fgmres(m, jac, rhs, relative_tolerance=1e-12, restart=8)
+    assert res["global_reductions"] <= 2 * res["iterations"] + 4 + res["arnoldi_norm_verifications"]
+
+
+def test_c0_bicgstab_jacobi_converges(c0):
+    s = c0
+    m = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    pc = orc.OPreconditioner(m, orc.OPreconditioner.JACOBI)
+    x, res = orc.bicgstab(m, pc, s.rhs)
+    assert res["status"] == 0
+    r = s.rhs - m.spmv(x)
+    assert np.linalg.norm(r) / np.linalg.norm(s.rhs) <= 1e-8
+    assert res["relative_residual_norm"] <= 1e-8
