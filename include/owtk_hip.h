@@ -69,6 +69,18 @@ int owtk_context_synchronize(owtk_context ctx);
  * (execution.hpp:26). */
 const char* owtk_policy_name(void);
 
+/* Live kernel timing with HIP events on the context stream; plays the role of
+ * the reference's ScopedSolverTimer / TimedOperator instrumentation
+ * (core.hpp:150-182, benchmarks/benchmark_split.cpp:107-121) per kernel class.
+ * Categories: 0 SpMV FP64, 1 SpMV FP32, 2 SpMV BF16, 3 preconditioner apply
+ * FP64, 4 preconditioner apply FP32/BF16, 5 learned preconditioner. */
+int owtk_profile_enable(owtk_context ctx, int enable);
+int owtk_profile_reset(owtk_context ctx);
+/* Sum over recorded launches of one category: launch count, total kernel
+ * milliseconds, total algorithmic HBM bytes. */
+int owtk_profile_read(owtk_context ctx, int category, uint64_t* launches, double* total_ms,
+                      int64_t* bytes);
+
 /* ---------------------------------------------------------- device memory -- */
 int owtk_malloc(owtk_context ctx, size_t bytes, void** device_pointer);
 int owtk_free(owtk_context ctx, void* device_pointer);
@@ -164,7 +176,8 @@ typedef struct owtk_precond_params {
     int64_t block_nodes;      /* block-Jacobi partition: consecutive node ranges
                                  of this size; 0 = one block over all owned nodes */
     int32_t fill_level;       /* ILU(k) level (0 = ILU(0)) */
-    int32_t reserved;
+    int32_t schedule;         /* triangular sweeps: 0 auto, 1 level sets, 2 one-wave chains
+                                 (same results; a performance choice) */
     void* learned_model;      /* owtk_gnn for OWTK_PC_LEARNED */
 } owtk_precond_params;
 

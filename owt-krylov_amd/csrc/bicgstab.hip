@@ -10,8 +10,9 @@
 //   t   = A y            + <t, s>, <t,t> -> omega
 //   x  += alpha z; x += omega y; r = s - omega t + <shadow, r>, ||r||
 //                                        -> next rho, cadence test
-// Scalar logic (breakdown guards, cadence, counters) runs in the finalizer of
-// the kernel's last block.  Once a stop condition is recorded every later
+// Scalar logic (breakdown guards, cadence, counters) runs in the one-block
+// finisher launched after each reducing kernel.  Once a stop condition is
+// recorded every later
 // launch returns immediately, so the device state equals the reference's at
 // the point it would have returned.  The host synchronises only at the
 // reference's convergence-check/replacement cadence (every
@@ -78,10 +79,16 @@ __global__ __launch_bounds__(256) void k_rho(const double* __restrict__ shadow, 
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
         red[0] = __builtin_fma(shadow[i], r[i], red[0]);
-    dev::grid_reduce<1>(red, 0, scratch, [&](const double* t) {
-        if (threadIdx.x == 0) fin_rho(st, t[0], st->iteration);
-    });
+    dev::store_partials<1>(red, 0, scratch);
 }
+
+struct FinRho {
+    BicgState* st;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x == 0) fin_rho(st, t[0], st->iteration);
+    }
+};
 
 // p = r (first) or p = r + beta (p - omega v)
 __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const double* __restrict__ r,
@@ -110,7 +117,15 @@ __global__ __launch_bounds__(256) void k_s_update(double* __restrict__ s, const 
         red[0] = __builtin_fma(si, si, red[0]);
         red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(si));
     }
-    dev::grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+    dev::store_partials<2>(red, 2, scratch);
+}
+
+struct FinSnorm {
+    BicgState* st;
+    const double* s;
+    std::int64_t n;
+    __device__ void operator()(const double* t) const
+    {
         const double nrm = dev::block_norm(t[0], t[1], s, n);
         if (threadIdx.x == 0) {
             st->snorm = nrm;
@@ -119,8 +134,8 @@ __global__ __launch_bounds__(256) void k_s_update(double* __restrict__ s, const 
                 st->stop_iteration = st->iteration;
             }
         }
-    });
-}
+    }
+};
 
 // x += alpha z; x += omega y; r = s - omega t; <shadow, r>, ||r||
 __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, double* __restrict__ r,
@@ -143,12 +158,21 @@ __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, doubl
         red[1] = __builtin_fma(ri, ri, red[1]);
         red[2] = dev::combine<dev::Combine::max>(red[2], __builtin_fabs(ri));
     }
-    dev::grid_reduce<3>(red, 4, scratch, [&](const double* tot) {
+    dev::store_partials<3>(red, 4, scratch);
+}
+
+struct FinXr {
+    BicgState* st;
+    const double* r;
+    std::int64_t n;
+    __device__ void operator()(const double* tot) const
+    {
         const long long it = st->iteration;
         const bool replace = st->replace_interval > 0 && it % st->replace_interval == 0;
         const bool check = it % st->check_interval == 0;
         double rn = 0.0;
         if (replace || check) rn = dev::block_norm(tot[1], tot[2], r, n);
+        __syncthreads();  // every thread has read the state before thread 0 updates it
         if (threadIdx.x != 0) return;
         if (replace || check) {
             st->rnorm = rn;
@@ -167,8 +191,8 @@ __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, doubl
         }
         st->iteration = it + 1;
         fin_rho(st, tot[0], it + 1);
-    });
-}
+    }
+};
 
 // v = A z epilogue: <shadow, v> -> alpha
 struct FinAlpha {
@@ -300,6 +324,7 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     }
     if (initial <= thr) return finish(status_converged);
 
+    const unsigned int g = dev::stream_grid(n, 4);
     auto restart_state = [&](long long it) {
         hst->rho_prev = 1;
         hst->alpha = 1;
@@ -311,7 +336,8 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
         OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         kern::fill_zero_bytes(st, p, n * sizeof(double));
         kern::fill_zero_bytes(st, v, n * sizeof(double));
-        k_rho<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+        k_rho<<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+        dev::launch_finish<1>(st, FinRho{dst}, g, 0, scratch, nullptr);
         OWTK_LAUNCH_CHECK();
     };
     std::memset(hst, 0, sizeof(BicgState));
@@ -325,7 +351,6 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     restart_state(1);
 
     const int* stop = &dst->stop;
-    const unsigned int g = dev::stream_grid(n, 4);
     auto enqueue_iteration = [&] {
         k_p_update<<<g, kBlock, 0, st>>>(p, r, v, n, dst);
         OWTK_LAUNCH_CHECK();
@@ -339,6 +364,7 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             spmv_f64(m, z, e);
         }
         k_s_update<<<g, kBlock, 0, st>>>(s, r, v, n, dst, scratch);
+        dev::launch_finish<2>(st, FinSnorm{dst, s, n}, g, 2, scratch, stop);
         OWTK_LAUNCH_CHECK();
         apply_pc_f64(pc, s, yv, n, st, stop);
         {
@@ -350,6 +376,7 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             spmv_f64(m, yv, e);
         }
         k_xr_update<<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+        dev::launch_finish<3>(st, FinXr{dst, r, n}, g, 4, scratch, stop);
         OWTK_LAUNCH_CHECK();
     };
     const long long maxit = static_cast<long long>(o->maximum_iterations);

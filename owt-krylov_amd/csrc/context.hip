@@ -57,6 +57,36 @@ void owtk_context_s::activate() const { OWTK_HIP(hipSetDevice(device)); }
 
 void owtk_context_s::sync() const { OWTK_HIP(hipStreamSynchronize(stream)); }
 
+hipEvent_t owtk_context_s::take_event()
+{
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    OWTK_HIP(hipEventCreate(&e));
+    return e;
+}
+
+namespace owtk {
+
+ProfileScope::ProfileScope(owtk_context ctx, int category, std::int64_t bytes) : ctx_(ctx)
+{
+    if (!ctx_ || !ctx_->profiling) return;
+    ProfileRecord r{category, ctx_->take_event(), ctx_->take_event(), bytes};
+    OWTK_HIP(hipEventRecord(r.start, ctx_->stream));
+    ctx_->records.push_back(r);
+    index_ = ctx_->records.size() - 1;
+}
+
+ProfileScope::~ProfileScope()
+{
+    if (index_ != static_cast<std::size_t>(-1)) (void)hipEventRecord(ctx_->records[index_].stop, ctx_->stream);
+}
+
+} // namespace owtk
+
 using namespace owtk;
 
 extern "C" {
@@ -103,12 +133,60 @@ int owtk_context_create(int device, owtk_context* out)
     });
 }
 
+int owtk_profile_enable(owtk_context ctx, int enable)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->profiling = enable != 0;
+    });
+}
+
+int owtk_profile_reset(owtk_context ctx)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->activate();
+        ctx->sync();
+        for (auto& r : ctx->records) {
+            ctx->event_pool.push_back(r.start);
+            ctx->event_pool.push_back(r.stop);
+        }
+        ctx->records.clear();
+    });
+}
+
+int owtk_profile_read(owtk_context ctx, int category, uint64_t* launches, double* total_ms, int64_t* bytes)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!launches || !total_ms || !bytes) throw std::invalid_argument("null output");
+        ctx->activate();
+        ctx->sync();
+        *launches = 0;
+        *total_ms = 0.0;
+        *bytes = 0;
+        for (const auto& r : ctx->records) {
+            if (r.category != category) continue;
+            float ms = 0.0f;
+            OWTK_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+            *launches += 1;
+            *total_ms += ms;
+            *bytes += r.bytes;
+        }
+    });
+}
+
 int owtk_context_destroy(owtk_context ctx)
 {
     return guarded([&] {
         if (!ctx) return;
         ctx->activate();
         (void)hipStreamSynchronize(ctx->stream);
+        for (auto& r : ctx->records) {
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
         hipStream_t st = ctx->stream;
         delete ctx;
         (void)hipStreamDestroy(st);

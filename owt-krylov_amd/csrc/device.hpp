@@ -125,40 +125,40 @@ __device__ inline void block_reduce(double (&v)[K], std::uint64_t max_mask)
 // Scratch of one context for cross-block reductions.
 struct ReduceScratch {
     double* partials;      // [kMaxReduceValues][kMaxReduceBlocks]
-    unsigned int* ticket;  // one counter, returned to zero by the last block
+    unsigned int* ticket;  // unused by the two-kernel scheme, kept for layout
 };
 
-// Every block calls this with its per-thread values.  The block partials go to
-// scratch; the last block to arrive sums all partials in a fixed order (so the
-// result is deterministic for a fixed grid) and then calls fin(totals) with
-// ALL threads of that block (totals in LDS, identical on every thread), so a
-// finalizer may run block-cooperative work; single-thread updates inside fin
-// must be guarded by threadIdx.x == 0.  Only the last block returns from here
-// past the early exit.
-template<int K, class Fin>
-__device__ inline void grid_reduce(double (&v)[K], std::uint64_t max_mask, ReduceScratch scratch,
-                                   Fin&& fin)
+// Cross-block reductions run in two launches.  The streaming kernel calls
+// store_partials: each block reduces its values and writes one partial per
+// value.  A one-block finisher (k_finish, launched right after on the same
+// stream) then sums the partials in a fixed order, so results are
+// deterministic for a fixed grid, and runs the caller's finalizer with all of
+// its threads (totals in LDS, identical on every thread); single-thread state
+// updates inside a finalizer are guarded by threadIdx.x == 0.  The kernel
+// boundary makes the partials visible across XCDs without any fence or atomic
+// inside the streaming kernel.
+template<int K>
+__device__ inline void store_partials(double (&v)[K], std::uint64_t max_mask, ReduceScratch scratch)
 {
-    __shared__ bool is_last;
-    __shared__ double totals[K > 0 ? K : 1];
     block_reduce<K>(v, max_mask);
-    const int nblocks = gridDim.x;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) scratch.partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned int arrived = atomicAdd(scratch.ticket, 1U);
-        is_last = arrived == static_cast<unsigned int>(nblocks - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+template<int K, class Fin>
+__global__ __launch_bounds__(256) void k_finish(Fin fin, unsigned int nblocks, std::uint64_t max_mask,
+                                                ReduceScratch scratch, const int* stop)
+{
+    if (stop != nullptr && *stop != 0) return;
+    __shared__ double totals[K > 0 ? K : 1];
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const bool is_max = (max_mask >> k) & 1;
         double a = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        for (unsigned int b = threadIdx.x; b < nblocks; b += blockDim.x) {
             const double p = scratch.partials[k * kMaxReduceBlocks + b];
             a = is_max ? combine<Combine::max>(a, p) : combine<Combine::sum>(a, p);
         }
@@ -168,10 +168,16 @@ __device__ inline void grid_reduce(double (&v)[K], std::uint64_t max_mask, Reduc
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) totals[k] = acc[k];
-        *scratch.ticket = 0U;
     }
     __syncthreads();
     fin(static_cast<const double*>(totals));
+}
+
+template<int K, class Fin>
+inline void launch_finish(hipStream_t st, const Fin& fin, unsigned int nblocks, std::uint64_t max_mask,
+                          ReduceScratch scratch, const int* stop)
+{
+    k_finish<K, Fin><<<1, kBlock, 0, st>>>(fin, nblocks, max_mask, scratch, stop);
 }
 
 // Sum over the block, result on every thread.

@@ -118,7 +118,16 @@ __global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std:
             if (j < count) red[j] = __builtin_fma(wi, static_cast<double>(V[j * ld + i]), red[j]);
         red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
     }
-    dev::grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+    dev::store_partials<K>(red, 0, scratch);
+}
+
+template<int K>
+struct FinA {
+    FgState* s;
+    int k;
+    int count;
+    __device__ void operator()(const double* t) const
+    {
         if (threadIdx.x != 0) return;
         s->original = t[K - 1];
         double projected = s->original;
@@ -136,8 +145,8 @@ __global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std:
         s->direct = 0;
         s->need_scale = 0;
         if (!s->do_reorth) decide(s, k);
-    });
-}
+    }
+};
 
 // Pass B: w -= sum h_j v_j; with re-orthogonalisation also c_j = <w, v_j>, <w, w>.
 template<int K, class T>
@@ -167,7 +176,16 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
             red[K - 1] = __builtin_fma(wr, wr, red[K - 1]);
         }
     }
-    dev::grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+    dev::store_partials<K>(red, 0, scratch);
+}
+
+template<int K>
+struct FinB {
+    FgState* s;
+    int k;
+    int count;
+    __device__ void operator()(const double* t) const
+    {
         if (threadIdx.x != 0 || !s->do_reorth) return;
         double nsq = t[K - 1];
         for (int j = 0; j < count; ++j) {
@@ -178,8 +196,8 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
         s->nsq = nsq;
         s->reorth_count += 1;
         decide(s, k);
-    });
-}
+    }
+};
 
 // Pass C: second projection, then v_{k+1} (direct) or the verified norm; then
 // the Givens rotation and residual estimate (krylov_solvers.hpp:369-433).
@@ -214,9 +232,20 @@ __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std:
             }
         }
     }
-    dev::grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+    dev::store_partials<2>(red, 2, scratch);
+}
+
+template<class T>
+struct FinC {
+    FgState* s;
+    int k;
+    const T* w;
+    std::int64_t n;
+    __device__ void operator()(const double* t) const
+    {
         double verified = 0.0;
         if (s->verify) verified = dev::block_norm(t[0], t[1], w, n);
+        __syncthreads();  // every thread has read the state before thread 0 updates it
         if (threadIdx.x != 0) return;
         if (s->verify) {
             s->verif_count += 1;
@@ -254,8 +283,8 @@ __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std:
             s->cycle_done = 1;
             s->halt = 1;
         }
-    });
-}
+    }
+};
 
 // Pass D: v_{k+1} = w / h after a verified norm.
 template<class T>
@@ -495,9 +524,13 @@ struct FgEngine {
                 ReduceScratch scratch)
     {
         const int count = k + 1;
+        const int* halt = &s->halt;
         k_fg_pass_a<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        dev::launch_finish<K>(st, FinA<K>{s, k, count}, gr, 0, scratch, halt);
         k_fg_pass_b<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt);
         k_fg_pass_c<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
+        dev::launch_finish<2>(st, FinC<T>{s, k, w, n}, gr, 2, scratch, halt);
         k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);
         OWTK_LAUNCH_CHECK();
     }

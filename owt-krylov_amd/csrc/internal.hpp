@@ -84,6 +84,18 @@ struct PinnedBuffer {
 
 } // namespace owtk
 
+namespace owtk {
+
+// Live kernel timing: HIP events around selected launches on the context
+// stream, read back after the timed region (owtk_profile_*).
+struct ProfileRecord {
+    int category;
+    hipEvent_t start, stop;
+    std::int64_t bytes;
+};
+
+} // namespace owtk
+
 struct owtk_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -92,11 +104,32 @@ struct owtk_context_s {
     owtk::DeviceArray<unsigned int> ticket;  // grid_reduce arrival counter
     owtk::DeviceArray<double> scalars;       // small device scalar results
     owtk::PinnedBuffer pinned;
+    bool profiling = false;
+    std::vector<owtk::ProfileRecord> records;
+    std::vector<hipEvent_t> event_pool;
 
     owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
     void activate() const;  // hipSetDevice(device)
     void sync() const;
+    hipEvent_t take_event();
 };
+
+namespace owtk {
+
+// Records start/stop events around one launch when profiling is enabled.
+class ProfileScope {
+public:
+    ProfileScope(owtk_context ctx, int category, std::int64_t bytes);
+    ~ProfileScope();
+    ProfileScope(const ProfileScope&) = delete;
+    ProfileScope& operator=(const ProfileScope&) = delete;
+
+private:
+    owtk_context ctx_;
+    std::size_t index_ = static_cast<std::size_t>(-1);
+};
+
+} // namespace owtk
 
 namespace owtk {
 
@@ -169,3 +202,20 @@ struct owtk_matrix_s {
     }
 };
 
+
+namespace owtk {
+
+enum ProfileCategory : int {
+    prof_spmv_f64 = 0,
+    prof_spmv_f32 = 1,
+    prof_spmv_bf16 = 2,
+    prof_precond_f64 = 3,
+    prof_precond_low = 4,
+    prof_learned = 5,
+    prof_categories = 8,
+};
+
+// Algorithmic HBM bytes of one SpMV with value size sv and vector size sx.
+std::int64_t spmv_bytes(owtk_matrix m, int sv, int sx);
+
+} // namespace owtk

@@ -72,15 +72,24 @@ __global__ __launch_bounds__(256) void k_dot(const X* __restrict__ a, const X* _
     const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
         red[0] = __builtin_fma(static_cast<double>(a[i]), static_cast<double>(b[i]), red[0]);
-    grid_reduce<1>(red, 0, scratch, [&](const double* t) {
-        if (threadIdx.x == 0) *out = t[0];
-    });
+    store_partials<1>(red, 0, scratch);
 }
+
+struct FinStore {
+    double* out;
+    int count;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x < count) out[threadIdx.x] = t[threadIdx.x];
+    }
+};
 
 template<class X>
 inline void dot_to(hipStream_t st, ReduceScratch scratch, const X* a, const X* b, std::int64_t n, double* out)
 {
-    k_dot<X><<<stream_grid(n, 4), kBlock, 0, st>>>(a, b, n, scratch, out);
+    const unsigned int g = stream_grid(n, 4);
+    k_dot<X><<<g, kBlock, 0, st>>>(a, b, n, scratch, out);
+    launch_finish<1>(st, FinStore{out, 1}, g, 0, scratch, nullptr);
     OWTK_LAUNCH_CHECK();
 }
 
@@ -95,16 +104,27 @@ __global__ __launch_bounds__(256) void k_norm(const X* __restrict__ a, std::int6
         red[0] = __builtin_fma(v, v, red[0]);
         red[1] = combine<Combine::max>(red[1], __builtin_fabs(v));
     }
-    grid_reduce<2>(red, 2, scratch, [&](const double* t) {
+    store_partials<2>(red, 2, scratch);
+}
+
+template<class X>
+struct FinNorm {
+    const X* a;
+    std::int64_t n;
+    double* out;
+    __device__ void operator()(const double* t) const
+    {
         const double nrm = block_norm(t[0], t[1], a, n);
         if (threadIdx.x == 0) *out = nrm;
-    });
-}
+    }
+};
 
 template<class X>
 inline void norm_to(hipStream_t st, ReduceScratch scratch, const X* a, std::int64_t n, double* out)
 {
-    k_norm<X><<<stream_grid(n, 4), kBlock, 0, st>>>(a, n, scratch, out);
+    const unsigned int g = stream_grid(n, 4);
+    k_norm<X><<<g, kBlock, 0, st>>>(a, n, scratch, out);
+    launch_finish<2>(st, FinNorm<X>{a, n, out}, g, 2, scratch, nullptr);
     OWTK_LAUNCH_CHECK();
 }
 
@@ -133,15 +153,15 @@ __global__ __launch_bounds__(256) void k_axpy_dot(std::int64_t n, double alpha, 
         y[i] = v;
         red[0] = __builtin_fma(w[i], v, red[0]);
     }
-    grid_reduce<1>(red, 0, scratch, [&](const double* t) {
-        if (threadIdx.x == 0) *out = t[0];
-    });
+    store_partials<1>(red, 0, scratch);
 }
 
 inline void axpy_dot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, double alpha, const double* x,
                         double* y, const double* w, double* out)
 {
-    k_axpy_dot<<<stream_grid(n, 4), kBlock, 0, st>>>(n, alpha, x, y, w, scratch, out);
+    const unsigned int g = stream_grid(n, 4);
+    k_axpy_dot<<<g, kBlock, 0, st>>>(n, alpha, x, y, w, scratch, out);
+    launch_finish<1>(st, FinStore{out, 1}, g, 0, scratch, nullptr);
     OWTK_LAUNCH_CHECK();
 }
 
@@ -166,13 +186,21 @@ __global__ __launch_bounds__(256) void k_multidot(std::int64_t n, int k, BasisPo
             if (j < k) red[j] = __builtin_fma(wi, basis.v[j][i], red[j]);
         red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
     }
-    grid_reduce<K>(red, 0, scratch, [&](const double* t) {
+    store_partials<K>(red, 0, scratch);
+}
+
+template<int K>
+struct FinMulti {
+    double* out;
+    int k;
+    __device__ void operator()(const double* t) const
+    {
         if (threadIdx.x == 0) {
             for (int j = 0; j < k; ++j) out[j] = t[j];
             out[k] = t[K - 1];
         }
-    });
-}
+    }
+};
 
 inline void multidot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, int k, const double* const* vs,
                         const double* w, double* out)
@@ -183,11 +211,22 @@ inline void multidot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, i
     if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
     const unsigned int g = static_cast<unsigned int>(blocks);
-    if (k + 1 <= 4) k_multidot<4><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
-    else if (k + 1 <= 8) k_multidot<8><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
-    else if (k + 1 <= 16) k_multidot<16><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
-    else if (k + 1 <= 32) k_multidot<32><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
-    else k_multidot<63><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+    if (k + 1 <= 4) {
+        k_multidot<4><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+        launch_finish<4>(st, FinMulti<4>{out, k}, g, 0, scratch, nullptr);
+    } else if (k + 1 <= 8) {
+        k_multidot<8><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+        launch_finish<8>(st, FinMulti<8>{out, k}, g, 0, scratch, nullptr);
+    } else if (k + 1 <= 16) {
+        k_multidot<16><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+        launch_finish<16>(st, FinMulti<16>{out, k}, g, 0, scratch, nullptr);
+    } else if (k + 1 <= 32) {
+        k_multidot<32><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+        launch_finish<32>(st, FinMulti<32>{out, k}, g, 0, scratch, nullptr);
+    } else {
+        k_multidot<63><<<g, kBlock, 0, st>>>(n, k, bp, w, scratch, out);
+        launch_finish<63>(st, FinMulti<63>{out, k}, g, 0, scratch, nullptr);
+    }
     OWTK_LAUNCH_CHECK();
 }
 

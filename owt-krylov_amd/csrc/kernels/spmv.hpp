@@ -104,6 +104,13 @@ struct EpiDot2 : EpiNone {
     }
 };
 
+// Finisher adaptor: runs an epilogue's finalize in the one-block finisher.
+template<class E>
+struct EpiFinish {
+    E e;
+    __device__ void operator()(const double* t) const { e.finalize(t); }
+};
+
 } // namespace owtk::dev
 
 namespace owtk::kern {
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
 #pragma unroll
         for (int c = 0; c < BS; ++c) e.elem(row * BS + c, acc[c], red);
     }
-    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+    if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
 }
 
 // Component-diagonal (and split) SpMV for any block size, reference CSR
@@ -198,7 +205,7 @@ __global__ __launch_bounds__(256) void spmv_csr_cd(const std::int32_t* __restric
             e.elem(row * bs + c, acc, red);
         }
     }
-    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+    if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
 }
 
 // Dense BSR, SELL-64 layout, one lane per row, BS x BS blocks.
@@ -242,7 +249,7 @@ __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restr
 #pragma unroll
         for (int r = 0; r < BS; ++r) e.elem(row * BS + r, acc[r], red);
     }
-    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+    if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
 }
 
 // Dense BSR for any block size, reference CSR layout: one thread per
@@ -275,7 +282,7 @@ __global__ __launch_bounds__(256) void spmv_csr_bsr(const std::int32_t* __restri
         }
         e.elem(i, acc, red);
     }
-    if constexpr (E::K > 0) grid_reduce<E::K>(red, E::kMaxMask, scratch, [&](const double* t) { e.finalize(t); });
+    if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
 }
 
 } // namespace owtk::kern
@@ -292,12 +299,17 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
     const auto scratch = m->ctx->scratch();
     const auto& s = m->s;
     const std::int64_t rows = m->owned;
+    constexpr int sv = static_cast<int>(sizeof(V));
+    constexpr int category = sv == 8 ? prof_spmv_f64 : (sv == 4 ? prof_spmv_f32 : prof_spmv_bf16);
+    ProfileScope prof(m->ctx, category, spmv_bytes(m, sv, static_cast<int>(sizeof(X))));
+    unsigned int launched = 0;
     if (s.layout == Layout::sell) {
         const std::int64_t blocks = (rows + kBlock - 1) / kBlock;
         if (E::K > 0 && blocks > kMaxReduceBlocks)
             throw unsupported_error("fused SpMV reductions support at most 4,194,304 rows");
         if (blocks > 0x7fffffff) throw std::invalid_argument("too many rows");
         const unsigned int grid = static_cast<unsigned int>(blocks);
+        launched = grid;
         if (m->dense()) {
             switch (m->bs) {
             case 2: spmv_sell_bsr<2, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
@@ -323,10 +335,12 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
     } else {
         if (m->dense()) {
             const unsigned int grid = stream_grid(rows * m->bs);
+            launched = grid;
             spmv_csr_bsr<V, X, E><<<grid, kBlock, 0, st>>>(s.offsets.get(), s.columns.get(), vals, x, rows, m->bs, e, scratch);
         } else {
             std::int64_t blocks = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
             if (blocks > kMaxReduceBlocks) blocks = kMaxReduceBlocks;
+            launched = static_cast<unsigned int>(blocks);
             if (m->has_diag_array())
                 spmv_csr_cd<V, X, true, E><<<static_cast<unsigned int>(blocks), kBlock, 0, st>>>(s.offsets.get(), s.columns.get(), vals, diag, x, rows, m->bs, e, scratch);
             else
@@ -334,6 +348,10 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
         }
     }
     OWTK_LAUNCH_CHECK();
+    if constexpr (E::K > 0) {
+        launch_finish<E::K>(st, EpiFinish<E>{e}, launched, E::kMaxMask, scratch, e.stop);
+        OWTK_LAUNCH_CHECK();
+    }
 }
 
 // FP64 SpMV with an epilogue (the solver path).

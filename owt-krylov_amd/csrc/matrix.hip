@@ -183,6 +183,8 @@ owtk_matrix create_matrix(owtk_context ctx, int kind, std::int64_t owned, std::i
     return m.release();
 }
 
+} // namespace
+
 std::int64_t spmv_bytes(owtk_matrix m, int sv, int sx)
 {
     const std::int64_t n = m->owned, b = m->bs;
@@ -191,7 +193,6 @@ std::int64_t spmv_bytes(owtk_matrix m, int sv, int sx)
     return m->entries * (b * sv + 4) + diag + n * 4 + 2 * n * b * sx;
 }
 
-} // namespace
 } // namespace owtk
 
 extern "C" {

@@ -195,7 +195,8 @@ struct TriArgs {
 // ILU(k) numeric factorization (IluLevelPreconditioner::factorize,
 // preconditioner.hpp:439-485), one workgroup per (block, component), rows in
 // forward level order.  F is the FP64 factor, [c][entry].
-__global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int64_t* __restrict__ offsets,
+__global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int32_t* __restrict__ lvl_rows,
+                                                    const std::int64_t* __restrict__ offsets,
                                                     const std::int64_t* __restrict__ diag_pos,
                                                     const std::int64_t* __restrict__ upd_ptr,
                                                     const std::int64_t* __restrict__ upd_target,
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int64_
     for (int l = l_begin; l < l_end; ++l) {
         const int p0 = a.flev_pos[l], p1 = a.flev_pos[l + 1];
         for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-            const std::int64_t row = r0 + a.frow[r0 + p];
+            const std::int64_t row = r0 + lvl_rows[r0 + p];
             const std::int64_t dpos = diag_pos[row];
             for (std::int64_t e = offsets[row]; e < dpos; ++e) {
                 const std::int64_t k = columns_global[e];
@@ -322,6 +323,131 @@ __global__ __launch_bounds__(256) void k_tri_apply(TriArgs a, const V* __restric
     }
 }
 
+__device__ inline double lane_value(double v, int lane)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ inline float lane_value(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// One sweep of a chain-scheduled triangular solve by one wave.  Rows are
+// taken in chunks of 64 in sweep order (ascending rows forward, descending
+// backward), lane l owning the chunk's l-th row.  Rows outside the chunk that
+// are already final are read from LDS; inside the chunk, step s finalises
+// lane s's row and broadcasts it with v_readlane, and every lane records the
+// values its entries need.  A lane consumes its entries strictly in their
+// stored order as soon as each value is known, so every row accumulates in
+// the reference's order and results equal the level-scheduled kernel's.
+template<TriMode MODE, bool FWD, class X, class V, int E>
+__device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const std::int32_t* __restrict__ cols,
+                                   const V* __restrict__ vals, const X* __restrict__ dv, X* y, int nb, int lane)
+{
+    for (int cs = 0; cs < nb; cs += 64) {
+        const int p = cs + lane;
+        const bool valid = p < nb;
+        const int i = FWD ? p : nb - 1 - p;
+        int cnt = 0;
+        std::int64_t e0 = 0;
+        if (valid) {
+            e0 = ptr[p];
+            cnt = static_cast<int>(ptr[p + 1] - e0);
+        }
+        int col[E];
+        X val[E], xv[E];
+        unsigned int avail = 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            col[k] = -1;
+            val[k] = X(0);
+            xv[k] = X(0);
+            if (k < cnt) {
+                col[k] = cols[e0 + k];
+                val[k] = static_cast<X>(vload(vals, e0 + k));
+            }
+        }
+        // rows before this chunk in sweep order are final and sit in LDS
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const bool done = FWD ? (k < cnt && col[k] < cs) : (k < cnt && col[k] > nb - 1 - cs);
+            if (done) {
+                xv[k] = y[col[k]];
+                avail |= 1u << k;
+            }
+        }
+        X acc = X(0);
+        if (valid) {
+            if constexpr (MODE == TriMode::ssor && !FWD) acc = dv[i] * y[i];
+            else acc = y[i];
+        }
+        int next = 0;
+        auto advance = [&]() {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                if (k == next && k < cnt && ((avail >> k) & 1u)) {
+                    acc = dev::dfma(-val[k], xv[k], acc);
+                    next = k + 1;
+                }
+            }
+        };
+        advance();
+        const int steps = nb - cs < 64 ? nb - cs : 64;
+        for (int s = 0; s < steps; ++s) {
+            X fin = acc;
+            if (lane == s) {
+                if constexpr (MODE == TriMode::ssor) fin = acc / dv[i];
+                else if constexpr (!FWD) fin = acc * dv[i];
+                y[i] = fin;
+            }
+            const X v = lane_value(fin, s);
+            const int row_s = FWD ? cs + s : nb - 1 - cs - s;
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                if (k < cnt && col[k] == row_s) {
+                    xv[k] = v;
+                    avail |= 1u << k;
+                }
+            }
+            advance();
+        }
+        __syncthreads();
+    }
+}
+
+template<TriMode MODE, class X, class V, int E>
+__global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict__ fval, const V* __restrict__ bval,
+                                                  const X* __restrict__ dval, const X* in, X* out, X scale,
+                                                  const int* stop)
+{
+    extern __shared__ unsigned char smem_raw[];
+    if (stop && *stop) return;
+    const int bs = a.bs;
+    const int c = blockIdx.x % bs;
+    const int b = blockIdx.x / bs;
+    const std::int64_t r0 = a.row0[b];
+    const int nb = a.nrows[b];
+    const int lane = threadIdx.x;
+    X* y = reinterpret_cast<X*>(smem_raw);
+    for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
+    __syncthreads();
+    const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
+    chain_sweep<MODE, true, X, V, E>(a.fptr + r0, a.fcol, fval + static_cast<std::int64_t>(c) * a.fnnz, dv, y, nb,
+                                     lane);
+    chain_sweep<MODE, false, X, V, E>(a.bptr + r0, a.bcol, bval + static_cast<std::int64_t>(c) * a.bnnz, dv, y, nb,
+                                      lane);
+    for (int i = lane; i < nb; i += 64) {
+        X v = y[i];
+        if constexpr (MODE == TriMode::ssor)
+            if (scale != X(1)) v = v * scale;
+        out[(r0 + i) * bs + c] = v;
+    }
+}
+
 // ---------------------------------------------------------------- symbolic --
 
 // Encoded gather source of matrix value (row, CSR entry e): >= 0 index of
@@ -380,6 +506,8 @@ struct TriHost {
     std::vector<std::int64_t> f_off, f_col, f_src, b_off, b_col, b_src;
 };
 
+std::size_t lds_limit(owtk_precond pc);
+
 void build_schedule(owtk_precond pc, const TriHost& t)
 {
     auto& tri = pc->tri;
@@ -394,40 +522,59 @@ void build_schedule(owtk_precond pc, const TriHost& t)
     tri.h_blev.assign(tri.blocks + 1, 0);
     tri.h_flev_pos.clear();
     tri.h_blev_pos.clear();
-    std::vector<std::int32_t> frow(n), brow(n);
-    std::vector<std::int64_t> fptr(n + 1, 0), bptr(n + 1, 0);
-    std::vector<std::int32_t> fcol, bcol;
-    std::vector<std::int64_t> fsrc, bsrc;
-    fcol.reserve(t.f_col.size());
-    bcol.reserve(t.b_col.size());
-    std::vector<std::int32_t> level(n, 0);
     tri.max_block_rows = tri.max_flevels = tri.max_blevels = tri.max_level_width = 0;
+    // pass 1: levels (forward from the top, backward from the bottom), depths, widths
+    std::vector<std::int32_t> flevel(n, 0), blevel(n, 0);
+    std::vector<int> fdepth(tri.blocks, 0), bdepth(tri.blocks, 0);
+    int emax = 0;
+    for (std::int64_t i = 0; i < n; ++i)
+        emax = std::max<int>(emax, static_cast<int>(std::max(t.f_off[i + 1] - t.f_off[i], t.b_off[i + 1] - t.b_off[i])));
+    std::int64_t depth_sum = 0;
     for (std::int64_t b = 0; b < tri.blocks; ++b) {
         const std::int64_t r0 = b * bn;
         const std::int64_t nb = std::min(bn, n - r0);
         tri.h_row0[b] = static_cast<std::int32_t>(r0);
         tri.h_nrows[b] = static_cast<std::int32_t>(nb);
         tri.max_block_rows = std::max(tri.max_block_rows, nb);
+        for (std::int64_t i = r0; i < r0 + nb; ++i) {
+            int lv = 0;
+            for (std::int64_t e = t.f_off[i]; e < t.f_off[i + 1]; ++e) lv = std::max(lv, flevel[t.f_col[e]] + 1);
+            flevel[i] = lv;
+            fdepth[b] = std::max(fdepth[b], lv + 1);
+        }
+        for (std::int64_t i = r0 + nb; i-- > r0;) {
+            int lv = 0;
+            for (std::int64_t e = t.b_off[i]; e < t.b_off[i + 1]; ++e) lv = std::max(lv, blevel[t.b_col[e]] + 1);
+            blevel[i] = lv;
+            bdepth[b] = std::max(bdepth[b], lv + 1);
+        }
+        tri.max_flevels = std::max<std::int64_t>(tri.max_flevels, fdepth[b]);
+        tri.max_blevels = std::max<std::int64_t>(tri.max_blevels, bdepth[b]);
+        depth_sum += std::max(fdepth[b], bdepth[b]);
+    }
+    // Chain sweeps win when levels are narrow (average width below 16 rows):
+    // a level then costs a barrier for a handful of rows.
+    tri.emax = emax;
+    const bool chain_ok = emax <= 16 && static_cast<std::size_t>(tri.max_block_rows) * sizeof(double) <= lds_limit(pc);
+    if (pc->schedule == 2 && !chain_ok)
+        throw unsupported_error("chain sweeps need at most 16 entries per row and a block that fits in LDS");
+    tri.chain = pc->schedule == 2 || (pc->schedule == 0 && chain_ok && depth_sum * 16 > n);
+    // pass 2: level lists and apply arrays
+    std::vector<std::int32_t> frow(n), brow(n), frow_lvl(n);
+    std::vector<std::int64_t> fptr(n + 1, 0), bptr(n + 1, 0);
+    std::vector<std::int32_t> fcol, bcol;
+    std::vector<std::int64_t> fsrc, bsrc;
+    fcol.reserve(t.f_col.size());
+    bcol.reserve(t.b_col.size());
+    for (std::int64_t b = 0; b < tri.blocks; ++b) {
+        const std::int64_t r0 = tri.h_row0[b];
+        const std::int64_t nb = tri.h_nrows[b];
         for (int dir = 0; dir < 2; ++dir) {
             const auto& off = dir == 0 ? t.f_off : t.b_off;
             const auto& col = dir == 0 ? t.f_col : t.b_col;
             const auto& src = dir == 0 ? t.f_src : t.b_src;
-            int depth = 0;
-            if (dir == 0) {
-                for (std::int64_t i = r0; i < r0 + nb; ++i) {
-                    int lv = 0;
-                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) lv = std::max(lv, level[col[e]] + 1);
-                    level[i] = lv;
-                    depth = std::max(depth, lv + 1);
-                }
-            } else {
-                for (std::int64_t i = r0 + nb; i-- > r0;) {
-                    int lv = 0;
-                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) lv = std::max(lv, level[col[e]] + 1);
-                    level[i] = lv;
-                    depth = std::max(depth, lv + 1);
-                }
-            }
+            const auto& level = dir == 0 ? flevel : blevel;
+            const int depth = dir == 0 ? fdepth[b] : bdepth[b];
             // counting sort by level, stable in row order (forward) / reverse row order (backward)
             std::vector<std::int64_t> count(depth + 1, 0);
             for (std::int64_t i = r0; i < r0 + nb; ++i) ++count[level[i] + 1];
@@ -438,15 +585,19 @@ void build_schedule(owtk_precond pc, const TriHost& t)
             for (int l = 0; l <= depth; ++l) lev_pos.push_back(static_cast<std::int32_t>(count[l]));
             for (int l = 0; l < depth; ++l) tri.max_level_width = std::max(tri.max_level_width, count[l + 1] - count[l]);
             lev[b + 1] = static_cast<std::int32_t>(lev_pos.size());
-            if (dir == 0) tri.max_flevels = std::max<std::int64_t>(tri.max_flevels, depth);
-            else tri.max_blevels = std::max<std::int64_t>(tri.max_blevels, depth);
             std::vector<std::int64_t> next(count.begin(), count.end() - 1);
-            auto& rowmap = dir == 0 ? frow : brow;
-            auto place = [&](std::int64_t i) { rowmap[r0 + next[level[i]]++] = static_cast<std::int32_t>(i - r0); };
+            std::vector<std::int32_t> lvl_map(nb);
+            auto place = [&](std::int64_t i) { lvl_map[next[level[i]]++] = static_cast<std::int32_t>(i - r0); };
             if (dir == 0)
                 for (std::int64_t i = r0; i < r0 + nb; ++i) place(i);
             else
                 for (std::int64_t i = r0 + nb; i-- > r0;) place(i);
+            if (dir == 0) std::copy(lvl_map.begin(), lvl_map.end(), frow_lvl.begin() + r0);
+            auto& rowmap = dir == 0 ? frow : brow;
+            for (std::int64_t p = 0; p < nb; ++p) {
+                if (tri.chain) rowmap[r0 + p] = static_cast<std::int32_t>(dir == 0 ? p : nb - 1 - p);
+                else rowmap[r0 + p] = lvl_map[p];
+            }
             auto& ptr = dir == 0 ? fptr : bptr;
             auto& ocol = dir == 0 ? fcol : bcol;
             auto& osrc = dir == 0 ? fsrc : bsrc;
@@ -458,10 +609,8 @@ void build_schedule(owtk_precond pc, const TriHost& t)
                 }
                 ptr[r0 + p + 1] = static_cast<std::int64_t>(ocol.size());
             }
-            // blocks are consecutive, so ptr[r0] is the previous block's end
         }
     }
-    // fix: make ptr monotone across block starts
     tri.fnnz = static_cast<std::int64_t>(fcol.size());
     tri.bnnz = static_cast<std::int64_t>(bcol.size());
     upload(tri.row0, tri.h_row0);
@@ -472,6 +621,7 @@ void build_schedule(owtk_precond pc, const TriHost& t)
     upload(tri.blev_pos, tri.h_blev_pos);
     upload(tri.frow, frow);
     upload(tri.brow, brow);
+    upload(tri.frow_lvl, frow_lvl);
     upload(tri.fptr, fptr);
     upload(tri.bptr, bptr);
     upload(tri.fcol, fcol);
@@ -489,6 +639,7 @@ TriArgs tri_args(const TriSchedule& t)
 
 std::size_t lds_limit(owtk_precond pc)
 {
+    // device LDS limit per workgroup (cached per device)
     static thread_local int cached_device = -1;
     static thread_local std::size_t cached = 0;
     if (cached_device != pc->m->ctx->device) {
@@ -785,7 +936,7 @@ void ilu_numeric(owtk_precond pc)
                                                       m->v64.get(), m->d64.get(), 1.0);
     OWTK_LAUNCH_CHECK();
     const unsigned int groups = static_cast<unsigned int>(t.blocks * bs);
-    k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), f.offsets.get(), f.diag_pos.get(), f.upd_ptr.get(),
+    k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), t.frow_lvl.get(), f.offsets.get(), f.diag_pos.get(), f.upd_ptr.get(),
                                         f.upd_target.get(), f.upd_upper.get(), f.columns.get(), f.values.get(),
                                         f.nnz, t.dval64.get(), pc->flag.get());
     OWTK_LAUNCH_CHECK();
@@ -812,7 +963,16 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
     const int threads = t.max_level_width > 128 ? 256 : (t.max_level_width > 64 ? 128 : 64);
     const X scale = static_cast<X>(pc->omega * (2.0 - pc->omega));
     const hipStream_t st = pc->m->ctx->stream;
-    if (need <= lds_limit(pc)) {
+    ProfileScope prof(pc->m->ctx, sizeof(X) == 8 ? prof_precond_f64 : prof_precond_low, precond_apply_bytes(pc));
+    if (t.chain) {
+        const TriArgs args = tri_args(t);
+        if (t.emax <= 4)
+            k_tri_chain<MODE, X, V, 4><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        else if (t.emax <= 8)
+            k_tri_chain<MODE, X, V, 8><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        else
+            k_tri_chain<MODE, X, V, 16><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
+    } else if (need <= lds_limit(pc)) {
         k_tri_apply<MODE, X, V, true><<<groups, threads, need, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
     } else {
         k_tri_apply<MODE, X, V, false><<<groups, threads, 0, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
@@ -950,6 +1110,8 @@ int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_p
         pc->minimum_diagonal = params->minimum_diagonal;
         pc->block_nodes = params->block_nodes;
         pc->fill_level = params->fill_level;
+        pc->schedule = params->schedule;
+        if (pc->schedule < 0 || pc->schedule > 2) throw std::invalid_argument("unknown triangular schedule");
         if (pc->storage < OWTK_F64 || pc->storage > OWTK_BF16) throw std::invalid_argument("unknown storage type");
         if (pc->block_nodes < 0) throw std::invalid_argument("negative block size");
         pc->flag.allocate(4);

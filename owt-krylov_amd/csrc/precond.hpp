@@ -27,8 +27,12 @@ struct TriSchedule {
     // level -> first position (size levels+1), positions are block-local
     DeviceArray<std::int32_t> flev_pos, blev_pos;
     std::vector<std::int32_t> h_flev_pos, h_blev_pos;
-    // position (global, level order) -> block-local row
+    // apply position (global) -> block-local row: level order, or plain row
+    // order (forward) / reverse row order (backward) for chain sweeps
     DeviceArray<std::int32_t> frow, brow;
+    DeviceArray<std::int32_t> frow_lvl;  // forward level order (numeric factorization)
+    bool chain = false;  // one-wave chain sweeps instead of level barriers
+    int emax = 0;        // most entries of one row in one direction
     // entry pointers per position (global), block-local columns
     DeviceArray<std::int64_t> fptr, bptr;
     DeviceArray<std::int32_t> fcol, bcol;
@@ -67,6 +71,7 @@ struct owtk_precond_s {
     double minimum_diagonal = 0.0;
     std::int64_t block_nodes = 0;
     int fill_level = 0;
+    int schedule = 0;  // 0 auto, 1 level sets, 2 chains
     std::uint64_t updates = 0;
     // Jacobi: inverse diagonal (component-diagonal) or inverse blocks (dense)
     owtk::DeviceArray<double> inv64;

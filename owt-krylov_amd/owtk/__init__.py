@@ -64,6 +64,22 @@ class Context:
     def synchronize(self) -> None:
         check(lib().owtk_context_synchronize(self.handle))
 
+    # live kernel timing (HIP events on this context's stream)
+    PROF_SPMV_F64, PROF_SPMV_F32, PROF_SPMV_BF16, PROF_PRECOND_F64, PROF_PRECOND_LOW, PROF_LEARNED = range(6)
+
+    def profile(self, enable: bool = True) -> None:
+        check(lib().owtk_profile_enable(self.handle, int(enable)))
+
+    def profile_reset(self) -> None:
+        check(lib().owtk_profile_reset(self.handle))
+
+    def profile_read(self, category: int) -> dict:
+        n, ms, b = C.c_uint64(), C.c_double(), C.c_int64()
+        check(lib().owtk_profile_read(self.handle, category, C.byref(n), C.byref(ms), C.byref(b)))
+        return {"launches": n.value, "total_ms": ms.value, "bytes": b.value,
+                "avg_ms": ms.value / n.value if n.value else float("nan"),
+                "gbps": (b.value / (ms.value * 1e-3) / 1e9) if ms.value > 0 else float("nan")}
+
     def empty(self, count: int, dtype=np.float64) -> "DeviceBuffer":
         return DeviceBuffer(self, count, dtype)
 
@@ -233,12 +249,13 @@ class Matrix:
 
 class Preconditioner:
     def __init__(self, matrix: Matrix, kind: int, *, storage: int = F64, omega: float = 1.0,
-                 minimum_diagonal: float = 0.0, block_nodes: int = 0, fill_level: int = 0, model=None):
+                 minimum_diagonal: float = 0.0, block_nodes: int = 0, fill_level: int = 0, model=None,
+                 schedule: int = 0):
         self.matrix = matrix
         p = _lib.PrecondParams()
         lib().owtk_precond_params_default(C.byref(p), kind)
         p.storage, p.omega, p.minimum_diagonal = storage, omega, minimum_diagonal
-        p.block_nodes, p.fill_level = block_nodes, fill_level
+        p.block_nodes, p.fill_level, p.schedule = block_nodes, fill_level, schedule
         if model is not None:
             p.learned_model = model.handle
         self._model = model

@@ -64,7 +64,7 @@ class PrecondParams(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("storage", C.c_int32), ("omega", C.c_double),
         ("minimum_diagonal", C.c_double), ("block_nodes", C.c_int64), ("fill_level", C.c_int32),
-        ("reserved", C.c_int32), ("learned_model", C.c_void_p),
+        ("schedule", C.c_int32), ("learned_model", C.c_void_p),
     ]
 
 
@@ -130,6 +130,10 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_version": (C.c_char_p, []),
     "owtk_policy_name": (C.c_char_p, []),
     "owtk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "owtk_profile_enable": (C.c_int, [_P, C.c_int]),
+    "owtk_profile_reset": (C.c_int, [_P]),
+    "owtk_profile_read": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int64)]),
     "owtk_context_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "owtk_context_destroy": (C.c_int, [_P]),
     "owtk_context_synchronize": (C.c_int, [_P]),

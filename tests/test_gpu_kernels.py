@@ -184,27 +184,29 @@ def test_ssor_closed_form(ctx, block, omega):
         assert np.all(out[2 * block:] == 0)  # output ghosts untouched (zero-initialised buffer)
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
 @pytest.mark.parametrize("omega", [1.0, 1.3])
 @pytest.mark.parametrize("block_nodes", [0, 512])
-def test_ssor_c0_bitwise(ctx, c0, omega, block_nodes):
+def test_ssor_c0_bitwise(ctx, c0, omega, block_nodes, schedule):
     s = c0
     om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
     m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
     r = np.random.default_rng(11).uniform(-1, 1, s.nodes * 3)
     ref = orc.OPreconditioner(om, 2, omega=omega, block_nodes=block_nodes).apply(r)
-    pc = owtk.Preconditioner(m, owtk.PC_SSOR, omega=omega, block_nodes=block_nodes)
+    pc = owtk.Preconditioner(m, owtk.PC_SSOR, omega=omega, block_nodes=block_nodes, schedule=schedule)
     assert np.array_equal(gpu_apply(ctx, pc, r), ref)
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
 @pytest.mark.parametrize("fill", [0, 1, 2])
-def test_ilu_audit_factors_bitwise(ctx, fill):
+def test_ilu_audit_factors_bitwise(ctx, fill, schedule):
     n, comps, offsets, columns, values = rc.ilu_audit_matrix()
     for update in range(2):
         vals = values if not update else values * np.array([1.1 + 0.02 * (e % 3) for e in range(values.size)])
         om = orc.OMatrix.csr(n, 1, comps, offsets, columns, vals)
         opc = orc.OPreconditioner(om, 3, fill_level=fill)
         m = owtk.Matrix.csr(ctx, n, 1, comps, offsets, columns, vals)
-        pc = owtk.Preconditioner(m, owtk.PC_ILU, fill_level=fill)
+        pc = owtk.Preconditioner(m, owtk.PC_ILU, fill_level=fill, schedule=schedule)
         for a, b in zip(pc.ilu_factors(), opc.ilu_factors()):
             assert np.array_equal(a, b)
         r = np.random.default_rng(12).uniform(-1, 1, (n + 1) * comps)
@@ -238,12 +240,13 @@ def test_ilu_zero_pivot(ctx):
     assert np.allclose(gpu_apply(ctx, pc, [3.0, 3.0]), [1.0, 1.0], atol=1e-14)
 
 
-def test_ilu_block_jacobi_medium_bitwise(ctx):
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_ilu_block_jacobi_medium_bitwise(ctx, schedule):
     """512 x 256 grid, Hilbert order, 4096-vertex block-Jacobi ILU(0)."""
     s = owtk.generate_split(512, 256, 3, 1, ordering=owtk.ORDER_HILBERT)
     om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
     m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
-    pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=4096)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=4096, schedule=schedule)
     opc = orc.OPreconditioner(om, 3, block_nodes=4096)
     r = np.random.default_rng(14).uniform(-1, 1, s.nodes * 3)
     assert np.array_equal(gpu_apply(ctx, pc, r), opc.apply(r))
@@ -258,6 +261,17 @@ def test_ilu_low_precision_storage(ctx, c0):
     r = np.random.default_rng(15).uniform(-1, 1, s.nodes * 3)
     ref = orc.OPreconditioner(om, 3, block_nodes=512).apply(r)
     for storage, tol in ((owtk.F32, 1e-5), (owtk.BF16, 5e-2)):
-        pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512, storage=storage)
+        pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512, storage=storage, schedule=1)
         out = gpu_apply(ctx, pc, r.astype(np.float32), np.float32)
         assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize("storage", [owtk.F32, owtk.BF16])
+def test_chain_and_level_agree_low_precision(ctx, c0, storage):
+    s = c0
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    r = np.random.default_rng(16).uniform(-1, 1, s.nodes * 3).astype(np.float32)
+    for kind in (owtk.PC_ILU, owtk.PC_SSOR):
+        a = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=1)
+        b = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=2)
+        assert np.array_equal(gpu_apply(ctx, a, r, np.float32), gpu_apply(ctx, b, r, np.float32))
