@@ -27,6 +27,15 @@ template<class V> __device__ inline auto load_value(const V* p, std::int64_t i)
     else return p[i];
 }
 
+// Matrix values are read once per SpMV: stream them past the caches so the
+// gathered vector stays resident.
+template<class V> __device__ inline auto load_value_nt(const V* p, std::int64_t i)
+{
+    if constexpr (sizeof(V) == 2)
+        return bf16_to_float(__builtin_nontemporal_load(reinterpret_cast<const std::uint16_t*>(p) + i));
+    else return __builtin_nontemporal_load(p + i);
+}
+
 // ----------------------------------------------------------------- epilogues
 struct EpiNone {
     static constexpr int K = 0;
@@ -158,7 +167,25 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
         }
         const std::int64_t base = slice_ptr[slice];
         const int len = row_len[row];
-        for (int k = 0; k < len; ++k) {
+        // Mesh rows hold at most kUnroll entries: the unrolled, predicated loop
+        // lets every column, value and gather load of the row issue before the
+        // first FMA.  Longer rows finish in the generic loop.
+        constexpr int kUnroll = 8;
+        std::int64_t colk[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k)
+            colk[k] = k < len ? static_cast<std::int64_t>(__builtin_nontemporal_load(&cols[base + 64 * k + lane])) : row;
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            if (k < len) {
+                const V* vk = vals + (base + 64 * k) * BS + lane;
+#pragma unroll
+                for (int c = 0; c < BS; ++c)
+                    acc[c] = dfma(static_cast<A>(load_value_nt(vk, 64 * c)), static_cast<A>(x[colk[k] * BS + c]),
+                                  acc[c]);
+            }
+        }
+        for (int k = kUnroll; k < len; ++k) {
             const std::int64_t run = base + 64 * k;
             const std::int64_t col = cols[run + lane];
             const V* vk = vals + run * BS + lane;

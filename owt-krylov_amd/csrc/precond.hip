@@ -188,6 +188,8 @@ struct TriArgs {
     const std::int64_t* bptr;
     const std::int32_t* fcol;
     const std::int32_t* bcol;
+    const std::int32_t* fnout;
+    const std::int32_t* bnout;
     std::int64_t fnnz, bnnz, rows;
     int bs;
 };
@@ -338,88 +340,74 @@ __device__ inline float lane_value(float v, int lane)
 
 // One sweep of a chain-scheduled triangular solve by one wave.  Rows are
 // taken in chunks of 64 in sweep order (ascending rows forward, descending
-// backward), lane l owning the chunk's l-th row.  Rows outside the chunk that
-// are already final are read from LDS; inside the chunk, step s finalises
-// lane s's row and broadcasts it with v_readlane, and every lane records the
-// values its entries need.  A lane consumes its entries strictly in their
-// stored order as soon as each value is known, so every row accumulates in
-// the reference's order and results equal the level-scheduled kernel's.
-template<TriMode MODE, bool FWD, class X, class V, int E>
-__device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const std::int32_t* __restrict__ cols,
-                                   const V* __restrict__ vals, const X* __restrict__ dv, X* y, int nb, int lane)
+// backward), lane l owning the chunk's l-th row.  Each row's entries are
+// stored as [entries whose row lies before the chunk] then [entries inside
+// the chunk, by the step at which that row is finished].  The first group is
+// final already and read from LDS.  The second group is scattered into an
+// LDS tile, tile[step][lane] = coefficient (zero elsewhere); step s then
+// broadcasts lane s's finished value with v_readlane and every lane folds in
+// its tile entry for step s.  A lane's entries all precede its own step, so
+// its value is final from its step on and is stored once at the chunk end.
+// Forward sweeps of sorted rows accumulate in the reference's order; backward
+// sweeps take their in-chunk entries in the order they become available.
+template<TriMode MODE, bool FWD, class X, class V, int EI>
+__device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const std::int32_t* __restrict__ nout,
+                                   const std::int32_t* __restrict__ cols, const V* __restrict__ vals,
+                                   const X* __restrict__ dv, X* y, X* tile, int nb, int lane)
 {
     for (int cs = 0; cs < nb; cs += 64) {
         const int p = cs + lane;
         const bool valid = p < nb;
         const int i = FWD ? p : nb - 1 - p;
-        int cnt = 0;
-        std::int64_t e0 = 0;
-        if (valid) {
-            e0 = ptr[p];
-            cnt = static_cast<int>(ptr[p + 1] - e0);
-        }
-        int col[E];
-        X val[E], xv[E];
-        unsigned int avail = 0;
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            col[k] = -1;
-            val[k] = X(0);
-            xv[k] = X(0);
-            if (k < cnt) {
-                col[k] = cols[e0 + k];
-                val[k] = static_cast<X>(vload(vals, e0 + k));
-            }
-        }
-        // rows before this chunk in sweep order are final and sit in LDS
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const bool done = FWD ? (k < cnt && col[k] < cs) : (k < cnt && col[k] > nb - 1 - cs);
-            if (done) {
-                xv[k] = y[col[k]];
-                avail |= 1u << k;
-            }
-        }
         X acc = X(0);
+        X dmine = X(1);
+        int st[EI];
+#pragma unroll
+        for (int k = 0; k < EI; ++k) st[k] = -1;
         if (valid) {
-            if constexpr (MODE == TriMode::ssor && !FWD) acc = dv[i] * y[i];
+            const std::int64_t e0 = ptr[p];
+            const std::int64_t e1 = ptr[p + 1];
+            const std::int64_t em = e0 + nout[p];
+            if constexpr (MODE == TriMode::ssor || !FWD) dmine = dv[i];
+            if constexpr (MODE == TriMode::ssor && !FWD) acc = dmine * y[i];
             else acc = y[i];
-        }
-        int next = 0;
-        auto advance = [&]() {
+            for (std::int64_t e = e0; e < em; ++e) acc = dev::dfma(-static_cast<X>(vload(vals, e)), y[cols[e]], acc);
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                if (k == next && k < cnt && ((avail >> k) & 1u)) {
-                    acc = dev::dfma(-val[k], xv[k], acc);
-                    next = k + 1;
+            for (int k = 0; k < EI; ++k) {
+                if (em + k < e1) {
+                    const int c = cols[em + k];
+                    st[k] = FWD ? c - cs : nb - 1 - cs - c;
+                    tile[st[k] * 64 + lane] = static_cast<X>(vload(vals, em + k));
                 }
             }
-        };
-        advance();
+        }
+        __syncthreads();
         const int steps = nb - cs < 64 ? nb - cs : 64;
-        for (int s = 0; s < steps; ++s) {
-            X fin = acc;
-            if (lane == s) {
-                if constexpr (MODE == TriMode::ssor) fin = acc / dv[i];
-                else if constexpr (!FWD) fin = acc * dv[i];
-                y[i] = fin;
-            }
-            const X v = lane_value(fin, s);
-            const int row_s = FWD ? cs + s : nb - 1 - cs - s;
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                if (k < cnt && col[k] == row_s) {
-                    xv[k] = v;
-                    avail |= 1u << k;
-                }
+        for (int s = 0; s < 64; ++s) {
+            if (s < steps) {
+                const X coef = tile[s * 64 + lane];
+                X fin;
+                if constexpr (MODE == TriMode::ssor) fin = acc / dmine;
+                else if constexpr (!FWD) fin = acc * dmine;
+                else fin = acc;
+                const X v = lane_value(fin, s);
+                acc = coef != X(0) ? dev::dfma(-coef, v, acc) : acc;
             }
-            advance();
         }
+        X mine;
+        if constexpr (MODE == TriMode::ssor) mine = acc / dmine;
+        else if constexpr (!FWD) mine = acc * dmine;
+        else mine = acc;
+        if (valid) y[i] = mine;
+#pragma unroll
+        for (int k = 0; k < EI; ++k)
+            if (st[k] >= 0) tile[st[k] * 64 + lane] = X(0);
         __syncthreads();
     }
 }
 
-template<TriMode MODE, class X, class V, int E>
+template<TriMode MODE, class X, class V, int EI>
 __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict__ fval, const V* __restrict__ bval,
                                                   const X* __restrict__ dval, const X* in, X* out, X scale,
                                                   const int* stop)
@@ -432,14 +420,16 @@ __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict
     const std::int64_t r0 = a.row0[b];
     const int nb = a.nrows[b];
     const int lane = threadIdx.x;
-    X* y = reinterpret_cast<X*>(smem_raw);
+    X* tile = reinterpret_cast<X*>(smem_raw);  // [64 steps][64 lanes]
+    X* y = tile + 64 * 64;                     // [nb]
+    for (int s = 0; s < 64; ++s) tile[s * 64 + lane] = X(0);
     for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
     __syncthreads();
     const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
-    chain_sweep<MODE, true, X, V, E>(a.fptr + r0, a.fcol, fval + static_cast<std::int64_t>(c) * a.fnnz, dv, y, nb,
-                                     lane);
-    chain_sweep<MODE, false, X, V, E>(a.bptr + r0, a.bcol, bval + static_cast<std::int64_t>(c) * a.bnnz, dv, y, nb,
-                                      lane);
+    chain_sweep<MODE, true, X, V, EI>(a.fptr + r0, a.fnout + r0, a.fcol, fval + static_cast<std::int64_t>(c) * a.fnnz,
+                                      dv, y, tile, nb, lane);
+    chain_sweep<MODE, false, X, V, EI>(a.bptr + r0, a.bnout + r0, a.bcol,
+                                       bval + static_cast<std::int64_t>(c) * a.bnnz, dv, y, tile, nb, lane);
     for (int i = lane; i < nb; i += 64) {
         X v = y[i];
         if constexpr (MODE == TriMode::ssor)
@@ -555,12 +545,14 @@ void build_schedule(owtk_precond pc, const TriHost& t)
     // Chain sweeps win when levels are narrow (average width below 16 rows):
     // a level then costs a barrier for a handful of rows.
     tri.emax = emax;
-    const bool chain_ok = emax <= 16 && static_cast<std::size_t>(tri.max_block_rows) * sizeof(double) <= lds_limit(pc);
+    const bool chain_ok = emax <= 16
+        && (static_cast<std::size_t>(tri.max_block_rows) + 64 * 64) * sizeof(double) <= lds_limit(pc);
     if (pc->schedule == 2 && !chain_ok)
         throw unsupported_error("chain sweeps need at most 16 entries per row and a block that fits in LDS");
     tri.chain = pc->schedule == 2 || (pc->schedule == 0 && chain_ok && depth_sum * 16 > n);
     // pass 2: level lists and apply arrays
-    std::vector<std::int32_t> frow(n), brow(n), frow_lvl(n);
+    std::vector<std::int32_t> frow(n), brow(n), frow_lvl(n), fnout(n, 0), bnout(n, 0);
+    tri.ei_max = 0;
     std::vector<std::int64_t> fptr(n + 1, 0), bptr(n + 1, 0);
     std::vector<std::int32_t> fcol, bcol;
     std::vector<std::int64_t> fsrc, bsrc;
@@ -601,11 +593,39 @@ void build_schedule(owtk_precond pc, const TriHost& t)
             auto& ptr = dir == 0 ? fptr : bptr;
             auto& ocol = dir == 0 ? fcol : bcol;
             auto& osrc = dir == 0 ? fsrc : bsrc;
+            auto& onout = dir == 0 ? fnout : bnout;
+            std::vector<std::pair<int, std::int64_t>> inside;
             for (std::int64_t p = 0; p < nb; ++p) {
                 const std::int64_t i = r0 + rowmap[r0 + p];
-                for (std::int64_t e = off[i]; e < off[i + 1]; ++e) {
-                    ocol.push_back(static_cast<std::int32_t>(col[e] - r0));
-                    osrc.push_back(src[e]);
+                if (!tri.chain) {
+                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) {
+                        ocol.push_back(static_cast<std::int32_t>(col[e] - r0));
+                        osrc.push_back(src[e]);
+                    }
+                } else {
+                    // chain layout: entries from rows before this chunk, then the
+                    // in-chunk entries by the step that finishes their row
+                    const std::int64_t cs = (p / 64) * 64;
+                    inside.clear();
+                    std::int32_t outside = 0;
+                    for (std::int64_t e = off[i]; e < off[i + 1]; ++e) {
+                        const std::int64_t c = col[e] - r0;
+                        const bool in_chunk = dir == 0 ? c >= cs : c <= nb - 1 - cs;
+                        if (in_chunk) {
+                            inside.push_back({static_cast<int>(dir == 0 ? c - cs : nb - 1 - cs - c), e});
+                        } else {
+                            ocol.push_back(static_cast<std::int32_t>(c));
+                            osrc.push_back(src[e]);
+                            ++outside;
+                        }
+                    }
+                    std::sort(inside.begin(), inside.end());
+                    for (const auto& [step, e] : inside) {
+                        ocol.push_back(static_cast<std::int32_t>(col[e] - r0));
+                        osrc.push_back(src[e]);
+                    }
+                    onout[r0 + p] = outside;
+                    tri.ei_max = std::max<int>(tri.ei_max, static_cast<int>(inside.size()));
                 }
                 ptr[r0 + p + 1] = static_cast<std::int64_t>(ocol.size());
             }
@@ -622,6 +642,8 @@ void build_schedule(owtk_precond pc, const TriHost& t)
     upload(tri.frow, frow);
     upload(tri.brow, brow);
     upload(tri.frow_lvl, frow_lvl);
+    upload(tri.fnout, fnout);
+    upload(tri.bnout, bnout);
     upload(tri.fptr, fptr);
     upload(tri.bptr, bptr);
     upload(tri.fcol, fcol);
@@ -634,7 +656,7 @@ TriArgs tri_args(const TriSchedule& t)
 {
     return TriArgs{t.row0.get(), t.nrows.get(), t.flev.get(), t.blev.get(), t.flev_pos.get(), t.blev_pos.get(),
                    t.frow.get(), t.brow.get(), t.fptr.get(), t.bptr.get(), t.fcol.get(), t.bcol.get(),
-                   t.fnnz, t.bnnz, t.rows, t.bs};
+                   t.fnout.get(), t.bnout.get(), t.fnnz, t.bnnz, t.rows, t.bs};
 }
 
 std::size_t lds_limit(owtk_precond pc)
@@ -966,12 +988,15 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
     ProfileScope prof(pc->m->ctx, sizeof(X) == 8 ? prof_precond_f64 : prof_precond_low, precond_apply_bytes(pc));
     if (t.chain) {
         const TriArgs args = tri_args(t);
-        if (t.emax <= 4)
-            k_tri_chain<MODE, X, V, 4><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
-        else if (t.emax <= 8)
-            k_tri_chain<MODE, X, V, 8><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        const std::size_t chain_lds = need + 64 * 64 * sizeof(X);
+        if (t.ei_max <= 2)
+            k_tri_chain<MODE, X, V, 2><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        else if (t.ei_max <= 4)
+            k_tri_chain<MODE, X, V, 4><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        else if (t.ei_max <= 8)
+            k_tri_chain<MODE, X, V, 8><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
         else
-            k_tri_chain<MODE, X, V, 16><<<groups, 64, need, st>>>(args, fv, bv, dv, in, out, scale, stop);
+            k_tri_chain<MODE, X, V, 16><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
     } else if (need <= lds_limit(pc)) {
         k_tri_apply<MODE, X, V, true><<<groups, threads, need, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
     } else {

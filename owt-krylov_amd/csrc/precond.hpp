@@ -33,6 +33,8 @@ struct TriSchedule {
     DeviceArray<std::int32_t> frow_lvl;  // forward level order (numeric factorization)
     bool chain = false;  // one-wave chain sweeps instead of level barriers
     int emax = 0;        // most entries of one row in one direction
+    int ei_max = 0;      // chain layout: most in-chunk entries of one row
+    DeviceArray<std::int32_t> fnout, bnout;  // chain layout: entries before the chunk
     // entry pointers per position (global), block-local columns
     DeviceArray<std::int64_t> fptr, bptr;
     DeviceArray<std::int32_t> fcol, bcol;

@@ -143,6 +143,15 @@ def test_norm_extremes(ctx):
 
 
 # ---------------------------------------------------------- preconditioners
+def assert_parity(got, ref, schedule, tol=1e-14):
+    """Level sweeps reproduce the oracle bit for bit; chain sweeps accumulate
+    backward entries in the order they become available (a few ulp)."""
+    if schedule == 1:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
+
+
 def gpu_apply(ctx, pc, r, dtype=np.float64):
     rd = ctx.upload(np.asarray(r, dtype))
     od = ctx.empty(rd.count, dtype)
@@ -194,7 +203,7 @@ def test_ssor_c0_bitwise(ctx, c0, omega, block_nodes, schedule):
     r = np.random.default_rng(11).uniform(-1, 1, s.nodes * 3)
     ref = orc.OPreconditioner(om, 2, omega=omega, block_nodes=block_nodes).apply(r)
     pc = owtk.Preconditioner(m, owtk.PC_SSOR, omega=omega, block_nodes=block_nodes, schedule=schedule)
-    assert np.array_equal(gpu_apply(ctx, pc, r), ref)
+    assert_parity(gpu_apply(ctx, pc, r), ref, schedule)
 
 
 @pytest.mark.parametrize("schedule", [1, 2])
@@ -210,13 +219,13 @@ def test_ilu_audit_factors_bitwise(ctx, fill, schedule):
         for a, b in zip(pc.ilu_factors(), opc.ilu_factors()):
             assert np.array_equal(a, b)
         r = np.random.default_rng(12).uniform(-1, 1, (n + 1) * comps)
-        assert np.array_equal(gpu_apply(ctx, pc, r)[:n * comps], opc.apply(r))
+        assert_parity(gpu_apply(ctx, pc, r)[:n * comps], opc.apply(r), schedule)
 
 
 def test_ilu_numeric_update_reuses_symbolic(ctx, c0):
     s = c0
     m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
-    pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=256)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=256, schedule=1)
     d2, e2 = s.diagonal * 1.1, s.off_diagonal * 0.9
     m.update_values(d2, e2)
     pc.update_values()
@@ -249,7 +258,7 @@ def test_ilu_block_jacobi_medium_bitwise(ctx, schedule):
     pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=4096, schedule=schedule)
     opc = orc.OPreconditioner(om, 3, block_nodes=4096)
     r = np.random.default_rng(14).uniform(-1, 1, s.nodes * 3)
-    assert np.array_equal(gpu_apply(ctx, pc, r), opc.apply(r))
+    assert_parity(gpu_apply(ctx, pc, r), opc.apply(r), schedule)
     info = pc.info()
     assert info.blocks == 32 and info.forward_levels > 1
 
@@ -274,4 +283,4 @@ def test_chain_and_level_agree_low_precision(ctx, c0, storage):
     for kind in (owtk.PC_ILU, owtk.PC_SSOR):
         a = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=1)
         b = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=2)
-        assert np.array_equal(gpu_apply(ctx, a, r, np.float32), gpu_apply(ctx, b, r, np.float32))
+        assert_parity(gpu_apply(ctx, b, r, np.float32), gpu_apply(ctx, a, r, np.float32), 2, tol=1e-6)
