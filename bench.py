@@ -1,9 +1,10 @@
 """Headline benchmark: time to true relative residual 1e-8 on the 2M-vertex
-block-3 grid, mixed-precision FGMRES vs plain FP64 BiCGSTAB + block-Jacobi
-ILU(0), plus the split SpMV's HBM bandwidth.
+block-3 grid, learned-preconditioner + mixed-precision FGMRES vs plain FP64
+BiCGSTAB + block-Jacobi ILU(0), plus the split SpMV's HBM bandwidth.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-    python bench.py --kernel spmv|copy --steps K     (single-kernel loops for rocprofv3 counters)
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32|bf16]
+    python bench.py --kernel spmv|copy|learned --steps K   (kernel loops for rocprofv3 counters)
+    python bench.py --workload batched                   (512 uneven systems, batched BiCGSTAB)
 
 One "step" is one full solve of the headline method from x0 = 0 on the
 resident 2048 x 1024 system; the FP64 baseline solve is timed the same way in
@@ -31,7 +32,18 @@ METRIC = "ms to true rel. residual 1e-8: learned+mixed vs plain FP64; SpMV HBM G
 PEAK_HBM_GBPS = 8000.0
 NX, NY, BS, SEED = 2048, 1024, 3, 0
 BLOCK_NODES = 4096
+BATCH_SYSTEMS = 512
+BATCH_GRAPH_ITERATIONS = 8
 RTOL = 1e-8
+MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
+KERNEL_NAMES = {
+    "spmv_f64": "split SpMV, FP64 (spmv_sell_cd<3,double,double,true,EpiStore>)",
+    "spmv_f32": "split SpMV, FP32 storage (spmv_sell_cd<3,float,float,...>)",
+    "spmv_bf16": "split SpMV, BF16 storage (spmv_sell_cd<3,bf16,float,...>)",
+    "precond_f64": "block-Jacobi ILU(0) triangular sweeps, FP64 (k_tri_chain)",
+    "precond_low": "block-Jacobi ILU(0) triangular sweeps, low precision (k_tri_chain)",
+    "learned": "learned series step: off-diagonal SpMV + fused update (spmv_sell_cd<3,V,float,false,EpiPoly>)",
+}
 
 
 def log(*a):
@@ -82,11 +94,11 @@ def timed_solves(ctx, label, steps, warmup, run, x):
         times.append((time.perf_counter() - t0) * 1e3)
         results.append(res)
         log(f"[bench] {label} step {k + 1}/{steps}: {times[-1]:.2f} ms, it={res['iterations']}, "
-            f"status={res['status_name']}")
+            f"status={res['status_name']}, fallback steps={res.get('fallback_refinements', 0)}")
     ctx.profile(False)
     prof = {name: ctx.profile_read(cat) for name, cat in (
-        ("spmv_f64", ctx.PROF_SPMV_F64), ("spmv_f32", ctx.PROF_SPMV_F32), ("precond_f64", ctx.PROF_PRECOND_F64),
-        ("precond_low", ctx.PROF_PRECOND_LOW))}
+        ("spmv_f64", ctx.PROF_SPMV_F64), ("spmv_f32", ctx.PROF_SPMV_F32), ("spmv_bf16", ctx.PROF_SPMV_BF16),
+        ("precond_f64", ctx.PROF_PRECOND_F64), ("precond_low", ctx.PROF_PRECOND_LOW), ("learned", ctx.PROF_LEARNED))}
     return times, results, prof
 
 
@@ -142,17 +154,26 @@ def cpu_baseline(s, gpu_iterations):
 
 
 def kernel_loop(kind, steps):
-    """Single-kernel loop (rocprofv3 counter runs): FP64 split SpMV or a copy
-    kernel of known size (FP64 -> FP32 conversion: 8 B read, 4 B written)."""
+    """Kernel loops for rocprofv3 counter runs: FP64 split SpMV, the learned
+    preconditioner apply (FP32 storage), or a copy kernel of known size
+    (FP64 -> FP32 conversion: 8 B read, 4 B written)."""
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
         n = s.nodes * BS
         x = ctx.upload(np.random.default_rng(0).uniform(-1, 1, n))
         y = ctx.empty(n)
         yf = ctx.empty(n, np.float32)
+        pc = None
+        if kind == "learned":
+            m.add_storage(owtk.F32)
+            model = owtk.GnnModel.load(ctx, MODEL)
+            pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=owtk.F32)
+            xf = ctx.upload(np.random.default_rng(0).uniform(-1, 1, n).astype(np.float32))
         for k in range(steps):
             if kind == "spmv":
                 m.spmv(x, y)
+            elif kind == "learned":
+                pc.apply(xf, yf)
             else:
                 owtk.check(owtk.lib().owtk_convert(ctx.handle, n, owtk.F32, yf.ptr, owtk.F64, x.ptr))
         ctx.synchronize()
@@ -161,13 +182,103 @@ def kernel_loop(kind, steps):
                           "spmv_bytes_f64": info.spmv_bytes_f64, "copy_bytes": 12 * n}))
 
 
+def batched_bench(args):
+    """512 independent block-3 systems, BiCGSTAB + Jacobi to rtol 1e-8 in one
+    graph-replayed batch; systems/s and per-system stop times."""
+    from owtk import workloads
+
+    t0 = time.perf_counter()
+    systems, rhs, grids = workloads.batched_systems(BATCH_SYSTEMS)
+    vertices = sum(s[0] for s in systems)
+    log(f"[bench] batched: {len(systems)} systems, {vertices} vertices, generated in {time.perf_counter() - t0:.1f}s")
+    opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=20000)
+    with owtk.Context(0) as ctx:
+        t0 = time.perf_counter()
+        bt = owtk.Batch(ctx, BS, systems)
+        ctx.synchronize()
+        log(f"[bench] batch upload {time.perf_counter() - t0:.1f}s")
+        b = ctx.upload(rhs)
+        x = ctx.empty(rhs.size)
+        tile, ms_it = bt.autotune(b, iterations=8)
+        log(f"[bench] autotuner: tile {tile} rows, {ms_it:.3f} ms per all-active iteration")
+        for _ in range(args.warmup):
+            x.zero()
+            bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+        times, last = [], None
+        for k in range(args.steps):
+            x.zero()
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            last = bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+            ctx.synchronize()
+            times.append((time.perf_counter() - t1) * 1e3)
+            log(f"[bench] batched solve {k + 1}/{args.steps}: {times[-1]:.1f} ms")
+        ms = float(np.median(times))
+        stop = np.array([r["stop_ms"] for r in last])
+        its = np.array([r["iterations"] for r in last])
+        converged = sum(r["status_name"] == "converged" for r in last)
+        worst = max(r["relative_residual_norm"] for r in last)
+        # the same systems one at a time through the single-system solver
+        t1 = time.perf_counter()
+        seq_its, off = [], 0
+        for (n, o, c, v) in systems:
+            m = owtk.Matrix.csr(ctx, n, 0, BS, o, c, v)
+            pc = owtk.Preconditioner(m, owtk.PC_JACOBI)
+            bs_ = ctx.upload(rhs[off:off + n * BS])
+            xs = ctx.empty(n * BS)
+            seq_its.append(owtk.bicgstab(m, pc, bs_, xs, opts)["iterations"])
+            off += n * BS
+        ctx.synchronize()
+        seq_ms = (time.perf_counter() - t1) * 1e3
+        cpu = None
+        if not args.no_cpu:
+            sys.path.insert(0, str(ROOT))
+            from oracle import oracle as orc
+            order = np.argsort([s[0] for s in systems])
+            sample = [int(i) for i in order[::max(1, len(order) // 24)]][:24]
+            offs = np.concatenate([[0], np.cumsum([s[0] * BS for s in systems])])
+            t1 = time.perf_counter()
+            for i in sample:
+                n, o, c, v = systems[i]
+                om = orc.OMatrix.csr(n, 0, BS, o, c, v)
+                opc = orc.OPreconditioner(om, orc.OPreconditioner.JACOBI)
+                orc.bicgstab(om, opc, rhs[offs[i]:offs[i + 1]], relative_tolerance=RTOL, maximum_iterations=20000)
+            cpu_s = time.perf_counter() - t1
+            frac = sum(systems[i][0] for i in sample) / vertices
+            cpu = {"value": round(len(sample) / cpu_s * frac * len(systems) / len(sample), 3), "unit": "systems/s",
+                   "cores": 1, "kind": "port",
+                   "sample": f"oracle BiCGSTAB + Jacobi on {len(sample)} of the systems spread over the size range "
+                             f"({cpu_s:.1f}s), scaled by their share ({frac:.3f}) of all vertices"}
+        out = {
+            "metric": "batched uneven systems: systems/s (BiCGSTAB + Jacobi to rtol 1e-8)", "value": round(
+                len(systems) / (ms / 1e3), 2), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{len(systems)} systems, grids log-uniform 32x16..512x256, block 3, "
+                                   f"edge-contrast sigma U(0,1.5), seeds 3000+, {vertices} vertices in all",
+                       "tile_rows": tile, "graph_iterations": BATCH_GRAPH_ITERATIONS},
+            "per_system_ms": {"p50": float(np.percentile(stop, 50)), "p99": float(np.percentile(stop, 99)),
+                              "max": float(stop.max())},
+            "iterations": {"p50": float(np.median(its)), "max": int(its.max())},
+            "converged": int(converged), "max_true_rel": worst,
+            "sequential_single_system_ms": round(seq_ms, 1),
+            "sequential_systems_per_s": round(len(systems) / (seq_ms / 1e3), 2),
+            "iteration_count_max_diff_vs_single": int(np.max(np.abs(np.array(seq_its) - its))),
+            "autotune_ms_per_iteration": ms_it, "cpu_baseline": cpu,
+        }
+        print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["spmv", "copy"], default=None)
+    ap.add_argument("--kernel", choices=["spmv", "copy", "learned"], default=None)
+    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32")
+    ap.add_argument("--workload", choices=["headline", "batched"], default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -176,6 +287,8 @@ def main():
     if args.kernel:
         kernel_loop(args.kernel, args.steps)
         return 0
+    if args.workload == "batched":
+        return batched_bench(args)
 
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
@@ -184,18 +297,27 @@ def main():
         x = ctx.empty(n)
         r = ctx.empty(n)
         bnorm = ctx.norm(b)
+        storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.storage]
 
+        setup = {}
         t0 = time.perf_counter()
         ilu64 = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=BLOCK_NODES)
         ctx.synchronize()
-        setup64 = time.perf_counter() - t0
+        setup["ilu0_fp64_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        ilu32 = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=BLOCK_NODES, storage=owtk.F32)
+        ilu_low = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=BLOCK_NODES, storage=storage)
         ctx.synchronize()
-        setup32 = time.perf_counter() - t0
+        setup["ilu0_low_s"] = time.perf_counter() - t0
+        model = owtk.GnnModel.load(ctx, MODEL)
+        t0 = time.perf_counter()
+        learned = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=storage)
+        ctx.synchronize()
+        setup["learned_s"] = time.perf_counter() - t0
+        plain = owtk.Preconditioner(m, owtk.PC_LEARNED, model=owtk.GnnModel(ctx, BS, model.degree), storage=storage)
         info = ilu64.info()
-        log(f"[bench] ILU(0) set-up fp64 {setup64:.2f}s fp32 {setup32:.2f}s, {info.blocks} blocks, "
-            f"levels fwd {info.forward_levels} bwd {info.backward_levels}")
+        log(f"[bench] set-up: ILU(0) fp64 {setup['ilu0_fp64_s']:.2f}s, low {setup['ilu0_low_s']:.2f}s "
+            f"({info.blocks} blocks, levels fwd {info.forward_levels}); learned (network inference) "
+            f"{setup['learned_s'] * 1e3:.1f} ms, degree {model.degree}")
 
         # plain FP64 baseline: BiCGSTAB + block-Jacobi ILU(0)
         opts64 = owtk.SolverOptions(relative_tolerance=RTOL)
@@ -204,14 +326,25 @@ def main():
         rel64 = true_rel(ctx, m, b, x, r, bnorm)
         x64 = x.download()
 
-        # mixed precision: FP64 refinement around FP32-storage FGMRES(30) + FP32 ILU(0)
-        ropts = owtk.refinement_options(inner_storage=owtk.F32, inner_restart=30, inner_relative_tolerance=1e-4,
-                                        relative_tolerance=RTOL)
-        tmx, resmx, profmx = timed_solves(ctx, "mixed fgmres-ir+ilu0(f32)", args.steps, args.warmup,
-                                          lambda: owtk.fgmres_refined(m, ilu32, b, x, ropts), x)
-        relmx = true_rel(ctx, m, b, x, r, bnorm)
-        xmx = x.download()
-        err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+        def refined(label, pc, fallback=None):
+            ropts = owtk.refinement_options(inner_storage=storage, inner_restart=30, inner_relative_tolerance=1e-4,
+                                            relative_tolerance=RTOL)
+            t, res, prof = timed_solves(ctx, label, args.steps, args.warmup,
+                                        lambda: owtk.fgmres_refined(m, pc, b, x, ropts, fallback=fallback), x)
+            rel = true_rel(ctx, m, b, x, r, bnorm)
+            err = float(np.linalg.norm(x.download() - x64) / np.linalg.norm(x64))
+            last = res[-1]
+            return {"ms": round(float(np.median(t)), 3), "ms_min": round(min(t), 3), "ms_max": round(max(t), 3),
+                    "inner_iterations": last["inner_iterations"], "refinements": last["refinement_steps"],
+                    "fallback_refinements": last["fallback_refinements"], "status": last["status_name"],
+                    "true_rel": rel, "error_vs_fp64": err}, prof
+
+        # headline: learned preconditioner + mixed precision (ILU(0) fallback)
+        head, profh = refined(f"learned+mixed ({args.storage})", learned, fallback=ilu_low)
+        # the same method without the network (plain truncated series)
+        noet, _ = refined(f"series+mixed ({args.storage}), no network", plain, fallback=ilu_low)
+        # mixed precision with the fallback preconditioner alone
+        milu, _ = refined(f"ilu0+mixed ({args.storage})", ilu_low)
 
         # split SpMV: the path's data-parallel hot kernel and the metric's "SpMV HBM GB/s"
         sp = spmv_timing(ctx, m, n)
@@ -219,45 +352,47 @@ def main():
             f"f32 {sp['f32']['avg_ms'] * 1e3:.1f} us {sp['f32']['gbps']:.0f} GB/s, "
             f"bf16 {sp['bf16']['avg_ms'] * 1e3:.1f} us {sp['bf16']['gbps']:.0f} GB/s")
         ms64 = float(np.median(t64))
-        msmx = float(np.median(tmx))
-        headline = msmx
+        headline = head["ms"]
+        # the headline solve's dominant profiled kernel
+        kname, kp = max(((k, v) for k, v in profh.items() if v["launches"]), key=lambda kv: kv[1]["total_ms"])
         roof = {
-            "kernel": "split SpMV, FP64 values and vectors (spmv_sell_cd<3,double,double,true,EpiStore>)",
-            "bound": "hbm", "achieved": round(sp["f64"]["gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-            "frac": round(sp["f64"]["gbps"] / PEAK_HBM_GBPS, 4), "traffic": None,
-            "avg_launch_ms": round(sp["f64"]["avg_ms"], 5), "bytes_per_launch": sp["f64"]["bytes"],
+            "kernel": KERNEL_NAMES[kname] + " (largest share of the headline solve)",
+            "bound": "hbm", "achieved": round(kp["gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(kp["gbps"] / PEAK_HBM_GBPS, 4), "traffic": None,
+            "avg_launch_ms": round(kp["avg_ms"], 5), "bytes_per_launch": int(kp["bytes"] // max(kp["launches"], 1)),
+            "launches_per_solve": kp["launches"] // max(args.steps, 1),
+            "share_of_solve_ms": round(kp["total_ms"] / max(args.steps, 1), 3),
         }
-        kern = max(profmx.items(), key=lambda kv: kv[1]["total_ms"])
-        dominant = {"kernel": f"{kern[0]} (largest share of the headline solve)",
-                    "achieved_gbps": round(kern[1]["gbps"], 1), "avg_launch_ms": round(kern[1]["avg_ms"], 5),
-                    "share_ms": round(kern[1]["total_ms"] / max(args.steps, 1), 3)}
+        spmv_roof = {"kernel": KERNEL_NAMES["spmv_f64"] + " (dedicated loop)", "achieved": round(sp["f64"]["gbps"], 1),
+                     "peak": PEAK_HBM_GBPS, "frac": round(sp["f64"]["gbps"] / PEAK_HBM_GBPS, 4),
+                     "avg_launch_ms": round(sp["f64"]["avg_ms"], 5), "bytes_per_launch": sp["f64"]["bytes"]}
         cpu = None
         if not args.no_cpu:
             spmv_s, setup_s, iter_s = cpu_baseline(s, res64[-1]["iterations"])
-            model, cores = cpu_info()
+            model_name, cores = cpu_info()
             est_ms = iter_s * res64[-1]["iterations"] * 1e3
             cpu = {"value": round(est_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
                    "sample": (f"oracle FP64 BiCGSTAB + 4096-block ILU(0) on this grid, 3 timed iterations "
                               f"({iter_s * 1e3:.0f} ms each) x the {res64[-1]['iterations']} iterations the "
                               f"solve needs; ILU set-up {setup_s:.1f}s excluded; split SpMV "
-                              f"{m.info().spmv_bytes_f64 / spmv_s / 1e9:.1f} GB/s; host {model}, "
+                              f"{m.info().spmv_bytes_f64 / spmv_s / 1e9:.1f} GB/s; host {model_name}, "
                               f"{cores} hardware threads")}
         out = {
             "metric": METRIC, "value": round(headline, 3), "unit": "ms", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(headline, 3),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+            "dtype": f"{args.storage} storage, f32 inner arithmetic, f64 refinement", "data": "synthetic",
             "config": {"workload": "2048x1024 triangulated grid, 2,097,152 vertices, block 3, split view, "
-                                   "Hilbert order, seed 0; rtol 1e-8; block-Jacobi ILU(0) on 4096-vertex blocks",
-                       "headline": "FP64-refined FGMRES(30), FP32 matrix + FP32 ILU(0) storage",
-                       "baseline": "FP64 BiCGSTAB + FP64 ILU(0)"},
+                                   "Hilbert order, seed 0; rtol 1e-8 (true FP64 residual)",
+                       "headline": f"FP64-refined FGMRES(30), {args.storage} matrix storage, learned vertex-block "
+                                   f"preconditioner (degree {model.degree}, {MODEL.name}), ILU(0) fallback",
+                       "baseline": "FP64 BiCGSTAB + FP64 block-Jacobi ILU(0), 4096-vertex blocks"},
             "fp64_bicgstab_ilu0_ms": round(ms64, 3), "speedup_vs_fp64": round(ms64 / headline, 3),
             "fp64_iterations": res64[-1]["iterations"], "fp64_true_rel": rel64,
-            "mixed_inner_iterations": resmx[-1]["inner_iterations"],
-            "mixed_refinements": resmx[-1]["refinement_steps"], "mixed_true_rel": relmx,
-            "mixed_error_vs_fp64": err,
-            "spmv": sp, "roofline": roof, "dominant_kernel": dominant, "cpu_baseline": cpu,
-            "kernels": {"fp64": prof64, "mixed": profmx},
+            "learned_mixed": head, "series_mixed_no_network": noet, "ilu0_mixed": milu,
+            "setup_s": {k: round(v, 4) for k, v in setup.items()},
+            "spmv": sp, "spmv_roofline": spmv_roof, "roofline": roof, "cpu_baseline": cpu,
+            "kernels": {"fp64": prof64, "learned_mixed": profh},
         }
         print(json.dumps(out))
     return 0

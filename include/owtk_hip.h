@@ -243,6 +243,7 @@ typedef struct owtk_solver_result {
     /* iterative refinement (owtk_fgmres_refined) only */
     uint64_t refinement_steps;
     uint64_t inner_iterations;
+    uint64_t fallback_refinements; /* refinement steps run with the fallback preconditioner */
 } owtk_solver_result;
 
 /* Defaults of SolverOptions<double>. */
@@ -267,10 +268,103 @@ typedef struct owtk_refinement_options {
     double inner_relative_tolerance;
     uint64_t inner_maximum_iterations;
     uint64_t maximum_refinements;
+    double fallback_ratio;       /* owtk_fgmres_refined_fallback: switch when an outer step
+                                    leaves more than this fraction of the true residual (0.5) */
 } owtk_refinement_options;
 void owtk_refinement_options_default(owtk_refinement_options* options);
 int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, double* x,
                         const owtk_refinement_options* options, owtk_solver_result* result);
+/* The same driver with a fallback inner preconditioner (no reference
+ * counterpart): the true FP64 residual judges every outer step; when a step
+ * with inner_pc leaves more than fallback_ratio of the previous true residual
+ * (or the inner solve breaks down or produces non-finite values), the step is
+ * undone and the remaining steps use fallback_pc (e.g. learned -> block-Jacobi
+ * ILU(0)).  result->fallback_refinements counts the steps that used it. */
+int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_precond fallback_pc,
+                                 const double* b, double* x, const owtk_refinement_options* options,
+                                 owtk_solver_result* result);
+
+/* ------------------------------------------- learned preconditioner (GNN) -- */
+/* No reference counterpart: the reference defers learned preconditioners
+ * (REVIEW_AND_IMPROVEMENT_PLAN.md:377-379).  A vertex-block graph network
+ * (3 message-passing layers, width 32, BF16 weights, FP32 arithmetic) maps a
+ * split matrix to per-vertex, per-component weights gamma_k, k = 0..degree,
+ * of the preconditioner
+ *     u_0 = D^-1 r,  u_(k+1) = -D^-1 A_off u_k,  z = sum_k gamma_k u_k
+ * (OWTK_PC_LEARNED with params.learned_model = the model; storage selects the
+ * matrix values used by the apply).  A fresh model has a zero network and
+ * gamma = 1: the plain truncated Neumann series.
+ * Training: each problem holds a split matrix and P (residual, error) pairs;
+ * the loss is (1/P) sum_p |D^-1/2 A (M r_p - e_p)|^2 / |D^-1/2 A e_p|^2.
+ * Gradients accumulate on the device (owtk_gnn_accumulate) and an Adam step
+ * applies and clears them.  Parameters are a flat FP32 vector in the order
+ * W_in[32][2b+1], b_in[32], then per layer W_s[32][32], W_n[32][32], b[32],
+ * then W_out[(degree+1)b][32], b_out, g[(degree+1)b]. */
+typedef struct owtk_gnn_s* owtk_gnn;
+typedef struct owtk_gnn_problem_s* owtk_gnn_problem;
+
+int owtk_gnn_create(owtk_context ctx, int32_t block_size, int32_t degree, owtk_gnn* out);
+int owtk_gnn_destroy(owtk_gnn model);
+int owtk_gnn_info(owtk_gnn model, int32_t* block_size, int32_t* degree, int64_t* parameter_count);
+/* Host FP32 parameters in and out; setting them resets the Adam state. */
+int owtk_gnn_set_parameters(owtk_gnn model, const float* params);
+int owtk_gnn_get_parameters(owtk_gnn model, float* params);
+/* Network inference on a split matrix: gamma[k][vertex*b + c] to host. */
+int owtk_gnn_coefficients(owtk_gnn model, owtk_matrix m, float* gamma);
+/* residuals/errors: device FP64, pairs x (owned*b).  The pattern must be
+ * structurally symmetric (the backward pass uses the transposed matrix). */
+int owtk_gnn_problem_create(owtk_gnn model, owtk_matrix m, int32_t pairs, const double* residuals,
+                            const double* errors, owtk_gnn_problem* out);
+int owtk_gnn_problem_destroy(owtk_gnn_problem problem);
+int owtk_gnn_zero_grad(owtk_gnn model);
+/* gradient += weight * dL/dtheta, accumulated loss += weight * L (device). */
+int owtk_gnn_accumulate(owtk_gnn model, owtk_gnn_problem problem, double weight);
+/* Accumulated loss and gradient to host (either may be NULL). */
+int owtk_gnn_read(owtk_gnn model, double* loss, float* gradient);
+/* Adam update from the accumulated gradient; returns the accumulated loss
+ * (may be NULL) and clears gradient and loss. */
+int owtk_gnn_adam_step(owtk_gnn model, double learning_rate, double beta1, double beta2, double epsilon,
+                       double* loss);
+
+/* ------------------------------------------------------- batched BiCGSTAB -- */
+/* No reference counterpart (the reference solves one system per call).  Many
+ * independent component-diagonal block-CSR systems (BlockCsrMatrix form,
+ * block_csr.hpp:192-236, diagonal entry included) of uneven size, solved by
+ * BiCGSTAB + Jacobi with the recurrence, breakdown guards, check cadence and
+ * residual replacement of bicgstab (krylov_solvers.hpp:542-745) per system,
+ * all systems in one hipGraph-captured iteration sequence with per-system
+ * convergence masks.  Arrays are the systems' CSR concatenated: system s owns
+ * vertices [vertex_offsets[s], vertex_offsets[s+1]); row_offsets (size
+ * vertices+1) and columns index the concatenation (columns must stay inside
+ * the row's system); values are [entry][component]. */
+typedef struct owtk_batch_s* owtk_batch;
+typedef struct owtk_batch_result {
+    int32_t status;            /* owtk_solver_status */
+    int32_t breakdown_reason;
+    uint64_t iterations;
+    double true_residual_norm; /* last true residual |b - A x| of the system */
+    double rhs_norm;
+    double relative_residual_norm;
+    double stop_ms;            /* ms from the solve's start to the graph replay
+                                  in which the system stopped */
+} owtk_batch_result;
+
+int owtk_batch_create(owtk_context ctx, int32_t systems, int32_t block_size, const int64_t* vertex_offsets,
+                      const int64_t* row_offsets, const int32_t* columns, const double* values,
+                      owtk_batch* out);
+int owtk_batch_destroy(owtk_batch batch);
+/* Rows per workgroup tile: 64, 128, 256 or 512 (no tile spans two systems). */
+int owtk_batch_set_tile(owtk_batch batch, int32_t tile_rows);
+/* Times `iterations` iterations with every system active for each tile size
+ * and keeps the fastest; returns it and its ms per iteration. */
+int owtk_batch_autotune(owtk_batch batch, const double* b, int32_t iterations, int32_t* tile_rows,
+                        double* ms_per_iteration);
+/* b, x: device FP64 over all systems' unknowns (x holds the initial guesses).
+ * graph_iterations: iterations per captured graph (the host checks the
+ * number of running systems once per replay).  results: host array of
+ * `systems` entries, may be NULL. */
+int owtk_batch_bicgstab(owtk_batch batch, const double* b, double* x, const owtk_solver_options* options,
+                        int32_t graph_iterations, owtk_batch_result* results);
 
 /* --------------------------------------------------------- vector kernels -- */
 /* dot with FP64 accumulation (SerialReduction::dot, reduction.hpp:114-145). */

@@ -1,0 +1,938 @@
+// Batched BiCGSTAB + Jacobi over many independent component-diagonal systems
+// of uneven size.  No reference counterpart (the reference solves one system
+// per call); each system follows the recurrence, breakdown guards, check
+// cadence and residual replacement of bicgstab (krylov_solvers.hpp:542-745)
+// exactly, with right Jacobi preconditioning (preconditioner.hpp:25-117).
+//
+// Layout.  The systems' rows are concatenated; each system's rows are padded
+// to a multiple of 64 in a SELL-64 matrix (slot (s, k, lane) at
+// slice_ptr[s] + 64k + lane, values [slot & ~63][c][lane]) so that a wave's
+// k-th entries are one contiguous load and no slice straddles two systems.
+// Vectors stay unpadded.  A tile of T rows (T = 64..512, one row per thread)
+// never straddles two systems either: tile -> system is a table, so a block
+// reads its system's mask first and returns at once once that system stopped.
+//
+// Iteration.  Eleven launches: 5 streaming kernels (update p, v = A M p with
+// <r^,v>, s = r - alpha v with |s|^2, t = A M s with <t,s> and <t,t>, update
+// x and r with <r^,r> and |r|^2) each followed by a per-system finisher (one
+// wave per system summing that system's tile partials in a fixed order:
+// deterministic), plus the masked true-residual kernel of the convergence and
+// replacement checks.  G iterations are captured once into a hipGraph and the
+// graph is replayed until every system has stopped; the host reads one
+// counter per replay.  The tile size is chosen by owtk_batch_autotune.
+#include "internal.hpp"
+#include "solver_common.hpp"
+#include "kernels/blas1.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+using namespace owtk;
+
+namespace {
+
+constexpr int kMaxTile = 512;
+constexpr int kPartials = 3;
+
+// per-system state
+struct SysState {
+    double rho, rho_prev, alpha, omega, beta;
+    double thr, bnorm, snorm, tiny, breakdown_tolerance;
+    double rr, true_norm;
+    int phase;        // 1 running, 0 stopped
+    int first;        // next iteration starts the recurrence (p = r)
+    int restart;      // next iteration also resets r^ = r
+    int early;        // |s| <= threshold this iteration
+    int need_true;    // true residual this iteration
+    int final;        // iteration limit reached
+    int iters, status, reason;
+    int check_interval, replace_interval, max_iterations;
+};
+
+struct Tiling {
+    int tile = 0;
+    std::int64_t blocks = 0;
+    DeviceArray<std::int32_t> blk_sys;     // tile -> system
+    DeviceArray<std::int64_t> blk_row0;    // tile -> first padded row
+    DeviceArray<std::int32_t> sys_blk0;    // system -> first tile (size systems + 1)
+    DeviceArray<double> partials;          // [kPartials][blocks]
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_iterations = 0;
+    const double* graph_b = nullptr;  // the graph's kernels bind b and x
+    double* graph_x = nullptr;
+    ~Tiling()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+} // namespace
+
+struct owtk_batch_s {
+    owtk_context ctx = nullptr;
+    int systems = 0;
+    int bs = 1;
+    std::int64_t vertices = 0;
+    std::vector<std::int64_t> h_vprefix;   // unpadded vertex offset per system
+    std::vector<std::int64_t> h_pprefix;   // padded row offset per system
+    std::int64_t padded_rows = 0;
+    DeviceArray<std::int64_t> vprefix, pprefix;
+    DeviceArray<std::int32_t> slice_ptr, row_len, cols;
+    DeviceArray<double> vals, dinv;
+    std::int64_t entries = 0;
+    // work vectors (unpadded)
+    DeviceArray<double> r, rhat, p, v, s, t, phat, shat;
+    DeviceArray<SysState> state;
+    DeviceArray<int> active_count;
+    std::unique_ptr<Tiling> tiling;
+    int tile = 256;
+};
+
+namespace {
+
+struct Ptrs {
+    const std::int32_t* blk_sys;
+    const std::int64_t* blk_row0;
+    const std::int64_t* vprefix;
+    const std::int64_t* pprefix;
+    const std::int32_t* slice_ptr;
+    const std::int32_t* row_len;
+    const std::int32_t* cols;
+    const double* vals;
+    const double* dinv;
+    SysState* st;
+    double* partials;
+    std::int64_t blocks;
+    int bs;
+};
+
+// Row of this thread: system, unpadded vertex (or -1 past the system's end)
+// and padded row.  Returns false when the system has stopped.
+struct RowRef {
+    int sys;
+    std::int64_t vertex;  // -1: padding
+    std::int64_t prow;
+};
+
+__device__ inline RowRef row_of(const Ptrs& P)
+{
+    RowRef rr;
+    rr.sys = P.blk_sys[blockIdx.x];
+    rr.prow = P.blk_row0[blockIdx.x] + threadIdx.x;
+    const std::int64_t local = rr.prow - P.pprefix[rr.sys];
+    const std::int64_t n = P.vprefix[rr.sys + 1] - P.vprefix[rr.sys];
+    rr.vertex = local < n ? P.vprefix[rr.sys] + local : -1;
+    return rr;
+}
+
+// y_c = sum over the row's entries (diagonal included) of a_c x_col,c
+template<int BS>
+__device__ inline void row_spmv(const Ptrs& P, std::int64_t prow, const double* __restrict__ x, double (&y)[BS])
+{
+    const std::int64_t slice = prow >> 6;
+    const int lane = static_cast<int>(prow & 63);
+    const std::int64_t base = P.slice_ptr[slice];
+    const int len = P.row_len[prow];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) y[c] = 0.0;
+    for (int k = 0; k < len; ++k) {
+        const std::int64_t run = base + 64 * k;
+        const std::int64_t col = P.cols[run + lane];
+        const double* vk = P.vals + run * BS + lane;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) y[c] = __builtin_fma(vk[64 * c], x[col * BS + c], y[c]);
+    }
+}
+
+// Block sum of K values; thread 0 writes partials[k][blockIdx.x].
+template<int K>
+__device__ inline void store_tile_partials(double (&v)[K], const Ptrs& P)
+{
+    __shared__ double lds[kMaxTile / 64][K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = dev::wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int waves = static_cast<int>(blockDim.x >> 6);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double acc = lds[0][k];
+            for (int w = 1; w < waves; ++w) acc += lds[w][k];
+            P.partials[k * P.blocks + blockIdx.x] = acc;
+        }
+    }
+}
+
+// ------------------------------------------------------ streaming kernels --
+
+// Initial residual: r = b - A x, r^ = r, p = v = 0; partials |r|^2, |b|^2.
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_init(Ptrs P, const double* __restrict__ b, const double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ rhat,
+                                                    double* __restrict__ p, double* __restrict__ v)
+{
+    const RowRef rr = row_of(P);
+    double red[2] = {0.0, 0.0};
+    if (rr.vertex >= 0) {
+        double ax[BS];
+        row_spmv<BS>(P, rr.prow, x, ax);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            const double ri = b[i] - ax[c];
+            r[i] = ri;
+            rhat[i] = ri;
+            p[i] = 0.0;
+            v[i] = 0.0;
+            red[0] = __builtin_fma(ri, ri, red[0]);
+            red[1] = __builtin_fma(b[i], b[i], red[1]);
+        }
+    }
+    store_tile_partials<2>(red, P);
+}
+
+// p = r (first) or r + beta (p - omega v); r^ = r on restart; phat = M p.
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_update_p(Ptrs P, const double* __restrict__ r, double* __restrict__ rhat,
+                                                        double* __restrict__ p, const double* __restrict__ v,
+                                                        double* __restrict__ phat)
+{
+    const RowRef rr = row_of(P);
+    const SysState& st = P.st[rr.sys];
+    if (st.phase != 1 || rr.vertex < 0) return;
+    const bool first = st.first != 0;
+    const bool restart = st.restart != 0;
+    const double beta = st.beta, omega = st.omega;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+        const std::int64_t i = rr.vertex * BS + c;
+        const double ri = r[i];
+        if (restart) rhat[i] = ri;
+        const double pi = first ? ri : ri + beta * (p[i] - omega * v[i]);
+        p[i] = pi;
+        phat[i] = P.dinv[i] * pi;
+    }
+}
+
+// v = A phat; partial <r^, v>
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_spmv_v(Ptrs P, const double* __restrict__ phat,
+                                                      const double* __restrict__ rhat, double* __restrict__ v)
+{
+    const RowRef rr = row_of(P);
+    if (P.st[rr.sys].phase != 1) return;
+    double red[1] = {0.0};
+    if (rr.vertex >= 0) {
+        double y[BS];
+        row_spmv<BS>(P, rr.prow, phat, y);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            v[i] = y[c];
+            red[0] = __builtin_fma(rhat[i], y[c], red[0]);
+        }
+    }
+    store_tile_partials<1>(red, P);
+}
+
+// s = r - alpha v; shat = M s; partial |s|^2
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_update_s(Ptrs P, const double* __restrict__ r, const double* __restrict__ v,
+                                                        double* __restrict__ s, double* __restrict__ shat)
+{
+    const RowRef rr = row_of(P);
+    const SysState& st = P.st[rr.sys];
+    if (st.phase != 1) return;
+    const double alpha = st.alpha;
+    double red[1] = {0.0};
+    if (rr.vertex >= 0) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            const double si = r[i] - alpha * v[i];
+            s[i] = si;
+            shat[i] = P.dinv[i] * si;
+            red[0] = __builtin_fma(si, si, red[0]);
+        }
+    }
+    store_tile_partials<1>(red, P);
+}
+
+// t = A shat; partials <t, s>, <t, t>.  Early exit: x += alpha phat only.
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_spmv_t(Ptrs P, const double* __restrict__ shat,
+                                                      const double* __restrict__ s, double* __restrict__ t,
+                                                      const double* __restrict__ phat, double* __restrict__ x)
+{
+    const RowRef rr = row_of(P);
+    const SysState& st = P.st[rr.sys];
+    if (st.phase != 1) return;
+    if (st.early) {
+        if (rr.vertex >= 0) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const std::int64_t i = rr.vertex * BS + c;
+                x[i] = __builtin_fma(st.alpha, phat[i], x[i]);
+            }
+        }
+        return;
+    }
+    double red[2] = {0.0, 0.0};
+    if (rr.vertex >= 0) {
+        double y[BS];
+        row_spmv<BS>(P, rr.prow, shat, y);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            t[i] = y[c];
+            red[0] = __builtin_fma(y[c], s[i], red[0]);
+            red[1] = __builtin_fma(y[c], y[c], red[1]);
+        }
+    }
+    store_tile_partials<2>(red, P);
+}
+
+// x += alpha phat + omega shat; r = s - omega t; partials <r^, r>, |r|^2
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_update_x(Ptrs P, const double* __restrict__ phat,
+                                                        const double* __restrict__ shat, const double* __restrict__ s,
+                                                        const double* __restrict__ t, const double* __restrict__ rhat,
+                                                        double* __restrict__ x, double* __restrict__ r)
+{
+    const RowRef rr = row_of(P);
+    const SysState& st = P.st[rr.sys];
+    if (st.phase != 1 || st.early) return;
+    const double alpha = st.alpha, omega = st.omega;
+    double red[2] = {0.0, 0.0};
+    if (rr.vertex >= 0) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            x[i] = __builtin_fma(omega, shat[i], __builtin_fma(alpha, phat[i], x[i]));
+            const double ri = s[i] - omega * t[i];
+            r[i] = ri;
+            red[0] = __builtin_fma(rhat[i], ri, red[0]);
+            red[1] = __builtin_fma(ri, ri, red[1]);
+        }
+    }
+    store_tile_partials<2>(red, P);
+}
+
+// True residual of the systems that asked for it: r = b - A x; partial |r|^2.
+template<int BS>
+__global__ __launch_bounds__(kMaxTile) void kb_true_residual(Ptrs P, const double* __restrict__ b,
+                                                             const double* __restrict__ x, double* __restrict__ r)
+{
+    const RowRef rr = row_of(P);
+    const SysState& st = P.st[rr.sys];
+    if (st.phase != 1 || !st.need_true) return;
+    double red[1] = {0.0};
+    if (rr.vertex >= 0) {
+        double ax[BS];
+        row_spmv<BS>(P, rr.prow, x, ax);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            const double ri = b[i] - ax[c];
+            r[i] = ri;
+            red[0] = __builtin_fma(ri, ri, red[0]);
+        }
+    }
+    store_tile_partials<1>(red, P);
+}
+
+// ------------------------------------------------------------- finishers --
+// One wave per system: sum the system's tile partials in a fixed order.
+
+struct Sums {
+    double v[kPartials];
+};
+
+template<int K>
+__device__ inline Sums system_sums(const Ptrs& P, const std::int32_t* sys_blk0, int sys)
+{
+    Sums out;
+    const int lane = threadIdx.x;
+    const std::int32_t b0 = sys_blk0[sys], b1 = sys_blk0[sys + 1];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (std::int32_t b = b0 + lane; b < b1; b += 64) acc += P.partials[k * P.blocks + b];
+        out.v[k] = dev::wave_sum(acc);
+    }
+    return out;
+}
+
+enum Stage : int { fin_init = 0, fin_alpha = 1, fin_s = 2, fin_omega = 3, fin_rho = 4, fin_true = 5 };
+
+__device__ inline void stop(SysState& st, int status, int reason)
+{
+    st.phase = 0;
+    st.status = status;
+    st.reason = reason;
+}
+
+// Breakdown check of rho at the start of iteration `it` (krylov_solvers.hpp:611-620).
+__device__ inline bool rho_ok(SysState& st, double rho, int it)
+{
+    if (!__builtin_isfinite(rho) || __builtin_fabs(rho) <= st.tiny) {
+        st.iters = it;
+        stop(st, status_breakdown, __builtin_isfinite(rho) ? reason_biorthogonality_loss : reason_non_finite_scalar);
+        return false;
+    }
+    return true;
+}
+
+__device__ inline void restart_recurrence(SysState& st)
+{
+    st.first = 1;
+    st.restart = 1;
+    st.rho_prev = 1.0;
+    st.alpha = 1.0;
+    st.omega = 1.0;
+}
+
+template<int STAGE>
+__global__ __launch_bounds__(64) void kb_finish(Ptrs P, const std::int32_t* __restrict__ sys_blk0)
+{
+    const int sys = blockIdx.x;
+    SysState& st = P.st[sys];
+    if (STAGE != fin_init && st.phase != 1) return;
+    if (STAGE == fin_true && !st.need_true) return;
+    constexpr int K = (STAGE == fin_init || STAGE == fin_omega || STAGE == fin_rho) ? 2 : 1;
+    if (STAGE == fin_omega && st.early) return;
+    if (STAGE == fin_rho && st.early) {
+        if (threadIdx.x == 0) st.need_true = 1;
+        return;
+    }
+    const Sums sm = system_sums<K>(P, sys_blk0, sys);
+    if (threadIdx.x != 0) return;
+    if constexpr (STAGE == fin_init) {
+        const double rn = __builtin_sqrt(sm.v[0]);
+        st.bnorm = __builtin_sqrt(sm.v[1]);
+        st.thr = fmax(st.thr, st.rr * st.bnorm);  // thr holds atol, rr holds rtol on entry
+        st.rr = rn;
+        st.true_norm = rn;
+        st.iters = 0;
+        st.early = st.need_true = st.final = st.restart = 0;
+        st.first = 1;
+        st.rho_prev = st.alpha = st.omega = 1.0;
+        st.beta = 0.0;
+        if (!__builtin_isfinite(rn) || !__builtin_isfinite(st.thr)) {
+            stop(st, status_breakdown, reason_non_finite_scalar);
+        } else if (rn <= st.thr) {
+            stop(st, status_converged, reason_none);
+        } else {
+            st.phase = 1;
+            st.status = status_maximum_iterations;
+            st.reason = reason_none;
+            st.rho = sm.v[0];  // <r^, r> with r^ = r
+            rho_ok(st, st.rho, 1);
+        }
+    } else if constexpr (STAGE == fin_alpha) {
+        st.iters += 1;
+        st.first = 0;
+        st.restart = 0;
+        const double sv = sm.v[0];
+        if (!__builtin_isfinite(sv) || __builtin_fabs(sv) <= st.tiny) {
+            stop(st, status_breakdown, __builtin_isfinite(sv) ? reason_alpha_denominator : reason_non_finite_scalar);
+            return;
+        }
+        st.alpha = st.rho / sv;
+    } else if constexpr (STAGE == fin_s) {
+        st.snorm = __builtin_sqrt(sm.v[0]);
+        st.early = st.snorm <= st.thr ? 1 : 0;
+    } else if constexpr (STAGE == fin_omega) {
+        const double num = sm.v[0], den = sm.v[1];
+        if (!__builtin_isfinite(den) || __builtin_fabs(den) <= st.tiny) {
+            stop(st, status_breakdown, __builtin_isfinite(den) ? reason_omega_denominator : reason_non_finite_scalar);
+            return;
+        }
+        const double omega = num / den;
+        const double angle = __builtin_fabs(num) / __builtin_sqrt(den) / st.snorm;
+        if (!__builtin_isfinite(omega) || omega == 0.0 || angle <= st.breakdown_tolerance) {
+            stop(st, status_breakdown, __builtin_isfinite(omega) ? reason_omega_zero : reason_non_finite_scalar);
+            return;
+        }
+        st.omega = omega;
+    } else if constexpr (STAGE == fin_rho) {
+        const int it = st.iters;
+        const double rho_new = sm.v[0];
+        const double rec = __builtin_sqrt(sm.v[1]);
+        st.rr = rec;
+        const bool replace = st.replace_interval > 0 && it % st.replace_interval == 0;
+        const bool check = st.check_interval > 0 && it % st.check_interval == 0;
+        if (replace || (check && rec <= st.thr)) {
+            st.need_true = 1;
+            return;
+        }
+        if (it >= st.max_iterations) {
+            st.need_true = 1;
+            st.final = 1;
+            return;
+        }
+        st.rho_prev = st.rho;
+        if (!rho_ok(st, rho_new, it + 1)) return;
+        if (__builtin_fabs(st.omega) <= st.tiny) {
+            st.iters = it + 1;
+            stop(st, status_breakdown, reason_omega_zero);
+            return;
+        }
+        st.beta = (rho_new / st.rho_prev) * (st.alpha / st.omega);
+        st.rho = rho_new;
+    } else {  // fin_true
+        const double tn = __builtin_sqrt(sm.v[0]);
+        st.true_norm = tn;
+        st.need_true = 0;
+        st.early = 0;
+        if (!__builtin_isfinite(tn)) {
+            stop(st, status_breakdown, reason_non_finite_scalar);
+            return;
+        }
+        if (tn <= st.thr) {
+            stop(st, status_converged, reason_none);
+            return;
+        }
+        if (st.final || st.iters >= st.max_iterations) {
+            stop(st, status_maximum_iterations, reason_none);
+            return;
+        }
+        restart_recurrence(st);
+        st.rho = sm.v[0];  // <r^, r> with r^ = r
+        if (!rho_ok(st, st.rho, st.iters + 1)) return;
+    }
+}
+
+__global__ void kb_count_active(const SysState* __restrict__ st, int systems, int* __restrict__ count)
+{
+    int c = 0;
+    for (int s = threadIdx.x; s < systems; s += blockDim.x) c += st[s].phase == 1 ? 1 : 0;
+    const double total = dev::block_sum_all(static_cast<double>(c));
+    if (threadIdx.x == 0) *count = static_cast<int>(total);
+}
+
+// ------------------------------------------------------------------ host --
+
+Ptrs make_ptrs(owtk_batch bt, const Tiling& tl)
+{
+    Ptrs P{};
+    P.blk_sys = tl.blk_sys.get();
+    P.blk_row0 = tl.blk_row0.get();
+    P.vprefix = bt->vprefix.get();
+    P.pprefix = bt->pprefix.get();
+    P.slice_ptr = bt->slice_ptr.get();
+    P.row_len = bt->row_len.get();
+    P.cols = bt->cols.get();
+    P.vals = bt->vals.get();
+    P.dinv = bt->dinv.get();
+    P.st = bt->state.get();
+    P.partials = tl.partials.get();
+    P.blocks = tl.blocks;
+    P.bs = bt->bs;
+    return P;
+}
+
+std::unique_ptr<Tiling> make_tiling(owtk_batch bt, int tile)
+{
+    auto tl = std::make_unique<Tiling>();
+    tl->tile = tile;
+    std::vector<std::int32_t> bsys, sb0(bt->systems + 1, 0);
+    std::vector<std::int64_t> brow;
+    for (int s = 0; s < bt->systems; ++s) {
+        sb0[s] = static_cast<std::int32_t>(bsys.size());
+        const std::int64_t n = bt->h_vprefix[s + 1] - bt->h_vprefix[s];
+        for (std::int64_t r0 = 0; r0 < n; r0 += tile) {
+            bsys.push_back(s);
+            brow.push_back(bt->h_pprefix[s] + r0);
+        }
+    }
+    sb0[bt->systems] = static_cast<std::int32_t>(bsys.size());
+    if (bsys.empty()) throw std::invalid_argument("empty batch");
+    if (bsys.size() > 0x7fffffffULL) throw std::invalid_argument("batch too large");
+    tl->blocks = static_cast<std::int64_t>(bsys.size());
+    tl->blk_sys.allocate(bsys.size());
+    tl->blk_row0.allocate(brow.size());
+    tl->sys_blk0.allocate(sb0.size());
+    tl->partials.allocate(kPartials * bsys.size());
+    OWTK_HIP(hipMemcpy(tl->blk_sys.get(), bsys.data(), bsys.size() * 4, hipMemcpyHostToDevice));
+    OWTK_HIP(hipMemcpy(tl->blk_row0.get(), brow.data(), brow.size() * 8, hipMemcpyHostToDevice));
+    OWTK_HIP(hipMemcpy(tl->sys_blk0.get(), sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
+    return tl;
+}
+
+template<int BS>
+void launch_iteration(owtk_batch bt, Tiling& tl, const double* b, double* x)
+{
+    const hipStream_t st = bt->ctx->stream;
+    const Ptrs P = make_ptrs(bt, tl);
+    const unsigned int grid = static_cast<unsigned int>(tl.blocks);
+    const unsigned int T = static_cast<unsigned int>(tl.tile);
+    const unsigned int S = static_cast<unsigned int>(bt->systems);
+    const std::int32_t* sb0 = tl.sys_blk0.get();
+    kb_update_p<BS><<<grid, T, 0, st>>>(P, bt->r.get(), bt->rhat.get(), bt->p.get(), bt->v.get(), bt->phat.get());
+    kb_spmv_v<BS><<<grid, T, 0, st>>>(P, bt->phat.get(), bt->rhat.get(), bt->v.get());
+    kb_finish<fin_alpha><<<S, 64, 0, st>>>(P, sb0);
+    kb_update_s<BS><<<grid, T, 0, st>>>(P, bt->r.get(), bt->v.get(), bt->s.get(), bt->shat.get());
+    kb_finish<fin_s><<<S, 64, 0, st>>>(P, sb0);
+    kb_spmv_t<BS><<<grid, T, 0, st>>>(P, bt->shat.get(), bt->s.get(), bt->t.get(), bt->phat.get(), x);
+    kb_finish<fin_omega><<<S, 64, 0, st>>>(P, sb0);
+    kb_update_x<BS><<<grid, T, 0, st>>>(P, bt->phat.get(), bt->shat.get(), bt->s.get(), bt->t.get(), bt->rhat.get(), x,
+                                        bt->r.get());
+    kb_finish<fin_rho><<<S, 64, 0, st>>>(P, sb0);
+    kb_true_residual<BS><<<grid, T, 0, st>>>(P, b, x, bt->r.get());
+    kb_finish<fin_true><<<S, 64, 0, st>>>(P, sb0);
+    OWTK_LAUNCH_CHECK();
+}
+
+void iteration(owtk_batch bt, Tiling& tl, const double* b, double* x)
+{
+    switch (bt->bs) {
+    case 1: launch_iteration<1>(bt, tl, b, x); break;
+    case 2: launch_iteration<2>(bt, tl, b, x); break;
+    case 3: launch_iteration<3>(bt, tl, b, x); break;
+    default: launch_iteration<4>(bt, tl, b, x); break;
+    }
+}
+
+template<int BS>
+void launch_init(owtk_batch bt, Tiling& tl, const double* b, const double* x)
+{
+    const hipStream_t st = bt->ctx->stream;
+    const Ptrs P = make_ptrs(bt, tl);
+    kb_init<BS><<<static_cast<unsigned int>(tl.blocks), static_cast<unsigned int>(tl.tile), 0, st>>>(
+        P, b, x, bt->r.get(), bt->rhat.get(), bt->p.get(), bt->v.get());
+    kb_finish<fin_init><<<static_cast<unsigned int>(bt->systems), 64, 0, st>>>(P, tl.sys_blk0.get());
+    OWTK_LAUNCH_CHECK();
+}
+
+void init(owtk_batch bt, Tiling& tl, const double* b, const double* x)
+{
+    switch (bt->bs) {
+    case 1: launch_init<1>(bt, tl, b, x); break;
+    case 2: launch_init<2>(bt, tl, b, x); break;
+    case 3: launch_init<3>(bt, tl, b, x); break;
+    default: launch_init<4>(bt, tl, b, x); break;
+    }
+}
+
+// Graph of G iterations on (b, x); rebuilt when b, x or G change.
+void ensure_graph(owtk_batch bt, Tiling& tl, const double* b, double* x, int g)
+{
+    if (tl.exec && tl.graph_b == b && tl.graph_x == x && tl.graph_iterations == g) return;
+    if (tl.exec) {
+        OWTK_HIP(hipGraphExecDestroy(tl.exec));
+        tl.exec = nullptr;
+    }
+    if (tl.graph) {
+        OWTK_HIP(hipGraphDestroy(tl.graph));
+        tl.graph = nullptr;
+    }
+    const hipStream_t st = bt->ctx->stream;
+    OWTK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int k = 0; k < g; ++k) iteration(bt, tl, b, x);
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    OWTK_HIP(hipStreamEndCapture(st, &tl.graph));
+    OWTK_HIP(hipGraphInstantiate(&tl.exec, tl.graph, nullptr, nullptr, 0));
+    tl.graph_iterations = g;
+    tl.graph_b = b;
+    tl.graph_x = x;
+}
+
+void set_options(owtk_batch bt, const owtk_solver_options* o)
+{
+    std::vector<SysState> h(bt->systems);
+    for (auto& s : h) {
+        std::memset(&s, 0, sizeof(s));
+        s.thr = o->absolute_tolerance;   // combined with rtol * |b| by the init finisher
+        s.rr = o->relative_tolerance;
+        s.tiny = 1024.0 * std::numeric_limits<double>::min();
+        s.breakdown_tolerance = o->breakdown_tolerance;
+        s.check_interval = static_cast<int>(std::min<std::uint64_t>(o->convergence_check_interval, 1u << 30));
+        s.replace_interval = static_cast<int>(std::min<std::uint64_t>(o->residual_replacement_interval, 1u << 30));
+        s.max_iterations = static_cast<int>(std::min<std::uint64_t>(o->maximum_iterations, 1u << 30));
+        s.status = status_invalid_input;
+    }
+    OWTK_HIP(hipMemcpyAsync(bt->state.get(), h.data(), h.size() * sizeof(SysState), hipMemcpyHostToDevice,
+                            bt->ctx->stream));
+    OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));  // h is a host temporary
+}
+
+int read_active(owtk_batch bt)
+{
+    const hipStream_t st = bt->ctx->stream;
+    kb_count_active<<<1, 256, 0, st>>>(bt->state.get(), bt->systems, bt->active_count.get());
+    OWTK_LAUNCH_CHECK();
+    int n = 0;
+    OWTK_HIP(hipMemcpyAsync(&n, bt->active_count.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    OWTK_HIP(hipStreamSynchronize(st));
+    return n;
+}
+
+// One batched solve; times[s] (may be null): ms from start to the end of the
+// graph replay in which system s stopped.
+void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options* o, int g,
+           std::vector<double>* stop_ms, std::vector<int>* stop_iters)
+{
+    Tiling& tl = *bt->tiling;
+    set_options(bt, o);
+    const auto t0 = std::chrono::steady_clock::now();
+    init(bt, tl, b, x);
+    ensure_graph(bt, tl, b, x, g);
+    std::vector<SysState> h(bt->systems);
+    if (stop_ms) stop_ms->assign(bt->systems, 0.0);
+    std::vector<char> done(bt->systems, 0);
+    auto snapshot = [&]() {
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        OWTK_HIP(hipMemcpyAsync(h.data(), bt->state.get(), h.size() * sizeof(SysState), hipMemcpyDeviceToHost,
+                                bt->ctx->stream));
+        OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
+        int active = 0;
+        for (int s = 0; s < bt->systems; ++s) {
+            if (h[s].phase == 1) {
+                ++active;
+            } else if (!done[s]) {
+                done[s] = 1;
+                if (stop_ms) (*stop_ms)[s] = ms;
+            }
+        }
+        return active;
+    };
+    int active = stop_ms ? snapshot() : read_active(bt);
+    const std::uint64_t cap = o->maximum_iterations + 2;
+    std::uint64_t replays = 0;
+    while (active > 0 && replays * static_cast<std::uint64_t>(g) <= cap) {
+        OWTK_HIP(hipGraphLaunch(tl.exec, bt->ctx->stream));
+        ++replays;
+        active = stop_ms ? snapshot() : read_active(bt);
+    }
+    if (active > 0) throw std::runtime_error("batched BiCGSTAB did not stop within its iteration limit");
+    if (stop_iters) {
+        if (!stop_ms) snapshot();
+        stop_iters->resize(bt->systems);
+        for (int s = 0; s < bt->systems; ++s) (*stop_iters)[s] = h[s].iters;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int owtk_batch_create(owtk_context ctx, int32_t systems, int32_t block_size, const int64_t* vertex_offsets,
+                      const int64_t* row_offsets, const int32_t* columns, const double* values, owtk_batch* out)
+{
+    return guarded([&] {
+        if (!ctx || !vertex_offsets || !row_offsets || !columns || !values || !out)
+            throw std::invalid_argument("null argument");
+        if (systems < 1) throw std::invalid_argument("a batch needs at least one system");
+        if (block_size < 1 || block_size > 4) throw std::invalid_argument("batched solver supports block size 1..4");
+        *out = nullptr;
+        ctx->activate();
+        auto bt = std::make_unique<owtk_batch_s>();
+        bt->ctx = ctx;
+        bt->systems = systems;
+        bt->bs = block_size;
+        bt->h_vprefix.assign(vertex_offsets, vertex_offsets + systems + 1);
+        if (bt->h_vprefix[0] != 0) throw std::invalid_argument("vertex_offsets must start at 0");
+        for (int s = 0; s < systems; ++s)
+            if (bt->h_vprefix[s + 1] <= bt->h_vprefix[s]) throw std::invalid_argument("every system needs a vertex");
+        const std::int64_t N = bt->h_vprefix[systems];
+        bt->vertices = N;
+        const int B = block_size;
+        // padded row space and SELL slices
+        bt->h_pprefix.assign(systems + 1, 0);
+        for (int s = 0; s < systems; ++s)
+            bt->h_pprefix[s + 1] = bt->h_pprefix[s] + ((bt->h_vprefix[s + 1] - bt->h_vprefix[s] + 63) / 64) * 64;
+        const std::int64_t PR = bt->h_pprefix[systems];
+        bt->padded_rows = PR;
+        const std::int64_t slices = PR / 64;
+        std::vector<std::int32_t> slice_ptr(slices + 1, 0), row_len(PR, 0);
+        std::vector<std::int64_t> prow_vertex(PR, -1);
+        for (int s = 0; s < systems; ++s)
+            for (std::int64_t v = bt->h_vprefix[s]; v < bt->h_vprefix[s + 1]; ++v)
+                prow_vertex[bt->h_pprefix[s] + (v - bt->h_vprefix[s])] = v;
+        std::int64_t slots = 0;
+        for (std::int64_t sl = 0; sl < slices; ++sl) {
+            int width = 0;
+            for (int lane = 0; lane < 64; ++lane) {
+                const std::int64_t v = prow_vertex[sl * 64 + lane];
+                if (v < 0) continue;
+                const std::int64_t len = row_offsets[v + 1] - row_offsets[v];
+                if (len < 0) throw std::invalid_argument("row offsets must be non-decreasing");
+                row_len[sl * 64 + lane] = static_cast<std::int32_t>(len);
+                width = std::max<int>(width, static_cast<int>(len));
+            }
+            slots += static_cast<std::int64_t>(width) * 64;
+            if (slots >= (std::int64_t(1) << 31)) throw std::invalid_argument("batch too large for int32 slots");
+            slice_ptr[sl + 1] = static_cast<std::int32_t>(slots);
+        }
+        std::vector<std::int32_t> cols(std::max<std::int64_t>(slots, 1), 0);
+        std::vector<double> vals(std::max<std::int64_t>(slots, 1) * B, 0.0);
+        std::vector<double> dinv(N * B, 0.0);
+        int sys = 0;
+        for (std::int64_t p = 0; p < PR; ++p) {
+            const std::int64_t v = prow_vertex[p];
+            if (v < 0) continue;
+            while (v >= bt->h_vprefix[sys + 1]) ++sys;
+            const std::int64_t sl = p / 64;
+            const int lane = static_cast<int>(p % 64);
+            bool diag = false;
+            for (std::int64_t e = row_offsets[v]; e < row_offsets[v + 1]; ++e) {
+                const std::int64_t k = e - row_offsets[v];
+                const std::int64_t slot = slice_ptr[sl] + 64 * k + lane;
+                const std::int32_t c = columns[e];
+                if (c < bt->h_vprefix[sys] || c >= bt->h_vprefix[sys + 1])
+                    throw std::invalid_argument("column outside its system");
+                cols[slot] = c;
+                for (int q = 0; q < B; ++q) vals[(slot & ~std::int64_t(63)) * B + 64 * q + (slot & 63)] = values[e * B + q];
+                if (c == v) {
+                    diag = true;
+                    for (int q = 0; q < B; ++q) {
+                        const double d = values[e * B + q];
+                        // JacobiPreconditioner rejects zero and non-finite diagonals (preconditioner.hpp:62-68)
+                        if (d == 0.0 || !std::isfinite(d)) throw std::invalid_argument("singular diagonal");
+                        dinv[v * B + q] = 1.0 / d;
+                    }
+                }
+            }
+            if (!diag) throw std::invalid_argument("row without diagonal entry");
+        }
+        bt->entries = row_offsets[N];
+        auto up = [](auto& d, const auto& h) {
+            d.allocate(h.size());
+            OWTK_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+        };
+        up(bt->slice_ptr, slice_ptr);
+        up(bt->row_len, row_len);
+        up(bt->cols, cols);
+        up(bt->vals, vals);
+        up(bt->dinv, dinv);
+        up(bt->vprefix, bt->h_vprefix);
+        up(bt->pprefix, bt->h_pprefix);
+        const std::size_t n = static_cast<std::size_t>(N * B);
+        for (auto* a : {&bt->r, &bt->rhat, &bt->p, &bt->v, &bt->s, &bt->t, &bt->phat, &bt->shat}) {
+            a->allocate(n);
+            kern::fill_zero_bytes(ctx->stream, a->get(), a->bytes());
+        }
+        bt->state.allocate(systems);
+        bt->active_count.allocate(1);
+        bt->tiling = make_tiling(bt.get(), bt->tile);
+        ctx->sync();
+        *out = bt.release();
+    });
+}
+
+int owtk_batch_destroy(owtk_batch bt)
+{
+    return guarded([&] {
+        if (!bt) return;
+        bt->ctx->activate();
+        bt->ctx->sync();
+        delete bt;
+    });
+}
+
+int owtk_batch_set_tile(owtk_batch bt, int32_t tile_rows)
+{
+    return guarded([&] {
+        if (!bt) throw std::invalid_argument("null batch");
+        if (tile_rows != 64 && tile_rows != 128 && tile_rows != 256 && tile_rows != 512)
+            throw std::invalid_argument("tile rows must be 64, 128, 256 or 512");
+        bt->ctx->activate();
+        if (tile_rows == bt->tile) return;
+        bt->ctx->sync();
+        bt->tiling = make_tiling(bt, tile_rows);
+        bt->tile = tile_rows;
+    });
+}
+
+int owtk_batch_autotune(owtk_batch bt, const double* b, int32_t iterations, int32_t* tile_rows, double* ms_per_iteration)
+{
+    return guarded([&] {
+        if (!bt || !b) throw std::invalid_argument("null argument");
+        if (iterations < 1) throw std::invalid_argument("autotune needs at least one iteration");
+        bt->ctx->activate();
+        DeviceArray<double> x(static_cast<std::size_t>(bt->vertices * bt->bs));
+        owtk_solver_options o;
+        owtk_solver_options_default(&o);
+        o.relative_tolerance = 0.0;   // nothing converges: every system stays active
+        o.absolute_tolerance = 0.0;
+        o.convergence_check_interval = 1u << 30;
+        o.residual_replacement_interval = 0;
+        o.maximum_iterations = 1u << 30;
+        int best = bt->tile;
+        double best_ms = std::numeric_limits<double>::infinity();
+        for (int tile : {64, 128, 256, 512}) {
+            owtk_batch_set_tile(bt, tile);
+            Tiling& tl = *bt->tiling;
+            double ms_best_trial = std::numeric_limits<double>::infinity();
+            for (int trial = 0; trial < 2; ++trial) {
+                kern::fill_zero_bytes(bt->ctx->stream, x.get(), x.bytes());
+                set_options(bt, &o);
+                init(bt, tl, b, x.get());
+                ensure_graph(bt, tl, b, x.get(), iterations);
+                OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                OWTK_HIP(hipGraphLaunch(tl.exec, bt->ctx->stream));
+                OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                ms_best_trial = std::min(ms_best_trial, ms / iterations);
+            }
+            if (ms_best_trial < best_ms) {
+                best_ms = ms_best_trial;
+                best = tile;
+            }
+        }
+        owtk_batch_set_tile(bt, best);
+        bt->tiling->graph_b = nullptr;  // the trial graphs bound the scratch x
+        if (tile_rows) *tile_rows = best;
+        if (ms_per_iteration) *ms_per_iteration = best_ms;
+    });
+}
+
+int owtk_batch_bicgstab(owtk_batch bt, const double* b, double* x, const owtk_solver_options* options,
+                        int32_t graph_iterations, owtk_batch_result* results)
+{
+    return guarded([&] {
+        if (!bt || !b || !x || !options) throw std::invalid_argument("null argument");
+        if (graph_iterations < 1 || graph_iterations > 1024) throw std::invalid_argument("graph_iterations must be 1..1024");
+        if (!valid_options(options)) throw std::invalid_argument("invalid solver options");
+        bt->ctx->activate();
+        std::vector<double> ms;
+        std::vector<int> its;
+        solve(bt, b, x, options, graph_iterations, results ? &ms : nullptr, results ? &its : nullptr);
+        if (results) {
+            std::vector<SysState> h(bt->systems);
+            OWTK_HIP(hipMemcpy(h.data(), bt->state.get(), h.size() * sizeof(SysState), hipMemcpyDeviceToHost));
+            for (int s = 0; s < bt->systems; ++s) {
+                results[s].status = h[s].status;
+                results[s].breakdown_reason = h[s].reason;
+                results[s].iterations = static_cast<std::uint64_t>(h[s].iters);
+                results[s].true_residual_norm = h[s].true_norm;
+                results[s].rhs_norm = h[s].bnorm;
+                results[s].relative_residual_norm = h[s].bnorm > 0 ? h[s].true_norm / h[s].bnorm : h[s].true_norm;
+                results[s].stop_ms = ms[s];
+            }
+        }
+    });
+}
+
+} // extern "C"

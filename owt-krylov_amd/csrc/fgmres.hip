@@ -21,6 +21,7 @@
 #include <cmath>
 #include <functional>
 #include <limits>
+#include <memory>
 #include <vector>
 
 using namespace owtk;
@@ -630,8 +631,8 @@ int fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const
 }
 
 // FP64 iterative refinement around an inner low-precision FGMRES.
-int fgmres_refined_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_refinement_options* ro,
-                       owtk_solver_result* res)
+int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, const double* b, double* x,
+                       const owtk_refinement_options* ro, owtk_solver_result* res)
 {
     const auto t0 = std::chrono::steady_clock::now();
     init_result(res);
@@ -645,6 +646,10 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, const double* b, double* 
     }
     if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage == OWTK_F64)
         throw std::invalid_argument("inner preconditioner needs F32 or BF16 storage");
+    if (fallback && fallback->kind != OWTK_PC_IDENTITY && fallback->storage == OWTK_F64)
+        throw std::invalid_argument("fallback preconditioner needs F32 or BF16 storage");
+    if (fallback && !(ro->fallback_ratio > 0.0 && ro->fallback_ratio <= 1.0))
+        throw std::invalid_argument("fallback_ratio must be in (0, 1]");
     owtk_context ctx = m->ctx;
     const hipStream_t st = ctx->stream;
     const auto scratch = ctx->scratch();
@@ -695,6 +700,10 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, const double* b, double* 
         return finish(status_breakdown);
     }
     auto inner = low_engine(m, pc, dtype);
+    std::unique_ptr<FgEngine<float>> backup;  // built on the first switch
+    bool use_fallback = false;
+    DeviceArray<double> x_prev;
+    if (fallback) x_prev.allocate(static_cast<std::size_t>(n));
     owtk_solver_options io = *o;
     io.relative_tolerance = ro->inner_relative_tolerance;
     io.absolute_tolerance = 0.0;
@@ -707,21 +716,41 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, const double* b, double* 
         kern::convert(st, rf.get(), r.get(), static_cast<std::size_t>(n));
         kern::fill_zero_bytes(st, d.get(), d.bytes());
         owtk_solver_result ir;
-        inner.run(rf.get(), d.get(), &io, &ir, cnt);
+        const bool guarded_step = fallback && !use_fallback;
+        if (guarded_step)
+            OWTK_HIP(hipMemcpyAsync(x_prev.get(), x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (use_fallback) backup->run(rf.get(), d.get(), &io, &ir, cnt);
+        else inner.run(rf.get(), d.get(), &io, &ir, cnt);
         res->inner_iterations += ir.iterations;
         res->iterations = res->inner_iterations;
         ++steps;
         res->refinement_steps = steps;
-        if (ir.status == status_breakdown && ir.iterations == 0) {
+        if (use_fallback) ++res->fallback_refinements;
+        const bool inner_failed = ir.status == status_breakdown && ir.iterations == 0;
+        if (inner_failed && !guarded_step) {
             res->breakdown_reason = ir.breakdown_reason;
             return finish(status_breakdown);
         }
-        k_add_f32_to_f64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(x, d.get(), n);
-        OWTK_LAUNCH_CHECK();
+        if (!inner_failed) {
+            k_add_f32_to_f64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(x, d.get(), n);
+            OWTK_LAUNCH_CHECK();
+        }
         const double previous = rn;
         rn = outer_residual();
         res->recursive_residual_norm = ir.true_residual_norm;
         set_residual(res, ir.true_residual_norm, rn, rhs_norm);
+        if (guarded_step && (inner_failed || !std::isfinite(rn) || !(rn <= ro->fallback_ratio * previous))) {
+            // the true residual judges the step: undo it unless it helped,
+            // then continue with the fallback preconditioner
+            if (!(rn < previous)) {
+                OWTK_HIP(hipMemcpyAsync(x, x_prev.get(), n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                rn = outer_residual();
+                set_residual(res, rn, rn, rhs_norm);
+            }
+            use_fallback = true;
+            backup = std::make_unique<FgEngine<float>>(low_engine(m, fallback, dtype));
+            continue;
+        }
         if (!std::isfinite(rn)) {
             res->breakdown_reason = reason_non_finite_scalar;
             return finish(status_breakdown);
@@ -763,6 +792,7 @@ void owtk_refinement_options_default(owtk_refinement_options* o)
     o->inner_relative_tolerance = 1e-4;
     o->inner_maximum_iterations = 300;
     o->maximum_refinements = 50;
+    o->fallback_ratio = 0.5;
 }
 
 int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* options,
@@ -781,7 +811,17 @@ int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, d
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
         m->ctx->activate();
-        fgmres_refined_run(m, inner_pc, b, x, options, result);
+        fgmres_refined_run(m, inner_pc, nullptr, b, x, options, result);
+    });
+}
+
+int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_precond fallback_pc, const double* b,
+                                 double* x, const owtk_refinement_options* options, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result || !fallback_pc) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        fgmres_refined_run(m, inner_pc, fallback_pc, b, x, options, result);
     });
 }
 

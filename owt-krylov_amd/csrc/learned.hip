@@ -1,0 +1,1210 @@
+// Learned vertex-block preconditioner: graph-network training and inference
+// kernels and the learned polynomial apply.  No reference counterpart (the
+// reference defers learned preconditioners, REVIEW_AND_IMPROVEMENT_PLAN.md:
+// 377-379); it plugs into the same preconditioner interface as Jacobi, SSOR
+// and ILU (krylov_solvers.hpp:52-60).
+//
+// Preconditioner.  For a component-diagonal split matrix A = D + A_off,
+//     u_0 = D^-1 r,   u_(k+1) = -D^-1 A_off u_k,   z = sum_k gamma_k * u_k,
+// a truncated Neumann series of the Jacobi splitting whose terms carry
+// per-vertex, per-component weights gamma_k.  gamma = 1 is the plain series.
+// An apply is K off-diagonal SpMVs with a fused epilogue: HBM-bound, no
+// triangular dependency chains.
+//
+// Network (weights BF16, arithmetic FP32), one thread per vertex:
+//   features  x_i = [rho_ic, log(max(1 - rho_ic, 1e-6)) / 8, deg_i / 8],
+//             rho_ic = sum_e |a_ec| / d_ic
+//   weights   w_e = (1/b) sum_c |a_ec| / d_ic
+//   encoder   h0 = tanh(W_in x + b_in)
+//   3 layers  agg_i = sum_e w_e h_col(e),  h = tanh(W_s h + W_n agg + b)
+//   decoder   o = W_out h3 + b_out,  gamma_kic = g_kc (1 + 0.5 tanh(o_(kb+c),i))
+//
+// Training loss over (residual, error) pairs that the solver produced:
+//   L = (1/P) sum_p |D^-1/2 A (z_p - e_p)|^2 / |D^-1/2 A e_p|^2,  z_p = M r_p.
+// The basis vectors u_k(r_p) do not depend on the network, so a training
+// problem keeps them resident in HBM and a step costs one SpMV and one
+// transposed SpMV per pair plus the network's forward and backward passes.
+// Weight gradients are per-block partial outer products summed in a fixed
+// order by a finisher kernel: deterministic, no atomics.
+#include "learned.hpp"
+#include "kernels/blas1.hpp"
+#include "kernels/spmv.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+using namespace owtk;
+
+namespace owtk {
+namespace {
+
+using dev::kBlock;
+constexpr int W = kGnnWidth;
+constexpr int kTile = 64;            // vertices per LDS tile in the gradient reductions
+constexpr int kReduceGrid = 240;     // blocks of the gradient reductions
+constexpr int kMaxOuterOutputs = kGnnMaxOutputs * W + kGnnMaxOutputs;
+constexpr int kOutPerThread = (kMaxOuterOutputs + kBlock - 1) / kBlock;
+
+unsigned int rows_grid(std::int64_t n)
+{
+    const std::int64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 0x7fffffffLL) throw std::invalid_argument("too many vertices");
+    return static_cast<unsigned int>(std::max<std::int64_t>(g, 1));
+}
+
+// ------------------------------------------------------------------ kernels --
+
+// Matrix values -> graph arrays (FP32).  slot: device slot of each CSR entry.
+__global__ __launch_bounds__(256) void k_gnn_gather(std::int64_t n, int b, const std::int32_t* __restrict__ offsets,
+                                                    const std::int64_t* __restrict__ slot, int sell,
+                                                    const double* __restrict__ v64, const double* __restrict__ d64,
+                                                    float* __restrict__ vals, float* __restrict__ diag)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < b; ++c) {
+        const std::int64_t di = sell ? (i & ~std::int64_t(63)) * b + 64 * c + (i & 63) : i * b + c;
+        diag[i * b + c] = static_cast<float>(d64[di]);
+    }
+    for (std::int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+        const std::int64_t s = slot[e];
+        for (int c = 0; c < b; ++c) {
+            const std::int64_t vi = sell ? (s & ~std::int64_t(63)) * b + 64 * c + (s & 63) : s * b + c;
+            vals[static_cast<std::int64_t>(e) * b + c] = static_cast<float>(v64[vi]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gnn_features(std::int64_t n, int b, int f0,
+                                                      const std::int32_t* __restrict__ offsets,
+                                                      const float* __restrict__ vals, const float* __restrict__ diag,
+                                                      float* __restrict__ x, float* __restrict__ w)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    float inv[4];
+    for (int c = 0; c < b; ++c) inv[c] = 1.0f / diag[i * b + c];
+    const std::int32_t e0 = offsets[i], e1 = offsets[i + 1];
+    for (std::int32_t e = e0; e < e1; ++e) {
+        float we = 0.f;
+        for (int c = 0; c < b; ++c) {
+            const float a = __builtin_fabsf(vals[static_cast<std::int64_t>(e) * b + c]);
+            s[c] += a;
+            we += a * inv[c];
+        }
+        w[e] = we / static_cast<float>(b);
+    }
+    for (int c = 0; c < b; ++c) {
+        const float rho = s[c] * inv[c];
+        x[i * f0 + c] = rho;
+        x[i * f0 + b + c] = logf(fmaxf(1.0f - rho, 1e-6f)) * 0.125f;
+    }
+    x[i * f0 + 2 * b] = static_cast<float>(e1 - e0) * 0.125f;
+}
+
+// BF16 parameters -> LDS floats.
+__device__ inline void load_weights(float* dst, const std::uint16_t* __restrict__ p16, std::int64_t off, int count)
+{
+    for (int t = threadIdx.x; t < count; t += blockDim.x) dst[t] = dev::bf16_to_float(p16[off + t]);
+}
+
+__global__ __launch_bounds__(256) void k_gnn_encode(std::int64_t n, int f0, const float* __restrict__ x,
+                                                    const std::uint16_t* __restrict__ p16, std::int64_t off_w,
+                                                    std::int64_t off_b, float* __restrict__ h)
+{
+    __shared__ float sw[W * kGnnMaxFeatures];
+    __shared__ float sb[W];
+    load_weights(sw, p16, off_w, W * f0);
+    load_weights(sb, p16, off_b, W);
+    __syncthreads();
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float xi[kGnnMaxFeatures];
+#pragma unroll
+    for (int f = 0; f < kGnnMaxFeatures; ++f) xi[f] = f < f0 ? x[i * f0 + f] : 0.f;
+    float4* out = reinterpret_cast<float4*>(h + i * W);
+    for (int q = 0; q < W / 4; ++q) {
+        float r[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = 4 * q + j;
+            float acc = sb[o];
+#pragma unroll
+            for (int f = 0; f < kGnnMaxFeatures; ++f)
+                if (f < f0) acc = __builtin_fmaf(sw[o * f0 + f], xi[f], acc);
+            r[j] = tanhf(acc);
+        }
+        out[q] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+__device__ inline void load_row(float (&v)[W], const float* __restrict__ p)
+{
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int j = 0; j < W / 4; ++j) {
+        const float4 t = q[j];
+        v[4 * j] = t.x;
+        v[4 * j + 1] = t.y;
+        v[4 * j + 2] = t.z;
+        v[4 * j + 3] = t.w;
+    }
+}
+
+__device__ inline void store_row(float* __restrict__ p, const float (&v)[W])
+{
+    float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+    for (int j = 0; j < W / 4; ++j) q[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+}
+
+// One message-passing layer.  agg_out (training) keeps the aggregate.
+__global__ __launch_bounds__(256) void k_gnn_layer(std::int64_t n, const std::int32_t* __restrict__ offsets,
+                                                   const std::int32_t* __restrict__ cols, const float* __restrict__ w,
+                                                   const float* __restrict__ hin,
+                                                   const std::uint16_t* __restrict__ p16, std::int64_t off_ws,
+                                                   std::int64_t off_wn, std::int64_t off_b,
+                                                   float* __restrict__ hout, float* __restrict__ agg_out)
+{
+    __shared__ float sws[W * W];
+    __shared__ float swn[W * W];
+    __shared__ float sb[W];
+    load_weights(sws, p16, off_ws, W * W);
+    load_weights(swn, p16, off_wn, W * W);
+    load_weights(sb, p16, off_b, W);
+    __syncthreads();
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) a[k] = 0.f;
+    for (std::int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+        const float we = w[e];
+        const float4* hj = reinterpret_cast<const float4*>(hin + static_cast<std::int64_t>(cols[e]) * W);
+#pragma unroll
+        for (int q = 0; q < W / 4; ++q) {
+            const float4 t = hj[q];
+            a[4 * q] = __builtin_fmaf(we, t.x, a[4 * q]);
+            a[4 * q + 1] = __builtin_fmaf(we, t.y, a[4 * q + 1]);
+            a[4 * q + 2] = __builtin_fmaf(we, t.z, a[4 * q + 2]);
+            a[4 * q + 3] = __builtin_fmaf(we, t.w, a[4 * q + 3]);
+        }
+    }
+    if (agg_out) store_row(agg_out + i * W, a);
+    float h[W];
+    load_row(h, hin + i * W);
+    float4* out = reinterpret_cast<float4*>(hout + i * W);
+    for (int q = 0; q < W / 4; ++q) {
+        float r[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = 4 * q + j;
+            float acc = sb[o];
+#pragma unroll
+            for (int k = 0; k < W; ++k) acc = __builtin_fmaf(sws[o * W + k], h[k], acc);
+#pragma unroll
+            for (int k = 0; k < W; ++k) acc = __builtin_fmaf(swn[o * W + k], a[k], acc);
+            r[j] = tanhf(acc);
+        }
+        out[q] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gnn_decode(std::int64_t n, int b, int outs, const float* __restrict__ h3,
+                                                    const std::uint16_t* __restrict__ p16, std::int64_t off_w,
+                                                    std::int64_t off_b, std::int64_t off_g,
+                                                    float* __restrict__ gamma, float* __restrict__ t_out)
+{
+    __shared__ float sw[kGnnMaxOutputs * W];
+    __shared__ float sb[kGnnMaxOutputs];
+    __shared__ float sg[kGnnMaxOutputs];
+    load_weights(sw, p16, off_w, outs * W);
+    load_weights(sb, p16, off_b, outs);
+    load_weights(sg, p16, off_g, outs);
+    __syncthreads();
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float h[W];
+    load_row(h, h3 + i * W);
+    const std::int64_t nb = n * b;
+    for (int o = 0; o < outs; ++o) {
+        float acc = sb[o];
+#pragma unroll
+        for (int k = 0; k < W; ++k) acc = __builtin_fmaf(sw[o * W + k], h[k], acc);
+        const float t = tanhf(acc);
+        if (t_out) t_out[i * outs + o] = t;
+        const int kk = o / b, c = o - kk * b;
+        gamma[kk * nb + i * b + c] = sg[o] * (1.0f + kGnnModulation * t);
+    }
+}
+
+// ------------------------------------------------------- backward kernels --
+
+// d gamma -> d o (do_out), d g contributions (q_out) and d h3.
+__global__ __launch_bounds__(256) void k_gnn_decode_bwd(std::int64_t n, int b, int outs,
+                                                        const float* __restrict__ dgamma,
+                                                        const float* __restrict__ t,
+                                                        const std::uint16_t* __restrict__ p16, std::int64_t off_w,
+                                                        std::int64_t off_g, float* __restrict__ do_out,
+                                                        float* __restrict__ q_out, float* __restrict__ dh)
+{
+    __shared__ float sw[kGnnMaxOutputs * W];
+    __shared__ float sg[kGnnMaxOutputs];
+    load_weights(sw, p16, off_w, outs * W);
+    load_weights(sg, p16, off_g, outs);
+    __syncthreads();
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const std::int64_t nb = n * b;
+    float acc[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc[k] = 0.f;
+    for (int o = 0; o < outs; ++o) {
+        const int kk = o / b, c = o - kk * b;
+        const float dg = dgamma[kk * nb + i * b + c];
+        const float tt = t[i * outs + o];
+        q_out[i * outs + o] = dg * (1.0f + kGnnModulation * tt);
+        const float d = dg * sg[o] * kGnnModulation * (1.0f - tt * tt);
+        do_out[i * outs + o] = d;
+#pragma unroll
+        for (int k = 0; k < W; ++k) acc[k] = __builtin_fmaf(sw[o * W + k], d, acc[k]);
+    }
+    store_row(dh + i * W, acc);
+}
+
+// du = dh (1 - h^2); s = W_s^T du (written to dprev); v = W_n^T du.
+__global__ __launch_bounds__(256) void k_gnn_layer_bwd(std::int64_t n, const float* __restrict__ h,
+                                                       const float* __restrict__ dh,
+                                                       const std::uint16_t* __restrict__ p16, std::int64_t off_ws,
+                                                       std::int64_t off_wn, float* __restrict__ du_out,
+                                                       float* __restrict__ dprev, float* __restrict__ v_out)
+{
+    __shared__ float sws[W * W];
+    __shared__ float swn[W * W];
+    load_weights(sws, p16, off_ws, W * W);
+    load_weights(swn, p16, off_wn, W * W);
+    __syncthreads();
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float hv[W], du[W];
+    load_row(hv, h + i * W);
+    load_row(du, dh + i * W);
+#pragma unroll
+    for (int k = 0; k < W; ++k) du[k] *= 1.0f - hv[k] * hv[k];
+    store_row(du_out + i * W, du);
+    float s[W], v[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        s[k] = 0.f;
+        v[k] = 0.f;
+    }
+#pragma unroll
+    for (int o = 0; o < W; ++o) {
+        const float d = du[o];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            s[k] = __builtin_fmaf(sws[o * W + k], d, s[k]);
+            v[k] = __builtin_fmaf(swn[o * W + k], d, v[k]);
+        }
+    }
+    store_row(dprev + i * W, s);
+    store_row(v_out + i * W, v);
+}
+
+// dprev_j += sum_(e in row j) w_mirror(e) v_col(e): the transposed aggregation
+// (pattern symmetric, weight of entry (col, j) found through the mirror).
+__global__ __launch_bounds__(256) void k_gnn_agg_t(std::int64_t n, const std::int32_t* __restrict__ offsets,
+                                                   const std::int32_t* __restrict__ cols,
+                                                   const std::int32_t* __restrict__ mirror,
+                                                   const float* __restrict__ w, const float* __restrict__ v,
+                                                   float* __restrict__ dprev)
+{
+    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float acc[W];
+    load_row(acc, dprev + j * W);
+    for (std::int32_t e = offsets[j]; e < offsets[j + 1]; ++e) {
+        const float we = w[mirror[e]];
+        const float4* vi = reinterpret_cast<const float4*>(v + static_cast<std::int64_t>(cols[e]) * W);
+#pragma unroll
+        for (int q = 0; q < W / 4; ++q) {
+            const float4 t = vi[q];
+            acc[4 * q] = __builtin_fmaf(we, t.x, acc[4 * q]);
+            acc[4 * q + 1] = __builtin_fmaf(we, t.y, acc[4 * q + 1]);
+            acc[4 * q + 2] = __builtin_fmaf(we, t.z, acc[4 * q + 2]);
+            acc[4 * q + 3] = __builtin_fmaf(we, t.w, acc[4 * q + 3]);
+        }
+    }
+    store_row(dprev + j * W, acc);
+}
+
+__global__ __launch_bounds__(256) void k_tanh_bwd(std::int64_t count, const float* __restrict__ h,
+                                                  float* __restrict__ d)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count; t += step)
+        d[t] *= 1.0f - h[t] * h[t];
+}
+
+// Partial sums over vertex tiles of A_i B_i^T (R x C) and A_i (R):
+// partials[block][R*C + R].  B may be null (C = 0).
+__global__ __launch_bounds__(256) void k_outer_partials(std::int64_t n, const float* __restrict__ A, int R,
+                                                        const float* __restrict__ B, int C,
+                                                        float* __restrict__ partials)
+{
+    __shared__ float sa[kTile * kGnnMaxOutputs];
+    __shared__ float sb[kTile * W];
+    const int outs = R * C + R;
+    float acc[kOutPerThread];
+#pragma unroll
+    for (int j = 0; j < kOutPerThread; ++j) acc[j] = 0.f;
+    for (std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * kTile; tile < n;
+         tile += static_cast<std::int64_t>(gridDim.x) * kTile) {
+        const int cnt = static_cast<int>(std::min<std::int64_t>(kTile, n - tile));
+        for (int q = threadIdx.x; q < cnt * R; q += blockDim.x) sa[q] = A[tile * R + q];
+        for (int q = threadIdx.x; q < cnt * C; q += blockDim.x) sb[q] = B[tile * C + q];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kOutPerThread; ++j) {
+            const int o = threadIdx.x + kBlock * j;
+            if (o < R * C) {
+                const int r = o / C, c = o - (o / C) * C;
+                float s = 0.f;
+                for (int t = 0; t < cnt; ++t) s = __builtin_fmaf(sa[t * R + r], sb[t * C + c], s);
+                acc[j] += s;
+            } else if (o < outs) {
+                const int r = o - R * C;
+                float s = 0.f;
+                for (int t = 0; t < cnt; ++t) s += sa[t * R + r];
+                acc[j] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < kOutPerThread; ++j) {
+        const int o = threadIdx.x + kBlock * j;
+        if (o < outs) partials[static_cast<std::int64_t>(blockIdx.x) * outs + o] = acc[j];
+    }
+}
+
+// grad[mat + r*C + c] += sum of partials (and grad[vec + r] for the sums).
+__global__ __launch_bounds__(256) void k_outer_finish(const float* __restrict__ partials, int blocks, int R, int C,
+                                                      float* __restrict__ mat, float* __restrict__ vec)
+{
+    const int outs = R * C + R;
+    const int o = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= outs) return;
+    double s = 0.0;
+    for (int k = 0; k < blocks; ++k) s += static_cast<double>(partials[static_cast<std::int64_t>(k) * outs + o]);
+    if (o < R * C) {
+        if (mat) mat[o] += static_cast<float>(s);
+    } else if (vec) {
+        vec[o - R * C] += static_cast<float>(s);
+    }
+}
+
+// ------------------------------------------------------------ loss kernels --
+
+// z[p] = sum_k gamma_k * u[p][k]
+__global__ __launch_bounds__(256) void k_loss_z(std::int64_t nb, int pairs, int k1, const float* __restrict__ gamma,
+                                                const float* __restrict__ u, float* __restrict__ z)
+{
+    const std::int64_t total = nb * pairs;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t p = t / nb, q = t - p * nb;
+        float s = 0.f;
+        for (int k = 0; k < k1; ++k) s = __builtin_fmaf(gamma[k * nb + q], u[(p * k1 + k) * nb + q], s);
+        z[t] = s;
+    }
+}
+
+// y = D^-1/2 A z_p - target_p; ys = D^-1/2 y; partial sums of coef_p y^2.
+// Grid: (row blocks, pairs).
+__global__ __launch_bounds__(256) void k_loss_y(std::int64_t n, int b, const std::int32_t* __restrict__ offsets,
+                                                const std::int32_t* __restrict__ cols,
+                                                const float* __restrict__ vals, const float* __restrict__ diag,
+                                                const float* __restrict__ dis, const float* __restrict__ z,
+                                                const float* __restrict__ target, const float* __restrict__ coef,
+                                                float* __restrict__ ys, double* __restrict__ partials)
+{
+    const std::int64_t nb = n * b;
+    const int p = blockIdx.y;
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double local = 0.0;
+    if (i < n) {
+        const float* zp = z + p * nb;
+        for (int c = 0; c < b; ++c) {
+            float acc = diag[i * b + c] * zp[i * b + c];
+            for (std::int32_t e = offsets[i]; e < offsets[i + 1]; ++e)
+                acc = __builtin_fmaf(vals[static_cast<std::int64_t>(e) * b + c], zp[static_cast<std::int64_t>(cols[e]) * b + c], acc);
+            const float d = dis[i * b + c];
+            const float y = d * acc - target[p * nb + i * b + c];
+            ys[p * nb + i * b + c] = d * y;
+            local += static_cast<double>(y) * static_cast<double>(y);
+        }
+        local *= static_cast<double>(coef[p]);
+    }
+    const double s = dev::block_sum_all(local);
+    if (threadIdx.x == 0) partials[static_cast<std::int64_t>(p) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_loss_finish(const double* __restrict__ partials, int count, double scale, double* __restrict__ loss)
+{
+    double s = 0.0;
+    for (int t = threadIdx.x; t < count; t += blockDim.x) s += partials[t];
+    s = dev::block_sum_all(s);
+    if (threadIdx.x == 0) loss[0] += scale * s;
+}
+
+// g_p = gcoef_p A^T ys_p   (transposed SpMV through the mirror index)
+__global__ __launch_bounds__(256) void k_loss_g(std::int64_t n, int b, const std::int32_t* __restrict__ offsets,
+                                                const std::int32_t* __restrict__ cols,
+                                                const std::int32_t* __restrict__ mirror,
+                                                const float* __restrict__ vals, const float* __restrict__ diag,
+                                                const float* __restrict__ ys, const float* __restrict__ gcoef,
+                                                float* __restrict__ gout)
+{
+    const std::int64_t nb = n * b;
+    const int p = blockIdx.y;
+    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float* yp = ys + p * nb;
+    for (int c = 0; c < b; ++c) {
+        float g = diag[j * b + c] * yp[j * b + c];
+        for (std::int32_t e = offsets[j]; e < offsets[j + 1]; ++e)
+            g = __builtin_fmaf(vals[static_cast<std::int64_t>(mirror[e]) * b + c],
+                               yp[static_cast<std::int64_t>(cols[e]) * b + c], g);
+        gout[p * nb + j * b + c] = g * gcoef[p];
+    }
+}
+
+// dgamma_k = sum_p g_p * u[p][k]
+__global__ __launch_bounds__(256) void k_loss_dgamma(std::int64_t nb, int pairs, int k1, const float* __restrict__ g,
+                                                     const float* __restrict__ u, float* __restrict__ dgamma)
+{
+    const std::int64_t total = nb * k1;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t k = t / nb, q = t - k * nb;
+        float s = 0.f;
+        for (int p = 0; p < pairs; ++p) s = __builtin_fmaf(g[p * nb + q], u[(p * k1 + k) * nb + q], s);
+        dgamma[t] = s;
+    }
+}
+
+// ------------------------------------------------- training data kernels --
+
+// u0 = r / d, target base (FP32 copies of one pair's FP64 vectors).
+__global__ __launch_bounds__(256) void k_basis_init(std::int64_t nb, const double* __restrict__ r,
+                                                    const float* __restrict__ diag, float* __restrict__ u0)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nb; t += step)
+        u0[t] = static_cast<float>(r[t]) / diag[t];
+}
+
+// u_next = -D^-1 A_off u   (graph arrays)
+__global__ __launch_bounds__(256) void k_basis_step(std::int64_t n, int b, const std::int32_t* __restrict__ offsets,
+                                                    const std::int32_t* __restrict__ cols,
+                                                    const float* __restrict__ vals, const float* __restrict__ diag,
+                                                    const float* __restrict__ u, float* __restrict__ un)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < b; ++c) {
+        float acc = 0.f;
+        for (std::int32_t e = offsets[i]; e < offsets[i + 1]; ++e)
+            acc = __builtin_fmaf(vals[static_cast<std::int64_t>(e) * b + c], u[static_cast<std::int64_t>(cols[e]) * b + c], acc);
+        un[i * b + c] = -acc / diag[i * b + c];
+    }
+}
+
+// target = D^-1/2 A e  (FP64 arithmetic from the FP32 graph values)
+__global__ __launch_bounds__(256) void k_target(std::int64_t n, int b, const std::int32_t* __restrict__ offsets,
+                                                const std::int32_t* __restrict__ cols,
+                                                const float* __restrict__ vals, const float* __restrict__ diag,
+                                                const double* __restrict__ e, float* __restrict__ target,
+                                                float* __restrict__ dis)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < b; ++c) {
+        double acc = static_cast<double>(diag[i * b + c]) * e[i * b + c];
+        for (std::int32_t en = offsets[i]; en < offsets[i + 1]; ++en)
+            acc = __builtin_fma(static_cast<double>(vals[static_cast<std::int64_t>(en) * b + c]),
+                                e[static_cast<std::int64_t>(cols[en]) * b + c], acc);
+        const double s = 1.0 / __builtin_sqrt(static_cast<double>(diag[i * b + c]));
+        target[i * b + c] = static_cast<float>(s * acc);
+        dis[i * b + c] = static_cast<float>(s);
+    }
+}
+
+// ------------------------------------------------------------------- Adam --
+__global__ __launch_bounds__(256) void k_adam(std::int64_t count, float* __restrict__ params, float* __restrict__ grad,
+                                              float* __restrict__ m, float* __restrict__ v,
+                                              std::uint16_t* __restrict__ p16, float lr, float b1, float b2, float eps,
+                                              float bc1, float bc2)
+{
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const float g = grad[t];
+    const float mt = b1 * m[t] + (1.0f - b1) * g;
+    const float vt = b2 * v[t] + (1.0f - b2) * g * g;
+    m[t] = mt;
+    v[t] = vt;
+    const float p = params[t] - lr * (mt / bc1) / (__builtin_sqrtf(vt / bc2) + eps);
+    params[t] = p;
+    p16[t] = dev::float_to_bf16(p);
+    grad[t] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_round_bf16(std::int64_t count, const float* __restrict__ params,
+                                                    std::uint16_t* __restrict__ p16)
+{
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < count) p16[t] = dev::float_to_bf16(params[t]);
+}
+
+// ------------------------------------------------------------ apply kernels --
+
+template<class X, class D>
+__global__ __launch_bounds__(256) void k_poly_init(std::int64_t nb, const X* __restrict__ in, const D* __restrict__ dinv,
+                                                   const float* __restrict__ gamma0, X* __restrict__ u0,
+                                                   X* __restrict__ z, const int* stop)
+{
+    if (stop != nullptr && *stop != 0) return;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nb; t += step) {
+        const X u = in[t] * static_cast<X>(dinv[t]);
+        u0[t] = u;
+        z[t] = static_cast<X>(gamma0[t]) * u;
+    }
+}
+
+} // namespace
+} // namespace owtk
+
+namespace owtk::dev {
+
+// u_next = -D^-1 (A_off u), z += gamma_k * u_next
+template<class X>
+struct EpiPoly : EpiNone {
+    X* u_next;
+    const X* dinv;
+    const float* gamma;
+    X* z;
+    template<class A>
+    __device__ void elem(std::int64_t i, A acc, double*) const
+    {
+        const X u = -static_cast<X>(acc) * dinv[i];
+        u_next[i] = u;
+        z[i] = dfma(static_cast<X>(gamma[i]), u, z[i]);
+    }
+};
+
+} // namespace owtk::dev
+
+namespace owtk {
+namespace {
+
+// Off-diagonal SpMV of a split matrix with epilogue e.
+template<class V, class X, class E>
+void offdiag_spmv(owtk_matrix m, const V* vals, const X* x, E e)
+{
+    using namespace owtk::kern;
+    const hipStream_t st = m->ctx->stream;
+    const auto scratch = m->ctx->scratch();
+    const auto& s = m->s;
+    const std::int64_t rows = m->owned;
+    if (s.layout == Layout::sell) {
+        const unsigned int grid = rows_grid(rows);
+        switch (m->bs) {
+        case 1: spmv_sell_cd<1, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
+        case 2: spmv_sell_cd<2, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
+        case 3: spmv_sell_cd<3, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
+        default: spmv_sell_cd<4, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
+        }
+    } else {
+        std::int64_t blocks = (rows + dev::kWavesPerBlock - 1) / dev::kWavesPerBlock;
+        blocks = std::max<std::int64_t>(1, std::min<std::int64_t>(blocks, dev::kMaxReduceBlocks));
+        spmv_csr_cd<V, X, false, E><<<static_cast<unsigned int>(blocks), kBlock, 0, st>>>(
+            s.offsets.get(), s.columns.get(), vals, nullptr, x, rows, m->bs, e, scratch);
+    }
+    OWTK_LAUNCH_CHECK();
+}
+
+// Host-side mirror index of a structurally symmetric pattern.
+std::vector<std::int32_t> mirror_index(std::int64_t n, const std::vector<std::int32_t>& off,
+                                       const std::vector<std::int32_t>& col)
+{
+    const std::int64_t nnz = off[n];
+    // per row: entries sorted by column
+    std::vector<std::int32_t> order(nnz);
+    std::iota(order.begin(), order.end(), 0);
+    for (std::int64_t i = 0; i < n; ++i)
+        std::sort(order.begin() + off[i], order.begin() + off[i + 1],
+                  [&](std::int32_t a, std::int32_t b) { return col[a] < col[b]; });
+    std::vector<std::int32_t> mir(nnz);
+    for (std::int64_t i = 0; i < n; ++i) {
+        for (std::int32_t e = off[i]; e < off[i + 1]; ++e) {
+            const std::int32_t j = col[e];
+            auto first = order.begin() + off[j], last = order.begin() + off[j + 1];
+            auto it = std::lower_bound(first, last, static_cast<std::int32_t>(i),
+                                       [&](std::int32_t a, std::int32_t v) { return col[a] < v; });
+            if (it == last || col[*it] != i)
+                throw std::invalid_argument("learned preconditioner training needs a structurally symmetric pattern");
+            mir[e] = *it;
+        }
+    }
+    return mir;
+}
+
+template<class T>
+void upload_vec(DeviceArray<T>& d, const std::vector<T>& h)
+{
+    d.allocate(std::max<std::size_t>(h.size(), 1));
+    if (!h.empty()) OWTK_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void check_matrix(owtk_gnn g, owtk_matrix m)
+{
+    if (!m) throw std::invalid_argument("null matrix");
+    if (m->kind != OWTK_MATRIX_SPLIT) throw unsupported_error("learned preconditioner needs a split matrix");
+    if (m->ghost != 0) throw unsupported_error("learned preconditioner needs a matrix without ghost nodes");
+    if (m->bs != g->b) throw std::invalid_argument("matrix block size differs from the model's");
+    if (m->ctx != g->ctx) throw std::invalid_argument("matrix and model belong to different contexts");
+}
+
+void build_graph(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with_mirror)
+{
+    check_matrix(model, m);
+    const hipStream_t st = m->ctx->stream;
+    g.n = m->owned;
+    g.b = m->bs;
+    g.nnz = m->h_offsets[g.n];
+    upload_vec(g.offsets, m->h_offsets);
+    upload_vec(g.cols, m->h_columns);
+    if (with_mirror) upload_vec(g.mirror, mirror_index(g.n, m->h_offsets, m->h_columns));
+    std::vector<std::int64_t> slot(m->s.slot_of_entry.begin(), m->s.slot_of_entry.end());
+    if (slot.empty()) {
+        slot.resize(g.nnz);
+        std::iota(slot.begin(), slot.end(), 0);
+    }
+    DeviceArray<std::int64_t> dslot;
+    upload_vec(dslot, slot);
+    g.vals.allocate(std::max<std::int64_t>(g.nnz * g.b, 1));
+    g.diag.allocate(g.n * g.b);
+    g.x.allocate(g.n * model->f0);
+    g.w.allocate(std::max<std::int64_t>(g.nnz, 1));
+    const unsigned int grid = rows_grid(g.n);
+    k_gnn_gather<<<grid, kBlock, 0, st>>>(g.n, g.b, g.offsets.get(), dslot.get(), m->s.layout == Layout::sell ? 1 : 0,
+                                          m->v64.get(), m->d64.get(), g.vals.get(), g.diag.get());
+    OWTK_LAUNCH_CHECK();
+    k_gnn_features<<<grid, kBlock, 0, st>>>(g.n, g.b, model->f0, g.offsets.get(), g.vals.get(), g.diag.get(),
+                                            g.x.get(), g.w.get());
+    OWTK_LAUNCH_CHECK();
+    OWTK_HIP(hipStreamSynchronize(st));  // dslot is released on return
+}
+
+// Forward pass.  h: (layers+1) * n * W floats when `train`, else 2 * n * W
+// (ping-pong); agg (train only): layers * n * W; t (train only): n * outs.
+void forward(owtk_gnn model, const GnnGraph& g, float* h, float* agg, float* t, float* gamma, bool train)
+{
+    const hipStream_t st = model->ctx->stream;
+    const std::int64_t n = g.n;
+    const unsigned int grid = rows_grid(n);
+    const std::uint16_t* p16 = model->params16.get();
+    k_gnn_encode<<<grid, kBlock, 0, st>>>(n, model->f0, g.x.get(), p16, model->off_w_in, model->off_b_in, h);
+    OWTK_LAUNCH_CHECK();
+    float* cur = h;
+    for (int l = 0; l < kGnnLayers; ++l) {
+        float* next = train ? h + static_cast<std::int64_t>(l + 1) * n * W : h + ((l + 1) & 1) * n * W;
+        k_gnn_layer<<<grid, kBlock, 0, st>>>(n, g.offsets.get(), g.cols.get(), g.w.get(), cur, p16, model->off_ws[l],
+                                             model->off_wn[l], model->off_b[l], next,
+                                             train ? agg + static_cast<std::int64_t>(l) * n * W : nullptr);
+        OWTK_LAUNCH_CHECK();
+        cur = next;
+    }
+    k_gnn_decode<<<grid, kBlock, 0, st>>>(n, g.b, model->outs, cur, p16, model->off_w_out, model->off_b_out,
+                                          model->off_g, gamma, train ? t : nullptr);
+    OWTK_LAUNCH_CHECK();
+}
+
+void outer_reduce(owtk_gnn model, std::int64_t n, const float* A, int R, const float* B, int C, float* mat, float* vec)
+{
+    const hipStream_t st = model->ctx->stream;
+    const int blocks = static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(kReduceGrid, (n + kTile - 1) / kTile)));
+    k_outer_partials<<<blocks, kBlock, 0, st>>>(n, A, R, B, C, model->partials.get());
+    OWTK_LAUNCH_CHECK();
+    const int outs = R * C + R;
+    k_outer_finish<<<(outs + kBlock - 1) / kBlock, kBlock, 0, st>>>(model->partials.get(), blocks, R, C, mat, vec);
+    OWTK_LAUNCH_CHECK();
+}
+
+// Workspace layout (floats per vertex): h (layers+1)*W | agg layers*W | t outs |
+// do outs | q outs | dh W | dprev W | du W | v W
+struct TrainViews {
+    float *h, *agg, *t, *dout, *q, *dh, *dprev, *du, *v;
+};
+
+TrainViews ensure_workspace(owtk_gnn model, std::int64_t n, int pairs, int k1, int b)
+{
+    const std::int64_t per = (kGnnLayers + 1) * W + kGnnLayers * W + 3 * model->outs + 4 * W;
+    if (n > model->ws_nodes || pairs > model->ws_pairs) {
+        const std::int64_t nn = std::max(n, model->ws_nodes);
+        const std::int64_t pp = std::max<std::int64_t>(pairs, model->ws_pairs);
+        model->act.allocate(nn * per);
+        model->gamma.allocate(nn * b * k1);
+        model->dgamma.allocate(nn * b * k1);
+        model->z.allocate(nn * b * pp);
+        model->ys.allocate(nn * b * pp);
+        model->loss_partials.allocate(rows_grid(nn) * pp);
+        model->coef.allocate(2 * pp);
+        model->ws_nodes = nn;
+        model->ws_pairs = pp;
+    }
+    TrainViews v{};
+    float* p = model->act.get();
+    v.h = p; p += (kGnnLayers + 1) * W * n;
+    v.agg = p; p += kGnnLayers * W * n;
+    v.t = p; p += model->outs * n;
+    v.dout = p; p += model->outs * n;
+    v.q = p; p += model->outs * n;
+    v.dh = p; p += W * n;
+    v.dprev = p; p += W * n;
+    v.du = p; p += W * n;
+    v.v = p;
+    return v;
+}
+
+void accumulate(owtk_gnn model, owtk_gnn_problem prob, double weight)
+{
+    const GnnGraph& g = prob->g;
+    const hipStream_t st = model->ctx->stream;
+    const std::int64_t n = g.n, nb = g.n * g.b;
+    const int k1 = model->degree + 1, P = prob->pairs;
+    TrainViews v = ensure_workspace(model, n, P, k1, g.b);
+    // per-pair coefficients: loss weight, gradient weight
+    std::vector<float> coef(2 * P);
+    for (int p = 0; p < P; ++p) {
+        coef[p] = static_cast<float>(1.0 / prob->norms[p]);
+        coef[P + p] = static_cast<float>(2.0 * weight / (P * prob->norms[p]));
+    }
+    OWTK_HIP(hipMemcpyAsync(model->coef.get(), coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    OWTK_HIP(hipStreamSynchronize(st));  // coef is a host temporary
+
+    forward(model, g, v.h, v.agg, v.t, model->gamma.get(), true);
+    const unsigned int grid = rows_grid(n);
+    k_loss_z<<<dev::stream_grid(nb * P, 4), kBlock, 0, st>>>(nb, P, k1, model->gamma.get(), prob->u.get(), model->z.get());
+    OWTK_LAUNCH_CHECK();
+    k_loss_y<<<dim3(grid, P), kBlock, 0, st>>>(n, g.b, g.offsets.get(), g.cols.get(), g.vals.get(), g.diag.get(),
+                                               prob->dis.get(), model->z.get(), prob->target.get(), model->coef.get(),
+                                               model->ys.get(), model->loss_partials.get());
+    OWTK_LAUNCH_CHECK();
+    k_loss_finish<<<1, kBlock, 0, st>>>(model->loss_partials.get(), static_cast<int>(grid) * P, weight / P,
+                                        model->loss.get());
+    OWTK_LAUNCH_CHECK();
+    // z is free again: it holds g_p = gcoef_p A^T ys_p
+    k_loss_g<<<dim3(grid, P), kBlock, 0, st>>>(n, g.b, g.offsets.get(), g.cols.get(), g.mirror.get(), g.vals.get(),
+                                               g.diag.get(), model->ys.get(), model->coef.get() + P, model->z.get());
+    OWTK_LAUNCH_CHECK();
+    k_loss_dgamma<<<dev::stream_grid(nb * k1, 4), kBlock, 0, st>>>(nb, P, k1, model->z.get(), prob->u.get(),
+                                                                   model->dgamma.get());
+    OWTK_LAUNCH_CHECK();
+
+    float* grad = model->grad.get();
+    const std::uint16_t* p16 = model->params16.get();
+    const float* h3 = v.h + static_cast<std::int64_t>(kGnnLayers) * n * W;
+    k_gnn_decode_bwd<<<grid, kBlock, 0, st>>>(n, g.b, model->outs, model->dgamma.get(), v.t, p16, model->off_w_out,
+                                              model->off_g, v.dout, v.q, v.dh);
+    OWTK_LAUNCH_CHECK();
+    outer_reduce(model, n, v.dout, model->outs, h3, W, grad + model->off_w_out, grad + model->off_b_out);
+    outer_reduce(model, n, v.q, model->outs, nullptr, 0, nullptr, grad + model->off_g);
+    float* dh = v.dh;
+    float* dprev = v.dprev;
+    for (int l = kGnnLayers - 1; l >= 0; --l) {
+        const float* hout = v.h + static_cast<std::int64_t>(l + 1) * n * W;
+        const float* hin = v.h + static_cast<std::int64_t>(l) * n * W;
+        const float* agg = v.agg + static_cast<std::int64_t>(l) * n * W;
+        k_gnn_layer_bwd<<<grid, kBlock, 0, st>>>(n, hout, dh, p16, model->off_ws[l], model->off_wn[l], v.du, dprev, v.v);
+        OWTK_LAUNCH_CHECK();
+        k_gnn_agg_t<<<grid, kBlock, 0, st>>>(n, g.offsets.get(), g.cols.get(), g.mirror.get(), g.w.get(), v.v, dprev);
+        OWTK_LAUNCH_CHECK();
+        outer_reduce(model, n, v.du, W, hin, W, grad + model->off_ws[l], grad + model->off_b[l]);
+        outer_reduce(model, n, v.du, W, agg, W, grad + model->off_wn[l], nullptr);
+        std::swap(dh, dprev);
+    }
+    k_tanh_bwd<<<dev::stream_grid(n * W, 4), kBlock, 0, st>>>(n * W, v.h, dh);
+    OWTK_LAUNCH_CHECK();
+    outer_reduce(model, n, dh, W, g.x.get(), model->f0, grad + model->off_w_in, grad + model->off_b_in);
+}
+
+void refresh_bf16(owtk_gnn g)
+{
+    k_round_bf16<<<rows_grid(g->count), kBlock, 0, g->ctx->stream>>>(g->count, g->params.get(), g->params16.get());
+    OWTK_LAUNCH_CHECK();
+}
+
+} // namespace
+
+// ------------------------------------------------ preconditioner hooks --
+
+void learned_setup(owtk_precond pc)
+{
+    if (!pc->learned) throw std::invalid_argument("learned preconditioner needs a model");
+    auto* model = static_cast<owtk_gnn>(pc->learned);
+    check_matrix(model, pc->m);
+    if (pc->storage == OWTK_F32 && !pc->m->v32.get()) throw std::invalid_argument("matrix has no FP32 storage");
+    if (pc->storage == OWTK_BF16 && !pc->m->v16.get()) throw std::invalid_argument("matrix has no BF16 storage");
+    auto st = std::make_unique<LearnedPc>();
+    st->model = model;
+    const std::int64_t nb = pc->n();
+    if (pc->storage == OWTK_F64) {
+        st->u64a.allocate(nb);
+        st->u64b.allocate(nb);
+        st->dinv64.allocate(nb);
+    } else {
+        st->u32a.allocate(nb);
+        st->u32b.allocate(nb);
+        st->dinv32.allocate(nb);
+    }
+    st->gamma.allocate(nb * (model->degree + 1));
+    pc->learned_state = st.release();
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_inverse_diag(std::int64_t nb, const double* __restrict__ d64, int sell, int b,
+                                                      double* __restrict__ inv64, float* __restrict__ inv32)
+{
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const std::int64_t i = t / b;
+    const int c = static_cast<int>(t - i * b);
+    const std::int64_t di = sell ? (i & ~std::int64_t(63)) * b + 64 * c + (i & 63) : t;
+    const double inv = 1.0 / d64[di];
+    if (inv64) inv64[t] = inv;
+    if (inv32) inv32[t] = static_cast<float>(inv);
+}
+} // namespace
+
+void learned_numeric(owtk_precond pc)
+{
+    auto* st = static_cast<LearnedPc*>(pc->learned_state);
+    owtk_gnn model = st->model;
+    owtk_matrix m = pc->m;
+    const hipStream_t s = m->ctx->stream;
+    GnnGraph g;
+    build_graph(model, m, g, false);
+    DeviceArray<float> h(2 * g.n * W);
+    forward(model, g, h.get(), nullptr, nullptr, st->gamma.get(), false);
+    const std::int64_t nb = pc->n();
+    k_inverse_diag<<<rows_grid(nb), kBlock, 0, s>>>(nb, m->d64.get(), m->s.layout == Layout::sell ? 1 : 0,
+                                                    m->bs, st->dinv64.get(), st->dinv32.get());
+    OWTK_LAUNCH_CHECK();
+    OWTK_HIP(hipStreamSynchronize(s));  // graph and h are released on return
+}
+
+// Algorithmic bytes of one series step: off-diagonal values + columns + row
+// lengths, dinv, u gathered (once), gamma_k, z read and written, u_next
+// written.
+std::int64_t learned_step_bytes(owtk_precond pc)
+{
+    const std::int64_t nb = pc->n();
+    const int sv = pc->storage == OWTK_F64 ? 8 : (pc->storage == OWTK_F32 ? 4 : 2);
+    const int sx = pc->storage == OWTK_F64 ? 8 : 4;
+    return pc->m->entries * (pc->m->bs * sv + 4) + pc->m->owned * 4 + nb * (5 * sx + 4);
+}
+
+// Apply: init (in, dinv, gamma_0 read; u_0, z written) + K steps.
+std::int64_t learned_apply_bytes(owtk_precond pc)
+{
+    auto* st = static_cast<LearnedPc*>(pc->learned_state);
+    const std::int64_t nb = pc->n();
+    const int sx = pc->storage == OWTK_F64 ? 8 : 4;
+    const std::int64_t K = st ? st->model->degree : 0;
+    const std::int64_t init = nb * (4 * sx + 4);
+    return init + K * learned_step_bytes(pc);
+}
+
+template<class V, class X, class Dv>
+void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const int* stop, X* ua, X* ub, const Dv* dinv)
+{
+    auto* st = static_cast<LearnedPc*>(pc->learned_state);
+    owtk_matrix m = pc->m;
+    const std::int64_t nb = pc->n();
+    const std::int64_t step_bytes = learned_step_bytes(pc);
+    k_poly_init<X, Dv><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, dinv, st->gamma.get(), ua, out, stop);
+    OWTK_LAUNCH_CHECK();
+    for (int k = 1; k <= st->model->degree; ++k) {
+        dev::EpiPoly<X> e;
+        e.stop = stop;
+        e.u_next = ub;
+        e.dinv = dinv;
+        e.gamma = st->gamma.get() + static_cast<std::int64_t>(k) * nb;
+        e.z = out;
+        ProfileScope prof(m->ctx, prof_learned, step_bytes);  // one series-step kernel
+        offdiag_spmv<V, X, dev::EpiPoly<X>>(m, vals, ua, e);
+        std::swap(ua, ub);
+    }
+}
+
+void learned_apply_f64(owtk_precond pc, const double* in, double* out, const int* stop)
+{
+    auto* st = static_cast<LearnedPc*>(pc->learned_state);
+    if (pc->storage != OWTK_F64) throw std::invalid_argument("FP64 apply needs F64 preconditioner storage");
+    learned_apply_t<double, double, double>(pc, pc->m->v64.get(), in, out, stop, st->u64a.get(), st->u64b.get(),
+                                            st->dinv64.get());
+}
+
+void learned_apply_f32(owtk_precond pc, const float* in, float* out, const int* stop)
+{
+    auto* st = static_cast<LearnedPc*>(pc->learned_state);
+    if (pc->storage == OWTK_F32)
+        learned_apply_t<float, float, float>(pc, pc->m->v32.get(), in, out, stop, st->u32a.get(), st->u32b.get(),
+                                             st->dinv32.get());
+    else if (pc->storage == OWTK_BF16)
+        learned_apply_t<std::uint16_t, float, float>(pc, pc->m->v16.get(), in, out, stop, st->u32a.get(),
+                                                     st->u32b.get(), st->dinv32.get());
+    else
+        throw std::invalid_argument("FP32 apply needs F32 or BF16 preconditioner storage");
+}
+
+void learned_release(owtk_precond pc)
+{
+    delete static_cast<LearnedPc*>(pc->learned_state);
+    pc->learned_state = nullptr;
+}
+
+} // namespace owtk
+
+// ------------------------------------------------------------------- C ABI --
+extern "C" {
+
+int owtk_gnn_create(owtk_context ctx, int32_t block_size, int32_t degree, owtk_gnn* out)
+{
+    return guarded([&] {
+        if (!ctx || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        if (block_size < 1 || block_size > 4) throw std::invalid_argument("block size must be 1..4");
+        if (degree < 0 || (degree + 1) * block_size > kGnnMaxOutputs)
+            throw std::invalid_argument("(degree + 1) * block size must not exceed 32");
+        ctx->activate();
+        auto g = std::make_unique<owtk_gnn_s>();
+        g->ctx = ctx;
+        g->b = block_size;
+        g->degree = degree;
+        g->f0 = 2 * block_size + 1;
+        g->outs = (degree + 1) * block_size;
+        std::int64_t pos = 0;
+        g->off_w_in = pos; pos += W * g->f0;
+        g->off_b_in = pos; pos += W;
+        for (int l = 0; l < kGnnLayers; ++l) {
+            g->off_ws[l] = pos; pos += W * W;
+            g->off_wn[l] = pos; pos += W * W;
+            g->off_b[l] = pos; pos += W;
+        }
+        g->off_w_out = pos; pos += g->outs * W;
+        g->off_b_out = pos; pos += g->outs;
+        g->off_g = pos; pos += g->outs;
+        g->count = pos;
+        g->params.allocate(pos);
+        g->grad.allocate(pos);
+        g->adam_m.allocate(pos);
+        g->adam_v.allocate(pos);
+        g->params16.allocate(pos);
+        g->loss.allocate(1);
+        g->partials.allocate(static_cast<std::size_t>(kReduceGrid) * kMaxOuterOutputs);
+        // default parameters: zero network, g = 1 (plain Neumann series)
+        std::vector<float> init(pos, 0.0f);
+        for (int o = 0; o < g->outs; ++o) init[g->off_g + o] = 1.0f;
+        OWTK_HIP(hipMemcpy(g->params.get(), init.data(), pos * sizeof(float), hipMemcpyHostToDevice));
+        OWTK_HIP(hipMemset(g->grad.get(), 0, pos * sizeof(float)));
+        OWTK_HIP(hipMemset(g->adam_m.get(), 0, pos * sizeof(float)));
+        OWTK_HIP(hipMemset(g->adam_v.get(), 0, pos * sizeof(float)));
+        OWTK_HIP(hipMemset(g->loss.get(), 0, sizeof(double)));
+        refresh_bf16(g.get());
+        ctx->sync();
+        *out = g.release();
+    });
+}
+
+int owtk_gnn_destroy(owtk_gnn g)
+{
+    return guarded([&] {
+        if (!g) return;
+        g->ctx->activate();
+        g->ctx->sync();
+        delete g;
+    });
+}
+
+int owtk_gnn_info(owtk_gnn g, int32_t* block_size, int32_t* degree, int64_t* parameter_count)
+{
+    return guarded([&] {
+        if (!g) throw std::invalid_argument("null model");
+        if (block_size) *block_size = g->b;
+        if (degree) *degree = g->degree;
+        if (parameter_count) *parameter_count = g->count;
+    });
+}
+
+int owtk_gnn_set_parameters(owtk_gnn g, const float* params)
+{
+    return guarded([&] {
+        if (!g || !params) throw std::invalid_argument("null argument");
+        g->ctx->activate();
+        const std::size_t bytes = g->count * sizeof(float);
+        OWTK_HIP(hipMemcpy(g->params.get(), params, bytes, hipMemcpyHostToDevice));
+        OWTK_HIP(hipMemset(g->adam_m.get(), 0, bytes));
+        OWTK_HIP(hipMemset(g->adam_v.get(), 0, bytes));
+        OWTK_HIP(hipMemset(g->grad.get(), 0, bytes));
+        g->adam_steps = 0;
+        refresh_bf16(g);
+        g->ctx->sync();
+    });
+}
+
+int owtk_gnn_get_parameters(owtk_gnn g, float* params)
+{
+    return guarded([&] {
+        if (!g || !params) throw std::invalid_argument("null argument");
+        g->ctx->activate();
+        g->ctx->sync();
+        OWTK_HIP(hipMemcpy(params, g->params.get(), g->count * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int owtk_gnn_coefficients(owtk_gnn model, owtk_matrix m, float* gamma)
+{
+    return guarded([&] {
+        if (!model || !gamma) throw std::invalid_argument("null argument");
+        model->ctx->activate();
+        GnnGraph g;
+        build_graph(model, m, g, false);
+        const std::int64_t nb = g.n * g.b;
+        DeviceArray<float> h(2 * g.n * W);
+        DeviceArray<float> gam(nb * (model->degree + 1));
+        forward(model, g, h.get(), nullptr, nullptr, gam.get(), false);
+        model->ctx->sync();
+        OWTK_HIP(hipMemcpy(gamma, gam.get(), gam.bytes(), hipMemcpyDeviceToHost));
+    });
+}
+
+int owtk_gnn_problem_create(owtk_gnn model, owtk_matrix m, int32_t pairs, const double* residuals,
+                            const double* errors, owtk_gnn_problem* out)
+{
+    return guarded([&] {
+        if (!model || !out || !residuals || !errors) throw std::invalid_argument("null argument");
+        if (pairs < 1) throw std::invalid_argument("a training problem needs at least one pair");
+        *out = nullptr;
+        model->ctx->activate();
+        auto p = std::make_unique<owtk_gnn_problem_s>();
+        p->model = model;
+        p->pairs = pairs;
+        build_graph(model, m, p->g, true);
+        const GnnGraph& g = p->g;
+        const hipStream_t st = model->ctx->stream;
+        const std::int64_t n = g.n, nb = g.n * g.b;
+        const int k1 = model->degree + 1;
+        p->u.allocate(static_cast<std::int64_t>(pairs) * k1 * nb);
+        p->target.allocate(static_cast<std::int64_t>(pairs) * nb);
+        p->dis.allocate(nb);
+        const unsigned int grid = rows_grid(n);
+        for (int q = 0; q < pairs; ++q) {
+            float* u = p->u.get() + static_cast<std::int64_t>(q) * k1 * nb;
+            k_basis_init<<<dev::stream_grid(nb, 4), kBlock, 0, st>>>(nb, residuals + q * nb, g.diag.get(), u);
+            OWTK_LAUNCH_CHECK();
+            for (int k = 1; k < k1; ++k) {
+                k_basis_step<<<grid, kBlock, 0, st>>>(n, g.b, g.offsets.get(), g.cols.get(), g.vals.get(), g.diag.get(),
+                                                      u + (k - 1) * nb, u + k * nb);
+                OWTK_LAUNCH_CHECK();
+            }
+            k_target<<<grid, kBlock, 0, st>>>(n, g.b, g.offsets.get(), g.cols.get(), g.vals.get(), g.diag.get(),
+                                              errors + q * nb, p->target.get() + q * nb, p->dis.get());
+            OWTK_LAUNCH_CHECK();
+        }
+        std::vector<float> tgt(static_cast<std::size_t>(pairs) * nb);
+        OWTK_HIP(hipStreamSynchronize(st));
+        OWTK_HIP(hipMemcpy(tgt.data(), p->target.get(), tgt.size() * sizeof(float), hipMemcpyDeviceToHost));
+        p->norms.resize(pairs);
+        for (int q = 0; q < pairs; ++q) {
+            double s = 0.0;
+            for (std::int64_t i = 0; i < nb; ++i) s += static_cast<double>(tgt[q * nb + i]) * tgt[q * nb + i];
+            if (!(s > 0.0) || !std::isfinite(s)) throw std::invalid_argument("training pair with zero or non-finite error");
+            p->norms[q] = s;
+        }
+        *out = p.release();
+    });
+}
+
+int owtk_gnn_problem_destroy(owtk_gnn_problem p)
+{
+    return guarded([&] {
+        if (!p) return;
+        p->model->ctx->activate();
+        p->model->ctx->sync();
+        delete p;
+    });
+}
+
+int owtk_gnn_accumulate(owtk_gnn g, owtk_gnn_problem p, double weight)
+{
+    return guarded([&] {
+        if (!g || !p) throw std::invalid_argument("null argument");
+        if (p->model != g) throw std::invalid_argument("problem belongs to another model");
+        g->ctx->activate();
+        accumulate(g, p, weight);
+    });
+}
+
+int owtk_gnn_read(owtk_gnn g, double* loss, float* grad)
+{
+    return guarded([&] {
+        if (!g) throw std::invalid_argument("null model");
+        g->ctx->activate();
+        g->ctx->sync();
+        if (loss) OWTK_HIP(hipMemcpy(loss, g->loss.get(), sizeof(double), hipMemcpyDeviceToHost));
+        if (grad) OWTK_HIP(hipMemcpy(grad, g->grad.get(), g->count * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int owtk_gnn_zero_grad(owtk_gnn g)
+{
+    return guarded([&] {
+        if (!g) throw std::invalid_argument("null model");
+        g->ctx->activate();
+        OWTK_HIP(hipMemsetAsync(g->grad.get(), 0, g->count * sizeof(float), g->ctx->stream));
+        OWTK_HIP(hipMemsetAsync(g->loss.get(), 0, sizeof(double), g->ctx->stream));
+    });
+}
+
+int owtk_gnn_adam_step(owtk_gnn g, double lr, double beta1, double beta2, double eps, double* loss)
+{
+    return guarded([&] {
+        if (!g) throw std::invalid_argument("null model");
+        if (!(lr > 0.0) || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0) || !(eps > 0.0))
+            throw std::invalid_argument("invalid Adam hyper-parameters");
+        g->ctx->activate();
+        ++g->adam_steps;
+        const double bc1 = 1.0 - std::pow(beta1, static_cast<double>(g->adam_steps));
+        const double bc2 = 1.0 - std::pow(beta2, static_cast<double>(g->adam_steps));
+        k_adam<<<rows_grid(g->count), kBlock, 0, g->ctx->stream>>>(
+            g->count, g->params.get(), g->grad.get(), g->adam_m.get(), g->adam_v.get(), g->params16.get(),
+            static_cast<float>(lr), static_cast<float>(beta1), static_cast<float>(beta2), static_cast<float>(eps),
+            static_cast<float>(bc1), static_cast<float>(bc2));
+        OWTK_LAUNCH_CHECK();
+        if (loss) {
+            g->ctx->sync();
+            OWTK_HIP(hipMemcpy(loss, g->loss.get(), sizeof(double), hipMemcpyDeviceToHost));
+        }
+        OWTK_HIP(hipMemsetAsync(g->loss.get(), 0, sizeof(double), g->ctx->stream));
+    });
+}
+
+} // extern "C"

@@ -6,6 +6,7 @@
 // GPU.  This is the reference's "symbolic once, numeric many" contract
 // (IluLevelPreconditioner::update_values, preconditioner.hpp:269-311).
 #include "precond.hpp"
+#include "learned.hpp"
 #include "kernels/blas1.hpp"
 #include "host/host.hpp"
 
@@ -1038,6 +1039,9 @@ void precond_apply_f64(owtk_precond pc, const double* in, double* out, const int
         tri_launch<TriMode::ilu, double, double>(pc, pc->tri.fval64.get(), pc->tri.bval64.get(),
                                                  pc->tri.dval64.get(), in, out, stop);
         return;
+    case OWTK_PC_LEARNED:
+        learned_apply_f64(pc, in, out, stop);
+        return;
     default:
         throw unsupported_error("preconditioner kind not available on the FP64 path");
     }
@@ -1081,6 +1085,9 @@ void precond_apply_f32(owtk_precond pc, const float* in, float* out, const int* 
         }
         return;
     }
+    case OWTK_PC_LEARNED:
+        learned_apply_f32(pc, in, out, stop);
+        return;
     default:
         throw unsupported_error("preconditioner kind not available on the FP32 path");
     }
@@ -1095,6 +1102,7 @@ std::int64_t precond_apply_bytes(owtk_precond pc)
     case OWTK_PC_IDENTITY: return 2 * n * sx;
     case OWTK_PC_JACOBI:
         return pc->m->dense() ? n * pc->m->bs * sv + 2 * n * sx : n * sv + 2 * n * sx;
+    case OWTK_PC_LEARNED: return learned_apply_bytes(pc);
     default: {
         const auto& t = pc->tri;
         const int bs = pc->m->bs;
@@ -1136,6 +1144,7 @@ int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_p
         pc->block_nodes = params->block_nodes;
         pc->fill_level = params->fill_level;
         pc->schedule = params->schedule;
+        pc->learned = params->learned_model;
         if (pc->schedule < 0 || pc->schedule > 2) throw std::invalid_argument("unknown triangular schedule");
         if (pc->storage < OWTK_F64 || pc->storage > OWTK_BF16) throw std::invalid_argument("unknown storage type");
         if (pc->block_nodes < 0) throw std::invalid_argument("negative block size");
@@ -1155,6 +1164,15 @@ int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_p
             ilu_setup(pc.get());
             ilu_numeric(pc.get());
             break;
+        case OWTK_PC_LEARNED:
+            learned_setup(pc.get());
+            try {
+                learned_numeric(pc.get());
+            } catch (...) {
+                learned_release(pc.get());
+                throw;
+            }
+            break;
         default: throw unsupported_error("preconditioner kind not available");
         }
         pc->updates = 1;
@@ -1172,6 +1190,7 @@ int owtk_precond_update_values(owtk_precond pc)
         case OWTK_PC_JACOBI: jacobi_numeric(pc); break;
         case OWTK_PC_SSOR: ssor_numeric(pc); break;
         case OWTK_PC_ILU: ilu_numeric(pc); break;
+        case OWTK_PC_LEARNED: learned_numeric(pc); break;
         default: break;
         }
         ++pc->updates;
@@ -1246,6 +1265,7 @@ int owtk_precond_destroy(owtk_precond pc)
         if (!pc) return;
         pc->m->ctx->activate();
         pc->m->ctx->sync();
+        if (pc->kind == OWTK_PC_LEARNED) learned_release(pc);
         delete pc;
     });
 }

@@ -82,6 +82,7 @@ struct owtk_precond_s {
     owtk::IluFactor ilu;
     owtk::DeviceArray<int> flag;   // numeric failure flag (zero pivot / singular)
     void* learned = nullptr;       // owtk_gnn model for OWTK_PC_LEARNED
+    void* learned_state = nullptr; // owtk::LearnedPc (coefficients, inverse diagonal, work vectors)
     std::int64_t n() const { return m->owned * m->bs; }
 };
 

@@ -247,6 +247,157 @@ class Matrix:
             pass
 
 
+class Batch:
+    """Many independent component-diagonal CSR systems of uneven size solved
+    together by batched BiCGSTAB + Jacobi (owtk_batch_*).  systems: list of
+    (nodes, row_offsets, columns, values) in the reference's BlockCsrMatrix
+    form (diagonal entry included)."""
+
+    def __init__(self, ctx: Context, block: int, systems):
+        self.ctx, self.block = ctx, block
+        nodes = [int(s[0]) for s in systems]
+        self.vertex_offsets = np.concatenate([[0], np.cumsum(nodes)]).astype(np.int64)
+        offs, cols, vals, base_e = [], [], [], 0
+        for k, (n, o, c, v) in enumerate(systems):
+            o = np.asarray(o, np.int64)
+            offs.append(o[:-1] + base_e)
+            cols.append(np.asarray(c, np.int64) + self.vertex_offsets[k])
+            vals.append(np.asarray(v, np.float64))
+            base_e += int(o[-1])
+        self.row_offsets = np.concatenate(offs + [np.array([base_e])]).astype(np.int64)
+        self.columns = np.concatenate(cols).astype(np.int32)
+        self.values = np.concatenate(vals)
+        self.systems = len(systems)
+        self.unknowns = int(self.vertex_offsets[-1]) * block
+        self.handle = C.c_void_p()
+        check(lib().owtk_batch_create(ctx.handle, self.systems, block, _ptr(self.vertex_offsets),
+                                      _ptr(self.row_offsets), _ptr(self.columns), _ptr(self.values),
+                                      C.byref(self.handle)))
+
+    def set_tile(self, rows: int) -> None:
+        check(lib().owtk_batch_set_tile(self.handle, rows))
+
+    def autotune(self, b: DeviceBuffer, iterations: int = 8):
+        tile, ms = C.c_int32(), C.c_double()
+        check(lib().owtk_batch_autotune(self.handle, b.ptr, iterations, C.byref(tile), C.byref(ms)))
+        return tile.value, ms.value
+
+    def bicgstab(self, b: DeviceBuffer, x: DeviceBuffer, options=None, graph_iterations: int = 8) -> list[dict]:
+        opts = options if options is not None else SolverOptions()
+        res = (_lib.BatchResult * self.systems)()
+        check(lib().owtk_batch_bicgstab(self.handle, b.ptr, x.ptr, C.byref(opts), graph_iterations, res))
+        out = []
+        for r in res:
+            d = {name: getattr(r, name) for name, _ in r._fields_}
+            d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+            out.append(d)
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_batch_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GnnModel:
+    """Vertex-block graph network of the learned preconditioner
+    (owtk_gnn_*).  A fresh model is the plain truncated Neumann series."""
+
+    def __init__(self, ctx: Context, block: int, degree: int):
+        self.ctx = ctx
+        self.handle = C.c_void_p()
+        check(lib().owtk_gnn_create(ctx.handle, block, degree, C.byref(self.handle)))
+        b, d, n = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().owtk_gnn_info(self.handle, C.byref(b), C.byref(d), C.byref(n)))
+        self.block, self.degree, self.parameter_count = b.value, d.value, n.value
+
+    def get_parameters(self) -> np.ndarray:
+        out = np.zeros(self.parameter_count, np.float32)
+        check(lib().owtk_gnn_get_parameters(self.handle, _ptr(out)))
+        return out
+
+    def set_parameters(self, params) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.parameter_count:
+            raise ValueError(f"expected {self.parameter_count} parameters, got {p.size}")
+        check(lib().owtk_gnn_set_parameters(self.handle, _ptr(p)))
+
+    def coefficients(self, matrix: "Matrix") -> np.ndarray:
+        out = np.zeros((self.degree + 1) * matrix.owned * matrix.block, np.float32)
+        check(lib().owtk_gnn_coefficients(self.handle, matrix.handle, _ptr(out)))
+        return out.reshape(self.degree + 1, matrix.owned, matrix.block)
+
+    def problem(self, matrix: "Matrix", residuals: DeviceBuffer, errors: DeviceBuffer, pairs: int) -> "GnnProblem":
+        return GnnProblem(self, matrix, residuals, errors, pairs)
+
+    def zero_grad(self) -> None:
+        check(lib().owtk_gnn_zero_grad(self.handle))
+
+    def accumulate(self, problem: "GnnProblem", weight: float = 1.0) -> None:
+        check(lib().owtk_gnn_accumulate(self.handle, problem.handle, weight))
+
+    def read(self):
+        loss = C.c_double()
+        grad = np.zeros(self.parameter_count, np.float32)
+        check(lib().owtk_gnn_read(self.handle, C.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> float:
+        loss = C.c_double()
+        check(lib().owtk_gnn_adam_step(self.handle, lr, beta1, beta2, eps, C.byref(loss)))
+        return loss.value
+
+    def save(self, path) -> None:
+        np.savez(path, block=self.block, degree=self.degree, params=self.get_parameters())
+
+    @classmethod
+    def load(cls, ctx: Context, path) -> "GnnModel":
+        with np.load(path, allow_pickle=False) as f:
+            model = cls(ctx, int(f["block"]), int(f["degree"]))
+            model.set_parameters(f["params"])
+        return model
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_gnn_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GnnProblem:
+    """One training instance: a split matrix and its (residual, error) pairs,
+    resident on the device (owtk_gnn_problem_*)."""
+
+    def __init__(self, model: GnnModel, matrix: "Matrix", residuals: DeviceBuffer, errors: DeviceBuffer,
+                 pairs: int):
+        self.model, self.matrix = model, matrix
+        self.handle = C.c_void_p()
+        check(lib().owtk_gnn_problem_create(model.handle, matrix.handle, pairs, residuals.ptr, errors.ptr,
+                                            C.byref(self.handle)))
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_gnn_problem_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Preconditioner:
     def __init__(self, matrix: Matrix, kind: int, *, storage: int = F64, omega: float = 1.0,
                  minimum_diagonal: float = 0.0, block_nodes: int = 0, fill_level: int = 0, model=None,
@@ -342,11 +493,15 @@ def refinement_options(**kw) -> _lib.RefinementOptions:
 
 
 def fgmres_refined(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
-                   options=None) -> dict:
+                   options=None, fallback: Preconditioner | None = None) -> dict:
     opts = options if options is not None else refinement_options()
     res = _lib.SolverResult()
-    check(lib().owtk_fgmres_refined(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
-                                    C.byref(opts), C.byref(res)))
+    if fallback is None:
+        check(lib().owtk_fgmres_refined(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
+                                        C.byref(opts), C.byref(res)))
+    else:
+        check(lib().owtk_fgmres_refined_fallback(matrix.handle, pc.handle if pc is not None else None,
+                                                 fallback.handle, b.ptr, x.ptr, C.byref(opts), C.byref(res)))
     d = res.as_dict()
     d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
     d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"

@@ -95,7 +95,7 @@ class SolverResult(C.Structure):
         ("initial_residual_norm", C.c_double), ("recursive_residual_norm", C.c_double),
         ("true_residual_norm", C.c_double), ("relative_residual_norm", C.c_double),
         ("solve_seconds", C.c_double), ("refinement_steps", C.c_uint64),
-        ("inner_iterations", C.c_uint64),
+        ("inner_iterations", C.c_uint64), ("fallback_refinements", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -106,8 +106,14 @@ class RefinementOptions(C.Structure):
     _fields_ = [
         ("outer", SolverOptions), ("inner_storage", C.c_int32), ("inner_restart", C.c_int32),
         ("inner_relative_tolerance", C.c_double), ("inner_maximum_iterations", C.c_uint64),
-        ("maximum_refinements", C.c_uint64),
+        ("maximum_refinements", C.c_uint64), ("fallback_ratio", C.c_double),
     ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("breakdown_reason", C.c_int32), ("iterations", C.c_uint64),
+                ("true_residual_norm", C.c_double), ("rhs_norm", C.c_double),
+                ("relative_residual_norm", C.c_double), ("stop_ms", C.c_double)]
 
 
 class GridSpec(C.Structure):
@@ -166,11 +172,30 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_fgmres": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_refinement_options_default": (None, [C.POINTER(RefinementOptions)]),
     "owtk_fgmres_refined": (C.c_int, [_P, _P, _P, _P, C.POINTER(RefinementOptions), C.POINTER(SolverResult)]),
+    "owtk_fgmres_refined_fallback": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(RefinementOptions),
+                                               C.POINTER(SolverResult)]),
     "owtk_dot": (C.c_int, [_P, C.c_int, C.c_size_t, _P, _P, _DP]),
     "owtk_norm": (C.c_int, [_P, C.c_int, C.c_size_t, _P, _DP]),
     "owtk_axpy": (C.c_int, [_P, C.c_size_t, C.c_double, _P, _P]),
     "owtk_axpy_dot": (C.c_int, [_P, C.c_size_t, C.c_double, _P, _P, _P, _DP]),
     "owtk_multidot": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(_P), _P, _DP]),
+    "owtk_batch_create": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, C.POINTER(_P)]),
+    "owtk_batch_destroy": (C.c_int, [_P]),
+    "owtk_batch_set_tile": (C.c_int, [_P, C.c_int32]),
+    "owtk_batch_autotune": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), _DP]),
+    "owtk_batch_bicgstab": (C.c_int, [_P, _P, _P, C.POINTER(SolverOptions), C.c_int32, C.POINTER(BatchResult)]),
+    "owtk_gnn_create": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "owtk_gnn_destroy": (C.c_int, [_P]),
+    "owtk_gnn_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "owtk_gnn_set_parameters": (C.c_int, [_P, _P]),
+    "owtk_gnn_get_parameters": (C.c_int, [_P, _P]),
+    "owtk_gnn_coefficients": (C.c_int, [_P, _P, _P]),
+    "owtk_gnn_problem_create": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.POINTER(_P)]),
+    "owtk_gnn_problem_destroy": (C.c_int, [_P]),
+    "owtk_gnn_zero_grad": (C.c_int, [_P]),
+    "owtk_gnn_accumulate": (C.c_int, [_P, _P, C.c_double]),
+    "owtk_gnn_read": (C.c_int, [_P, _DP, _P]),
+    "owtk_gnn_adam_step": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _DP]),
     "owtk_grid_spec_default": (None, [C.POINTER(GridSpec), C.c_int32, C.c_int32, C.c_int32, C.c_uint64]),
     "owtk_grid_sizes": (C.c_int, [C.POINTER(GridSpec), _I64P, _I64P]),
     "owtk_grid_generate_split": (C.c_int, [C.POINTER(GridSpec), _P, _P, _P, _P, _P, _P, _P]),

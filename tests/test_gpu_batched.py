@@ -1,0 +1,104 @@
+"""Batched BiCGSTAB + Jacobi on the GPU against the single-system solver and
+the CPU oracle (the reference's bicgstab restated)."""
+import numpy as np
+import pytest
+
+import owtk
+from owtk import workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch(ctx):
+    systems, rhs, grids = workloads.batched_systems(count=12, nx_min=8, nx_max=64, seed0=7000)
+    return systems, rhs, grids
+
+
+def single_solve(ctx, system, rhs, opts):
+    n, o, c, v = system
+    m = owtk.Matrix.csr(ctx, n, 0, 3, o, c, v)
+    pc = owtk.Preconditioner(m, owtk.PC_JACOBI)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    res = owtk.bicgstab(m, pc, b, x, opts)
+    return res, x.download()
+
+
+@pytest.mark.parametrize("tile", [64, 256, 512])
+def test_batched_matches_single_system(ctx, small_batch, tile):
+    systems, rhs, grids = small_batch
+    bt = owtk.Batch(ctx, 3, systems)
+    bt.set_tile(tile)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    opts = owtk.SolverOptions(relative_tolerance=1e-8)
+    results = bt.bicgstab(b, x, opts, graph_iterations=4)
+    xb = x.download()
+    off = 0
+    singles = []
+    for k, (sysk, res) in enumerate(zip(systems, results)):
+        n = sysk[0] * 3
+        single, xs = single_solve(ctx, sysk, rhs[off:off + n], opts)
+        singles.append(single["iterations"])
+        assert res["status_name"] == "converged", (k, res)
+        assert res["relative_residual_norm"] <= 1e-8
+        # same recurrence and cadence; reduction order differs, so the step at
+        # which |s| or |r| first crosses the threshold may move by a check interval
+        assert abs(res["iterations"] - single["iterations"]) <= max(2, 0.1 * single["iterations"]), \
+            (k, res["iterations"], single["iterations"])
+        err = np.linalg.norm(xb[off:off + n] - xs) / np.linalg.norm(xs)
+        assert err < 1e-6, (k, err)
+        off += n
+    print(f"tile {tile}: iterations", [r["iterations"] for r in results])
+    print("single-system iterations", singles)
+
+
+def test_batched_matches_oracle_iterations(ctx, small_batch):
+    systems, rhs, grids = small_batch
+    bt = owtk.Batch(ctx, 3, systems[:4])
+    n_all = sum(s[0] for s in systems[:4]) * 3
+    b = ctx.upload(rhs[:n_all])
+    x = ctx.empty(n_all)
+    results = bt.bicgstab(b, x, owtk.SolverOptions(relative_tolerance=1e-8))
+    off = 0
+    for sysk, res in zip(systems[:4], results):
+        n, o, c, v = sysk
+        om = orc.OMatrix.csr(n, 0, 3, o, c, v)
+        pc = orc.OPreconditioner(om, orc.OPreconditioner.JACOBI)
+        xo, ro = orc.bicgstab(om, pc, rhs[off:off + n * 3], relative_tolerance=1e-8)
+        print(res["iterations"], ro["iterations"])
+        assert ro["status"] == 0 and res["status_name"] == "converged"
+        assert abs(res["iterations"] - ro["iterations"]) <= max(2, 0.1 * ro["iterations"])
+        off += n * 3
+
+
+def test_masks_and_zero_rhs(ctx, small_batch):
+    systems, rhs, grids = small_batch
+    rhs = rhs.copy()
+    n0 = systems[0][0] * 3
+    rhs[:n0] = 0.0  # system 0: b = 0, x0 = 0 -> converged at iteration 0
+    bt = owtk.Batch(ctx, 3, systems)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    results = bt.bicgstab(b, x, owtk.SolverOptions(relative_tolerance=1e-8, maximum_iterations=5),
+                          graph_iterations=2)
+    assert results[0]["status_name"] == "converged" and results[0]["iterations"] == 0
+    assert np.all(x.download()[:n0] == 0.0)
+    # an iteration cap of 5 stops the others with maximum_iterations after exactly 5
+    for r in results[1:]:
+        assert r["status_name"] in ("maximum_iterations", "converged")
+        if r["status_name"] == "maximum_iterations":
+            assert r["iterations"] == 5
+
+
+def test_autotune_picks_a_valid_tile(ctx, small_batch):
+    systems, rhs, grids = small_batch
+    bt = owtk.Batch(ctx, 3, systems)
+    b = ctx.upload(rhs)
+    tile, ms = bt.autotune(b, iterations=4)
+    assert tile in (64, 128, 256, 512) and ms > 0
+    x = ctx.empty(rhs.size)
+    results = bt.bicgstab(b, x, owtk.SolverOptions(relative_tolerance=1e-8))
+    assert all(r["status_name"] == "converged" for r in results)
