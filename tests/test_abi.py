@@ -50,3 +50,13 @@ def test_options_defaults_mirror_reference():
     assert o.restart == 30 and o.residual_replacement_interval == 50
     assert o.gmres_orthogonalization == 0
     assert o.breakdown_tolerance == 64 * 2.220446049250313e-16
+
+
+def test_cpp_header_compiles():
+    """include/owtk/hip_policy.hpp and the C header compile as C++20 and C."""
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-I", str(root / "include"),
+                    str(root / "tests" / "cpp" / "test_hip_policy.cpp")], check=True)
+    subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-Wall", "-Werror", "-x", "c", str(root / "include" / "owtk_hip.h")],
+                   check=True)
