@@ -5,6 +5,7 @@ BiCGSTAB + block-Jacobi ILU(0), plus the split SpMV's HBM bandwidth.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32|bf16]
     python bench.py --kernel spmv|copy|learned --steps K   (kernel loops for rocprofv3 counters)
     python bench.py --workload batched                   (512 uneven systems, batched BiCGSTAB)
+    python bench.py --workload bsr                       (dense 4x4 BSR, BF16 FGMRES(40)-IR + SSOR)
 
 One "step" is one full solve of the headline method from x0 = 0 on the
 resident 2048 x 1024 system; the FP64 baseline solve is timed the same way in
@@ -33,6 +34,7 @@ PEAK_HBM_GBPS = 8000.0
 NX, NY, BS, SEED = 2048, 1024, 3, 0
 BLOCK_NODES = 4096
 BATCH_SYSTEMS = 512
+BSR_GRID = 1024
 BATCH_GRAPH_ITERATIONS = 8
 RTOL = 1e-8
 MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
@@ -270,6 +272,81 @@ def batched_bench(args):
     return 0
 
 
+def bsr_bench(args):
+    """Dense 4x4-block BSR on the 1024x1024 grid: BF16 matrix and multicolour
+    SSOR storage, FP32 accumulation, FP64-refined FGMRES(40) to rtol 1e-10;
+    error against an FP64 GPU solve."""
+    t0 = time.perf_counter()
+    n, off, col, val, rhs = owtk.generate_bsr(BSR_GRID, BSR_GRID, 4, SEED, ordering=owtk.ORDER_HILBERT)
+    log(f"[bench] bsr: {n} vertices, {off[-1]} blocks, generated in {time.perf_counter() - t0:.1f}s")
+    with owtk.Context(0) as ctx:
+        m = owtk.Matrix.bsr(ctx, n, 0, 4, off, col, val)
+        m.add_storage(owtk.BF16)
+        b = ctx.upload(rhs)
+        x = ctx.empty(rhs.size)
+        r = ctx.empty(rhs.size)
+        bn = ctx.norm(b)
+        # FP64 reference solution: FP64 FGMRES(40) + FP64 SSOR to 1e-12
+        pc64 = owtk.Preconditioner(m, owtk.PC_SSOR)
+        ref = owtk.fgmres(m, pc64, b, x, owtk.SolverOptions(relative_tolerance=1e-12, restart=40,
+                                                           maximum_iterations=5000))
+        x64 = x.download()
+        log(f"[bench] bsr FP64 reference: {ref['status_name']} after {ref['iterations']} iterations")
+        pc16 = owtk.Preconditioner(m, owtk.PC_SSOR, storage=owtk.BF16)
+        ropts = owtk.refinement_options(inner_storage=owtk.BF16, inner_restart=40, inner_relative_tolerance=1e-4,
+                                        relative_tolerance=1e-10)
+        times, res = [], None
+        for k in range(args.warmup + args.steps):
+            x.zero()
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            res = owtk.fgmres_refined(m, pc16, b, x, ropts)
+            ctx.synchronize()
+            if k >= args.warmup:
+                times.append((time.perf_counter() - t1) * 1e3)
+                log(f"[bench] bsr bf16 fgmres-ir+ssor step {len(times)}/{args.steps}: {times[-1]:.1f} ms, "
+                    f"inner it={res['inner_iterations']}, refinements={res['refinement_steps']}")
+        m.residual(b, x, r)
+        rel = ctx.norm(r) / bn
+        err = float(np.linalg.norm(x.download() - x64) / np.linalg.norm(x64))
+        info = m.info()
+        sp = {}
+        xs = ctx.upload(np.random.default_rng(1).uniform(-1, 1, rhs.size))
+        ys = ctx.empty(rhs.size)
+        xf = ctx.upload(np.random.default_rng(1).uniform(-1, 1, rhs.size).astype(np.float32))
+        yf = ctx.empty(rhs.size, np.float32)
+        for name, dt, cat, xv, yv in (("f64", owtk.F64, ctx.PROF_SPMV_F64, xs, ys),
+                                      ("bf16", owtk.BF16, ctx.PROF_SPMV_BF16, xf, yf)):
+            for _ in range(5):
+                m.spmv(xv, yv, dt)
+            ctx.synchronize()
+            ctx.profile_reset()
+            ctx.profile(True)
+            for _ in range(30):
+                m.spmv(xv, yv, dt)
+            ctx.profile(False)
+            p = ctx.profile_read(cat)
+            sp[name] = {"avg_ms": p["avg_ms"], "gbps": p["gbps"], "bytes": p["bytes"] // max(p["launches"], 1)}
+        log(f"[bench] bsr spmv f64 {sp['f64']['avg_ms'] * 1e3:.1f} us {sp['f64']['gbps']:.0f} GB/s, "
+            f"bf16 {sp['bf16']['avg_ms'] * 1e3:.1f} us {sp['bf16']['gbps']:.0f} GB/s")
+        ms = float(np.median(times))
+        out = {
+            "metric": "ms to true rel. residual 1e-10 (dense 4x4 BSR, BF16 storage)", "value": round(ms, 3),
+            "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 storage, f32 accumulation, f64 refinement", "data": "synthetic",
+            "config": {"workload": f"{BSR_GRID}x{BSR_GRID} triangulated grid, {n} vertices, dense 4x4 blocks "
+                                   "(diag N(0,1)+6I, off N(0,0.15)U(0.1,1)), Hilbert order, seed 0",
+                       "method": "FP64-refined FGMRES(40), BF16 matrix + BF16 multicolour block SSOR"},
+            "inner_iterations": res["inner_iterations"], "refinements": res["refinement_steps"],
+            "status": res["status_name"], "true_rel": rel, "error_vs_fp64": err,
+            "fp64_reference_iterations": ref["iterations"], "spmv": sp,
+            "spmv_bytes": {"f64": info.spmv_bytes_f64, "bf16": info.spmv_bytes_bf16},
+        }
+        print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,7 +355,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", choices=["spmv", "copy", "learned"], default=None)
     ap.add_argument("--storage", choices=["f32", "bf16"], default="f32")
-    ap.add_argument("--workload", choices=["headline", "batched"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "batched", "bsr"], default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -289,6 +366,8 @@ def main():
         return 0
     if args.workload == "batched":
         return batched_bench(args)
+    if args.workload == "bsr":
+        return bsr_bench(args)
 
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)

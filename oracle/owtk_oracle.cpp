@@ -320,6 +320,131 @@ struct Ssor : Pc {
     }
 };
 
+// Dense-block SSOR in multicolour order.  No reference counterpart: the
+// reference's DenseBlockCsrMatrix has no diagonal() and no SSOR
+// (dense_block_csr.hpp; preconditioner.hpp:28-34).  This restates the
+// project's definition: greedy colouring of the symmetrised owned graph in
+// row order; forward y_i = D_i^-1 (r_i - omega sum_(colour j < colour i)
+// A_ij y_j), backward z_i = D_i^-1 (s D_i y_i - omega sum_(colour j >
+// colour i) A_ij z_j), s = omega (2 - omega); D_i^-1 by Gauss-Jordan with
+// partial pivoting.  Pinned in tests by the textbook identity: the result
+// equals s (D + omega U)^-1 D (D + omega L)^-1 r of the colour-permuted
+// matrix, solved densely.
+struct DenseSsor : Pc {
+    const Matrix* a;
+    double omega;
+    int bs;
+    std::vector<std::int32_t> colour, order;
+    std::vector<double> inv, dblk, work;
+    DenseSsor(const Matrix* m, double w) : a(m), omega(w), bs(m->bs)
+    {
+        if (!(w > 0 && w < 2)) throw std::invalid_argument("SSOR omega must be in (0,2)");
+        const std::int64_t n = a->owned;
+        std::vector<std::vector<std::int32_t>> adj(n);
+        for (std::int64_t i = 0; i < n; ++i)
+            for (std::int32_t e = a->off[i]; e < a->off[i + 1]; ++e) {
+                const std::int64_t j = a->col[e];
+                if (j == i || j >= n) continue;
+                adj[i].push_back(static_cast<std::int32_t>(j));
+                adj[j].push_back(static_cast<std::int32_t>(i));
+            }
+        colour.assign(n, -1);
+        for (std::int64_t i = 0; i < n; ++i) {
+            std::vector<char> used(adj[i].size() + 2, 0);
+            for (std::int32_t j : adj[i])
+                if (colour[j] >= 0 && colour[j] < static_cast<std::int32_t>(used.size())) used[colour[j]] = 1;
+            std::int32_t c = 0;
+            while (used[c]) ++c;
+            colour[i] = c;
+        }
+        order.resize(n);
+        for (std::int64_t i = 0; i < n; ++i) order[i] = static_cast<std::int32_t>(i);
+        std::stable_sort(order.begin(), order.end(), [&](std::int32_t x, std::int32_t y) { return colour[x] < colour[y]; });
+        update();
+    }
+    void update() override
+    {
+        const std::int64_t n = a->owned;
+        const int b2 = bs * bs;
+        inv.assign(n * b2, 0.0);
+        dblk.assign(n * b2, 0.0);
+        work.assign(n * bs, 0.0);
+        for (std::int64_t i = 0; i < n; ++i) {
+            const double* blk = nullptr;
+            for (std::int32_t e = a->off[i]; e < a->off[i + 1]; ++e)
+                if (a->col[e] == i) blk = a->val + static_cast<std::int64_t>(e) * b2;
+            if (!blk) throw std::invalid_argument("dense-block CSR row has no diagonal block");
+            std::copy(blk, blk + b2, dblk.begin() + i * b2);
+            std::vector<double> m(blk, blk + b2), r(b2, 0.0);
+            for (int k = 0; k < bs; ++k) r[k * bs + k] = 1.0;
+            for (int k = 0; k < bs; ++k) {
+                int p = k;
+                for (int q = k + 1; q < bs; ++q)
+                    if (std::abs(m[q * bs + k]) > std::abs(m[p * bs + k])) p = q;
+                if (!(std::abs(m[p * bs + k]) > 0.0) || !std::isfinite(m[p * bs + k]))
+                    throw std::invalid_argument("SSOR preconditioner has a singular diagonal block");
+                if (p != k)
+                    for (int c = 0; c < bs; ++c) {
+                        std::swap(m[k * bs + c], m[p * bs + c]);
+                        std::swap(r[k * bs + c], r[p * bs + c]);
+                    }
+                const double ip = 1.0 / m[k * bs + k];
+                for (int c = 0; c < bs; ++c) {
+                    m[k * bs + c] *= ip;
+                    r[k * bs + c] *= ip;
+                }
+                for (int q = 0; q < bs; ++q) {
+                    if (q == k) continue;
+                    const double f = m[q * bs + k];
+                    for (int c = 0; c < bs; ++c) {
+                        m[q * bs + c] = std::fma(-f, m[k * bs + c], m[q * bs + c]);
+                        r[q * bs + c] = std::fma(-f, r[k * bs + c], r[q * bs + c]);
+                    }
+                }
+            }
+            std::copy(r.begin(), r.end(), inv.begin() + i * b2);
+        }
+        ++updates;
+    }
+    void sweep(std::int64_t i, bool forward, const double* in, double* out)
+    {
+        const int b2 = bs * bs;
+        std::vector<double> acc(bs), xj(bs);
+        if (forward) {
+            for (int r = 0; r < bs; ++r) acc[r] = in[i * bs + r];
+        } else {
+            const double s = omega * (2.0 - omega);
+            for (int r = 0; r < bs; ++r) {
+                double d = 0.0;
+                for (int q = 0; q < bs; ++q) d = std::fma(dblk[i * b2 + r * bs + q], work[i * bs + q], d);
+                acc[r] = d * s;
+            }
+        }
+        const double* src = forward ? work.data() : out;
+        for (std::int32_t e = a->off[i]; e < a->off[i + 1]; ++e) {
+            const std::int64_t j = a->col[e];
+            if (j >= a->owned) continue;
+            if (forward ? !(colour[j] < colour[i]) : !(colour[j] > colour[i])) continue;
+            const double* blk = a->val + static_cast<std::int64_t>(e) * b2;
+            for (int q = 0; q < bs; ++q) xj[q] = src[j * bs + q];
+            for (int r = 0; r < bs; ++r)
+                for (int q = 0; q < bs; ++q) acc[r] = std::fma(-(omega * blk[r * bs + q]), xj[q], acc[r]);
+        }
+        double* dst = forward ? work.data() : out;
+        for (int r = 0; r < bs; ++r) {
+            double t = 0.0;
+            for (int q = 0; q < bs; ++q) t = std::fma(inv[i * b2 + r * bs + q], acc[q], t);
+            dst[i * bs + r] = t;
+        }
+    }
+    void apply(const double* in, double* out) override
+    {
+        const std::int64_t n = a->owned;
+        for (std::int64_t k = 0; k < n; ++k) sweep(order[k], true, in, out);
+        for (std::int64_t k = n; k-- > 0;) sweep(order[k], false, in, out);
+    }
+};
+
 // IluLevelPreconditioner (preconditioner.hpp:253-500) on the merged local CSR,
 // optionally block-restricted (block-Jacobi ILU).
 struct Ilu : Pc {
@@ -953,7 +1078,10 @@ int oracle_pc_create(const oracle_matrix* m, const oracle_pc_params* p, void** o
         switch (p->kind) {
         case 0: h->pc = std::make_unique<Identity>(h->matrix.owned * h->matrix.bs); break;
         case 1: h->pc = std::make_unique<Jacobi>(&h->matrix, mind); break;
-        case 2: h->pc = std::make_unique<Ssor>(&h->matrix, p->omega, p->block_nodes); break;
+        case 2:
+            if (h->matrix.kind == kBsr) h->pc = std::make_unique<DenseSsor>(&h->matrix, p->omega);
+            else h->pc = std::make_unique<Ssor>(&h->matrix, p->omega, p->block_nodes);
+            break;
         case 3: h->pc = std::make_unique<Ilu>(&h->matrix, p->block_nodes, p->fill_level); break;
         default: throw std::invalid_argument("unknown preconditioner");
         }

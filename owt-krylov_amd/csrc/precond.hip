@@ -8,6 +8,7 @@
 #include "precond.hpp"
 #include "learned.hpp"
 #include "kernels/blas1.hpp"
+#include "kernels/spmv.hpp"
 #include "host/host.hpp"
 
 #include <algorithm>
@@ -173,6 +174,79 @@ __global__ __launch_bounds__(256) void k_block_jacobi_apply(const V* __restrict_
 #pragma unroll
         for (int c = 0; c < BS; ++c) t = dev::dfma(static_cast<X>(blk[i * BS + c]), r[c], t);
         out[row * BS + i] = t;
+    }
+}
+
+// Dense-block SSOR in multicolour order (no reference counterpart: the
+// reference's DenseBlockCsrMatrix has no SSOR).  Rows of one colour are
+// independent, so each colour is one parallel launch:
+//   forward  y_i = D_i^-1 (r_i - omega sum_(colour j < colour i) A_ij y_j)
+//   backward z_i = D_i^-1 (s D_i y_i - omega sum_(colour j > colour i) A_ij z_j),
+// s = omega (2 - omega), i.e. z = s (D + omega U)^-1 D (D + omega L)^-1 r in
+// the colour-permuted order.  Ghost columns are dropped (rank-local, like the
+// component-diagonal SSOR).  Entries are visited in the row's stored order.
+template<int BS, class V, class X, class I, bool FWD>
+__global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict__ rows, std::int64_t count,
+                                                 const std::int32_t* __restrict__ slice_ptr,
+                                                 const std::int32_t* __restrict__ row_len,
+                                                 const std::int32_t* __restrict__ cols, const V* __restrict__ vals,
+                                                 const std::int64_t* __restrict__ dslot,
+                                                 const std::int32_t* __restrict__ color, int c, std::int64_t owned,
+                                                 double omega, double scale, const X* __restrict__ in,
+                                                 X* __restrict__ y, X* __restrict__ out, const I* __restrict__ inv,
+                                                 const int* stop)
+{
+    using A = typename dev::acc_type<X>::type;
+    if (stop && *stop) return;
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const std::int64_t i = rows[t];
+    const A w = static_cast<A>(omega);
+    A acc[BS];
+    if constexpr (FWD) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r) acc[r] = static_cast<A>(in[i * BS + r]);
+    } else {
+        A yi[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) yi[r] = static_cast<A>(y[i * BS + r]);
+        const std::int64_t ds = dslot[i] * (BS * BS);
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            A d = A(0);
+#pragma unroll
+            for (int q = 0; q < BS; ++q) d = dev::dfma(static_cast<A>(dev::load_value(vals, ds + r * BS + q)), yi[q], d);
+            acc[r] = d * static_cast<A>(scale);
+        }
+    }
+    const std::int64_t base = slice_ptr[i >> 6];
+    const int lane = static_cast<int>(i & 63);
+    const int len = row_len[i];
+    const X* src = FWD ? y : out;
+    for (int k = 0; k < len; ++k) {
+        const std::int64_t slot = base + 64 * k + lane;
+        const std::int64_t j = cols[slot];
+        if (j >= owned) continue;
+        const int cj = color[j];
+        if (FWD ? !(cj < c) : !(cj > c)) continue;
+        A xj[BS];
+#pragma unroll
+        for (int q = 0; q < BS; ++q) xj[q] = static_cast<A>(src[j * BS + q]);
+        const std::int64_t b0 = slot * (BS * BS);
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int q = 0; q < BS; ++q)
+                acc[r] = dev::dfma(-(w * static_cast<A>(dev::load_value(vals, b0 + r * BS + q))), xj[q], acc[r]);
+    }
+    X* dst = FWD ? y : out;
+    const I* blk = inv + i * (BS * BS);
+#pragma unroll
+    for (int r = 0; r < BS; ++r) {
+        A s = A(0);
+#pragma unroll
+        for (int q = 0; q < BS; ++q) s = dev::dfma(static_cast<A>(blk[r * BS + q]), acc[q], s);
+        dst[i * BS + r] = static_cast<X>(s);
     }
 }
 
@@ -788,11 +862,88 @@ void jacobi_numeric(owtk_precond pc)
 }
 
 // --------------------------------------------------------------------- SSOR --
+// Greedy colouring of the symmetrised owned-row graph in row order: each row
+// takes the smallest colour none of its neighbours holds.
+std::vector<std::int32_t> greedy_colours(owtk_matrix m)
+{
+    const std::int64_t n = m->owned;
+    std::vector<std::vector<std::int32_t>> adj(n);
+    for (std::int64_t i = 0; i < n; ++i)
+        for (std::int32_t e = m->h_offsets[i]; e < m->h_offsets[i + 1]; ++e) {
+            const std::int64_t j = m->h_columns[e];
+            if (j == i || j >= n) continue;
+            adj[i].push_back(static_cast<std::int32_t>(j));
+            adj[j].push_back(static_cast<std::int32_t>(i));
+        }
+    std::vector<std::int32_t> colour(n, -1);
+    std::vector<std::int64_t> seen;
+    for (std::int64_t i = 0; i < n; ++i) {
+        seen.assign(adj[i].size() + 2, -1);
+        for (std::int32_t j : adj[i])
+            if (colour[j] >= 0 && colour[j] < static_cast<std::int32_t>(seen.size())) seen[colour[j]] = i;
+        std::int32_t c = 0;
+        while (seen[c] == i) ++c;
+        colour[i] = c;
+    }
+    return colour;
+}
+
+void dense_ssor_setup(owtk_precond pc)
+{
+    owtk_matrix m = pc->m;
+    if (m->bs > 4 || m->s.layout != Layout::sell) throw unsupported_error("dense-block SSOR supports block sizes up to 4");
+    jacobi_setup(pc);  // diagonal block slots + inverse storage
+    const std::vector<std::int32_t> colour = greedy_colours(m);
+    const std::int32_t colours = colour.empty() ? 0 : *std::max_element(colour.begin(), colour.end()) + 1;
+    pc->h_mc_ptr.assign(colours + 1, 0);
+    for (std::int32_t c : colour) ++pc->h_mc_ptr[c + 1];
+    for (std::int32_t c = 0; c < colours; ++c) pc->h_mc_ptr[c + 1] += pc->h_mc_ptr[c];
+    std::vector<std::int32_t> rows(colour.size());
+    std::vector<std::int64_t> fill(pc->h_mc_ptr.begin(), pc->h_mc_ptr.end() - 1);
+    for (std::int64_t i = 0; i < static_cast<std::int64_t>(colour.size()); ++i) rows[fill[colour[i]]++] = static_cast<std::int32_t>(i);
+    upload(pc->mc_color, colour);
+    upload(pc->mc_rows, rows);
+    const std::size_t nb = static_cast<std::size_t>(m->owned * m->bs);
+    if (pc->storage == OWTK_F64) pc->work64.allocate(std::max<std::size_t>(nb, 1));
+    else pc->work32.allocate(std::max<std::size_t>(nb, 1));
+}
+
+template<class V, class X, class I>
+void dense_ssor_apply(owtk_precond pc, const V* vals, const X* in, X* out, const I* inv, X* y, const int* stop)
+{
+    owtk_matrix m = pc->m;
+    const hipStream_t st = m->ctx->stream;
+    const auto& s = m->s;
+    const double scale = pc->omega * (2.0 - pc->omega);
+    const int colours = static_cast<int>(pc->h_mc_ptr.size()) - 1;
+    auto launch = [&](int c, auto fwd) {
+        constexpr bool FWD = decltype(fwd)::value;
+        const std::int64_t first = pc->h_mc_ptr[c], count = pc->h_mc_ptr[c + 1] - first;
+        if (count == 0) return;
+        const unsigned int g = static_cast<unsigned int>((count + kBlock - 1) / kBlock);
+        const std::int32_t* rows = pc->mc_rows.get() + first;
+        switch (m->bs) {
+        case 1: k_mc_ssor<1, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
+        case 2: k_mc_ssor<2, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
+        case 3: k_mc_ssor<3, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
+        default: k_mc_ssor<4, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
+        }
+        OWTK_LAUNCH_CHECK();
+    };
+    const int category = sizeof(X) == 8 ? prof_precond_f64 : prof_precond_low;
+    ProfileScope prof(m->ctx, category, precond_apply_bytes(pc));
+    for (int c = 0; c < colours; ++c) launch(c, std::true_type{});
+    for (int c = colours - 1; c >= 0; --c) launch(c, std::false_type{});
+}
+
 void ssor_setup(owtk_precond pc)
 {
     owtk_matrix m = pc->m;
-    if (m->dense()) throw unsupported_error("dense-block SSOR is not available for this block size");
     if (!(pc->omega > 0.0 && pc->omega < 2.0)) throw std::invalid_argument("SSOR omega must be in (0,2)");
+    if (m->dense()) {
+        dense_ssor_setup(pc);
+        return;
+    }
     const LocalRows L = local_rows(m, pc->block_nodes);
     TriHost t;
     const std::int64_t n = m->owned;
@@ -825,6 +976,14 @@ void ssor_setup(owtk_precond pc)
 void ssor_numeric(owtk_precond pc)
 {
     owtk_matrix m = pc->m;
+    if (m->dense()) {
+        try {
+            jacobi_numeric(pc);  // diagonal-block inverses (FP64, and FP32 copy for low storage)
+        } catch (const std::invalid_argument&) {
+            throw std::invalid_argument("SSOR preconditioner has a singular diagonal block");
+        }
+        return;
+    }
     auto& t = pc->tri;
     const hipStream_t st = m->ctx->stream;
     const int bs = m->bs;
@@ -1032,6 +1191,10 @@ void precond_apply_f64(owtk_precond pc, const double* in, double* out, const int
         OWTK_LAUNCH_CHECK();
         return;
     case OWTK_PC_SSOR:
+        if (m->dense()) {
+            dense_ssor_apply<double, double, double>(pc, m->v64.get(), in, out, pc->inv64.get(), pc->work64.get(), stop);
+            return;
+        }
         tri_launch<TriMode::ssor, double, double>(pc, pc->tri.fval64.get(), pc->tri.bval64.get(),
                                                   pc->tri.dval64.get(), in, out, stop);
         return;
@@ -1074,6 +1237,17 @@ void precond_apply_f32(owtk_precond pc, const float* in, float* out, const int* 
         return;
     case OWTK_PC_SSOR:
     case OWTK_PC_ILU: {
+        if (m->dense()) {  // dense-block SSOR (ILU is rejected at creation)
+            if (pc->storage == OWTK_F32) {
+                if (!m->v32.get()) throw std::invalid_argument("matrix has no FP32 storage");
+                dense_ssor_apply<float, float, float>(pc, m->v32.get(), in, out, pc->inv32.get(), pc->work32.get(), stop);
+            } else {
+                if (!m->v16.get()) throw std::invalid_argument("matrix has no BF16 storage");
+                dense_ssor_apply<std::uint16_t, float, float>(pc, m->v16.get(), in, out, pc->inv32.get(),
+                                                              pc->work32.get(), stop);
+            }
+            return;
+        }
         const auto& t = pc->tri;
         const bool ilu = pc->kind == OWTK_PC_ILU;
         if (pc->storage == OWTK_F32) {
@@ -1103,6 +1277,14 @@ std::int64_t precond_apply_bytes(owtk_precond pc)
     case OWTK_PC_JACOBI:
         return pc->m->dense() ? n * pc->m->bs * sv + 2 * n * sx : n * sv + 2 * n * sx;
     case OWTK_PC_LEARNED: return learned_apply_bytes(pc);
+    case OWTK_PC_SSOR:
+        if (pc->m->dense()) {
+            // both sweeps read every block once (lower or upper), the diagonal
+            // block and inverse once each; in, y, out
+            const std::int64_t b2 = std::int64_t(pc->m->bs) * pc->m->bs;
+            return pc->m->entries * (b2 * sv + 4) + pc->m->owned * (b2 * sv + b2 * sv + 8) + 4 * n * sx;
+        }
+        [[fallthrough]];
     default: {
         const auto& t = pc->tri;
         const int bs = pc->m->bs;

@@ -81,6 +81,11 @@ struct owtk_precond_s {
     owtk::TriSchedule tri;
     owtk::IluFactor ilu;
     owtk::DeviceArray<int> flag;   // numeric failure flag (zero pivot / singular)
+    // dense-block SSOR: multicolour order (colour per row, rows grouped by colour)
+    owtk::DeviceArray<std::int32_t> mc_color, mc_rows;
+    std::vector<std::int64_t> h_mc_ptr;
+    owtk::DeviceArray<double> work64;
+    owtk::DeviceArray<float> work32;
     void* learned = nullptr;       // owtk_gnn model for OWTK_PC_LEARNED
     void* learned_state = nullptr; // owtk::LearnedPc (coefficients, inverse diagonal, work vectors)
     std::int64_t n() const { return m->owned * m->bs; }

@@ -16,6 +16,21 @@ def small_batch(ctx):
     return systems, rhs, grids
 
 
+def oracle_envelope(system, rhs, tol=1e-8):
+    """Iteration counts of the oracle BiCGSTAB + Jacobi (the reference's
+    recurrence) under perturbations of b at the 1e-15 level.  High-contrast
+    systems move by several iterations under such perturbations, so a
+    different (equally valid) reduction order may too."""
+    n, o, c, v = system
+    om = orc.OMatrix.csr(n, 0, 3, o, c, v)
+    pc = orc.OPreconditioner(om, orc.OPreconditioner.JACOBI)
+    its = []
+    for eps in (0.0, 1e-15, -1e-15, 3e-15, -3e-15):
+        _, r = orc.bicgstab(om, pc, rhs * (1 + eps * np.sin(np.arange(rhs.size))), relative_tolerance=tol)
+        its.append(r["iterations"])
+    return min(its), max(its)
+
+
 def single_solve(ctx, system, rhs, opts):
     n, o, c, v = system
     m = owtk.Matrix.csr(ctx, n, 0, 3, o, c, v)
@@ -44,10 +59,11 @@ def test_batched_matches_single_system(ctx, small_batch, tile):
         singles.append(single["iterations"])
         assert res["status_name"] == "converged", (k, res)
         assert res["relative_residual_norm"] <= 1e-8
-        # same recurrence and cadence; reduction order differs, so the step at
-        # which |s| or |r| first crosses the threshold may move by a check interval
-        assert abs(res["iterations"] - single["iterations"]) <= max(2, 0.1 * single["iterations"]), \
-            (k, res["iterations"], single["iterations"])
+        # same recurrence and cadence; only the reduction order differs
+        lo, hi = oracle_envelope(sysk, rhs[off:off + n])
+        assert lo - 2 <= res["iterations"] <= hi + 2, (k, res["iterations"], single["iterations"], lo, hi)
+        if hi - lo <= 1:
+            assert abs(res["iterations"] - single["iterations"]) <= 1, (k, res["iterations"], single["iterations"])
         err = np.linalg.norm(xb[off:off + n] - xs) / np.linalg.norm(xs)
         assert err < 1e-6, (k, err)
         off += n
@@ -68,9 +84,10 @@ def test_batched_matches_oracle_iterations(ctx, small_batch):
         om = orc.OMatrix.csr(n, 0, 3, o, c, v)
         pc = orc.OPreconditioner(om, orc.OPreconditioner.JACOBI)
         xo, ro = orc.bicgstab(om, pc, rhs[off:off + n * 3], relative_tolerance=1e-8)
-        print(res["iterations"], ro["iterations"])
+        lo, hi = oracle_envelope(sysk, rhs[off:off + n * 3])
+        print(res["iterations"], ro["iterations"], (lo, hi))
         assert ro["status"] == 0 and res["status_name"] == "converged"
-        assert abs(res["iterations"] - ro["iterations"]) <= max(2, 0.1 * ro["iterations"])
+        assert lo - 2 <= res["iterations"] <= hi + 2
         off += n * 3
 
 

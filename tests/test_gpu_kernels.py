@@ -284,3 +284,28 @@ def test_chain_and_level_agree_low_precision(ctx, c0, storage):
         a = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=1)
         b = owtk.Preconditioner(m, kind, block_nodes=512, storage=storage, schedule=2)
         assert_parity(gpu_apply(ctx, b, r, np.float32), gpu_apply(ctx, a, r, np.float32), 2, tol=1e-6)
+
+
+@pytest.mark.parametrize("bs", [2, 3, 4])
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_dense_ssor_matches_oracle(ctx, bs, omega):
+    n, off, col, val, rhs = owtk.generate_bsr(48, 32, bs, 1, ordering=owtk.ORDER_HILBERT)
+    om = orc.OMatrix.bsr(n, 0, bs, off, col, val)
+    ref = orc.OPreconditioner(om, orc.OPreconditioner.SSOR, omega=omega).apply(rhs)
+    m = owtk.Matrix.bsr(ctx, n, 0, bs, off, col, val)
+    m.add_storage(owtk.F32)
+    m.add_storage(owtk.BF16)
+    pc = owtk.Preconditioner(m, owtk.PC_SSOR, omega=omega)
+    x, y = ctx.upload(rhs), ctx.empty(rhs.size)
+    pc.apply(x, y)
+    got = y.download()
+    err64 = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"bs {bs} omega {omega}: FP64 max relative difference {err64:.2e}, bitwise {np.array_equal(got, ref)}")
+    assert err64 <= 1e-13
+    for storage, tol in ((owtk.F32, 1e-5), (owtk.BF16, 5e-2)):
+        pcl = owtk.Preconditioner(m, owtk.PC_SSOR, omega=omega, storage=storage)
+        xl, yl = ctx.upload(rhs.astype(np.float32)), ctx.empty(rhs.size, np.float32)
+        pcl.apply(xl, yl)
+        err = np.max(np.abs(yl.download() - ref)) / np.max(np.abs(ref))
+        print(f"  storage {storage}: {err:.2e}")
+        assert err <= tol

@@ -149,3 +149,21 @@ def test_refined_fgmres_reaches_fp64_accuracy(ctx, c0, storage):
     print("refined", res["refinement_steps"], res["inner_iterations"], true_rel)
     assert res["status_name"] == "converged" and true_rel <= 1e-8
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+
+
+def test_dense_bsr_bf16_refined_fgmres_with_ssor(ctx):
+    """Dense 4x4 BSR system (the c2 generator on a small grid): BF16 matrix and
+    multicolour SSOR storage inside FP64-refined FGMRES(40) to rtol 1e-10."""
+    n, off, col, val, rhs = owtk.generate_bsr(96, 96, 4, 0, ordering=owtk.ORDER_HILBERT)
+    m = owtk.Matrix.bsr(ctx, n, 0, 4, off, col, val)
+    m.add_storage(owtk.BF16)
+    pc = owtk.Preconditioner(m, owtk.PC_SSOR, storage=owtk.BF16)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    res = owtk.fgmres_refined(m, pc, b, x, owtk.refinement_options(relative_tolerance=1e-10, inner_storage=owtk.BF16,
+                                                                  inner_restart=40))
+    om = orc.OMatrix.bsr(n, 0, 4, off, col, val)
+    rel = np.linalg.norm(rhs - om.spmv(x.download())) / np.linalg.norm(rhs)
+    print(res["status_name"], res["inner_iterations"], res["refinement_steps"], rel)
+    assert res["status_name"] == "converged"
+    assert rel <= 1e-10

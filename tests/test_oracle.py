@@ -238,3 +238,56 @@ def test_c0_bicgstab_jacobi_converges(c0):
     r = s.rhs - m.spmv(x)
     assert np.linalg.norm(r) / np.linalg.norm(s.rhs) <= 1e-8
     assert res["relative_residual_norm"] <= 1e-8
+
+
+def _greedy_colours(n, offsets, columns):
+    adj = [set() for _ in range(n)]
+    for i in range(n):
+        for j in columns[offsets[i]:offsets[i + 1]]:
+            if j != i and j < n:
+                adj[i].add(int(j))
+                adj[int(j)].add(i)
+    colour = [-1] * n
+    for i in range(n):
+        used = {colour[j] for j in adj[i] if colour[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        colour[i] = c
+    return np.array(colour)
+
+
+@pytest.mark.parametrize("omega", [0.7, 1.0, 1.4])
+def test_dense_ssor_textbook_form(omega):
+    """Dense-block multicolour SSOR equals s (D + w U)^-1 D (D + w L)^-1 r of
+    the colour-permuted matrix (dense solve)."""
+    import owtk
+    bs = 4
+    n, off, col, val, rhs = owtk.generate_bsr(6, 4, bs, 3, ordering=owtk.ORDER_RCM)
+    colour = _greedy_colours(n, off, col)
+    order = np.argsort(colour, kind="stable")
+    pos = np.empty(n, int)
+    pos[order] = np.arange(n)
+    dense = np.zeros((n * bs, n * bs))
+    for i in range(n):
+        for e in range(off[i], off[i + 1]):
+            j = col[e]
+            dense[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs] = val[e * bs * bs:(e + 1) * bs * bs].reshape(bs, bs)
+    perm = np.concatenate([np.arange(o * bs, (o + 1) * bs) for o in order])
+    P = dense[np.ix_(perm, perm)]
+    D = np.zeros_like(P)
+    L = np.zeros_like(P)
+    U = np.zeros_like(P)
+    for bi in range(n):
+        for bj in range(n):
+            blk = P[bi * bs:(bi + 1) * bs, bj * bs:(bj + 1) * bs]
+            (D if bi == bj else (L if bj < bi else U))[bi * bs:(bi + 1) * bs, bj * bs:(bj + 1) * bs] = blk
+    r = np.random.default_rng(5).uniform(-1, 1, n * bs)
+    s = omega * (2 - omega)
+    zp = s * np.linalg.solve(D + omega * U, D @ np.linalg.solve(D + omega * L, r[perm]))
+    ref = np.empty_like(zp)
+    ref[perm] = zp
+    om = orc.OMatrix.bsr(n, 0, bs, off, col, val)
+    pc = orc.OPreconditioner(om, orc.OPreconditioner.SSOR, omega=omega)
+    z = pc.apply(r)
+    assert np.max(np.abs(z - ref)) <= 1e-12 * np.max(np.abs(ref))
