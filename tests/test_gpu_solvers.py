@@ -167,3 +167,30 @@ def test_dense_bsr_bf16_refined_fgmres_with_ssor(ctx):
     print(res["status_name"], res["inner_iterations"], res["refinement_steps"], rel)
     assert res["status_name"] == "converged"
     assert rel <= 1e-10
+
+
+def test_cold_front_golden_fgmres(ctx):
+    """The reference tests' real-data cold-front matrix (golden fixture): GPU
+    FGMRES reaches the test's absolute tolerance on the true residual with
+    the oracle's iteration count, in FP64 and through FP32 refinement."""
+    from golden.fixtures import cold_front
+    n, off, col, val, rhs, restart, maxit, atol = cold_front()
+    om = orc.OMatrix.csr(n, 0, 1, off, col, val)
+    _, ro = orc.fgmres(om, None, rhs, relative_tolerance=0.0, absolute_tolerance=atol, restart=restart,
+                       maximum_iterations=maxit)
+    m = owtk.Matrix.csr(ctx, n, 0, 1, off, col, val)
+    b = ctx.upload(rhs)
+    x = ctx.empty(n)
+    res = owtk.fgmres(m, None, b, x, owtk.SolverOptions(relative_tolerance=0.0, absolute_tolerance=atol,
+                                                        restart=restart, maximum_iterations=maxit))
+    true = np.linalg.norm(rhs - om.spmv(x.download()))
+    print("gpu", res["iterations"], true, "oracle", ro["iterations"], ro["true_residual_norm"])
+    assert res["status_name"] == "converged" and true <= atol
+    assert abs(res["iterations"] - ro["iterations"]) <= 1
+    assert abs(res["true_residual_norm"] - true) <= 1e-12 + 1e-6 * true
+    x.zero()
+    rr = owtk.fgmres_refined(m, None, b, x, owtk.refinement_options(relative_tolerance=0.0, absolute_tolerance=atol,
+                                                                   inner_storage=owtk.F32, inner_restart=restart))
+    true = np.linalg.norm(rhs - om.spmv(x.download()))
+    print("refined", rr["status_name"], rr["refinement_steps"], rr["inner_iterations"], true)
+    assert rr["status_name"] == "converged" and true <= atol
