@@ -38,6 +38,10 @@ BSR_GRID = 1024
 BATCH_GRAPH_ITERATIONS = 8
 RTOL = 1e-8
 MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
+# HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
+# kernel's known input; WRITE_SIZE by 1), c1 workload.
+TRAFFIC_BYTES = {("spmv", "f64"): 520_558_592, ("learned", "f32"): 369_938_432}
 KERNEL_NAMES = {
     "spmv_f64": "split SpMV, FP64 (spmv_sell_cd<3,double,double,true,EpiStore>)",
     "spmv_f32": "split SpMV, FP32 storage (spmv_sell_cd<3,float,float,...>)",
@@ -80,12 +84,13 @@ def true_rel(ctx, m, b, x, r, bnorm):
 
 
 def timed_solves(ctx, label, steps, warmup, run, x):
+    """Warm-up solves, `steps` timed solves (no kernel timing events in the
+    timed region), then one extra solve with per-launch HIP-event timing for
+    the per-kernel breakdown (not part of the timing)."""
     for _ in range(warmup):
         x.zero()
         run()
     ctx.synchronize()
-    ctx.profile_reset()
-    ctx.profile(True)
     times, results = [], []
     for k in range(steps):
         x.zero()
@@ -97,6 +102,12 @@ def timed_solves(ctx, label, steps, warmup, run, x):
         results.append(res)
         log(f"[bench] {label} step {k + 1}/{steps}: {times[-1]:.2f} ms, it={res['iterations']}, "
             f"status={res['status_name']}, fallback steps={res.get('fallback_refinements', 0)}")
+    x.zero()
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    run()
+    ctx.synchronize()
     ctx.profile(False)
     prof = {name: ctx.profile_read(cat) for name, cat in (
         ("spmv_f64", ctx.PROF_SPMV_F64), ("spmv_f32", ctx.PROF_SPMV_F32), ("spmv_bf16", ctx.PROF_SPMV_BF16),
@@ -105,9 +116,9 @@ def timed_solves(ctx, label, steps, warmup, run, x):
 
 
 def spmv_timing(ctx, m, n, reps=50):
-    """Back-to-back split SpMVs on the context stream, timed with HIP events
-    (one event pair per launch): average kernel time and algorithmic GB/s per
-    storage type."""
+    """Back-to-back split SpMVs on the context stream, timed by one HIP event
+    pair around the launches: average time per launch (kernel plus the
+    launch gap) and algorithmic GB/s per storage type."""
     x64 = ctx.upload(np.random.default_rng(1).uniform(-1, 1, n))
     y64 = ctx.empty(n)
     x32 = ctx.upload(np.random.default_rng(1).uniform(-1, 1, n).astype(np.float32))
@@ -119,15 +130,43 @@ def spmv_timing(ctx, m, n, reps=50):
         for _ in range(5):
             m.spmv(x, y, dtype)
         ctx.synchronize()
+        # algorithmic bytes of one launch (from the library's own accounting)
         ctx.profile_reset()
         ctx.profile(True)
+        m.spmv(x, y, dtype)
+        ctx.profile(False)
+        nbytes = ctx.profile_read(cat)["bytes"]
+        ctx.synchronize()
+        ctx.region_begin()
         for _ in range(reps):
             m.spmv(x, y, dtype)
-        ctx.profile(False)
-        p = ctx.profile_read(cat)
-        out[name] = {"avg_ms": p["avg_ms"], "gbps": p["gbps"], "bytes": p["bytes"] // max(p["launches"], 1),
-                     "launches": p["launches"]}
+        avg = ctx.region_end() / reps
+        out[name] = {"avg_ms": avg, "gbps": nbytes / (avg * 1e-3) / 1e9, "bytes": nbytes, "launches": reps}
     return out
+
+
+def learned_timing(ctx, pc, n, reps=30):
+    """The learned preconditioner applied back to back outside a solver (no
+    stopped launches): per-launch HIP-event time of its series-step kernel,
+    and the whole apply (init + steps) timed by one event pair."""
+    xf = ctx.upload(np.random.default_rng(2).uniform(-1, 1, n).astype(np.float32))
+    yf = ctx.empty(n, np.float32)
+    for _ in range(3):
+        pc.apply(xf, yf)
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(reps):
+        pc.apply(xf, yf)
+    ctx.profile(False)
+    p = ctx.profile_read(ctx.PROF_LEARNED)
+    ctx.region_begin()
+    for _ in range(reps):
+        pc.apply(xf, yf)
+    apply_ms = ctx.region_end() / reps
+    return {"step_avg_ms": p["avg_ms"], "step_gbps": p["gbps"], "step_bytes": p["bytes"] // max(p["launches"], 1),
+            "step_launches": p["launches"], "apply_ms": apply_ms,
+            "apply_gbps": pc.info().apply_bytes / (apply_ms * 1e-3) / 1e9, "apply_bytes": pc.info().apply_bytes}
 
 
 def cpu_baseline(s, gpu_iterations):
@@ -354,7 +393,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", choices=["spmv", "copy", "learned"], default=None)
-    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32")
+    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
+                    help="matrix storage of the inner FGMRES (SpMV and ILU fallback)")
+    ap.add_argument("--pc-storage", choices=["f32", "bf16"], default=None,
+                    help="matrix storage the learned preconditioner's series reads (default: --storage)")
+    ap.add_argument("--orth", type=int, default=0, help="Gram-Schmidt of the inner FGMRES (0 CGS2, 1 one-sync)")
     ap.add_argument("--workload", choices=["headline", "batched", "bsr"], default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -377,6 +420,7 @@ def main():
         r = ctx.empty(n)
         bnorm = ctx.norm(b)
         storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.storage]
+        pc_storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.pc_storage or args.storage]
 
         setup = {}
         t0 = time.perf_counter()
@@ -389,10 +433,11 @@ def main():
         setup["ilu0_low_s"] = time.perf_counter() - t0
         model = owtk.GnnModel.load(ctx, MODEL)
         t0 = time.perf_counter()
-        learned = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=storage)
+        learned = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=pc_storage)
         ctx.synchronize()
         setup["learned_s"] = time.perf_counter() - t0
-        plain = owtk.Preconditioner(m, owtk.PC_LEARNED, model=owtk.GnnModel(ctx, BS, model.degree), storage=storage)
+        plain = owtk.Preconditioner(m, owtk.PC_LEARNED, model=owtk.GnnModel(ctx, BS, model.degree),
+                                    storage=pc_storage)
         info = ilu64.info()
         log(f"[bench] set-up: ILU(0) fp64 {setup['ilu0_fp64_s']:.2f}s, low {setup['ilu0_low_s']:.2f}s "
             f"({info.blocks} blocks, levels fwd {info.forward_levels}); learned (network inference) "
@@ -408,6 +453,7 @@ def main():
         def refined(label, pc, fallback=None):
             ropts = owtk.refinement_options(inner_storage=storage, inner_restart=30, inner_relative_tolerance=1e-4,
                                             relative_tolerance=RTOL)
+            ropts.outer.gmres_orthogonalization = args.orth
             t, res, prof = timed_solves(ctx, label, args.steps, args.warmup,
                                         lambda: owtk.fgmres_refined(m, pc, b, x, ropts, fallback=fallback), x)
             rel = true_rel(ctx, m, b, x, r, bnorm)
@@ -432,19 +478,27 @@ def main():
             f"bf16 {sp['bf16']['avg_ms'] * 1e3:.1f} us {sp['bf16']['gbps']:.0f} GB/s")
         ms64 = float(np.median(t64))
         headline = head["ms"]
-        # the headline solve's dominant profiled kernel
-        kname, kp = max(((k, v) for k, v in profh.items() if v["launches"]), key=lambda kv: kv[1]["total_ms"])
+        # The headline solve's dominant kernel is the learned series step.  Its
+        # in-solve event times include launches made no-ops by the solver's stop
+        # flag, so the roofline is taken from back-to-back applies instead.
+        lt = learned_timing(ctx, learned, n)
+        log(f"[bench] learned series step {lt['step_avg_ms'] * 1e3:.1f} us ({lt['step_gbps']:.0f} GB/s), "
+            f"apply {lt['apply_ms'] * 1e3:.1f} us ({lt['apply_gbps']:.0f} GB/s)")
+        traffic = TRAFFIC_BYTES.get(("learned", args.pc_storage or args.storage))
         roof = {
-            "kernel": KERNEL_NAMES[kname] + " (largest share of the headline solve)",
-            "bound": "hbm", "achieved": round(kp["gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-            "frac": round(kp["gbps"] / PEAK_HBM_GBPS, 4), "traffic": None,
-            "avg_launch_ms": round(kp["avg_ms"], 5), "bytes_per_launch": int(kp["bytes"] // max(kp["launches"], 1)),
-            "launches_per_solve": kp["launches"] // max(args.steps, 1),
-            "share_of_solve_ms": round(kp["total_ms"] / max(args.steps, 1), 3),
+            "kernel": KERNEL_NAMES["learned"] + " (largest share of the headline solve)",
+            "bound": "hbm", "achieved": round(lt["step_gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(lt["step_gbps"] / PEAK_HBM_GBPS, 4), "traffic": traffic,
+            "avg_launch_ms": round(lt["step_avg_ms"], 5), "bytes_per_launch": lt["step_bytes"],
+            "timing": "HIP event pair per launch, 30 back-to-back applies outside the solver",
+            "apply_ms": round(lt["apply_ms"], 5), "apply_gbps": round(lt["apply_gbps"], 1),
+            "in_solve_launches": profh["learned"]["launches"],
         }
         spmv_roof = {"kernel": KERNEL_NAMES["spmv_f64"] + " (dedicated loop)", "achieved": round(sp["f64"]["gbps"], 1),
                      "peak": PEAK_HBM_GBPS, "frac": round(sp["f64"]["gbps"] / PEAK_HBM_GBPS, 4),
-                     "avg_launch_ms": round(sp["f64"]["avg_ms"], 5), "bytes_per_launch": sp["f64"]["bytes"]}
+                     "avg_launch_ms": round(sp["f64"]["avg_ms"], 5), "bytes_per_launch": sp["f64"]["bytes"],
+                     "traffic": TRAFFIC_BYTES.get(("spmv", "f64")),
+                     "timing": "one HIP event pair around 50 back-to-back launches"}
         cpu = None
         if not args.no_cpu:
             spmv_s, setup_s, iter_s = cpu_baseline(s, res64[-1]["iterations"])
@@ -460,7 +514,8 @@ def main():
             "metric": METRIC, "value": round(headline, 3), "unit": "ms", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(headline, 3),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-            "dtype": f"{args.storage} storage, f32 inner arithmetic, f64 refinement", "data": "synthetic",
+            "dtype": f"{args.storage} matrix / {args.pc_storage or args.storage} preconditioner storage, f32 inner "
+                     f"arithmetic, f64 refinement", "data": "synthetic",
             "config": {"workload": "2048x1024 triangulated grid, 2,097,152 vertices, block 3, split view, "
                                    "Hilbert order, seed 0; rtol 1e-8 (true FP64 residual)",
                        "headline": f"FP64-refined FGMRES(30), {args.storage} matrix storage, learned vertex-block "

@@ -78,6 +78,9 @@ int owtk_profile_enable(owtk_context ctx, int enable);
 int owtk_profile_reset(owtk_context ctx);
 /* Sum over recorded launches of one category: launch count, total kernel
  * milliseconds, total algorithmic HBM bytes. */
+/* Region timing: one HIP event pair on the context stream around whatever is
+ * launched between begin (begin = 1) and end (begin = 0, returns ms). */
+int owtk_profile_region(owtk_context ctx, int begin, double* ms);
 int owtk_profile_read(owtk_context ctx, int category, uint64_t* launches, double* total_ms,
                       int64_t* bytes);
 

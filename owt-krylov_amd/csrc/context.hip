@@ -176,6 +176,28 @@ int owtk_profile_read(owtk_context ctx, int category, uint64_t* launches, double
     });
 }
 
+int owtk_profile_region(owtk_context ctx, int begin, double* ms)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->activate();
+        if (!ctx->region_start) {
+            OWTK_HIP(hipEventCreate(&ctx->region_start));
+            OWTK_HIP(hipEventCreate(&ctx->region_stop));
+        }
+        if (begin) {
+            OWTK_HIP(hipEventRecord(ctx->region_start, ctx->stream));
+            return;
+        }
+        if (!ms) throw std::invalid_argument("null output");
+        OWTK_HIP(hipEventRecord(ctx->region_stop, ctx->stream));
+        OWTK_HIP(hipEventSynchronize(ctx->region_stop));
+        float v = 0.0f;
+        OWTK_HIP(hipEventElapsedTime(&v, ctx->region_start, ctx->region_stop));
+        *ms = v;
+    });
+}
+
 int owtk_context_destroy(owtk_context ctx)
 {
     return guarded([&] {
@@ -187,6 +209,8 @@ int owtk_context_destroy(owtk_context ctx)
             (void)hipEventDestroy(r.stop);
         }
         for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+        if (ctx->region_start) (void)hipEventDestroy(ctx->region_start);
+        if (ctx->region_stop) (void)hipEventDestroy(ctx->region_stop);
         hipStream_t st = ctx->stream;
         delete ctx;
         (void)hipStreamDestroy(st);

@@ -101,8 +101,28 @@ __global__ __launch_bounds__(256) void k_fg_v0(T* __restrict__ v0, const T* __re
         v0[i] = static_cast<T>(static_cast<double>(r[i]) / beta);
 }
 
+// L consecutive elements moved as one 16-byte access (L = 1: scalar).  The
+// passes walk packets; the host picks L > 1 only when n and the basis stride
+// are multiples of L (so every packet is 16-byte aligned).
+template<class T, int L>
+struct alignas(sizeof(T) * L) Packet {
+    T v[L];
+};
+
+template<class T, int L>
+__device__ inline Packet<T, L> load_packet(const T* p)
+{
+    return *reinterpret_cast<const Packet<T, L>*>(p);
+}
+
+template<class T, int L>
+__device__ inline void store_packet(T* p, const Packet<T, L>& x)
+{
+    *reinterpret_cast<Packet<T, L>*>(p) = x;
+}
+
 // Pass A: h_j = <w, v_j>, j <= k, and <w, w>.
-template<int K, class T>
+template<int K, class T, int L>
 __global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std::int64_t ld, int count,
                                                    const T* __restrict__ w, std::int64_t n, FgState* s, int k,
                                                    ReduceScratch scratch)
@@ -112,12 +132,24 @@ __global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std:
 #pragma unroll
     for (int j = 0; j < K; ++j) red[j] = 0.0;
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
-        const double wi = static_cast<double>(w[i]);
+    const std::int64_t packets = n / L;
+    for (std::int64_t ip = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ip < packets; ip += step) {
+        const std::int64_t i = ip * L;
+        const Packet<T, L> wp = load_packet<T, L>(w + i);
 #pragma unroll
-        for (int j = 0; j < K - 1; ++j)
-            if (j < count) red[j] = __builtin_fma(wi, static_cast<double>(V[j * ld + i]), red[j]);
-        red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
+        for (int j = 0; j < K - 1; ++j) {
+            if (j < count) {
+                const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    red[j] = __builtin_fma(static_cast<double>(wp.v[l]), static_cast<double>(vp.v[l]), red[j]);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const double wi = static_cast<double>(wp.v[l]);
+            red[K - 1] = __builtin_fma(wi, wi, red[K - 1]);
+        }
     }
     dev::store_partials<K>(red, 0, scratch);
 }
@@ -150,7 +182,7 @@ struct FinA {
 };
 
 // Pass B: w -= sum h_j v_j; with re-orthogonalisation also c_j = <w, v_j>, <w, w>.
-template<int K, class T>
+template<int K, class T, int L>
 __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std::int64_t ld, int count,
                                                    T* __restrict__ w, std::int64_t n, FgState* s, int k,
                                                    ReduceScratch scratch)
@@ -164,17 +196,40 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
 #pragma unroll
     for (int j = 0; j < K; ++j) red[j] = 0.0;
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
-        double wi = static_cast<double>(w[i]);
-        for (int j = 0; j < count; ++j) wi = __builtin_fma(-h[j], static_cast<double>(V[j * ld + i]), wi);
-        const T wt = static_cast<T>(wi);
-        w[i] = wt;
-        if (reorth) {
-            const double wr = static_cast<double>(wt);
+    const std::int64_t packets = n / L;
+    for (std::int64_t ip = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ip < packets; ip += step) {
+        const std::int64_t i = ip * L;
+        const Packet<T, L> wp = load_packet<T, L>(w + i);
+        double wi[L];
 #pragma unroll
-            for (int j = 0; j < K - 1; ++j)
-                if (j < count) red[j] = __builtin_fma(wr, static_cast<double>(V[j * ld + i]), red[j]);
-            red[K - 1] = __builtin_fma(wr, wr, red[K - 1]);
+        for (int l = 0; l < L; ++l) wi[l] = static_cast<double>(wp.v[l]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (j < count) {
+                const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+#pragma unroll
+                for (int l = 0; l < L; ++l) wi[l] = __builtin_fma(-h[j], static_cast<double>(vp.v[l]), wi[l]);
+            }
+        }
+        Packet<T, L> wt;
+#pragma unroll
+        for (int l = 0; l < L; ++l) wt.v[l] = static_cast<T>(wi[l]);
+        store_packet<T, L>(w + i, wt);
+        if (reorth) {
+#pragma unroll
+            for (int j = 0; j < K - 1; ++j) {
+                if (j < count) {
+                    const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        red[j] = __builtin_fma(static_cast<double>(wt.v[l]), static_cast<double>(vp.v[l]), red[j]);
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const double wr = static_cast<double>(wt.v[l]);
+                red[K - 1] = __builtin_fma(wr, wr, red[K - 1]);
+            }
         }
     }
     dev::store_partials<K>(red, 0, scratch);
@@ -202,7 +257,7 @@ struct FinB {
 
 // Pass C: second projection, then v_{k+1} (direct) or the verified norm; then
 // the Givens rotation and residual estimate (krylov_solvers.hpp:369-433).
-template<int K, class T>
+template<int K, class T, int L>
 __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std::int64_t ld, int count,
                                                    T* __restrict__ w, T* __restrict__ vnext, std::int64_t n,
                                                    FgState* s, int k, ReduceScratch scratch)
@@ -217,19 +272,40 @@ __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std:
     const double inv = s->inv_h;
     double red[2] = {0.0, 0.0};
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
-        double wi = static_cast<double>(w[i]);
-        if (reorth)
-            for (int j = 0; j < count; ++j) wi = __builtin_fma(-c[j], static_cast<double>(V[j * ld + i]), wi);
-        const T wt = static_cast<T>(wi);
+    const std::int64_t packets = n / L;
+    for (std::int64_t ip = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ip < packets; ip += step) {
+        const std::int64_t i = ip * L;
+        const Packet<T, L> wp = load_packet<T, L>(w + i);
+        double wi[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) wi[l] = static_cast<double>(wp.v[l]);
+        if (reorth) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j < count) {
+                    const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+#pragma unroll
+                    for (int l = 0; l < L; ++l) wi[l] = __builtin_fma(-c[j], static_cast<double>(vp.v[l]), wi[l]);
+                }
+            }
+        }
+        Packet<T, L> wt;
+#pragma unroll
+        for (int l = 0; l < L; ++l) wt.v[l] = static_cast<T>(wi[l]);
         if (direct) {
-            vnext[i] = static_cast<T>(static_cast<double>(wt) * inv);
+            Packet<T, L> vn;
+#pragma unroll
+            for (int l = 0; l < L; ++l) vn.v[l] = static_cast<T>(static_cast<double>(wt.v[l]) * inv);
+            store_packet<T, L>(vnext + i, vn);
         } else {
-            w[i] = wt;
+            store_packet<T, L>(w + i, wt);
             if (verify) {
-                const double wr = static_cast<double>(wt);
-                red[0] = __builtin_fma(wr, wr, red[0]);
-                red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(wr));
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    const double wr = static_cast<double>(wt.v[l]);
+                    red[0] = __builtin_fma(wr, wr, red[0]);
+                    red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(wr));
+                }
             }
         }
     }
@@ -292,7 +368,8 @@ template<class T>
 __global__ __launch_bounds__(256) void k_fg_pass_d(const T* __restrict__ w, T* __restrict__ vnext, std::int64_t n,
                                                    const FgState* s)
 {
-    if (s->stop || !s->need_scale) return;
+    // after the cycle has stopped the next basis vector is not needed
+    if (s->halt || s->stop || !s->need_scale) return;
     const double inv = s->inv_h;
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
@@ -375,9 +452,32 @@ struct Counters {
 // The FGMRES algorithm over vectors of type T.  op(x, y, stop) computes y = A x;
 // residual(b, x, r, norm_slot) computes r = b - A x and its norm; pc(in, out,
 // stop) applies the preconditioner.
+// Per-engine device workspace (Krylov basis, state) kept across runs, plus
+// the pinned flags and events of the throttled launch loop.
+struct EngineWork {
+    DeviceArray<unsigned char> vecs;
+    DeviceArray<FgState> dstate;
+    PinnedBuffer flags;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    EngineWork() = default;
+    EngineWork(const EngineWork&) = delete;
+    EngineWork& operator=(const EngineWork&) = delete;
+    ~EngineWork()
+    {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// Columns launched between two looks at the device's stop flag.  The host
+// waits for the check two blocks back, so the GPU always has a block queued
+// while at most two blocks of no-op launches follow a stop.
+constexpr int kCheckEvery = 2;
+
 template<class T>
 struct FgEngine {
     owtk_context ctx;
+    std::unique_ptr<EngineWork> work;
     std::int64_t n = 0;       // owned size
     std::int64_t local = 0;   // local size (SpMV inputs)
     std::function<void(const T*, T*, const int*)> op;
@@ -394,13 +494,24 @@ struct FgEngine {
         const auto scratch = ctx->scratch();
         const int m = static_cast<int>(o->restart);
         const std::int64_t ld = local;
-        DeviceArray<T> vecs(static_cast<std::size_t>((2 * m + 3) * ld));
-        kern::fill_zero_bytes(st, vecs.get(), vecs.bytes());
-        T* r = vecs.get();
+        if (!work) work = std::make_unique<EngineWork>();
+        const std::size_t need = static_cast<std::size_t>((2 * m + 3) * ld) * sizeof(T);
+        if (work->vecs.bytes() < need) {
+            // ghost entries of the basis are never written, so they stay zero
+            work->vecs.allocate(need);
+            kern::fill_zero_bytes(st, work->vecs.get(), work->vecs.bytes());
+        }
+        if (!work->dstate.get()) work->dstate.allocate(1);
+        if (!work->ev[0]) {
+            for (auto& e : work->ev) OWTK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            work->flags.ensure(2 * sizeof(int));
+        }
+        T* r = reinterpret_cast<T*>(work->vecs.get());
         T* w = r + ld;
         T* V = w + ld;                  // m+1 vectors
         T* Z = V + static_cast<std::int64_t>(m + 1) * ld;  // m vectors
-        DeviceArray<FgState> dstate(1);
+        DeviceArray<FgState>& dstate = work->dstate;
+        int* flag = static_cast<int*>(work->flags.ptr);
         ctx->pinned.ensure(sizeof(FgState) + 256);
         auto* hs = static_cast<FgState*>(ctx->pinned.ptr);
         double* d_norm = ctx->scalars.get() + 16;
@@ -441,6 +552,8 @@ struct FgEngine {
         if (initial <= thr) return finish(status_converged);
         double beta = initial;
         const unsigned int g = dev::stream_grid(n, 4);
+        // 1024 blocks: more would multiply the per-block K-value reductions and
+        // the finisher's partial sums faster than they add bandwidth
         std::int64_t red_blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
         if (red_blocks > 1024) red_blocks = 1024;
         if (red_blocks < 1) red_blocks = 1;
@@ -459,6 +572,7 @@ struct FgEngine {
             OWTK_HIP(hipMemcpyAsync(dstate.get(), hs, sizeof(FgState), hipMemcpyHostToDevice, st));
             const std::uint64_t left = o->maximum_iterations - res->iterations;
             const int cols = static_cast<int>(std::min<std::uint64_t>(static_cast<std::uint64_t>(m), left));
+            bool have_prev = false;
             for (int k = 0; k < cols; ++k) {
                 T* vk = V + static_cast<std::int64_t>(k) * ld;
                 T* zk = Z + static_cast<std::int64_t>(k) * ld;
@@ -466,6 +580,17 @@ struct FgEngine {
                 pc(vk, zk, halt);
                 op(zk, w, halt);
                 launch_passes(k, V, ld, w, vnext, dstate.get(), gr, st, scratch);
+                if ((k + 1) % kCheckEvery == 0 && k + 1 < cols) {
+                    // the launches after a stop are no-ops: stop issuing them
+                    const int slot = ((k + 1) / kCheckEvery) & 1;
+                    OWTK_HIP(hipMemcpyAsync(&flag[slot], halt, sizeof(int), hipMemcpyDeviceToHost, st));
+                    OWTK_HIP(hipEventRecord(work->ev[slot], st));
+                    if (have_prev) {
+                        OWTK_HIP(hipEventSynchronize(work->ev[slot ^ 1]));
+                        if (flag[slot ^ 1] != 0) break;
+                    }
+                    have_prev = true;
+                }
             }
             OWTK_HIP(hipMemcpyAsync(hs, dstate.get(), sizeof(FgState), hipMemcpyDeviceToHost, st));
             ctx->sync();
@@ -524,13 +649,22 @@ struct FgEngine {
     void passes(int k, T* V, std::int64_t ld, T* w, T* vnext, FgState* s, unsigned int gr, hipStream_t st,
                 ReduceScratch scratch)
     {
+        constexpr int L = 16 / static_cast<int>(sizeof(T));
+        if (n % L == 0 && ld % L == 0) passes_l<K, L>(k, V, ld, w, vnext, s, gr, st, scratch);
+        else passes_l<K, 1>(k, V, ld, w, vnext, s, gr, st, scratch);
+    }
+
+    template<int K, int L>
+    void passes_l(int k, T* V, std::int64_t ld, T* w, T* vnext, FgState* s, unsigned int gr, hipStream_t st,
+                  ReduceScratch scratch)
+    {
         const int count = k + 1;
         const int* halt = &s->halt;
-        k_fg_pass_a<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        k_fg_pass_a<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
         dev::launch_finish<K>(st, FinA<K>{s, k, count}, gr, 0, scratch, halt);
-        k_fg_pass_b<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
+        k_fg_pass_b<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
         dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt);
-        k_fg_pass_c<K, T><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
+        k_fg_pass_c<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
         dev::launch_finish<2>(st, FinC<T>{s, k, w, n}, gr, 2, scratch, halt);
         k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);
         OWTK_LAUNCH_CHECK();

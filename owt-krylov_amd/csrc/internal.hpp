@@ -107,6 +107,7 @@ struct owtk_context_s {
     bool profiling = false;
     std::vector<owtk::ProfileRecord> records;
     std::vector<hipEvent_t> event_pool;
+    hipEvent_t region_start = nullptr, region_stop = nullptr;  // owtk_profile_region
 
     owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
     void activate() const;  // hipSetDevice(device)

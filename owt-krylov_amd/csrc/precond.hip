@@ -25,6 +25,8 @@ namespace {
 using dev::kBlock;
 
 constexpr double kTol = 64.0 * std::numeric_limits<double>::epsilon();
+// LDS per chain workgroup above which the block's solution stays in global memory
+constexpr std::size_t kChainLdsBudget = 24 * 1024;
 
 enum Flag : int { flag_ok = 0, flag_zero_pivot = 1, flag_singular_diag = 2, flag_no_diag = 3 };
 
@@ -428,7 +430,7 @@ __device__ inline float lane_value(float v, int lane)
 template<TriMode MODE, bool FWD, class X, class V, int EI>
 __device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const std::int32_t* __restrict__ nout,
                                    const std::int32_t* __restrict__ cols, const V* __restrict__ vals,
-                                   const X* __restrict__ dv, X* y, X* tile, int nb, int lane)
+                                   const X* __restrict__ dv, X* y, int ys, X* tile, int nb, int lane)
 {
     for (int cs = 0; cs < nb; cs += 64) {
         const int p = cs + lane;
@@ -444,9 +446,10 @@ __device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const s
             const std::int64_t e1 = ptr[p + 1];
             const std::int64_t em = e0 + nout[p];
             if constexpr (MODE == TriMode::ssor || !FWD) dmine = dv[i];
-            if constexpr (MODE == TriMode::ssor && !FWD) acc = dmine * y[i];
-            else acc = y[i];
-            for (std::int64_t e = e0; e < em; ++e) acc = dev::dfma(-static_cast<X>(vload(vals, e)), y[cols[e]], acc);
+            if constexpr (MODE == TriMode::ssor && !FWD) acc = dmine * y[i * ys];
+            else acc = y[i * ys];
+            for (std::int64_t e = e0; e < em; ++e)
+                acc = dev::dfma(-static_cast<X>(vload(vals, e)), y[static_cast<std::int64_t>(cols[e]) * ys], acc);
 #pragma unroll
             for (int k = 0; k < EI; ++k) {
                 if (em + k < e1) {
@@ -474,7 +477,7 @@ __device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const s
         if constexpr (MODE == TriMode::ssor) mine = acc / dmine;
         else if constexpr (!FWD) mine = acc * dmine;
         else mine = acc;
-        if (valid) y[i] = mine;
+        if (valid) y[i * ys] = mine;
 #pragma unroll
         for (int k = 0; k < EI; ++k)
             if (st[k] >= 0) tile[st[k] * 64 + lane] = X(0);
@@ -482,7 +485,11 @@ __device__ inline void chain_sweep(const std::int64_t* __restrict__ ptr, const s
     }
 }
 
-template<TriMode MODE, class X, class V, int EI>
+// GLOBAL_Y: the block's solution lives in `out` itself (L2-resident, strided by
+// the block size) instead of LDS, so a workgroup needs only the 64 x 64 tile
+// and more of them fit on a CU.  The block's rows belong to this one wave, so
+// no other wave reads or writes them.
+template<TriMode MODE, class X, class V, int EI, bool GLOBAL_Y>
 __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict__ fval, const V* __restrict__ bval,
                                                   const X* __restrict__ dval, const X* in, X* out, X scale,
                                                   const int* stop)
@@ -496,19 +503,30 @@ __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict
     const int nb = a.nrows[b];
     const int lane = threadIdx.x;
     X* tile = reinterpret_cast<X*>(smem_raw);  // [64 steps][64 lanes]
-    X* y = tile + 64 * 64;                     // [nb]
     for (int s = 0; s < 64; ++s) tile[s * 64 + lane] = X(0);
-    for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
+    X* y;
+    int ys;
+    if constexpr (GLOBAL_Y) {
+        y = out + r0 * bs + c;
+        ys = bs;
+        if (in != out)
+            for (int i = lane; i < nb; i += 64) y[i * ys] = in[(r0 + i) * bs + c];
+    } else {
+        y = tile + 64 * 64;  // [nb]
+        ys = 1;
+        for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
+    }
     __syncthreads();
     const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
     chain_sweep<MODE, true, X, V, EI>(a.fptr + r0, a.fnout + r0, a.fcol, fval + static_cast<std::int64_t>(c) * a.fnnz,
-                                      dv, y, tile, nb, lane);
+                                      dv, y, ys, tile, nb, lane);
     chain_sweep<MODE, false, X, V, EI>(a.bptr + r0, a.bnout + r0, a.bcol,
-                                       bval + static_cast<std::int64_t>(c) * a.bnnz, dv, y, tile, nb, lane);
+                                       bval + static_cast<std::int64_t>(c) * a.bnnz, dv, y, ys, tile, nb, lane);
+    const bool scaled = MODE == TriMode::ssor && scale != X(1);
+    if (GLOBAL_Y && !scaled) return;
     for (int i = lane; i < nb; i += 64) {
-        X v = y[i];
-        if constexpr (MODE == TriMode::ssor)
-            if (scale != X(1)) v = v * scale;
+        X v = y[i * ys];
+        if (scaled) v = v * scale;
         out[(r0 + i) * bs + c] = v;
     }
 }
@@ -1148,15 +1166,23 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
     ProfileScope prof(pc->m->ctx, sizeof(X) == 8 ? prof_precond_f64 : prof_precond_low, precond_apply_bytes(pc));
     if (t.chain) {
         const TriArgs args = tri_args(t);
-        const std::size_t chain_lds = need + 64 * 64 * sizeof(X);
-        if (t.ei_max <= 2)
-            k_tri_chain<MODE, X, V, 2><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
-        else if (t.ei_max <= 4)
-            k_tri_chain<MODE, X, V, 4><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
-        else if (t.ei_max <= 8)
-            k_tri_chain<MODE, X, V, 8><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
-        else
-            k_tri_chain<MODE, X, V, 16><<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        // Large blocks keep their solution in global memory (L2): the LDS
+        // then holds only the tile, and more one-wave workgroups fit per CU.
+        const std::size_t tile_lds = 64 * 64 * sizeof(X);
+        const bool global_y = need + tile_lds > kChainLdsBudget;
+        const std::size_t chain_lds = global_y ? tile_lds : need + tile_lds;
+        auto launch = [&](auto ei, auto gy) {
+            k_tri_chain<MODE, X, V, decltype(ei)::value, decltype(gy)::value>
+                <<<groups, 64, chain_lds, st>>>(args, fv, bv, dv, in, out, scale, stop);
+        };
+        auto pick = [&](auto ei) {
+            if (global_y) launch(ei, std::true_type{});
+            else launch(ei, std::false_type{});
+        };
+        if (t.ei_max <= 2) pick(std::integral_constant<int, 2>{});
+        else if (t.ei_max <= 4) pick(std::integral_constant<int, 4>{});
+        else if (t.ei_max <= 8) pick(std::integral_constant<int, 8>{});
+        else pick(std::integral_constant<int, 16>{});
     } else if (need <= lds_limit(pc)) {
         k_tri_apply<MODE, X, V, true><<<groups, threads, need, st>>>(tri_args(t), fv, bv, dv, in, out, scale, stop);
     } else {

@@ -73,6 +73,15 @@ class Context:
     def profile_reset(self) -> None:
         check(lib().owtk_profile_reset(self.handle))
 
+    def region_begin(self) -> None:
+        check(lib().owtk_profile_region(self.handle, 1, None))
+
+    def region_end(self) -> float:
+        """ms on the context stream since region_begin (HIP events)."""
+        ms = C.c_double()
+        check(lib().owtk_profile_region(self.handle, 0, C.byref(ms)))
+        return ms.value
+
     def profile_read(self, category: int) -> dict:
         n, ms, b = C.c_uint64(), C.c_double(), C.c_int64()
         check(lib().owtk_profile_read(self.handle, category, C.byref(n), C.byref(ms), C.byref(b)))

@@ -138,6 +138,7 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "owtk_profile_enable": (C.c_int, [_P, C.c_int]),
     "owtk_profile_reset": (C.c_int, [_P]),
+    "owtk_profile_region": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "owtk_profile_read": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),
     "owtk_context_create": (C.c_int, [C.c_int, C.POINTER(_P)]),

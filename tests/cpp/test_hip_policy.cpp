@@ -62,7 +62,7 @@ Split chain(std::size_t n, std::size_t b)
         for (long j : {long(i) - 1, long(i) + 1}) {
             if (j < 0 || j >= long(n)) continue;
             s.col.push_back(j);
-            for (std::size_t c = 0; c < b; ++c) s.vals.push_back(-1.0 - 0.1 * c - 0.01 * i);
+            for (std::size_t c = 0; c < b; ++c) s.vals.push_back(-1.0 - 0.1 * c - 0.05 * static_cast<double>(i % 7));
         }
         s.off.push_back(static_cast<std::int64_t>(s.col.size()));
     }
