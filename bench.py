@@ -41,14 +41,17 @@ MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
 # kernel's known input; WRITE_SIZE by 1), c1 workload.
-TRAFFIC_BYTES = {("spmv", "f64"): 520_558_592, ("learned", "f32"): 369_938_432}
+TRAFFIC_BYTES = {("spmv", "f64"): 520_558_592, ("learned", "f32"): 369_938_432, ("learned", "bf16"): 295_703_360}
 KERNEL_NAMES = {
+    "learned_bf16": "learned series step, BF16 matrix: off-diagonal SpMV + fused update "
+                    "(spmv_sell_cd<3,bf16,float,false,EpiPoly>)",
     "spmv_f64": "split SpMV, FP64 (spmv_sell_cd<3,double,double,true,EpiStore>)",
     "spmv_f32": "split SpMV, FP32 storage (spmv_sell_cd<3,float,float,...>)",
     "spmv_bf16": "split SpMV, BF16 storage (spmv_sell_cd<3,bf16,float,...>)",
     "precond_f64": "block-Jacobi ILU(0) triangular sweeps, FP64 (k_tri_chain)",
     "precond_low": "block-Jacobi ILU(0) triangular sweeps, low precision (k_tri_chain)",
-    "learned": "learned series step: off-diagonal SpMV + fused update (spmv_sell_cd<3,V,float,false,EpiPoly>)",
+    "learned": "learned series step, FP32 matrix: off-diagonal SpMV + fused update "
+               "(spmv_sell_cd<3,float,float,false,EpiPoly>)",
 }
 
 
@@ -194,7 +197,7 @@ def cpu_baseline(s, gpu_iterations):
     return spmv_s, setup_s, iter_s
 
 
-def kernel_loop(kind, steps):
+def kernel_loop(kind, steps, pc_storage="f32"):
     """Kernel loops for rocprofv3 counter runs: FP64 split SpMV, the learned
     preconditioner apply (FP32 storage), or a copy kernel of known size
     (FP64 -> FP32 conversion: 8 B read, 4 B written)."""
@@ -206,9 +209,10 @@ def kernel_loop(kind, steps):
         yf = ctx.empty(n, np.float32)
         pc = None
         if kind == "learned":
-            m.add_storage(owtk.F32)
+            st = {"f32": owtk.F32, "bf16": owtk.BF16}[pc_storage]
+            m.add_storage(st)
             model = owtk.GnnModel.load(ctx, MODEL)
-            pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=owtk.F32)
+            pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=st)
             xf = ctx.upload(np.random.default_rng(0).uniform(-1, 1, n).astype(np.float32))
         for k in range(steps):
             if kind == "spmv":
@@ -395,9 +399,11 @@ def main():
     ap.add_argument("--kernel", choices=["spmv", "copy", "learned"], default=None)
     ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
                     help="matrix storage of the inner FGMRES (SpMV and ILU fallback)")
-    ap.add_argument("--pc-storage", choices=["f32", "bf16"], default=None,
-                    help="matrix storage the learned preconditioner's series reads (default: --storage)")
-    ap.add_argument("--orth", type=int, default=0, help="Gram-Schmidt of the inner FGMRES (0 CGS2, 1 one-sync)")
+    ap.add_argument("--pc-storage", choices=["f32", "bf16"], default="bf16",
+                    help="matrix storage the learned preconditioner's series reads")
+    ap.add_argument("--orth", type=int, default=2,
+                    help="Gram-Schmidt of the inner FGMRES: 0 iterated CGS, 1 adaptive, 2 one-synchronization "
+                         "(GmresOrthogonalization, core.hpp:32-39); inner products stay FP64")
     ap.add_argument("--workload", choices=["headline", "batched", "bsr"], default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -405,7 +411,7 @@ def main():
               "(the solve shards by vertex partition; see DESIGN.md)", file=sys.stderr)
         return 2
     if args.kernel:
-        kernel_loop(args.kernel, args.steps)
+        kernel_loop(args.kernel, args.steps, args.pc_storage)
         return 0
     if args.workload == "batched":
         return batched_bench(args)
@@ -420,7 +426,7 @@ def main():
         r = ctx.empty(n)
         bnorm = ctx.norm(b)
         storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.storage]
-        pc_storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.pc_storage or args.storage]
+        pc_storage = {"f32": owtk.F32, "bf16": owtk.BF16}[args.pc_storage]
 
         setup = {}
         t0 = time.perf_counter()
@@ -484,9 +490,10 @@ def main():
         lt = learned_timing(ctx, learned, n)
         log(f"[bench] learned series step {lt['step_avg_ms'] * 1e3:.1f} us ({lt['step_gbps']:.0f} GB/s), "
             f"apply {lt['apply_ms'] * 1e3:.1f} us ({lt['apply_gbps']:.0f} GB/s)")
-        traffic = TRAFFIC_BYTES.get(("learned", args.pc_storage or args.storage))
+        traffic = TRAFFIC_BYTES.get(("learned", args.pc_storage))
         roof = {
-            "kernel": KERNEL_NAMES["learned"] + " (largest share of the headline solve)",
+            "kernel": KERNEL_NAMES["learned" if args.pc_storage == "f32" else "learned_bf16"]
+                      + " (largest share of the headline solve)",
             "bound": "hbm", "achieved": round(lt["step_gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(lt["step_gbps"] / PEAK_HBM_GBPS, 4), "traffic": traffic,
             "avg_launch_ms": round(lt["step_avg_ms"], 5), "bytes_per_launch": lt["step_bytes"],
@@ -514,12 +521,14 @@ def main():
             "metric": METRIC, "value": round(headline, 3), "unit": "ms", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(headline, 3),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-            "dtype": f"{args.storage} matrix / {args.pc_storage or args.storage} preconditioner storage, f32 inner "
+            "dtype": f"{args.storage} matrix / {args.pc_storage} preconditioner storage, f32 inner "
                      f"arithmetic, f64 refinement", "data": "synthetic",
             "config": {"workload": "2048x1024 triangulated grid, 2,097,152 vertices, block 3, split view, "
                                    "Hilbert order, seed 0; rtol 1e-8 (true FP64 residual)",
-                       "headline": f"FP64-refined FGMRES(30), {args.storage} matrix storage, learned vertex-block "
-                                   f"preconditioner (degree {model.degree}, {MODEL.name}), ILU(0) fallback",
+                       "headline": f"FP64-refined FGMRES(30) (Gram-Schmidt mode {args.orth}, FP64 inner products), "
+                                   f"{args.storage} matrix storage, learned vertex-block preconditioner on "
+                                   f"{args.pc_storage} matrix storage (degree {model.degree}, {MODEL.name}), "
+                                   f"ILU(0) fallback",
                        "baseline": "FP64 BiCGSTAB + FP64 block-Jacobi ILU(0), 4096-vertex blocks"},
             "fp64_bicgstab_ilu0_ms": round(ms64, 3), "speedup_vs_fp64": round(ms64 / headline, 3),
             "fp64_iterations": res64[-1]["iterations"], "fp64_true_rel": rel64,

@@ -201,6 +201,8 @@ typedef struct owtk_precond_info {
     int64_t backward_levels;
     int64_t factor_entries;   /* stored factor entries (ILU) */
     int64_t apply_bytes;      /* algorithmic HBM bytes of one apply */
+    int64_t rows;             /* owned vertices */
+    int64_t block_size;
 } owtk_precond_info;
 int owtk_precond_info_get(owtk_precond pc, owtk_precond_info* info);
 /* Copy the ILU factors to the host in the reference layout (factor_row_offsets,
@@ -256,6 +258,12 @@ void owtk_solver_options_default(owtk_solver_options* options);
  * vector resident on the device.  pc == NULL means IdentityPreconditioner. */
 int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                   const owtk_solver_options* options, owtk_solver_result* result);
+/* pipelined_bicgstab(op, b, x, options, pc) (pipelined_bicgstab.hpp:20-244):
+ * the fused-reduction variant (three global reductions per iteration), the
+ * recursive residual tested every iteration, replacement every
+ * residual_replacement_interval iterations restarting the recurrence. */
+int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                            const owtk_solver_options* options, owtk_solver_result* result);
 /* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident. */
 int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                 const owtk_solver_options* options, owtk_solver_result* result);
@@ -286,6 +294,21 @@ int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, d
 int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_precond fallback_pc,
                                  const double* b, double* x, const owtk_refinement_options* options,
                                  owtk_solver_result* result);
+
+/* ------------------------------------------------------------ matrix files -- */
+/* The reference's binary fixture format (benchmarks/matrix_analysis/
+ * ilu_fill.cpp:20-44), little-endian: u64 {n, bins, nnz}, u64 offsets[n+1],
+ * u64 columns[nnz], f64 values[nnz*bins] (component-diagonal BlockCsrMatrix,
+ * diagonal included).  Read: call with NULL arrays for the sizes, then with
+ * arrays of n+1 / nnz / nnz*bins elements (int32 indices on the device path). */
+int owtk_read_block_csr(const char* path, int64_t* nodes, int64_t* block_size, int64_t* entries,
+                        int32_t* row_offsets, int32_t* columns, double* values);
+int owtk_write_block_csr(const char* path, int64_t nodes, int64_t block_size, const int32_t* row_offsets,
+                         const int32_t* columns, const double* values);
+/* ILU factors in the format ilu_fill.cpp writes (ilu_fill.cpp:36-41): u64
+ * {n, bins, factor_nnz}, u64 factor offsets, u64 factor columns, f64 factor
+ * values, f64 inverse diagonal. */
+int owtk_write_ilu_factors(owtk_precond pc, const char* path);
 
 /* ------------------------------------------- learned preconditioner (GNN) -- */
 /* No reference counterpart: the reference defers learned preconditioners

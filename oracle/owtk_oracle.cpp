@@ -646,6 +646,13 @@ struct Solve {
         ++r.global_reductions;
         return norm(n, x);
     }
+    // detail::norm_from_squared (krylov_solvers.hpp:95-108): the fused
+    // squared norm unless it over/underflowed, then an extra scaled norm
+    double nfs(const double* x, double squared)
+    {
+        if (std::isfinite(squared) && squared >= std::numeric_limits<double>::min()) return std::sqrt(squared);
+        return nrm(x);
+    }
     // detail::true_residual (krylov_solvers.hpp:62-74)
     void true_residual(const double* b, const double* x, double* res, double* work)
     {
@@ -681,6 +688,133 @@ void init_result(oracle_result& r)
     r.status = invalid_input;
     r.initial_residual_norm = r.recursive_residual_norm = r.true_residual_norm = r.relative_residual_norm =
         std::numeric_limits<double>::quiet_NaN();
+}
+
+// pipelined_bicgstab (pipelined_bicgstab.hpp:20-244): fused scalar products
+// (<shadow, v>; <t, s>, <t, t>, <s, s>; <shadow, r>, <r, r>), a convergence
+// test on the recursive residual every iteration, and residual replacement
+// every residual_replacement_interval iterations (which only restarts).
+void pipelined_bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o,
+                        oracle_result& r)
+{
+    init_result(r);
+    if (!valid_options(o)) return;
+    Solve s(a, pc, r);
+    const std::int64_t L = s.local;
+    const std::int64_t n = s.n;
+    std::vector<double> residual(L), shadow(L), search(L), osearch(L), inter(L), ointer(L), psearch(L), pinter(L),
+        work(L);
+    auto mark = [&](Reason reason) {
+        r.status = breakdown;
+        r.breakdown_reason = reason;
+    };
+    s.true_residual(b, x, residual.data(), work.data());
+    std::copy_n(residual.begin(), n, shadow.begin());
+    std::copy_n(residual.begin(), n, search.begin());
+    // one fused reduction: |b|^2, |r|^2
+    ++r.global_reductions;
+    const double rhs_norm = s.nfs(b, dot(n, b, b));
+    double rnorm = s.nfs(residual.data(), dot(n, residual.data(), residual.data()));
+    const double thr = threshold_of(rhs_norm, o);
+    const double tiny = 1024 * std::numeric_limits<double>::min();
+    r.initial_residual_norm = rnorm;
+    s.set_residual(rnorm, rnorm, rhs_norm);
+    if (!std::isfinite(rnorm) || !std::isfinite(thr)) {
+        mark(non_finite_scalar);
+        return;
+    }
+    if (rnorm <= thr) {
+        r.status = converged;
+        return;
+    }
+    double rho = s.d(shadow.data(), residual.data());
+    double alpha = 1, omega = 1;
+    auto restart = [&] {
+        std::copy_n(residual.begin(), n, shadow.begin());
+        std::copy_n(residual.begin(), n, search.begin());
+        rho = s.d(shadow.data(), residual.data());
+        alpha = omega = 1;
+    };
+    for (std::uint64_t it = 1; it <= o.maximum_iterations; ++it) {
+        r.iterations = it;
+        s.prec(search.data(), psearch.data());
+        s.op(psearch.data(), osearch.data());
+        const double sv = s.d(shadow.data(), osearch.data());
+        if (!std::isfinite(sv) || std::abs(sv) <= tiny) {
+            mark(alpha_denominator);
+            return;
+        }
+        alpha = rho / sv;
+        for (std::int64_t i = 0; i < n; ++i) inter[i] = residual[i] - alpha * osearch[i];
+        s.prec(inter.data(), pinter.data());
+        s.op(pinter.data(), ointer.data());
+        ++r.global_reductions;  // one fused reduction of three products
+        const double num = dot(n, ointer.data(), inter.data());
+        const double den = dot(n, ointer.data(), ointer.data());
+        const double inorm = s.nfs(inter.data(), dot(n, inter.data(), inter.data()));
+        if (!std::isfinite(inorm)) {
+            mark(non_finite_scalar);
+            return;
+        }
+        if (inorm <= thr || !std::isfinite(den) || std::abs(den) <= tiny) {
+            if (inorm <= thr) {
+                for (std::int64_t i = 0; i < n; ++i) x[i] = std::fma(alpha, psearch[i], x[i]);
+                s.true_residual(b, x, residual.data(), work.data());
+                const double tn = s.nrm(residual.data());
+                s.set_residual(inorm, tn, rhs_norm);
+                if (tn <= thr) {
+                    r.status = converged;
+                    return;
+                }
+                restart();
+                continue;
+            }
+            mark(omega_denominator);
+            return;
+        }
+        omega = num / den;
+        const double angle = std::abs(num) / std::sqrt(den) / inorm;
+        if (!std::isfinite(omega) || omega == 0.0 || angle <= o.breakdown_tolerance) {
+            mark(omega_zero);
+            return;
+        }
+        for (std::int64_t i = 0; i < n; ++i) {
+            x[i] = std::fma(alpha, psearch[i], x[i]);
+            x[i] = std::fma(omega, pinter[i], x[i]);
+            residual[i] = inter[i] - omega * ointer[i];
+        }
+        ++r.global_reductions;  // <shadow, r>, <r, r> fused
+        const double prev = rho;
+        rho = dot(n, shadow.data(), residual.data());
+        rnorm = s.nfs(residual.data(), dot(n, residual.data(), residual.data()));
+        r.recursive_residual_norm = rnorm;
+        if (rnorm <= thr) {
+            s.true_residual(b, x, residual.data(), work.data());
+            const double tn = s.nrm(residual.data());
+            s.set_residual(rnorm, tn, rhs_norm);
+            if (tn <= thr) {
+                r.status = converged;
+                return;
+            }
+            restart();
+            continue;
+        }
+        if (!std::isfinite(prev) || std::abs(prev) <= tiny) {
+            mark(biorthogonality_loss);
+            return;
+        }
+        const double beta = (rho / prev) * (alpha / omega);
+        for (std::int64_t i = 0; i < n; ++i) search[i] = residual[i] + beta * (search[i] - omega * osearch[i]);
+        if (o.residual_replacement_interval > 0 && it % o.residual_replacement_interval == 0) {
+            s.true_residual(b, x, residual.data(), work.data());
+            restart();
+        }
+    }
+    s.true_residual(b, x, residual.data(), work.data());
+    const double tn = s.nrm(residual.data());
+    s.set_residual(rnorm, tn, rhs_norm);
+    r.status = tn <= thr ? converged : maximum_iterations;
+    if (!std::isfinite(tn)) mark(non_finite_scalar);
 }
 
 // bicgstab (krylov_solvers.hpp:542-745)
@@ -1124,6 +1258,17 @@ int oracle_bicgstab(const oracle_matrix* m, void* pc_handle, const double* b, do
         Identity id(a.owned * a.bs);
         Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
         bicgstab(a, pc, b, x, *o, *r);
+    });
+}
+
+int oracle_pipelined_bicgstab(const oracle_matrix* m, void* pc_handle, const double* b, double* x,
+                              const oracle_options* o, oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        pipelined_bicgstab(a, pc, b, x, *o, *r);
     });
 }
 

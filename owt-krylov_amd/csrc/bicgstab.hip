@@ -47,7 +47,15 @@ struct BicgState {
     int stage;
     int reason;
     long long stop_iteration;
+    long long extra_reductions;  // pipelined: fused squared norms that fell back to a scaled norm
 };
+
+// norm_from_squared (krylov_solvers.hpp:95-108) charges one more reduction
+// when the fused squared norm is zero, underflows or is not finite
+__device__ __host__ inline bool norm_fallback(double squared)
+{
+    return !(__builtin_isfinite(squared) && squared >= 2.2250738585072014e-308);
+}
 
 __device__ void fin_rho(BicgState* st, double rho, long long it)
 {
@@ -232,6 +240,134 @@ struct FinOmega {
             st->stage = 3;
             st->stop_iteration = st->iteration;
         }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Pipelined BiCGSTAB (pipelined_bicgstab.hpp:20-244).  Same vectors; the
+// scalar products are fused as in the reference: <shadow, v>;
+// <t, s>, <t, t>, <s, s> in the second SpMV's epilogue; <shadow, r>, <r, r>
+// with the x/r update.  The recursive residual is tested every iteration.
+enum PStop : int { pstop_replace = 5 };
+
+// s = r - alpha v (no reduction: |s| comes with the omega products)
+__global__ __launch_bounds__(256) void k_s_plain(double* __restrict__ s, const double* __restrict__ r,
+                                                 const double* __restrict__ v, std::int64_t n, const BicgState* st)
+{
+    if (st->stop) return;
+    const double alpha = st->alpha;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+        s[i] = __builtin_fma(-alpha, v[i], r[i]);
+}
+
+// t = A y epilogue: <t, s>, <t, t>, <s, s>, max |s| -> |s|, early exit, omega
+struct EpiOmegaP : dev::EpiNone {
+    static constexpr int K = 4;
+    static constexpr std::uint64_t kMaxMask = 8;
+    double* y;
+    const double* s;
+    std::int64_t n;
+    BicgState* st;
+    __device__ void elem(std::int64_t i, double tv, double* red) const
+    {
+        y[i] = tv;
+        const double si = s[i];
+        red[0] = __builtin_fma(tv, si, red[0]);
+        red[1] = __builtin_fma(tv, tv, red[1]);
+        red[2] = __builtin_fma(si, si, red[2]);
+        red[3] = dev::combine<dev::Combine::max>(red[3], __builtin_fabs(si));
+    }
+    __device__ void finalize(const double* t) const
+    {
+        // norm_from_squared: the fused square unless it over/underflowed
+        const double snorm = dev::block_norm(t[2], t[3], s, n);
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        st->snorm = snorm;
+        if (norm_fallback(t[2])) st->extra_reductions += 1;
+        const double num = t[0], den = t[1];
+        if (!__builtin_isfinite(snorm)) {
+            st->stop = stop_breakdown;
+            st->reason = reason_non_finite_scalar;
+            st->stage = 3;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        if (snorm <= st->threshold) {
+            st->stop = stop_early;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        if (!__builtin_isfinite(den) || __builtin_fabs(den) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = reason_omega_denominator;
+            st->stage = 3;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        const double omega = num / den;
+        st->omega = omega;
+        const double angle = __builtin_fabs(num) / __builtin_sqrt(den) / snorm;
+        if (!__builtin_isfinite(omega) || omega == 0.0 || angle <= st->breakdown_tolerance) {
+            st->stop = stop_breakdown;
+            st->reason = reason_omega_zero;
+            st->stage = 3;
+            st->stop_iteration = st->iteration;
+        }
+    }
+};
+
+// after x/r: next rho, recursive |r| (tested every iteration), beta, cadence
+struct FinNextP {
+    BicgState* st;
+    const double* r;
+    std::int64_t n;
+    __device__ void operator()(const double* tot) const
+    {
+        const double rn = dev::block_norm(tot[1], tot[2], r, n);
+        __syncthreads();  // every thread has read the state before thread 0 updates it
+        if (threadIdx.x != 0) return;
+        const long long it = st->iteration;
+        const double prev = st->rho;
+        st->rho = tot[0];
+        if (norm_fallback(tot[1])) st->extra_reductions += 1;
+        st->recursive = rn;
+        st->rnorm = rn;
+        if (rn <= st->threshold) {
+            st->stop = stop_cadence;
+            st->stop_iteration = it;
+            return;
+        }
+        if (!__builtin_isfinite(prev) || __builtin_fabs(prev) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = reason_biorthogonality_loss;
+            st->stage = 4;
+            st->stop_iteration = it;
+            return;
+        }
+        st->beta = (tot[0] / prev) * (st->alpha / st->omega);
+        st->first = 0;
+        if (st->replace_interval > 0 && it % st->replace_interval == 0) {
+            st->stop = pstop_replace;
+            st->stop_iteration = it;
+            return;
+        }
+        if (it >= st->max_iterations) {
+            st->stop = stop_max;
+            st->stop_iteration = it;
+            return;
+        }
+        st->iteration = it + 1;
+    }
+};
+
+// rho = <shadow, r> at (re)start, without the BiCGSTAB start-of-iteration guard
+struct FinRhoP {
+    BicgState* st;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x == 0) st->rho = t[0];
     }
 };
 
@@ -463,6 +599,231 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     return finish(status);
 }
 
+
+// pipelined_bicgstab (pipelined_bicgstab.hpp:20-244) with the reference's
+// counters: per full iteration 2 preconditioner applies, 2 operator applies
+// and 3 fused reductions.
+int pipelined_bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* o,
+                           owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    init_result(res);
+    if (!valid_options(o)) return OWTK_OK;
+    if (pc && pc->m->owned * pc->m->bs != m->owned * m->bs)
+        throw std::invalid_argument("preconditioner layout mismatch");
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
+        throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const auto scratch = ctx->scratch();
+    const std::int64_t n = m->owned * m->bs;
+    const std::int64_t L = m->local() * m->bs;
+    Workspace w;
+    w.local = L;
+    w.vecs.allocate(static_cast<std::size_t>(9 * L));
+    kern::fill_zero_bytes(st, w.vecs.get(), w.vecs.bytes());
+    w.state.allocate(1);
+    double* r = w.v(0);
+    double* shadow = w.v(1);
+    double* p = w.v(2);
+    double* z = w.v(3);
+    double* v = w.v(4);
+    double* s = w.v(5);
+    double* yv = w.v(6);
+    double* t = w.v(7);
+    BicgState* dst = w.state.get();
+    ctx->pinned.ensure(sizeof(BicgState) + 64);
+    auto* hst = static_cast<BicgState*>(ctx->pinned.ptr);
+    std::uint64_t ops = 0, pcs = 0, reds = 0;
+    auto read_state = [&] {
+        OWTK_HIP(hipMemcpyAsync(hst, dst, sizeof(BicgState), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+    };
+    auto write_state = [&] { OWTK_HIP(hipMemcpyAsync(dst, hst, sizeof(BicgState), hipMemcpyHostToDevice, st)); };
+    double* d_norms = ctx->scalars.get();
+    double norms[2];
+    auto true_residual = [&](double* norm_slot) {
+        dev::EpiResidual e;
+        e.y = r;
+        e.b = b;
+        e.n = n;
+        e.norm_out = norm_slot;
+        spmv_f64(m, x, e);
+        ++ops;
+    };
+    auto read_norm = [&](double* slot) {
+        OWTK_HIP(hipMemcpyAsync(&norms[0], slot, sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        return norms[0];
+    };
+    std::uint64_t initial_extra = 0;
+    auto finish = [&](int status) {
+        res->status = status;
+        res->operator_applications = ops;
+        res->preconditioner_applications = pcs;
+        res->global_reductions = reds + initial_extra + static_cast<std::uint64_t>(hst->extra_reductions);
+        if (o->collect_timings)
+            res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return OWTK_OK;
+    };
+
+    std::memset(hst, 0, sizeof(BicgState));
+    write_state();
+    true_residual(&d_norms[1]);
+    kern::norm_to(st, scratch, b, n, &d_norms[0]);
+    ++reds;  // |b|^2 and |r|^2 are one fused reduction in the reference
+    OWTK_HIP(hipMemcpyAsync(norms, d_norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double rhs_norm = norms[0], initial = norms[1];
+    // the squares the reference reduces, recovered from the norms
+    initial_extra = static_cast<std::uint64_t>(norm_fallback(rhs_norm * rhs_norm))
+                  + static_cast<std::uint64_t>(norm_fallback(initial * initial));
+    const double thr = threshold_of(rhs_norm, o);
+    res->initial_residual_norm = initial;
+    set_residual(res, initial, initial, rhs_norm);
+    if (!std::isfinite(initial) || !std::isfinite(thr)) {
+        res->breakdown_reason = reason_non_finite_scalar;
+        return finish(status_breakdown);
+    }
+    if (initial <= thr) return finish(status_converged);
+
+    const unsigned int g = dev::stream_grid(n, 4);
+    // shadow = r, p = r (first), rho = <shadow, r>
+    auto restart = [&](long long it) {
+        hst->alpha = 1;
+        hst->omega = 1;
+        hst->first = 1;
+        hst->stop = stop_none;
+        hst->iteration = it;
+        write_state();
+        OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        k_rho<<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+        dev::launch_finish<1>(st, FinRhoP{dst}, g, 0, scratch, nullptr);
+        OWTK_LAUNCH_CHECK();
+        ++reds;
+    };
+    std::memset(hst, 0, sizeof(BicgState));
+    hst->threshold = thr;
+    hst->tiny = 1024 * std::numeric_limits<double>::min();
+    hst->breakdown_tolerance = o->breakdown_tolerance;
+    hst->recursive = initial;
+    hst->max_iterations = static_cast<long long>(o->maximum_iterations);
+    hst->check_interval = 1;
+    hst->replace_interval = static_cast<long long>(o->residual_replacement_interval);
+    restart(1);
+
+    const int* stop = &dst->stop;
+    auto enqueue_iteration = [&] {
+        k_p_update<<<g, kBlock, 0, st>>>(p, r, v, n, dst);  // p = r (first) or r + beta (p - omega v)
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, p, z, n, st, stop);
+        {
+            dev::EpiDot1<FinAlpha> e;
+            e.y = v;
+            e.w = shadow;
+            e.stop = stop;
+            e.fin = FinAlpha{dst};
+            spmv_f64(m, z, e);
+        }
+        k_s_plain<<<g, kBlock, 0, st>>>(s, r, v, n, dst);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, s, yv, n, st, stop);
+        {
+            EpiOmegaP e;
+            e.y = t;
+            e.s = s;
+            e.n = n;
+            e.st = dst;
+            e.stop = stop;
+            spmv_f64(m, yv, e);
+        }
+        k_xr_update<<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+        dev::launch_finish<3>(st, FinNextP{dst, r, n}, g, 4, scratch, stop);
+        OWTK_LAUNCH_CHECK();
+    };
+    const long long maxit = static_cast<long long>(o->maximum_iterations);
+    constexpr long long kBatch = 8;  // iterations between looks at the device state
+    auto count_full = [&](long long a, long long b_incl) {
+        if (b_incl < a) return;
+        const auto k = static_cast<std::uint64_t>(b_incl - a + 1);
+        pcs += 2 * k;
+        ops += 2 * k;
+        reds += 3 * k;
+    };
+    long long it = 1;
+    while (true) {
+        const long long end = std::min(maxit, it + kBatch - 1);
+        for (long long k = it; k <= end; ++k) enqueue_iteration();
+        read_state();
+        const int code = hst->stop;
+        const long long j = hst->stop_iteration;
+        if (code == stop_none) {
+            count_full(it, end);
+            res->iterations = static_cast<std::uint64_t>(end);
+            res->recursive_residual_norm = hst->recursive;
+            it = end + 1;
+            continue;
+        }
+        res->iterations = static_cast<std::uint64_t>(j);
+        if (code == stop_breakdown) {
+            count_full(it, j - 1);
+            if (hst->stage == 1) { pcs += 1; ops += 1; reds += 1; }
+            else if (hst->stage == 3) { pcs += 2; ops += 2; reds += 2; }
+            else count_full(j, j);
+            res->breakdown_reason = hst->reason;
+            res->recursive_residual_norm = hst->recursive;
+            return finish(status_breakdown);
+        }
+        if (code == stop_max) {
+            count_full(it, j);
+            res->recursive_residual_norm = hst->recursive;
+            break;
+        }
+        if (code == pstop_replace) {
+            count_full(it, j);
+            res->recursive_residual_norm = hst->recursive;
+            true_residual(&d_norms[2]);  // replacement: no convergence test, restart
+            it = j + 1;
+            restart(it);  // the reference restarts (one reduction) even after the last iteration
+            if (it > maxit) break;
+            continue;
+        }
+        double rec;
+        if (code == stop_early) {
+            count_full(it, j - 1);
+            pcs += 2;
+            ops += 2;
+            reds += 2;
+            kern::axpy(st, n, hst->alpha, z, x);
+            rec = hst->snorm;
+        } else {
+            count_full(it, j);
+            rec = hst->rnorm;
+        }
+        true_residual(&d_norms[2]);
+        ++reds;
+        const double tn = read_norm(&d_norms[2]);
+        set_residual(res, rec, tn, rhs_norm);
+        if (!std::isfinite(tn)) {
+            res->breakdown_reason = reason_non_finite_scalar;
+            return finish(status_breakdown);
+        }
+        if (tn <= thr) return finish(status_converged);
+        it = j + 1;
+        restart(it);
+        if (it > maxit) break;
+    }
+    true_residual(&d_norms[3]);
+    ++reds;
+    const double tn = read_norm(&d_norms[3]);
+    set_residual(res, res->recursive_residual_norm, tn, rhs_norm);
+    int status = tn <= thr ? status_converged : status_maximum_iterations;
+    if (!std::isfinite(tn)) {
+        status = status_breakdown;
+        res->breakdown_reason = reason_non_finite_scalar;
+    }
+    return finish(status);
+}
 } // namespace owtk
 
 extern "C" int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
@@ -472,5 +833,15 @@ extern "C" int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, do
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
         m->ctx->activate();
         bicgstab_run(m, pc, b, x, options, result);
+    });
+}
+
+extern "C" int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                       const owtk_solver_options* options, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        pipelined_bicgstab_run(m, pc, b, x, options, result);
     });
 }

@@ -153,16 +153,30 @@ __global__ __launch_bounds__(256) void k_finish(Fin fin, unsigned int nblocks, s
 {
     if (stop != nullptr && *stop != 0) return;
     __shared__ double totals[K > 0 ? K : 1];
+    // The partials were written by other XCDs, so each load waits a full
+    // memory latency.  A thread issues all loads of a chunk (U blocks x K
+    // values) before its first combine; the combine order per thread stays
+    // b = t, t + 256, ..., so totals do not depend on U.
+    constexpr int U = K >= 32 ? 1 : (K >= 16 ? 2 : 4);
     double acc[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const bool is_max = (max_mask >> k) & 1;
-        double a = 0.0;
-        for (unsigned int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-            const double p = scratch.partials[k * kMaxReduceBlocks + b];
-            a = is_max ? combine<Combine::max>(a, p) : combine<Combine::sum>(a, p);
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (unsigned int base = threadIdx.x; base < nblocks; base += U * blockDim.x) {
+        double p[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned int b = base + u * blockDim.x;
+#pragma unroll
+            for (int k = 0; k < K; ++k) p[u][k] = b < nblocks ? scratch.partials[k * kMaxReduceBlocks + b] : 0.0;
         }
-        acc[k] = a;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (base + u * blockDim.x >= nblocks) break;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                acc[k] = ((max_mask >> k) & 1) ? combine<Combine::max>(acc[k], p[u][k])
+                                               : combine<Combine::sum>(acc[k], p[u][k]);
+        }
     }
     block_reduce<K>(acc, max_mask);
     if (threadIdx.x == 0) {

@@ -1436,6 +1436,8 @@ int owtk_precond_info_get(owtk_precond pc, owtk_precond_info* info)
         info->backward_levels = pc->tri.max_blevels;
         info->factor_entries = pc->ilu.nnz;
         info->apply_bytes = precond_apply_bytes(pc);
+        info->rows = pc->m->owned;
+        info->block_size = pc->m->bs;
     });
 }
 

@@ -19,7 +19,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
                    UnsupportedError, ZeroPivotError, check, lib)
 
 __all__ = [
-    "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "fgmres",
+    "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab", "fgmres",
     "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
@@ -439,6 +439,10 @@ class Preconditioner:
         check(lib().owtk_precond_info_get(self.handle, C.byref(info)))
         return info
 
+    def write_factors(self, path) -> None:
+        """ILU factors in the reference's ilu_fill.cpp output format."""
+        check(lib().owtk_write_ilu_factors(self.handle, str(path).encode()))
+
     def ilu_factors(self):
         n = C.c_int64()
         check(lib().owtk_precond_ilu_factors(self.handle, C.byref(n), None, None, None, None))
@@ -484,6 +488,12 @@ def _solve(fn, matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: De
 
 def bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
     return _solve(lib().owtk_bicgstab, matrix, pc, b, x, options)
+
+
+def pipelined_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
+                       options=None) -> dict:
+    """pipelined_bicgstab (pipelined_bicgstab.hpp:20-244) on the device."""
+    return _solve(lib().owtk_pipelined_bicgstab, matrix, pc, b, x, options)
 
 
 def fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
@@ -572,6 +582,26 @@ def generate_bsr(nx: int, ny: int, block: int, seed: int = 0, **kw):
     rhs = np.zeros(n * block)
     check(lib().owtk_grid_generate_bsr(C.byref(spec), _ptr(offsets), _ptr(columns), _ptr(values), _ptr(rhs)))
     return n, offsets, columns, values, rhs
+
+
+def read_block_csr(path) -> tuple:
+    """(nodes, block, row_offsets, columns, values) of a system file in the
+    reference's binary fixture format (owtk_read_block_csr)."""
+    n, b, e = C.c_int64(), C.c_int64(), C.c_int64()
+    p = str(path).encode()
+    check(lib().owtk_read_block_csr(p, C.byref(n), C.byref(b), C.byref(e), None, None, None))
+    off = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(e.value, np.int32)
+    val = np.zeros(e.value * b.value)
+    check(lib().owtk_read_block_csr(p, C.byref(n), C.byref(b), C.byref(e), _ptr(off), _ptr(col), _ptr(val)))
+    return n.value, b.value, off, col, val
+
+
+def write_block_csr(path, nodes: int, block: int, offsets, columns, values) -> None:
+    o = np.ascontiguousarray(offsets, np.int32)
+    c = np.ascontiguousarray(columns, np.int32)
+    v = np.ascontiguousarray(values, np.float64)
+    check(lib().owtk_write_block_csr(str(path).encode(), nodes, block, _ptr(o), _ptr(c), _ptr(v)))
 
 
 def split_to_csr(nodes: int, block: int, offsets, columns, diagonal, off_diagonal):

@@ -72,7 +72,8 @@ class PrecondInfo(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("storage", C.c_int32), ("blocks", C.c_int64),
         ("forward_levels", C.c_int64), ("backward_levels", C.c_int64),
-        ("factor_entries", C.c_int64), ("apply_bytes", C.c_int64),
+        ("factor_entries", C.c_int64), ("apply_bytes", C.c_int64), ("rows", C.c_int64),
+        ("block_size", C.c_int64),
     ]
 
 
@@ -170,6 +171,7 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_precond_destroy": (C.c_int, [_P]),
     "owtk_solver_options_default": (None, [C.POINTER(SolverOptions)]),
     "owtk_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
+    "owtk_pipelined_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_fgmres": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_refinement_options_default": (None, [C.POINTER(RefinementOptions)]),
     "owtk_fgmres_refined": (C.c_int, [_P, _P, _P, _P, C.POINTER(RefinementOptions), C.POINTER(SolverResult)]),
@@ -185,6 +187,10 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_batch_set_tile": (C.c_int, [_P, C.c_int32]),
     "owtk_batch_autotune": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), _DP]),
     "owtk_batch_bicgstab": (C.c_int, [_P, _P, _P, C.POINTER(SolverOptions), C.c_int32, C.POINTER(BatchResult)]),
+    "owtk_read_block_csr": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), _P, _P, _P]),
+    "owtk_write_block_csr": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, _P]),
+    "owtk_write_ilu_factors": (C.c_int, [_P, C.c_char_p]),
     "owtk_gnn_create": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),
     "owtk_gnn_destroy": (C.c_int, [_P]),
     "owtk_gnn_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

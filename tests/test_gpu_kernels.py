@@ -309,3 +309,32 @@ def test_dense_ssor_matches_oracle(ctx, bs, omega):
         err = np.max(np.abs(yl.download() - ref)) / np.max(np.abs(ref))
         print(f"  storage {storage}: {err:.2e}")
         assert err <= tol
+
+
+def test_ilu_factor_file_matches_oracle(ctx, tmp_path):
+    """System read from a reference-format file, ILU(0) factored on the GPU and
+    written in the ilu_fill.cpp output format; the file's factors equal the
+    oracle's (IluLevelPreconditioner restated)."""
+    s = owtk.generate_split(40, 24, 3, 5, ordering=owtk.ORDER_RCM)
+    o, c, v = s.csr()
+    sysfile = tmp_path / "system.bin"
+    owtk.write_block_csr(sysfile, s.nodes, 3, o, c, v)
+    n, b, o2, c2, v2 = owtk.read_block_csr(sysfile)
+    m = owtk.Matrix.csr(ctx, n, 0, b, o2, c2, v2)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU)
+    factfile = tmp_path / "factors.bin"
+    pc.write_factors(factfile)
+    raw = factfile.read_bytes()
+    hn, hb, nnz = np.frombuffer(raw[:24], np.uint64).tolist()
+    pos = 24
+    off = np.frombuffer(raw[pos:pos + 8 * (hn + 1)], np.uint64); pos += 8 * (hn + 1)
+    col = np.frombuffer(raw[pos:pos + 8 * nnz], np.uint64); pos += 8 * nnz
+    val = np.frombuffer(raw[pos:pos + 8 * nnz * hb], np.float64); pos += 8 * nnz * hb
+    inv = np.frombuffer(raw[pos:], np.float64)
+    om = orc.OMatrix.csr(n, 0, b, o2, c2, v2)
+    ref = orc.OPreconditioner(om, orc.OPreconditioner.ILU)
+    ro, rc_, rv, ri = ref.ilu_factors()
+    assert (hn, hb) == (n, b)
+    assert np.array_equal(off, ro) and np.array_equal(col, rc_)
+    assert np.max(np.abs(val - rv)) <= 1e-13 * np.max(np.abs(rv))
+    assert np.max(np.abs(inv - ri)) <= 1e-13 * np.max(np.abs(ri))

@@ -291,3 +291,41 @@ def test_dense_ssor_textbook_form(omega):
     pc = orc.OPreconditioner(om, orc.OPreconditioner.SSOR, omega=omega)
     z = pc.apply(r)
     assert np.max(np.abs(z - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_pipelined_bicgstab_dense_recurrence():
+    """Oracle pipelined BiCGSTAB on the audit's 12 x 12 cyclic matrix
+    (test_bicgstab_ilu_audit.cpp:104-131) against a long-double dense
+    restatement of the pipelined recurrence (pipelined_bicgstab.hpp:86-244):
+    the first four iterates agree, and the solve converges."""
+    import refcases as rc
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    nrow = a.shape[0]
+    A = a.astype(np.longdouble)
+    x = np.array(x0, np.longdouble)
+    r = rhs.astype(np.longdouble) - A @ x
+    shadow = r.copy()
+    p = r.copy()
+    rho = shadow @ r
+    xs = []
+    for _ in range(4):
+        v = A @ p
+        alpha = rho / (shadow @ v)
+        s = r - alpha * v
+        t = A @ s
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        prev, rho = rho, shadow @ r
+        beta = (rho / prev) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        xs.append(x.astype(np.float64).copy())
+    om = orc.OMatrix.csr(nrow, 0, 1, np.array(offsets, np.int32), np.array(columns, np.int32), np.array(values))
+    for k, xr in enumerate(xs, start=1):
+        xo, res = orc.pipelined_bicgstab(om, None, rhs, x0=x0, relative_tolerance=0.0, maximum_iterations=k,
+                                         residual_replacement_interval=0)
+        assert res["iterations"] == k
+        assert np.allclose(xo[:nrow], xr, rtol=1e-12, atol=1e-14), k
+    xo, res = orc.pipelined_bicgstab(om, None, rhs, x0=x0, relative_tolerance=1e-12)
+    assert res["status"] == 0
+    assert np.linalg.norm(rhs - a.astype(np.float64) @ xo[:nrow]) <= 1.1e-12 * np.linalg.norm(rhs)
