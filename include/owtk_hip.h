@@ -309,6 +309,14 @@ int owtk_write_block_csr(const char* path, int64_t nodes, int64_t block_size, co
  * {n, bins, factor_nnz}, u64 factor offsets, u64 factor columns, f64 factor
  * values, f64 inverse diagonal. */
 int owtk_write_ilu_factors(owtk_precond pc, const char* path);
+/* read_state (benchmarks/specwave_limon.py:66-100): the application's per-rank
+ * final state files, "SWSTATE1" + u64 {width, owned, ns, nd, count} + u64
+ * ids[owned] + f32/f64 values[owned][ns*nd].  The files together must cover
+ * global vertices 0..count-1 exactly once with finite, non-negative values;
+ * the same malformed inputs are rejected.  values == NULL: *vertices only;
+ * else values receives count*ns*nd doubles ordered by global id. */
+int owtk_read_state(const char* const* paths, int32_t files, int64_t ns, int64_t nd, int64_t* vertices,
+                    double* values);
 
 /* ------------------------------------------- learned preconditioner (GNN) -- */
 /* No reference counterpart: the reference defers learned preconditioners

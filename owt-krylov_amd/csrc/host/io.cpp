@@ -6,12 +6,15 @@
 //            u64 factor_columns[factor_nnz], f64 factor_values[factor_nnz * bins],
 //            f64 inverse_diagonal[n * bins]
 // so application-exported systems run on the device path and the device ILU
-// factors can be compared with the reference's own dumps.
+// factors can be compared with the reference's own dumps; and the
+// application's per-rank final state files (SWSTATE1).
 #include "../owtk_hip_api.hpp"
 #include "host.hpp"
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -136,6 +139,71 @@ int owtk_write_ilu_factors(owtk_precond pc, const char* path)
         file.write(buf.data(), buf.size());
         file.write(val.data(), val.size());
         file.write(inv.data(), inv.size());
+    });
+}
+
+// Per-rank final state files of the application (specwave_limon.py:66-100):
+//   "SWSTATE1", u64 {width, owned, ns, nd, count}, u64 ids[owned],
+//   f{width} values[owned][ns*nd]
+// The files of all ranks together must cover global vertices 0..count-1
+// exactly once; values must be finite and non-negative (a spectrum).  The
+// output is ordered by global id, FP32 files widened to FP64.
+int owtk_read_state(const char* const* paths, int32_t files, int64_t ns, int64_t nd, int64_t* vertices,
+                    double* values)
+{
+    return owtk::guarded([&] {
+        if (!paths || !vertices || files < 0) throw std::invalid_argument("null argument");
+        if (files == 0) throw std::invalid_argument("no final state files");
+        if (ns <= 0 || nd <= 0) throw std::invalid_argument("invalid state dimensions");
+        const std::uint64_t comps = static_cast<std::uint64_t>(ns) * static_cast<std::uint64_t>(nd);
+        std::uint64_t global = 0, width = 0;
+        std::vector<std::uint64_t> ids;
+        std::vector<double> vals;
+        for (int32_t f = 0; f < files; ++f) {
+            if (!paths[f]) throw std::invalid_argument("null path");
+            File file(paths[f], "rb");
+            char magic[8];
+            if (std::fread(magic, 1, 8, file.f) != 8 || std::memcmp(magic, "SWSTATE1", 8) != 0)
+                throw std::invalid_argument(std::string("invalid state magic: ") + paths[f]);
+            std::uint64_t h[5];
+            if (std::fread(h, sizeof(std::uint64_t), 5, file.f) != 5)
+                throw std::invalid_argument(std::string("truncated state header: ") + paths[f]);
+            const std::uint64_t w = h[0], owned = h[1], count = h[4];
+            if ((w != 4 && w != 8) || h[2] != static_cast<std::uint64_t>(ns) || h[3] != static_cast<std::uint64_t>(nd)
+                || count == 0 || owned > count || (global && count != global) || (width && w != width))
+                throw std::invalid_argument(std::string("inconsistent state dimensions: ") + paths[f]);
+            const unsigned long long expect = 48ULL + owned * (8ULL + comps * w);
+            if (static_cast<unsigned long long>(file.size()) != expect)
+                throw std::invalid_argument(std::string("invalid state file size: ") + paths[f]);
+            global = count;
+            width = w;
+            const std::size_t first = ids.size();
+            ids.resize(first + owned);
+            file.read(ids.data() + first, owned);
+            const std::size_t vfirst = vals.size();
+            vals.resize(vfirst + owned * comps);
+            if (w == 8) {
+                file.read(vals.data() + vfirst, owned * comps);
+            } else {
+                std::vector<float> tmp(owned * comps);
+                file.read(tmp.data(), tmp.size());
+                for (std::size_t i = 0; i < tmp.size(); ++i) vals[vfirst + i] = tmp[i];
+            }
+        }
+        if (ids.size() != global) throw std::invalid_argument("final state does not cover every global vertex exactly once");
+        std::vector<std::int64_t> where(global, -1);
+        for (std::size_t i = 0; i < ids.size(); ++i) {
+            if (ids[i] >= global || where[ids[i]] >= 0)
+                throw std::invalid_argument("final state does not cover every global vertex exactly once");
+            where[ids[i]] = static_cast<std::int64_t>(i);
+        }
+        for (const double v : vals)
+            if (!std::isfinite(v) || v < 0) throw std::invalid_argument("nonfinite or negative final spectrum");
+        *vertices = static_cast<int64_t>(global);
+        if (!values) return;  // size query
+        for (std::uint64_t g = 0; g < global; ++g)
+            std::memcpy(values + g * comps, vals.data() + static_cast<std::uint64_t>(where[g]) * comps,
+                        comps * sizeof(double));
     });
 }
 

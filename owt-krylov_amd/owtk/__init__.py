@@ -20,7 +20,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
 
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab", "fgmres",
-    "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "device_count",
+    "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
 ]
@@ -602,6 +602,19 @@ def write_block_csr(path, nodes: int, block: int, offsets, columns, values) -> N
     c = np.ascontiguousarray(columns, np.int32)
     v = np.ascontiguousarray(values, np.float64)
     check(lib().owtk_write_block_csr(str(path).encode(), nodes, block, _ptr(o), _ptr(c), _ptr(v)))
+
+
+def read_state(paths, ns: int, nd: int) -> np.ndarray:
+    """The application's per-rank final state files (SWSTATE1,
+    specwave_limon.py:66-100) as one [vertices, ns*nd] FP64 array ordered by
+    global vertex id (owtk_read_state)."""
+    paths = [str(p).encode() for p in paths]
+    arr = (C.c_char_p * max(1, len(paths)))(*paths)
+    n = C.c_int64()
+    check(lib().owtk_read_state(arr, len(paths), ns, nd, C.byref(n), None))
+    out = np.zeros((n.value, ns * nd))
+    check(lib().owtk_read_state(arr, len(paths), ns, nd, C.byref(n), _ptr(out)))
+    return out
 
 
 def split_to_csr(nodes: int, block: int, offsets, columns, diagonal, off_diagonal):

@@ -191,6 +191,8 @@ PROTOTYPES: dict[str, tuple] = {
                                       C.POINTER(C.c_int64), _P, _P, _P]),
     "owtk_write_block_csr": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, _P]),
     "owtk_write_ilu_factors": (C.c_int, [_P, C.c_char_p]),
+    "owtk_read_state": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, C.c_int64, C.c_int64, C.POINTER(C.c_int64),
+                                  _P]),
     "owtk_gnn_create": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),
     "owtk_gnn_destroy": (C.c_int, [_P]),
     "owtk_gnn_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

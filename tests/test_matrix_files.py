@@ -69,3 +69,55 @@ def test_malformed_files_rejected(tmp_path, damage):
     path.write_bytes(bytes(raw))
     with pytest.raises(owtk.InvalidArgumentError):
         owtk.read_block_csr(path)
+
+
+# --------------------------------------------------------------- state files --
+# The cases of the reference's own check of read_state
+# (benchmarks/test_specwave_limon.py:38-66), on files written here the same way.
+def _state(path, ids, values, width=8, ns=1, nd=2):
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"SWSTATE1" + struct.pack("=5Q", width, len(ids), ns, nd, 2))
+        np.asarray(ids, dtype="=u8").tofile(f)
+        np.asarray(values, dtype=f"=f{width}").tofile(f)
+    return path
+
+
+def test_state_reordered_owned_spectrum(tmp_path):
+    first = _state(tmp_path / "rank0.bin", [1], [3, 4])
+    second = _state(tmp_path / "rank1.bin", [0], [1, 2])
+    assert np.array_equal(owtk.read_state([first, second], 1, 2), [[1, 2], [3, 4]])
+
+
+def test_state_float_spectrum(tmp_path):
+    single = tmp_path / "single.bin"
+    import struct
+    with open(single, "wb") as f:
+        f.write(b"SWSTATE1" + struct.pack("=5Q", 4, 2, 1, 2, 2))
+        np.asarray([0, 1], dtype="=u8").tofile(f)
+        np.asarray([1, 2, 3, 4], dtype="=f4").tofile(f)
+    assert np.array_equal(owtk.read_state([single], 1, 2), [[1, 2], [3, 4]])
+
+
+@pytest.mark.parametrize("case", ["missing rank", "duplicate global IDs", "wrong dimensions", "nan", "infinity",
+                                  "negative", "truncated-header", "truncated-data", "extra-data", "wrong-magic"])
+def test_state_rejected(tmp_path, case):
+    first = _state(tmp_path / "rank0.bin", [1], [3, 4])
+    second = _state(tmp_path / "rank1.bin", [0], [1, 2])
+    if case == "missing rank":
+        files, dims = [first], (1, 2)
+    elif case == "duplicate global IDs":
+        files, dims = [first, first], (1, 2)
+    elif case == "wrong dimensions":
+        files, dims = [first, second], (2, 2)
+    elif case in ("nan", "infinity", "negative"):
+        bad = {"nan": [1, np.nan], "infinity": [1, np.inf], "negative": [1, -1]}[case]
+        files, dims = [first, _state(tmp_path / "bad.bin", [0], bad)], (1, 2)
+    else:
+        data = {"truncated-header": b"SWSTATE1", "truncated-data": first.read_bytes()[:-1],
+                "extra-data": first.read_bytes() + b"X", "wrong-magic": b"invalid!"}[case]
+        path = tmp_path / "bad.bin"
+        path.write_bytes(data)
+        files, dims = [path], (1, 2)
+    with pytest.raises(owtk.OwtkError):
+        owtk.read_state(files, *dims)
