@@ -264,6 +264,13 @@ int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
  * residual_replacement_interval iterations restarting the recurrence. */
 int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                             const owtk_solver_options* options, owtk_solver_result* result);
+/* communication_hiding_bicgstab(op, b, x, options, pc)
+ * (pipelined_bicgstab.hpp:317-597): the 15-vector Cools/PETSc recurrence,
+ * two fused reductions per iteration, each followed by one preconditioner +
+ * operator application; false convergence and replacement restart the
+ * recurrence (a failed restart is a biorthogonality_loss breakdown). */
+int owtk_communication_hiding_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                       const owtk_solver_options* options, owtk_solver_result* result);
 /* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident. */
 int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                 const owtk_solver_options* options, owtk_solver_result* result);

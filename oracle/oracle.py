@@ -74,6 +74,8 @@ def lib():
         L.oracle_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_fgmres.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_pipelined_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
+                                                           C.POINTER(Result)]
         _lib = L
     return _lib
 
@@ -210,6 +212,17 @@ def pipelined_bicgstab(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, 
     r = Result()
     _check(lib().oracle_pipelined_bicgstab(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o),
                                            C.byref(r)))
+    return x, r.as_dict()
+
+
+def communication_hiding_bicgstab(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    r = Result()
+    _check(lib().oracle_communication_hiding_bicgstab(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x),
+                                                      C.byref(o), C.byref(r)))
     return x, r.as_dict()
 
 

@@ -817,6 +817,183 @@ void pipelined_bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, con
     if (!std::isfinite(tn)) mark(non_finite_scalar);
 }
 
+// communication_hiding_bicgstab (pipelined_bicgstab.hpp:317-597): the
+// 15-vector Cools/PETSc recurrence with two fused reductions per iteration
+// (<q,y>, <y,y>, <q,q>; <r,r>, <r,rp>, <s,rp>, <w,rp>, <z,rp>), each followed
+// by one preconditioner + operator application, and a restart of the
+// recurrence (two reductions, two applications) after a false convergence or
+// a residual replacement.
+void communication_hiding_bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o,
+                                   oracle_result& res)
+{
+    init_result(res);
+    if (!valid_options(o)) return;
+    Solve sv(a, pc, res);
+    const std::int64_t L = sv.local;
+    const std::int64_t n = sv.n;
+    std::vector<double> r(L), rp(L), s(L), s2(L), p2(L), r2(L), w(L), w2(L), z(L), z2(L), t(L), v(L), q(L), q2(L),
+        y(L), work(L);
+    auto mark = [&](Reason reason) {
+        res.status = breakdown;
+        res.breakdown_reason = reason;
+    };
+    sv.true_residual(b, x, r.data(), work.data());
+    std::copy_n(r.begin(), n, rp.begin());
+    ++res.global_reductions;  // |b|^2, |r|^2 fused
+    const double rhs_norm = sv.nfs(b, dot(n, b, b));
+    double rnorm = sv.nfs(r.data(), dot(n, r.data(), r.data()));
+    const double thr = threshold_of(rhs_norm, o);
+    const double tiny = 1024 * std::numeric_limits<double>::min();
+    res.initial_residual_norm = rnorm;
+    sv.set_residual(rnorm, rnorm, rhs_norm);
+    if (!std::isfinite(rnorm) || !std::isfinite(thr)) {
+        mark(non_finite_scalar);
+        return;
+    }
+    if (rnorm <= thr) {
+        res.status = converged;
+        return;
+    }
+    double rho = 0, alpha = 0, beta = 0, omega = 0;
+    auto initialize = [&]() -> bool {
+        std::copy_n(r.begin(), n, rp.begin());
+        rho = sv.d(r.data(), rp.data());
+        if (!std::isfinite(rho) || std::abs(rho) <= tiny) return false;
+        sv.prec(r.data(), r2.data());
+        sv.op(r2.data(), w.data());
+        const double d2 = sv.d(w.data(), rp.data());
+        if (!std::isfinite(d2) || std::abs(d2) <= tiny) return false;
+        sv.prec(w.data(), w2.data());
+        sv.op(w2.data(), t.data());
+        alpha = rho / d2;
+        beta = 0;
+        omega = 0;
+        return true;
+    };
+    if (!initialize()) {
+        mark(biorthogonality_loss);
+        return;
+    }
+    bool first = true;
+    for (std::uint64_t it = 1; it <= o.maximum_iterations; ++it) {
+        res.iterations = it;
+        if (first) {
+            std::copy_n(r2.begin(), n, p2.begin());
+            std::copy_n(w.begin(), n, s.begin());
+            std::copy_n(w2.begin(), n, s2.begin());
+            std::copy_n(t.begin(), n, z.begin());
+            first = false;
+        } else {
+            for (std::int64_t i = 0; i < n; ++i) {
+                p2[i] = r2[i] + beta * (p2[i] - omega * s2[i]);
+                s[i] = w[i] + beta * (s[i] - omega * z[i]);
+                s2[i] = w2[i] + beta * (s2[i] - omega * z2[i]);
+                z[i] = t[i] + beta * (z[i] - omega * v[i]);
+            }
+        }
+        for (std::int64_t i = 0; i < n; ++i) {
+            q[i] = r[i] - alpha * s[i];
+            q2[i] = r2[i] - alpha * s2[i];
+            y[i] = w[i] - alpha * z[i];
+        }
+        ++res.global_reductions;  // <q,y>, <y,y>, <q,q> fused
+        const double qy = dot(n, q.data(), y.data());
+        const double yy = dot(n, y.data(), y.data());
+        const double qq = dot(n, q.data(), q.data());
+        sv.prec(z.data(), z2.data());
+        sv.op(z2.data(), v.data());
+        const double inorm = sv.nfs(q.data(), qq);
+        if (!std::isfinite(inorm)) {
+            mark(non_finite_scalar);
+            return;
+        }
+        if (inorm <= thr || !std::isfinite(yy) || std::abs(yy) <= tiny) {
+            if (inorm <= thr) {
+                for (std::int64_t i = 0; i < n; ++i) x[i] = std::fma(alpha, p2[i], x[i]);
+                sv.true_residual(b, x, r.data(), work.data());
+                const double tn = sv.nrm(r.data());
+                sv.set_residual(inorm, tn, rhs_norm);
+                if (tn <= thr) {
+                    res.status = converged;
+                    return;
+                }
+                if (!initialize()) {
+                    mark(biorthogonality_loss);
+                    return;
+                }
+                first = true;
+                continue;
+            }
+            mark(std::isfinite(yy) ? omega_denominator : non_finite_scalar);
+            return;
+        }
+        omega = qy / yy;
+        const double angle = std::abs(qy) / std::sqrt(yy) / inorm;
+        if (!std::isfinite(omega) || omega == 0.0 || angle <= o.breakdown_tolerance) {
+            mark(omega_zero);
+            return;
+        }
+        for (std::int64_t i = 0; i < n; ++i) x[i] = std::fma(alpha, p2[i], x[i]);
+        for (std::int64_t i = 0; i < n; ++i) x[i] = std::fma(omega, q2[i], x[i]);
+        for (std::int64_t i = 0; i < n; ++i) {
+            r[i] = q[i] - omega * y[i];
+            r2[i] = q2[i] - omega * (w2[i] - alpha * z2[i]);
+            w[i] = y[i] - omega * (t[i] - alpha * v[i]);
+        }
+        const double prev = rho;
+        ++res.global_reductions;  // five products fused
+        const double rr = dot(n, r.data(), r.data());
+        rho = dot(n, r.data(), rp.data());
+        const double srp = dot(n, s.data(), rp.data());
+        const double wrp = dot(n, w.data(), rp.data());
+        const double zrp = dot(n, z.data(), rp.data());
+        sv.prec(w.data(), w2.data());
+        sv.op(w2.data(), t.data());
+        rnorm = sv.nfs(r.data(), rr);
+        res.recursive_residual_norm = rnorm;
+        if (rnorm <= thr) {
+            sv.true_residual(b, x, r.data(), work.data());
+            const double tn = sv.nrm(r.data());
+            sv.set_residual(rnorm, tn, rhs_norm);
+            if (tn <= thr) {
+                res.status = converged;
+                return;
+            }
+            if (!initialize()) {
+                mark(biorthogonality_loss);
+                return;
+            }
+            first = true;
+            continue;
+        }
+        if (o.residual_replacement_interval > 0 && it % o.residual_replacement_interval == 0) {
+            sv.true_residual(b, x, r.data(), work.data());
+            if (!initialize()) {
+                mark(biorthogonality_loss);
+                return;
+            }
+            first = true;
+            continue;
+        }
+        if (!std::isfinite(rho) || !std::isfinite(prev) || std::abs(rho) <= tiny || std::abs(prev) <= tiny) {
+            mark(std::isfinite(rho) && std::isfinite(prev) ? biorthogonality_loss : non_finite_scalar);
+            return;
+        }
+        beta = (rho / prev) * (alpha / omega);
+        const double den = wrp + beta * srp - beta * omega * zrp;
+        if (!std::isfinite(den) || std::abs(den) <= tiny) {
+            mark(std::isfinite(den) ? alpha_denominator : non_finite_scalar);
+            return;
+        }
+        alpha = rho / den;
+    }
+    sv.true_residual(b, x, r.data(), work.data());
+    const double tn = sv.nrm(r.data());
+    sv.set_residual(rnorm, tn, rhs_norm);
+    res.status = tn <= thr ? converged : maximum_iterations;
+    if (!std::isfinite(tn)) mark(non_finite_scalar);
+}
+
 // bicgstab (krylov_solvers.hpp:542-745)
 void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
 {
@@ -1269,6 +1446,17 @@ int oracle_pipelined_bicgstab(const oracle_matrix* m, void* pc_handle, const dou
         Identity id(a.owned * a.bs);
         Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
         pipelined_bicgstab(a, pc, b, x, *o, *r);
+    });
+}
+
+int oracle_communication_hiding_bicgstab(const oracle_matrix* m, void* pc_handle, const double* b, double* x,
+                                         const oracle_options* o, oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        communication_hiding_bicgstab(a, pc, b, x, *o, *r);
     });
 }
 

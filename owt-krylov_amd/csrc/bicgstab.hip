@@ -371,6 +371,247 @@ struct FinRhoP {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Communication-hiding BiCGSTAB (pipelined_bicgstab.hpp:317-597).  Fifteen
+// vectors; per iteration one streaming pass updates the four auxiliary
+// directions and forms q, q2, y with their three products, and one pass
+// updates x, r, r2, w with five products.  The preconditioner + operator
+// pair that the reference overlaps with each reduction runs right after its
+// finisher, on the same stream.
+enum ChStop : int { chstop_init = 6 };
+
+struct ChState {
+    double rho, alpha, beta, omega;
+    double threshold, tiny, breakdown_tolerance;
+    double qnorm, rnorm, recursive;
+    long long iteration, max_iterations, replace_interval;
+    int first;
+    int stop;
+    int stage;  // 1: first reduction, 2: second reduction, 10/11: restart rho / <w, rp>
+    int reason;
+    long long stop_iteration;
+    long long extra_reductions;
+};
+
+struct ChVecs {
+    double *r, *rp, *s, *s2, *p2, *r2, *w, *w2, *z, *z2, *t, *v, *q, *q2, *y;
+};
+
+// restart: rp = r and <r, rp>
+__global__ __launch_bounds__(256) void k_ch_rho(const double* __restrict__ r, double* __restrict__ rp, std::int64_t n,
+                                                ReduceScratch scratch)
+{
+    double red[1] = {0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double ri = r[i];
+        rp[i] = ri;
+        red[0] = __builtin_fma(ri, ri, red[0]);
+    }
+    dev::store_partials<1>(red, 0, scratch);
+}
+
+struct FinChRho {
+    ChState* st;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x != 0) return;
+        const double rho = t[0];
+        st->rho = rho;
+        if (!__builtin_isfinite(rho) || __builtin_fabs(rho) <= st->tiny) {
+            st->stop = chstop_init;
+            st->stage = 10;
+            st->stop_iteration = st->iteration;
+        }
+    }
+};
+
+// restart: <A M r, rp> -> alpha
+struct FinChD2 {
+    ChState* st;
+    __device__ void operator()(double d2) const
+    {
+        if (!__builtin_isfinite(d2) || __builtin_fabs(d2) <= st->tiny) {
+            st->stop = chstop_init;
+            st->stage = 11;
+            st->stop_iteration = st->iteration;
+            return;
+        }
+        st->alpha = st->rho / d2;
+        st->beta = 0.0;
+        st->omega = 0.0;
+        st->first = 1;
+    }
+};
+
+// direction updates (or the restart copies), q, q2, y; <q,y>, <y,y>, <q,q>, max|q|
+__global__ __launch_bounds__(256) void k_ch_first(ChVecs V, std::int64_t n, const ChState* st, ReduceScratch scratch)
+{
+    if (st->stop) return;
+    const bool first = st->first != 0;
+    const double alpha = st->alpha, beta = st->beta, omega = st->omega;
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double r2 = V.r2[i], w = V.w[i], w2 = V.w2[i], t = V.t[i];
+        double p2, s, s2, z;
+        if (first) {
+            p2 = r2;
+            s = w;
+            s2 = w2;
+            z = t;
+        } else {
+            p2 = __builtin_fma(beta, __builtin_fma(-omega, V.s2[i], V.p2[i]), r2);
+            s = __builtin_fma(beta, __builtin_fma(-omega, V.z[i], V.s[i]), w);
+            s2 = __builtin_fma(beta, __builtin_fma(-omega, V.z2[i], V.s2[i]), w2);
+            z = __builtin_fma(beta, __builtin_fma(-omega, V.v[i], V.z[i]), t);
+        }
+        V.p2[i] = p2;
+        V.s[i] = s;
+        V.s2[i] = s2;
+        V.z[i] = z;
+        const double q = __builtin_fma(-alpha, s, V.r[i]);
+        const double y = __builtin_fma(-alpha, z, w);
+        V.q[i] = q;
+        V.q2[i] = __builtin_fma(-alpha, s2, r2);
+        V.y[i] = y;
+        red[0] = __builtin_fma(q, y, red[0]);
+        red[1] = __builtin_fma(y, y, red[1]);
+        red[2] = __builtin_fma(q, q, red[2]);
+        red[3] = dev::combine<dev::Combine::max>(red[3], __builtin_fabs(q));
+    }
+    dev::store_partials<4>(red, 8, scratch);
+}
+
+struct FinChFirst {
+    ChState* st;
+    const double* q;
+    std::int64_t n;
+    __device__ void operator()(const double* t) const
+    {
+        const double qn = dev::block_norm(t[2], t[3], q, n);
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        st->qnorm = qn;
+        if (norm_fallback(t[2])) st->extra_reductions += 1;
+        const long long it = st->iteration;
+        const double qy = t[0], yy = t[1];
+        if (!__builtin_isfinite(qn)) {
+            st->stop = stop_breakdown;
+            st->reason = reason_non_finite_scalar;
+            st->stage = 1;
+            st->stop_iteration = it;
+            return;
+        }
+        if (qn <= st->threshold) {
+            st->stop = stop_early;
+            st->stop_iteration = it;
+            return;
+        }
+        if (!__builtin_isfinite(yy) || __builtin_fabs(yy) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(yy) ? reason_omega_denominator : reason_non_finite_scalar;
+            st->stage = 1;
+            st->stop_iteration = it;
+            return;
+        }
+        const double omega = qy / yy;
+        st->omega = omega;
+        const double angle = __builtin_fabs(qy) / __builtin_sqrt(yy) / qn;
+        if (!__builtin_isfinite(omega) || omega == 0.0 || angle <= st->breakdown_tolerance) {
+            st->stop = stop_breakdown;
+            st->reason = reason_omega_zero;
+            st->stage = 1;
+            st->stop_iteration = it;
+            return;
+        }
+        st->first = 0;
+    }
+};
+
+// x += alpha p2; x += omega q2; r, r2, w; <r,r>, max|r|, <r,rp>, <s,rp>, <w,rp>, <z,rp>
+__global__ __launch_bounds__(256) void k_ch_second(ChVecs V, double* __restrict__ x, std::int64_t n,
+                                                   const ChState* st, ReduceScratch scratch)
+{
+    if (st->stop) return;
+    const double alpha = st->alpha, omega = st->omega;
+    double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double xi = __builtin_fma(alpha, V.p2[i], x[i]);
+        x[i] = __builtin_fma(omega, V.q2[i], xi);
+        const double y = V.y[i];
+        const double r = __builtin_fma(-omega, y, V.q[i]);
+        V.r[i] = r;
+        V.r2[i] = __builtin_fma(-omega, __builtin_fma(-alpha, V.z2[i], V.w2[i]), V.q2[i]);
+        const double w = __builtin_fma(-omega, __builtin_fma(-alpha, V.v[i], V.t[i]), y);
+        V.w[i] = w;
+        const double rp = V.rp[i];
+        red[0] = __builtin_fma(r, r, red[0]);
+        red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(r));
+        red[2] = __builtin_fma(r, rp, red[2]);
+        red[3] = __builtin_fma(V.s[i], rp, red[3]);
+        red[4] = __builtin_fma(w, rp, red[4]);
+        red[5] = __builtin_fma(V.z[i], rp, red[5]);
+    }
+    dev::store_partials<6>(red, 2, scratch);
+}
+
+struct FinChSecond {
+    ChState* st;
+    const double* r;
+    std::int64_t n;
+    __device__ void operator()(const double* t) const
+    {
+        const double rn = dev::block_norm(t[0], t[1], r, n);
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        if (norm_fallback(t[0])) st->extra_reductions += 1;
+        const long long it = st->iteration;
+        const double prev = st->rho;
+        const double rho = t[2];
+        st->rho = rho;
+        st->rnorm = rn;
+        st->recursive = rn;
+        if (rn <= st->threshold) {
+            st->stop = stop_cadence;
+            st->stop_iteration = it;
+            return;
+        }
+        if (st->replace_interval > 0 && it % st->replace_interval == 0) {
+            st->stop = pstop_replace;
+            st->stop_iteration = it;
+            return;
+        }
+        if (!__builtin_isfinite(rho) || !__builtin_isfinite(prev) || __builtin_fabs(rho) <= st->tiny
+            || __builtin_fabs(prev) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(rho) && __builtin_isfinite(prev) ? reason_biorthogonality_loss
+                                                                             : reason_non_finite_scalar;
+            st->stage = 2;
+            st->stop_iteration = it;
+            return;
+        }
+        const double beta = (rho / prev) * (st->alpha / st->omega);
+        st->beta = beta;
+        const double den = t[4] + beta * t[3] - beta * st->omega * t[5];
+        if (!__builtin_isfinite(den) || __builtin_fabs(den) <= st->tiny) {
+            st->stop = stop_breakdown;
+            st->reason = __builtin_isfinite(den) ? reason_alpha_denominator : reason_non_finite_scalar;
+            st->stage = 2;
+            st->stop_iteration = it;
+            return;
+        }
+        st->alpha = rho / den;
+        if (it >= st->max_iterations) {
+            st->stop = stop_max;
+            st->stop_iteration = it;
+            return;
+        }
+        st->iteration = it + 1;
+    }
+};
+
 struct Workspace {
     DeviceArray<double> vecs;  // 8 owned-size vectors (+ ghosts for SpMV inputs)
     DeviceArray<BicgState> state;
@@ -816,7 +1057,250 @@ int pipelined_bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, doub
     true_residual(&d_norms[3]);
     ++reds;
     const double tn = read_norm(&d_norms[3]);
-    set_residual(res, res->recursive_residual_norm, tn, rhs_norm);
+    set_residual(res, hst->recursive, tn, rhs_norm);  // the last recursive |r| (not an early-exit |s|)
+    int status = tn <= thr ? status_converged : status_maximum_iterations;
+    if (!std::isfinite(tn)) {
+        status = status_breakdown;
+        res->breakdown_reason = reason_non_finite_scalar;
+    }
+    return finish(status);
+}
+
+// communication_hiding_bicgstab (pipelined_bicgstab.hpp:317-597) with the
+// reference's counters: per full iteration 2 preconditioner applies, 2
+// operator applies and 2 fused reductions; a restart of the recurrence 2 of
+// each.
+int communication_hiding_bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                      const owtk_solver_options* o, owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    init_result(res);
+    if (!valid_options(o)) return OWTK_OK;
+    if (pc && pc->m->owned * pc->m->bs != m->owned * m->bs)
+        throw std::invalid_argument("preconditioner layout mismatch");
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
+        throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const auto scratch = ctx->scratch();
+    const std::int64_t n = m->owned * m->bs;
+    const std::int64_t L = m->local() * m->bs;
+    DeviceArray<double> vecs;
+    vecs.allocate(static_cast<std::size_t>(15 * L));
+    kern::fill_zero_bytes(st, vecs.get(), vecs.bytes());
+    auto vec = [&](int k) { return vecs.get() + static_cast<std::int64_t>(k) * L; };
+    ChVecs V{vec(0), vec(1), vec(2), vec(3), vec(4), vec(5), vec(6), vec(7),
+             vec(8), vec(9), vec(10), vec(11), vec(12), vec(13), vec(14)};
+    DeviceArray<ChState> state;
+    state.allocate(1);
+    ChState* dst = state.get();
+    ctx->pinned.ensure(sizeof(ChState) + 64);
+    auto* hst = static_cast<ChState*>(ctx->pinned.ptr);
+    std::uint64_t ops = 0, pcs = 0, reds = 0, initial_extra = 0;
+    auto read_state = [&] {
+        OWTK_HIP(hipMemcpyAsync(hst, dst, sizeof(ChState), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+    };
+    auto write_state = [&] { OWTK_HIP(hipMemcpyAsync(dst, hst, sizeof(ChState), hipMemcpyHostToDevice, st)); };
+    double* d_norms = ctx->scalars.get();
+    double norms[2];
+    auto true_residual = [&](double* norm_slot) {
+        dev::EpiResidual e;
+        e.y = V.r;
+        e.b = b;
+        e.n = n;
+        e.norm_out = norm_slot;
+        spmv_f64(m, x, e);
+        ++ops;
+    };
+    auto read_norm = [&](double* slot) {
+        OWTK_HIP(hipMemcpyAsync(&norms[0], slot, sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        return norms[0];
+    };
+    auto finish = [&](int status) {
+        res->status = status;
+        res->operator_applications = ops;
+        res->preconditioner_applications = pcs;
+        res->global_reductions = reds + initial_extra + static_cast<std::uint64_t>(hst->extra_reductions);
+        if (o->collect_timings)
+            res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return OWTK_OK;
+    };
+
+    std::memset(hst, 0, sizeof(ChState));
+    write_state();
+    true_residual(&d_norms[1]);
+    kern::norm_to(st, scratch, b, n, &d_norms[0]);
+    ++reds;  // |b|^2 and |r|^2 are one fused reduction in the reference
+    OWTK_HIP(hipMemcpyAsync(norms, d_norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double rhs_norm = norms[0], initial = norms[1];
+    initial_extra = static_cast<std::uint64_t>(norm_fallback(rhs_norm * rhs_norm))
+                  + static_cast<std::uint64_t>(norm_fallback(initial * initial));
+    const double thr = threshold_of(rhs_norm, o);
+    res->initial_residual_norm = initial;
+    set_residual(res, initial, initial, rhs_norm);
+    if (!std::isfinite(initial) || !std::isfinite(thr)) {
+        res->breakdown_reason = reason_non_finite_scalar;
+        return finish(status_breakdown);
+    }
+    if (initial <= thr) return finish(status_converged);
+
+    std::memset(hst, 0, sizeof(ChState));
+    hst->threshold = thr;
+    hst->tiny = 1024 * std::numeric_limits<double>::min();
+    hst->breakdown_tolerance = o->breakdown_tolerance;
+    hst->recursive = initial;
+    hst->max_iterations = static_cast<long long>(o->maximum_iterations);
+    hst->replace_interval = static_cast<long long>(o->residual_replacement_interval);
+    const unsigned int g = dev::stream_grid(n, 4);
+    const int* stop = &dst->stop;
+    auto plain_op = [&](const double* in, double* out) {
+        dev::EpiStore<double> e;
+        e.y = out;
+        e.stop = stop;
+        spmv_f64(m, in, e);
+    };
+    // initialize_recurrence: rp = r, rho = <r, rp>, r2 = M r, w = A r2,
+    // alpha = rho / <w, rp>, w2 = M w, t = A w2.  Returns false on a
+    // breakdown, with the reference's counts of what ran.
+    auto initialize = [&](long long it) {
+        hst->stop = stop_none;
+        hst->iteration = it;
+        write_state();
+        k_ch_rho<<<g, kBlock, 0, st>>>(V.r, V.rp, n, scratch);
+        dev::launch_finish<1>(st, FinChRho{dst}, g, 0, scratch, nullptr);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, V.r, V.r2, n, st, stop);
+        {
+            dev::EpiDot1<FinChD2> e;
+            e.y = V.w;
+            e.w = V.rp;
+            e.stop = stop;
+            e.fin = FinChD2{dst};
+            spmv_f64(m, V.r2, e);
+        }
+        apply_pc_f64(pc, V.w, V.w2, n, st, stop);
+        plain_op(V.w2, V.t);
+        read_state();
+        if (hst->stop == chstop_init) {
+            reds += hst->stage == 10 ? 1 : 2;
+            if (hst->stage == 11) {
+                pcs += 1;
+                ops += 1;
+            }
+            return false;
+        }
+        reds += 2;
+        pcs += 2;
+        ops += 2;
+        return true;
+    };
+    if (!initialize(1)) {
+        res->breakdown_reason = reason_biorthogonality_loss;
+        return finish(status_breakdown);
+    }
+
+    auto enqueue_iteration = [&] {
+        k_ch_first<<<g, kBlock, 0, st>>>(V, n, dst, scratch);
+        dev::launch_finish<4>(st, FinChFirst{dst, V.q, n}, g, 8, scratch, stop);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, V.z, V.z2, n, st, stop);
+        plain_op(V.z2, V.v);
+        k_ch_second<<<g, kBlock, 0, st>>>(V, x, n, dst, scratch);
+        dev::launch_finish<6>(st, FinChSecond{dst, V.r, n}, g, 2, scratch, stop);
+        OWTK_LAUNCH_CHECK();
+        apply_pc_f64(pc, V.w, V.w2, n, st, stop);
+        plain_op(V.w2, V.t);
+    };
+    auto count_full = [&](long long a, long long b_incl) {
+        if (b_incl < a) return;
+        const auto k = static_cast<std::uint64_t>(b_incl - a + 1);
+        pcs += 2 * k;
+        ops += 2 * k;
+        reds += 2 * k;
+    };
+    // restart after a false convergence or a replacement (the reference's
+    // `continue`): a failed restart is a biorthogonality breakdown
+    auto restart_or_fail = [&](long long next) { return initialize(next); };
+    const long long maxit = static_cast<long long>(o->maximum_iterations);
+    constexpr long long kBatch = 8;  // iterations between looks at the device state
+    long long it = 1;
+    while (true) {
+        const long long end = std::min(maxit, it + kBatch - 1);
+        for (long long k = it; k <= end; ++k) enqueue_iteration();
+        read_state();
+        const int code = hst->stop;
+        const long long j = hst->stop_iteration;
+        if (code == stop_none) {
+            count_full(it, end);
+            res->iterations = static_cast<std::uint64_t>(end);
+            res->recursive_residual_norm = hst->recursive;
+            it = end + 1;
+            continue;
+        }
+        res->iterations = static_cast<std::uint64_t>(j);
+        res->recursive_residual_norm = hst->recursive;
+        if (code == stop_breakdown) {
+            count_full(it, j - 1);
+            if (hst->stage == 1) {
+                pcs += 1;
+                ops += 1;
+                reds += 1;
+            } else {
+                count_full(j, j);
+            }
+            res->breakdown_reason = hst->reason;
+            return finish(status_breakdown);
+        }
+        if (code == stop_max) {
+            count_full(it, j);
+            break;
+        }
+        if (code == pstop_replace) {
+            count_full(it, j);
+            true_residual(&d_norms[2]);  // replacement: no convergence test
+            if (!restart_or_fail(j + 1)) {
+                res->breakdown_reason = reason_biorthogonality_loss;
+                return finish(status_breakdown);
+            }
+            it = j + 1;
+            if (it > maxit) break;
+            continue;
+        }
+        double rec;
+        if (code == stop_early) {
+            count_full(it, j - 1);
+            pcs += 1;
+            ops += 1;
+            reds += 1;
+            kern::axpy(st, n, hst->alpha, V.p2, x);
+            rec = hst->qnorm;
+        } else {
+            count_full(it, j);
+            rec = hst->rnorm;
+        }
+        true_residual(&d_norms[2]);
+        ++reds;
+        const double tn = read_norm(&d_norms[2]);
+        set_residual(res, rec, tn, rhs_norm);
+        if (!std::isfinite(tn)) {
+            res->breakdown_reason = reason_non_finite_scalar;
+            return finish(status_breakdown);
+        }
+        if (tn <= thr) return finish(status_converged);
+        if (!restart_or_fail(j + 1)) {
+            res->breakdown_reason = reason_biorthogonality_loss;
+            return finish(status_breakdown);
+        }
+        it = j + 1;
+        if (it > maxit) break;
+    }
+    true_residual(&d_norms[3]);
+    ++reds;
+    const double tn = read_norm(&d_norms[3]);
+    set_residual(res, hst->recursive, tn, rhs_norm);  // the last recursive |r| (not an early-exit |s|)
     int status = tn <= thr ? status_converged : status_maximum_iterations;
     if (!std::isfinite(tn)) {
         status = status_breakdown;
@@ -843,5 +1327,15 @@ extern "C" int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const dou
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
         m->ctx->activate();
         pipelined_bicgstab_run(m, pc, b, x, options, result);
+    });
+}
+
+extern "C" int owtk_communication_hiding_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                                  const owtk_solver_options* options, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        communication_hiding_bicgstab_run(m, pc, b, x, options, result);
     });
 }

@@ -531,6 +531,201 @@ __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict
     }
 }
 
+// Chain sweeps over an ELL copy of the factors.  A sweep's rows are cut into
+// the same 64-row chunks; for chunk g and slot k < K the 64 rows' k-th entry
+// (chain order) is contiguous: idx[(g*K + k)*64 + lane] and
+// val[c][(g*K + k)*64 + lane].  idx >= 0: block-local column of an entry on a
+// row of an earlier chunk; idx <= -2: in-chunk entry finished at step
+// -idx - 2; -1: padding.  A chunk's description is then K coalesced 16-bit
+// loads with no pointer chase: the next chunk's indices are fetched at the
+// start of a chunk and its values (only the slots that hold an entry) at its
+// end, so both are in flight during the chunk's 64 steps.  In-chunk
+// coefficients go to a packed strictly-lower-triangular LDS tile (position p
+// only depends on steps s < p), half the dense tile, so more one-wave
+// workgroups fit per CU.
+__device__ constexpr int tri_base(int s) { return (s * (127 - s)) / 2; }  // first packed entry of step s
+
+// The sweep's workgroup is one wave.  LDS instructions of a wave complete in
+// issue order, and so do its vector memory instructions to one address, so
+// lanes see each other's earlier tile and solution writes without a
+// workgroup barrier; only the compiler must keep the order.  (__syncthreads
+// would also wait for every outstanding load, including the next chunk's
+// prefetch.)
+__device__ inline void wave_order() { __asm__ volatile("" ::: "memory"); }
+
+template<class X>
+__device__ inline X chain_finish(X acc, X, X, std::integral_constant<int, 0>)  // ILU forward: unit diagonal
+{
+    return acc;
+}
+template<class X>
+__device__ inline X chain_finish(X acc, X d, X, std::integral_constant<int, 1>)  // ILU backward: inverse diagonal
+{
+    return acc * d;
+}
+// SSOR: acc / d from the reciprocal and one fma-based correction (Markstein:
+// with rinv = RN(1/d) and q = RN(acc rinv), the residual acc - q d is exact
+// and RN(q + residual rinv) is the correctly rounded quotient), which keeps
+// the division's long latency off the step-to-step dependency chain.
+template<class X>
+__device__ inline X chain_finish(X acc, X d, X rinv, std::integral_constant<int, 2>)
+{
+    const X q = acc * rinv;
+    const X r = dev::dfma(-q, d, acc);
+    return dev::dfma(r, rinv, q);
+}
+
+template<TriMode MODE, bool FWD>
+using FinishKind = std::integral_constant<int, MODE == TriMode::ssor ? 2 : (FWD ? 0 : 1)>;
+
+template<TriMode MODE, bool FWD, class X, class V, int E>
+__device__ __attribute__((always_inline)) inline void ell_sweep(const std::int16_t* __restrict__ idx, const V* __restrict__ val, std::int64_t g0,
+                                 int K, const X* __restrict__ dv, X* y, int ys, X* tri, int nb, int lane)
+{
+    const int chunks = (nb + 63) >> 6;
+    const std::int64_t stride = static_cast<std::int64_t>(K) * 64;  // one chunk's slots
+    int ci[E];
+    X cv[E];
+    X cd = X(1);
+    {
+        const std::int64_t base = g0 * stride + lane;
+#pragma unroll
+        for (int k = 0; k < E; ++k) ci[k] = k < K ? static_cast<int>(__builtin_nontemporal_load(&idx[base + k * 64])) : -1;
+#pragma unroll
+        for (int k = 0; k < E; ++k) cv[k] = ci[k] != -1 ? static_cast<X>(vload(val, base + k * 64)) : X(0);
+        if constexpr (MODE == TriMode::ssor || !FWD)
+            if (lane < nb) cd = dv[FWD ? lane : nb - 1 - lane];
+    }
+    for (int q = 0; q < chunks; ++q) {
+        const int p = q * 64 + lane;
+        const bool valid = p < nb;
+        const int i = FWD ? p : nb - 1 - p;
+        // the row's value and its entries on earlier chunks: all loads first
+        X yo[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) yo[k] = ci[k] >= 0 ? y[static_cast<std::int64_t>(ci[k]) * ys] : X(0);
+        X acc = valid ? y[i * ys] : X(0);
+        // next chunk's indices and diagonal, in flight during this chunk's steps
+        int ni[E];
+        X nd = X(1);
+        const bool more = q + 1 < chunks;
+        const std::int64_t nbase = (g0 + q + 1) * stride + lane;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            ni[k] = more && k < K ? static_cast<int>(__builtin_nontemporal_load(&idx[nbase + k * 64])) : -1;
+        if constexpr (MODE == TriMode::ssor || !FWD)
+            if (more && p + 64 < nb) nd = dv[FWD ? p + 64 : nb - 1 - (p + 64)];
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (ci[k] <= -2) tri[1 + tri_base(-ci[k] - 2) + lane + ci[k] + 1] = cv[k];
+        if constexpr (MODE == TriMode::ssor && !FWD) acc = cd * acc;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (ci[k] >= 0) acc = dev::dfma(-cv[k], yo[k], acc);
+        X rinv = X(1);
+        if constexpr (MODE == TriMode::ssor) rinv = X(1) / cd;
+        wave_order();
+        // step st: lane st's row is final; every later lane folds it in with
+        // its tile coefficient (lanes at or before st read the zero guard, so
+        // a finished row keeps its value)
+        auto step = [&](int st) {
+            const X coef = tri[lane > st ? tri_base(st) + lane - st : 0];
+            const X v = lane_value(chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{}), st);
+            acc = dev::dfma(-coef, v, acc);
+        };
+        const int steps = nb - q * 64 < 64 ? nb - q * 64 : 64;
+        if (steps == 64) {
+#pragma unroll
+            for (int st = 0; st < 64; ++st) step(st);
+        } else {
+            for (int st = 0; st < steps; ++st) step(st);
+        }
+        if (valid) y[i * ys] = chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{});
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (ci[k] <= -2) tri[1 + tri_base(-ci[k] - 2) + lane + ci[k] + 1] = X(0);
+        // next chunk's values: only the slots that hold an entry
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            cv[k] = ni[k] != -1 ? static_cast<X>(vload(val, nbase + k * 64)) : X(0);
+            ci[k] = ni[k];
+        }
+        cd = nd;
+        wave_order();
+    }
+}
+
+struct EllArgs {
+    const std::int32_t* chunk0;  // [blocks] first global chunk of each block
+    const std::int16_t* fidx;
+    const std::int16_t* bidx;
+    std::int64_t fslots, bslots;  // values per component: chunks * K * 64
+    int fk, bk;                   // slots per row of each sweep
+};
+
+template<TriMode MODE, class X, class V, int E, bool GLOBAL_Y>
+__global__ __launch_bounds__(64) void k_tri_ell(TriArgs a, EllArgs l, const V* __restrict__ fval,
+                                                const V* __restrict__ bval, const X* __restrict__ dval, const X* in,
+                                                X* out, X scale, const int* stop)
+{
+    extern __shared__ unsigned char smem_raw[];
+    if (stop && *stop) return;
+    const int bs = a.bs;
+    // the bs component waves of one block on one XCD: they read the same indices
+    const unsigned int wg = kern::xcd_block(blockIdx.x, gridDim.x);
+    const int c = static_cast<int>(wg % bs);
+    const int b = static_cast<int>(wg / bs);
+    const std::int64_t r0 = a.row0[b];
+    const int nb = a.nrows[b];
+    const int lane = threadIdx.x;
+    constexpr int kTri = 1 + tri_base(64);  // one guard entry + the packed triangle
+    X* tri = reinterpret_cast<X*>(smem_raw);
+    for (int s = lane; s < kTri; s += 64) tri[s] = X(0);
+    X* y;
+    int ys;
+    if constexpr (GLOBAL_Y) {
+        y = out + r0 * bs + c;
+        ys = bs;
+        if (in != out)
+            for (int i = lane; i < nb; i += 64) y[i * ys] = in[(r0 + i) * bs + c];
+    } else {
+        y = tri + kTri;  // [nb]
+        ys = 1;
+        for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
+    }
+    __syncthreads();
+    const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
+    const std::int64_t g0 = l.chunk0[b];
+    ell_sweep<MODE, true, X, V, E>(l.fidx, fval + static_cast<std::int64_t>(c) * l.fslots, g0, l.fk, dv, y, ys, tri,
+                                   nb, lane);
+    __syncthreads();
+    ell_sweep<MODE, false, X, V, E>(l.bidx, bval + static_cast<std::int64_t>(c) * l.bslots, g0, l.bk, dv, y, ys, tri,
+                                    nb, lane);
+    const bool scaled = MODE == TriMode::ssor && scale != X(1);
+    if (GLOBAL_Y && !scaled) return;
+    for (int i = lane; i < nb; i += 64) {
+        X v = y[i * ys];
+        if (scaled) v = v * scale;
+        out[(r0 + i) * bs + c] = v;
+    }
+}
+
+// ELL values from the chain-order values: dst[c][slot] = src[c][map[slot]] (0 for padding)
+template<class V>
+__global__ __launch_bounds__(256) void k_ell_gather(V* __restrict__ dst, const V* __restrict__ src,
+                                                    const std::int64_t* __restrict__ map, std::int64_t slots,
+                                                    std::int64_t nnz, int bs)
+{
+    const std::int64_t total = slots * bs;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t c = t / slots;
+        const std::int64_t sl = t - c * slots;
+        const std::int64_t e = map[sl];
+        dst[t] = e >= 0 ? src[c * nnz + e] : V(0);
+    }
+}
+
 // ---------------------------------------------------------------- symbolic --
 
 // Encoded gather source of matrix value (row, CSR entry e): >= 0 index of
@@ -590,6 +785,68 @@ struct TriHost {
 };
 
 std::size_t lds_limit(owtk_precond pc);
+
+// ELL copy of a chain layout (see k_tri_ell): slot k of position p holds the
+// p-th row's k-th chain-order entry, encoded as its block-local column (entry
+// on an earlier chunk) or -(step + 2) (in-chunk entry).  Needs block-local
+// columns below 2^15 and at most 16 entries per row.
+void build_ell(TriSchedule& tri, const std::vector<std::int64_t>& fptr, const std::vector<std::int32_t>& fnout,
+               const std::vector<std::int32_t>& fcol, const std::vector<std::int64_t>& bptr,
+               const std::vector<std::int32_t>& bnout, const std::vector<std::int32_t>& bcol)
+{
+    tri.ell = false;
+    if (!tri.chain || tri.max_block_rows > 32767) return;
+    std::vector<std::int32_t> chunk0(tri.blocks);
+    std::int64_t chunks = 0;
+    for (std::int64_t b = 0; b < tri.blocks; ++b) {
+        chunk0[b] = static_cast<std::int32_t>(chunks);
+        chunks += (tri.h_nrows[b] + 63) / 64;
+    }
+    const std::int64_t n = tri.rows;
+    auto kmax = [&](const std::vector<std::int64_t>& ptr) {
+        std::int64_t k = 1;
+        for (std::int64_t i = 0; i < n; ++i) k = std::max(k, ptr[i + 1] - ptr[i]);
+        return k;
+    };
+    const std::int64_t fk = kmax(fptr), bk = kmax(bptr);
+    if (fk > 16 || bk > 16 || chunks > 0x7fffffff) return;
+    for (int dir = 0; dir < 2; ++dir) {
+        const auto& ptr = dir == 0 ? fptr : bptr;
+        const auto& nout = dir == 0 ? fnout : bnout;
+        const auto& col = dir == 0 ? fcol : bcol;
+        const std::int64_t K = dir == 0 ? fk : bk;
+        std::vector<std::int16_t> idx(static_cast<std::size_t>(chunks * K * 64), -1);
+        std::vector<std::int64_t> map(idx.size(), -1);
+        for (std::int64_t b = 0; b < tri.blocks; ++b) {
+            const std::int64_t r0 = tri.h_row0[b];
+            const std::int64_t nb = tri.h_nrows[b];
+            for (std::int64_t p = 0; p < nb; ++p) {
+                const std::int64_t q = p / 64, lane = p % 64, cs = q * 64;
+                const std::int64_t g = chunk0[b] + q;
+                const std::int64_t e0 = ptr[r0 + p], e1 = ptr[r0 + p + 1], no = nout[r0 + p];
+                for (std::int64_t e = e0; e < e1; ++e) {
+                    const std::int64_t c = col[e];
+                    std::int64_t code = c;
+                    if (e - e0 >= no) {
+                        const std::int64_t step = dir == 0 ? c - cs : nb - 1 - cs - c;
+                        if (step < 0 || step >= lane) throw std::logic_error("chain layout: in-chunk entry order");
+                        code = -(step + 2);
+                    }
+                    const std::size_t slot = static_cast<std::size_t>((g * K + (e - e0)) * 64 + lane);
+                    idx[slot] = static_cast<std::int16_t>(code);
+                    map[slot] = e;
+                }
+            }
+        }
+        upload(dir == 0 ? tri.fidx_ell : tri.bidx_ell, idx);
+        upload(dir == 0 ? tri.fmap_ell : tri.bmap_ell, map);
+        (dir == 0 ? tri.fslots : tri.bslots) = static_cast<std::int64_t>(idx.size());
+    }
+    tri.fk = static_cast<int>(fk);
+    tri.bk = static_cast<int>(bk);
+    upload(tri.chunk0, chunk0);
+    tri.ell = true;
+}
 
 void build_schedule(owtk_precond pc, const TriHost& t)
 {
@@ -726,6 +983,7 @@ void build_schedule(owtk_precond pc, const TriHost& t)
     }
     tri.fnnz = static_cast<std::int64_t>(fcol.size());
     tri.bnnz = static_cast<std::int64_t>(bcol.size());
+    build_ell(tri, fptr, fnout, fcol, bptr, bnout, bcol);
     upload(tri.row0, tri.h_row0);
     upload(tri.nrows, tri.h_nrows);
     upload(tri.flev, tri.h_flev);
@@ -782,6 +1040,28 @@ void refresh_storage(owtk_precond pc)
     auto& t = pc->tri;
     const hipStream_t st = pc->m->ctx->stream;
     const int bs = pc->m->bs;
+    if (t.ell) {
+        // ELL values, FP64 from the chain-order factor, then each storage's copy
+        if (!t.fell64.get()) t.fell64.allocate(std::max<std::int64_t>(t.fslots * bs, 1));
+        if (!t.bell64.get()) t.bell64.allocate(std::max<std::int64_t>(t.bslots * bs, 1));
+        k_ell_gather<double><<<grid_for(t.fslots * bs), kBlock, 0, st>>>(t.fell64.get(), t.fval64.get(),
+                                                                         t.fmap_ell.get(), t.fslots, t.fnnz, bs);
+        k_ell_gather<double><<<grid_for(t.bslots * bs), kBlock, 0, st>>>(t.bell64.get(), t.bval64.get(),
+                                                                         t.bmap_ell.get(), t.bslots, t.bnnz, bs);
+        OWTK_LAUNCH_CHECK();
+        if (pc->storage == OWTK_F32) {
+            if (!t.fell32.get()) t.fell32.allocate(std::max<std::int64_t>(t.fslots * bs, 1));
+            if (!t.bell32.get()) t.bell32.allocate(std::max<std::int64_t>(t.bslots * bs, 1));
+            kern::convert(st, t.fell32.get(), t.fell64.get(), static_cast<std::size_t>(t.fslots * bs));
+            kern::convert(st, t.bell32.get(), t.bell64.get(), static_cast<std::size_t>(t.bslots * bs));
+        } else if (pc->storage == OWTK_BF16) {
+            if (!t.fell16.get()) t.fell16.allocate(std::max<std::int64_t>(t.fslots * bs, 1));
+            if (!t.bell16.get()) t.bell16.allocate(std::max<std::int64_t>(t.bslots * bs, 1));
+            kern::k_to_bf16<<<grid_for(t.fslots * bs), kBlock, 0, st>>>(t.fell16.get(), t.fell64.get(), t.fslots * bs);
+            kern::k_to_bf16<<<grid_for(t.bslots * bs), kBlock, 0, st>>>(t.bell16.get(), t.bell64.get(), t.bslots * bs);
+            OWTK_LAUNCH_CHECK();
+        }
+    }
     if (pc->storage == OWTK_F32 || pc->storage == OWTK_BF16) {
         const std::int64_t nd = t.rows * bs;
         if (!t.dval32.get()) t.dval32.allocate(std::max<std::int64_t>(nd, 1));
@@ -1164,7 +1444,36 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
     const X scale = static_cast<X>(pc->omega * (2.0 - pc->omega));
     const hipStream_t st = pc->m->ctx->stream;
     ProfileScope prof(pc->m->ctx, sizeof(X) == 8 ? prof_precond_f64 : prof_precond_low, precond_apply_bytes(pc));
-    if (t.chain) {
+    if (t.chain && t.ell) {
+        const TriArgs args = tri_args(t);
+        const EllArgs ea{t.chunk0.get(), t.fidx_ell.get(), t.bidx_ell.get(), t.fslots, t.bslots, t.fk, t.bk};
+        const V* efv;
+        const V* ebv;
+        if constexpr (std::is_same_v<V, double>) {
+            efv = t.fell64.get();
+            ebv = t.bell64.get();
+        } else if constexpr (std::is_same_v<V, float>) {
+            efv = t.fell32.get();
+            ebv = t.bell32.get();
+        } else {
+            efv = t.fell16.get();
+            ebv = t.bell16.get();
+        }
+        const std::size_t tri_lds = static_cast<std::size_t>(1 + 64 * 63 / 2) * sizeof(X);
+        const bool global_y = need + tri_lds > kChainLdsBudget;
+        const std::size_t lds = global_y ? tri_lds : need + tri_lds;
+        const int k = std::max(t.fk, t.bk);
+        auto launch = [&](auto e, auto gy) {
+            k_tri_ell<MODE, X, V, decltype(e)::value, decltype(gy)::value>
+                <<<groups, 64, lds, st>>>(args, ea, efv, ebv, dv, in, out, scale, stop);
+        };
+        auto pick = [&](auto e) {
+            if (global_y) launch(e, std::true_type{});
+            else launch(e, std::false_type{});
+        };
+        if (k <= 8) pick(std::integral_constant<int, 8>{});
+        else pick(std::integral_constant<int, 16>{});
+    } else if (t.chain) {
         const TriArgs args = tri_args(t);
         // Large blocks keep their solution in global memory (L2): the LDS
         // then holds only the tile, and more one-wave workgroups fit per CU.

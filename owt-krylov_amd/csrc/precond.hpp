@@ -48,6 +48,17 @@ struct TriSchedule {
     DeviceArray<std::uint16_t> fval16, bval16;
     // gather sources of the apply values (numeric update)
     DeviceArray<std::int64_t> fsrc, bsrc, dsrc;
+    // ELL copy of the chain layout: per 64-row chunk and slot, the rows'
+    // entries contiguous (k_tri_ell); values [c][slot] in each storage
+    bool ell = false;
+    int fk = 0, bk = 0;                       // slots per row, forward / backward sweep
+    std::int64_t fslots = 0, bslots = 0;      // chunks * slots * 64
+    DeviceArray<std::int32_t> chunk0;         // first chunk of each block
+    DeviceArray<std::int16_t> fidx_ell, bidx_ell;
+    DeviceArray<std::int64_t> fmap_ell, bmap_ell;  // chain-order entry of each slot, -1 padding
+    DeviceArray<double> fell64, bell64;
+    DeviceArray<float> fell32, bell32;
+    DeviceArray<std::uint16_t> fell16, bell16;
 };
 
 struct IluFactor {

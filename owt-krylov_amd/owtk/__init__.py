@@ -19,7 +19,8 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
                    UnsupportedError, ZeroPivotError, check, lib)
 
 __all__ = [
-    "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab", "fgmres",
+    "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
+    "communication_hiding_bicgstab", "fgmres",
     "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
@@ -494,6 +495,12 @@ def pipelined_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffe
                        options=None) -> dict:
     """pipelined_bicgstab (pipelined_bicgstab.hpp:20-244) on the device."""
     return _solve(lib().owtk_pipelined_bicgstab, matrix, pc, b, x, options)
+
+
+def communication_hiding_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
+                                  options=None) -> dict:
+    """communication_hiding_bicgstab (pipelined_bicgstab.hpp:317-597) on the device."""
+    return _solve(lib().owtk_communication_hiding_bicgstab, matrix, pc, b, x, options)
 
 
 def fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:

@@ -172,6 +172,8 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_solver_options_default": (None, [C.POINTER(SolverOptions)]),
     "owtk_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_pipelined_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
+    "owtk_communication_hiding_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions),
+                                                     C.POINTER(SolverResult)]),
     "owtk_fgmres": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_refinement_options_default": (None, [C.POINTER(RefinementOptions)]),
     "owtk_fgmres_refined": (C.c_int, [_P, _P, _P, _P, C.POINTER(RefinementOptions), C.POINTER(SolverResult)]),

@@ -173,6 +173,74 @@ def test_pipelined_bicgstab_c0_matches_oracle(ctx, c0, pc_kind, replace):
             assert res[key] == ro[key], key
 
 
+@pytest.mark.parametrize("use_ilu", [False, True])
+def test_communication_hiding_recurrence(ctx, use_ilu):
+    """Device communication-hiding BiCGSTAB on the 12 x 12 audit matrix:
+    the first four iterates agree with the oracle (itself pinned by the
+    long-double dense BiCGSTAB), every counter matches."""
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)
+    om = orc.OMatrix.csr(12, 0, 1, offsets, columns, values)
+    pc = owtk.Preconditioner(m, owtk.PC_ILU) if use_ilu else None
+    opc = orc.OPreconditioner(om, 3) if use_ilu else None
+    for count in range(1, 5):
+        kw = dict(relative_tolerance=0.0, maximum_iterations=count, residual_replacement_interval=0)
+        x, res = solve(ctx, owtk.communication_hiding_bicgstab, m, pc, rhs, x0, **kw)
+        xo, ro = orc.communication_hiding_bicgstab(om, opc, rhs, x0=x0, **kw)
+        assert res["status_name"] == "maximum_iterations" and res["iterations"] == count
+        assert np.max(np.abs(x[:12] - xo[:12])) < 2e-11, count
+        for key in COUNTERS:
+            assert res[key] == ro[key], (count, key)
+
+
+def test_communication_hiding_exact_preconditioner_and_breakdowns(ctx):
+    vals = [4.0, -1, -2, 5, -1, -1, 3]
+    m = owtk.Matrix.csr(ctx, 3, 0, 1, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2], vals)
+    om = orc.OMatrix.csr(3, 0, 1, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2], vals)
+    rhs = om.spmv([1.0, 2.0, 3.0])
+    x, res = solve(ctx, owtk.communication_hiding_bicgstab, m, owtk.Preconditioner(m, owtk.PC_ILU), rhs,
+                   relative_tolerance=32 * EPS)
+    xo, ro = orc.communication_hiding_bicgstab(om, orc.OPreconditioner(om, 3), rhs, relative_tolerance=32 * EPS)
+    assert res["status_name"] == "converged" and res["iterations"] == ro["iterations"]
+    for key in COUNTERS:
+        assert res[key] == ro[key], key
+    assert np.max(np.abs(x[:3] - [1, 2, 3])) < 128 * EPS
+    # zero operator: the first restart's <A M r, rp> vanishes
+    m0 = owtk.Matrix.csr(ctx, 1, 0, 1, [0, 1], [0], [0.0])
+    _, res = solve(ctx, owtk.communication_hiding_bicgstab, m0, None, np.array([1.0]))
+    _, ro = orc.communication_hiding_bicgstab(orc.OMatrix.csr(1, 0, 1, [0, 1], [0], [0.0]), None, np.array([1.0]))
+    assert res["status_name"] == "breakdown" and res["reason_name"] == "biorthogonality_loss"
+    for key in COUNTERS:
+        assert res[key] == ro[key], key
+    m1 = owtk.Matrix.csr(ctx, 1, 0, 2, [0, 1], [0], [1.0, 1.0])
+    _, res = solve(ctx, owtk.communication_hiding_bicgstab, m1, None, np.array([np.inf, 1.0]))
+    assert res["status_name"] == "breakdown" and res["reason_name"] == "non_finite_scalar"
+
+
+@pytest.mark.parametrize("pc_kind", ["jacobi", "ssor", "ilu"])
+@pytest.mark.parametrize("replace", [50, 7])
+def test_communication_hiding_c0_matches_oracle(ctx, c0, pc_kind, replace):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    kind = {"jacobi": (1, owtk.PC_JACOBI), "ssor": (2, owtk.PC_SSOR), "ilu": (3, owtk.PC_ILU)}[pc_kind]
+    extra = {"block_nodes": 512} if pc_kind == "ilu" else {}
+    opc, pc = orc.OPreconditioner(om, kind[0], **extra), owtk.Preconditioner(m, kind[1], **extra)
+    xo, ro = orc.communication_hiding_bicgstab(om, opc, s.rhs, residual_replacement_interval=replace)
+    x, res = solve(ctx, owtk.communication_hiding_bicgstab, m, pc, s.rhs, residual_replacement_interval=replace)
+    print(pc_kind, replace, "oracle", ro["iterations"], ro["relative_residual_norm"], "gpu", res["iterations"],
+          res["relative_residual_norm"], flush=True)
+    assert res["status_name"] == "converged" and ro["status"] == 0
+    assert abs(res["iterations"] - ro["iterations"]) <= 1
+    true_rel = np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs)
+    assert true_rel <= 1e-8
+    assert abs(res["relative_residual_norm"] - true_rel) <= 1e-3 * true_rel
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    if res["iterations"] == ro["iterations"]:
+        for key in COUNTERS:
+            assert res[key] == ro[key], key
+
+
 def test_fgmres_direct_solve(ctx):
     a, offsets, columns, values, rhs, _ = rc.recurrence_matrix()
     m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)

@@ -329,3 +329,50 @@ def test_pipelined_bicgstab_dense_recurrence():
     xo, res = orc.pipelined_bicgstab(om, None, rhs, x0=x0, relative_tolerance=1e-12)
     assert res["status"] == 0
     assert np.linalg.norm(rhs - a.astype(np.float64) @ xo[:nrow]) <= 1.1e-12 * np.linalg.norm(rhs)
+
+
+@pytest.mark.parametrize("use_ilu", [False, True])
+def test_communication_hiding_bicgstab_equals_bicgstab_recurrence(use_ilu):
+    """The communication-hiding recurrence (pipelined_bicgstab.hpp:317-597)
+    carries auxiliary vectors (s = A M p, z = A M s, ...) but in exact
+    arithmetic its iterates are BiCGSTAB's.  The oracle's first four iterates
+    on the audit's 12 x 12 matrix agree with the long-double dense BiCGSTAB
+    of ilu_audit_reference.hpp:119-156, unpreconditioned and with the exact
+    ILU(0) of that matrix as M; the solve then converges with the counters
+    the recurrence implies."""
+    import refcases as rc
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    n = a.shape[0]
+    om = orc.OMatrix.csr(n, 0, 1, np.array(offsets, np.int32), np.array(columns, np.int32), np.array(values))
+    if use_ilu:
+        pattern = [[a[i, j] != 0 for j in range(n)] for i in range(n)]
+        lower, upper = rc.doolittle(a, pattern)
+        inverse = lambda r: rc.lu_solve(lower, upper, r)  # noqa: E731
+        pc = orc.OPreconditioner(om, 3)
+    else:
+        inverse, pc = (lambda r: r.copy()), None
+    history = rc.dense_bicgstab(a, rhs, x0, inverse, 4)
+    for k in range(1, 5):
+        xo, res = orc.communication_hiding_bicgstab(om, pc, rhs, x0=x0, relative_tolerance=0.0, maximum_iterations=k,
+                                                    residual_replacement_interval=0)
+        assert res["iterations"] == k and res["status"] == 1
+        assert np.max(np.abs(xo[:n] - history[k - 1].astype(np.float64))) < 1e-12, k
+        # initial norms, the restart's rho and <w, rp>, 2 per iteration, final norm
+        assert res["global_reductions"] == 1 + 2 + 2 * k + 1
+        # restart: 2 applications of each; iteration: 2 of each; final true residual: 1 op
+        assert res["preconditioner_applications"] == 2 + 2 * k
+        assert res["operator_applications"] == 1 + 2 + 2 * k + 1
+    xo, res = orc.communication_hiding_bicgstab(om, pc, rhs, x0=x0, relative_tolerance=1e-12)
+    assert res["status"] == 0
+    assert np.linalg.norm(rhs - a.astype(np.float64) @ xo[:n]) <= 1.1e-12 * np.linalg.norm(rhs)
+
+
+def test_communication_hiding_bicgstab_breakdowns():
+    # zero operator: <r, rp> is fine but <A M r, rp> = 0 stops the first restart
+    om = orc.OMatrix.csr(1, 0, 1, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([0.0]))
+    _, res = orc.communication_hiding_bicgstab(om, None, np.array([1.0]))
+    assert res["status"] == 2 and res["breakdown_reason"] == 3
+    assert res["global_reductions"] == 3 and res["operator_applications"] == 2
+    om = orc.OMatrix.csr(1, 0, 2, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([1.0, 1.0]))
+    _, res = orc.communication_hiding_bicgstab(om, None, np.array([np.inf, 1.0]))
+    assert res["status"] == 2 and res["breakdown_reason"] == 9
