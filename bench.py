@@ -114,7 +114,8 @@ def timed_solves(ctx, label, steps, warmup, run, x):
     ctx.profile(False)
     prof = {name: ctx.profile_read(cat) for name, cat in (
         ("spmv_f64", ctx.PROF_SPMV_F64), ("spmv_f32", ctx.PROF_SPMV_F32), ("spmv_bf16", ctx.PROF_SPMV_BF16),
-        ("precond_f64", ctx.PROF_PRECOND_F64), ("precond_low", ctx.PROF_PRECOND_LOW), ("learned", ctx.PROF_LEARNED))}
+        ("precond_f64", ctx.PROF_PRECOND_F64), ("precond_low", ctx.PROF_PRECOND_LOW), ("learned", ctx.PROF_LEARNED),
+        ("learned_ends", ctx.PROF_LEARNED_ENDS))}
     return times, results, prof
 
 
@@ -150,8 +151,9 @@ def spmv_timing(ctx, m, n, reps=50):
 
 def learned_timing(ctx, pc, n, reps=30):
     """The learned preconditioner applied back to back outside a solver (no
-    stopped launches): per-launch HIP-event time of its series-step kernel,
-    and the whole apply (init + steps) timed by one event pair."""
+    stopped launches): per-launch HIP-event time of its middle series-step
+    kernel, and the whole apply (first, middle and last steps) timed by one
+    event pair."""
     xf = ctx.upload(np.random.default_rng(2).uniform(-1, 1, n).astype(np.float32))
     yf = ctx.empty(n, np.float32)
     for _ in range(3):

@@ -73,7 +73,8 @@ const char* owtk_policy_name(void);
  * the reference's ScopedSolverTimer / TimedOperator instrumentation
  * (core.hpp:150-182, benchmarks/benchmark_split.cpp:107-121) per kernel class.
  * Categories: 0 SpMV FP64, 1 SpMV FP32, 2 SpMV BF16, 3 preconditioner apply
- * FP64, 4 preconditioner apply FP32/BF16, 5 learned preconditioner. */
+ * FP64, 4 preconditioner apply FP32/BF16, 5 learned preconditioner series
+ * steps (middle), 6 learned preconditioner first and last steps. */
 int owtk_profile_enable(owtk_context ctx, int enable);
 int owtk_profile_reset(owtk_context ctx);
 /* Sum over recorded launches of one category: launch count, total kernel

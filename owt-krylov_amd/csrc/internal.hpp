@@ -212,7 +212,8 @@ enum ProfileCategory : int {
     prof_spmv_bf16 = 2,
     prof_precond_f64 = 3,
     prof_precond_low = 4,
-    prof_learned = 5,
+    prof_learned = 5,        // learned preconditioner: middle series steps
+    prof_learned_ends = 6,   // learned preconditioner: first (init folded in) and last series steps
     prof_categories = 8,
 };
 

@@ -43,6 +43,10 @@ struct EpiNone {
     const int* stop = nullptr;
     __device__ bool stopped() const { return stop != nullptr && *stop != 0; }
     __device__ void finalize(const double*) const {}
+    // the operand value the kernel multiplies at vector index i (an epilogue
+    // may form it on the fly, e.g. a diagonally scaled input)
+    template<class X>
+    __device__ X xval(const X* x, std::int64_t i) const { return x[i]; }
 };
 
 template<class X>
@@ -181,8 +185,8 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
                 const V* vk = vals + (base + 64 * k) * BS + lane;
 #pragma unroll
                 for (int c = 0; c < BS; ++c)
-                    acc[c] = dfma(static_cast<A>(load_value_nt(vk, 64 * c)), static_cast<A>(x[colk[k] * BS + c]),
-                                  acc[c]);
+                    acc[c] = dfma(static_cast<A>(load_value_nt(vk, 64 * c)),
+                                  static_cast<A>(e.xval(x, colk[k] * BS + c)), acc[c]);
             }
         }
         for (int k = kUnroll; k < len; ++k) {
@@ -191,7 +195,8 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
             const V* vk = vals + run * BS + lane;
 #pragma unroll
             for (int c = 0; c < BS; ++c)
-                acc[c] = dfma(static_cast<A>(load_value(vk, 64 * c)), static_cast<A>(x[col * BS + c]), acc[c]);
+                acc[c] = dfma(static_cast<A>(load_value(vk, 64 * c)), static_cast<A>(e.xval(x, col * BS + c)),
+                              acc[c]);
         }
 #pragma unroll
         for (int c = 0; c < BS; ++c) e.elem(row * BS + c, acc[c], red);
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(256) void spmv_csr_cd(const std::int32_t* __restric
             for (std::int32_t en = begin; en < end; ++en) {
                 const std::int64_t col = cols[en];
                 acc = dfma(static_cast<A>(load_value(vals, static_cast<std::int64_t>(en) * bs + c)),
-                           static_cast<A>(x[col * bs + c]), acc);
+                           static_cast<A>(e.xval(x, col * bs + c)), acc);
             }
             e.elem(row * bs + c, acc, red);
         }

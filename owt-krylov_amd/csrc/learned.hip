@@ -592,19 +592,46 @@ __global__ __launch_bounds__(256) void k_poly_init(std::int64_t nb, const X* __r
 
 namespace owtk::dev {
 
-// u_next = -D^-1 (A_off u), z += gamma_k * u_next
+// u_next = -D^-1 (A_off u), z += gamma_k * u_next.  The last step of an
+// apply does not store u_next (nothing reads it).
 template<class X>
 struct EpiPoly : EpiNone {
     X* u_next;
     const X* dinv;
     const float* gamma;
     X* z;
+    bool store_u = true;
     template<class A>
     __device__ void elem(std::int64_t i, A acc, double*) const
     {
         const X u = -static_cast<X>(acc) * dinv[i];
-        u_next[i] = u;
+        if (store_u) u_next[i] = u;
         z[i] = dfma(static_cast<X>(gamma[i]), u, z[i]);
+    }
+};
+
+// First step with the initialisation folded in: the operand u_0 = D^-1 r is
+// formed at each gathered index (x = r), and z = gamma_0 u_0 + gamma_1 u_1
+// is written without a prior z.  The same roundings as u_0 = r * dinv,
+// z = gamma_0 * u_0, z = fma(gamma_1, u_1, z).
+template<class X>
+struct EpiPolyFirst : EpiNone {
+    X* u_next;
+    const X* dinv;
+    const X* r;
+    const float* gamma0;
+    const float* gamma1;
+    X* z;
+    bool store_u = true;
+    __device__ X xval(const X* x, std::int64_t i) const { return x[i] * dinv[i]; }
+    template<class A>
+    __device__ void elem(std::int64_t i, A acc, double*) const
+    {
+        const X d = dinv[i];
+        const X u = -static_cast<X>(acc) * d;
+        if (store_u) u_next[i] = u;
+        const X z0 = static_cast<X>(gamma0[i]) * (r[i] * d);
+        z[i] = dfma(static_cast<X>(gamma1[i]), u, z0);
     }
 };
 
@@ -921,15 +948,29 @@ std::int64_t learned_step_bytes(owtk_precond pc)
     return pc->m->entries * (pc->m->bs * sv + 4) + pc->m->owned * 4 + nb * (5 * sx + 4);
 }
 
-// Apply: init (in, dinv, gamma_0 read; u_0, z written) + K steps.
+// First step (init folded in): matrix; r, dinv, gamma_0, gamma_1 read;
+// u_1 (unless it is also the last step) and z written.  Last step: a middle
+// step without the u_next write.
+std::int64_t learned_end_step_bytes(owtk_precond pc, bool first, bool last)
+{
+    const std::int64_t nb = pc->n();
+    const int sx = pc->storage == OWTK_F64 ? 8 : 4;
+    const std::int64_t matrix = learned_step_bytes(pc) - nb * (5 * sx + 4);
+    if (first) return matrix + nb * ((last ? 3 : 4) * sx + 8);
+    return matrix + nb * (4 * sx + 4);
+}
+
+// Apply: degree 0 is the init kernel alone; otherwise first, middle and last steps.
 std::int64_t learned_apply_bytes(owtk_precond pc)
 {
     auto* st = static_cast<LearnedPc*>(pc->learned_state);
     const std::int64_t nb = pc->n();
     const int sx = pc->storage == OWTK_F64 ? 8 : 4;
     const std::int64_t K = st ? st->model->degree : 0;
-    const std::int64_t init = nb * (4 * sx + 4);
-    return init + K * learned_step_bytes(pc);
+    if (K == 0) return nb * (4 * sx + 4);
+    if (K == 1) return learned_end_step_bytes(pc, true, true);
+    return learned_end_step_bytes(pc, true, false) + (K - 2) * learned_step_bytes(pc)
+         + learned_end_step_bytes(pc, false, true);
 }
 
 template<class V, class X, class Dv>
@@ -939,16 +980,39 @@ void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const 
     owtk_matrix m = pc->m;
     const std::int64_t nb = pc->n();
     const std::int64_t step_bytes = learned_step_bytes(pc);
-    k_poly_init<X, Dv><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, dinv, st->gamma.get(), ua, out, stop);
-    OWTK_LAUNCH_CHECK();
-    for (int k = 1; k <= st->model->degree; ++k) {
+    const int degree = st->model->degree;
+    if (degree == 0) {
+        k_poly_init<X, Dv><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, dinv, st->gamma.get(), ua,
+                                                                                     out, stop);
+        OWTK_LAUNCH_CHECK();
+        return;
+    }
+    static_assert(std::is_same_v<X, Dv>, "series steps read the inverse diagonal in the vector type");
+    {
+        dev::EpiPolyFirst<X> e;
+        e.stop = stop;
+        e.u_next = ub;
+        e.dinv = dinv;
+        e.r = in;
+        e.gamma0 = st->gamma.get();
+        e.gamma1 = st->gamma.get() + nb;
+        e.z = out;
+        e.store_u = degree > 1;
+        ProfileScope prof(m->ctx, prof_learned_ends, learned_end_step_bytes(pc, true, degree == 1));
+        offdiag_spmv<V, X, dev::EpiPolyFirst<X>>(m, vals, in, e);
+        std::swap(ua, ub);
+    }
+    for (int k = 2; k <= degree; ++k) {
         dev::EpiPoly<X> e;
         e.stop = stop;
         e.u_next = ub;
         e.dinv = dinv;
         e.gamma = st->gamma.get() + static_cast<std::int64_t>(k) * nb;
         e.z = out;
-        ProfileScope prof(m->ctx, prof_learned, step_bytes);  // one series-step kernel
+        e.store_u = k < degree;
+        // middle steps are the series-step kernel proper; the last step is an end step
+        ProfileScope prof(m->ctx, k < degree ? prof_learned : prof_learned_ends,
+                          k < degree ? step_bytes : learned_end_step_bytes(pc, false, true));
         offdiag_spmv<V, X, dev::EpiPoly<X>>(m, vals, ua, e);
         std::swap(ua, ub);
     }

@@ -66,7 +66,8 @@ class Context:
         check(lib().owtk_context_synchronize(self.handle))
 
     # live kernel timing (HIP events on this context's stream)
-    PROF_SPMV_F64, PROF_SPMV_F32, PROF_SPMV_BF16, PROF_PRECOND_F64, PROF_PRECOND_LOW, PROF_LEARNED = range(6)
+    PROF_SPMV_F64, PROF_SPMV_F32, PROF_SPMV_BF16, PROF_PRECOND_F64, PROF_PRECOND_LOW, PROF_LEARNED, \
+        PROF_LEARNED_ENDS = range(7)
 
     def profile(self, enable: bool = True) -> None:
         check(lib().owtk_profile_enable(self.handle, int(enable)))
