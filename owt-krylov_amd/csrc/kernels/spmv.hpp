@@ -37,6 +37,21 @@ template<class V> __device__ inline auto load_value_nt(const V* p, std::int64_t 
 }
 
 // ----------------------------------------------------------------- epilogues
+// A row's BS contiguous components as one load / store (the compiler emits
+// one wide access per lane: 64 lanes x BS elements, fully coalesced).
+template<int BS, class X>
+__device__ inline void load_row(const X* __restrict__ p, X (&v)[BS])
+{
+#pragma unroll
+    for (int c = 0; c < BS; ++c) v[c] = p[c];
+}
+template<int BS, class X>
+__device__ inline void store_row(X* __restrict__ p, const X (&v)[BS])
+{
+#pragma unroll
+    for (int c = 0; c < BS; ++c) p[c] = v[c];
+}
+
 struct EpiNone {
     static constexpr int K = 0;
     static constexpr std::uint64_t kMaxMask = 0;
@@ -54,7 +69,30 @@ struct EpiStore : EpiNone {
     X* y;
     template<class A>
     __device__ void elem(std::int64_t i, A value, double*) const { y[i] = static_cast<X>(value); }
+    // all components of a row in one contiguous store
+    template<int BS, class A>
+    __device__ void elem_row(std::int64_t i0, const A (&acc)[BS], double*) const
+    {
+        X v[BS];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) v[c] = static_cast<X>(acc[c]);
+        store_row<BS>(y + i0, v);
+    }
 };
+
+// Epilogue of one row's BS components: the epilogue's elem_row when it has
+// one (its loads and stores then cover the row's components contiguously),
+// else elem per component.
+template<int BS, class E, class A>
+__device__ inline void epi_row(const E& e, std::int64_t i0, const A (&acc)[BS], double* red)
+{
+    if constexpr (requires { e.template elem_row<BS>(i0, acc, red); }) {
+        e.template elem_row<BS>(i0, acc, red);
+    } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) e.elem(i0 + c, acc[c], red);
+    }
+}
 
 // y = b - A x, reduced: sum y^2, max |y|; norm written by finalize.
 struct EpiResidual : EpiNone {
@@ -198,8 +236,7 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
                 acc[c] = dfma(static_cast<A>(load_value(vk, 64 * c)), static_cast<A>(e.xval(x, col * BS + c)),
                               acc[c]);
         }
-#pragma unroll
-        for (int c = 0; c < BS; ++c) e.elem(row * BS + c, acc[c], red);
+        epi_row<BS>(e, row * BS, acc, red);
     }
     if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
 }

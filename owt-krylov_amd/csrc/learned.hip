@@ -608,6 +608,22 @@ struct EpiPoly : EpiNone {
         if (store_u) u_next[i] = u;
         z[i] = dfma(static_cast<X>(gamma[i]), u, z[i]);
     }
+    template<int BS, class A>
+    __device__ void elem_row(std::int64_t i0, const A (&acc)[BS], double*) const
+    {
+        X d[BS], zz[BS], u[BS];
+        float g[BS];
+        load_row<BS>(dinv + i0, d);
+        load_row<BS>(gamma + i0, g);
+        load_row<BS>(z + i0, zz);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            u[c] = -static_cast<X>(acc[c]) * d[c];
+            zz[c] = dfma(static_cast<X>(g[c]), u[c], zz[c]);
+        }
+        if (store_u) store_row<BS>(u_next + i0, u);
+        store_row<BS>(z + i0, zz);
+    }
 };
 
 // First step with the initialisation folded in: the operand u_0 = D^-1 r is
@@ -632,6 +648,23 @@ struct EpiPolyFirst : EpiNone {
         if (store_u) u_next[i] = u;
         const X z0 = static_cast<X>(gamma0[i]) * (r[i] * d);
         z[i] = dfma(static_cast<X>(gamma1[i]), u, z0);
+    }
+    template<int BS, class A>
+    __device__ void elem_row(std::int64_t i0, const A (&acc)[BS], double*) const
+    {
+        X d[BS], rr[BS], u[BS], zz[BS];
+        float g0[BS], g1[BS];
+        load_row<BS>(dinv + i0, d);
+        load_row<BS>(r + i0, rr);
+        load_row<BS>(gamma0 + i0, g0);
+        load_row<BS>(gamma1 + i0, g1);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            u[c] = -static_cast<X>(acc[c]) * d[c];
+            zz[c] = dfma(static_cast<X>(g1[c]), u[c], static_cast<X>(g0[c]) * (rr[c] * d[c]));
+        }
+        if (store_u) store_row<BS>(u_next + i0, u);
+        store_row<BS>(z + i0, zz);
     }
 };
 
