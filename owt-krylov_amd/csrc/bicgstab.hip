@@ -278,6 +278,20 @@ struct EpiOmegaP : dev::EpiNone {
         red[2] = __builtin_fma(si, si, red[2]);
         red[3] = dev::combine<dev::Combine::max>(red[3], __builtin_fabs(si));
     }
+    template<int BS>
+    __device__ void elem_row(std::int64_t i0, const double (&tv)[BS], double* red) const
+    {
+        double ss[BS];
+        dev::load_row<BS>(s + i0, ss);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            red[0] = __builtin_fma(tv[c], ss[c], red[0]);
+            red[1] = __builtin_fma(tv[c], tv[c], red[1]);
+            red[2] = __builtin_fma(ss[c], ss[c], red[2]);
+            red[3] = dev::combine<dev::Combine::max>(red[3], __builtin_fabs(ss[c]));
+        }
+        dev::store_row<BS>(y + i0, tv);
+    }
     __device__ void finalize(const double* t) const
     {
         // norm_from_squared: the fused square unless it over/underflowed

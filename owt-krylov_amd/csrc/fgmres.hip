@@ -417,6 +417,20 @@ struct EpiResidualT : dev::EpiNone {
         red[0] = __builtin_fma(rd, rd, red[0]);
         red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(rd));
     }
+    template<int BS, class A>
+    __device__ void elem_row(std::int64_t i0, const A (&ax)[BS], double* red) const
+    {
+        X r[BS];
+        dev::load_row<BS>(b + i0, r);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            r[c] = static_cast<X>(r[c] - static_cast<X>(ax[c]));
+            const double rd = static_cast<double>(r[c]);
+            red[0] = __builtin_fma(rd, rd, red[0]);
+            red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(rd));
+        }
+        dev::store_row<BS>(y + i0, r);
+    }
     __device__ void finalize(const double* t) const
     {
         const double nrm = dev::block_norm(t[0], t[1], y, n);

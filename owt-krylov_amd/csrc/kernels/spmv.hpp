@@ -109,6 +109,19 @@ struct EpiResidual : EpiNone {
         red[0] = __builtin_fma(r, r, red[0]);
         red[1] = combine<Combine::max>(red[1], __builtin_fabs(r));
     }
+    template<int BS>
+    __device__ void elem_row(std::int64_t i0, const double (&ax)[BS], double* red) const
+    {
+        double r[BS];
+        load_row<BS>(b + i0, r);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            r[c] = r[c] - ax[c];
+            red[0] = __builtin_fma(r[c], r[c], red[0]);
+            red[1] = combine<Combine::max>(red[1], __builtin_fabs(r[c]));
+        }
+        store_row<BS>(y + i0, r);
+    }
     __device__ void finalize(const double* t) const
     {
         const double nrm = block_norm(t[0], t[1], y, n);
@@ -129,6 +142,15 @@ struct EpiDot1 : EpiNone {
         y[i] = v;
         red[0] = __builtin_fma(w[i], v, red[0]);
     }
+    template<int BS>
+    __device__ void elem_row(std::int64_t i0, const double (&v)[BS], double* red) const
+    {
+        double ww[BS];
+        load_row<BS>(w + i0, ww);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) red[0] = __builtin_fma(ww[c], v[c], red[0]);
+        store_row<BS>(y + i0, v);
+    }
     __device__ void finalize(const double* t) const
     {
         if (threadIdx.x == 0) fin(t[0]);
@@ -148,6 +170,18 @@ struct EpiDot2 : EpiNone {
         y[i] = v;
         red[0] = __builtin_fma(v, w[i], red[0]);
         red[1] = __builtin_fma(v, v, red[1]);
+    }
+    template<int BS>
+    __device__ void elem_row(std::int64_t i0, const double (&v)[BS], double* red) const
+    {
+        double ww[BS];
+        load_row<BS>(w + i0, ww);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            red[0] = __builtin_fma(v[c], ww[c], red[0]);
+            red[1] = __builtin_fma(v[c], v[c], red[1]);
+        }
+        store_row<BS>(y + i0, v);
     }
     __device__ void finalize(const double* t) const
     {

@@ -1014,14 +1014,16 @@ void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const 
     const std::int64_t nb = pc->n();
     const std::int64_t step_bytes = learned_step_bytes(pc);
     const int degree = st->model->degree;
-    if (degree == 0) {
+    // The fused first step gathers `in` while it writes `out`, so an in-place
+    // apply takes the separate initialisation kernel instead.
+    const bool fuse = degree > 0 && in != out;
+    if (!fuse) {
         k_poly_init<X, Dv><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, dinv, st->gamma.get(), ua,
                                                                                      out, stop);
         OWTK_LAUNCH_CHECK();
-        return;
     }
     static_assert(std::is_same_v<X, Dv>, "series steps read the inverse diagonal in the vector type");
-    {
+    if (fuse) {
         dev::EpiPolyFirst<X> e;
         e.stop = stop;
         e.u_next = ub;
@@ -1035,7 +1037,7 @@ void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const 
         offdiag_spmv<V, X, dev::EpiPolyFirst<X>>(m, vals, in, e);
         std::swap(ua, ub);
     }
-    for (int k = 2; k <= degree; ++k) {
+    for (int k = fuse ? 2 : 1; k <= degree; ++k) {
         dev::EpiPoly<X> e;
         e.stop = stop;
         e.u_next = ub;
