@@ -341,6 +341,25 @@ inline owtk_solver_result bicgstab(DeviceMatrix& a, const Preconditioner* pc, co
     return r;
 }
 
+// pipelined_bicgstab (pipelined_bicgstab.hpp:20-244)
+inline owtk_solver_result pipelined_bicgstab(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
+                                             DeviceVector<double>& x, const owtk_solver_options& o)
+{
+    owtk_solver_result r;
+    check(owtk_pipelined_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
+    return r;
+}
+
+// communication_hiding_bicgstab (pipelined_bicgstab.hpp:317-597)
+inline owtk_solver_result communication_hiding_bicgstab(DeviceMatrix& a, const Preconditioner* pc,
+                                                        const DeviceVector<double>& b, DeviceVector<double>& x,
+                                                        const owtk_solver_options& o)
+{
+    owtk_solver_result r;
+    check(owtk_communication_hiding_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
+    return r;
+}
+
 inline owtk_solver_result fgmres(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
                                  DeviceVector<double>& x, const owtk_solver_options& o)
 {

@@ -123,5 +123,26 @@ int main()
         std::printf("layout mismatch not rejected\n");
         return 1;
     }
+    // the production selection on device vectors: split SSOR + pipelined BiCGSTAB
+    Context ctx(0);
+    DeviceMatrix dm(ctx, s);
+    auto pp = Preconditioner::params(OWTK_PC_SSOR);
+    Preconditioner ssor(dm, pp);
+    DeviceVector<double> db(ctx, 3000), dx(ctx, 3000);
+    db.upload(b.v.data(), 3000);
+    owtk_solver_options o;
+    owtk_solver_options_default(&o);
+    o.relative_tolerance = 1e-10;
+    const owtk_solver_result pr = pipelined_bicgstab(dm, &ssor, db, dx, o);
+    Vec px(3000, 3000), pr_ax(3000, 3000);
+    dx.download(px.v.data(), 3000);
+    host_apply(s, px, pr_ax);
+    double prn = 0;
+    for (std::size_t i = 0; i < 3000; ++i) prn += (b.v[i] - pr_ax.v[i]) * (b.v[i] - pr_ax.v[i]);
+    const double prel = std::sqrt(prn / bn);
+    std::printf("pipelined bicgstab + ssor: status %d iterations %llu true rel %.3e reductions %llu\n", pr.status,
+                static_cast<unsigned long long>(pr.iterations), prel,
+                static_cast<unsigned long long>(pr.global_reductions));
+    if (pr.status != 0 || prel > 1e-10) return 1;
     return (res.status == Status::converged && rel <= 1e-10) ? 0 : 1;
 }
