@@ -310,6 +310,7 @@ owtk_solver_options to_c(const Options& o)
     c.residual_replacement_interval = o.residual_replacement_interval;
     c.gmres_orthogonalization = static_cast<std::int32_t>(o.gmres_orthogonalization);
     c.collect_timings = o.collect_timings ? 1 : 0;
+    if constexpr (requires { o.idr_shadow_space; }) c.idr_shadow_space = o.idr_shadow_space;
     return c;
 }
 
@@ -357,6 +358,15 @@ inline owtk_solver_result communication_hiding_bicgstab(DeviceMatrix& a, const P
 {
     owtk_solver_result r;
     check(owtk_communication_hiding_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
+    return r;
+}
+
+// idrs (idrs.hpp:128-505), s = o.idr_shadow_space
+inline owtk_solver_result idrs(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
+                               DeviceVector<double>& x, const owtk_solver_options& o)
+{
+    owtk_solver_result r;
+    check(owtk_idrs(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
     return r;
 }
 

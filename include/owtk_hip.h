@@ -227,6 +227,7 @@ typedef struct owtk_solver_options {
     uint64_t residual_replacement_interval;
     int32_t gmres_orthogonalization; /* 0 iterated CGS, 1 adaptive CGS, 2 one-sync CGS */
     int32_t collect_timings;
+    uint64_t idr_shadow_space;       /* IDR(s): s (SolverOptions::idr_shadow_space, core.hpp:97) */
 } owtk_solver_options;
 
 /* Mirrors SolverResult<T> (core.hpp:118-146); status and breakdown_reason use
@@ -272,6 +273,15 @@ int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, dou
  * recurrence (a failed restart is a biorthogonality_loss breakdown). */
 int owtk_communication_hiding_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                                        const owtk_solver_options* options, owtk_solver_result* result);
+/* idrs(op, b, x, options, pc) (idrs.hpp:128-505): right-preconditioned IDR(s)
+ * with s = options->idr_shadow_space (2..16 on the device; 1 is routed through
+ * bicgstab as in the reference).  Shadow vectors are the reference's draws
+ * (std::mt19937_64 seeded 42 + restart * 0x9e3779b97f4a7c15, normal
+ * distribution), orthonormalised on the device; the moment matrix and small
+ * dense solves run on the host.  Counters follow the reference's: one
+ * global reduction per inner product or norm it takes. */
+int owtk_idrs(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+              const owtk_solver_options* options, owtk_solver_result* result);
 /* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident. */
 int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                 const owtk_solver_options* options, owtk_solver_result* result);

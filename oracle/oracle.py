@@ -32,7 +32,7 @@ class Options(C.Structure):
                 ("breakdown_tolerance", C.c_double), ("maximum_iterations", C.c_uint64),
                 ("convergence_check_interval", C.c_uint64), ("restart", C.c_uint64),
                 ("residual_replacement_interval", C.c_uint64), ("gmres_orthogonalization", C.c_int32),
-                ("collect_timings", C.c_int32)]
+                ("collect_timings", C.c_int32), ("idr_shadow_space", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -76,6 +76,7 @@ def lib():
         L.oracle_pipelined_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                                            C.POINTER(Result)]
+        L.oracle_idrs.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         _lib = L
     return _lib
 
@@ -188,7 +189,7 @@ class OPreconditioner:
 
 
 def options(**kw) -> Options:
-    o = Options(1e-8, 0.0, 64 * np.finfo(np.float64).eps, 1000, 10, 30, 50, 0, 0)
+    o = Options(1e-8, 0.0, 64 * np.finfo(np.float64).eps, 1000, 10, 30, 50, 0, 0, 4)
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -223,6 +224,16 @@ def communication_hiding_bicgstab(matrix: OMatrix, pc: OPreconditioner | None, b
     r = Result()
     _check(lib().oracle_communication_hiding_bicgstab(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x),
                                                       C.byref(o), C.byref(r)))
+    return x, r.as_dict()
+
+
+def idrs(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    r = Result()
+    _check(lib().oracle_idrs(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o), C.byref(r)))
     return x, r.as_dict()
 
 

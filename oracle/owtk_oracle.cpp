@@ -28,6 +28,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -593,6 +594,8 @@ enum Reason {
     alpha_denominator = 4,
     omega_denominator = 5,
     omega_zero = 6,
+    idr_shadow_space = 7,
+    idr_small_system = 8,
     non_finite_scalar = 9,
 };
 
@@ -604,6 +607,7 @@ typedef struct oracle_options {
     double relative_tolerance, absolute_tolerance, breakdown_tolerance;
     std::uint64_t maximum_iterations, convergence_check_interval, restart, residual_replacement_interval;
     std::int32_t gmres_orthogonalization, collect_timings;
+    std::uint64_t idr_shadow_space;
 } oracle_options;
 
 typedef struct oracle_result {
@@ -1133,6 +1137,309 @@ void gmres_update(std::int64_t n, const double* base, const std::vector<std::vec
     }
 }
 
+// solve_dense_system (idrs.hpp:34-77): Gaussian elimination with partial
+// pivoting, false when a pivot is non-finite or at most `tol`.
+bool solve_dense(std::vector<double> a, std::vector<double> rhs, std::vector<double>& out, double tol)
+{
+    const std::size_t n = rhs.size();
+    out.assign(n, 0.0);
+    for (std::size_t col = 0; col < n; ++col) {
+        std::size_t piv = col;
+        for (std::size_t row = col + 1; row < n; ++row)
+            if (std::abs(a[row * n + col]) > std::abs(a[piv * n + col])) piv = row;
+        if (!std::isfinite(a[piv * n + col]) || std::abs(a[piv * n + col]) <= tol) return false;
+        if (piv != col) {
+            for (std::size_t e = col; e < n; ++e) std::swap(a[col * n + e], a[piv * n + e]);
+            std::swap(rhs[col], rhs[piv]);
+        }
+        for (std::size_t row = col + 1; row < n; ++row) {
+            const double f = a[row * n + col] / a[col * n + col];
+            for (std::size_t e = col; e < n; ++e) a[row * n + e] -= f * a[col * n + e];
+            rhs[row] -= f * rhs[col];
+        }
+    }
+    for (std::size_t k = n; k-- > 0;) {
+        double v = rhs[k];
+        for (std::size_t col = k + 1; col < n; ++col) v -= a[k * n + col] * out[col];
+        out[k] = v / a[k * n + k];
+    }
+    return true;
+}
+
+// initialize_idr_shadow (idrs.hpp:79-118), serial reduction (rank 0): the
+// draws of std::mt19937_64 seeded 42 + epoch * 0x9e3779b97f4a7c15 through
+// std::normal_distribution<double>, Gram-Schmidt orthonormalised; with s = 1
+// the residual itself, normalised.
+bool idr_shadow(std::vector<std::vector<double>>& shadow, const double* residual, std::int64_t n, Solve& sv,
+                double tol, std::uint64_t epoch)
+{
+    std::mt19937_64 gen(42ULL + epoch * 0x9e3779b97f4a7c15ULL);
+    std::normal_distribution<double> dist(0.0, 1.0);
+    const std::size_t first = shadow.size() == 1 ? 1 : 0;
+    if (shadow.size() == 1) {
+        std::copy_n(residual, n, shadow[0].begin());
+        const double nv = sv.nrm(shadow[0].data());
+        if (nv <= tol) return false;
+        for (std::int64_t i = 0; i < n; ++i) shadow[0][i] *= 1.0 / nv;
+    }
+    for (std::size_t k = first; k < shadow.size(); ++k) {
+        for (std::int64_t i = 0; i < n; ++i) shadow[k][i] = dist(gen);
+        for (std::size_t p = 0; p < k; ++p) {
+            const double proj = sv.d(shadow[p].data(), shadow[k].data());
+            for (std::int64_t i = 0; i < n; ++i) shadow[k][i] = std::fma(-proj, shadow[p][i], shadow[k][i]);
+        }
+        const double nv = sv.nrm(shadow[k].data());
+        if (nv <= tol) return false;
+        const double inv = 1.0 / nv;
+        for (std::int64_t i = 0; i < n; ++i) shadow[k][i] *= inv;
+    }
+    return true;
+}
+
+void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r);
+
+// idrs (idrs.hpp:128-505): right-preconditioned IDR(s), serial reduction.
+void idrs(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
+{
+    init_result(r);
+    const std::size_t s = static_cast<std::size_t>(o.idr_shadow_space);
+    const std::int64_t n = a.owned * a.bs;
+    if (!valid_options(o) || s == 0 || s > static_cast<std::size_t>(n)) return;
+    if (s == 1) {  // IDR(1) is BiCGSTAB's recurrence
+        bicgstab(a, pc, b, x, o, r);
+        return;
+    }
+    Solve sv(a, pc, r);
+    const std::int64_t L = sv.local;
+    std::vector<double> residual(L), projected(L), pre(L), work(L), tmp(L);
+    std::vector<std::vector<double>> G(s, std::vector<double>(L)), Z(s, std::vector<double>(L)),
+        P(s, std::vector<double>(L));
+    auto mark = [&](Reason reason) {
+        r.status = breakdown;
+        r.breakdown_reason = reason;
+    };
+    auto axpy = [&](double alpha, const std::vector<double>& xv, std::vector<double>& yv) {
+        for (std::int64_t i = 0; i < n; ++i) yv[i] = std::fma(alpha, xv[i], yv[i]);
+    };
+    auto axpy_x = [&](double alpha, const std::vector<double>& xv) {
+        for (std::int64_t i = 0; i < n; ++i) x[i] = std::fma(alpha, xv[i], x[i]);
+    };
+    sv.true_residual(b, x, residual.data(), tmp.data());
+    const double rhs_norm = sv.nrm(b);
+    double rnorm = sv.nrm(residual.data());
+    const double thr = threshold_of(rhs_norm, o);
+    r.initial_residual_norm = rnorm;
+    sv.set_residual(rnorm, rnorm, rhs_norm);
+    if (!std::isfinite(rnorm) || !std::isfinite(thr)) {
+        mark(non_finite_scalar);
+        return;
+    }
+    if (rnorm <= thr) {
+        r.status = converged;
+        return;
+    }
+    std::vector<double> M(s * s, 0.0), f(s, 0.0), am, arhs, c;
+    std::size_t epoch = 0, restarts = 0;
+    double omega = 1.0;
+    auto restart_basis = [&]() -> bool {
+        if (!idr_shadow(P, residual.data(), n, sv, o.breakdown_tolerance, epoch++)) return false;
+        std::fill(M.begin(), M.end(), 0.0);
+        for (std::size_t k = 0; k < s; ++k) {
+            M[k * s + k] = 1.0;
+            std::fill_n(G[k].begin(), n, 0.0);
+            std::fill_n(Z[k].begin(), n, 0.0);
+            f[k] = sv.d(P[k].data(), residual.data());
+        }
+        omega = 1.0;
+        return true;
+    };
+    auto restart_or_fail = [&]() -> bool {
+        ++restarts;
+        return restarts <= 4 && restart_basis();
+    };
+    auto pivot_small = [&](std::size_t col) -> bool {
+        double scale = 0.0;
+        for (std::size_t row = col; row < s; ++row) scale = std::max(scale, std::abs(M[row * s + col]));
+        const double piv = M[col * s + col];
+        const double tol = std::max(std::numeric_limits<double>::min(), o.breakdown_tolerance * scale);
+        return !std::isfinite(piv) || std::abs(piv) <= tol;
+    };
+    if (!restart_basis()) {
+        mark(idr_shadow_space);
+        return;
+    }
+    while (r.iterations < o.maximum_iterations) {
+        bool restarted = false;
+        for (std::size_t k = 0; k < s && r.iterations < o.maximum_iterations; ++k) {
+            const std::size_t act = s - k;
+            am.assign(act * act, 0.0);
+            arhs.assign(act, 0.0);
+            double mscale = 0.0;
+            for (std::size_t row = 0; row < act; ++row) {
+                arhs[row] = f[k + row];
+                for (std::size_t col = 0; col < act; ++col) {
+                    const double v = M[(k + row) * s + (k + col)];
+                    am[row * act + col] = v;
+                    mscale = std::max(mscale, std::abs(v));
+                }
+            }
+            const double dtol = std::max(std::numeric_limits<double>::min(),
+                                         64 * std::numeric_limits<double>::epsilon() * mscale);
+            if (!solve_dense(am, arhs, c, dtol)) {
+                if (!restart_or_fail()) {
+                    mark(idr_small_system);
+                    return;
+                }
+                restarted = true;
+                break;
+            }
+            // v = r - G_k:s c; z = omega M^-1 v + Z_k:s c
+            std::copy_n(residual.begin(), n, projected.begin());
+            for (std::size_t col = k; col < s; ++col) axpy(-c[col - k], G[col], projected);
+            sv.prec(projected.data(), pre.data());
+            for (std::int64_t i = 0; i < n; ++i) pre[i] *= omega;
+            for (std::size_t col = k; col < s; ++col) axpy(c[col - k], Z[col], pre);
+            sv.op(pre.data(), work.data());
+            for (std::size_t prev = 0; prev < k; ++prev) {
+                const double piv = M[prev * s + prev];
+                if (pivot_small(prev)) {
+                    if (!restart_or_fail()) {
+                        mark(idr_small_system);
+                        return;
+                    }
+                    restarted = true;
+                    break;
+                }
+                const double alpha = sv.d(P[prev].data(), work.data()) / piv;
+                axpy(-alpha, G[prev], work);
+                axpy(-alpha, Z[prev], pre);
+            }
+            if (restarted) break;
+            for (std::size_t row = 0; row < k; ++row) M[row * s + k] = 0.0;
+            for (std::size_t row = k; row < s; ++row) M[row * s + k] = sv.d(P[row].data(), work.data());
+            const double piv = M[k * s + k];
+            if (pivot_small(k)) {
+                if (!restart_or_fail()) {
+                    mark(idr_small_system);
+                    return;
+                }
+                restarted = true;
+                break;
+            }
+            std::copy_n(work.begin(), n, G[k].begin());
+            std::copy_n(pre.begin(), n, Z[k].begin());
+            const double beta = f[k] / piv;
+            axpy_x(beta, Z[k]);
+            axpy(-beta, G[k], residual);
+            for (std::size_t row = k + 1; row < s; ++row) f[row] -= beta * M[row * s + k];
+            f[k] = 0.0;
+            ++r.iterations;
+            restarts = 0;
+            rnorm = sv.nrm(residual.data());
+            r.recursive_residual_norm = rnorm;
+            if (!std::isfinite(rnorm)) {
+                mark(non_finite_scalar);
+                return;
+            }
+            if (rnorm <= thr) {
+                sv.true_residual(b, x, residual.data(), tmp.data());
+                const double tn = sv.nrm(residual.data());
+                sv.set_residual(rnorm, tn, rhs_norm);
+                if (tn <= thr) {
+                    r.status = converged;
+                    return;
+                }
+                rnorm = tn;
+                if (!restart_or_fail()) {
+                    r.status = diverged;
+                    return;
+                }
+                restarted = true;
+                break;
+            }
+            if (o.residual_replacement_interval > 0 && r.iterations % o.residual_replacement_interval == 0) {
+                sv.true_residual(b, x, residual.data(), tmp.data());
+                rnorm = sv.nrm(residual.data());
+                sv.set_residual(rnorm, rnorm, rhs_norm);
+                if (rnorm <= thr) {
+                    r.status = converged;
+                    return;
+                }
+                if (!restart_basis()) {
+                    mark(idr_shadow_space);
+                    return;
+                }
+                restarted = true;
+                break;
+            }
+        }
+        if (restarted) continue;
+        if (r.iterations >= o.maximum_iterations) break;
+        // stabilising minimum-residual step, angle-guarded (omega >= 0.7 of the angle)
+        sv.prec(residual.data(), pre.data());
+        sv.op(pre.data(), work.data());
+        const double num = sv.d(work.data(), residual.data());
+        const double den = sv.d(work.data(), work.data());
+        if (!std::isfinite(den) || den <= std::numeric_limits<double>::min()) {
+            mark(omega_denominator);
+            return;
+        }
+        omega = num / den;
+        const double adn = std::sqrt(std::max(0.0, den)) * std::max(rnorm, std::numeric_limits<double>::min());
+        const double angle = adn > 0 ? std::abs(num) / adn : 1.0;
+        constexpr double min_angle = 0.7;
+        if (angle > 0 && angle < min_angle) omega *= min_angle / angle;
+        if (!std::isfinite(omega) || std::abs(omega) <= std::numeric_limits<double>::min()) {
+            mark(omega_denominator);
+            return;
+        }
+        axpy_x(omega, pre);
+        axpy(-omega, work, residual);
+        ++r.iterations;
+        restarts = 0;
+        rnorm = sv.nrm(residual.data());
+        r.recursive_residual_norm = rnorm;
+        if (!std::isfinite(rnorm)) {
+            mark(non_finite_scalar);
+            return;
+        }
+        if (rnorm <= thr) {
+            sv.true_residual(b, x, residual.data(), tmp.data());
+            const double tn = sv.nrm(residual.data());
+            sv.set_residual(rnorm, tn, rhs_norm);
+            if (tn <= thr) {
+                r.status = converged;
+                return;
+            }
+            rnorm = tn;
+            if (!restart_or_fail()) {
+                r.status = diverged;
+                return;
+            }
+            continue;
+        }
+        if (o.residual_replacement_interval > 0 && r.iterations % o.residual_replacement_interval == 0) {
+            sv.true_residual(b, x, residual.data(), tmp.data());
+            rnorm = sv.nrm(residual.data());
+            sv.set_residual(rnorm, rnorm, rhs_norm);
+            if (rnorm <= thr) {
+                r.status = converged;
+                return;
+            }
+            if (!restart_basis()) {
+                mark(idr_shadow_space);
+                return;
+            }
+            continue;
+        }
+        for (std::size_t row = 0; row < s; ++row) f[row] = sv.d(P[row].data(), residual.data());
+    }
+    sv.true_residual(b, x, residual.data(), tmp.data());
+    const double tn = sv.nrm(residual.data());
+    sv.set_residual(rnorm, tn, rhs_norm);
+    r.status = maximum_iterations;
+}
+
 // fgmres (krylov_solvers.hpp:159-486)
 void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
 {
@@ -1457,6 +1764,17 @@ int oracle_communication_hiding_bicgstab(const oracle_matrix* m, void* pc_handle
         Identity id(a.owned * a.bs);
         Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
         communication_hiding_bicgstab(a, pc, b, x, *o, *r);
+    });
+}
+
+int oracle_idrs(const oracle_matrix* m, void* pc_handle, const double* b, double* x, const oracle_options* o,
+                oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        idrs(a, pc, b, x, *o, *r);
     });
 }
 

@@ -928,6 +928,7 @@ void owtk_solver_options_default(owtk_solver_options* o)
     o->residual_replacement_interval = 50;
     o->gmres_orthogonalization = 0;
     o->collect_timings = 0;
+    o->idr_shadow_space = 4;
 }
 
 void owtk_refinement_options_default(owtk_refinement_options* o)

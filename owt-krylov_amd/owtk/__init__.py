@@ -20,7 +20,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
 
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
-    "communication_hiding_bicgstab", "fgmres",
+    "communication_hiding_bicgstab", "idrs", "fgmres",
     "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
@@ -496,6 +496,11 @@ def pipelined_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffe
                        options=None) -> dict:
     """pipelined_bicgstab (pipelined_bicgstab.hpp:20-244) on the device."""
     return _solve(lib().owtk_pipelined_bicgstab, matrix, pc, b, x, options)
+
+
+def idrs(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
+    """idrs (idrs.hpp:128-505) on the device; s = options.idr_shadow_space."""
+    return _solve(lib().owtk_idrs, matrix, pc, b, x, options)
 
 
 def communication_hiding_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,

@@ -83,7 +83,7 @@ class SolverOptions(C.Structure):
         ("breakdown_tolerance", C.c_double), ("maximum_iterations", C.c_uint64),
         ("convergence_check_interval", C.c_uint64), ("restart", C.c_uint64),
         ("residual_replacement_interval", C.c_uint64), ("gmres_orthogonalization", C.c_int32),
-        ("collect_timings", C.c_int32),
+        ("collect_timings", C.c_int32), ("idr_shadow_space", C.c_uint64),
     ]
 
 
@@ -172,6 +172,7 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_solver_options_default": (None, [C.POINTER(SolverOptions)]),
     "owtk_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_pipelined_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
+    "owtk_idrs": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
     "owtk_communication_hiding_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions),
                                                      C.POINTER(SolverResult)]),
     "owtk_fgmres": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),

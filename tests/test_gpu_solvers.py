@@ -241,6 +241,80 @@ def test_communication_hiding_c0_matches_oracle(ctx, c0, pc_kind, replace):
             assert res[key] == ro[key], key
 
 
+@pytest.mark.parametrize("sh", [2, 4])
+def test_idrs_recurrence_vs_oracle(ctx, sh):
+    """Device IDR(s) (idrs.hpp:128-505) on the audit's 12 x 12 matrix: the
+    iterates of the first steps agree with the oracle (same shadow draws),
+    counters match, and the solve terminates within n + n/s steps (the IDR
+    theorem) at the dense solution."""
+    import math
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)
+    om = orc.OMatrix.csr(12, 0, 1, offsets, columns, values)
+    for count in range(1, 7):
+        kw = dict(relative_tolerance=0.0, maximum_iterations=count, residual_replacement_interval=0,
+                  idr_shadow_space=sh)
+        x, res = solve(ctx, owtk.idrs, m, None, rhs, x0, **kw)
+        xo, ro = orc.idrs(om, None, rhs, x0=x0, **kw)
+        assert res["status_name"] == "maximum_iterations" and res["iterations"] == count == ro["iterations"]
+        assert np.max(np.abs(x[:12] - xo[:12])) < 1e-10 * np.max(np.abs(xo[:12])), count
+        for key in COUNTERS:
+            assert res[key] == ro[key], (count, key)
+    exact = np.linalg.solve(a.astype(np.float64), rhs)
+    x, res = solve(ctx, owtk.idrs, m, None, rhs, x0, relative_tolerance=1e-13, idr_shadow_space=sh,
+                   residual_replacement_interval=0)
+    assert res["status_name"] == "converged" and res["iterations"] <= 12 + math.ceil(12 / sh)
+    assert np.max(np.abs(x[:12] - exact)) <= 1e-12 * np.max(np.abs(exact))
+
+
+def test_idrs_edge_cases(ctx, c0):
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)
+    # IDR(1) is the BiCGSTAB recurrence
+    x1, r1 = solve(ctx, owtk.idrs, m, None, rhs, x0, idr_shadow_space=1, relative_tolerance=1e-12)
+    xb, rb = solve(ctx, owtk.bicgstab, m, None, rhs, x0, relative_tolerance=1e-12)
+    assert np.array_equal(x1, xb) and r1["iterations"] == rb["iterations"]
+    # s beyond the system dimension is invalid input; beyond 16 unsupported on the device
+    _, r = solve(ctx, owtk.idrs, m, None, rhs, x0, idr_shadow_space=13)
+    assert r["status_name"] == "invalid_input"
+    s = c0
+    ms = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    with pytest.raises(owtk.UnsupportedError):
+        solve(ctx, owtk.idrs, ms, None, s.rhs, idr_shadow_space=17)
+    # the reference's scaled right-hand side (test_core.cpp:784-798) and non-finite data
+    pc = owtk.Preconditioner(ms, owtk.PC_JACOBI)
+    _, r = solve(ctx, owtk.idrs, ms, pc, s.rhs * 1e-18, idr_shadow_space=2)
+    assert r["status_name"] == "converged"
+    one = owtk.Matrix.csr(ctx, 2, 0, 1, [0, 1, 2], [0, 1], [1.0, 1.0])
+    for bad in (np.nan, np.inf):
+        _, r = solve(ctx, owtk.idrs, one, None, np.array([bad, 1.0]), idr_shadow_space=2, maximum_iterations=3)
+        assert r["status_name"] != "converged" and not np.isfinite(r["initial_residual_norm"])
+
+
+@pytest.mark.parametrize("pc_kind", ["jacobi", "ssor", "ilu"])
+@pytest.mark.parametrize("sh", [2, 4, 8])
+def test_idrs_c0_matches_oracle(ctx, c0, pc_kind, sh):
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    kind = {"jacobi": (1, owtk.PC_JACOBI), "ssor": (2, owtk.PC_SSOR), "ilu": (3, owtk.PC_ILU)}[pc_kind]
+    extra = {"block_nodes": 512} if pc_kind == "ilu" else {}
+    opc, pc = orc.OPreconditioner(om, kind[0], **extra), owtk.Preconditioner(m, kind[1], **extra)
+    xo, ro = orc.idrs(om, opc, s.rhs, idr_shadow_space=sh)
+    x, res = solve(ctx, owtk.idrs, m, pc, s.rhs, idr_shadow_space=sh)
+    print(pc_kind, sh, "oracle", ro["iterations"], ro["relative_residual_norm"], "gpu", res["iterations"],
+          res["relative_residual_norm"], flush=True)
+    assert res["status_name"] == "converged" and ro["status"] == 0
+    # the same recurrence and shadow draws; reductions round differently
+    assert abs(res["iterations"] - ro["iterations"]) <= max(2, ro["iterations"] // 20)
+    true_rel = np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs)
+    assert true_rel <= 1e-8
+    assert abs(res["relative_residual_norm"] - true_rel) <= 1e-3 * true_rel
+    if res["iterations"] == ro["iterations"]:
+        for key in COUNTERS:
+            assert res[key] == ro[key], key
+
+
 def test_fgmres_direct_solve(ctx):
     a, offsets, columns, values, rhs, _ = rc.recurrence_matrix()
     m = owtk.Matrix.csr(ctx, 12, 0, 1, offsets, columns, values)

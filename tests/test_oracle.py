@@ -376,3 +376,56 @@ def test_communication_hiding_bicgstab_breakdowns():
     om = orc.OMatrix.csr(1, 0, 2, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([1.0, 1.0]))
     _, res = orc.communication_hiding_bicgstab(om, None, np.array([np.inf, 1.0]))
     assert res["status"] == 2 and res["breakdown_reason"] == 9
+
+
+@pytest.mark.parametrize("s", [2, 3, 4, 6, 12])
+def test_idrs_finite_termination(s):
+    """IDR(s) (idrs.hpp:128-505) reaches the exact solution of an n x n
+    system within n + n/s matrix-vector steps (Sonneveld & van Gijzen, SIAM J.
+    Sci. Comput. 31 (2008), the IDR theorem): the oracle on the audit's 12 x 12
+    nonsymmetric matrix stops within that bound at the solution of
+    numpy.linalg.solve."""
+    import math
+    import refcases as rc
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    n = a.shape[0]
+    om = orc.OMatrix.csr(n, 0, 1, np.array(offsets, np.int32), np.array(columns, np.int32), np.array(values))
+    exact = np.linalg.solve(a.astype(np.float64), rhs)
+    x, res = orc.idrs(om, None, rhs, x0=x0, relative_tolerance=1e-13, idr_shadow_space=s,
+                      residual_replacement_interval=0)
+    assert res["status"] == 0
+    assert res["iterations"] <= n + math.ceil(n / s)
+    assert np.max(np.abs(x[:n] - exact)) <= 1e-13 * np.max(np.abs(exact))
+
+
+def test_idrs_one_is_bicgstab():
+    """IDR(1) is routed through the BiCGSTAB recurrence (idrs.hpp:165-172)."""
+    import refcases as rc
+    a, offsets, columns, values, rhs, x0 = rc.recurrence_matrix()
+    om = orc.OMatrix.csr(12, 0, 1, np.array(offsets, np.int32), np.array(columns, np.int32), np.array(values))
+    xb, rb = orc.bicgstab(om, None, rhs, x0=x0, relative_tolerance=1e-13)
+    x1, r1 = orc.idrs(om, None, rhs, x0=x0, relative_tolerance=1e-13, idr_shadow_space=1)
+    assert np.array_equal(xb, x1) and rb == r1
+
+
+def test_idrs_reference_cases(c0):
+    """The reference's own IDR cases (test_core.cpp:776-798,
+    test_foundation.cpp:110-145): IDR(2) + Jacobi converges; the same with the
+    right-hand side scaled by 1e-18 (pivots are relative to their column);
+    non-finite data is never accepted."""
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    pc = orc.OPreconditioner(om, 1)
+    x, res = orc.idrs(om, pc, s.rhs, idr_shadow_space=2)
+    assert res["status"] == 0
+    assert np.linalg.norm(s.rhs - om.spmv(x)) <= 1e-8 * np.linalg.norm(s.rhs)
+    xs, rs = orc.idrs(om, pc, s.rhs * 1e-18, idr_shadow_space=2)
+    assert rs["status"] == 0
+    one = orc.OMatrix.csr(2, 0, 1, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([1.0, 1.0]))
+    for bad in (np.nan, np.inf):
+        for in_rhs in (True, False):
+            b = np.ones(2)
+            x0 = np.ones(2)
+            (b if in_rhs else x0)[0] = bad
+            _, r = orc.idrs(one, None, b, x0=x0, idr_shadow_space=2, maximum_iterations=3)
+            assert r["status"] != 0 and not np.isfinite(r["initial_residual_norm"])
