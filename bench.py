@@ -395,8 +395,9 @@ def bsr_bench(args):
 def pipelined_bench(args):
     """The reference's production selection (split SSOR + pipelined BiCGSTAB)
     on the 2048x1024 grid: FP64 pipelined and communication-hiding BiCGSTAB
-    beside the standard recurrence, all with block-Jacobi SSOR (4096-vertex
-    blocks, the one-GPU analogue of the per-rank SSOR), rtol 1e-8."""
+    beside the standard recurrence and IDR(4), all with block-Jacobi SSOR
+    (4096-vertex blocks, the one-GPU analogue of the per-rank SSOR), rtol
+    1e-8."""
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
         n = s.nodes * BS
@@ -407,11 +408,13 @@ def pipelined_bench(args):
         pc = owtk.Preconditioner(m, owtk.PC_SSOR, block_nodes=BLOCK_NODES)
         opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=2000)
         out_methods = {}
-        for name, fn in (("pipelined_bicgstab", owtk.pipelined_bicgstab),
-                         ("communication_hiding_bicgstab", owtk.communication_hiding_bicgstab),
-                         ("bicgstab", owtk.bicgstab)):
+        idr_opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=2000, idr_shadow_space=4)
+        for name, fn, o in (("pipelined_bicgstab", owtk.pipelined_bicgstab, opts),
+                            ("communication_hiding_bicgstab", owtk.communication_hiding_bicgstab, opts),
+                            ("bicgstab", owtk.bicgstab, opts),
+                            ("idrs4", owtk.idrs, idr_opts)):
             times, results, prof = timed_solves(ctx, f"{name}+ssor", args.steps, args.warmup,
-                                                lambda f=fn: f(m, pc, b, x, opts), x)
+                                                lambda f=fn, oo=o: f(m, pc, b, x, oo), x)
             res = results[-1]
             rel = true_rel(ctx, m, b, x, r, bnorm)
             ms = float(np.median(times))
@@ -422,6 +425,9 @@ def pipelined_bench(args):
                 "preconditioner_applications": res["preconditioner_applications"],
                 "ssor_apply_ms": round(prof["precond_f64"]["avg_ms"], 4),
                 "spmv_ms": round(prof["spmv_f64"]["avg_ms"], 4)}
+            if name.startswith("idrs"):
+                out_methods[name]["note"] = ("shadow-space draws (seeded std::mt19937_64, as the reference) are "
+                                             "generated on the host in the warm-up solve and kept per context")
         head = out_methods["pipelined_bicgstab"]
         cpu = None
         if not args.no_cpu:

@@ -7,8 +7,9 @@
 // reference performs as a sequence of axpys over one output run as one pass
 // with the same per-element operation order.  The shadow vectors are the
 // reference's draws (std::mt19937_64 + std::normal_distribution<double>,
-// seed 42 + restart * 0x9e3779b97f4a7c15), orthonormalised on the device.
-// IDR(1) is the BiCGSTAB recurrence, as in the reference.
+// seed 42 + restart * 0x9e3779b97f4a7c15), orthonormalised on the device;
+// the draws of a size and restart are generated once per context.  IDR(1)
+// is the BiCGSTAB recurrence, as in the reference.
 #include "precond.hpp"
 #include "solver_common.hpp"
 #include "kernels/blas1.hpp"
@@ -228,14 +229,35 @@ int idrs_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const o
     std::uint64_t epoch = 0;
     int restarts = 0;
     double omega = 1.0;
-    std::vector<double> draws(static_cast<std::size_t>(n));
+    // The raw draws of one restart epoch: s x n normals from the seeded
+    // generator, on the host (the sequence is serial), then kept in the
+    // context for later solves of the same size (up to kDrawCacheBytes).
+    constexpr std::size_t kDrawCacheBytes = std::size_t(4) << 30;
+    auto draws_of = [&](std::uint64_t e) -> const double* {
+        const auto key = std::make_tuple(n, s, e);
+        auto it = ctx->idr_draws.find(key);
+        if (it != ctx->idr_draws.end()) return it->second.get();
+        std::vector<double> h(static_cast<std::size_t>(s) * n);
+        std::mt19937_64 gen(42ULL + e * 0x9e3779b97f4a7c15ULL);
+        std::normal_distribution<double> dist(0.0, 1.0);
+        for (auto& d : h) d = dist(gen);
+        const std::size_t bytes = h.size() * sizeof(double);
+        if (ctx->idr_draw_bytes + bytes > kDrawCacheBytes) {
+            ctx->sync();  // earlier copies out of the cached draws have completed
+            ctx->idr_draws.clear();
+            ctx->idr_draw_bytes = 0;
+        }
+        DeviceArray<double> d(h.size());
+        OWTK_HIP(hipMemcpy(d.get(), h.data(), bytes, hipMemcpyHostToDevice));
+        ctx->idr_draw_bytes += bytes;
+        return ctx->idr_draws.emplace(key, std::move(d)).first->second.get();
+    };
     // initialize_idr_shadow (idrs.hpp:79-118), rank 0 of a serial reduction
     auto shadow = [&]() -> bool {
-        std::mt19937_64 gen(42ULL + epoch++ * 0x9e3779b97f4a7c15ULL);
-        std::normal_distribution<double> dist(0.0, 1.0);
+        const double* drawn = draws_of(epoch++);
         for (int k = 0; k < s; ++k) {
-            for (std::int64_t i = 0; i < n; ++i) draws[i] = dist(gen);
-            OWTK_HIP(hipMemcpyAsync(P(k), draws.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+            OWTK_HIP(hipMemcpyAsync(P(k), drawn + static_cast<std::int64_t>(k) * n, n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, st));
             for (int p = 0; p < k; ++p) {
                 const double proj = dotv(P(p), P(k));
                 kern::axpy(st, n, -proj, P(p), P(k));

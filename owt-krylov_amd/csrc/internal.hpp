@@ -9,8 +9,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace owtk {
@@ -108,6 +110,11 @@ struct owtk_context_s {
     std::vector<owtk::ProfileRecord> records;
     std::vector<hipEvent_t> event_pool;
     hipEvent_t region_start = nullptr, region_stop = nullptr;  // owtk_profile_region
+    // IDR(s) shadow draws by (owned entries, s, restart epoch): a fixed
+    // function of those three (seeded std::mt19937_64 + normal_distribution),
+    // kept on the device for later solves of the same size
+    std::map<std::tuple<std::int64_t, int, std::uint64_t>, owtk::DeviceArray<double>> idr_draws;
+    std::size_t idr_draw_bytes = 0;
 
     owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
     void activate() const;  // hipSetDevice(device)
