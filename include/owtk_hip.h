@@ -313,6 +313,43 @@ int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_prec
                                  const double* b, double* x, const owtk_refinement_options* options,
                                  owtk_solver_result* result);
 
+/* -------------------------------------------------------------- multigrid -- */
+/* AggregationMultigrid (multigrid.hpp:41-421): two-level aggregation solver
+ * over a component-diagonal BlockCsrMatrix (OWTK_MATRIX_CSR).  Set-up on the
+ * host as the reference (greedy aggregates, row-averaged coarse matrix,
+ * inverse diagonals, greedy red/black colouring); every cycle operation on
+ * the device.  The operator of solve() is the fine matrix itself. */
+typedef struct owtk_multigrid_s* owtk_multigrid;
+typedef enum owtk_multigrid_smoother {  /* MultigridSmoother (multigrid.hpp:23-27) */
+    OWTK_MG_JACOBI = 0,
+    OWTK_MG_RED_BLACK = 1,  /* red_black_gauss_seidel */
+    OWTK_MG_CHEBYSHEV = 2
+} owtk_multigrid_smoother;
+typedef enum owtk_multigrid_cycle {     /* MultigridCycle (multigrid.hpp:17-21) */
+    OWTK_MG_V = 0,
+    OWTK_MG_W = 1,
+    OWTK_MG_FULL = 2
+} owtk_multigrid_cycle;
+typedef struct owtk_multigrid_options { /* MultigridOptions (multigrid.hpp:29-37) */
+    uint64_t pre_smoothing_steps;       /* 2 */
+    uint64_t post_smoothing_steps;      /* 2 */
+    uint64_t coarse_smoothing_steps;    /* 24 */
+    uint64_t target_aggregate_size;     /* 4 */
+    double jacobi_weight;               /* 0.8 */
+    int32_t smoother;                   /* owtk_multigrid_smoother, OWTK_MG_JACOBI */
+    uint64_t chebyshev_levels;          /* 8 */
+} owtk_multigrid_options;
+void owtk_multigrid_options_default(owtk_multigrid_options* options);
+/* AggregationMultigrid(fine_matrix, options) */
+int owtk_multigrid_create(owtk_matrix fine, const owtk_multigrid_options* options, owtk_multigrid* out);
+int owtk_multigrid_destroy(owtk_multigrid mg);
+/* coarse_matrix() and fine_to_coarse(): NULL arrays for the sizes first */
+int owtk_multigrid_coarse(owtk_multigrid mg, int64_t* nodes, int64_t* entries, int32_t* row_offsets,
+                          int32_t* columns, double* values, int64_t* fine_to_coarse);
+/* solve(fine_operator, rhs, solution, options, cycle) (multigrid.hpp:74-221) */
+int owtk_multigrid_solve(owtk_multigrid mg, const double* b, double* x, const owtk_solver_options* options,
+                         int32_t cycle, owtk_solver_result* result);
+
 /* ------------------------------------------------------------ matrix files -- */
 /* The reference's binary fixture format (benchmarks/matrix_analysis/
  * ilu_fill.cpp:20-44), little-endian: u64 {n, bins, nnz}, u64 offsets[n+1],

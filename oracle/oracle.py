@@ -35,6 +35,12 @@ class Options(C.Structure):
                 ("collect_timings", C.c_int32), ("idr_shadow_space", C.c_uint64)]
 
 
+class MgOptions(C.Structure):
+    _fields_ = [("pre_smoothing_steps", C.c_uint64), ("post_smoothing_steps", C.c_uint64),
+                ("coarse_smoothing_steps", C.c_uint64), ("target_aggregate_size", C.c_uint64),
+                ("jacobi_weight", C.c_double), ("smoother", C.c_int32), ("chebyshev_levels", C.c_uint64)]
+
+
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("breakdown_reason", C.c_int32), ("iterations", C.c_uint64),
                 ("operator_applications", C.c_uint64), ("preconditioner_applications", C.c_uint64),
@@ -77,6 +83,11 @@ def lib():
         L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                                            C.POINTER(Result)]
         L.oracle_idrs.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        L.oracle_mg_create.argtypes = [C.POINTER(Matrix), C.POINTER(MgOptions), C.POINTER(P)]
+        L.oracle_mg_solve.argtypes = [P, P, P, C.POINTER(Options), C.c_int, C.POINTER(Result)]
+        L.oracle_mg_coarse.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), P, P, P, P]
+        L.oracle_mg_destroy.argtypes = [P]
+        L.oracle_mg_destroy.restype = None
         _lib = L
     return _lib
 
@@ -245,3 +256,44 @@ def fgmres(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
     r = Result()
     _check(lib().oracle_fgmres(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o), C.byref(r)))
     return x, r.as_dict()
+
+
+class OMultigrid:
+    """AggregationMultigrid (multigrid.hpp:41-421) over a CSR-kind OMatrix.
+    smoother: 0 Jacobi, 1 red-black Gauss-Seidel, 2 Chebyshev; cycle: 0 V,
+    1 W, 2 full."""
+    JACOBI, RED_BLACK, CHEBYSHEV = 0, 1, 2
+    V, W, FULL = 0, 1, 2
+
+    def __init__(self, matrix: OMatrix, pre=2, post=2, coarse_steps=24, aggregate=4, weight=0.8, smoother=0,
+                 chebyshev_levels=8):
+        self.matrix = matrix
+        o = MgOptions(pre, post, coarse_steps, aggregate, weight, smoother, chebyshev_levels)
+        self.h = C.c_void_p()
+        _check(lib().oracle_mg_create(C.byref(matrix.s), C.byref(o), C.byref(self.h)))
+
+    def coarse(self):
+        nodes, entries = C.c_int64(), C.c_int64()
+        _check(lib().oracle_mg_coarse(self.h, C.byref(nodes), C.byref(entries), None, None, None, None))
+        off = np.zeros(nodes.value + 1, np.int32)
+        col = np.zeros(entries.value, np.int32)
+        val = np.zeros(entries.value * self.matrix.block)
+        f2c = np.zeros(self.matrix.owned, np.int64)
+        _check(lib().oracle_mg_coarse(self.h, C.byref(nodes), C.byref(entries), _p(off), _p(col), _p(val), _p(f2c)))
+        return nodes.value, off, col, val, f2c
+
+    def solve(self, b, x0=None, cycle=0, **kw):
+        b = np.ascontiguousarray(b, np.float64)
+        n_local = (self.matrix.owned + self.matrix.ghost) * self.matrix.block
+        x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+        o = options(**kw)
+        r = Result()
+        _check(lib().oracle_mg_solve(self.h, _p(b), _p(x), C.byref(o), cycle, C.byref(r)))
+        return x, r.as_dict()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().oracle_mg_destroy(self.h)
+        except Exception:
+            pass

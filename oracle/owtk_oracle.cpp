@@ -1440,6 +1440,251 @@ void idrs(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_opti
     r.status = maximum_iterations;
 }
 
+// AggregationMultigrid (multigrid.hpp:41-421): greedy aggregates, the
+// row-averaged coarse matrix, Jacobi / red-black Gauss-Seidel / Chebyshev
+// smoothing, V, W and full cycles.  The fine matrix is a component-diagonal
+// BlockCsrMatrix (kind CSR).
+struct MgOptions {
+    std::uint64_t pre, post, coarse_steps, aggregate;
+    double jacobi_weight;
+    std::int32_t smoother;  // 0 jacobi, 1 red-black Gauss-Seidel, 2 Chebyshev
+    std::uint64_t chebyshev_levels;
+};
+
+struct Multigrid {
+    Matrix fine;
+    MgOptions opt;
+    int bs;
+    std::int64_t n;
+    std::vector<std::int64_t> f2c, sizes;
+    std::vector<std::int32_t> coff, ccol;  // coarse matrix, CSR kind
+    std::vector<double> cval;
+    Matrix coarse;
+    std::vector<double> fine_inv, coarse_inv;
+    std::vector<std::int64_t> fine_red, fine_black, coarse_red, coarse_black;
+
+    Multigrid(const Matrix& a, const MgOptions& o) : fine(a), opt(o), bs(a.bs), n(a.owned)
+    {
+        if (a.kind != 0) throw std::invalid_argument("multigrid needs a component-diagonal BlockCsrMatrix");
+        if (o.aggregate == 0) throw std::invalid_argument("multigrid aggregate size must be positive");
+        // build_aggregates (multigrid.hpp:237-262)
+        constexpr std::int64_t none = -1;
+        f2c.assign(n, none);
+        for (std::int64_t seed = 0; seed < n; ++seed) {
+            if (f2c[seed] != none) continue;
+            const std::int64_t agg = static_cast<std::int64_t>(sizes.size());
+            std::int64_t size = 1;
+            f2c[seed] = agg;
+            for (std::int32_t e = a.off[seed]; e < a.off[seed + 1] && static_cast<std::uint64_t>(size) < o.aggregate;
+                 ++e) {
+                const std::int64_t nb = a.col[e];
+                if (nb < n && f2c[nb] == none) {
+                    f2c[nb] = agg;
+                    ++size;
+                }
+            }
+            sizes.push_back(size);
+        }
+        // build_coarse_matrix (multigrid.hpp:264-305)
+        const std::int64_t nc = static_cast<std::int64_t>(sizes.size());
+        std::vector<std::map<std::int64_t, std::vector<double>>> rows(nc);
+        for (std::int64_t r = 0; r < n; ++r) {
+            const std::int64_t cr = f2c[r];
+            for (std::int32_t e = a.off[r]; e < a.off[r + 1]; ++e) {
+                const std::int64_t fc = a.col[e];
+                if (fc >= n) continue;
+                auto [pos, inserted] = rows[cr].try_emplace(f2c[fc], bs, 0.0);
+                const double* coef = a.val + static_cast<std::int64_t>(e) * bs;
+                for (int c = 0; c < bs; ++c) pos->second[c] += coef[c] / static_cast<double>(sizes[cr]);
+            }
+        }
+        coff.assign(nc + 1, 0);
+        for (std::int64_t r = 0; r < nc; ++r) {
+            for (const auto& [col, block] : rows[r]) {
+                ccol.push_back(static_cast<std::int32_t>(col));
+                cval.insert(cval.end(), block.begin(), block.end());
+            }
+            coff[r + 1] = static_cast<std::int32_t>(ccol.size());
+        }
+        coarse = Matrix{0, nc, 0, bs, coff.data(), ccol.data(), cval.data(), nullptr};
+        fine_inv = inverse_diagonal(fine);
+        coarse_inv = inverse_diagonal(coarse);
+        coloring(fine, fine_red, fine_black);
+        coloring(coarse, coarse_red, coarse_black);
+    }
+    // BlockCsrMatrix::diagonal, inverted (multigrid.hpp:223-235)
+    static std::vector<double> inverse_diagonal(const Matrix& m)
+    {
+        std::vector<double> d(static_cast<std::size_t>(m.owned) * m.bs, 0.0);
+        for (std::int64_t r = 0; r < m.owned; ++r)
+            for (std::int32_t e = m.off[r]; e < m.off[r + 1]; ++e)
+                if (m.col[e] == r)
+                    for (int c = 0; c < m.bs; ++c) d[r * m.bs + c] = m.val[static_cast<std::int64_t>(e) * m.bs + c];
+        for (double& v : d) {
+            if (v == 0.0) throw std::runtime_error("zero diagonal in multigrid smoother");
+            v = 1.0 / v;
+        }
+        return d;
+    }
+    // build_coloring (multigrid.hpp:345-363)
+    static void coloring(const Matrix& m, std::vector<std::int64_t>& red, std::vector<std::int64_t>& black)
+    {
+        std::vector<int> colors(m.owned, -1);
+        for (std::int64_t r = 0; r < m.owned; ++r) {
+            bool nred = false;
+            for (std::int32_t e = m.off[r]; e < m.off[r + 1]; ++e) {
+                const std::int64_t nb = m.col[e];
+                nred = nred || (nb < m.owned && colors[nb] == 0);
+            }
+            colors[r] = nred ? 1 : 0;
+            (colors[r] == 0 ? red : black).push_back(r);
+        }
+    }
+    double weight(std::uint64_t step) const
+    {
+        if (opt.smoother != 2) return opt.jacobi_weight;
+        const std::uint64_t levels = std::max<std::uint64_t>(1, opt.chebyshev_levels);
+        const double pi = std::acos(-1.0);
+        const double rho = 0.99;
+        const double angle = pi * static_cast<double>(2 * (step % levels) + 1) / static_cast<double>(2 * levels);
+        const double cs = std::cos(angle);
+        return 2.0 / (1.0 + std::sqrt(1.0 - rho * rho * cs * cs));
+    }
+    // gauss_seidel_rows (multigrid.hpp:365-395)
+    static void gs_rows(const Matrix& m, const double* rhs, double* x, const std::vector<std::int64_t>& rows)
+    {
+        std::vector<double> rr(m.bs);
+        for (const std::int64_t r : rows) {
+            std::copy_n(rhs + r * m.bs, m.bs, rr.data());
+            const double* diag = nullptr;
+            for (std::int32_t e = m.off[r]; e < m.off[r + 1]; ++e) {
+                const std::int64_t c = m.col[e];
+                const double* coef = m.val + static_cast<std::int64_t>(e) * m.bs;
+                if (c == r) {
+                    diag = coef;
+                    continue;
+                }
+                for (int k = 0; k < m.bs; ++k) rr[k] -= coef[k] * x[c * m.bs + k];
+            }
+            if (!diag) throw std::runtime_error("multigrid smoother row has no diagonal");
+            for (int k = 0; k < m.bs; ++k) x[r * m.bs + k] = rr[k] / diag[k];
+        }
+    }
+    // smooth (multigrid.hpp:307-331), the coarse level
+    void smooth_coarse(const double* rhs, double* x, std::uint64_t steps) const
+    {
+        const std::int64_t nn = coarse.owned * bs;
+        std::vector<double> applied(nn);
+        for (std::uint64_t st = 0; st < steps; ++st) {
+            if (opt.smoother == 1) {
+                gs_rows(coarse, rhs, x, coarse_red);
+                gs_rows(coarse, rhs, x, coarse_black);
+            } else {
+                spmv_csr(coarse, x, applied.data());
+                const double w = opt.smoother == 2 ? weight(st) : opt.jacobi_weight;
+                for (std::int64_t i = 0; i < nn; ++i) x[i] += w * coarse_inv[i] * (rhs[i] - applied[i]);
+            }
+        }
+    }
+    // restrict_average / prolong_add (multigrid.hpp:397-418)
+    void restrict_avg(const double* f, double* c) const
+    {
+        std::fill_n(c, coarse.owned * bs, 0.0);
+        for (std::int64_t node = 0; node < n; ++node) {
+            const std::int64_t agg = f2c[node];
+            for (int k = 0; k < bs; ++k) c[agg * bs + k] += f[node * bs + k] / static_cast<double>(sizes[agg]);
+        }
+    }
+    void prolong_add(const double* c, double* f) const
+    {
+        for (std::int64_t node = 0; node < n; ++node)
+            for (int k = 0; k < bs; ++k) f[node * bs + k] += c[f2c[node] * bs + k];
+    }
+    // solve (multigrid.hpp:74-221), fine operator = the fine matrix
+    void solve(const double* b, double* x, const oracle_options& o, int cycle, oracle_result& r) const
+    {
+        init_result(r);
+        if (!valid_options(o)) return;
+        Identity id(n * bs);
+        Solve sv(fine, id, r);
+        const std::int64_t nn = n * bs;
+        std::vector<double> residual(fine.local() * bs), work(fine.local() * bs);
+        const double rhs_norm = sv.nrm(b);
+        if (!std::isfinite(rhs_norm)) {
+            r.status = breakdown;
+            r.breakdown_reason = non_finite_scalar;
+            return;
+        }
+        const double thr = threshold_of(rhs_norm, o);
+        sv.true_residual(b, x, residual.data(), work.data());
+        double nv = sv.nrm(residual.data());
+        r.initial_residual_norm = nv;
+        sv.set_residual(nv, nv, rhs_norm);
+        if (!std::isfinite(nv)) {
+            r.status = breakdown;
+            r.breakdown_reason = non_finite_scalar;
+            return;
+        }
+        if (nv <= thr) {
+            r.status = converged;
+            return;
+        }
+        auto smooth_fine = [&](std::uint64_t steps) {
+            for (std::uint64_t st = 0; st < steps; ++st) {
+                if (opt.smoother == 1) {
+                    for (const auto* rows : {&fine_red, &fine_black}) {
+                        sv.op(x, work.data());
+                        for (const std::int64_t row : *rows)
+                            for (int k = 0; k < bs; ++k) {
+                                const std::int64_t i = row * bs + k;
+                                x[i] += fine_inv[i] * (b[i] - work[i]);
+                            }
+                    }
+                } else {
+                    sv.op(x, work.data());
+                    const double w = weight(st);
+                    for (std::int64_t i = 0; i < nn; ++i) x[i] += w * fine_inv[i] * (b[i] - work[i]);
+                }
+            }
+        };
+        const std::int64_t ncn = coarse.owned * bs;
+        std::vector<double> crhs(ncn), csol(ncn);
+        if (cycle == 2) {  // full
+            restrict_avg(b, crhs.data());
+            std::fill(csol.begin(), csol.end(), 0.0);
+            smooth_coarse(crhs.data(), csol.data(), opt.coarse_steps);
+            std::fill_n(x, nn, 0.0);
+            prolong_add(csol.data(), x);
+        }
+        for (std::uint64_t it = 1; it <= o.maximum_iterations; ++it) {
+            r.iterations = it;
+            smooth_fine(opt.pre);
+            const int corrections = cycle == 1 ? 2 : 1;
+            for (int k = 0; k < corrections; ++k) {
+                sv.true_residual(b, x, residual.data(), work.data());
+                restrict_avg(residual.data(), crhs.data());
+                std::fill(csol.begin(), csol.end(), 0.0);
+                smooth_coarse(crhs.data(), csol.data(), opt.coarse_steps);
+                prolong_add(csol.data(), x);
+            }
+            smooth_fine(opt.post);
+            sv.true_residual(b, x, residual.data(), work.data());
+            nv = sv.nrm(residual.data());
+            sv.set_residual(nv, nv, rhs_norm);
+            if (!std::isfinite(nv)) {
+                r.status = breakdown;
+                r.breakdown_reason = non_finite_scalar;
+                return;
+            }
+            if (nv <= thr) {
+                r.status = converged;
+                return;
+            }
+        }
+        r.status = maximum_iterations;
+    }
+};
+
 // fgmres (krylov_solvers.hpp:159-486)
 void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
 {
@@ -1777,6 +2022,33 @@ int oracle_idrs(const oracle_matrix* m, void* pc_handle, const double* b, double
         idrs(a, pc, b, x, *o, *r);
     });
 }
+
+int oracle_mg_create(const oracle_matrix* m, const MgOptions* o, void** out)
+{
+    return guard([&] { *out = new Multigrid(to_matrix(m), *o); });
+}
+
+int oracle_mg_solve(void* h, const double* b, double* x, const oracle_options* o, int cycle, oracle_result* r)
+{
+    return guard([&] { static_cast<Multigrid*>(h)->solve(b, x, *o, cycle, *r); });
+}
+
+// coarse matrix and aggregate map (NULL arrays: sizes only)
+int oracle_mg_coarse(void* h, std::int64_t* nodes, std::int64_t* entries, std::int32_t* off, std::int32_t* col,
+                     double* val, std::int64_t* f2c)
+{
+    return guard([&] {
+        const auto* mg = static_cast<Multigrid*>(h);
+        *nodes = mg->coarse.owned;
+        *entries = static_cast<std::int64_t>(mg->ccol.size());
+        if (off) std::copy(mg->coff.begin(), mg->coff.end(), off);
+        if (col) std::copy(mg->ccol.begin(), mg->ccol.end(), col);
+        if (val) std::copy(mg->cval.begin(), mg->cval.end(), val);
+        if (f2c) std::copy(mg->f2c.begin(), mg->f2c.end(), f2c);
+    });
+}
+
+void oracle_mg_destroy(void* h) { delete static_cast<Multigrid*>(h); }
 
 int oracle_fgmres(const oracle_matrix* m, void* pc_handle, const double* b, double* x, const oracle_options* o,
                   oracle_result* r)

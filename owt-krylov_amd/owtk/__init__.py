@@ -20,7 +20,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
 
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
-    "communication_hiding_bicgstab", "idrs", "fgmres",
+    "communication_hiding_bicgstab", "idrs", "Multigrid", "fgmres",
     "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
@@ -256,6 +256,57 @@ class Matrix:
                 lib().owtk_matrix_destroy(self.handle)
         except Exception:
             pass
+
+
+class Multigrid:
+    """AggregationMultigrid (multigrid.hpp:41-421) on the device over a
+    CSR-kind Matrix.  smoother: MG_JACOBI, MG_RED_BLACK, MG_CHEBYSHEV; solve
+    cycles MG_V, MG_W, MG_FULL."""
+
+    def __init__(self, matrix: Matrix, **kw):
+        self.matrix = matrix
+        o = _lib.MultigridOptions()
+        lib().owtk_multigrid_options_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self.handle = C.c_void_p()
+        check(lib().owtk_multigrid_create(matrix.handle, C.byref(o), C.byref(self.handle)))
+
+    def coarse(self):
+        """(nodes, row_offsets, columns, values, fine_to_coarse) of the coarse level."""
+        n, e = C.c_int64(), C.c_int64()
+        check(lib().owtk_multigrid_coarse(self.handle, C.byref(n), C.byref(e), None, None, None, None))
+        off = np.zeros(n.value + 1, np.int32)
+        col = np.zeros(e.value, np.int32)
+        val = np.zeros(e.value * self.matrix.block)
+        f2c = np.zeros(self.matrix.owned, np.int64)
+        check(lib().owtk_multigrid_coarse(self.handle, C.byref(n), C.byref(e), _ptr(off), _ptr(col), _ptr(val),
+                                          _ptr(f2c)))
+        return n.value, off, col, val, f2c
+
+    def solve(self, b: DeviceBuffer, x: DeviceBuffer, options=None, cycle: int = 0) -> dict:
+        opts = options if options is not None else SolverOptions()
+        res = _lib.SolverResult()
+        check(lib().owtk_multigrid_solve(self.handle, b.ptr, x.ptr, C.byref(opts), cycle, C.byref(res)))
+        d = res.as_dict()
+        d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+        d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
+        return d
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_multigrid_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MG_JACOBI, MG_RED_BLACK, MG_CHEBYSHEV = 0, 1, 2
+MG_V, MG_W, MG_FULL = 0, 1, 2
 
 
 class Batch:

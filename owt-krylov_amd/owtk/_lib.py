@@ -87,6 +87,14 @@ class SolverOptions(C.Structure):
     ]
 
 
+class MultigridOptions(C.Structure):
+    _fields_ = [
+        ("pre_smoothing_steps", C.c_uint64), ("post_smoothing_steps", C.c_uint64),
+        ("coarse_smoothing_steps", C.c_uint64), ("target_aggregate_size", C.c_uint64),
+        ("jacobi_weight", C.c_double), ("smoother", C.c_int32), ("chebyshev_levels", C.c_uint64),
+    ]
+
+
 class SolverResult(C.Structure):
     _fields_ = [
         ("status", C.c_int32), ("breakdown_reason", C.c_int32),
@@ -194,6 +202,11 @@ PROTOTYPES: dict[str, tuple] = {
                                       C.POINTER(C.c_int64), _P, _P, _P]),
     "owtk_write_block_csr": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, _P]),
     "owtk_write_ilu_factors": (C.c_int, [_P, C.c_char_p]),
+    "owtk_multigrid_options_default": (None, [C.POINTER(MultigridOptions)]),
+    "owtk_multigrid_create": (C.c_int, [_P, C.POINTER(MultigridOptions), C.POINTER(_P)]),
+    "owtk_multigrid_destroy": (C.c_int, [_P]),
+    "owtk_multigrid_coarse": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, _P, _P, _P]),
+    "owtk_multigrid_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolverOptions), C.c_int32, C.POINTER(SolverResult)]),
     "owtk_read_state": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   _P]),
     "owtk_gnn_create": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),

@@ -429,3 +429,57 @@ def test_idrs_reference_cases(c0):
             (b if in_rhs else x0)[0] = bad
             _, r = orc.idrs(one, None, b, x0=x0, idr_shadow_space=2, maximum_iterations=3)
             assert r["status"] != 0 and not np.isfinite(r["initial_residual_norm"])
+
+
+def _dense_components(n, bs, off, col, val):
+    a = np.zeros((bs, n, n))
+    for r in range(n):
+        for e in range(off[r], off[r + 1]):
+            a[:, r, col[e]] += val[e * bs:(e + 1) * bs]
+    return a
+
+
+@pytest.mark.parametrize("aggregate", [1, 2, 4])
+def test_multigrid_coarse_matrix_closed_form(c0, aggregate):
+    """AggregationMultigrid's coarse operator (multigrid.hpp:264-305) is the
+    row-averaged Galerkin product C = S^-1 R^T A R per component, R the 0/1
+    aggregate indicator and S the aggregate sizes; aggregates partition the
+    nodes, respect the target size and are seeded in row order."""
+    s = c0
+    co, cc, cv = s.csr()
+    om = orc.OMatrix.csr(s.nodes, 0, 3, co, cc, cv)
+    mg = orc.OMultigrid(om, aggregate=aggregate)
+    nc, off, col, val, f2c = mg.coarse()
+    assert f2c.min() == 0 and f2c.max() == nc - 1
+    sizes = np.bincount(f2c, minlength=nc)
+    assert sizes.max() <= aggregate and sizes.min() >= 1
+    first = np.array([np.flatnonzero(f2c == k)[0] for k in range(nc)])
+    assert np.all(np.diff(first) > 0)  # aggregate k is seeded before aggregate k + 1
+    R = np.zeros((s.nodes, nc))
+    R[np.arange(s.nodes), f2c] = 1.0
+    A = _dense_components(s.nodes, 3, co, cc, cv)
+    got = _dense_components(nc, 3, off, col, val)
+    for c in range(3):
+        ref = (R.T @ A[c] @ R) / sizes[:, None]
+        assert np.max(np.abs(got[c] - ref)) <= 1e-14 * np.max(np.abs(ref))
+    if aggregate == 1:
+        assert np.array_equal(f2c, np.arange(s.nodes))
+
+
+@pytest.mark.parametrize("smoother", [0, 1, 2])
+@pytest.mark.parametrize("cycle", [0, 1, 2])
+def test_multigrid_reference_block_system(smoother, cycle):
+    """The reference test's 3-node block system (test_core.cpp:474-490,
+    800-844): aggregates of 2, 40 coarse smoothing steps; V, W and full cycles
+    with Jacobi, red-black Gauss-Seidel and Chebyshev converge to 1e-9 at the
+    exact solution 1..6."""
+    om = orc.OMatrix.csr(3, 0, 2, [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2],
+                         [4, 5, -1, -1, -1, -1, 4, 5, -1, -1, -1, -1, 3, 4])
+    exact = np.arange(1, 7, dtype=float)
+    mg = orc.OMultigrid(om, aggregate=2, coarse_steps=40, smoother=smoother)
+    nc, off, col, val, f2c = mg.coarse()
+    assert f2c.tolist() == [0, 0, 1]
+    assert val.tolist() == [3.0, 4.0, -0.5, -0.5, -1.0, -1.0, 3.0, 4.0]
+    x, res = mg.solve(om.spmv(exact), cycle=cycle, relative_tolerance=1e-9, maximum_iterations=100)
+    assert res["status"] == 0
+    assert np.max(np.abs(x - exact)) < 1e-8
