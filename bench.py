@@ -459,6 +459,70 @@ def pipelined_bench(args):
     return 0
 
 
+def multigrid_bench(args):
+    """Two-level aggregation multigrid (multigrid.hpp) on the 512x256 grid,
+    CSR form (BlockCsrMatrix with the diagonal inline), rtol 1e-8: V cycles
+    with Jacobi and red-black Gauss-Seidel smoothing (aggregates of 4, 2+2
+    fine and 24 coarse smoothing steps)."""
+    nx, ny = 512, 256
+    s = owtk.generate_split(nx, ny, BS, SEED, ordering=owtk.ORDER_HILBERT)
+    co, cc, cv = s.csr()
+    out_methods = {}
+    with owtk.Context(0) as ctx:
+        m = owtk.Matrix.csr(ctx, s.nodes, 0, BS, co, cc, cv)
+        b = ctx.upload(s.rhs)
+        x = ctx.empty(s.nodes * BS)
+        r = ctx.empty(s.nodes * BS)
+        bnorm = ctx.norm(b)
+        opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=5000)
+        for name, smoother, cycle in (("v_jacobi", owtk.MG_JACOBI, owtk.MG_V),
+                                      ("v_red_black", owtk.MG_RED_BLACK, owtk.MG_V),
+                                      ("w_red_black", owtk.MG_RED_BLACK, owtk.MG_W)):
+            t0 = time.perf_counter()
+            mg = owtk.Multigrid(m, smoother=smoother)
+            ctx.synchronize()
+            setup_s = time.perf_counter() - t0
+            times, results, _ = timed_solves(ctx, f"multigrid {name}", args.steps, args.warmup,
+                                             lambda g=mg, c=cycle: g.solve(b, x, opts, cycle=c), x)
+            res = results[-1]
+            ms = float(np.median(times))
+            out_methods[name] = {"ms": round(ms, 2), "iterations": res["iterations"], "status": res["status_name"],
+                                 "ms_per_cycle": round(ms / max(res["iterations"], 1), 4),
+                                 "true_rel": true_rel(ctx, m, b, x, r, bnorm), "setup_s": round(setup_s, 3),
+                                 "coarse_nodes": mg.coarse()[0]}
+            if smoother == owtk.MG_RED_BLACK:
+                out_methods[name]["note"] = ("coarse Gauss-Seidel keeps the reference's sequential row order: "
+                                             "1,026 dependency levels among the black rows at this size")
+            mg.close()
+    head = out_methods["v_jacobi"]
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as orc
+        om = orc.OMatrix.csr(s.nodes, 0, BS, co, cc, cv)
+        omg = orc.OMultigrid(om, smoother=orc.OMultigrid.JACOBI)
+        cycles = 5
+        t0 = time.perf_counter()
+        omg.solve(s.rhs, cycle=orc.OMultigrid.V, relative_tolerance=0.0, maximum_iterations=cycles)
+        cyc_s = (time.perf_counter() - t0) / cycles
+        model_name, cores = cpu_info()
+        cpu = {"value": round(cyc_s * head["iterations"] * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": (f"oracle multigrid, Jacobi V cycles on this grid: {cycles} timed cycles "
+                          f"({cyc_s * 1e3:.1f} ms each) x the {head['iterations']} cycles the solve needs; "
+                          f"host {model_name}, {cores} hardware threads")}
+    out = {
+        "metric": "ms to true rel. residual 1e-8 (aggregation multigrid, FP64)", "value": head["ms"], "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{nx}x{ny} triangulated grid, {s.nodes:,} vertices, block 3, CSR form, Hilbert "
+                               "order, seed 0; rtol 1e-8",
+                   "method": "AggregationMultigrid (multigrid.hpp), V cycles, Jacobi smoothing (the default)"},
+        "methods": out_methods, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,7 +537,8 @@ def main():
     ap.add_argument("--orth", type=int, default=2,
                     help="Gram-Schmidt of the inner FGMRES: 0 iterated CGS, 1 adaptive, 2 one-synchronization "
                          "(GmresOrthogonalization, core.hpp:32-39); inner products stay FP64")
-    ap.add_argument("--workload", choices=["headline", "batched", "bsr", "pipelined"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "batched", "bsr", "pipelined", "multigrid"],
+                    default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -488,6 +553,8 @@ def main():
         return bsr_bench(args)
     if args.workload == "pipelined":
         return pipelined_bench(args)
+    if args.workload == "multigrid":
+        return multigrid_bench(args)
 
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)

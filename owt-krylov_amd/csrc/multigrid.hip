@@ -5,9 +5,10 @@
 // colourings.  Every operation of a cycle runs on the device:
 //   Jacobi / Chebyshev smoothing  SpMV + one update pass
 //   red-black Gauss-Seidel (fine) full residual per colour + colour update
-//   red-black Gauss-Seidel (coarse, true Gauss-Seidel rows) level schedule
-//     of each colour's rows: rows of a level touch no earlier row of the
-//     same colour, so each level is one parallel pass and the result is the
+//   red-black Gauss-Seidel (coarse, true Gauss-Seidel rows, gauss_seidel_rows
+//     multigrid.hpp:365-395) level schedule of each colour's rows: rows of
+//     a level touch no earlier row of the same colour, so each level is one
+//     parallel pass, one workgroup walks the levels, and the result is the
 //     sequential sweep's
 //   restriction                   per aggregate, its members in ascending
 //                                 order (the reference's summation order)
@@ -40,6 +41,7 @@ struct owtk_multigrid_s {
     std::int64_t n_red = 0, n_black = 0;
     // coarse Gauss-Seidel: CSR on the device and the level schedule of each colour
     owtk::DeviceArray<std::int32_t> coff, ccol, gs_rows;
+    owtk::DeviceArray<std::int64_t> gs_levels;  // device copy of gs_level_ptr
     owtk::DeviceArray<double> cval;
     std::vector<std::int64_t> gs_level_ptr;  // red levels, then black levels
     std::int64_t red_levels = 0;
@@ -106,30 +108,37 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const double* __restrict__ c
         fine[t] = fine[t] + coarse[static_cast<std::int64_t>(f2c[t / bs]) * bs + t % bs];
 }
 
-// gauss_seidel_rows (multigrid.hpp:365-395) on the rows of one level
-__global__ __launch_bounds__(256) void k_mg_gs(const std::int32_t* __restrict__ rows, std::int64_t count, int bs,
-                                               const std::int32_t* __restrict__ off,
-                                               const std::int32_t* __restrict__ col,
-                                               const double* __restrict__ val, const double* __restrict__ rhs,
-                                               double* x)
+// One colour's Gauss-Seidel sweep, all levels in one workgroup: the rows of
+// a level in parallel, a barrier between levels (a level's reads of earlier
+// levels' writes stay inside the workgroup).
+__global__ __launch_bounds__(1024) void k_mg_gs_sweep(const std::int32_t* __restrict__ rows,
+                                                      const std::int64_t* __restrict__ level_ptr, std::int64_t first,
+                                                      std::int64_t last, int bs,
+                                                      const std::int32_t* __restrict__ off,
+                                                      const std::int32_t* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ rhs,
+                                                      double* x)
 {
-    const std::int64_t total = count * bs;
-    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
-        const std::int64_t r = rows[t / bs];
-        const int k = static_cast<int>(t % bs);
-        double rr = rhs[r * bs + k];
-        double diag = 0.0;
-        for (std::int32_t e = off[r]; e < off[r + 1]; ++e) {
-            const std::int64_t c = col[e];
-            const double coef = val[static_cast<std::int64_t>(e) * bs + k];
-            if (c == r) {
-                diag = coef;
-                continue;
+    for (std::int64_t l = first; l < last; ++l) {
+        const std::int64_t a = level_ptr[l];
+        const std::int64_t total = (level_ptr[l + 1] - a) * bs;
+        for (std::int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+            const std::int64_t r = rows[a + t / bs];
+            const int k = static_cast<int>(t % bs);
+            double rr = rhs[r * bs + k];
+            double diag = 0.0;
+            for (std::int32_t e = off[r]; e < off[r + 1]; ++e) {
+                const std::int64_t c = col[e];
+                const double coef = val[static_cast<std::int64_t>(e) * bs + k];
+                if (c == r) {
+                    diag = coef;
+                    continue;
+                }
+                rr = rr - coef * x[c * bs + k];
             }
-            rr = rr - coef * x[c * bs + k];
+            x[r * bs + k] = rr / diag;
         }
-        x[r * bs + k] = rr / diag;
+        __syncthreads();
     }
 }
 
@@ -305,6 +314,7 @@ void mg_setup(owtk_multigrid_s* mg)
     mg->red_levels = static_cast<std::int64_t>(mg->gs_level_ptr.size()) - 1;
     gs_levels(nc, mg->h_coff, mg->h_ccol, tptr, tcol, cblack, gs_rows, mg->gs_level_ptr);
     upload(mg->gs_rows, gs_rows);
+    upload(mg->gs_levels, mg->gs_level_ptr);
     upload(mg->coff, mg->h_coff);
     upload(mg->ccol, mg->h_ccol);
     upload(mg->cval, mg->h_cval);
@@ -400,12 +410,12 @@ int mg_solve(owtk_multigrid_s* mg, const double* b, double* x, const owtk_solver
     auto smooth_coarse = [&](std::uint64_t steps) {
         for (std::uint64_t s = 0; s < steps; ++s) {
             if (opt.smoother == OWTK_MG_RED_BLACK) {
+                // red levels, then black levels (red rows read the previous black values)
                 const std::int64_t levels = static_cast<std::int64_t>(mg->gs_level_ptr.size()) - 1;
-                for (std::int64_t l = 0; l < levels; ++l) {
-                    const std::int64_t a = mg->gs_level_ptr[l], c = mg->gs_level_ptr[l + 1] - a;
-                    k_mg_gs<<<grid(c * bs), kBlock, 0, st>>>(mg->gs_rows.get() + a, c, bs, mg->coff.get(),
-                                                             mg->ccol.get(), mg->cval.get(), crhs, csol);
-                }
+                k_mg_gs_sweep<<<1, 1024, 0, st>>>(mg->gs_rows.get(), mg->gs_levels.get(), 0, mg->red_levels, bs,
+                                                   mg->coff.get(), mg->ccol.get(), mg->cval.get(), crhs, csol);
+                k_mg_gs_sweep<<<1, 1024, 0, st>>>(mg->gs_rows.get(), mg->gs_levels.get(), mg->red_levels, levels,
+                                                   bs, mg->coff.get(), mg->ccol.get(), mg->cval.get(), crhs, csol);
             } else {
                 dev::EpiStore<double> e;
                 e.y = capplied;
