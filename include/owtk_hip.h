@@ -313,6 +313,36 @@ int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_prec
                                  const double* b, double* x, const owtk_refinement_options* options,
                                  owtk_solver_result* result);
 
+/* ------------------------------------------------------------- recycling -- */
+/* RecycleSpace<double> (recycling.hpp:24-187): up to `capacity` pairs
+ * (U_i, C_i = A U_i) with C orthonormal, kept on the device across solves;
+ * `layout` fixes the vector layout (owned + ghost entries).  All vector
+ * arguments are device pointers.  telemetry (nullable) receives the
+ * operator applications and global reductions, as SolverResult* telemetry
+ * in the reference. */
+typedef struct owtk_recycle_space_s* owtk_recycle_space;
+int owtk_recycle_space_create(owtk_matrix layout, uint64_t capacity, owtk_recycle_space* out);
+int owtk_recycle_space_destroy(owtk_recycle_space space);
+int owtk_recycle_space_size(owtk_recycle_space space, int64_t* size);
+int owtk_recycle_space_clear(owtk_recycle_space space);
+/* copies of U_i and C_i (owned entries) */
+int owtk_recycle_space_get(owtk_recycle_space space, int64_t index, double* u, double* c);
+/* add_candidate (recycling.hpp:79-135); *added = 0 when the candidate is
+ * numerically contained in the space */
+int owtk_recycle_space_add_candidate(owtk_recycle_space space, owtk_matrix op, const double* candidate,
+                                     double dependence_tolerance, int32_t* added, owtk_solver_result* telemetry);
+/* refresh (recycling.hpp:65-73): C = A U after an operator update */
+int owtk_recycle_space_refresh(owtk_recycle_space space, owtk_matrix op, owtk_solver_result* telemetry);
+/* project_initial_guess (recycling.hpp:138-163): x += U C^T residual */
+int owtk_recycle_space_project(owtk_recycle_space space, const double* residual, double* x,
+                               owtk_solver_result* telemetry);
+/* recycled_fgmres (recycling.hpp:291-360): augmented FGMRES; with
+ * update_recycle_space the converged correction x - x0 becomes the next
+ * candidate */
+int owtk_recycled_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space space,
+                         const owtk_solver_options* options, int32_t update_recycle_space,
+                         owtk_solver_result* result);
+
 /* -------------------------------------------------------------- multigrid -- */
 /* AggregationMultigrid (multigrid.hpp:41-421): two-level aggregation solver
  * over a component-diagonal BlockCsrMatrix (OWTK_MATRIX_CSR).  Set-up on the

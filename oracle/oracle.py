@@ -83,6 +83,16 @@ def lib():
         L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                                            C.POINTER(Result)]
         L.oracle_idrs.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        L.oracle_recycle_create.argtypes = [C.c_uint64, C.POINTER(P)]
+        L.oracle_recycle_destroy.argtypes = [P]
+        L.oracle_recycle_destroy.restype = None
+        L.oracle_recycle_size.argtypes = [P]
+        L.oracle_recycle_size.restype = C.c_int64
+        L.oracle_recycle_get.argtypes = [P, C.c_int64, P, P]
+        L.oracle_recycle_add.argtypes = [P, C.POINTER(Matrix), P, C.c_double, C.POINTER(C.c_int),
+                                         C.POINTER(Result)]
+        L.oracle_recycled_fgmres.argtypes = [C.POINTER(Matrix), P, P, P, P, C.POINTER(Options), C.c_int,
+                                             C.POINTER(Result)]
         L.oracle_mg_create.argtypes = [C.POINTER(Matrix), C.POINTER(MgOptions), C.POINTER(P)]
         L.oracle_mg_solve.argtypes = [P, P, P, C.POINTER(Options), C.c_int, C.POINTER(Result)]
         L.oracle_mg_coarse.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), P, P, P, P]
@@ -297,3 +307,45 @@ class OMultigrid:
                 lib().oracle_mg_destroy(self.h)
         except Exception:
             pass
+
+
+class ORecycleSpace:
+    """RecycleSpace (recycling.hpp:24-187): U, C = A U, C orthonormal."""
+
+    def __init__(self, capacity: int):
+        self.h = C.c_void_p()
+        _check(lib().oracle_recycle_create(capacity, C.byref(self.h)))
+
+    def size(self) -> int:
+        return lib().oracle_recycle_size(self.h)
+
+    def get(self, i: int, n: int):
+        u, c = np.zeros(n), np.zeros(n)
+        _check(lib().oracle_recycle_get(self.h, i, _p(u), _p(c)))
+        return u, c
+
+    def add_candidate(self, matrix: OMatrix, candidate, tol=128 * np.finfo(np.float64).eps):
+        cand = np.ascontiguousarray(candidate, np.float64)
+        added = C.c_int()
+        r = Result()
+        _check(lib().oracle_recycle_add(self.h, C.byref(matrix.s), _p(cand), tol, C.byref(added), C.byref(r)))
+        return bool(added.value), r.as_dict()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().oracle_recycle_destroy(self.h)
+        except Exception:
+            pass
+
+
+def recycled_fgmres(matrix: OMatrix, pc: OPreconditioner | None, b, space: ORecycleSpace, x0=None, update=True,
+                    **kw):
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    r = Result()
+    _check(lib().oracle_recycled_fgmres(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), space.h, C.byref(o),
+                                        1 if update else 0, C.byref(r)))
+    return x, r.as_dict()

@@ -630,10 +630,19 @@ struct Solve {
     std::int64_t n;  // owned size
     std::int64_t local;
     Solve(const Matrix& m, Pc& p, oracle_result& res) : a(m), pc(p), r(res), n(m.owned * m.bs), local(m.local() * m.bs) {}
+    // recycled_fgmres's RecycledOperator (recycling.hpp:195-220): right after a
+    // projected preconditioner apply, the operator returns its cached image
+    double* cached = nullptr;
+    bool* cache_ready = nullptr;
     void op(const double* x, double* y)
     {
-        apply(a, x, y);
         ++r.operator_applications;
+        if (cache_ready && *cache_ready) {
+            std::copy_n(cached, n, y);
+            *cache_ready = false;
+            return;
+        }
+        apply(a, x, y);
     }
     void prec(const double* in, double* out)
     {
@@ -1686,11 +1695,14 @@ struct Multigrid {
 };
 
 // fgmres (krylov_solvers.hpp:159-486)
-void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r)
+void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r,
+            double* cached = nullptr, bool* cache_ready = nullptr)
 {
     init_result(r);
     if (!valid_options(o) || o.restart == 0) return;
     Solve s(a, pc, r);
+    s.cached = cached;
+    s.cache_ready = cache_ready;
     const std::size_t m = o.restart;
     const std::int64_t L = s.local;
     std::vector<double> residual(L), work(L);
@@ -1834,6 +1846,132 @@ void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_op
         }
     }
     r.status = maximum_iterations;
+}
+
+// RecycleSpace (recycling.hpp:24-187): U and C = A U with C orthonormal,
+// at most `capacity` pairs (the oldest dropped first).
+struct RecycleSpace {
+    std::size_t capacity;
+    std::vector<std::vector<double>> U, C;
+    explicit RecycleSpace(std::uint64_t cap) : capacity(cap)
+    {
+        if (cap == 0) throw std::invalid_argument("recycle space capacity must be positive");
+    }
+    // add_candidate (recycling.hpp:79-135): A-image CGS2 against C
+    bool add_candidate(const Matrix& a, const double* cand, oracle_result* tel, double tol)
+    {
+        const std::int64_t n = a.owned * a.bs;
+        std::vector<double> v(cand, cand + n), img(a.local() * a.bs, 0.0), vin(a.local() * a.bs, 0.0);
+        std::copy(cand, cand + n, vin.begin());
+        apply(a, vin.data(), img.data());
+        if (tel) ++tel->operator_applications;
+        const double orig = norm(n, img.data());
+        if (tel) ++tel->global_reductions;
+        if (!std::isfinite(orig) || orig == 0.0) return false;
+        for (int pass = 0; pass < 2 && !C.empty(); ++pass) {
+            std::vector<double> coef(C.size());
+            for (std::size_t i = 0; i < C.size(); ++i) coef[i] = dot(n, C[i].data(), img.data());
+            if (tel) ++tel->global_reductions;
+            for (std::size_t i = 0; i < C.size(); ++i) {
+                for (std::int64_t k = 0; k < n; ++k) img[k] = std::fma(-coef[i], C[i][k], img[k]);
+                for (std::int64_t k = 0; k < n; ++k) v[k] = std::fma(-coef[i], U[i][k], v[k]);
+            }
+        }
+        const double nv = norm(n, img.data());
+        if (tel) ++tel->global_reductions;
+        if (!std::isfinite(nv) || nv == 0.0 || nv / orig <= tol) return false;
+        const double inv = 1.0 / nv;
+        img.resize(n);
+        for (auto& e : img) e *= inv;
+        for (auto& e : v) e *= inv;
+        if (C.size() == capacity) {
+            C.erase(C.begin());
+            U.erase(U.begin());
+        }
+        C.push_back(std::move(img));
+        U.push_back(std::move(v));
+        return true;
+    }
+    // refresh (recycling.hpp:65-73): rebuild C = A U from the stored U
+    void refresh(const Matrix& a, oracle_result* tel)
+    {
+        RecycleSpace fresh(capacity);
+        for (const auto& u : U) fresh.add_candidate(a, u.data(), tel, 128 * std::numeric_limits<double>::epsilon());
+        *this = std::move(fresh);
+    }
+    // project_initial_guess (recycling.hpp:138-163): x += U C^T r
+    void project(const double* res, double* x, std::int64_t n, oracle_result* tel)
+    {
+        if (C.empty()) return;
+        std::vector<double> coef(C.size());
+        for (std::size_t i = 0; i < C.size(); ++i) coef[i] = dot(n, C[i].data(), res);
+        if (tel) ++tel->global_reductions;
+        for (std::size_t i = 0; i < C.size(); ++i)
+            for (std::int64_t k = 0; k < n; ++k) x[k] = std::fma(coef[i], U[i][k], x[k]);
+    }
+};
+
+// RecycledPreconditioner (recycling.hpp:222-289): z = M v, then the operator
+// image A z is projected off C (and z off U) and cached for the next operator
+// application.
+struct RecycledPc : Pc {
+    const Matrix* a;
+    Pc* inner;
+    const RecycleSpace* space;
+    std::vector<double> cached;
+    bool ready = false;
+    std::uint64_t extra_reductions = 0;
+    void apply(const double* in, double* out) override
+    {
+        const std::int64_t n = a->owned * a->bs;
+        inner->apply(in, out);
+        std::vector<double> xin(a->local() * a->bs, 0.0);
+        std::copy(out, out + n, xin.begin());
+        ::apply(*a, xin.data(), cached.data());
+        if (!space->C.empty()) {
+            std::vector<double> coef(space->C.size());
+            for (std::size_t i = 0; i < space->C.size(); ++i) coef[i] = dot(n, space->C[i].data(), cached.data());
+            ++extra_reductions;
+            for (std::size_t i = 0; i < space->C.size(); ++i) {
+                for (std::int64_t k = 0; k < n; ++k) out[k] = std::fma(-coef[i], space->U[i][k], out[k]);
+                for (std::int64_t k = 0; k < n; ++k) cached[k] = std::fma(-coef[i], space->C[i][k], cached[k]);
+            }
+        }
+        ready = true;
+    }
+};
+
+// recycled_fgmres (recycling.hpp:291-360)
+void recycled_fgmres(const Matrix& a, Pc& pc, const double* b, double* x, RecycleSpace& space,
+                     const oracle_options& o, bool update, oracle_result& r)
+{
+    oracle_result tel;
+    std::memset(&tel, 0, sizeof(tel));
+    init_result(r);
+    if (!valid_options(o) || o.restart == 0) return;
+    const std::int64_t n = a.owned * a.bs;
+    std::vector<double> x0(x, x + n);
+    if (!space.C.empty()) {
+        space.refresh(a, &tel);
+        std::vector<double> work(a.local() * a.bs), res(n);
+        apply(a, x, work.data());
+        ++tel.operator_applications;
+        for (std::int64_t i = 0; i < n; ++i) res[i] = b[i] - work[i];
+        space.project(res.data(), x, n, &tel);
+    }
+    RecycledPc rpc;
+    rpc.a = &a;
+    rpc.inner = &pc;
+    rpc.space = &space;
+    rpc.cached.assign(a.local() * a.bs, 0.0);
+    fgmres(a, rpc, b, x, o, r, rpc.cached.data(), &rpc.ready);
+    r.operator_applications += tel.operator_applications;
+    r.global_reductions += tel.global_reductions + rpc.extra_reductions;
+    if (update && r.status == converged) {
+        std::vector<double> corr(n);
+        for (std::int64_t i = 0; i < n; ++i) corr[i] = x[i] - x0[i];
+        space.add_candidate(a, corr.data(), &r, o.breakdown_tolerance);
+    }
 }
 
 template<class F>
@@ -2020,6 +2158,45 @@ int oracle_idrs(const oracle_matrix* m, void* pc_handle, const double* b, double
         Identity id(a.owned * a.bs);
         Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
         idrs(a, pc, b, x, *o, *r);
+    });
+}
+
+int oracle_recycle_create(std::uint64_t capacity, void** out)
+{
+    return guard([&] { *out = new RecycleSpace(capacity); });
+}
+
+void oracle_recycle_destroy(void* h) { delete static_cast<RecycleSpace*>(h); }
+
+std::int64_t oracle_recycle_size(void* h) { return static_cast<std::int64_t>(static_cast<RecycleSpace*>(h)->C.size()); }
+
+// U_i and C_i (owned entries)
+int oracle_recycle_get(void* h, std::int64_t i, double* u, double* c)
+{
+    return guard([&] {
+        auto* sp = static_cast<RecycleSpace*>(h);
+        std::copy(sp->U.at(i).begin(), sp->U.at(i).end(), u);
+        std::copy(sp->C.at(i).begin(), sp->C.at(i).end(), c);
+    });
+}
+
+int oracle_recycle_add(void* h, const oracle_matrix* m, const double* candidate, double tol, int* added,
+                       oracle_result* telemetry)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        *added = static_cast<RecycleSpace*>(h)->add_candidate(a, candidate, telemetry, tol) ? 1 : 0;
+    });
+}
+
+int oracle_recycled_fgmres(const oracle_matrix* m, void* pc_handle, const double* b, double* x, void* space,
+                           const oracle_options* o, int update, oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        recycled_fgmres(a, pc, b, x, *static_cast<RecycleSpace*>(space), *o, update != 0, *r);
     });
 }
 

@@ -18,6 +18,7 @@
 #include "kernels/spmv.hpp"
 
 #include <chrono>
+#include <deque>
 #include <cmath>
 #include <functional>
 #include <limits>
@@ -912,6 +913,242 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
     return finish(status_converged);
 }
 
+// ------------------------------------------------------------ recycling --
+// RecycleSpace and recycled_fgmres (recycling.hpp:24-360): U and C = A U
+// (C orthonormal) on the device.  Inside the solve the projected
+// preconditioner forms z = M v and A z, takes the coefficients C^T A z with
+// one multi-dot pass and projects both, all on the device (no host read);
+// the operator then returns the projected image.
+struct RecycleVectors {
+    std::deque<DeviceArray<double>> U, C;  // owned + ghost length, ghosts zero
+};
+
+namespace {
+
+// out = fma(-c_i, U_i, out), img = fma(-c_i, C_i, img) for i in order,
+// c read from device memory; sign = +1 forms x = fma(c_i, U_i, x) only
+__global__ __launch_bounds__(256) void k_rc_project(double* __restrict__ out, double* __restrict__ img,
+                                                    kern::BasisPointers u, kern::BasisPointers c,
+                                                    const double* __restrict__ coef, int count, std::int64_t n,
+                                                    double sign, const int* stop)
+{
+    if (stop && *stop) return;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += step) {
+        double o = out[t];
+        double g = img ? img[t] : 0.0;
+        for (int i = 0; i < count; ++i) {
+            const double a = sign * coef[i];
+            o = __builtin_fma(a, u.v[i][t], o);
+            if (img) g = __builtin_fma(a, c.v[i][t], g);
+        }
+        out[t] = o;
+        if (img) img[t] = g;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rc_scale2(double* __restrict__ a, double* __restrict__ b, double s,
+                                                   std::int64_t n)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += step) {
+        a[t] = a[t] * s;
+        b[t] = b[t] * s;
+    }
+}
+
+void count(owtk_solver_result* tel, std::uint64_t ops, std::uint64_t reds)
+{
+    if (!tel) return;
+    tel->operator_applications += ops;
+    tel->global_reductions += reds;
+}
+
+} // namespace
+
+} // namespace owtk
+
+struct owtk_recycle_space_s {
+    owtk_context ctx = nullptr;
+    std::int64_t n = 0, local = 0;
+    std::size_t capacity = 0;
+    owtk::RecycleVectors v;
+    owtk::DeviceArray<double> coef;  // capacity + 1 coefficients
+};
+
+namespace owtk {
+namespace {
+
+kern::BasisPointers pointers(const std::deque<DeviceArray<double>>& list)
+{
+    kern::BasisPointers p{};
+    for (std::size_t i = 0; i < list.size(); ++i) p.v[i] = list[i].get();
+    return p;
+}
+
+double read_scalar(owtk_context ctx, const double* d)
+{
+    double v;
+    OWTK_HIP(hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    return v;
+}
+
+// add_candidate (recycling.hpp:79-135)
+bool rc_add(owtk_recycle_space sp, owtk_matrix m, const double* cand, double tol, owtk_solver_result* tel)
+{
+    owtk_context ctx = sp->ctx;
+    const hipStream_t st = ctx->stream;
+    const auto scratch = ctx->scratch();
+    const std::int64_t n = sp->n;
+    DeviceArray<double> vec(static_cast<std::size_t>(sp->local)), img(static_cast<std::size_t>(sp->local));
+    kern::fill_zero_bytes(st, vec.get(), vec.bytes());
+    kern::fill_zero_bytes(st, img.get(), img.bytes());
+    OWTK_HIP(hipMemcpyAsync(vec.get(), cand, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    {
+        dev::EpiStore<double> e;
+        e.y = img.get();
+        spmv_f64(m, vec.get(), e);
+    }
+    double* d_norm = ctx->scalars.get() + 24;
+    kern::norm_to(st, scratch, img.get(), n, d_norm);
+    const double orig = read_scalar(ctx, d_norm);
+    count(tel, 1, 1);
+    if (!std::isfinite(orig) || orig == 0.0) return false;
+    const int k = static_cast<int>(sp->v.C.size());
+    const unsigned int g = dev::stream_grid(n, 4);
+    for (int pass = 0; pass < 2 && k > 0; ++pass) {
+        const auto cp = pointers(sp->v.C);
+        kern::multidot_to(st, scratch, n, k, cp.v, img.get(), sp->coef.get());
+        count(tel, 0, 1);
+        k_rc_project<<<g, kBlock, 0, st>>>(vec.get(), img.get(), pointers(sp->v.U), cp, sp->coef.get(), k, n, -1.0,
+                                           nullptr);
+        OWTK_LAUNCH_CHECK();
+    }
+    kern::norm_to(st, scratch, img.get(), n, d_norm);
+    const double nv = read_scalar(ctx, d_norm);
+    count(tel, 0, 1);
+    if (!std::isfinite(nv) || nv == 0.0 || nv / orig <= tol) return false;
+    k_rc_scale2<<<g, kBlock, 0, st>>>(img.get(), vec.get(), 1.0 / nv, n);
+    OWTK_LAUNCH_CHECK();
+    if (sp->v.C.size() == sp->capacity) {
+        ctx->sync();  // earlier kernels may still read the dropped pair
+        sp->v.C.pop_front();
+        sp->v.U.pop_front();
+    }
+    sp->v.C.push_back(std::move(img));
+    sp->v.U.push_back(std::move(vec));
+    return true;
+}
+
+// refresh (recycling.hpp:65-73): C = A U rebuilt from the stored U
+void rc_refresh(owtk_recycle_space sp, owtk_matrix m, owtk_solver_result* tel)
+{
+    RecycleVectors old = std::move(sp->v);
+    sp->v = RecycleVectors{};
+    for (const auto& u : old.U) rc_add(sp, m, u.get(), 128 * std::numeric_limits<double>::epsilon(), tel);
+    sp->ctx->sync();  // the old vectors are read by the launches above
+}
+
+// project_initial_guess (recycling.hpp:138-163): x += U C^T r
+void rc_project(owtk_recycle_space sp, const double* r, double* x, owtk_solver_result* tel)
+{
+    const int k = static_cast<int>(sp->v.C.size());
+    if (k == 0) return;
+    const hipStream_t st = sp->ctx->stream;
+    const auto cp = pointers(sp->v.C);
+    kern::multidot_to(st, sp->ctx->scratch(), sp->n, k, cp.v, r, sp->coef.get());
+    count(tel, 0, 1);
+    k_rc_project<<<dev::stream_grid(sp->n, 4), kBlock, 0, st>>>(x, nullptr, pointers(sp->v.U), cp, sp->coef.get(), k,
+                                                                 sp->n, 1.0, nullptr);
+    OWTK_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_rc_residual(const double* __restrict__ b, const double* __restrict__ ax,
+                                                     double* __restrict__ r, std::int64_t n)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += step)
+        r[t] = b[t] - ax[t];
+}
+
+__global__ __launch_bounds__(256) void k_rc_diff(const double* __restrict__ x, const double* __restrict__ x0,
+                                                 double* __restrict__ out, std::int64_t n)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += step)
+        out[t] = x[t] - x0[t];
+}
+
+} // namespace
+
+// recycled_fgmres (recycling.hpp:291-360)
+int recycled_fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space sp,
+                        const owtk_solver_options* o, bool update, owtk_solver_result* res)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    init_result(res);
+    if (!valid_options(o) || o->restart == 0) return OWTK_OK;
+    if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
+        throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
+    if (sp->n != m->owned * m->bs || sp->local != m->local() * m->bs)
+        throw std::invalid_argument("recycle space vector layout mismatch");
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const std::int64_t n = sp->n;
+    const unsigned int g = dev::stream_grid(n, 4);
+    owtk_solver_result tel{};
+    DeviceArray<double> x0(static_cast<std::size_t>(n));
+    OWTK_HIP(hipMemcpyAsync(x0.get(), x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (!sp->v.C.empty()) {
+        rc_refresh(sp, m, &tel);
+        DeviceArray<double> work(static_cast<std::size_t>(sp->local)), r(static_cast<std::size_t>(n));
+        dev::EpiStore<double> e;
+        e.y = work.get();
+        spmv_f64(m, x, e);
+        ++tel.operator_applications;
+        k_rc_residual<<<g, kBlock, 0, st>>>(b, work.get(), r.get(), n);
+        OWTK_LAUNCH_CHECK();
+        rc_project(sp, r.get(), x, &tel);
+        ctx->sync();
+    }
+    auto eng = fp64_engine(m, pc);
+    const int k = static_cast<int>(sp->v.C.size());
+    DeviceArray<double> cached(static_cast<std::size_t>(sp->local));
+    kern::fill_zero_bytes(st, cached.get(), cached.bytes());
+    const auto up = pointers(sp->v.U), cp = pointers(sp->v.C);
+    double* dcoef = sp->coef.get();
+    const auto scratch = ctx->scratch();
+    auto base_pc = eng.pc;
+    auto base_op = eng.op;
+    // RecycledPreconditioner::apply (recycling.hpp:236-262)
+    eng.pc = [&, base_pc, base_op](const double* in, double* out, const int* stop) {
+        base_pc(in, out, stop);
+        base_op(out, cached.get(), stop);
+        if (k > 0) {
+            kern::multidot_to(st, scratch, n, k, cp.v, cached.get(), dcoef);
+            k_rc_project<<<g, kBlock, 0, st>>>(out, cached.get(), up, cp, dcoef, k, n, -1.0, stop);
+            OWTK_LAUNCH_CHECK();
+        }
+    };
+    // RecycledOperator::apply (recycling.hpp:206-214): the cached projected image
+    eng.op = [&](const double*, double* out, const int* stop) { copy_stop_f64(st, cached.get(), out, n, stop); };
+    Counters cnt;
+    eng.run(b, x, o, res, cnt);
+    res->operator_applications += tel.operator_applications;
+    res->global_reductions += tel.global_reductions + (k > 0 ? res->preconditioner_applications : 0);
+    if (update && res->status == status_converged) {
+        DeviceArray<double> corr(static_cast<std::size_t>(n));
+        k_rc_diff<<<g, kBlock, 0, st>>>(x, x0.get(), corr.get(), n);
+        OWTK_LAUNCH_CHECK();
+        rc_add(sp, m, corr.get(), o->breakdown_tolerance, res);
+    }
+    ctx->sync();
+    if (o->collect_timings)
+        res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return OWTK_OK;
+}
+
 } // namespace owtk
 
 extern "C" {
@@ -971,6 +1208,113 @@ int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_prec
         if (!m || !b || !x || !options || !result || !fallback_pc) throw std::invalid_argument("null argument");
         m->ctx->activate();
         fgmres_refined_run(m, inner_pc, fallback_pc, b, x, options, result);
+    });
+}
+
+} // extern "C"
+
+extern "C" {
+
+int owtk_recycle_space_create(owtk_matrix layout, uint64_t capacity, owtk_recycle_space* out)
+{
+    return guarded([&] {
+        if (!layout || !out) throw std::invalid_argument("null argument");
+        if (capacity == 0) throw std::invalid_argument("recycle space capacity must be positive");
+        if (capacity > static_cast<std::uint64_t>(kern::kMaxMultidot))
+            throw unsupported_error("recycle space capacity above 62 is not supported on the device path");
+        layout->ctx->activate();
+        auto sp = std::make_unique<owtk_recycle_space_s>();
+        sp->ctx = layout->ctx;
+        sp->n = layout->owned * layout->bs;
+        sp->local = layout->local() * layout->bs;
+        sp->capacity = capacity;
+        sp->coef.allocate(capacity + 2);
+        *out = sp.release();
+    });
+}
+
+int owtk_recycle_space_destroy(owtk_recycle_space sp)
+{
+    return guarded([&] {
+        if (!sp) return;
+        sp->ctx->activate();
+        sp->ctx->sync();
+        delete sp;
+    });
+}
+
+int owtk_recycle_space_size(owtk_recycle_space sp, int64_t* size)
+{
+    return guarded([&] {
+        if (!sp || !size) throw std::invalid_argument("null argument");
+        *size = static_cast<int64_t>(sp->v.C.size());
+    });
+}
+
+int owtk_recycle_space_clear(owtk_recycle_space sp)
+{
+    return guarded([&] {
+        if (!sp) throw std::invalid_argument("null argument");
+        sp->ctx->activate();
+        sp->ctx->sync();
+        sp->v = owtk::RecycleVectors{};
+    });
+}
+
+int owtk_recycle_space_get(owtk_recycle_space sp, int64_t index, double* u, double* c)
+{
+    return guarded([&] {
+        if (!sp || !u || !c) throw std::invalid_argument("null argument");
+        if (index < 0 || index >= static_cast<int64_t>(sp->v.C.size())) throw std::invalid_argument("index out of range");
+        sp->ctx->activate();
+        const hipStream_t st = sp->ctx->stream;
+        OWTK_HIP(hipMemcpyAsync(u, sp->v.U[index].get(), sp->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        OWTK_HIP(hipMemcpyAsync(c, sp->v.C[index].get(), sp->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        sp->ctx->sync();
+    });
+}
+
+int owtk_recycle_space_add_candidate(owtk_recycle_space sp, owtk_matrix op, const double* candidate,
+                                     double dependence_tolerance, int32_t* added, owtk_solver_result* telemetry)
+{
+    return guarded([&] {
+        if (!sp || !op || !candidate || !added) throw std::invalid_argument("null argument");
+        if (sp->n != op->owned * op->bs || sp->local != op->local() * op->bs)
+            throw std::invalid_argument("recycle space vector layout mismatch");
+        op->ctx->activate();
+        *added = owtk::rc_add(sp, op, candidate, dependence_tolerance, telemetry) ? 1 : 0;
+        sp->ctx->sync();
+    });
+}
+
+int owtk_recycle_space_refresh(owtk_recycle_space sp, owtk_matrix op, owtk_solver_result* telemetry)
+{
+    return guarded([&] {
+        if (!sp || !op) throw std::invalid_argument("null argument");
+        if (sp->n != op->owned * op->bs || sp->local != op->local() * op->bs)
+            throw std::invalid_argument("recycle space vector layout mismatch");
+        op->ctx->activate();
+        owtk::rc_refresh(sp, op, telemetry);
+    });
+}
+
+int owtk_recycle_space_project(owtk_recycle_space sp, const double* residual, double* x, owtk_solver_result* telemetry)
+{
+    return guarded([&] {
+        if (!sp || !residual || !x) throw std::invalid_argument("null argument");
+        sp->ctx->activate();
+        owtk::rc_project(sp, residual, x, telemetry);
+        sp->ctx->sync();
+    });
+}
+
+int owtk_recycled_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space space,
+                         const owtk_solver_options* options, int32_t update_recycle_space, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !space || !options || !result) throw std::invalid_argument("null argument");
+        m->ctx->activate();
+        owtk::recycled_fgmres_run(m, pc, b, x, space, options, update_recycle_space != 0, result);
     });
 }
 

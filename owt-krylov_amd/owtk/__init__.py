@@ -20,7 +20,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
 
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
-    "communication_hiding_bicgstab", "idrs", "Multigrid", "fgmres",
+    "communication_hiding_bicgstab", "idrs", "Multigrid", "RecycleSpace", "recycled_fgmres", "fgmres",
     "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
@@ -256,6 +256,62 @@ class Matrix:
                 lib().owtk_matrix_destroy(self.handle)
         except Exception:
             pass
+
+
+class RecycleSpace:
+    """RecycleSpace<double> (recycling.hpp:24-187) on the device: U and
+    C = A U, C orthonormal, at most `capacity` pairs, kept across solves."""
+
+    def __init__(self, layout: Matrix, capacity: int):
+        self.matrix = layout
+        self.handle = C.c_void_p()
+        check(lib().owtk_recycle_space_create(layout.handle, capacity, C.byref(self.handle)))
+
+    def size(self) -> int:
+        n = C.c_int64()
+        check(lib().owtk_recycle_space_size(self.handle, C.byref(n)))
+        return n.value
+
+    def clear(self) -> None:
+        check(lib().owtk_recycle_space_clear(self.handle))
+
+    def get(self, ctx: "Context", i: int):
+        """(U_i, C_i) as host arrays."""
+        n = self.matrix.owned_size
+        u, c = ctx.empty(n), ctx.empty(n)
+        check(lib().owtk_recycle_space_get(self.handle, i, u.ptr, c.ptr))
+        return u.download(), c.download()
+
+    def add_candidate(self, candidate: DeviceBuffer, tol: float = 128 * np.finfo(np.float64).eps) -> tuple:
+        added = C.c_int32()
+        tel = _lib.SolverResult()
+        check(lib().owtk_recycle_space_add_candidate(self.handle, self.matrix.handle, candidate.ptr, tol,
+                                                     C.byref(added), C.byref(tel)))
+        return bool(added.value), tel.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_recycle_space_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def recycled_fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
+                    space: RecycleSpace, options=None, update: bool = True) -> dict:
+    """recycled_fgmres (recycling.hpp:291-360) on the device."""
+    opts = options if options is not None else SolverOptions()
+    res = _lib.SolverResult()
+    check(lib().owtk_recycled_fgmres(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
+                                     space.handle, C.byref(opts), 1 if update else 0, C.byref(res)))
+    d = res.as_dict()
+    d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+    d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
+    return d
 
 
 class Multigrid:

@@ -202,6 +202,17 @@ PROTOTYPES: dict[str, tuple] = {
                                       C.POINTER(C.c_int64), _P, _P, _P]),
     "owtk_write_block_csr": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, _P]),
     "owtk_write_ilu_factors": (C.c_int, [_P, C.c_char_p]),
+    "owtk_recycle_space_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "owtk_recycle_space_destroy": (C.c_int, [_P]),
+    "owtk_recycle_space_size": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "owtk_recycle_space_clear": (C.c_int, [_P]),
+    "owtk_recycle_space_get": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "owtk_recycle_space_add_candidate": (C.c_int, [_P, _P, _P, C.c_double, C.POINTER(C.c_int32),
+                                                   C.POINTER(SolverResult)]),
+    "owtk_recycle_space_refresh": (C.c_int, [_P, _P, C.POINTER(SolverResult)]),
+    "owtk_recycle_space_project": (C.c_int, [_P, _P, _P, C.POINTER(SolverResult)]),
+    "owtk_recycled_fgmres": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(SolverOptions), C.c_int32,
+                                       C.POINTER(SolverResult)]),
     "owtk_multigrid_options_default": (None, [C.POINTER(MultigridOptions)]),
     "owtk_multigrid_create": (C.c_int, [_P, C.POINTER(MultigridOptions), C.POINTER(_P)]),
     "owtk_multigrid_destroy": (C.c_int, [_P]),

@@ -483,3 +483,43 @@ def test_multigrid_reference_block_system(smoother, cycle):
     x, res = mg.solve(om.spmv(exact), cycle=cycle, relative_tolerance=1e-9, maximum_iterations=100)
     assert res["status"] == 0
     assert np.max(np.abs(x - exact)) < 1e-8
+
+
+def test_recycled_fgmres_properties(c0):
+    """recycled_fgmres / RecycleSpace (recycling.hpp:24-360):
+    - with an empty space and no update it is fgmres, bit for bit;
+    - the space keeps C = A U orthonormal;
+    - re-solving a right-hand side the space already spans needs no Arnoldi
+      step: x0 + U C^T r0 is the solution (GCRO's exact projection);
+    - a nearby right-hand side needs fewer iterations than plain FGMRES;
+    - at capacity the oldest pair is dropped."""
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    n = s.nodes * 3
+    pc = orc.OPreconditioner(om, 1)
+    space = orc.ORecycleSpace(2)
+    xf, rf = orc.fgmres(om, pc, s.rhs)
+    x0, r0 = orc.recycled_fgmres(om, pc, s.rhs, space, update=False)
+    assert np.array_equal(xf, x0) and rf == r0 and space.size() == 0
+    x1, r1 = orc.recycled_fgmres(om, pc, s.rhs, space)
+    assert r1["status"] == 0 and space.size() == 1
+    x2, r2 = orc.recycled_fgmres(om, pc, s.rhs, space)
+    assert r2["status"] == 0 and r2["iterations"] == 0
+    b3 = s.rhs * (1 + 0.01 * np.random.default_rng(3).standard_normal(n))
+    _, rp = orc.fgmres(om, pc, b3)
+    _, r3 = orc.recycled_fgmres(om, pc, b3, space)
+    assert r3["status"] == 0 and r3["iterations"] < rp["iterations"]
+    assert space.size() == 2
+    U, Cm = zip(*(space.get(i, n) for i in range(space.size())))
+    G = np.array([[c1 @ c2 for c2 in Cm] for c1 in Cm])
+    assert np.max(np.abs(G - np.eye(len(Cm)))) < 1e-13
+    for u, c in zip(U, Cm):
+        assert np.max(np.abs(om.spmv(u) - c)) < 1e-12
+    # capacity 2: a third independent candidate evicts the oldest
+    u_old = U[0]
+    added, _ = space.add_candidate(om, np.random.default_rng(4).standard_normal(n))
+    assert added and space.size() == 2
+    assert not np.array_equal(space.get(0, n)[0], u_old)
+    # a candidate already in the space is rejected
+    added, _ = space.add_candidate(om, space.get(1, n)[0])
+    assert not added
