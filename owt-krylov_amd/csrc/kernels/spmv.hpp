@@ -212,8 +212,20 @@ __device__ inline unsigned int xcd_block(unsigned int b, unsigned int n)
     return (b % 8) * per + b / 8;
 }
 
+// Slots a SELL row kernel loads without a branch: the smallest of 4, 6 and 8
+// covering the widest slice (rows longer than 8 finish in a loop).
+inline int sell_unroll(int max_width) { return max_width <= 4 ? 4 : (max_width <= 6 ? 6 : 8); }
+
 // Component-diagonal (and split) SpMV, SELL-64 layout, one lane per row.
-template<int BS, class V, class X, bool DIAG, class E>
+//
+// The first U slots of a row are loaded unconditionally: a slot index past
+// the slice's width is clamped to its last run (same cache lines, always in
+// bounds), and each product is kept only for k < len by a select.  A load
+// under a divergent branch makes the compiler wait for it at the branch's
+// end, which serialised the row's memory round trips; with no branch every
+// column, value and gather load of the row is in flight before the first FMA.
+// The FMAs still run in CSR order over k < len, so results are unchanged.
+template<int BS, class V, class X, bool DIAG, class E, int U>
 __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restrict__ slice_ptr,
                                                     const std::int32_t* __restrict__ row_len,
                                                     const std::int32_t* __restrict__ cols,
@@ -242,26 +254,36 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
             for (int c = 0; c < BS; ++c) acc[c] = A(0);
         }
         const std::int64_t base = slice_ptr[slice];
+        const int width = static_cast<int>((slice_ptr[slice + 1] - base) >> 6);
         const int len = row_len[row];
-        // Mesh rows hold at most kUnroll entries: the unrolled, predicated loop
-        // lets every column, value and gather load of the row issue before the
-        // first FMA.  Longer rows finish in the generic loop.
-        constexpr int kUnroll = 8;
-        std::int64_t colk[kUnroll];
+        if (width > 0) {
+            std::int32_t colk[U];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k)
-            colk[k] = k < len ? static_cast<std::int64_t>(__builtin_nontemporal_load(&cols[base + 64 * k + lane])) : row;
+            for (int k = 0; k < U; ++k) {
+                const int kk = k < width ? k : width - 1;
+                colk[k] = __builtin_nontemporal_load(&cols[base + 64 * kk + lane]);
+            }
+            A v[U][BS], xv[U][BS];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-            if (k < len) {
-                const V* vk = vals + (base + 64 * k) * BS + lane;
+            for (int k = 0; k < U; ++k) {
+                const int kk = k < width ? k : width - 1;
+                const V* vk = vals + (base + 64 * kk) * BS + lane;
 #pragma unroll
-                for (int c = 0; c < BS; ++c)
-                    acc[c] = dfma(static_cast<A>(load_value_nt(vk, 64 * c)),
-                                  static_cast<A>(e.xval(x, colk[k] * BS + c)), acc[c]);
+                for (int c = 0; c < BS; ++c) {
+                    v[k][c] = static_cast<A>(load_value_nt(vk, 64 * c));
+                    xv[k][c] = static_cast<A>(e.xval(x, static_cast<std::int64_t>(colk[k]) * BS + c));
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const A f = dfma(v[k][c], xv[k][c], acc[c]);
+                    acc[c] = k < len ? f : acc[c];
+                }
             }
         }
-        for (int k = kUnroll; k < len; ++k) {
+        for (int k = U; k < len; ++k) {
             const std::int64_t run = base + 64 * k;
             const std::int64_t col = cols[run + lane];
             const V* vk = vals + run * BS + lane;
@@ -273,6 +295,32 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
         epi_row<BS>(e, row * BS, acc, red);
     }
     if constexpr (E::K > 0) store_partials<E::K>(red, E::kMaxMask, scratch);
+}
+
+// Launch spmv_sell_cd with the unroll that covers the matrix's slices.
+template<int BS, class V, class X, bool DIAG, class E>
+void launch_sell_cd(int max_width, unsigned int grid, hipStream_t st, const std::int32_t* slice_ptr,
+                    const std::int32_t* row_len, const std::int32_t* cols, const V* vals, const V* diag,
+                    const X* x, std::int64_t rows, E e, ReduceScratch scratch)
+{
+    switch (sell_unroll(max_width)) {
+    case 4: spmv_sell_cd<BS, V, X, DIAG, E, 4><<<grid, kBlock, 0, st>>>(slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    case 6: spmv_sell_cd<BS, V, X, DIAG, E, 6><<<grid, kBlock, 0, st>>>(slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    default: spmv_sell_cd<BS, V, X, DIAG, E, 8><<<grid, kBlock, 0, st>>>(slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    }
+}
+
+template<class V, class X, bool DIAG, class E>
+void launch_sell_cd_bs(int bs, int max_width, unsigned int grid, hipStream_t st, const std::int32_t* slice_ptr,
+                       const std::int32_t* row_len, const std::int32_t* cols, const V* vals, const V* diag,
+                       const X* x, std::int64_t rows, E e, ReduceScratch scratch)
+{
+    switch (bs) {
+    case 1: launch_sell_cd<1, V, X, DIAG, E>(max_width, grid, st, slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    case 2: launch_sell_cd<2, V, X, DIAG, E>(max_width, grid, st, slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    case 3: launch_sell_cd<3, V, X, DIAG, E>(max_width, grid, st, slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    default: launch_sell_cd<4, V, X, DIAG, E>(max_width, grid, st, slice_ptr, row_len, cols, vals, diag, x, rows, e, scratch); break;
+    }
 }
 
 // Component-diagonal (and split) SpMV for any block size, reference CSR
@@ -421,19 +469,11 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
             default: spmv_sell_bsr<1, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
             }
         } else if (m->has_diag_array()) {
-            switch (m->bs) {
-            case 1: spmv_sell_cd<1, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            case 2: spmv_sell_cd<2, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            case 3: spmv_sell_cd<3, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            default: spmv_sell_cd<4, V, X, true, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            }
+            launch_sell_cd_bs<V, X, true, E>(m->bs, s.max_width, grid, st, s.slice_ptr.get(), s.row_len.get(),
+                                             s.columns.get(), vals, diag, x, rows, e, scratch);
         } else {
-            switch (m->bs) {
-            case 1: spmv_sell_cd<1, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            case 2: spmv_sell_cd<2, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            case 3: spmv_sell_cd<3, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            default: spmv_sell_cd<4, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, diag, x, rows, e, scratch); break;
-            }
+            launch_sell_cd_bs<V, X, false, E>(m->bs, s.max_width, grid, st, s.slice_ptr.get(), s.row_len.get(),
+                                              s.columns.get(), vals, diag, x, rows, e, scratch);
         }
     } else {
         if (m->dense()) {

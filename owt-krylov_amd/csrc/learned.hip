@@ -684,12 +684,8 @@ void offdiag_spmv(owtk_matrix m, const V* vals, const X* x, E e)
     const std::int64_t rows = m->owned;
     if (s.layout == Layout::sell) {
         const unsigned int grid = rows_grid(rows);
-        switch (m->bs) {
-        case 1: spmv_sell_cd<1, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
-        case 2: spmv_sell_cd<2, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
-        case 3: spmv_sell_cd<3, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
-        default: spmv_sell_cd<4, V, X, false, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, nullptr, x, rows, e, scratch); break;
-        }
+        launch_sell_cd_bs<V, X, false, E>(m->bs, s.max_width, grid, st, s.slice_ptr.get(), s.row_len.get(),
+                                          s.columns.get(), vals, static_cast<const V*>(nullptr), x, rows, e, scratch);
     } else {
         std::int64_t blocks = (rows + dev::kWavesPerBlock - 1) / dev::kWavesPerBlock;
         blocks = std::max<std::int64_t>(1, std::min<std::int64_t>(blocks, dev::kMaxReduceBlocks));
