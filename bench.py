@@ -41,7 +41,7 @@ MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
 # kernel's known input; WRITE_SIZE by 1), c1 workload.
-TRAFFIC_BYTES = {("spmv", "f64"): 520_161_280, ("learned", "bf16"): 286_404_800}
+TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 286_407_885}
 KERNEL_NAMES = {
     "learned_bf16": "learned series step, BF16 matrix: off-diagonal SpMV + fused update "
                     "(spmv_sell_cd<3,bf16,float,false,EpiPoly>)",

@@ -140,10 +140,37 @@ __device__ inline void row_spmv(const Ptrs& P, std::int64_t prow, const double* 
     const std::int64_t slice = prow >> 6;
     const int lane = static_cast<int>(prow & 63);
     const std::int64_t base = P.slice_ptr[slice];
+    const int width = static_cast<int>((P.slice_ptr[slice + 1] - base) >> 6);
     const int len = P.row_len[prow];
 #pragma unroll
     for (int c = 0; c < BS; ++c) y[c] = 0.0;
-    for (int k = 0; k < len; ++k) {
+    // The first U slots load with no branch (index clamped into the slice,
+    // product kept for k < len), so all of them are in flight at once.
+    constexpr int U = 8;
+    if (width > 0) {
+        std::int32_t colk[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) colk[k] = P.cols[base + 64 * (k < width ? k : width - 1) + lane];
+        double v[U][BS], xv[U][BS];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const double* vk = P.vals + (base + 64 * (k < width ? k : width - 1)) * BS + lane;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                v[k][c] = vk[64 * c];
+                xv[k][c] = x[static_cast<std::int64_t>(colk[k]) * BS + c];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const double f = __builtin_fma(v[k][c], xv[k][c], y[c]);
+                y[c] = k < len ? f : y[c];
+            }
+        }
+    }
+    for (int k = U; k < len; ++k) {
         const std::int64_t run = base + 64 * k;
         const std::int64_t col = P.cols[run + lane];
         const double* vk = P.vals + run * BS + lane;
