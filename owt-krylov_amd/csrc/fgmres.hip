@@ -471,6 +471,7 @@ struct Counters {
 // the pinned flags and events of the throttled launch loop.
 struct EngineWork {
     DeviceArray<unsigned char> vecs;
+    std::int64_t ld = -1;  // vector stride the zeroed ghost entries fit
     DeviceArray<FgState> dstate;
     PinnedBuffer flags;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -484,6 +485,14 @@ struct EngineWork {
     }
 };
 
+// The context's FGMRES workspace for elements of `bytes` bytes.
+std::shared_ptr<EngineWork> engine_work(owtk_context ctx, std::size_t bytes)
+{
+    std::shared_ptr<void>& slot = ctx->fg_work[bytes == 8 ? 1 : 0];
+    if (!slot) slot = std::make_shared<EngineWork>();
+    return std::static_pointer_cast<EngineWork>(slot);
+}
+
 // Columns launched between two looks at the device's stop flag.  The host
 // waits for the check two blocks back, so the GPU always has a block queued
 // while at most two blocks of no-op launches follow a stop.
@@ -492,7 +501,7 @@ constexpr int kCheckEvery = 2;
 template<class T>
 struct FgEngine {
     owtk_context ctx;
-    std::unique_ptr<EngineWork> work;
+    std::shared_ptr<EngineWork> work;  // the context's workspace for T
     std::int64_t n = 0;       // owned size
     std::int64_t local = 0;   // local size (SpMV inputs)
     std::function<void(const T*, T*, const int*)> op;
@@ -509,12 +518,14 @@ struct FgEngine {
         const auto scratch = ctx->scratch();
         const int m = static_cast<int>(o->restart);
         const std::int64_t ld = local;
-        if (!work) work = std::make_unique<EngineWork>();
+        if (!work) work = engine_work(ctx, sizeof(T));
         const std::size_t need = static_cast<std::size_t>((2 * m + 3) * ld) * sizeof(T);
-        if (work->vecs.bytes() < need) {
-            // ghost entries of the basis are never written, so they stay zero
-            work->vecs.allocate(need);
+        if (work->vecs.bytes() < need || work->ld != ld) {
+            // ghost entries of the basis are never written, so after one zero
+            // fill they stay zero for every later run with the same stride
+            if (work->vecs.bytes() < need) work->vecs.allocate(need);
             kern::fill_zero_bytes(st, work->vecs.get(), work->vecs.bytes());
+            work->ld = ld;
         }
         if (!work->dstate.get()) work->dstate.allocate(1);
         if (!work->ev[0]) {
@@ -550,12 +561,15 @@ struct FgEngine {
             res->arnoldi_norm_verifications = cnt.verif;
             return OWTK_OK;
         };
+        // ||b - A x|| and ||b|| in two slots, read back together
         residual(b, x, r, d_norm);
         ++cnt.ops;
-        const double initial = read_norm();
-        if constexpr (sizeof(T) == 8) kern::norm_to(st, scratch, reinterpret_cast<const double*>(b), n, d_norm);
-        else kern::norm_to(st, scratch, reinterpret_cast<const float*>(b), n, d_norm);
-        const double rhs_norm = read_norm();
+        if constexpr (sizeof(T) == 8) kern::norm_to(st, scratch, reinterpret_cast<const double*>(b), n, d_norm + 1);
+        else kern::norm_to(st, scratch, reinterpret_cast<const float*>(b), n, d_norm + 1);
+        double pair[2];
+        OWTK_HIP(hipMemcpyAsync(pair, d_norm, sizeof(pair), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        const double initial = pair[0], rhs_norm = pair[1];
         cnt.reds += 2;
         const double thr = threshold_of(rhs_norm, o);
         res->initial_residual_norm = initial;
@@ -804,9 +818,23 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
     const auto scratch = ctx->scratch();
     const std::int64_t n = m->owned * m->bs;
     const std::int64_t L = m->local() * m->bs;
-    DeviceArray<double> r(static_cast<std::size_t>(L));
-    DeviceArray<float> rf(static_cast<std::size_t>(L)), d(static_cast<std::size_t>(L));
-    kern::fill_zero_bytes(st, rf.get(), rf.bytes());
+    // r (FP64), rf (r in FP32) and the correction d (FP32), each local-length
+    // with zero ghosts, plus the fallback's saved x: the context keeps them
+    // between solves and zero-fills them when the layout changes
+    const auto round256 = [](std::size_t b) { return (b + 255) & ~static_cast<std::size_t>(255); };
+    const std::size_t r_bytes = round256(static_cast<std::size_t>(L) * 8);
+    const std::size_t f_bytes = round256(static_cast<std::size_t>(L) * 4);
+    const std::size_t ws_bytes = r_bytes + 2 * f_bytes + (fallback ? round256(static_cast<std::size_t>(n) * 8) : 0);
+    if (ctx->refine_ws.bytes() < ws_bytes || ctx->refine_layout != L) {
+        if (ctx->refine_ws.bytes() < ws_bytes) ctx->refine_ws.allocate(ws_bytes);
+        kern::fill_zero_bytes(st, ctx->refine_ws.get(), ctx->refine_ws.bytes());
+        ctx->refine_layout = L;
+    }
+    unsigned char* ws = ctx->refine_ws.get();
+    double* r = reinterpret_cast<double*>(ws);
+    float* rf = reinterpret_cast<float*>(ws + r_bytes);
+    float* d = reinterpret_cast<float*>(ws + r_bytes + f_bytes);
+    double* x_prev = fallback ? reinterpret_cast<double*>(ws + r_bytes + 2 * f_bytes) : nullptr;
     double* d_norm = ctx->scalars.get() + 24;
     auto read = [&]() {
         double v;
@@ -815,22 +843,30 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
         return v;
     };
     Counters cnt;
-    auto outer_residual = [&]() {
+    auto launch_outer_residual = [&](double* slot) {
         dev::EpiResidual e;
-        e.y = r.get();
+        e.y = r;
         e.b = b;
         e.n = n;
-        e.norm_out = d_norm;
+        e.norm_out = slot;
         spmv_f64(m, x, e);
         ++cnt.ops;
         ++cnt.reds;
+    };
+    auto outer_residual = [&]() {
+        launch_outer_residual(d_norm);
         return read();
     };
+    // ||b|| and the initial ||b - A x||, read back together
     kern::norm_to(st, scratch, b, n, d_norm);
     ++cnt.reds;
-    const double rhs_norm = read();
+    launch_outer_residual(d_norm + 1);
+    double pair[2];
+    OWTK_HIP(hipMemcpyAsync(pair, d_norm, sizeof(pair), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double rhs_norm = pair[0];
     const double thr = threshold_of(rhs_norm, o);
-    double rn = outer_residual();
+    double rn = pair[1];
     res->initial_residual_norm = rn;
     set_residual(res, rn, rn, rhs_norm);
     auto finish = [&](int status) {
@@ -851,8 +887,6 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
     auto inner = low_engine(m, pc, dtype);
     std::unique_ptr<FgEngine<float>> backup;  // built on the first switch
     bool use_fallback = false;
-    DeviceArray<double> x_prev;
-    if (fallback) x_prev.allocate(static_cast<std::size_t>(n));
     owtk_solver_options io = *o;
     io.relative_tolerance = ro->inner_relative_tolerance;
     io.absolute_tolerance = 0.0;
@@ -862,14 +896,14 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
     while (!(rn <= thr)) {
         if (steps >= ro->maximum_refinements || res->inner_iterations >= o->maximum_iterations)
             return finish(status_maximum_iterations);
-        kern::convert(st, rf.get(), r.get(), static_cast<std::size_t>(n));
-        kern::fill_zero_bytes(st, d.get(), d.bytes());
+        kern::convert(st, rf, r, static_cast<std::size_t>(n));
+        kern::fill_zero_bytes(st, d, f_bytes);
         owtk_solver_result ir;
         const bool guarded_step = fallback && !use_fallback;
         if (guarded_step)
-            OWTK_HIP(hipMemcpyAsync(x_prev.get(), x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (use_fallback) backup->run(rf.get(), d.get(), &io, &ir, cnt);
-        else inner.run(rf.get(), d.get(), &io, &ir, cnt);
+            OWTK_HIP(hipMemcpyAsync(x_prev, x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (use_fallback) backup->run(rf, d, &io, &ir, cnt);
+        else inner.run(rf, d, &io, &ir, cnt);
         res->inner_iterations += ir.iterations;
         res->iterations = res->inner_iterations;
         ++steps;
@@ -881,7 +915,7 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
             return finish(status_breakdown);
         }
         if (!inner_failed) {
-            k_add_f32_to_f64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(x, d.get(), n);
+            k_add_f32_to_f64<<<dev::stream_grid(n, 4), kBlock, 0, st>>>(x, d, n);
             OWTK_LAUNCH_CHECK();
         }
         const double previous = rn;
@@ -892,7 +926,7 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
             // the true residual judges the step: undo it unless it helped,
             // then continue with the fallback preconditioner
             if (!(rn < previous)) {
-                OWTK_HIP(hipMemcpyAsync(x, x_prev.get(), n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                OWTK_HIP(hipMemcpyAsync(x, x_prev, n * sizeof(double), hipMemcpyDeviceToDevice, st));
                 rn = outer_residual();
                 set_residual(res, rn, rn, rhs_norm);
             }

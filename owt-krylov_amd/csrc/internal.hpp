@@ -115,6 +115,17 @@ struct owtk_context_s {
     // kept on the device for later solves of the same size
     std::map<std::tuple<std::int64_t, int, std::uint64_t>, owtk::DeviceArray<double>> idr_draws;
     std::size_t idr_draw_bytes = 0;
+    // FGMRES workspaces (basis, state, flags) kept between solves, one per
+    // element size ([0] FP32, [1] FP64): the restart-30 basis at 2M vertices
+    // is 1.6 GB, whose allocation and zero fill would otherwise recur on
+    // every call.  Calls on a context run one at a time and FGMRES engines
+    // never nest, so one workspace per element size serves every solve.
+    // Type-erased because the workspace type is private to the FGMRES code.
+    std::shared_ptr<void> fg_work[2];
+    // FP64 iterative-refinement vectors, kept between solves for the same
+    // reason; refine_layout is the vector length their zeroed ghosts fit.
+    owtk::DeviceArray<unsigned char> refine_ws;
+    std::int64_t refine_layout = -1;
 
     owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
     void activate() const;  // hipSetDevice(device)
