@@ -149,9 +149,10 @@ __device__ inline void store_partials(double (&v)[K], std::uint64_t max_mask, Re
 
 template<int K, class Fin>
 __global__ __launch_bounds__(256) void k_finish(Fin fin, unsigned int nblocks, std::uint64_t max_mask,
-                                                ReduceScratch scratch, const int* stop)
+                                                ReduceScratch scratch, const int* stop, const int* need)
 {
     if (stop != nullptr && *stop != 0) return;
+    if (need != nullptr && *need == 0) return;
     __shared__ double totals[K > 0 ? K : 1];
     // The partials were written by other XCDs, so each load waits a full
     // memory latency.  A thread issues all loads of a chunk (U blocks x K
@@ -187,11 +188,12 @@ __global__ __launch_bounds__(256) void k_finish(Fin fin, unsigned int nblocks, s
     fin(static_cast<const double*>(totals));
 }
 
+// stop: skip when *stop != 0; need (optional): skip when *need == 0.
 template<int K, class Fin>
 inline void launch_finish(hipStream_t st, const Fin& fin, unsigned int nblocks, std::uint64_t max_mask,
-                          ReduceScratch scratch, const int* stop)
+                          ReduceScratch scratch, const int* stop, const int* need = nullptr)
 {
-    k_finish<K, Fin><<<1, kBlock, 0, st>>>(fin, nblocks, max_mask, scratch, stop);
+    k_finish<K, Fin><<<1, kBlock, 0, st>>>(fin, nblocks, max_mask, scratch, stop, need);
 }
 
 // Sum over the block, result on every thread.

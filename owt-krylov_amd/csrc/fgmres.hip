@@ -183,16 +183,22 @@ struct FinA {
 };
 
 // Pass B: w -= sum h_j v_j; with re-orthogonalisation also c_j = <w, v_j>, <w, w>.
+// Without re-orthogonalisation and with h(k+1,k) already settled from the
+// projected norm, the new basis vector is final here: it is written straight
+// to v_{k+1} = w / h (rounded through T exactly as pass C would) and pass C
+// skips its vector work.
 template<int K, class T, int L>
 __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std::int64_t ld, int count,
-                                                   T* __restrict__ w, std::int64_t n, FgState* s, int k,
-                                                   ReduceScratch scratch)
+                                                   T* __restrict__ w, T* __restrict__ vnext, std::int64_t n,
+                                                   FgState* s, int k, ReduceScratch scratch)
 {
     if (s->halt) return;
     __shared__ double h[K];
     if (threadIdx.x < count) h[threadIdx.x] = Hd(s, threadIdx.x, k);
     __syncthreads();
     const bool reorth = s->do_reorth != 0;
+    const bool final_here = !reorth && s->direct != 0;
+    const double inv = s->inv_h;
     double red[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) red[j] = 0.0;
@@ -215,6 +221,13 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
         Packet<T, L> wt;
 #pragma unroll
         for (int l = 0; l < L; ++l) wt.v[l] = static_cast<T>(wi[l]);
+        if (final_here) {
+            Packet<T, L> vn;
+#pragma unroll
+            for (int l = 0; l < L; ++l) vn.v[l] = static_cast<T>(static_cast<double>(wt.v[l]) * inv);
+            store_packet<T, L>(vnext + i, vn);
+            continue;
+        }
         store_packet<T, L>(w + i, wt);
         if (reorth) {
 #pragma unroll
@@ -273,7 +286,8 @@ __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std:
     const double inv = s->inv_h;
     double red[2] = {0.0, 0.0};
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    const std::int64_t packets = n / L;
+    // pass B already wrote v_{k+1} when there was nothing left to project
+    const std::int64_t packets = (!reorth && direct) ? 0 : n / L;
     for (std::int64_t ip = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ip < packets; ip += step) {
         const std::int64_t i = ip * L;
         const Packet<T, L> wp = load_packet<T, L>(w + i);
@@ -691,8 +705,9 @@ struct FgEngine {
         const int* halt = &s->halt;
         k_fg_pass_a<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
         dev::launch_finish<K>(st, FinA<K>{s, k, count}, gr, 0, scratch, halt);
-        k_fg_pass_b<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
-        dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt);
+        k_fg_pass_b<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
+        // FinB only acts after a re-orthogonalisation
+        dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt, &s->do_reorth);
         k_fg_pass_c<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
         dev::launch_finish<2>(st, FinC<T>{s, k, w, n}, gr, 2, scratch, halt);
         k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);

@@ -334,8 +334,9 @@ def test_fgmres_direct_solve(ctx):
         assert np.max(np.abs(x - exact)) < 1e-11
 
 
+@pytest.mark.parametrize("mode", [0, 2])
 @pytest.mark.parametrize("pc_kind", ["jacobi", "ilu"])
-def test_fgmres_c0_matches_oracle(ctx, c0, pc_kind):
+def test_fgmres_c0_matches_oracle(ctx, c0, pc_kind, mode):
     s = c0
     om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
     m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
@@ -343,9 +344,9 @@ def test_fgmres_c0_matches_oracle(ctx, c0, pc_kind):
         opc, pc = orc.OPreconditioner(om, 3, block_nodes=512), owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=512)
     else:
         opc, pc = orc.OPreconditioner(om, 1), owtk.Preconditioner(m, owtk.PC_JACOBI)
-    xo, ro = orc.fgmres(om, opc, s.rhs)
-    x, res = solve(ctx, owtk.fgmres, m, pc, s.rhs)
-    print("oracle", ro["iterations"], "gpu", res["iterations"])
+    xo, ro = orc.fgmres(om, opc, s.rhs, gmres_orthogonalization=mode)
+    x, res = solve(ctx, owtk.fgmres, m, pc, s.rhs, gmres_orthogonalization=mode)
+    print("mode", mode, "oracle", ro["iterations"], "gpu", res["iterations"])
     assert res["status_name"] == "converged" and ro["status"] == 0
     assert abs(res["iterations"] - ro["iterations"]) <= 1
     assert np.linalg.norm(s.rhs - om.spmv(x)) / np.linalg.norm(s.rhs) <= 1e-8
