@@ -41,17 +41,17 @@ MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
 # kernel's known input; WRITE_SIZE by 1), c1 workload.
-TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 286_407_885}
+TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 261_230_592}
 KERNEL_NAMES = {
-    "learned_bf16": "learned series step, BF16 matrix: off-diagonal SpMV + fused update "
-                    "(spmv_sell_cd<3,bf16,float,false,EpiPoly>)",
+    "learned_bf16": "learned series step, BF16 column-scaled off-diagonal values: SpMV + fused update "
+                    "(spmv_sell_cd<3,bf16,float,false,EpiSeries>)",
     "spmv_f64": "split SpMV, FP64 (spmv_sell_cd<3,double,double,true,EpiStore>)",
     "spmv_f32": "split SpMV, FP32 storage (spmv_sell_cd<3,float,float,...>)",
     "spmv_bf16": "split SpMV, BF16 storage (spmv_sell_cd<3,bf16,float,...>)",
-    "precond_f64": "block-Jacobi ILU(0) triangular sweeps, FP64 (k_tri_chain)",
-    "precond_low": "block-Jacobi ILU(0) triangular sweeps, low precision (k_tri_chain)",
-    "learned": "learned series step, FP32 matrix: off-diagonal SpMV + fused update "
-               "(spmv_sell_cd<3,float,float,false,EpiPoly>)",
+    "precond_f64": "block-Jacobi ILU(0) triangular sweeps, FP64 (k_tri_ell)",
+    "precond_low": "block-Jacobi ILU(0) triangular sweeps, low precision (k_tri_ell)",
+    "learned": "learned series step, FP32 column-scaled off-diagonal values: SpMV + fused update "
+               "(spmv_sell_cd<3,float,float,false,EpiSeries>)",
 }
 
 

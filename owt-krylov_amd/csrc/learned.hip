@@ -573,17 +573,65 @@ __global__ __launch_bounds__(256) void k_round_bf16(std::int64_t count, const fl
 
 // ------------------------------------------------------------ apply kernels --
 
-template<class X, class D>
-__global__ __launch_bounds__(256) void k_poly_init(std::int64_t nb, const X* __restrict__ in, const D* __restrict__ dinv,
-                                                   const float* __restrict__ gamma0, X* __restrict__ u0,
-                                                   X* __restrict__ z, const int* stop)
+// w_0 = r and z = gamma'_0 r (the apply's first step run on its own: degree 0,
+// or an in-place apply, whose first step may not gather the input it overwrites).
+template<class X>
+__global__ __launch_bounds__(256) void k_series_init(std::int64_t nb, const X* __restrict__ in,
+                                                     const X* __restrict__ gamma0, X* __restrict__ w0,
+                                                     X* __restrict__ z, const int* stop)
 {
     if (stop != nullptr && *stop != 0) return;
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nb; t += step) {
-        const X u = in[t] * static_cast<X>(dinv[t]);
-        u0[t] = u;
-        z[t] = static_cast<X>(gamma0[t]) * u;
+        const X r = in[t];
+        if (w0) w0[t] = r;
+        z[t] = gamma0[t] * r;
+    }
+}
+
+// Column-scaled off-diagonal values a_ec / d_(col e, c) in the matrix's value
+// layout (SELL-64 or CSR), rounded once from the FP64 product.
+template<class V>
+__global__ __launch_bounds__(256) void k_scale_columns(std::int64_t count, int b, int sell,
+                                                       const std::int32_t* __restrict__ cols,
+                                                       const double* __restrict__ v64, const double* __restrict__ d64,
+                                                       V* __restrict__ out)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count; t += step) {
+        std::int64_t slot;
+        int c;
+        if (sell) {
+            const std::int64_t run = t / (64 * b);
+            const std::int64_t rem = t - run * 64 * b;
+            c = static_cast<int>(rem / 64);
+            slot = run * 64 + (rem & 63);
+        } else {
+            slot = t / b;
+            c = static_cast<int>(t - slot * b);
+        }
+        const std::int64_t j = cols[slot];
+        const std::int64_t dj = sell ? (j & ~std::int64_t(63)) * b + 64 * c + (j & 63) : j * b + c;
+        const double a = v64[t] * (1.0 / d64[dj]);
+        if constexpr (sizeof(V) == 2) out[t] = dev::float_to_bf16(static_cast<float>(a));
+        else out[t] = static_cast<V>(a);
+    }
+}
+
+// gamma'_k = gamma_k / d, rounded once from the FP64 product, k = 0..K.
+template<class X>
+__global__ __launch_bounds__(256) void k_scale_gamma(std::int64_t nb, int b, int sell, int k1,
+                                                     const float* __restrict__ gamma, const double* __restrict__ d64,
+                                                     X* __restrict__ out)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nb; t += step) {
+        const std::int64_t i = t / b;
+        const int c = static_cast<int>(t - i * b);
+        const std::int64_t di = sell ? (i & ~std::int64_t(63)) * b + 64 * c + (i & 63) : t;
+        const double inv = 1.0 / d64[di];
+        for (int k = 0; k < k1; ++k)
+            out[k * nb + t] = static_cast<X>(static_cast<double>(gamma[k * nb + t]) * inv);
     }
 }
 
@@ -592,78 +640,69 @@ __global__ __launch_bounds__(256) void k_poly_init(std::int64_t nb, const X* __r
 
 namespace owtk::dev {
 
-// u_next = -D^-1 (A_off u), z += gamma_k * u_next.  The last step of an
-// apply does not store u_next (nothing reads it).
+// w_next = -(A_off D^-1) w, z += gamma'_k w_next.  The last step of an apply
+// does not store w_next (nothing reads it).
 template<class X>
-struct EpiPoly : EpiNone {
-    X* u_next;
-    const X* dinv;
-    const float* gamma;
+struct EpiSeries : EpiNone {
+    X* w_next;
+    const X* gamma;
     X* z;
-    bool store_u = true;
+    bool store_w = true;
     template<class A>
     __device__ void elem(std::int64_t i, A acc, double*) const
     {
-        const X u = -static_cast<X>(acc) * dinv[i];
-        if (store_u) u_next[i] = u;
-        z[i] = dfma(static_cast<X>(gamma[i]), u, z[i]);
+        const X w = -static_cast<X>(acc);
+        if (store_w) w_next[i] = w;
+        z[i] = dfma(gamma[i], w, z[i]);
     }
     template<int BS, class A>
     __device__ void elem_row(std::int64_t i0, const A (&acc)[BS], double*) const
     {
-        X d[BS], zz[BS], u[BS];
-        float g[BS];
-        load_row<BS>(dinv + i0, d);
+        X g[BS], zz[BS], w[BS];
         load_row<BS>(gamma + i0, g);
         load_row<BS>(z + i0, zz);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
-            u[c] = -static_cast<X>(acc[c]) * d[c];
-            zz[c] = dfma(static_cast<X>(g[c]), u[c], zz[c]);
+            w[c] = -static_cast<X>(acc[c]);
+            zz[c] = dfma(g[c], w[c], zz[c]);
         }
-        if (store_u) store_row<BS>(u_next + i0, u);
+        if (store_w) store_row<BS>(w_next + i0, w);
         store_row<BS>(z + i0, zz);
     }
 };
 
-// First step with the initialisation folded in: the operand u_0 = D^-1 r is
-// formed at each gathered index (x = r), and z = gamma_0 u_0 + gamma_1 u_1
-// is written without a prior z.  The same roundings as u_0 = r * dinv,
-// z = gamma_0 * u_0, z = fma(gamma_1, u_1, z).
+// First step with the initialisation folded in: w_0 = r is gathered as it is,
+// and z = gamma'_0 r + gamma'_1 w_1 is written without a prior z, with the
+// roundings of z = gamma'_0 * r, z = fma(gamma'_1, w_1, z) (k_series_init
+// followed by an EpiSeries step).
 template<class X>
-struct EpiPolyFirst : EpiNone {
-    X* u_next;
-    const X* dinv;
+struct EpiSeriesFirst : EpiNone {
+    X* w_next;
     const X* r;
-    const float* gamma0;
-    const float* gamma1;
+    const X* gamma0;
+    const X* gamma1;
     X* z;
-    bool store_u = true;
-    __device__ X xval(const X* x, std::int64_t i) const { return x[i] * dinv[i]; }
+    bool store_w = true;
     template<class A>
     __device__ void elem(std::int64_t i, A acc, double*) const
     {
-        const X d = dinv[i];
-        const X u = -static_cast<X>(acc) * d;
-        if (store_u) u_next[i] = u;
-        const X z0 = static_cast<X>(gamma0[i]) * (r[i] * d);
-        z[i] = dfma(static_cast<X>(gamma1[i]), u, z0);
+        const X w = -static_cast<X>(acc);
+        if (store_w) w_next[i] = w;
+        z[i] = dfma(gamma1[i], w, gamma0[i] * r[i]);
     }
     template<int BS, class A>
     __device__ void elem_row(std::int64_t i0, const A (&acc)[BS], double*) const
     {
-        X d[BS], rr[BS], u[BS], zz[BS];
-        float g0[BS], g1[BS];
-        load_row<BS>(dinv + i0, d);
+        X rr[BS], g0[BS], g1[BS], w[BS], zz[BS];
         load_row<BS>(r + i0, rr);
         load_row<BS>(gamma0 + i0, g0);
         load_row<BS>(gamma1 + i0, g1);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
-            u[c] = -static_cast<X>(acc[c]) * d[c];
-            zz[c] = dfma(static_cast<X>(g1[c]), u[c], static_cast<X>(g0[c]) * (rr[c] * d[c]));
+            w[c] = -static_cast<X>(acc[c]);
+            zz[c] = dfma(g1[c], w[c], g0[c] * rr[c]);
         }
-        if (store_u) store_row<BS>(u_next + i0, u);
+        if (store_w) store_row<BS>(w_next + i0, w);
         store_row<BS>(z + i0, zz);
     }
 };
@@ -916,38 +955,24 @@ void learned_setup(owtk_precond pc)
     if (!pc->learned) throw std::invalid_argument("learned preconditioner needs a model");
     auto* model = static_cast<owtk_gnn>(pc->learned);
     check_matrix(model, pc->m);
-    if (pc->storage == OWTK_F32 && !pc->m->v32.get()) throw std::invalid_argument("matrix has no FP32 storage");
-    if (pc->storage == OWTK_BF16 && !pc->m->v16.get()) throw std::invalid_argument("matrix has no BF16 storage");
     auto st = std::make_unique<LearnedPc>();
     st->model = model;
     const std::int64_t nb = pc->n();
+    const std::size_t k1 = static_cast<std::size_t>(model->degree + 1);
+    const std::size_t sv = pc->storage == OWTK_F64 ? 8 : (pc->storage == OWTK_F32 ? 4 : 2);
+    st->scaled.allocate(static_cast<std::size_t>(std::max<std::int64_t>(pc->m->value_count, 1)) * sv);
     if (pc->storage == OWTK_F64) {
         st->u64a.allocate(nb);
         st->u64b.allocate(nb);
-        st->dinv64.allocate(nb);
+        st->sgamma64.allocate(nb * k1);
     } else {
         st->u32a.allocate(nb);
         st->u32b.allocate(nb);
-        st->dinv32.allocate(nb);
+        st->sgamma32.allocate(nb * k1);
     }
-    st->gamma.allocate(nb * (model->degree + 1));
+    st->gamma.allocate(nb * k1);
     pc->learned_state = st.release();
 }
-
-namespace {
-__global__ __launch_bounds__(256) void k_inverse_diag(std::int64_t nb, const double* __restrict__ d64, int sell, int b,
-                                                      double* __restrict__ inv64, float* __restrict__ inv32)
-{
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nb) return;
-    const std::int64_t i = t / b;
-    const int c = static_cast<int>(t - i * b);
-    const std::int64_t di = sell ? (i & ~std::int64_t(63)) * b + 64 * c + (i & 63) : t;
-    const double inv = 1.0 / d64[di];
-    if (inv64) inv64[t] = inv;
-    if (inv32) inv32[t] = static_cast<float>(inv);
-}
-} // namespace
 
 void learned_numeric(owtk_precond pc)
 {
@@ -960,53 +985,77 @@ void learned_numeric(owtk_precond pc)
     DeviceArray<float> h(2 * g.n * W);
     forward(model, g, h.get(), nullptr, nullptr, st->gamma.get(), false);
     const std::int64_t nb = pc->n();
-    k_inverse_diag<<<rows_grid(nb), kBlock, 0, s>>>(nb, m->d64.get(), m->s.layout == Layout::sell ? 1 : 0,
-                                                    m->bs, st->dinv64.get(), st->dinv32.get());
+    const int sell = m->s.layout == Layout::sell ? 1 : 0;
+    const int k1 = model->degree + 1;
+    const std::int64_t count = m->value_count;
+    const unsigned int gv = dev::stream_grid(count, 4);
+    switch (pc->storage) {
+    case OWTK_F64:
+        k_scale_columns<double><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(), m->d64.get(),
+                                                      reinterpret_cast<double*>(st->scaled.get()));
+        k_scale_gamma<double><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
+                                                                         m->d64.get(), st->sgamma64.get());
+        break;
+    case OWTK_F32:
+        k_scale_columns<float><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(), m->d64.get(),
+                                                     reinterpret_cast<float*>(st->scaled.get()));
+        k_scale_gamma<float><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
+                                                                        m->d64.get(), st->sgamma32.get());
+        break;
+    default:
+        k_scale_columns<std::uint16_t><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(),
+                                                             m->d64.get(),
+                                                             reinterpret_cast<std::uint16_t*>(st->scaled.get()));
+        k_scale_gamma<float><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
+                                                                        m->d64.get(), st->sgamma32.get());
+        break;
+    }
     OWTK_LAUNCH_CHECK();
     OWTK_HIP(hipStreamSynchronize(s));  // graph and h are released on return
 }
 
 // Algorithmic bytes of one series step: off-diagonal values + columns + row
-// lengths, dinv, u gathered (once), gamma_k, z read and written, u_next
-// written.
+// lengths, w gathered (once), gamma'_k, z read and written, w_next written.
 std::int64_t learned_step_bytes(owtk_precond pc)
 {
     const std::int64_t nb = pc->n();
     const int sv = pc->storage == OWTK_F64 ? 8 : (pc->storage == OWTK_F32 ? 4 : 2);
     const int sx = pc->storage == OWTK_F64 ? 8 : 4;
-    return pc->m->entries * (pc->m->bs * sv + 4) + pc->m->owned * 4 + nb * (5 * sx + 4);
+    return pc->m->entries * (pc->m->bs * sv + 4) + pc->m->owned * 4 + nb * (5 * sx);
 }
 
-// First step (init folded in): matrix; r, dinv, gamma_0, gamma_1 read;
-// u_1 (unless it is also the last step) and z written.  Last step: a middle
-// step without the u_next write.
+// First step (init folded in): matrix; r, gamma'_0, gamma'_1 read; w_1
+// (unless it is also the last step) and z written.  Last step: a middle step
+// without the w_next write.
 std::int64_t learned_end_step_bytes(owtk_precond pc, bool first, bool last)
 {
     const std::int64_t nb = pc->n();
     const int sx = pc->storage == OWTK_F64 ? 8 : 4;
-    const std::int64_t matrix = learned_step_bytes(pc) - nb * (5 * sx + 4);
-    if (first) return matrix + nb * ((last ? 3 : 4) * sx + 8);
-    return matrix + nb * (4 * sx + 4);
+    const std::int64_t matrix = learned_step_bytes(pc) - nb * (5 * sx);
+    if (first) return matrix + nb * ((last ? 4 : 5) * sx);
+    return matrix + nb * (4 * sx);
 }
 
-// Apply: degree 0 is the init kernel alone; otherwise first, middle and last steps.
+// Apply: degree 0 is the init kernel alone (r, gamma'_0 read, z written);
+// otherwise first, middle and last steps.
 std::int64_t learned_apply_bytes(owtk_precond pc)
 {
     auto* st = static_cast<LearnedPc*>(pc->learned_state);
     const std::int64_t nb = pc->n();
     const int sx = pc->storage == OWTK_F64 ? 8 : 4;
     const std::int64_t K = st ? st->model->degree : 0;
-    if (K == 0) return nb * (4 * sx + 4);
+    if (K == 0) return nb * (3 * sx);
     if (K == 1) return learned_end_step_bytes(pc, true, true);
     return learned_end_step_bytes(pc, true, false) + (K - 2) * learned_step_bytes(pc)
          + learned_end_step_bytes(pc, false, true);
 }
 
-template<class V, class X, class Dv>
-void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const int* stop, X* ua, X* ub, const Dv* dinv)
+template<class V, class X>
+void learned_apply_t(owtk_precond pc, const X* in, X* out, const int* stop, X* wa, X* wb, const X* sgamma)
 {
     auto* st = static_cast<LearnedPc*>(pc->learned_state);
     owtk_matrix m = pc->m;
+    const V* vals = reinterpret_cast<const V*>(st->scaled.get());
     const std::int64_t nb = pc->n();
     const std::int64_t step_bytes = learned_step_bytes(pc);
     const int degree = st->model->degree;
@@ -1014,38 +1063,35 @@ void learned_apply_t(owtk_precond pc, const V* vals, const X* in, X* out, const 
     // apply takes the separate initialisation kernel instead.
     const bool fuse = degree > 0 && in != out;
     if (!fuse) {
-        k_poly_init<X, Dv><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, dinv, st->gamma.get(), ua,
-                                                                                     out, stop);
+        k_series_init<X><<<dev::stream_grid(nb, 4), kBlock, 0, m->ctx->stream>>>(nb, in, sgamma, degree > 0 ? wa : nullptr,
+                                                                                 out, stop);
         OWTK_LAUNCH_CHECK();
     }
-    static_assert(std::is_same_v<X, Dv>, "series steps read the inverse diagonal in the vector type");
     if (fuse) {
-        dev::EpiPolyFirst<X> e;
+        dev::EpiSeriesFirst<X> e;
         e.stop = stop;
-        e.u_next = ub;
-        e.dinv = dinv;
+        e.w_next = wb;
         e.r = in;
-        e.gamma0 = st->gamma.get();
-        e.gamma1 = st->gamma.get() + nb;
+        e.gamma0 = sgamma;
+        e.gamma1 = sgamma + nb;
         e.z = out;
-        e.store_u = degree > 1;
+        e.store_w = degree > 1;
         ProfileScope prof(m->ctx, prof_learned_ends, learned_end_step_bytes(pc, true, degree == 1));
-        offdiag_spmv<V, X, dev::EpiPolyFirst<X>>(m, vals, in, e);
-        std::swap(ua, ub);
+        offdiag_spmv<V, X, dev::EpiSeriesFirst<X>>(m, vals, in, e);
+        std::swap(wa, wb);
     }
     for (int k = fuse ? 2 : 1; k <= degree; ++k) {
-        dev::EpiPoly<X> e;
+        dev::EpiSeries<X> e;
         e.stop = stop;
-        e.u_next = ub;
-        e.dinv = dinv;
-        e.gamma = st->gamma.get() + static_cast<std::int64_t>(k) * nb;
+        e.w_next = wb;
+        e.gamma = sgamma + static_cast<std::int64_t>(k) * nb;
         e.z = out;
-        e.store_u = k < degree;
+        e.store_w = k < degree;
         // middle steps are the series-step kernel proper; the last step is an end step
         ProfileScope prof(m->ctx, k < degree ? prof_learned : prof_learned_ends,
                           k < degree ? step_bytes : learned_end_step_bytes(pc, false, true));
-        offdiag_spmv<V, X, dev::EpiPoly<X>>(m, vals, ua, e);
-        std::swap(ua, ub);
+        offdiag_spmv<V, X, dev::EpiSeries<X>>(m, vals, wa, e);
+        std::swap(wa, wb);
     }
 }
 
@@ -1053,19 +1099,16 @@ void learned_apply_f64(owtk_precond pc, const double* in, double* out, const int
 {
     auto* st = static_cast<LearnedPc*>(pc->learned_state);
     if (pc->storage != OWTK_F64) throw std::invalid_argument("FP64 apply needs F64 preconditioner storage");
-    learned_apply_t<double, double, double>(pc, pc->m->v64.get(), in, out, stop, st->u64a.get(), st->u64b.get(),
-                                            st->dinv64.get());
+    learned_apply_t<double, double>(pc, in, out, stop, st->u64a.get(), st->u64b.get(), st->sgamma64.get());
 }
 
 void learned_apply_f32(owtk_precond pc, const float* in, float* out, const int* stop)
 {
     auto* st = static_cast<LearnedPc*>(pc->learned_state);
     if (pc->storage == OWTK_F32)
-        learned_apply_t<float, float, float>(pc, pc->m->v32.get(), in, out, stop, st->u32a.get(), st->u32b.get(),
-                                             st->dinv32.get());
+        learned_apply_t<float, float>(pc, in, out, stop, st->u32a.get(), st->u32b.get(), st->sgamma32.get());
     else if (pc->storage == OWTK_BF16)
-        learned_apply_t<std::uint16_t, float, float>(pc, pc->m->v16.get(), in, out, stop, st->u32a.get(),
-                                                     st->u32b.get(), st->dinv32.get());
+        learned_apply_t<std::uint16_t, float>(pc, in, out, stop, st->u32a.get(), st->u32b.get(), st->sgamma32.get());
     else
         throw std::invalid_argument("FP32 apply needs F32 or BF16 preconditioner storage");
 }

@@ -59,12 +59,19 @@ struct owtk_gnn_problem_s {
 namespace owtk {
 
 // Preconditioner hooks (OWTK_PC_LEARNED).
+// The apply evaluates z = sum_k gamma_k u_k, u_0 = D^-1 r,
+// u_(k+1) = -D^-1 A_off u_k, through w_k = D u_k:
+//   w_0 = r,  w_(k+1) = -(A_off D^-1) w_k,  z = sum_k (gamma_k D^-1) w_k,
+// so no step reads the inverse diagonal.  The column-scaled off-diagonal
+// values (matrix slot layout, storage type) and the scaled coefficients are
+// formed once per numeric set-up.
 struct LearnedPc {
     owtk_gnn model = nullptr;
-    DeviceArray<float> gamma;      // [k][n*b]
-    DeviceArray<double> dinv64;
-    DeviceArray<float> dinv32;
-    DeviceArray<double> u64a, u64b;
+    DeviceArray<float> gamma;             // [k][n*b], the network's coefficients
+    DeviceArray<unsigned char> scaled;    // A_off D^-1 values, value_count entries
+    DeviceArray<double> sgamma64;         // [k][n*b] gamma_k D^-1 (FP64 storage)
+    DeviceArray<float> sgamma32;          // [k][n*b] gamma_k D^-1 (FP32 and BF16 storage)
+    DeviceArray<double> u64a, u64b;       // w_k work vectors
     DeviceArray<float> u32a, u32b;
 };
 
