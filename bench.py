@@ -149,6 +149,29 @@ def spmv_timing(ctx, m, n, reps=50):
     return out
 
 
+def stream_ceiling(ctx, reps=20):
+    """The on-box streaming rate of a plain vector kernel, beside the 8 TB/s
+    spec: owtk_axpy (y += a x) over two 512 MiB FP64 vectors (1 GiB read,
+    512 MiB written per launch; the working set exceeds the 256 MB MALL),
+    one HIP event pair around back-to-back launches."""
+    count = 64 << 20
+    x = ctx.upload(np.full(count, 1e-3))
+    y = ctx.upload(np.zeros(count))
+    for _ in range(3):
+        ctx.axpy(1.0, x, y)
+    ctx.synchronize()
+    ctx.region_begin()
+    for _ in range(reps):
+        ctx.axpy(1.0, x, y)
+    ms = ctx.region_end() / reps
+    nbytes = 3 * count * 8
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    del x, y
+    return {"kernel": "owtk_axpy, y += a x over 2 x 512 MiB FP64", "bytes_per_launch": nbytes,
+            "avg_ms": round(ms, 4), "gbps": round(gbps, 1), "frac_of_peak": round(gbps / PEAK_HBM_GBPS, 4),
+            "note": "measured streaming rate on this box; roofline fractions use the 8.0 TB/s spec"}
+
+
 def learned_timing(ctx, pc, n, reps=30):
     """The learned preconditioner applied back to back outside a solver (no
     stopped launches): per-launch HIP-event time of its middle series-step
@@ -676,6 +699,8 @@ def main():
 
         # split SpMV: the path's data-parallel hot kernel and the metric's "SpMV HBM GB/s"
         sp = spmv_timing(ctx, m, n)
+        stream = stream_ceiling(ctx)
+        log(f"[bench] stream ceiling: {stream}")
         log(f"[bench] spmv f64 {sp['f64']['avg_ms'] * 1e3:.1f} us {sp['f64']['gbps']:.0f} GB/s, "
             f"f32 {sp['f32']['avg_ms'] * 1e3:.1f} us {sp['f32']['gbps']:.0f} GB/s, "
             f"bf16 {sp['bf16']['avg_ms'] * 1e3:.1f} us {sp['bf16']['gbps']:.0f} GB/s")
@@ -731,7 +756,7 @@ def main():
             "fp64_iterations": res64[-1]["iterations"], "fp64_true_rel": rel64,
             "learned_mixed": head, "series_mixed_no_network": noet, "ilu0_mixed": milu,
             "setup_s": {k: round(v, 4) for k, v in setup.items()},
-            "spmv": sp, "spmv_roofline": spmv_roof, "roofline": roof, "cpu_baseline": cpu,
+            "spmv": sp, "spmv_roofline": spmv_roof, "hbm_stream": stream, "roofline": roof, "cpu_baseline": cpu,
             "kernels": {"fp64": prof64, "learned_mixed": profh},
         }
         print(json.dumps(out))

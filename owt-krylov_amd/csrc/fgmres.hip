@@ -516,6 +516,7 @@ template<class T>
 struct FgEngine {
     owtk_context ctx;
     std::shared_ptr<EngineWork> work;  // the context's workspace for T
+    bool may_reorth = true;             // Gram-Schmidt modes 0 and 1 can re-orthogonalise
     std::int64_t n = 0;       // owned size
     std::int64_t local = 0;   // local size (SpMV inputs)
     std::function<void(const T*, T*, const int*)> op;
@@ -532,6 +533,7 @@ struct FgEngine {
         const auto scratch = ctx->scratch();
         const int m = static_cast<int>(o->restart);
         const std::int64_t ld = local;
+        may_reorth = o->gmres_orthogonalization == 0 || o->gmres_orthogonalization == 1;
         if (!work) work = engine_work(ctx, sizeof(T));
         const std::size_t need = static_cast<std::size_t>((2 * m + 3) * ld) * sizeof(T);
         if (work->vecs.bytes() < need || work->ld != ld) {
@@ -706,8 +708,9 @@ struct FgEngine {
         k_fg_pass_a<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, n, s, k, scratch);
         dev::launch_finish<K>(st, FinA<K>{s, k, count}, gr, 0, scratch, halt);
         k_fg_pass_b<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
-        // FinB only acts after a re-orthogonalisation
-        dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt, &s->do_reorth);
+        // FinB only acts after a re-orthogonalisation, which the single-pass
+        // mode never does
+        if (may_reorth) dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt, &s->do_reorth);
         k_fg_pass_c<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
         dev::launch_finish<2>(st, FinC<T>{s, k, w, n}, gr, 2, scratch, halt);
         k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);

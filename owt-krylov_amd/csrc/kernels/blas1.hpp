@@ -136,8 +136,54 @@ __global__ __launch_bounds__(256) void k_axpy(std::int64_t n, double alpha, cons
         y[i] = __builtin_fma(alpha, x[i], y[i]);
 }
 
+// 16-byte form: each thread moves 4 pairs per trip with every load issued
+// before the first store (full trips run without bounds tests; the tail
+// takes one pair per thread).  The per-element arithmetic is k_axpy's.
+typedef double double2_v __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_axpy2(std::int64_t n2, double alpha, const double2_v* __restrict__ x,
+                                               double2_v* __restrict__ y)
+{
+    constexpr int U = 4;
+    const std::int64_t T = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * T < n2; i += U * T) {
+        double2_v xv[U], yv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u] = __builtin_nontemporal_load(x + i + u * T);
+            yv[u] = y[i + u * T];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double2_v r;
+            r.x = __builtin_fma(alpha, xv[u].x, yv[u].x);
+            r.y = __builtin_fma(alpha, xv[u].y, yv[u].y);
+            y[i + u * T] = r;
+        }
+    }
+    for (; i < n2; i += T) {
+        const double2_v xv = x[i];
+        double2_v r = y[i];
+        r.x = __builtin_fma(alpha, xv.x, r.x);
+        r.y = __builtin_fma(alpha, xv.y, r.y);
+        y[i] = r;
+    }
+}
+
 inline void axpy(hipStream_t st, std::int64_t n, double alpha, const double* x, double* y)
 {
+    const bool vec = (reinterpret_cast<std::uintptr_t>(x) | reinterpret_cast<std::uintptr_t>(y)) % 16 == 0;
+    if (vec && n >= 2) {
+        const std::int64_t n2 = n / 2;
+        k_axpy2<<<stream_grid(n2, 8), kBlock, 0, st>>>(n2, alpha, reinterpret_cast<const double2_v*>(x),
+                                                       reinterpret_cast<double2_v*>(y));
+        OWTK_LAUNCH_CHECK();
+        if (n % 2) {
+            k_axpy<<<1, kBlock, 0, st>>>(1, alpha, x + n - 1, y + n - 1);
+            OWTK_LAUNCH_CHECK();
+        }
+        return;
+    }
     k_axpy<<<stream_grid(n, 4), kBlock, 0, st>>>(n, alpha, x, y);
     OWTK_LAUNCH_CHECK();
 }
