@@ -208,3 +208,31 @@ def test_fallback_to_ilu_when_learned_fails(ctx, lognormal):
     assert res["status_name"] == "converged"
     assert res["fallback_refinements"] >= 1
     assert rel <= 1.5e-9
+
+
+@pytest.mark.parametrize("storage,tol", [(owtk.F64, 1e-13), (owtk.F32, 2e-5), (owtk.BF16, 2e-2)])
+def test_learned_apply_follows_value_update(ctx, storage, tol):
+    """After new matrix values and update_values(), the apply (column-scaled
+    values and scaled coefficients rebuilt) matches the oracle on the new
+    values."""
+    s = system(seed=11)
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    if storage != owtk.F64:
+        m.add_storage(storage)
+    degree = 3
+    model = owtk.GnnModel(ctx, 3, degree)
+    model.set_parameters(perturbed_parameters(3, degree, 5))
+    pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=storage)
+    diag2, off2 = s.diagonal * 1.3, s.off_diagonal * 0.8
+    m.update_values(diag2, off2)
+    pc.update_values()
+    g = gnn.Graph(s.nodes, s.block, s.offsets, s.columns, diag2, off2)
+    gam = model.coefficients(m).astype(np.float64)
+    r = np.random.default_rng(12).uniform(-1, 1, s.nodes * 3)
+    ref = gnn.apply(g, gam, r.reshape(-1, 3)).ravel()
+    dt = np.float64 if storage == owtk.F64 else np.float32
+    x, y = ctx.upload(r.astype(dt)), ctx.empty(r.size, dt)
+    pc.apply(x, y)
+    err = np.abs(y.download() - ref).max() / np.abs(ref).max()
+    print(f"storage {storage}: relative error after the value update {err:.2e}")
+    assert err < tol

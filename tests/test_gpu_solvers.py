@@ -417,3 +417,36 @@ def test_cold_front_golden_fgmres(ctx):
     true = np.linalg.norm(rhs - om.spmv(x.download()))
     print("refined", rr["status_name"], rr["refinement_steps"], rr["inner_iterations"], true)
     assert rr["status_name"] == "converged" and true <= atol
+
+
+def _fgmres_on(ctx, s, refined):
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    bd, xd = ctx.upload(s.rhs), ctx.empty(m.local_size)
+    xd.zero()
+    if refined:
+        m.add_storage(owtk.F32)
+        pc = owtk.Preconditioner(m, owtk.PC_JACOBI, storage=owtk.F32)
+        res = owtk.fgmres_refined(m, pc, bd, xd, owtk.refinement_options(relative_tolerance=1e-9))
+    else:
+        pc = owtk.Preconditioner(m, owtk.PC_JACOBI)
+        res = owtk.fgmres(m, pc, bd, xd, owtk.SolverOptions(relative_tolerance=1e-9, restart=20))
+    return xd.download(), res["iterations"]
+
+
+@pytest.mark.parametrize("refined", [False, True])
+def test_fgmres_workspace_reuse_across_sizes(ctx, refined):
+    """The context keeps FGMRES and refinement workspaces between solves and
+    re-zeroes them when the vector layout changes: alternating systems of
+    different sizes on one context give bit-for-bit the results of fresh
+    contexts."""
+    small = owtk.generate_split(40, 24, 3, 3, ordering=owtk.ORDER_HILBERT)
+    large = owtk.generate_split(64, 40, 3, 4, ordering=owtk.ORDER_HILBERT)
+    fresh = {}
+    for name, s in (("small", small), ("large", large)):
+        with owtk.Context(0) as c:
+            fresh[name] = _fgmres_on(c, s, refined)
+    for name, s in (("large", large), ("small", small), ("large", large), ("small", small)):
+        x, its = _fgmres_on(ctx, s, refined)
+        print(name, "iterations", its, "fresh", fresh[name][1], flush=True)
+        assert its == fresh[name][1]
+        assert np.array_equal(x, fresh[name][0])
