@@ -485,7 +485,7 @@ struct Counters {
 // the pinned flags and events of the throttled launch loop.
 struct EngineWork {
     DeviceArray<unsigned char> vecs;
-    std::int64_t ld = -1;  // vector stride the zeroed ghost entries fit
+    std::int64_t ld = -1, owned = -1;  // vector stride and owned length the zeroed ghosts fit
     DeviceArray<FgState> dstate;
     PinnedBuffer flags;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -536,12 +536,13 @@ struct FgEngine {
         may_reorth = o->gmres_orthogonalization == 0 || o->gmres_orthogonalization == 1;
         if (!work) work = engine_work(ctx, sizeof(T));
         const std::size_t need = static_cast<std::size_t>((2 * m + 3) * ld) * sizeof(T);
-        if (work->vecs.bytes() < need || work->ld != ld) {
+        if (work->vecs.bytes() < need || work->ld != ld || work->owned != n) {
             // ghost entries of the basis are never written, so after one zero
-            // fill they stay zero for every later run with the same stride
+            // fill they stay zero for every later run with the same layout
             if (work->vecs.bytes() < need) work->vecs.allocate(need);
             kern::fill_zero_bytes(st, work->vecs.get(), work->vecs.bytes());
             work->ld = ld;
+            work->owned = n;
         }
         if (!work->dstate.get()) work->dstate.allocate(1);
         if (!work->ev[0]) {
@@ -843,10 +844,11 @@ int fgmres_refined_run(owtk_matrix m, owtk_precond pc, owtk_precond fallback, co
     const std::size_t r_bytes = round256(static_cast<std::size_t>(L) * 8);
     const std::size_t f_bytes = round256(static_cast<std::size_t>(L) * 4);
     const std::size_t ws_bytes = r_bytes + 2 * f_bytes + (fallback ? round256(static_cast<std::size_t>(n) * 8) : 0);
-    if (ctx->refine_ws.bytes() < ws_bytes || ctx->refine_layout != L) {
+    if (ctx->refine_ws.bytes() < ws_bytes || ctx->refine_layout != L || ctx->refine_owned != n) {
         if (ctx->refine_ws.bytes() < ws_bytes) ctx->refine_ws.allocate(ws_bytes);
         kern::fill_zero_bytes(st, ctx->refine_ws.get(), ctx->refine_ws.bytes());
         ctx->refine_layout = L;
+        ctx->refine_owned = n;
     }
     unsigned char* ws = ctx->refine_ws.get();
     double* r = reinterpret_cast<double*>(ws);

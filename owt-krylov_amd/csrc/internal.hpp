@@ -123,9 +123,10 @@ struct owtk_context_s {
     // Type-erased because the workspace type is private to the FGMRES code.
     std::shared_ptr<void> fg_work[2];
     // FP64 iterative-refinement vectors, kept between solves for the same
-    // reason; refine_layout is the vector length their zeroed ghosts fit.
+    // reason; refine_layout and refine_owned are the local and owned lengths
+    // their zeroed ghosts fit.
     owtk::DeviceArray<unsigned char> refine_ws;
-    std::int64_t refine_layout = -1;
+    std::int64_t refine_layout = -1, refine_owned = -1;
 
     owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
     void activate() const;  // hipSetDevice(device)
