@@ -45,6 +45,13 @@ struct owtk_multigrid_s {
     owtk::DeviceArray<double> cval;
     std::vector<std::int64_t> gs_level_ptr;  // red levels, then black levels
     std::int64_t red_levels = 0;
+    // the same rows in sweep (level) order as ELL: position p's k-th
+    // off-diagonal entry (CSR order) at k * positions + p, its values at
+    // (k * positions + p) * bs + c; padding slots hold the row itself and 0
+    owtk::DeviceArray<std::int32_t> gs_len, gs_col;
+    owtk::DeviceArray<double> gs_val, gs_diag;
+    std::int64_t gs_positions = 0;
+    int gs_k = 0;
     ~owtk_multigrid_s()
     {
         if (coarse) (void)owtk_matrix_destroy(coarse);
@@ -110,33 +117,69 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const double* __restrict__ c
 
 // One colour's Gauss-Seidel sweep, all levels in one workgroup: the rows of
 // a level in parallel, a barrier between levels (a level's reads of earlier
-// levels' writes stay inside the workgroup).
-__global__ __launch_bounds__(1024) void k_mg_gs_sweep(const std::int32_t* __restrict__ rows,
-                                                      const std::int64_t* __restrict__ level_ptr, std::int64_t first,
-                                                      std::int64_t last, int bs,
-                                                      const std::int32_t* __restrict__ off,
-                                                      const std::int32_t* __restrict__ col,
-                                                      const double* __restrict__ val, const double* __restrict__ rhs,
-                                                      double* x)
+// levels' writes stay inside the workgroup).  The rows come from a
+// level-ordered ELL copy: a work item's row, row length, diagonal and
+// right-hand side, and then eight of its slots at a time, load with no
+// dependence on one another (padding slots read the row's
+// own column, and their products are dropped by a select), so a level costs
+// one round trip for its structure and one for the gathered solution values
+// instead of a pointer chase through rows, offsets and columns.  The
+// arithmetic and its order are the CSR sweep's.
+struct GsArgs {
+    const std::int32_t* rows;
+    const std::int32_t* len;
+    const std::int32_t* col;
+    const double* val;
+    const double* diag;
+    const double* rhs;
+    std::int64_t positions;
+    int bs, K;
+};
+
+// One colour's Gauss-Seidel sweep, all levels in one workgroup: the rows of
+// a level in parallel, a barrier between levels (a level's reads of earlier
+// levels' writes stay inside the workgroup).  The rows come from a
+// level-ordered ELL copy: a work item's row, row length, diagonal and
+// right-hand side, and then eight of its slots at a time, load with no
+// dependence on one another (padding slots name the row itself, and their
+// products are dropped by a select), so a level costs one round trip for its
+// structure and one for the gathered solution values instead of a pointer
+// chase through rows, offsets and columns.  The arithmetic and its order are
+// the CSR sweep's.  (Loading the next level's structure during the current
+// one did not help: vector loads retire in issue order, so the gathers then
+// waited for the prefetch as well.)
+__global__ __launch_bounds__(1024) void k_mg_gs_ell(const std::int64_t* __restrict__ level_ptr, std::int64_t first,
+                                                    std::int64_t last, GsArgs g, double* x)
 {
+    constexpr int C = 8;
     for (std::int64_t l = first; l < last; ++l) {
         const std::int64_t a = level_ptr[l];
-        const std::int64_t total = (level_ptr[l + 1] - a) * bs;
+        const std::int64_t total = (level_ptr[l + 1] - a) * g.bs;
         for (std::int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-            const std::int64_t r = rows[a + t / bs];
-            const int k = static_cast<int>(t % bs);
-            double rr = rhs[r * bs + k];
-            double diag = 0.0;
-            for (std::int32_t e = off[r]; e < off[r + 1]; ++e) {
-                const std::int64_t c = col[e];
-                const double coef = val[static_cast<std::int64_t>(e) * bs + k];
-                if (c == r) {
-                    diag = coef;
-                    continue;
+            const std::int64_t p = a + t / g.bs;
+            const int k = static_cast<int>(t % g.bs);
+            const std::int64_t r = g.rows[p];
+            const int n = g.len[p];
+            const double d = g.diag[p * g.bs + k];
+            double rr = g.rhs[r * g.bs + k];
+            for (int k0 = 0; k0 < g.K; k0 += C) {
+                std::int64_t cj[C];
+                double v[C], xv[C];
+#pragma unroll
+                for (int u = 0; u < C; ++u) {
+                    const std::int64_t slot = static_cast<std::int64_t>(k0 + u < g.K ? k0 + u : g.K - 1) * g.positions + p;
+                    cj[u] = g.col[slot];
+                    v[u] = g.val[slot * g.bs + k];
                 }
-                rr = rr - coef * x[c * bs + k];
+#pragma unroll
+                for (int u = 0; u < C; ++u) xv[u] = x[cj[u] * g.bs + k];
+#pragma unroll
+                for (int u = 0; u < C; ++u) {
+                    const double f = rr - v[u] * xv[u];
+                    rr = k0 + u < n ? f : rr;
+                }
             }
-            x[r * bs + k] = rr / diag;
+            x[r * g.bs + k] = rr / d;
         }
         __syncthreads();
     }
@@ -318,6 +361,41 @@ void mg_setup(owtk_multigrid_s* mg)
     upload(mg->coff, mg->h_coff);
     upload(mg->ccol, mg->h_ccol);
     upload(mg->cval, mg->h_cval);
+    // level-ordered ELL copy of the coarse rows for the sweep kernel
+    const std::int64_t P = static_cast<std::int64_t>(gs_rows.size());
+    int K = 0;
+    for (const std::int32_t r : gs_rows) {
+        int off = 0;
+        for (std::int32_t e = mg->h_coff[r]; e < mg->h_coff[r + 1]; ++e) off += mg->h_ccol[e] != r;
+        K = std::max(K, off);
+    }
+    std::vector<std::int32_t> glen(static_cast<std::size_t>(std::max<std::int64_t>(P, 1)), 0);
+    std::vector<std::int32_t> gcol(static_cast<std::size_t>(std::max<std::int64_t>(P * K, 1)), 0);
+    std::vector<double> gval(static_cast<std::size_t>(std::max<std::int64_t>(P * K * bs, 1)), 0.0);
+    std::vector<double> gdiag(static_cast<std::size_t>(std::max<std::int64_t>(P * bs, 1)), 0.0);
+    for (std::int64_t p = 0; p < P; ++p) {
+        const std::int32_t r = gs_rows[p];
+        int k = 0;
+        for (int q = 0; q < K; ++q) gcol[static_cast<std::size_t>(q * P + p)] = r;
+        for (std::int32_t e = mg->h_coff[r]; e < mg->h_coff[r + 1]; ++e) {
+            const std::int32_t c = mg->h_ccol[e];
+            if (c == r) {
+                for (int j = 0; j < bs; ++j) gdiag[static_cast<std::size_t>(p * bs + j)] = mg->h_cval[static_cast<std::size_t>(e) * bs + j];
+                continue;
+            }
+            gcol[static_cast<std::size_t>(k * P + p)] = c;
+            for (int j = 0; j < bs; ++j)
+                gval[static_cast<std::size_t>((k * P + p) * bs + j)] = mg->h_cval[static_cast<std::size_t>(e) * bs + j];
+            ++k;
+        }
+        glen[static_cast<std::size_t>(p)] = k;
+    }
+    mg->gs_positions = P;
+    mg->gs_k = K;
+    upload(mg->gs_len, glen);
+    upload(mg->gs_col, gcol);
+    upload(mg->gs_val, gval);
+    upload(mg->gs_diag, gdiag);
 }
 
 double smoother_weight(const owtk_multigrid_options& o, std::uint64_t step)
@@ -412,10 +490,10 @@ int mg_solve(owtk_multigrid_s* mg, const double* b, double* x, const owtk_solver
             if (opt.smoother == OWTK_MG_RED_BLACK) {
                 // red levels, then black levels (red rows read the previous black values)
                 const std::int64_t levels = static_cast<std::int64_t>(mg->gs_level_ptr.size()) - 1;
-                k_mg_gs_sweep<<<1, 1024, 0, st>>>(mg->gs_rows.get(), mg->gs_levels.get(), 0, mg->red_levels, bs,
-                                                   mg->coff.get(), mg->ccol.get(), mg->cval.get(), crhs, csol);
-                k_mg_gs_sweep<<<1, 1024, 0, st>>>(mg->gs_rows.get(), mg->gs_levels.get(), mg->red_levels, levels,
-                                                   bs, mg->coff.get(), mg->ccol.get(), mg->cval.get(), crhs, csol);
+                const GsArgs ga{mg->gs_rows.get(), mg->gs_len.get(), mg->gs_col.get(), mg->gs_val.get(),
+                                mg->gs_diag.get(), crhs, mg->gs_positions, bs, mg->gs_k};
+                k_mg_gs_ell<<<1, 1024, 0, st>>>(mg->gs_levels.get(), 0, mg->red_levels, ga, csol);
+                k_mg_gs_ell<<<1, 1024, 0, st>>>(mg->gs_levels.get(), mg->red_levels, levels, ga, csol);
             } else {
                 dev::EpiStore<double> e;
                 e.y = capplied;
