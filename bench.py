@@ -621,8 +621,8 @@ def main():
                     default="headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the solve shards by vertex partition; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a multi-GPU solve would shard by "
+              "vertex partition, with a ghost halo per SpMV and one allreduce per scalar batch)", file=sys.stderr)
         return 2
     if args.kernel:
         kernel_loop(args.kernel, args.steps, args.pc_storage)
