@@ -116,6 +116,24 @@ __device__ inline Packet<T, L> load_packet(const T* p)
     return *reinterpret_cast<const Packet<T, L>*>(p);
 }
 
+// Basis vectors are streamed once per pass: read them with the non-temporal
+// hint so they do not displace what the preconditioner and the operator
+// reread every iteration.
+template<class T, int L>
+__device__ inline Packet<T, L> load_packet_nt(const T* p)
+{
+    Packet<T, L> out;
+    if constexpr (L == 1) {
+        out.v[0] = __builtin_nontemporal_load(p);
+    } else {
+        typedef T vec __attribute__((ext_vector_type(L)));
+        const vec v = __builtin_nontemporal_load(reinterpret_cast<const vec*>(p));
+#pragma unroll
+        for (int l = 0; l < L; ++l) out.v[l] = v[l];
+    }
+    return out;
+}
+
 template<class T, int L>
 __device__ inline void store_packet(T* p, const Packet<T, L>& x)
 {
@@ -140,7 +158,7 @@ __global__ __launch_bounds__(256) void k_fg_pass_a(const T* __restrict__ V, std:
 #pragma unroll
         for (int j = 0; j < K - 1; ++j) {
             if (j < count) {
-                const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+                const Packet<T, L> vp = load_packet_nt<T, L>(V + j * ld + i);
 #pragma unroll
                 for (int l = 0; l < L; ++l)
                     red[j] = __builtin_fma(static_cast<double>(wp.v[l]), static_cast<double>(vp.v[l]), red[j]);
@@ -213,7 +231,7 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (j < count) {
-                const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+                const Packet<T, L> vp = load_packet_nt<T, L>(V + j * ld + i);
 #pragma unroll
                 for (int l = 0; l < L; ++l) wi[l] = __builtin_fma(-h[j], static_cast<double>(vp.v[l]), wi[l]);
             }
@@ -233,7 +251,7 @@ __global__ __launch_bounds__(256) void k_fg_pass_b(const T* __restrict__ V, std:
 #pragma unroll
             for (int j = 0; j < K - 1; ++j) {
                 if (j < count) {
-                    const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+                    const Packet<T, L> vp = load_packet_nt<T, L>(V + j * ld + i);
 #pragma unroll
                     for (int l = 0; l < L; ++l)
                         red[j] = __builtin_fma(static_cast<double>(wt.v[l]), static_cast<double>(vp.v[l]), red[j]);
@@ -298,7 +316,7 @@ __global__ __launch_bounds__(256) void k_fg_pass_c(const T* __restrict__ V, std:
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (j < count) {
-                    const Packet<T, L> vp = load_packet<T, L>(V + j * ld + i);
+                    const Packet<T, L> vp = load_packet_nt<T, L>(V + j * ld + i);
 #pragma unroll
                     for (int l = 0; l < L; ++l) wi[l] = __builtin_fma(-c[j], static_cast<double>(vp.v[l]), wi[l]);
                 }
@@ -405,7 +423,7 @@ __global__ __launch_bounds__(256) void k_fg_update(T* __restrict__ x, const T* _
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
         double sum = 0.0, comp = 0.0;
         for (int j = 0; j < count; ++j) {
-            const double term = coef.y[j] * static_cast<double>(Z[j * ld + i]);
+            const double term = coef.y[j] * static_cast<double>(__builtin_nontemporal_load(&Z[j * ld + i]));
             const double updated = sum + term;
             comp += __builtin_fabs(sum) >= __builtin_fabs(term) ? (sum - updated) + term : (term - updated) + sum;
             sum = updated;
