@@ -55,6 +55,9 @@ __device__ inline void store_row(X* __restrict__ p, const X (&v)[BS])
 struct EpiNone {
     static constexpr int K = 0;
     static constexpr std::uint64_t kMaxMask = 0;
+    // The matrix is streamed with the non-temporal hint unless an epilogue
+    // belongs to a kernel that rereads the same matrix right away.
+    static constexpr bool kKeepMatrix = false;
     const int* stop = nullptr;
     __device__ bool stopped() const { return stop != nullptr && *stop != 0; }
     __device__ void finalize(const double*) const {}
@@ -261,7 +264,8 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
 #pragma unroll
             for (int k = 0; k < U; ++k) {
                 const int kk = k < width ? k : width - 1;
-                colk[k] = __builtin_nontemporal_load(&cols[base + 64 * kk + lane]);
+                if constexpr (E::kKeepMatrix) colk[k] = cols[base + 64 * kk + lane];
+                else colk[k] = __builtin_nontemporal_load(&cols[base + 64 * kk + lane]);
             }
             A v[U][BS], xv[U][BS];
 #pragma unroll
@@ -270,7 +274,8 @@ __global__ __launch_bounds__(256) void spmv_sell_cd(const std::int32_t* __restri
                 const V* vk = vals + (base + 64 * kk) * BS + lane;
 #pragma unroll
                 for (int c = 0; c < BS; ++c) {
-                    v[k][c] = static_cast<A>(load_value_nt(vk, 64 * c));
+                    if constexpr (E::kKeepMatrix) v[k][c] = static_cast<A>(load_value(vk, 64 * c));
+                    else v[k][c] = static_cast<A>(load_value_nt(vk, 64 * c));
                     xv[k][c] = static_cast<A>(e.xval(x, static_cast<std::int64_t>(colk[k]) * BS + c));
                 }
             }

@@ -644,6 +644,7 @@ namespace owtk::dev {
 // does not store w_next (nothing reads it).
 template<class X>
 struct EpiSeries : EpiNone {
+    static constexpr bool kKeepMatrix = true;  // the apply's next step rereads it
     X* w_next;
     const X* gamma;
     X* z;
@@ -677,6 +678,7 @@ struct EpiSeries : EpiNone {
 // followed by an EpiSeries step).
 template<class X>
 struct EpiSeriesFirst : EpiNone {
+    static constexpr bool kKeepMatrix = true;
     X* w_next;
     const X* r;
     const X* gamma0;
