@@ -41,7 +41,7 @@ MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
 # kernel's known input; WRITE_SIZE by 1), c1 workload.
-TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 261_230_592}
+TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 261_427_405}
 KERNEL_NAMES = {
     "learned_bf16": "learned series step, BF16 column-scaled off-diagonal values: SpMV + fused update "
                     "(spmv_sell_cd<3,bf16,float,false,EpiSeries>)",
