@@ -411,7 +411,10 @@ int owtk_read_state(const char* const* paths, int32_t files, int64_t ns, int64_t
  * of the preconditioner
  *     u_0 = D^-1 r,  u_(k+1) = -D^-1 A_off u_k,  z = sum_k gamma_k u_k
  * (OWTK_PC_LEARNED with params.learned_model = the model; storage selects the
- * matrix values used by the apply).  A fresh model has a zero network and
+ * precision of the preconditioner's own copy of the off-diagonal values,
+ * built from the FP64 values at set-up and at owtk_precond_update_values).
+ * The apply evaluates the same series through w_k = D u_k over column-scaled
+ * values, so no step reads D^-1.  A fresh model has a zero network and
  * gamma = 1: the plain truncated Neumann series.
  * Training: each problem holds a split matrix and P (residual, error) pairs;
  * the loss is (1/P) sum_p |D^-1/2 A (M r_p - e_p)|^2 / |D^-1/2 A e_p|^2.
