@@ -422,11 +422,24 @@ __global__ __launch_bounds__(256) void k_fg_update(T* __restrict__ x, const T* _
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
         double sum = 0.0, comp = 0.0;
-        for (int j = 0; j < count; ++j) {
-            const double term = coef.y[j] * static_cast<double>(__builtin_nontemporal_load(&Z[j * ld + i]));
-            const double updated = sum + term;
-            comp += __builtin_fabs(sum) >= __builtin_fabs(term) ? (sum - updated) + term : (term - updated) + sum;
-            sum = updated;
+        // eight basis values loaded before the sequential compensated sums
+        // (an index past count re-reads the last column and is not summed)
+        constexpr int C = 8;
+        for (int j0 = 0; j0 < count; j0 += C) {
+            T zv[C];
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                const std::int64_t jj = j0 + u < count ? j0 + u : count - 1;
+                zv[u] = __builtin_nontemporal_load(&Z[jj * ld + i]);
+            }
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                if (j0 + u >= count) break;
+                const double term = coef.y[j0 + u] * static_cast<double>(zv[u]);
+                const double updated = sum + term;
+                comp += __builtin_fabs(sum) >= __builtin_fabs(term) ? (sum - updated) + term : (term - updated) + sum;
+                sum = updated;
+            }
         }
         x[i] = static_cast<T>(static_cast<double>(x[i]) + (sum + comp));
     }
