@@ -116,14 +116,18 @@ __device__ inline Packet<T, L> load_packet(const T* p)
     return *reinterpret_cast<const Packet<T, L>*>(p);
 }
 
-// Basis vectors are streamed once per pass: read them with the non-temporal
-// hint so they do not displace what the preconditioner and the operator
-// reread every iteration.
+// Basis vectors are streamed once per pass: FP32 bases are read with the
+// non-temporal hint so they do not displace what the preconditioner and the
+// operator reread every iteration (the learned preconditioner's matrix stays
+// in the MALL).  FP64 bases are read normally: measured, the hint made the
+// FP64 FGMRES + ILU(0) solve 3% slower.
 template<class T, int L>
 __device__ inline Packet<T, L> load_packet_nt(const T* p)
 {
     Packet<T, L> out;
-    if constexpr (L == 1) {
+    if constexpr (sizeof(T) == 8) {
+        out = load_packet<T, L>(p);
+    } else if constexpr (L == 1) {
         out.v[0] = __builtin_nontemporal_load(p);
     } else {
         typedef T vec __attribute__((ext_vector_type(L)));
@@ -430,7 +434,8 @@ __global__ __launch_bounds__(256) void k_fg_update(T* __restrict__ x, const T* _
 #pragma unroll
             for (int u = 0; u < C; ++u) {
                 const std::int64_t jj = j0 + u < count ? j0 + u : count - 1;
-                zv[u] = __builtin_nontemporal_load(&Z[jj * ld + i]);
+                if constexpr (sizeof(T) == 8) zv[u] = Z[jj * ld + i];
+                else zv[u] = __builtin_nontemporal_load(&Z[jj * ld + i]);
             }
 #pragma unroll
             for (int u = 0; u < C; ++u) {
