@@ -243,14 +243,37 @@ __global__ __launch_bounds__(kMaxTile) void kb_update_p(Ptrs P, const double* __
     const bool first = st.first != 0;
     const bool restart = st.restart != 0;
     const double beta = st.beta, omega = st.omega;
+    // All four inputs load before any branch (p and v hold zeros from the
+    // initial kernel on a first iteration), so the row's loads overlap
+    // instead of waiting for each other around the restart/first branches.
+    const std::int64_t i0 = rr.vertex * BS;
+    double rv[BS], pv[BS], vv[BS], dv[BS];
 #pragma unroll
     for (int c = 0; c < BS; ++c) {
-        const std::int64_t i = rr.vertex * BS + c;
-        const double ri = r[i];
-        if (restart) rhat[i] = ri;
-        const double pi = first ? ri : ri + beta * (p[i] - omega * v[i]);
-        p[i] = pi;
-        phat[i] = P.dinv[i] * pi;
+        rv[c] = r[i0 + c];
+        pv[c] = p[i0 + c];
+        vv[c] = v[i0 + c];
+        dv[c] = P.dinv[i0 + c];
+    }
+    // Without this use the compiler sinks part of the p and v loads into the
+    // uniform not-first branch, a second round trip per row.
+#pragma unroll
+    for (int c = 0; c < BS; ++c) asm volatile("" ::"v"(pv[c]), "v"(vv[c]));
+    double pn[BS], ph[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+        const double upd = rv[c] + beta * (pv[c] - omega * vv[c]);
+        pn[c] = first ? rv[c] : upd;
+        ph[c] = dv[c] * pn[c];
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+        p[i0 + c] = pn[c];
+        phat[i0 + c] = ph[c];
+    }
+    if (restart) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) rhat[i0 + c] = rv[c];
     }
 }
 
