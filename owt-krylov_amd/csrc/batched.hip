@@ -243,9 +243,10 @@ __global__ __launch_bounds__(kMaxTile) void kb_update_p(Ptrs P, const double* __
     const bool first = st.first != 0;
     const bool restart = st.restart != 0;
     const double beta = st.beta, omega = st.omega;
-    // All four inputs load before any branch (p and v hold zeros from the
-    // initial kernel on a first iteration), so the row's loads overlap
-    // instead of waiting for each other around the restart/first branches.
+    // All four inputs load before any branch (on a first iteration, after the
+    // initial kernel or a restart, the p and v values loaded are discarded by
+    // the select), so the row's loads overlap instead of waiting for each
+    // other around the restart/first branches.
     const std::int64_t i0 = rr.vertex * BS;
     double rv[BS], pv[BS], vv[BS], dv[BS];
 #pragma unroll
