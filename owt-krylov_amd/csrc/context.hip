@@ -57,6 +57,14 @@ void owtk_context_s::activate() const { OWTK_HIP(hipSetDevice(device)); }
 
 void owtk_context_s::sync() const { OWTK_HIP(hipStreamSynchronize(stream)); }
 
+void owtk_context_s::reserve_partials(std::int64_t blocks)
+{
+    if (blocks <= partial_stride) return;
+    if (partials.get() != nullptr) sync();
+    partials.allocate(static_cast<std::size_t>(owtk::dev::kMaxReduceValues) * static_cast<std::size_t>(blocks));
+    partial_stride = blocks;
+}
+
 hipEvent_t owtk_context_s::take_event()
 {
     if (!event_pool.empty()) {
@@ -122,10 +130,8 @@ int owtk_context_create(int device, owtk_context* out)
         OWTK_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
         OWTK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        ctx->partials.allocate(static_cast<std::size_t>(dev::kMaxReduceValues) * dev::kMaxReduceBlocks);
-        ctx->ticket.allocate(64);
+        ctx->reserve_partials(dev::kMaxReduceBlocks);
         ctx->scalars.allocate(1024);
-        kern::fill_zero_bytes(ctx->stream, ctx->ticket.get(), ctx->ticket.bytes());
         kern::fill_zero_bytes(ctx->stream, ctx->scalars.get(), ctx->scalars.bytes());
         ctx->pinned.ensure(1 << 16);
         ctx->sync();

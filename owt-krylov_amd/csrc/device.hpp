@@ -122,10 +122,13 @@ __device__ inline void block_reduce(double (&v)[K], std::uint64_t max_mask)
     __syncthreads();
 }
 
-// Scratch of one context for cross-block reductions.
+// Partials of cross-block reductions: value k of block b lives at
+// partials[k * stride + b], so a reduction over G blocks needs stride >= G.
+// A context's buffer holds kMaxReduceValues rows of `stride` blocks and grows
+// (at matrix creation) for kernels that launch more blocks than that.
 struct ReduceScratch {
-    double* partials;      // [kMaxReduceValues][kMaxReduceBlocks]
-    unsigned int* ticket;  // unused by the two-kernel scheme, kept for layout
+    double* partials;
+    std::int64_t stride;
 };
 
 // Cross-block reductions run in two launches.  The streaming kernel calls
@@ -143,7 +146,7 @@ __device__ inline void store_partials(double (&v)[K], std::uint64_t max_mask, Re
     block_reduce<K>(v, max_mask);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) scratch.partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
+        for (int k = 0; k < K; ++k) scratch.partials[k * scratch.stride + blockIdx.x] = v[k];
     }
 }
 
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(256) void k_finish(Fin fin, unsigned int nblocks, s
         for (int u = 0; u < U; ++u) {
             const unsigned int b = base + u * blockDim.x;
 #pragma unroll
-            for (int k = 0; k < K; ++k) p[u][k] = b < nblocks ? scratch.partials[k * kMaxReduceBlocks + b] : 0.0;
+            for (int k = 0; k < K; ++k) p[u][k] = b < nblocks ? scratch.partials[k * scratch.stride + b] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {

@@ -102,8 +102,8 @@ struct owtk_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
-    owtk::DeviceArray<double> partials;      // [kMaxReduceValues][kMaxReduceBlocks]
-    owtk::DeviceArray<unsigned int> ticket;  // grid_reduce arrival counter
+    owtk::DeviceArray<double> partials;      // [kMaxReduceValues][partial_stride]
+    std::int64_t partial_stride = 0;         // blocks one reduction may span
     owtk::DeviceArray<double> scalars;       // small device scalar results
     owtk::PinnedBuffer pinned;
     bool profiling = false;
@@ -128,7 +128,10 @@ struct owtk_context_s {
     owtk::DeviceArray<unsigned char> refine_ws;
     std::int64_t refine_layout = -1, refine_owned = -1;
 
-    owtk::dev::ReduceScratch scratch() const { return {partials.get(), ticket.get()}; }
+    owtk::dev::ReduceScratch scratch() const { return {partials.get(), partial_stride}; }
+    // Grow the reduction partials to hold `blocks` blocks per value (waits for
+    // the stream first: queued kernels may still use the old buffer).
+    void reserve_partials(std::int64_t blocks);
     void activate() const;  // hipSetDevice(device)
     void sync() const;
     hipEvent_t take_event();

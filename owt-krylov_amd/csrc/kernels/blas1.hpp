@@ -12,6 +12,7 @@ inline namespace {  // kernels get internal linkage in every including TU
 using namespace owtk::dev;
 
 constexpr int kMaxMultidot = 62;
+constexpr std::int64_t kMultidotMaxBlocks = 1024;  // grid cap of the multi-dot
 
 __global__ __launch_bounds__(256) void k_fill_zero16(uint4* p, std::int64_t n)
 {
@@ -254,7 +255,7 @@ inline void multidot_to(hipStream_t st, ReduceScratch scratch, std::int64_t n, i
     BasisPointers bp{};
     for (int j = 0; j < k; ++j) bp.v[j] = vs[j];
     std::int64_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > kMultidotMaxBlocks) blocks = kMultidotMaxBlocks;
     if (blocks < 1) blocks = 1;
     const unsigned int g = static_cast<unsigned int>(blocks);
     if (k + 1 <= 4) {

@@ -461,8 +461,8 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
     unsigned int launched = 0;
     if (s.layout == Layout::sell) {
         const std::int64_t blocks = (rows + kBlock - 1) / kBlock;
-        if (E::K > 0 && blocks > kMaxReduceBlocks)
-            throw unsupported_error("fused SpMV reductions support at most 4,194,304 rows");
+        if (E::K > 0 && blocks > scratch.stride)
+            throw std::logic_error("reduction partials smaller than the SpMV grid");
         if (blocks > 0x7fffffff) throw std::invalid_argument("too many rows");
         const unsigned int grid = static_cast<unsigned int>(blocks);
         launched = grid;

@@ -179,6 +179,9 @@ owtk_matrix create_matrix(owtk_context ctx, int kind, std::int64_t owned, std::i
     const bool sell = (kind == OWTK_MATRIX_BSR) ? (bs <= 4) : (bs <= 4);
     if (sell) build_sell(m.get());
     else build_csr(m.get());
+    // The SELL SpMVs launch one block per 256 rows and fuse the solvers'
+    // reductions into the same pass: the context's partials must span them.
+    if (sell) ctx->reserve_partials((owned + dev::kBlock - 1) / dev::kBlock);
     upload_values(m.get(), values, diagonal);
     return m.release();
 }
