@@ -131,32 +131,58 @@ std::vector<double> to_double(const Range& r)
 
 } // namespace detail
 
-// Device copy of a reference matrix (BlockCsrMatrix, SplitBlockCsrMatrixView
-// or DenseBlockCsrMatrix, read through their accessors).
+// Host structure of a reference matrix (BlockCsrMatrix,
+// SplitBlockCsrMatrixView or DenseBlockCsrMatrix, read through their
+// accessors) as the device's int32 indices, and its values in the reference
+// layout.
+struct HostStructure {
+    std::int64_t owned = 0, ghost = 0, block = 0;
+    std::vector<std::int32_t> offsets, columns;
+    bool operator==(const HostStructure&) const = default;
+};
+
+template<class Matrix>
+HostStructure host_structure(const Matrix& m)
+{
+    HostStructure h;
+    h.owned = static_cast<std::int64_t>(m.owned_nodes());
+    h.ghost = static_cast<std::int64_t>(m.ghost_nodes());
+    h.block = static_cast<std::int64_t>(m.block_size());
+    if constexpr (detail::SplitView<Matrix>) {
+        h.offsets = detail::to_int32(m.off_diagonal_row_offsets());
+        h.columns = detail::to_int32(m.off_diagonal_columns());
+    } else {
+        h.offsets = detail::to_int32(m.row_offsets());
+        h.columns = detail::to_int32(m.column_indices());
+    }
+    return h;
+}
+
+// Device copy of a reference matrix.
 class DeviceMatrix {
 public:
     template<class Matrix>
-    DeviceMatrix(Context& ctx, const Matrix& m)
-        : ctx_(&ctx), owned_(m.owned_nodes()), ghost_(m.ghost_nodes()), block_(m.block_size())
+    DeviceMatrix(Context& ctx, const Matrix& m) : DeviceMatrix(ctx, m, host_structure(m))
+    {
+    }
+    template<class Matrix>
+    DeviceMatrix(Context& ctx, const Matrix& m, const HostStructure& h)
+        : ctx_(&ctx), owned_(h.owned), ghost_(h.ghost), block_(h.block)
     {
         if constexpr (detail::SplitView<Matrix>) {
-            const auto off = detail::to_int32(m.off_diagonal_row_offsets());
-            const auto col = detail::to_int32(m.off_diagonal_columns());
             const auto diag = detail::to_double(m.diagonal_values());
             const auto val = detail::to_double(m.off_diagonal_values());
-            check(owtk_matrix_create_split(ctx.get(), owned_, ghost_, block_, off.data(), col.data(), diag.data(),
-                                           val.data(), &m_));
+            check(owtk_matrix_create_split(ctx.get(), owned_, ghost_, block_, h.offsets.data(), h.columns.data(),
+                                           diag.data(), val.data(), &m_));
         } else if constexpr (detail::DenseBlocks<Matrix>) {
-            const auto off = detail::to_int32(m.row_offsets());
-            const auto col = detail::to_int32(m.column_indices());
             const auto val = detail::to_double(m.block_values());
-            check(owtk_matrix_create_bsr(ctx.get(), owned_, ghost_, block_, off.data(), col.data(), val.data(), &m_));
+            check(owtk_matrix_create_bsr(ctx.get(), owned_, ghost_, block_, h.offsets.data(), h.columns.data(),
+                                         val.data(), &m_));
         } else {
             static_assert(detail::ComponentCsr<Matrix>, "unsupported matrix type");
-            const auto off = detail::to_int32(m.row_offsets());
-            const auto col = detail::to_int32(m.column_indices());
             const auto val = detail::to_double(m.values());
-            check(owtk_matrix_create_csr(ctx.get(), owned_, ghost_, block_, off.data(), col.data(), val.data(), &m_));
+            check(owtk_matrix_create_csr(ctx.get(), owned_, ghost_, block_, h.offsets.data(), h.columns.data(),
+                                         val.data(), &m_));
         }
     }
     ~DeviceMatrix()
@@ -179,6 +205,22 @@ public:
     {
         check(owtk_matrix_update_values(m_, values, off_diagonal));
     }
+    // Numeric refresh from a reference matrix of the same structure.
+    template<class Matrix>
+    void update_values_from(const Matrix& m)
+    {
+        if constexpr (detail::SplitView<Matrix>) {
+            const auto diag = detail::to_double(m.diagonal_values());
+            const auto val = detail::to_double(m.off_diagonal_values());
+            update_values(diag.data(), val.data());
+        } else if constexpr (detail::DenseBlocks<Matrix>) {
+            const auto val = detail::to_double(m.block_values());
+            update_values(val.data());
+        } else {
+            const auto val = detail::to_double(m.values());
+            update_values(val.data());
+        }
+    }
     void add_storage(owtk_dtype dtype) { check(owtk_matrix_add_storage(m_, dtype)); }
 
     // y = A x on device vectors (the operator interface, krylov_solvers.hpp:42-50).
@@ -195,10 +237,12 @@ private:
 
 // The HIP execution policy: apply(matrix, input, output) for host vectors
 // with the reference's BlockVector accessors (data(), local_size(),
-// owned_size()).  The device copy of a matrix is made on first use and
-// reused while the matrix object lives at the same address; call refresh()
-// after a numeric change (the reference's views see value changes without a
-// call, split_block_csr.hpp:25-28: this policy copies them on refresh()).
+// owned_size()).  Like the reference's OpenMP Target policy, which maps the
+// matrix on every apply (execution.hpp:66-69, 115-119), every apply sees the
+// matrix's current values: a device copy is kept per matrix address, checked
+// against the matrix's sizes and index arrays on every apply (rebuilt when
+// they differ, e.g. a different matrix at a reused address) and its values
+// uploaded again.  Transfers are sized from the checked matrix.
 class HipExecutionPolicy {
 public:
     explicit HipExecutionPolicy(int device = 0) : state_(std::make_shared<State>(device)) {}
@@ -213,21 +257,25 @@ public:
     template<class Matrix, class Vector>
     void apply(const Matrix& matrix, const Vector& input, Vector& output) const
     {
-        if (input.local_size() != static_cast<std::size_t>((matrix.owned_nodes() + matrix.ghost_nodes())
-                                                           * matrix.block_size())
-            || output.owned_size() != static_cast<std::size_t>(matrix.owned_nodes() * matrix.block_size()))
+        const std::size_t local = static_cast<std::size_t>((matrix.owned_nodes() + matrix.ghost_nodes())
+                                                           * matrix.block_size());
+        const std::size_t owned = static_cast<std::size_t>(matrix.owned_nodes() * matrix.block_size());
+        if (input.local_size() != local || output.owned_size() != owned)
             throw std::invalid_argument("execution-policy vector layout mismatch");
         using T = std::remove_cvref_t<decltype(*input.data())>;
         static_assert(std::is_same_v<T, double>, "the HIP policy's apply is FP64");
         std::lock_guard<std::mutex> lock(state_->mutex);
         DeviceMatrix& dm = state_->mirror(matrix);
-        auto& x = state_->buffer(0, dm.local_size());
-        auto& y = state_->buffer(1, dm.owned_size());
-        x.upload(input.data(), dm.local_size());
+        if (dm.local_size() != local || dm.owned_size() != owned)
+            throw std::logic_error("device copy does not match the matrix");
+        auto& x = state_->buffer(0, local);
+        auto& y = state_->buffer(1, owned);
+        x.upload(input.data(), local);
         dm.apply(x, y);
-        y.download(output.data(), dm.owned_size());
+        y.download(output.data(), owned);
     }
 
+    // Drops the device copy of `matrix` (the next apply rebuilds it).
     template<class Matrix>
     void refresh(const Matrix& matrix) const
     {
@@ -238,19 +286,30 @@ public:
     Context& context() const noexcept { return state_->ctx; }
 
 private:
+    struct Mirror {
+        HostStructure structure;
+        std::unique_ptr<DeviceMatrix> matrix;
+    };
     struct State {
         explicit State(int device) : ctx(device) {}
         Context ctx;
         std::mutex mutex;
-        std::map<const void*, std::unique_ptr<DeviceMatrix>> mirrors;
+        std::map<const void*, Mirror> mirrors;
         std::unique_ptr<DeviceVector<double>> buffers[2];
 
         template<class Matrix>
         DeviceMatrix& mirror(const Matrix& m)
         {
-            auto& slot = mirrors[static_cast<const void*>(&m)];
-            if (!slot) slot = std::make_unique<DeviceMatrix>(ctx, m);
-            return *slot;
+            Mirror& slot = mirrors[static_cast<const void*>(&m)];
+            HostStructure h = host_structure(m);
+            if (slot.matrix && slot.structure == h) {
+                slot.matrix->update_values_from(m);
+            } else {
+                slot.matrix.reset();
+                slot.matrix = std::make_unique<DeviceMatrix>(ctx, m, h);
+                slot.structure = std::move(h);
+            }
+            return *slot.matrix;
         }
         DeviceVector<double>& buffer(int k, std::size_t n)
         {

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <optional>
 #include <vector>
 
 namespace {
@@ -100,6 +101,41 @@ int main()
             std::printf("apply mismatch at %zu: %.17g vs %.17g\n", i, y.v[i], yref.v[i]);
             return 1;
         }
+    // A different matrix at a reused address (smaller), then a value change
+    // without refresh(): both must be seen by the next apply.
+    {
+        std::optional<Split> slot;
+        slot.emplace(chain(700, 3));
+        Vec x1(2100, 2100), y1(2100, 2100), r1(2100, 2100);
+        for (std::size_t i = 0; i < 2100; ++i) x1.v[i] = std::cos(0.05 * i);
+        policy.apply(*slot, x1, y1);
+        const void* first = &*slot;
+        slot.reset();
+        slot.emplace(chain(400, 3));
+        if (static_cast<const void*>(&*slot) != first) {
+            std::printf("address not reused\n");
+            return 1;
+        }
+        Vec x2(1200, 1200), y2(1200, 1200), r2(1200, 1200);
+        for (std::size_t i = 0; i < 1200; ++i) x2.v[i] = std::sin(0.3 * i);
+        policy.apply(*slot, x2, y2);
+        host_apply(*slot, x2, r2);
+        for (std::size_t i = 0; i < 1200; ++i)
+            if (y2.v[i] != r2.v[i]) {
+                std::printf("reused-address apply mismatch at %zu\n", i);
+                return 1;
+            }
+        for (auto& v : slot->vals) v *= 1.5;
+        slot->diag[7] = 11.0;
+        policy.apply(*slot, x2, y2);
+        host_apply(*slot, x2, r2);
+        for (std::size_t i = 0; i < 1200; ++i)
+            if (y2.v[i] != r2.v[i]) {
+                std::printf("value change not seen at %zu\n", i);
+                return 1;
+            }
+        std::printf("reused address and in-place value change ok\n");
+    }
     Vec b(3000, 3000), sol(3000, 3000), r(3000, 3000);
     for (std::size_t i = 0; i < 3000; ++i) b.v[i] = std::cos(0.37 * i);
     const Result res = solve_bicgstab<Result>(policy, s, b, sol, Options{});

@@ -17,3 +17,4 @@ def test_cpp_policy_layer():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "apply bitwise ok" in out.stdout
+    assert "reused address and in-place value change ok" in out.stdout
