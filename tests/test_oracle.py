@@ -52,6 +52,117 @@ def test_csr_and_split_agree_bitwise(c0):
     np.testing.assert_allclose(y_csr, y_split, rtol=1e-14, atol=1e-14)
 
 
+# ------------------------------------------- FP32 / BF16 SpMV oracle pinning
+def _bf16(a):
+    """Round to bfloat16 (nearest even) by bit manipulation, as FP32."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+CASTS = {"f32": lambda a: np.asarray(a, np.float32), "bf16": _bf16}
+
+
+@pytest.mark.parametrize("cast", ["f32", "bf16"])
+def test_spmv_f32_known_answers(cast):
+    """test_core.cpp:31-58, 176-195 and 233-279 run in float: every value is a
+    small integer (exact in FP32 and BF16), so every sum is exact."""
+    f = CASTS[cast]
+    o, c, v, x, exp = rc.core_csr_case()
+    m = orc.OMatrix.csr(2, 0, 3, o, c, v)
+    assert m.spmv_f32(f(v), None, f(x)).tolist() == exp
+    o, c, v, x, exp = rc.core_bsr_case()
+    m = orc.OMatrix.bsr(2, 0, 2, o, c, v)
+    assert m.spmv_f32(f(v), None, f(x)).tolist() == exp
+    o, c, d, e, x, exp = rc.core_split_case()
+    d, e = np.array(d, float), np.array(e, float)
+    m = orc.OMatrix.split(2, 1, 2, o, c, d, e)
+    assert m.spmv_f32(f(e), f(d), f(x)).tolist() == exp
+    d[0], e[0] = 8, -2
+    m2 = orc.OMatrix.split(2, 1, 2, o, c, d, e)
+    assert m2.spmv_f32(f(e), f(d), f(x))[0] == 2
+
+
+def test_bf16_cast_known_values():
+    """The BF16 cast used against the oracle rounds to nearest, ties to even
+    (7 stored mantissa bits: the spacing at 1 is 2^-7)."""
+    v = np.array([1.0, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8, -(1.0 + 2.0**-9), 1.0 + 2.0**-7 + 2.0**-8, 2.0**-130],
+                 np.float32)
+    expected = [1.0, 1.0, 1.0 + 2.0**-6, -1.0, 1.0 + 2.0**-6, 2.0**-130]
+    assert _bf16(v).tolist() == expected
+
+
+@pytest.mark.parametrize("cast", ["f32", "bf16"])
+@pytest.mark.parametrize("kind", ["split", "csr", "bsr"])
+def test_spmv_f32_long_double_bound(c0, cast, kind):
+    """On c0 (and a dense 3x3-block BSR of the same graph): the FP32 result
+    of the storage-rounded values differs from the long-double product of the
+    same rounded values by at most (row entries + 1) u sum |a||x|, u = 2^-24
+    (FP32 accumulation), and is not the FP64 result rounded once (a
+    computation in FP64 would be more accurate than the bound detects, so the
+    test also requires some rows to differ from it)."""
+    f = CASTS[cast]
+    s = c0
+    rng = np.random.default_rng(11)
+    bs = 3
+    x = rng.uniform(-1, 1, s.nodes * bs).astype(np.float32)
+    if kind == "split":
+        vals, diag = f(s.off_diagonal), f(s.diagonal)
+        m = orc.OMatrix.split(s.nodes, 0, bs, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+        y = m.spmv_f32(vals, diag, x)
+        offsets, columns = np.asarray(s.offsets), np.asarray(s.columns)
+        width = 1
+    elif kind == "csr":
+        co, cc, cv = s.csr()
+        vals, diag = f(cv), None
+        m = orc.OMatrix.csr(s.nodes, 0, bs, co, cc, cv)
+        y = m.spmv_f32(vals, None, x)
+        offsets, columns = np.asarray(co), np.asarray(cc)
+        width = 1
+    else:
+        co, cc, _ = s.csr()
+        blocks = rng.normal(0.0, 1.0, len(cc) * bs * bs)
+        vals, diag = f(blocks), None
+        m = orc.OMatrix.bsr(s.nodes, 0, bs, co, cc, blocks)
+        y = m.spmv_f32(vals, None, x)
+        offsets, columns = np.asarray(co), np.asarray(cc)
+        width = bs
+    xl = x.astype(rc.LD)
+    vl = np.asarray(vals, np.float64).astype(rc.LD)
+    worst = 0.0
+    differs_from_rounded_exact = 0
+    u = 2.0**-24
+    for row in range(s.nodes):
+        lo, hi = offsets[row], offsets[row + 1]
+        for r in range(bs):
+            exact = rc.LD(0)
+            mag = rc.LD(0)
+            if kind == "split":
+                t = rc.LD(float(diag[row * bs + r])) * xl[row * bs + r]
+                exact += t
+                mag += abs(t)
+            for e in range(lo, hi):
+                col = columns[e]
+                if width == 1:
+                    t = vl[e * bs + r] * xl[col * bs + r]
+                    exact += t
+                    mag += abs(t)
+                else:
+                    for cc_ in range(bs):
+                        t = vl[(e * bs + r) * bs + cc_] * xl[col * bs + cc_]
+                        exact += t
+                        mag += abs(t)
+            terms = (hi - lo) * width + 2
+            err = abs(float(y[row * bs + r]) - float(exact))
+            bound = terms * u * float(mag)
+            assert err <= bound, (row, r, err, bound)
+            worst = max(worst, err / bound if bound > 0 else 0.0)
+            if float(y[row * bs + r]) != float(np.float32(float(exact))):
+                differs_from_rounded_exact += 1
+    assert differs_from_rounded_exact > 0
+    print(f"{kind}/{cast}: worst error {worst:.3f} of the bound; {differs_from_rounded_exact} rows carry FP32 rounding")
+
+
 def test_norm_extremes():
     # test_foundation.cpp:71-107
     tiny = np.finfo(np.float64).tiny
