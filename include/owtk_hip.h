@@ -503,6 +503,18 @@ int owtk_axpy_dot(owtk_context ctx, size_t n, double alpha, const double* x, dou
  * (krylov_solvers.hpp:273-282). vs: host array of k device pointers. */
 int owtk_multidot(owtk_context ctx, size_t n, size_t k, const double* const* vs,
                   const double* w, double* out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same multi-dot as owtk_multidot, with bitwise the
+ * same results, but it needs no context and allocates nothing: the
+ * cross-block partials live in a static device array of 516,096 bytes per
+ * device (at most 62 basis vectors, the same limit), held until the process
+ * ends.  Runs on the calling thread's current HIP device, on `stream` (a
+ * hipStream_t; NULL is the null stream), and returns without waiting.  out:
+ * DEVICE array of k+1 doubles.  vs: host array of k device pointers.  Calls
+ * that overlap on one device share the array and race. */
+int owtk_multidot_noalloc(void* stream, size_t n, size_t k, const double* const* vs,
+                          const double* w, double* out);
 
 /* ------------------------------------------- synthetic grids (host only) -- */
 typedef enum owtk_ordering {

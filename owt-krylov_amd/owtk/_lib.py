@@ -193,6 +193,7 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_axpy": (C.c_int, [_P, C.c_size_t, C.c_double, _P, _P]),
     "owtk_axpy_dot": (C.c_int, [_P, C.c_size_t, C.c_double, _P, _P, _P, _DP]),
     "owtk_multidot": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(_P), _P, _DP]),
+    "owtk_multidot_noalloc": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(_P), _P, _P]),
     "owtk_batch_create": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, C.POINTER(_P)]),
     "owtk_batch_destroy": (C.c_int, [_P]),
     "owtk_batch_set_tile": (C.c_int, [_P, C.c_int32]),
