@@ -13,12 +13,17 @@ series of the Jacobi splitting, which tends to A^-1).
 Definitions (b components, K = degree, width 32, 3 message-passing layers):
 
   features   rho_ic = sum_e |a_ec| / d_ic,
-             x_i    = [rho_i0..rho_i(b-1), log(max(1 - rho_ic, 1e-6)) / 8 ..., deg_i / 8]
+             x_i    = [rho_ic, log(max(1 - rho_ic, 1e-6)) / 8,
+                       log(deg_i max_e |a_ec| / sum_e |a_ec|) / 2   (c = 0..b-1),
+                       deg_i / 8]
+             (the third group is the row's edge anisotropy: 0 when every edge
+             of the row carries the same coefficient)
   edge weight w_e   = (1/b) sum_c |a_ec| / d_ic            (e in row i)
   encoder    h0 = tanh(W_in x + b_in)
   layer l    agg_i = sum_e w_e h_{col e},  h = tanh(W_s h + W_n agg + b)
   decoder    o = W_out h3 + b_out,  index k*b + c
-  coefficients gamma_kic = g_kc (1 + 0.5 tanh(o_kci))
+  coefficients gamma_kic = g_kc + o_kci   (additive, unbounded: W_out = b_out = 0
+             gives the global polynomial g alone)
   apply      u_0 = D^-1 r,  u_(k+1) = -D^-1 A_off u_k,  z = sum_k gamma_k * u_k
   loss       L = (1/P) sum_p |D^-1/2 A (z_p - e_p)|^2 / |D^-1/2 A e_p|^2
 """
@@ -30,11 +35,10 @@ import numpy as np
 
 WIDTH = 32
 LAYERS = 3
-MODULATION = 0.5
 
 
 def feature_count(b: int) -> int:
-    return 2 * b + 1
+    return 3 * b + 1
 
 
 @dataclass
@@ -127,9 +131,13 @@ def features(g: Graph):
     absoff = np.abs(g.off)
     s = np.zeros((g.n, g.b))
     np.add.at(s, g.rows, absoff)
+    mx = np.zeros((g.n, g.b))
+    np.maximum.at(mx, g.rows, absoff)
     rho = s / g.diag
     lg = np.log(np.maximum(1.0 - rho, 1e-6)) / 8.0
-    x = np.concatenate([rho, lg, (g.deg / 8.0)[:, None]], axis=1)
+    deg = np.maximum(g.deg, 1.0)[:, None]
+    aniso = np.log(np.maximum(deg * mx / np.maximum(s, 1e-300), 1.0)) / 2.0
+    x = np.concatenate([rho, lg, aniso, (g.deg / 8.0)[:, None]], axis=1)
     w = (absoff / g.diag[g.rows]).mean(axis=1)
     return x, w
 
@@ -144,19 +152,17 @@ def forward(g: Graph, p: dict, degree: int):
         aggs.append(agg)
         hs.append(np.tanh(hs[-1] @ p[f"ws{layer}"].T + agg @ p[f"wn{layer}"].T + p[f"b{layer}"]))
     o = hs[-1] @ p["w_out"].T + p["b_out"]
-    t = np.tanh(o)
     k1 = degree + 1
-    gam = (p["g"][None, :] * (1.0 + MODULATION * t)).reshape(g.n, k1, g.b).transpose(1, 0, 2)
-    return gam, dict(x=x, w=w, hs=hs, aggs=aggs, t=t)
+    gam = (p["g"][None, :] + o).reshape(g.n, k1, g.b).transpose(1, 0, 2)
+    return gam, dict(x=x, w=w, hs=hs, aggs=aggs)
 
 
 def backward(g: Graph, p: dict, degree: int, cache: dict, dgamma):
     """Gradient of a scalar with respect to the parameters, given d/dgamma."""
     k1 = degree + 1
     dg_flat = np.asarray(dgamma).transpose(1, 0, 2).reshape(g.n, k1 * g.b)
-    t = cache["t"]
-    grads = {"g": (dg_flat * (1.0 + MODULATION * t)).sum(0)}
-    do = dg_flat * p["g"][None, :] * MODULATION * (1.0 - t * t)
+    grads = {"g": dg_flat.sum(0)}
+    do = dg_flat
     hs, aggs, w = cache["hs"], cache["aggs"], cache["w"]
     grads["w_out"] = do.T @ hs[-1]
     grads["b_out"] = do.sum(0)

@@ -12,12 +12,16 @@
 // triangular dependency chains.
 //
 // Network (weights BF16, arithmetic FP32), one thread per vertex:
-//   features  x_i = [rho_ic, log(max(1 - rho_ic, 1e-6)) / 8, deg_i / 8],
-//             rho_ic = sum_e |a_ec| / d_ic
+//   features  x_i = [rho_ic, log(max(1 - rho_ic, 1e-6)) / 8,
+//                    log(deg_i max_e |a_ec| / sum_e |a_ec|) / 2, deg_i / 8],
+//             rho_ic = sum_e |a_ec| / d_ic (row dominance); the third group is
+//             the row's edge anisotropy (0 when all its edges are equal)
 //   weights   w_e = (1/b) sum_c |a_ec| / d_ic
 //   encoder   h0 = tanh(W_in x + b_in)
 //   3 layers  agg_i = sum_e w_e h_col(e),  h = tanh(W_s h + W_n agg + b)
-//   decoder   o = W_out h3 + b_out,  gamma_kic = g_kc (1 + 0.5 tanh(o_(kb+c),i))
+//   decoder   o = W_out h3 + b_out,  gamma_kic = g_kc + o_(kb+c),i
+//             (additive and unbounded; a zero decoder leaves the global
+//             polynomial g)
 //
 // Training loss over (residual, error) pairs that the solver produced:
 //   L = (1/P) sum_p |D^-1/2 A (z_p - e_p)|^2 / |D^-1/2 A e_p|^2,  z_p = M r_p.
@@ -86,6 +90,7 @@ __global__ __launch_bounds__(256) void k_gnn_features(std::int64_t n, int b, int
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s[4] = {0.f, 0.f, 0.f, 0.f};
+    float mx[4] = {0.f, 0.f, 0.f, 0.f};
     float inv[4];
     for (int c = 0; c < b; ++c) inv[c] = 1.0f / diag[i * b + c];
     const std::int32_t e0 = offsets[i], e1 = offsets[i + 1];
@@ -94,16 +99,19 @@ __global__ __launch_bounds__(256) void k_gnn_features(std::int64_t n, int b, int
         for (int c = 0; c < b; ++c) {
             const float a = __builtin_fabsf(vals[static_cast<std::int64_t>(e) * b + c]);
             s[c] += a;
+            mx[c] = fmaxf(mx[c], a);
             we += a * inv[c];
         }
         w[e] = we / static_cast<float>(b);
     }
+    const float deg = static_cast<float>(e1 - e0);
     for (int c = 0; c < b; ++c) {
         const float rho = s[c] * inv[c];
         x[i * f0 + c] = rho;
         x[i * f0 + b + c] = logf(fmaxf(1.0f - rho, 1e-6f)) * 0.125f;
+        x[i * f0 + 2 * b + c] = s[c] > 0.f ? logf(fmaxf(fmaxf(deg, 1.f) * mx[c] / s[c], 1.0f)) * 0.5f : 0.f;
     }
-    x[i * f0 + 2 * b] = static_cast<float>(e1 - e0) * 0.125f;
+    x[i * f0 + 3 * b] = deg * 0.125f;
 }
 
 // BF16 parameters -> LDS floats.
@@ -217,7 +225,7 @@ __global__ __launch_bounds__(256) void k_gnn_layer(std::int64_t n, const std::in
 __global__ __launch_bounds__(256) void k_gnn_decode(std::int64_t n, int b, int outs, const float* __restrict__ h3,
                                                     const std::uint16_t* __restrict__ p16, std::int64_t off_w,
                                                     std::int64_t off_b, std::int64_t off_g,
-                                                    float* __restrict__ gamma, float* __restrict__ t_out)
+                                                    float* __restrict__ gamma)
 {
     __shared__ float sw[kGnnMaxOutputs * W];
     __shared__ float sb[kGnnMaxOutputs];
@@ -235,27 +243,22 @@ __global__ __launch_bounds__(256) void k_gnn_decode(std::int64_t n, int b, int o
         float acc = sb[o];
 #pragma unroll
         for (int k = 0; k < W; ++k) acc = __builtin_fmaf(sw[o * W + k], h[k], acc);
-        const float t = tanhf(acc);
-        if (t_out) t_out[i * outs + o] = t;
         const int kk = o / b, c = o - kk * b;
-        gamma[kk * nb + i * b + c] = sg[o] * (1.0f + kGnnModulation * t);
+        gamma[kk * nb + i * b + c] = sg[o] + acc;
     }
 }
 
 // ------------------------------------------------------- backward kernels --
 
-// d gamma -> d o (do_out), d g contributions (q_out) and d h3.
+// d gamma -> d o (do_out, which is also each vertex's contribution to d g)
+// and d h3 = W_out^T d o.
 __global__ __launch_bounds__(256) void k_gnn_decode_bwd(std::int64_t n, int b, int outs,
                                                         const float* __restrict__ dgamma,
-                                                        const float* __restrict__ t,
                                                         const std::uint16_t* __restrict__ p16, std::int64_t off_w,
-                                                        std::int64_t off_g, float* __restrict__ do_out,
-                                                        float* __restrict__ q_out, float* __restrict__ dh)
+                                                        float* __restrict__ do_out, float* __restrict__ dh)
 {
     __shared__ float sw[kGnnMaxOutputs * W];
-    __shared__ float sg[kGnnMaxOutputs];
     load_weights(sw, p16, off_w, outs * W);
-    load_weights(sg, p16, off_g, outs);
     __syncthreads();
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -265,10 +268,7 @@ __global__ __launch_bounds__(256) void k_gnn_decode_bwd(std::int64_t n, int b, i
     for (int k = 0; k < W; ++k) acc[k] = 0.f;
     for (int o = 0; o < outs; ++o) {
         const int kk = o / b, c = o - kk * b;
-        const float dg = dgamma[kk * nb + i * b + c];
-        const float tt = t[i * outs + o];
-        q_out[i * outs + o] = dg * (1.0f + kGnnModulation * tt);
-        const float d = dg * sg[o] * kGnnModulation * (1.0f - tt * tt);
+        const float d = dgamma[kk * nb + i * b + c];
         do_out[i * outs + o] = d;
 #pragma unroll
         for (int k = 0; k < W; ++k) acc[k] = __builtin_fmaf(sw[o * W + k], d, acc[k]);
@@ -778,10 +778,11 @@ void check_matrix(owtk_gnn g, owtk_matrix m)
     if (m->ctx != g->ctx) throw std::invalid_argument("matrix and model belong to different contexts");
 }
 
-void build_graph(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with_mirror)
+// The graph's structure (CSR, mirror, the matrix slot of each entry) and
+// its arrays, built once per matrix pattern.
+void build_graph_structure(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with_mirror)
 {
     check_matrix(model, m);
-    const hipStream_t st = m->ctx->stream;
     g.n = m->owned;
     g.b = m->bs;
     g.nnz = m->h_offsets[g.n];
@@ -793,25 +794,35 @@ void build_graph(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with_mirror)
         slot.resize(g.nnz);
         std::iota(slot.begin(), slot.end(), 0);
     }
-    DeviceArray<std::int64_t> dslot;
-    upload_vec(dslot, slot);
+    upload_vec(g.slot, slot);
     g.vals.allocate(std::max<std::int64_t>(g.nnz * g.b, 1));
     g.diag.allocate(g.n * g.b);
     g.x.allocate(g.n * model->f0);
     g.w.allocate(std::max<std::int64_t>(g.nnz, 1));
+}
+
+// The graph's values and features from the matrix's current FP64 values.
+void graph_values(owtk_gnn model, owtk_matrix m, GnnGraph& g)
+{
+    const hipStream_t st = m->ctx->stream;
     const unsigned int grid = rows_grid(g.n);
-    k_gnn_gather<<<grid, kBlock, 0, st>>>(g.n, g.b, g.offsets.get(), dslot.get(), m->s.layout == Layout::sell ? 1 : 0,
+    k_gnn_gather<<<grid, kBlock, 0, st>>>(g.n, g.b, g.offsets.get(), g.slot.get(), m->s.layout == Layout::sell ? 1 : 0,
                                           m->v64.get(), m->d64.get(), g.vals.get(), g.diag.get());
     OWTK_LAUNCH_CHECK();
     k_gnn_features<<<grid, kBlock, 0, st>>>(g.n, g.b, model->f0, g.offsets.get(), g.vals.get(), g.diag.get(),
                                             g.x.get(), g.w.get());
     OWTK_LAUNCH_CHECK();
-    OWTK_HIP(hipStreamSynchronize(st));  // dslot is released on return
+}
+
+void build_graph(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with_mirror)
+{
+    build_graph_structure(model, m, g, with_mirror);
+    graph_values(model, m, g);
 }
 
 // Forward pass.  h: (layers+1) * n * W floats when `train`, else 2 * n * W
-// (ping-pong); agg (train only): layers * n * W; t (train only): n * outs.
-void forward(owtk_gnn model, const GnnGraph& g, float* h, float* agg, float* t, float* gamma, bool train)
+// (ping-pong); agg (train only): layers * n * W.
+void forward(owtk_gnn model, const GnnGraph& g, float* h, float* agg, float* gamma, bool train)
 {
     const hipStream_t st = model->ctx->stream;
     const std::int64_t n = g.n;
@@ -829,7 +840,7 @@ void forward(owtk_gnn model, const GnnGraph& g, float* h, float* agg, float* t, 
         cur = next;
     }
     k_gnn_decode<<<grid, kBlock, 0, st>>>(n, g.b, model->outs, cur, p16, model->off_w_out, model->off_b_out,
-                                          model->off_g, gamma, train ? t : nullptr);
+                                          model->off_g, gamma);
     OWTK_LAUNCH_CHECK();
 }
 
@@ -844,15 +855,15 @@ void outer_reduce(owtk_gnn model, std::int64_t n, const float* A, int R, const f
     OWTK_LAUNCH_CHECK();
 }
 
-// Workspace layout (floats per vertex): h (layers+1)*W | agg layers*W | t outs |
-// do outs | q outs | dh W | dprev W | du W | v W
+// Workspace layout (floats per vertex): h (layers+1)*W | agg layers*W |
+// do outs | dh W | dprev W | du W | v W
 struct TrainViews {
-    float *h, *agg, *t, *dout, *q, *dh, *dprev, *du, *v;
+    float *h, *agg, *dout, *dh, *dprev, *du, *v;
 };
 
 TrainViews ensure_workspace(owtk_gnn model, std::int64_t n, int pairs, int k1, int b)
 {
-    const std::int64_t per = (kGnnLayers + 1) * W + kGnnLayers * W + 3 * model->outs + 4 * W;
+    const std::int64_t per = (kGnnLayers + 1) * W + kGnnLayers * W + model->outs + 4 * W;
     if (n > model->ws_nodes || pairs > model->ws_pairs) {
         const std::int64_t nn = std::max(n, model->ws_nodes);
         const std::int64_t pp = std::max<std::int64_t>(pairs, model->ws_pairs);
@@ -870,9 +881,7 @@ TrainViews ensure_workspace(owtk_gnn model, std::int64_t n, int pairs, int k1, i
     float* p = model->act.get();
     v.h = p; p += (kGnnLayers + 1) * W * n;
     v.agg = p; p += kGnnLayers * W * n;
-    v.t = p; p += model->outs * n;
     v.dout = p; p += model->outs * n;
-    v.q = p; p += model->outs * n;
     v.dh = p; p += W * n;
     v.dprev = p; p += W * n;
     v.du = p; p += W * n;
@@ -896,7 +905,7 @@ void accumulate(owtk_gnn model, owtk_gnn_problem prob, double weight)
     OWTK_HIP(hipMemcpyAsync(model->coef.get(), coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice, st));
     OWTK_HIP(hipStreamSynchronize(st));  // coef is a host temporary
 
-    forward(model, g, v.h, v.agg, v.t, model->gamma.get(), true);
+    forward(model, g, v.h, v.agg, model->gamma.get(), true);
     const unsigned int grid = rows_grid(n);
     k_loss_z<<<dev::stream_grid(nb * P, 4), kBlock, 0, st>>>(nb, P, k1, model->gamma.get(), prob->u.get(), model->z.get());
     OWTK_LAUNCH_CHECK();
@@ -918,11 +927,11 @@ void accumulate(owtk_gnn model, owtk_gnn_problem prob, double weight)
     float* grad = model->grad.get();
     const std::uint16_t* p16 = model->params16.get();
     const float* h3 = v.h + static_cast<std::int64_t>(kGnnLayers) * n * W;
-    k_gnn_decode_bwd<<<grid, kBlock, 0, st>>>(n, g.b, model->outs, model->dgamma.get(), v.t, p16, model->off_w_out,
-                                              model->off_g, v.dout, v.q, v.dh);
+    k_gnn_decode_bwd<<<grid, kBlock, 0, st>>>(n, g.b, model->outs, model->dgamma.get(), p16, model->off_w_out,
+                                              v.dout, v.dh);
     OWTK_LAUNCH_CHECK();
     outer_reduce(model, n, v.dout, model->outs, h3, W, grad + model->off_w_out, grad + model->off_b_out);
-    outer_reduce(model, n, v.q, model->outs, nullptr, 0, nullptr, grad + model->off_g);
+    outer_reduce(model, n, v.dout, model->outs, nullptr, 0, nullptr, grad + model->off_g);
     float* dh = v.dh;
     float* dprev = v.dprev;
     for (int l = kGnnLayers - 1; l >= 0; --l) {
@@ -973,6 +982,8 @@ void learned_setup(owtk_precond pc)
         st->sgamma32.allocate(nb * k1);
     }
     st->gamma.allocate(nb * k1);
+    build_graph_structure(model, pc->m, st->graph, false);
+    st->h.allocate(static_cast<std::size_t>(2 * st->graph.n * W));
     pc->learned_state = st.release();
 }
 
@@ -982,10 +993,9 @@ void learned_numeric(owtk_precond pc)
     owtk_gnn model = st->model;
     owtk_matrix m = pc->m;
     const hipStream_t s = m->ctx->stream;
-    GnnGraph g;
-    build_graph(model, m, g, false);
-    DeviceArray<float> h(2 * g.n * W);
-    forward(model, g, h.get(), nullptr, nullptr, st->gamma.get(), false);
+    GnnGraph& g = st->graph;
+    graph_values(model, m, g);
+    forward(model, g, st->h.get(), nullptr, st->gamma.get(), false);
     const std::int64_t nb = pc->n();
     const int sell = m->s.layout == Layout::sell ? 1 : 0;
     const int k1 = model->degree + 1;
@@ -1013,7 +1023,6 @@ void learned_numeric(owtk_precond pc)
         break;
     }
     OWTK_LAUNCH_CHECK();
-    OWTK_HIP(hipStreamSynchronize(s));  // graph and h are released on return
 }
 
 // Algorithmic bytes of one series step: off-diagonal values + columns + row
@@ -1139,7 +1148,7 @@ int owtk_gnn_create(owtk_context ctx, int32_t block_size, int32_t degree, owtk_g
         g->ctx = ctx;
         g->b = block_size;
         g->degree = degree;
-        g->f0 = 2 * block_size + 1;
+        g->f0 = 3 * block_size + 1;
         g->outs = (degree + 1) * block_size;
         std::int64_t pos = 0;
         g->off_w_in = pos; pos += W * g->f0;
@@ -1160,8 +1169,29 @@ int owtk_gnn_create(owtk_context ctx, int32_t block_size, int32_t degree, owtk_g
         g->params16.allocate(pos);
         g->loss.allocate(1);
         g->partials.allocate(static_cast<std::size_t>(kReduceGrid) * kMaxOuterOutputs);
-        // default parameters: zero network, g = 1 (plain Neumann series)
+        // default parameters: Glorot-uniform encoder and layer weights (a
+        // fixed SplitMix64 stream), zero biases, zero decoder and g = 1, so a
+        // fresh model is the plain Neumann series and every weight receives a
+        // gradient from the first training step on
         std::vector<float> init(pos, 0.0f);
+        std::uint64_t state = 0x6a09e667f3bcc909ULL;
+        auto uniform = [&state](float lim) {
+            std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            z ^= z >> 31;
+            const double u = static_cast<double>(z >> 11) * 0x1.0p-53;  // [0, 1)
+            return static_cast<float>((2.0 * u - 1.0) * lim);
+        };
+        auto glorot = [&](std::int64_t off, int rows, int cols) {
+            const float lim = std::sqrt(6.0f / static_cast<float>(rows + cols));
+            for (std::int64_t t = 0; t < static_cast<std::int64_t>(rows) * cols; ++t) init[off + t] = uniform(lim);
+        };
+        glorot(g->off_w_in, W, g->f0);
+        for (int l = 0; l < kGnnLayers; ++l) {
+            glorot(g->off_ws[l], W, W);
+            glorot(g->off_wn[l], W, W);
+        }
         for (int o = 0; o < g->outs; ++o) init[g->off_g + o] = 1.0f;
         OWTK_HIP(hipMemcpy(g->params.get(), init.data(), pos * sizeof(float), hipMemcpyHostToDevice));
         OWTK_HIP(hipMemset(g->grad.get(), 0, pos * sizeof(float)));
@@ -1230,7 +1260,7 @@ int owtk_gnn_coefficients(owtk_gnn model, owtk_matrix m, float* gamma)
         const std::int64_t nb = g.n * g.b;
         DeviceArray<float> h(2 * g.n * W);
         DeviceArray<float> gam(nb * (model->degree + 1));
-        forward(model, g, h.get(), nullptr, nullptr, gam.get(), false);
+        forward(model, g, h.get(), nullptr, gam.get(), false);
         model->ctx->sync();
         OWTK_HIP(hipMemcpy(gamma, gam.get(), gam.bytes(), hipMemcpyDeviceToHost));
     });
@@ -1245,7 +1275,9 @@ int owtk_gnn_problem_create(owtk_gnn model, owtk_matrix m, int32_t pairs, const 
         *out = nullptr;
         model->ctx->activate();
         auto p = std::make_unique<owtk_gnn_problem_s>();
-        p->model = model;
+        p->ctx = model->ctx;
+        p->b = model->b;
+        p->degree = model->degree;
         p->pairs = pairs;
         build_graph(model, m, p->g, true);
         const GnnGraph& g = p->g;
@@ -1287,8 +1319,8 @@ int owtk_gnn_problem_destroy(owtk_gnn_problem p)
 {
     return guarded([&] {
         if (!p) return;
-        p->model->ctx->activate();
-        p->model->ctx->sync();
+        p->ctx->activate();
+        p->ctx->sync();
         delete p;
     });
 }
@@ -1297,7 +1329,8 @@ int owtk_gnn_accumulate(owtk_gnn g, owtk_gnn_problem p, double weight)
 {
     return guarded([&] {
         if (!g || !p) throw std::invalid_argument("null argument");
-        if (p->model != g) throw std::invalid_argument("problem belongs to another model");
+        if (p->ctx != g->ctx || p->b != g->b || p->degree != g->degree)
+            throw std::invalid_argument("problem was built for another context, block size or degree");
         g->ctx->activate();
         accumulate(g, p, weight);
     });

@@ -12,8 +12,7 @@ namespace owtk {
 constexpr int kGnnWidth = 32;        // hidden width
 constexpr int kGnnLayers = 3;        // message-passing layers
 constexpr int kGnnMaxOutputs = 32;   // (degree + 1) * block size
-constexpr int kGnnMaxFeatures = 9;   // 2 * block size + 1
-constexpr float kGnnModulation = 0.5f;
+constexpr int kGnnMaxFeatures = 13;  // 3 * block size + 1
 
 // Graph view of a component-diagonal split matrix for the network: CSR over
 // vertices (the matrix's own entry order), FP32 copies of the values, the
@@ -24,6 +23,7 @@ struct GnnGraph {
     DeviceArray<std::int32_t> offsets, cols, mirror;  // mirror: entry of (col, row); training only
     DeviceArray<float> vals, diag;                     // [e][c], [i][c]
     DeviceArray<float> x, w;                           // features [i][f0], weights [e]
+    DeviceArray<std::int64_t> slot;                    // matrix value slot of each CSR entry
 };
 
 } // namespace owtk
@@ -46,8 +46,12 @@ struct owtk_gnn_s {
     owtk::DeviceArray<float> coef;                  // per pair loss / gradient scale
 };
 
+// A training problem depends on the block size and the degree only (its
+// basis vectors u_k and targets), so every model of its context with that
+// block size and degree can train on it.
 struct owtk_gnn_problem_s {
-    owtk_gnn model = nullptr;
+    owtk_context ctx = nullptr;
+    int b = 1, degree = 0;
     int pairs = 0;
     owtk::GnnGraph g;
     owtk::DeviceArray<float> u;       // [p][k][n*b] basis vectors
@@ -67,6 +71,9 @@ namespace owtk {
 // formed once per numeric set-up.
 struct LearnedPc {
     owtk_gnn model = nullptr;
+    GnnGraph graph;                       // structure built once; values refreshed per update
+    DeviceArray<float> h;                 // network activations, 2 * n * width
+
     DeviceArray<float> gamma;             // [k][n*b], the network's coefficients
     DeviceArray<unsigned char> scaled;    // A_off D^-1 values, value_count entries
     DeviceArray<double> sgamma64;         // [k][n*b] gamma_k D^-1 (FP64 storage)

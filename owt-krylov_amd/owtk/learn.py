@@ -136,16 +136,23 @@ def fgmres_run(ctx, m, pc, b, x, restart=30, rtol=1e-8, maximum_iterations=3000)
 
 
 def compare(ctx, model, s, m, restart=30, rtol=1e-8, ilu_block=4096):
-    """FGMRES(restart) to rtol with the learned network, the same series
-    without it (untrained model), Jacobi and block-Jacobi ILU(0)."""
+    """FGMRES(restart) to rtol with the learned network; the network's
+    coefficients averaged over the vertices (one polynomial: no per-vertex
+    modulation); the learned global polynomial without the network (decoder
+    zeroed: gamma = g); the plain truncated series (untrained model: gamma =
+    1); Jacobi and block-Jacobi ILU(0)."""
     n = s.nodes * s.block
     b = ctx.upload(s.rhs)
     x = ctx.empty(n)
     r = ctx.empty(n)
     bn = ctx.norm(b)
     fresh = owtk.GnnModel(ctx, s.block, model.degree)
+    polynomial = model.without_network()
+    average = model.constant_polynomial(m)
     out = {}
     for name, make in (("learned", lambda: owtk.Preconditioner(m, owtk.PC_LEARNED, model=model)),
+                       ("vertex_average", lambda: owtk.Preconditioner(m, owtk.PC_LEARNED, model=average)),
+                       ("no_network", lambda: owtk.Preconditioner(m, owtk.PC_LEARNED, model=polynomial)),
                        ("neumann", lambda: owtk.Preconditioner(m, owtk.PC_LEARNED, model=fresh)),
                        ("jacobi", lambda: owtk.Preconditioner(m, owtk.PC_JACOBI)),
                        ("ilu0", lambda: owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=ilu_block))):
@@ -156,6 +163,14 @@ def compare(ctx, model, s, m, restart=30, rtol=1e-8, ilu_block=4096):
         out[name] = {"iterations": res["iterations"], "status": res["status_name"], "ms": round(ms, 3),
                      "true_rel": ctx.norm(r) / bn}
     return out
+
+
+def gamma_spread(model, m):
+    """Per (k, c): standard deviation of gamma over the vertices relative to
+    its mean magnitude (0: the network's output is one polynomial)."""
+    gam = model.coefficients(m).astype(np.float64)
+    return [[float(gam[k, :, c].std() / max(abs(gam[k, :, c].mean()), 1e-30)) for c in range(gam.shape[2])]
+            for k in range(gam.shape[0])]
 
 
 def main(argv=None):
@@ -197,7 +212,10 @@ def main(argv=None):
             after = evaluation_loss(model, held)
             model.save(args.out)
             print(json.dumps({"trained": args.out, "instances": args.instances, "grid": [args.nx, args.ny],
+                              "law": args.law, "sigma": args.sigma, "degree": args.degree, "epochs": args.epochs,
+                              "batch": args.batch, "lr": args.lr, "seed": args.seed,
                               "held_out_loss_before": before, "held_out_loss_after": after,
+                              "held_out_loss_without_network": evaluation_loss(model.without_network(), held),
                               "train_loss": hist}))
         else:
             model = owtk.GnnModel.load(ctx, args.model)
@@ -206,13 +224,19 @@ def main(argv=None):
                 s, m = instance(ctx, args.nx, args.ny, args.seed + k, args.sigma, law=law)
                 row = compare(ctx, model, s, m)
                 rows.append(row)
+                row["gamma_spread"] = gamma_spread(model, m)
                 log(f"[learn] field {k + 1}/{args.fields}: " + ", ".join(
-                    f"{name} {v['iterations']} it {v['ms']:.1f} ms" for name, v in row.items()))
+                    f"{name} {v['iterations']} it {v['ms']:.1f} ms" for name, v in row.items() if name != "gamma_spread"))
+            methods = [k for k in rows[0] if k != "gamma_spread"]
             summary = {name: {"median_iterations": float(np.median([r[name]["iterations"] for r in rows])),
                               "median_ms": float(np.median([r[name]["ms"] for r in rows])),
                               "max_true_rel": float(max(r[name]["true_rel"] for r in rows)),
                               "converged": sum(r[name]["status"] == "converged" for r in rows)}
-                       for name in rows[0]}
+                       for name in methods}
+            summary["iteration_ratio_vertex_average_over_learned"] = float(np.median(
+                [r["vertex_average"]["iterations"] / r["learned"]["iterations"] for r in rows]))
+            summary["iteration_ratio_no_network_over_learned"] = float(np.median(
+                [r["no_network"]["iterations"] / r["learned"]["iterations"] for r in rows]))
             print(json.dumps({"model": args.model, "fields": args.fields, "grid": [args.nx, args.ny],
                               "summary": summary, "per_field": rows}))
     return 0

@@ -163,22 +163,25 @@ def test_learned_preconditioned_fgmres(ctx, lognormal):
         assert rel <= 1.5e-10
 
 
-@pytest.mark.skipif(not (MODELS / "gnn_lognormal.npz").exists(), reason="no trained model committed")
-def test_trained_model_gains_over_plain_series(ctx):
-    """The committed network, on unseen lognormal fields, against the same
-    truncated series without it (identical apply cost)."""
+def test_trained_network_gains_on_unseen_full_size_fields(ctx):
+    """c3 evaluation size (1024x512, 524,288 vertices), two unseen lognormal
+    fields: the committed network against the same model with its decoder
+    zeroed (gamma = g, the learned global polynomial) and against its own
+    coefficients averaged over the vertices (one polynomial, no per-vertex
+    modulation), all at identical apply cost, FGMRES(30) to rtol 1e-8."""
     from owtk import learn
     model = owtk.GnnModel.load(ctx, MODELS / "gnn_lognormal.npz")
-    gains = []
     for seed in (2000, 2001):
-        s, m = learn.instance(ctx, 256, 128, seed, 1.5)
+        s, m = learn.instance(ctx, 1024, 512, seed, 1.5)
         row = learn.compare(ctx, model, s, m)
-        print(seed, {k: (v["iterations"], v["ms"]) for k, v in row.items()})
-        assert row["learned"]["status"] == "converged"
-        assert row["learned"]["true_rel"] <= 1.5e-8
-        gains.append(row["neumann"]["iterations"] / row["learned"]["iterations"])
-    print("iteration gain over the plain series:", gains)
-    assert min(gains) > 1.05
+        its = {k: v["iterations"] for k, v in row.items()}
+        print(seed, its, {k: round(v["ms"], 1) for k, v in row.items()})
+        for name in ("learned", "vertex_average", "no_network"):
+            assert row[name]["status"] == "converged"
+            assert row[name]["true_rel"] <= 1e-8
+        assert its["learned"] <= 0.9 * its["no_network"]
+        assert its["learned"] < its["vertex_average"]
+        assert its["learned"] < its["ilu0"]
 
 
 def test_fallback_to_ilu_when_learned_fails(ctx, lognormal):
