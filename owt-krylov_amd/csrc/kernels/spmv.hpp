@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restr
         for (int k = 0; k < len; ++k) {
             const std::int64_t slot = base + 64 * k + lane;
             const std::int64_t col = cols[slot];
-            const V* blk = vals + slot * (BS * BS);
+            const V* blk = vals + (slot - lane) * (BS * BS) + lane;  // element j at blk[64 j]
             A xc[BS];
 #pragma unroll
             for (int c = 0; c < BS; ++c) xc[c] = static_cast<A>(x[col * BS + c]);
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restr
             for (int r = 0; r < BS; ++r) {
                 A t = A(0);
 #pragma unroll
-                for (int c = 0; c < BS; ++c) t = dfma(static_cast<A>(load_value(blk, r * BS + c)), xc[c], t);
+                for (int c = 0; c < BS; ++c) t = dfma(static_cast<A>(load_value(blk, 64 * (r * BS + c))), xc[c], t);
                 acc[r] = acc[r] + t;
             }
         }

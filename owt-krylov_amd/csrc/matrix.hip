@@ -116,7 +116,11 @@ void upload_values(owtk_matrix m, const double* values, const double* diagonal)
         for (std::size_t en = b; en < e; ++en) {
             const std::int64_t slot = m->s.slot_of_entry[en];
             if (m->dense()) {
-                std::copy_n(values + en * bv, bv, staged.data() + slot * bv);
+                if (m->s.layout == Layout::sell)
+                    for (std::int64_t j = 0; j < bv; ++j)
+                        staged[dev::dense_index(slot, static_cast<int>(j), static_cast<int>(bv))] = values[en * bv + j];
+                else
+                    std::copy_n(values + en * bv, bv, staged.data() + slot * bv);
             } else {
                 for (int c = 0; c < m->bs; ++c) staged[m->value_index(slot, c)] = values[en * bv + c];
             }

@@ -29,6 +29,7 @@ using namespace owtk;
 
 struct owtk_multigrid_s {
     owtk_matrix fine = nullptr;
+    owtk_context ctx = nullptr;  // destroy may run after the fine matrix is gone
     owtk_matrix coarse = nullptr;  // owned
     owtk_multigrid_options opt{};
     int bs = 1;
@@ -582,6 +583,7 @@ int owtk_multigrid_create(owtk_matrix fine, const owtk_multigrid_options* option
         fine->ctx->activate();
         auto mg = std::make_unique<owtk_multigrid_s>();
         mg->fine = fine;
+        mg->ctx = fine->ctx;
         mg->opt = *options;
         mg_setup(mg.get());
         *out = mg.release();
@@ -592,7 +594,8 @@ int owtk_multigrid_destroy(owtk_multigrid mg)
 {
     return guarded([&] {
         if (!mg) return;
-        mg->fine->ctx->activate();
+        mg->ctx->activate();
+        mg->ctx->sync();
         delete mg;
     });
 }
@@ -616,6 +619,7 @@ int owtk_multigrid_solve(owtk_multigrid mg, const double* b, double* x, const ow
 {
     return guarded([&] {
         if (!mg || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        require_plain_options(options, "multigrid");
         mg->fine->ctx->activate();
         mg_solve(mg, b, x, options, cycle, result);
     });

@@ -117,12 +117,12 @@ __global__ __launch_bounds__(256) void k_block_inverse(double* __restrict__ inv,
     const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     double a[BS][BS], b[BS][BS];
-    const double* blk = values + slot[row] * (BS * BS);
+    const std::int64_t ds = slot[row];
 #pragma unroll
     for (int r = 0; r < BS; ++r)
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
-            a[r][c] = blk[r * BS + c];
+            a[r][c] = values[dev::dense_index(ds, r * BS + c, BS * BS)];
             b[r][c] = r == c ? 1.0 : 0.0;
         }
     for (int k = 0; k < BS; ++k) {
@@ -212,12 +212,14 @@ __global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict_
         A yi[BS];
 #pragma unroll
         for (int r = 0; r < BS; ++r) yi[r] = static_cast<A>(y[i * BS + r]);
-        const std::int64_t ds = dslot[i] * (BS * BS);
+        const std::int64_t ds = dslot[i];
 #pragma unroll
         for (int r = 0; r < BS; ++r) {
             A d = A(0);
 #pragma unroll
-            for (int q = 0; q < BS; ++q) d = dev::dfma(static_cast<A>(dev::load_value(vals, ds + r * BS + q)), yi[q], d);
+            for (int q = 0; q < BS; ++q)
+                d = dev::dfma(static_cast<A>(dev::load_value(vals, dev::dense_index(ds, r * BS + q, BS * BS))), yi[q],
+                              d);
             acc[r] = d * static_cast<A>(scale);
         }
     }
@@ -234,12 +236,13 @@ __global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict_
         A xj[BS];
 #pragma unroll
         for (int q = 0; q < BS; ++q) xj[q] = static_cast<A>(src[j * BS + q]);
-        const std::int64_t b0 = slot * (BS * BS);
 #pragma unroll
         for (int r = 0; r < BS; ++r)
 #pragma unroll
             for (int q = 0; q < BS; ++q)
-                acc[r] = dev::dfma(-(w * static_cast<A>(dev::load_value(vals, b0 + r * BS + q))), xj[q], acc[r]);
+                acc[r] = dev::dfma(-(w * static_cast<A>(dev::load_value(vals, dev::dense_index(slot, r * BS + q,
+                                                                                              BS * BS)))),
+                                   xj[q], acc[r]);
     }
     X* dst = FWD ? y : out;
     const I* blk = inv + i * (BS * BS);
@@ -314,6 +317,132 @@ __global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int32_
         const double d = Fc[diag_pos[row]];
         if (!(__builtin_isfinite(d) && __builtin_fabs(d) > kTol)) atomicExch(flag, static_cast<int>(flag_zero_pivot));
         inv[static_cast<std::int64_t>(c) * a.rows + row] = 1.0 / d;
+    }
+}
+
+// ILU(k) numeric factorization in 64-row chunks (the IKJ order of
+// IluLevelPreconditioner::factorize, preconditioner.hpp:439-485, per row),
+// one wave per (block, component), rows in natural order, one row per lane:
+//   1. a lane loads its row's entries into LDS (rv[slot][lane]), and the
+//      chunk's static program (IluChunkProgram) is staged in LDS;
+//   2. it gathers the final values of earlier chunks that its lower entries
+//      need (pivots u_kk and sources u_kj) in one batch (ga[t][lane]) and
+//      applies those lower entries in column order;
+//   3. the chunk's own lower entries run in 64 steps: at step s row s is
+//      final, and every lane whose row has an entry in column s divides by
+//      row s's pivot and updates its targets from row s's entries (read from
+//      LDS), which keeps each row's column order;
+//   4. the chunk's rows are stored, fenced for the next chunks' gathers.
+// Each row sees the same operations in the same order as the reference's
+// loop, so the factors are bitwise those of the level-scheduled kernel.
+struct IluChunkArgs {
+    const std::int32_t* chunk_first;
+    const std::int64_t* hdr;
+    const std::int32_t* src_a;
+    const std::uint32_t* op_a;
+    const std::uint32_t* op_b;
+    const std::uint16_t* upd_a;
+    const std::uint16_t* upd_b;
+    int kmax, max_src, max_a, max_nua, max_b, max_nub;
+};
+
+inline std::size_t ilu_chunk_lds(const IluChunkProgram& P)
+{
+    return static_cast<std::size_t>(64) * (8 * (P.kmax + P.max_src) + 4 * (P.max_a + P.max_b)
+                                           + 2 * (P.max_a * P.max_nua + P.max_b * P.max_nub));
+}
+
+__global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const std::int64_t* __restrict__ offsets,
+                                                          const std::int64_t* __restrict__ diag_pos,
+                                                          const std::int32_t* __restrict__ row0,
+                                                          const std::int32_t* __restrict__ nrows, int bs,
+                                                          double* __restrict__ F, std::int64_t nnz,
+                                                          std::int64_t rows, double* __restrict__ inv, int* flag)
+{
+    extern __shared__ double lds_d[];
+    const int lane = static_cast<int>(threadIdx.x);
+    const int c = static_cast<int>(blockIdx.x) % bs;
+    const int b = static_cast<int>(blockIdx.x) / bs;
+    double* Fc = F + static_cast<std::int64_t>(c) * nnz;
+    double* rv = lds_d;                                   // [kmax][64]
+    double* ga = rv + P.kmax * 64;                        // [max_src][64]
+    auto* opa = reinterpret_cast<std::uint32_t*>(ga + P.max_src * 64);  // [max_a][64]
+    std::uint32_t* opb = opa + P.max_a * 64;              // [max_b][64]
+    auto* upa = reinterpret_cast<std::uint16_t*>(opb + P.max_b * 64);  // [max_a*max_nua][64]
+    std::uint16_t* upb = upa + P.max_a * P.max_nua * 64;  // [max_b*max_nub][64]
+    for (int ch = P.chunk_first[b]; ch < P.chunk_first[b + 1]; ++ch) {
+        const std::int64_t* h = P.hdr + static_cast<std::int64_t>(ch) * 12;
+        const std::int64_t row_first = h[0];
+        const int cnt = static_cast<int>(h[1]);
+        const std::int64_t src_off = h[2], opa_off = h[4], upa_off = h[6], opb_off = h[8], upb_off = h[10];
+        const int nsrc = static_cast<int>(h[3]), na = static_cast<int>(h[5]), nua = static_cast<int>(h[7]);
+        const int nb = static_cast<int>(h[9]), nub = static_cast<int>(h[11]);
+        const bool active = lane < cnt;
+        std::int64_t base = 0;
+        int len = 0;
+        if (active) {
+            base = offsets[row_first + lane];
+            len = static_cast<int>(offsets[row_first + lane + 1] - base);
+        }
+        // 1. row values and the chunk's program, all loads independent
+        for (int q = 0; q < P.kmax; ++q) rv[q * 64 + lane] = q < len ? Fc[base + q] : 0.0;
+        for (int j = 0; j < na; ++j) opa[j * 64 + lane] = P.op_a[opa_off + j * 64 + lane];
+        for (int j = 0; j < na * nua; ++j) upa[j * 64 + lane] = P.upd_a[upa_off + j * 64 + lane];
+        for (int j = 0; j < nb; ++j) opb[j * 64 + lane] = P.op_b[opb_off + j * 64 + lane];
+        for (int j = 0; j < nb * nub; ++j) upb[j * 64 + lane] = P.upd_b[upb_off + j * 64 + lane];
+        // 2. earlier chunks: gather, then the lower entries in column order
+        for (int t = 0; t < nsrc; ++t) {
+            const std::int32_t idx = P.src_a[src_off + t * 64 + lane];
+            ga[t * 64 + lane] = idx >= 0 ? Fc[idx] : 0.0;
+        }
+        for (int j = 0; j < na; ++j) {
+            const std::uint32_t d = opa[j * 64 + lane];
+            if (d == 0xffffffffu) continue;
+            const int es = static_cast<int>(d & 255u), pt = static_cast<int>((d >> 8) & 255u);
+            const int nu = static_cast<int>((d >> 16) & 255u);
+            const double pivot = ga[pt * 64 + lane];
+            if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+            const double lv = rv[es * 64 + lane] / pivot;
+            rv[es * 64 + lane] = lv;
+            for (int u = 0; u < nu; ++u) {
+                const std::uint16_t w = upa[(j * nua + u) * 64 + lane];
+                const int ts = w & 255, st = w >> 8;
+                rv[ts * 64 + lane] = __builtin_fma(-lv, ga[st * 64 + lane], rv[ts * 64 + lane]);
+            }
+        }
+        // 3. the chunk's own lower entries, one pivot row per step
+        int jb = 0;
+        std::uint32_t d = nb > 0 ? opb[lane] : 0xffffffffu;
+        for (int s = 0; s < cnt; ++s) {
+            __builtin_amdgcn_wave_barrier();
+            if (d != 0xffffffffu && static_cast<int>(d & 255u) == s) {
+                const int es = static_cast<int>((d >> 8) & 255u), ps = static_cast<int>((d >> 16) & 255u);
+                const int nu = static_cast<int>(d >> 24);
+                const double pivot = rv[ps * 64 + s];
+                if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+                const double lv = rv[es * 64 + lane] / pivot;
+                rv[es * 64 + lane] = lv;
+                for (int u = 0; u < nu; ++u) {
+                    const std::uint16_t w = upb[(jb * nub + u) * 64 + lane];
+                    const int ts = w & 255, qs = w >> 8;
+                    rv[ts * 64 + lane] = __builtin_fma(-lv, rv[qs * 64 + s], rv[ts * 64 + lane]);
+                }
+                ++jb;
+                d = jb < nb ? opb[jb * 64 + lane] : 0xffffffffu;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // 4. store; later chunks of this wave gather these values
+        for (int q = 0; q < len; ++q) Fc[base + q] = rv[q * 64 + lane];
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
+    }
+    const std::int64_t r0 = row0[b], n = nrows[b];
+    for (std::int64_t i = lane; i < n; i += 64) {
+        const std::int64_t row = r0 + i;
+        const double dg = Fc[diag_pos[row]];
+        if (!(__builtin_isfinite(dg) && __builtin_fabs(dg) > kTol)) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+        inv[static_cast<std::int64_t>(c) * rows + row] = 1.0 / dg;
     }
 }
 
@@ -543,7 +672,7 @@ __global__ __launch_bounds__(64) void k_tri_chain(TriArgs a, const V* __restrict
 // coefficients go to a packed strictly-lower-triangular LDS tile (position p
 // only depends on steps s < p), half the dense tile, so more one-wave
 // workgroups fit per CU.
-__device__ constexpr int tri_base(int s) { return (s * (127 - s)) / 2; }  // first packed entry of step s
+__host__ __device__ constexpr int tri_base(int s) { return (s * (127 - s)) / 2; }  // first packed entry of step s
 
 // The sweep's workgroup is one wave.  LDS instructions of a wave complete in
 // issue order, and so do its vector memory instructions to one address, so
@@ -578,78 +707,140 @@ __device__ inline X chain_finish(X acc, X d, X rinv, std::integral_constant<int,
 template<TriMode MODE, bool FWD>
 using FinishKind = std::integral_constant<int, MODE == TriMode::ssor ? 2 : (FWD ? 0 : 1)>;
 
+// A 64-bit value from another lane chosen per lane (ds_bpermute); every lane
+// of the wave must execute it.
+template<class X>
+__device__ inline X lane_pick(X v, int src)
+{
+    if constexpr (sizeof(X) == 8) {
+        const long long bits = __double_as_longlong(v);
+        const int lo = __shfl(static_cast<int>(bits), src, 64);
+        const int hi = __shfl(static_cast<int>(bits >> 32), src, 64);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    } else {
+        return __shfl(v, src, 64);
+    }
+}
+
+// One sweep over the ELL copy.  The host classifies every slot (see
+// build_ell), so the chunk set-up is branch-free:
+//   idx >= 0              entry on a chunk before q-1: block-local row of y,
+//                         loaded one chunk ahead (it is final by then);
+//   -2018 <= idx <= -3    in-chunk entry: its value goes to packed tile
+//                         entry -idx - 2;
+//   idx <= -2019          entry on chunk q-1: lane -idx - 2019 of that
+//                         chunk, taken from the lanes' final values by a
+//                         cross-lane shuffle;
+//   idx == -1             padding (value 0, contributes nothing).
+// Every slot's index and value is loaded (padding values are zeros), tile
+// writes of other kinds go to a spare LDS entry, and out-of-chunk
+// contributions are selected, not branched on.  Indices are loaded two
+// chunks ahead, values, own row value, diagonal and older y one chunk ahead.
+// Each row folds its out-of-chunk entries in slot order, then the in-chunk
+// steps.  The steps read their tile coefficients at per-lane addresses
+// computed once per kernel (taddr), so a step is one LDS read, the
+// broadcast and one FMA.
 template<TriMode MODE, bool FWD, class X, class V, int E>
 __device__ __attribute__((always_inline)) inline void ell_sweep(const std::int16_t* __restrict__ idx, const V* __restrict__ val, std::int64_t g0,
-                                 int K, const X* __restrict__ dv, X* y, int ys, X* tri, int nb, int lane)
+                                 const X* __restrict__ dv, X* y, int ys, X* tri, const int (&taddr)[64], int nb,
+                                 int lane)
 {
+    constexpr int kTrash = 1 + tri_base(64);  // spare tile entry for writes that are not in-chunk
     const int chunks = (nb + 63) >> 6;
-    const std::int64_t stride = static_cast<std::int64_t>(K) * 64;  // one chunk's slots
-    int ci[E];
-    X cv[E];
+    constexpr std::int64_t stride = static_cast<std::int64_t>(E) * 64;  // one chunk's slots
+    auto row_of = [&](int pos) { return FWD ? pos : nb - 1 - pos; };
+    auto load_idx = [&](int q, int (&out)[E]) {
+        const std::int64_t base = (g0 + (q < chunks ? q : chunks - 1)) * stride + lane;
+#pragma unroll
+        for (int k = 0; k < E; ++k) out[k] = static_cast<int>(__builtin_nontemporal_load(&idx[base + k * 64]));
+    };
+    auto load_val = [&](int q, X (&out)[E]) {
+        const std::int64_t base = (g0 + (q < chunks ? q : chunks - 1)) * stride + lane;
+#pragma unroll
+        for (int k = 0; k < E; ++k) out[k] = static_cast<X>(vload(val, base + k * 64));
+    };
+    // older entries' y, for the slots of the chunk whose indices are `ix`
+    auto load_old = [&](const int (&ix)[E], X (&out)[E]) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) out[k] = y[static_cast<std::int64_t>(ix[k] > 0 ? ix[k] : 0) * ys];
+    };
+    int ci[E], ni[E];
+    X cv[E], pyo[E];
+    load_idx(0, ci);
+    load_idx(1, ni);
+    load_val(0, cv);
+#pragma unroll
+    for (int k = 0; k < E; ++k) pyo[k] = X(0);  // chunk 0 has no older chunks
+    const int p0 = lane < nb ? lane : nb - 1;
     X cd = X(1);
-    {
-        const std::int64_t base = g0 * stride + lane;
-#pragma unroll
-        for (int k = 0; k < E; ++k) ci[k] = k < K ? static_cast<int>(__builtin_nontemporal_load(&idx[base + k * 64])) : -1;
-#pragma unroll
-        for (int k = 0; k < E; ++k) cv[k] = ci[k] != -1 ? static_cast<X>(vload(val, base + k * 64)) : X(0);
-        if constexpr (MODE == TriMode::ssor || !FWD)
-            if (lane < nb) cd = dv[FWD ? lane : nb - 1 - lane];
-    }
+    if constexpr (MODE == TriMode::ssor || !FWD) cd = dv[row_of(p0)];
+    X cacc = lane < nb ? y[row_of(p0) * ys] : X(0);
+    X yprev = X(0);  // chunk q-1's final value of this lane's row
     for (int q = 0; q < chunks; ++q) {
         const int p = q * 64 + lane;
         const bool valid = p < nb;
-        const int i = FWD ? p : nb - 1 - p;
-        // the row's value and its entries on earlier chunks: all loads first
-        X yo[E];
+        // chunk q+1 (indices of q+2), in flight during this chunk's steps
+        int nni[E];
+        load_idx(q + 2, nni);
+        X ncv[E], npyo[E];
+        load_val(q + 1, ncv);
+        const int pn = p + 64 < nb ? p + 64 : nb - 1;
+        X nacc = y[row_of(pn) * ys], nd = X(1);
+        if constexpr (MODE == TriMode::ssor || !FWD) nd = dv[row_of(pn)];
+        // in-chunk coefficients to the tile
 #pragma unroll
-        for (int k = 0; k < E; ++k) yo[k] = ci[k] >= 0 ? y[static_cast<std::int64_t>(ci[k]) * ys] : X(0);
-        X acc = valid ? y[i * ys] : X(0);
-        // next chunk's indices and diagonal, in flight during this chunk's steps
-        int ni[E];
-        X nd = X(1);
-        const bool more = q + 1 < chunks;
-        const std::int64_t nbase = (g0 + q + 1) * stride + lane;
-#pragma unroll
-        for (int k = 0; k < E; ++k)
-            ni[k] = more && k < K ? static_cast<int>(__builtin_nontemporal_load(&idx[nbase + k * 64])) : -1;
-        if constexpr (MODE == TriMode::ssor || !FWD)
-            if (more && p + 64 < nb) nd = dv[FWD ? p + 64 : nb - 1 - (p + 64)];
-#pragma unroll
-        for (int k = 0; k < E; ++k)
-            if (ci[k] <= -2) tri[1 + tri_base(-ci[k] - 2) + lane + ci[k] + 1] = cv[k];
+        for (int k = 0; k < E; ++k) tri[ci[k] <= -3 && ci[k] >= -2018 ? -ci[k] - 2 : kTrash] = cv[k];
+        X acc = cacc;
         if constexpr (MODE == TriMode::ssor && !FWD) acc = cd * acc;
 #pragma unroll
-        for (int k = 0; k < E; ++k)
-            if (ci[k] >= 0) acc = dev::dfma(-cv[k], yo[k], acc);
+        for (int k = 0; k < E; ++k) {
+            const X sh = lane_pick(yprev, (-ci[k] - 2019) & 63);
+            const X yo = ci[k] >= 0 ? pyo[k] : sh;
+            const X upd = dev::dfma(-cv[k], yo, acc);
+            acc = ci[k] >= 0 || ci[k] <= -2019 ? upd : acc;
+        }
+        // Chunk q-1's results are stored only now, after the wait for the
+        // loads above: stores count with the loads in vmcnt, so a store
+        // issued before them would be waited for there.  The loads of chunk
+        // q+1's entries on chunks before q (q-1 among them) follow the store
+        // (one wave's accesses to one address stay in order); the next wait
+        // for them is a whole chunk of steps later.
+        if (q > 0) y[row_of(p - 64) * ys] = yprev;
+        load_old(ni, npyo);
         X rinv = X(1);
         if constexpr (MODE == TriMode::ssor) rinv = X(1) / cd;
         wave_order();
         // step st: lane st's row is final; every later lane folds it in with
         // its tile coefficient (lanes at or before st read the zero guard, so
         // a finished row keeps its value)
-        auto step = [&](int st) {
-            const X coef = tri[lane > st ? tri_base(st) + lane - st : 0];
-            const X v = lane_value(chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{}), st);
-            acc = dev::dfma(-coef, v, acc);
-        };
         const int steps = nb - q * 64 < 64 ? nb - q * 64 : 64;
         if (steps == 64) {
 #pragma unroll
-            for (int st = 0; st < 64; ++st) step(st);
+            for (int st = 0; st < 64; ++st) {
+                const X coef = tri[taddr[st]];
+                const X v = lane_value(chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{}), st);
+                acc = dev::dfma(-coef, v, acc);
+            }
         } else {
-            for (int st = 0; st < steps; ++st) step(st);
+            for (int st = 0; st < steps; ++st) {
+                const X coef = tri[lane > st ? tri_base(st) + lane - st : 0];
+                const X v = lane_value(chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{}), st);
+                acc = dev::dfma(-coef, v, acc);
+            }
         }
-        if (valid) y[i * ys] = chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{});
+        const X fin = chain_finish(acc, cd, rinv, FinishKind<MODE, FWD>{});
+        yprev = valid ? fin : X(0);
+        if (q + 1 == chunks && valid) y[row_of(p) * ys] = fin;  // the last chunk's results
 #pragma unroll
-        for (int k = 0; k < E; ++k)
-            if (ci[k] <= -2) tri[1 + tri_base(-ci[k] - 2) + lane + ci[k] + 1] = X(0);
-        // next chunk's values: only the slots that hold an entry
+        for (int k = 0; k < E; ++k) tri[ci[k] <= -3 && ci[k] >= -2018 ? -ci[k] - 2 : kTrash] = X(0);
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-            cv[k] = ni[k] != -1 ? static_cast<X>(vload(val, nbase + k * 64)) : X(0);
             ci[k] = ni[k];
+            ni[k] = nni[k];
+            cv[k] = ncv[k];
+            pyo[k] = npyo[k];
         }
+        cacc = nacc;
         cd = nd;
         wave_order();
     }
@@ -660,11 +851,13 @@ struct EllArgs {
     const std::int16_t* fidx;
     const std::int16_t* bidx;
     std::int64_t fslots, bslots;  // values per component: chunks * K * 64
-    int fk, bk;                   // slots per row of each sweep
+    int fk, bk;                   // slots per row of each sweep (both the kernel's E)
 };
 
+// Two waves per SIMD at least (at most 256 VGPRs), so all of a c1 apply's
+// 1,536 one-wave workgroups are resident at once.
 template<TriMode MODE, class X, class V, int E, bool GLOBAL_Y>
-__global__ __launch_bounds__(64) void k_tri_ell(TriArgs a, EllArgs l, const V* __restrict__ fval,
+__global__ __launch_bounds__(64, 2) void k_tri_ell(TriArgs a, EllArgs l, const V* __restrict__ fval,
                                                 const V* __restrict__ bval, const X* __restrict__ dval, const X* in,
                                                 X* out, X scale, const int* stop)
 {
@@ -678,7 +871,7 @@ __global__ __launch_bounds__(64) void k_tri_ell(TriArgs a, EllArgs l, const V* _
     const std::int64_t r0 = a.row0[b];
     const int nb = a.nrows[b];
     const int lane = threadIdx.x;
-    constexpr int kTri = 1 + tri_base(64);  // one guard entry + the packed triangle
+    constexpr int kTri = 2 + tri_base(64);  // guard entry, packed triangle, spare entry
     X* tri = reinterpret_cast<X*>(smem_raw);
     for (int s = lane; s < kTri; s += 64) tri[s] = X(0);
     X* y;
@@ -686,27 +879,71 @@ __global__ __launch_bounds__(64) void k_tri_ell(TriArgs a, EllArgs l, const V* _
     if constexpr (GLOBAL_Y) {
         y = out + r0 * bs + c;
         ys = bs;
-        if (in != out)
-            for (int i = lane; i < nb; i += 64) y[i * ys] = in[(r0 + i) * bs + c];
     } else {
         y = tri + kTri;  // [nb]
         ys = 1;
-        for (int i = lane; i < nb; i += 64) y[i] = in[(r0 + i) * bs + c];
+    }
+    // the right-hand side into y, 16 loads in flight per lane (in and out
+    // may be one vector, so a plain loop would wait for each load in turn)
+    if (!GLOBAL_Y || in != out) {
+        for (int i0 = 0; i0 < nb; i0 += 16 * 64) {
+            const bool full = i0 + 16 * 64 <= nb;  // whole batches store without per-lane branches
+            X v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int i = i0 + u * 64 + lane;
+                v[u] = in[(r0 + (i < nb ? i : nb - 1)) * bs + c];
+            }
+            if (full) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) y[(i0 + u * 64 + lane) * ys] = v[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int i = i0 + u * 64 + lane;
+                    if (i < nb) y[i * ys] = v[u];
+                }
+            }
+        }
     }
     __syncthreads();
     const X* dv = dval + static_cast<std::int64_t>(c) * a.rows + r0;
     const std::int64_t g0 = l.chunk0[b];
-    ell_sweep<MODE, true, X, V, E>(l.fidx, fval + static_cast<std::int64_t>(c) * l.fslots, g0, l.fk, dv, y, ys, tri,
+    // tile entry of step st for this lane (the zero guard at or before st);
+    // kept in registers for both sweeps
+    int taddr[64];
+#pragma unroll
+    for (int st = 0; st < 64; ++st) {
+        taddr[st] = lane > st ? tri_base(st) + lane - st : 0;
+        __asm__ volatile("" : "+v"(taddr[st]));
+    }
+    ell_sweep<MODE, true, X, V, E>(l.fidx, fval + static_cast<std::int64_t>(c) * l.fslots, g0, dv, y, ys, tri, taddr,
                                    nb, lane);
     __syncthreads();
-    ell_sweep<MODE, false, X, V, E>(l.bidx, bval + static_cast<std::int64_t>(c) * l.bslots, g0, l.bk, dv, y, ys, tri,
-                                    nb, lane);
+    ell_sweep<MODE, false, X, V, E>(l.bidx, bval + static_cast<std::int64_t>(c) * l.bslots, g0, dv, y, ys, tri,
+                                    taddr, nb, lane);
     const bool scaled = MODE == TriMode::ssor && scale != X(1);
     if (GLOBAL_Y && !scaled) return;
-    for (int i = lane; i < nb; i += 64) {
-        X v = y[i * ys];
-        if (scaled) v = v * scale;
-        out[(r0 + i) * bs + c] = v;
+    __syncthreads();
+    for (int i0 = 0; i0 < nb; i0 += 16 * 64) {
+        const bool full = i0 + 16 * 64 <= nb;
+        X v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = i0 + u * 64 + lane;
+            v[u] = y[(i < nb ? i : nb - 1) * ys];
+            if (scaled) v[u] = v[u] * scale;
+        }
+        if (full) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) out[(r0 + i0 + u * 64 + lane) * bs + c] = v[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int i = i0 + u * 64 + lane;
+                if (i < nb) out[(r0 + i) * bs + c] = v[u];
+            }
+        }
     }
 }
 
@@ -787,9 +1024,12 @@ struct TriHost {
 std::size_t lds_limit(owtk_precond pc);
 
 // ELL copy of a chain layout (see k_tri_ell): slot k of position p holds the
-// p-th row's k-th chain-order entry, encoded as its block-local column (entry
-// on an earlier chunk) or -(step + 2) (in-chunk entry).  Needs block-local
-// columns below 2^15 and at most 16 entries per row.
+// p-th row's k-th chain-order entry, encoded by where its value comes from:
+// its block-local column (entry on a chunk before the previous one),
+// -(t + 2) (in-chunk entry finished at step s, t = tri_base(s) + lane - s its
+// packed tile entry, 1..2016), -(l + 2019) (entry on the previous chunk,
+// finished by lane l) or -1 (padding).  Needs block-local columns below 2^15
+// and at most 16 entries per row.
 void build_ell(TriSchedule& tri, const std::vector<std::int64_t>& fptr, const std::vector<std::int32_t>& fnout,
                const std::vector<std::int32_t>& fcol, const std::vector<std::int64_t>& bptr,
                const std::vector<std::int32_t>& bnout, const std::vector<std::int32_t>& bcol)
@@ -808,8 +1048,10 @@ void build_ell(TriSchedule& tri, const std::vector<std::int64_t>& fptr, const st
         for (std::int64_t i = 0; i < n; ++i) k = std::max(k, ptr[i + 1] - ptr[i]);
         return k;
     };
-    const std::int64_t fk = kmax(fptr), bk = kmax(bptr);
-    if (fk > 16 || bk > 16 || chunks > 0x7fffffff) return;
+    const std::int64_t kneed = std::max(kmax(fptr), kmax(bptr));
+    if (kneed > 16 || chunks > 0x7fffffff) return;
+    // both sweeps padded to the kernel's slot count (4, 6, 8 or 16)
+    const std::int64_t fk = kneed <= 4 ? 4 : kneed <= 6 ? 6 : kneed <= 8 ? 8 : 16, bk = fk;
     for (int dir = 0; dir < 2; ++dir) {
         const auto& ptr = dir == 0 ? fptr : bptr;
         const auto& nout = dir == 0 ? fnout : bnout;
@@ -826,11 +1068,16 @@ void build_ell(TriSchedule& tri, const std::vector<std::int64_t>& fptr, const st
                 const std::int64_t e0 = ptr[r0 + p], e1 = ptr[r0 + p + 1], no = nout[r0 + p];
                 for (std::int64_t e = e0; e < e1; ++e) {
                     const std::int64_t c = col[e];
+                    const std::int64_t pc = dir == 0 ? c : nb - 1 - c;  // position of the column in the sweep
                     std::int64_t code = c;
                     if (e - e0 >= no) {
-                        const std::int64_t step = dir == 0 ? c - cs : nb - 1 - cs - c;
+                        const std::int64_t step = pc - cs;
                         if (step < 0 || step >= lane) throw std::logic_error("chain layout: in-chunk entry order");
-                        code = -(step + 2);
+                        code = -(tri_base(static_cast<int>(step)) + lane - step + 2);
+                    } else if (pc >= cs) {
+                        throw std::logic_error("chain layout: out-of-chunk entry inside the chunk");
+                    } else if (pc >= cs - 64) {
+                        code = -(pc - (cs - 64) + 2019);
                     }
                     const std::size_t slot = static_cast<std::size_t>((g * K + (e - e0)) * 64 + lane);
                     idx[slot] = static_cast<std::int16_t>(code);
@@ -1304,6 +1551,128 @@ void ssor_numeric(owtk_precond pc)
 }
 
 // ---------------------------------------------------------------------- ILU --
+// Program of k_ilu_factor_chunks (IluChunkProgram) from the factor pattern
+// and the update lists.  Left disabled (the level-scheduled k_ilu_factor
+// runs) when a row has more than 64 entries or a count does not fit the
+// program's byte fields.
+void build_ilu_chunks(owtk_precond pc, const std::vector<std::int64_t>& upd_ptr,
+                      const std::vector<std::int64_t>& upd_target, const std::vector<std::int64_t>& upd_upper)
+{
+    auto& f = pc->ilu;
+    auto& P = f.chunks;
+    const auto& tri = pc->tri;
+    P.enabled = false;
+    if (f.nnz > 0x7fffffffLL) return;
+    int kmax = 1;
+    for (std::int64_t row = 0; row < tri.rows; ++row)
+        kmax = std::max<int>(kmax, static_cast<int>(f.h_offsets[row + 1] - f.h_offsets[row]));
+    if (kmax > 64) return;
+    std::vector<std::int32_t> chunk_first(tri.blocks + 1, 0);
+    std::vector<std::int64_t> hdr;
+    std::vector<std::int32_t> src_a;
+    std::vector<std::uint32_t> op_a, op_b;
+    std::vector<std::uint16_t> upd_a, upd_b;
+    int max_src = 0, max_a = 0, max_nua = 0, max_b = 0, max_nub = 0;
+    struct OpA { int es, pt; std::vector<std::pair<int, int>> upd; };
+    struct OpB { int step, es, ps; std::vector<std::pair<int, int>> upd; };
+    std::int64_t chunks = 0;
+    for (std::int64_t b = 0; b < tri.blocks; ++b) {
+        chunk_first[b] = static_cast<std::int32_t>(chunks);
+        const std::int64_t r0 = tri.h_row0[b], nb = tri.h_nrows[b];
+        for (std::int64_t c0 = 0; c0 < nb; c0 += 64, ++chunks) {
+            const int cnt = static_cast<int>(std::min<std::int64_t>(64, nb - c0));
+            const std::int64_t first = r0 + c0;
+            std::vector<std::vector<std::int32_t>> srcs(64);
+            std::vector<std::vector<OpA>> as(64);
+            std::vector<std::vector<OpB>> bs_(64);
+            for (int lane = 0; lane < cnt; ++lane) {
+                const std::int64_t row = first + lane;
+                const std::int64_t rb = f.h_offsets[row];
+                for (std::int64_t e = rb; e < f.h_diag[row]; ++e) {
+                    const std::int64_t k = f.h_columns[e];
+                    if (k < first) {
+                        OpA op{static_cast<int>(e - rb), static_cast<int>(srcs[lane].size()), {}};
+                        srcs[lane].push_back(static_cast<std::int32_t>(f.h_diag[k]));
+                        for (std::int64_t u = upd_ptr[e]; u < upd_ptr[e + 1]; ++u) {
+                            op.upd.push_back({static_cast<int>(upd_target[u] - rb), static_cast<int>(srcs[lane].size())});
+                            srcs[lane].push_back(static_cast<std::int32_t>(upd_upper[u]));
+                        }
+                        as[lane].push_back(std::move(op));
+                    } else {
+                        const std::int64_t kb = f.h_offsets[k];
+                        OpB op{static_cast<int>(k - first), static_cast<int>(e - rb), static_cast<int>(f.h_diag[k] - kb), {}};
+                        for (std::int64_t u = upd_ptr[e]; u < upd_ptr[e + 1]; ++u)
+                            op.upd.push_back({static_cast<int>(upd_target[u] - rb), static_cast<int>(upd_upper[u] - kb)});
+                        bs_[lane].push_back(std::move(op));
+                    }
+                }
+            }
+            int nsrc = 0, na = 0, nua = 0, nbo = 0, nub = 0;
+            for (int lane = 0; lane < cnt; ++lane) {
+                nsrc = std::max<int>(nsrc, static_cast<int>(srcs[lane].size()));
+                na = std::max<int>(na, static_cast<int>(as[lane].size()));
+                nbo = std::max<int>(nbo, static_cast<int>(bs_[lane].size()));
+                for (const auto& o : as[lane]) nua = std::max<int>(nua, static_cast<int>(o.upd.size()));
+                for (const auto& o : bs_[lane]) nub = std::max<int>(nub, static_cast<int>(o.upd.size()));
+            }
+            if (nsrc > 255 || nua > 255 || nub > 255) return;
+            hdr.insert(hdr.end(), {first, cnt, static_cast<std::int64_t>(src_a.size()), nsrc,
+                                   static_cast<std::int64_t>(op_a.size()), na, static_cast<std::int64_t>(upd_a.size()),
+                                   nua, static_cast<std::int64_t>(op_b.size()), nbo,
+                                   static_cast<std::int64_t>(upd_b.size()), nub});
+            const std::size_t s0 = src_a.size(), a0 = op_a.size(), ua0 = upd_a.size(), b0 = op_b.size(),
+                              ub0 = upd_b.size();
+            src_a.resize(s0 + static_cast<std::size_t>(nsrc) * 64, -1);
+            op_a.resize(a0 + static_cast<std::size_t>(na) * 64, 0xffffffffu);
+            upd_a.resize(ua0 + static_cast<std::size_t>(na) * nua * 64, 0);
+            op_b.resize(b0 + static_cast<std::size_t>(nbo) * 64, 0xffffffffu);
+            upd_b.resize(ub0 + static_cast<std::size_t>(nbo) * nub * 64, 0);
+            for (int lane = 0; lane < cnt; ++lane) {
+                for (std::size_t t2 = 0; t2 < srcs[lane].size(); ++t2) src_a[s0 + t2 * 64 + lane] = srcs[lane][t2];
+                for (std::size_t j = 0; j < as[lane].size(); ++j) {
+                    const auto& o = as[lane][j];
+                    op_a[a0 + j * 64 + lane] = static_cast<std::uint32_t>(o.es) | (static_cast<std::uint32_t>(o.pt) << 8)
+                        | (static_cast<std::uint32_t>(o.upd.size()) << 16);
+                    for (std::size_t u = 0; u < o.upd.size(); ++u)
+                        upd_a[ua0 + (j * nua + u) * 64 + lane] =
+                            static_cast<std::uint16_t>(o.upd[u].first | (o.upd[u].second << 8));
+                }
+                for (std::size_t j = 0; j < bs_[lane].size(); ++j) {
+                    const auto& o = bs_[lane][j];
+                    op_b[b0 + j * 64 + lane] = static_cast<std::uint32_t>(o.step) | (static_cast<std::uint32_t>(o.es) << 8)
+                        | (static_cast<std::uint32_t>(o.ps) << 16) | (static_cast<std::uint32_t>(o.upd.size()) << 24);
+                    for (std::size_t u = 0; u < o.upd.size(); ++u)
+                        upd_b[ub0 + (j * nub + u) * 64 + lane] =
+                            static_cast<std::uint16_t>(o.upd[u].first | (o.upd[u].second << 8));
+                }
+            }
+            max_src = std::max(max_src, nsrc);
+            max_a = std::max(max_a, na);
+            max_nua = std::max(max_nua, nua);
+            max_b = std::max(max_b, nbo);
+            max_nub = std::max(max_nub, nub);
+        }
+    }
+    chunk_first[tri.blocks] = static_cast<std::int32_t>(chunks);
+    if (std::max({src_a.size(), op_a.size(), upd_a.size(), op_b.size(), upd_b.size()}) > 0x7fffffffULL) return;
+    P.kmax = kmax;
+    P.max_src = max_src;
+    P.max_a = max_a;
+    P.max_nua = max_nua;
+    P.max_b = max_b;
+    P.max_nub = max_nub;
+    const std::size_t lds = ilu_chunk_lds(P);
+    if (lds > 64 * 1024) return;
+    upload(P.chunk_first, chunk_first);
+    upload(P.hdr, hdr);
+    upload(P.src_a, src_a);
+    upload(P.op_a, op_a);
+    upload(P.op_b, op_b);
+    upload(P.upd_a, upd_a);
+    upload(P.upd_b, upd_b);
+    P.enabled = true;
+}
+
 void ilu_setup(owtk_precond pc)
 {
     owtk_matrix m = pc->m;
@@ -1398,6 +1767,7 @@ void ilu_setup(owtk_precond pc)
         t.b_off[row + 1] = static_cast<std::int64_t>(t.b_col.size());
     }
     build_schedule(pc, t);
+    build_ilu_chunks(pc, upd_ptr, upd_target, upd_upper);
     const int bs = m->bs;
     pc->tri.fval64.allocate(std::max<std::int64_t>(pc->tri.fnnz * bs, 1));
     pc->tri.bval64.allocate(std::max<std::int64_t>(pc->tri.bnnz * bs, 1));
@@ -1416,9 +1786,19 @@ void ilu_numeric(owtk_precond pc)
                                                       m->v64.get(), m->d64.get(), 1.0);
     OWTK_LAUNCH_CHECK();
     const unsigned int groups = static_cast<unsigned int>(t.blocks * bs);
-    k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), t.frow_lvl.get(), f.offsets.get(), f.diag_pos.get(), f.upd_ptr.get(),
-                                        f.upd_target.get(), f.upd_upper.get(), f.columns.get(), f.values.get(),
-                                        f.nnz, t.dval64.get(), pc->flag.get());
+    if (f.chunks.enabled) {
+        const auto& P = f.chunks;
+        const IluChunkArgs args{P.chunk_first.get(), P.hdr.get(), P.src_a.get(), P.op_a.get(), P.op_b.get(),
+                                P.upd_a.get(), P.upd_b.get(), P.kmax, P.max_src, P.max_a, P.max_nua, P.max_b,
+                                P.max_nub};
+        k_ilu_factor_chunks<<<groups, 64, ilu_chunk_lds(P), st>>>(args, f.offsets.get(), f.diag_pos.get(),
+                                                                  t.row0.get(), t.nrows.get(), bs, f.values.get(),
+                                                                  f.nnz, t.rows, t.dval64.get(), pc->flag.get());
+    } else {
+        k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), t.frow_lvl.get(), f.offsets.get(), f.diag_pos.get(),
+                                            f.upd_ptr.get(), f.upd_target.get(), f.upd_upper.get(), f.columns.get(),
+                                            f.values.get(), f.nnz, t.dval64.get(), pc->flag.get());
+    }
     OWTK_LAUNCH_CHECK();
     int flag = 0;
     read_flag(pc, &flag);
@@ -1459,7 +1839,7 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
             efv = t.fell16.get();
             ebv = t.bell16.get();
         }
-        const std::size_t tri_lds = static_cast<std::size_t>(1 + 64 * 63 / 2) * sizeof(X);
+        const std::size_t tri_lds = static_cast<std::size_t>(2 + 64 * 63 / 2) * sizeof(X);
         const bool global_y = need + tri_lds > kChainLdsBudget;
         const std::size_t lds = global_y ? tri_lds : need + tri_lds;
         const int k = std::max(t.fk, t.bk);
@@ -1471,7 +1851,9 @@ void tri_launch(owtk_precond pc, const V* fv, const V* bv, const X* dv, const X*
             if (global_y) launch(e, std::true_type{});
             else launch(e, std::false_type{});
         };
-        if (k <= 8) pick(std::integral_constant<int, 8>{});
+        if (k == 4) pick(std::integral_constant<int, 4>{});
+        else if (k == 6) pick(std::integral_constant<int, 6>{});
+        else if (k == 8) pick(std::integral_constant<int, 8>{});
         else pick(std::integral_constant<int, 16>{});
     } else if (t.chain) {
         const TriArgs args = tri_args(t);
@@ -1654,6 +2036,7 @@ int owtk_precond_create(owtk_matrix m, const owtk_precond_params* params, owtk_p
         m->ctx->activate();
         auto pc = std::make_unique<owtk_precond_s>();
         pc->m = m;
+        pc->ctx = m->ctx;
         pc->kind = params->kind;
         pc->storage = params->storage;
         pc->omega = params->omega;
@@ -1782,8 +2165,8 @@ int owtk_precond_destroy(owtk_precond pc)
 {
     return guarded([&] {
         if (!pc) return;
-        pc->m->ctx->activate();
-        pc->m->ctx->sync();
+        pc->ctx->activate();
+        pc->ctx->sync();
         if (pc->kind == OWTK_PC_LEARNED) learned_release(pc);
         delete pc;
     });

@@ -61,10 +61,32 @@ struct TriSchedule {
     DeviceArray<std::uint16_t> fell16, bell16;
 };
 
+// Program of the chunked numeric factorization (k_ilu_factor_chunks): rows
+// in 64-row chunks, one per lane.  For each chunk:
+//   srcA  [nsrc][64]  factor entries on earlier chunks a lane reads (-1 pad)
+//   opA   [nA][64]    lower entries whose pivot row lies on an earlier chunk,
+//                     in column order: eslot | pivot source << 8 | updates << 16
+//   updA  [nA*nuA][64] updates of those: target slot | source << 8
+//   opB   [nB][64]    lower entries whose pivot row lies in the chunk, in column
+//                     order: step | eslot << 8 | pivot slot << 16 | updates << 24
+//   updB  [nB*nuB][64] target slot | source slot (in the pivot row) << 8
+// (slots are positions in a row's factor entries; 0xffffffff pads opA/opB)
+// hdr per chunk: row of lane 0, rows, and offset and count of each array.
+struct IluChunkProgram {
+    bool enabled = false;
+    int kmax = 0, max_src = 0, max_a = 0, max_nua = 0, max_b = 0, max_nub = 0;
+    DeviceArray<std::int32_t> chunk_first;  // per block (blocks + 1)
+    DeviceArray<std::int64_t> hdr;          // per chunk: 12 values (see k_ilu_factor_chunks)
+    DeviceArray<std::int32_t> src_a;
+    DeviceArray<std::uint32_t> op_a, op_b;
+    DeviceArray<std::uint16_t> upd_a, upd_b;
+};
+
 struct IluFactor {
     // factor pattern, reference accessor layout (global rows, sorted columns)
     std::vector<std::int64_t> h_offsets, h_columns, h_diag;
     std::int64_t nnz = 0;
+    IluChunkProgram chunks;
     DeviceArray<std::int64_t> diag_pos;      // factor entry of each row's diagonal
     DeviceArray<std::int64_t> src;           // matrix gather source per factor entry (-1: fill)
     DeviceArray<std::int64_t> columns;       // global column per factor entry
@@ -78,6 +100,9 @@ struct IluFactor {
 
 struct owtk_precond_s {
     owtk_matrix m = nullptr;
+    // the context, kept apart from the matrix: destroy may run after the
+    // matrix is gone (a garbage collector finalizes in no fixed order)
+    owtk_context ctx = nullptr;
     int kind = OWTK_PC_IDENTITY;
     int storage = OWTK_F64;
     double omega = 1.0;

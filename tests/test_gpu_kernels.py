@@ -338,3 +338,21 @@ def test_ilu_factor_file_matches_oracle(ctx, tmp_path):
     assert np.array_equal(off, ro) and np.array_equal(col, rc_)
     assert np.max(np.abs(val - rv)) <= 1e-13 * np.max(np.abs(rv))
     assert np.max(np.abs(inv - ri)) <= 1e-13 * np.max(np.abs(ri))
+
+
+def test_destroy_order_independent(ctx, c0):
+    """A garbage collector finalizes objects of a reference cycle in no fixed
+    order: destroying a matrix before its preconditioner or multigrid
+    hierarchy must not touch the freed matrix."""
+    s = c0
+    m = owtk.Matrix.split(ctx, s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    pcs = [owtk.Preconditioner(m, kind, block_nodes=512) for kind in (owtk.PC_JACOBI, owtk.PC_SSOR, owtk.PC_ILU)]
+    co, cc, cv = s.csr()
+    mc = owtk.Matrix.csr(ctx, s.nodes, 0, 3, co, cc, cv)
+    mg = owtk.Multigrid(mc)
+    m.close()
+    mc.close()
+    for pc in pcs:
+        pc.close()
+    mg.close()
+    ctx.synchronize()
