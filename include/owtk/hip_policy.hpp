@@ -15,6 +15,9 @@
 #include <concepts>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -353,10 +356,242 @@ private:
     owtk_precond pc_ = nullptr;
 };
 
-// Options/result conversion from and to the reference's SolverOptions /
-// SolverResult (core.hpp:24-146): same field names, same enum values.
+// ---------------------------------------------------- reference-shaped solvers
+// SolverStatus, GmresOrthogonalization and BreakdownReason (core.hpp:24-52)
+// with the reference's values: the C ABI carries them as integers.
+enum class SolverStatus : std::int32_t { converged, maximum_iterations, breakdown, diverged, invalid_input };
+enum class GmresOrthogonalization : std::int32_t {
+    iterated_classical_gram_schmidt,
+    adaptive_classical_gram_schmidt,
+    one_synchronization_classical_gram_schmidt,
+};
+enum class BreakdownReason : std::int32_t {
+    none,
+    arnoldi_invariant_subspace,
+    singular_projected_system,
+    biorthogonality_loss,
+    alpha_denominator,
+    omega_denominator,
+    omega_zero,
+    idr_shadow_space,
+    idr_small_system,
+    non_finite_scalar,
+};
+
+// SolverOptions<T> (core.hpp:88-111): the same fields and defaults.  The
+// application test receives the device solution (count owned FP64 values).
+template<class T = double>
+struct SolverOptions {
+    T relative_tolerance = T(1e-8);
+    T absolute_tolerance = T(0);
+    T breakdown_tolerance = T(64) * std::numeric_limits<T>::epsilon();
+    std::size_t maximum_iterations = 1000;
+    std::size_t convergence_check_interval = 10;
+    std::size_t restart = 30;
+    std::size_t residual_replacement_interval = 50;
+    std::size_t idr_shadow_space = 4;
+    T relaxation = T(1);
+    bool verify_true_residual = true;
+    bool collect_timings = false;
+    std::function<bool(std::size_t, const T*, std::size_t)> convergence_test;
+    GmresOrthogonalization gmres_orthogonalization = GmresOrthogonalization::iterated_classical_gram_schmidt;
+};
+
+// SolverResult<T> (core.hpp:118-146): the same fields.  It converts to the
+// reference's SolverResult<T> (or any type with those fields), so
+// `owt::krylov::SolverResult<double> r = bicgstab(...)` compiles unchanged.
+template<class T = double>
+struct SolverResult {
+    SolverStatus status = SolverStatus::invalid_input;
+    BreakdownReason breakdown_reason = BreakdownReason::none;
+    std::size_t iterations = 0;
+    std::size_t operator_applications = 0;
+    std::size_t preconditioner_applications = 0;
+    std::size_t global_reductions = 0;
+    std::size_t reorthogonalizations = 0;
+    std::size_t arnoldi_norm_verifications = 0;
+    T initial_residual_norm = std::numeric_limits<T>::quiet_NaN();
+    T recursive_residual_norm = std::numeric_limits<T>::quiet_NaN();
+    T true_residual_norm = std::numeric_limits<T>::quiet_NaN();
+    T relative_residual_norm = std::numeric_limits<T>::quiet_NaN();
+    bool converged_by_application = false;
+    double seconds = 0;
+    // iterative refinement (fgmres_refined) only
+    std::size_t refinement_steps = 0, inner_iterations = 0, fallback_refinements = 0;
+
+    [[nodiscard]] constexpr bool converged() const noexcept { return status == SolverStatus::converged; }
+    [[nodiscard]] double solve_seconds() const noexcept { return seconds; }
+
+    template<class R>
+        requires(!std::is_same_v<R, SolverResult> && requires(R r) {
+            r.status;
+            r.breakdown_reason;
+            r.iterations;
+            r.relative_residual_norm;
+        })
+    operator R() const
+    {
+        R r{};
+        r.status = static_cast<decltype(r.status)>(status);
+        r.breakdown_reason = static_cast<decltype(r.breakdown_reason)>(breakdown_reason);
+        r.iterations = iterations;
+        r.operator_applications = operator_applications;
+        r.preconditioner_applications = preconditioner_applications;
+        r.global_reductions = global_reductions;
+        r.reorthogonalizations = reorthogonalizations;
+        r.arnoldi_norm_verifications = arnoldi_norm_verifications;
+        r.initial_residual_norm = initial_residual_norm;
+        r.recursive_residual_norm = recursive_residual_norm;
+        r.true_residual_norm = true_residual_norm;
+        r.relative_residual_norm = relative_residual_norm;
+        if constexpr (requires { r.converged_by_application; }) r.converged_by_application = converged_by_application;
+        if constexpr (requires { r.timings->solve_seconds; }) {
+            if (seconds > 0) {
+                using Timings = typename std::remove_cvref_t<decltype(r.timings)>::element_type;
+                r.timings = std::make_shared<Timings>();
+                r.timings->solve_seconds = seconds;
+            }
+        }
+        return r;
+    }
+};
+
+inline SolverResult<double> from_c(const owtk_solver_result& c)
+{
+    SolverResult<double> r;
+    r.status = static_cast<SolverStatus>(c.status);
+    r.breakdown_reason = static_cast<BreakdownReason>(c.breakdown_reason);
+    r.iterations = c.iterations;
+    r.operator_applications = c.operator_applications;
+    r.preconditioner_applications = c.preconditioner_applications;
+    r.global_reductions = c.global_reductions;
+    r.reorthogonalizations = c.reorthogonalizations;
+    r.arnoldi_norm_verifications = c.arnoldi_norm_verifications;
+    r.initial_residual_norm = c.initial_residual_norm;
+    r.recursive_residual_norm = c.recursive_residual_norm;
+    r.true_residual_norm = c.true_residual_norm;
+    r.relative_residual_norm = c.relative_residual_norm;
+    r.converged_by_application = c.converged_by_application != 0;
+    r.seconds = c.solve_seconds;
+    r.refinement_steps = c.refinement_steps;
+    r.inner_iterations = c.inner_iterations;
+    r.fallback_refinements = c.fallback_refinements;
+    return r;
+}
+
+// The reductions (reduction.hpp:103-299): how the device reduces a solve's
+// inner products and norms (owtk_reduction).
+struct SerialReduction {
+    static constexpr owtk_reduction kind = OWTK_REDUCTION_SERIAL;
+};
+struct CompensatedSerialReduction {
+    static constexpr owtk_reduction kind = OWTK_REDUCTION_COMPENSATED;
+};
+struct MixedPrecisionSerialReduction {
+    static constexpr owtk_reduction kind = OWTK_REDUCTION_MIXED_PRECISION;
+};
+
+// IdentityPreconditioner (preconditioner.hpp:17-23)
+struct IdentityPreconditioner {};
+
+// ArnoldiSnapshot<double> (core.hpp:462-534) kept on the device: the last
+// FGMRES cycle's unrotated Hessenberg columns and preconditioned basis
+// vectors.  preconditioned_basis(column) is the device address of the
+// vector (owned + ghost values, ghosts zero).
+class ArnoldiSnapshot {
+public:
+    explicit ArnoldiSnapshot(Context& ctx) { check(owtk_arnoldi_snapshot_create(ctx.get(), &snap_)); }
+    ~ArnoldiSnapshot()
+    {
+        if (snap_) (void)owtk_arnoldi_snapshot_destroy(snap_);
+    }
+    ArnoldiSnapshot(const ArnoldiSnapshot&) = delete;
+    ArnoldiSnapshot& operator=(const ArnoldiSnapshot&) = delete;
+    owtk_arnoldi_snapshot get() const noexcept { return snap_; }
+    void clear() { check(owtk_arnoldi_snapshot_clear(snap_)); }
+    std::size_t columns() const
+    {
+        std::uint64_t c = 0;
+        check(owtk_arnoldi_snapshot_columns(snap_, &c));
+        return static_cast<std::size_t>(c);
+    }
+    double h(std::size_t row, std::size_t column) const
+    {
+        const std::size_t c = columns();
+        if (column >= c || row > c) throw std::out_of_range("Arnoldi snapshot entry");
+        double v = 0;
+        check(owtk_arnoldi_snapshot_h(snap_, row, column, &v));
+        return v;
+    }
+    const double* preconditioned_basis(std::size_t column) const
+    {
+        if (column >= columns()) throw std::out_of_range("Arnoldi snapshot basis");
+        const double* p = nullptr;
+        check(owtk_arnoldi_snapshot_basis(snap_, column, &p));
+        return p;
+    }
+
+private:
+    owtk_arnoldi_snapshot snap_ = nullptr;
+};
+
+namespace detail {
+
+inline owtk_precond pc_handle(const Preconditioner& p) noexcept { return p.get(); }
+inline owtk_precond pc_handle(const Preconditioner* p) noexcept { return p ? p->get() : nullptr; }
+inline owtk_precond pc_handle(IdentityPreconditioner) noexcept { return nullptr; }
+inline owtk_precond pc_handle(std::nullptr_t) noexcept { return nullptr; }
+
+template<class R>
+constexpr owtk_reduction reduction_of()
+{
+    static_assert(requires { R::kind; },
+                  "reduction: owtk::hip::SerialReduction, CompensatedSerialReduction or MixedPrecisionSerialReduction");
+    return R::kind;
+}
+
+template<class F>
+struct function_args;
+template<class R, class A1, class A2>
+struct function_args<std::function<R(A1, A2)>> {
+    using second = std::remove_cvref_t<A2>;
+};
+
+// The application test behind the C callback.  A test over a device
+// pointer (owtk::hip::SolverOptions) gets the device solution; a test over
+// a host vector type (the reference's BlockVector<T>, constructed as
+// (owned, ghost, block)) gets a host copy of the owned values.
 template<class Options>
-owtk_solver_options to_c(const Options& o)
+struct TestBridge {
+    const Options* options = nullptr;
+    Context* ctx = nullptr;
+    std::size_t owned_nodes = 0, ghost_nodes = 0, block = 0;
+    std::exception_ptr error;
+
+    static int call(void* self_, std::uint64_t iteration, const double* device_solution, std::size_t count)
+    {
+        auto* self = static_cast<TestBridge*>(self_);
+        try {
+            const auto& test = self->options->convergence_test;
+            using F = std::remove_cvref_t<decltype(test)>;
+            if constexpr (std::is_invocable_r_v<bool, F, std::size_t, const double*, std::size_t>) {
+                return test(static_cast<std::size_t>(iteration), device_solution, count) ? 1 : 0;
+            } else {
+                using HostVector = typename function_args<F>::second;
+                HostVector host(self->owned_nodes, self->ghost_nodes, self->block);
+                check(owtk_memcpy_dtoh(self->ctx->get(), host.data(), device_solution, count * sizeof(double)));
+                return test(static_cast<std::size_t>(iteration), host) ? 1 : 0;
+            }
+        } catch (...) {
+            self->error = std::current_exception();
+            return 1;  // stop the solve; the exception is rethrown after it
+        }
+    }
+};
+
+// SolverOptions (ours or the reference's) to the C struct.
+template<class Options>
+owtk_solver_options to_c(const Options& o, owtk_reduction reduction = OWTK_REDUCTION_SERIAL)
 {
     owtk_solver_options c;
     owtk_solver_options_default(&c);
@@ -370,106 +605,204 @@ owtk_solver_options to_c(const Options& o)
     c.gmres_orthogonalization = static_cast<std::int32_t>(o.gmres_orthogonalization);
     c.collect_timings = o.collect_timings ? 1 : 0;
     if constexpr (requires { o.idr_shadow_space; }) c.idr_shadow_space = o.idr_shadow_space;
+    if constexpr (requires { o.relaxation; }) c.relaxation = static_cast<double>(o.relaxation);
+    if constexpr (requires { o.verify_true_residual; }) c.verify_true_residual = o.verify_true_residual ? 1 : 0;
+    c.reduction = reduction;
     return c;
 }
 
-template<class Result>
-void from_c(const owtk_solver_result& c, Result& r)
+// One solve through a C entry point with the options' test bridged.
+template<class Options, class Run>
+SolverResult<double> run_solver(DeviceMatrix& op, const DeviceVector<double>& rhs, DeviceVector<double>& solution,
+                                const Options& options, owtk_reduction reduction, Run&& run)
 {
-    using Status = decltype(r.status);
-    using Reason = decltype(r.breakdown_reason);
-    r.status = static_cast<Status>(c.status);
-    r.breakdown_reason = static_cast<Reason>(c.breakdown_reason);
-    r.iterations = c.iterations;
-    r.operator_applications = c.operator_applications;
-    r.preconditioner_applications = c.preconditioner_applications;
-    r.global_reductions = c.global_reductions;
-    r.reorthogonalizations = c.reorthogonalizations;
-    r.arnoldi_norm_verifications = c.arnoldi_norm_verifications;
-    r.initial_residual_norm = c.initial_residual_norm;
-    r.recursive_residual_norm = c.recursive_residual_norm;
-    r.true_residual_norm = c.true_residual_norm;
-    r.relative_residual_norm = c.relative_residual_norm;
+    if (rhs.size() < op.owned_size() || solution.size() < op.local_size())
+        throw std::invalid_argument("solver vector layout mismatch");
+    owtk_solver_options c = to_c(options, reduction);
+    TestBridge<Options> bridge;
+    if constexpr (requires { static_cast<bool>(options.convergence_test); }) {
+        if (static_cast<bool>(options.convergence_test)) {
+            bridge.options = &options;
+            bridge.ctx = &op.context();
+            bridge.owned_nodes = static_cast<std::size_t>(op.owned_nodes());
+            bridge.ghost_nodes = static_cast<std::size_t>(op.ghost_nodes());
+            bridge.block = static_cast<std::size_t>(op.block_size());
+            c.convergence_test = &TestBridge<Options>::call;
+            c.convergence_test_data = &bridge;
+        }
+    }
+    owtk_solver_result r;
+    check(run(c, r));
+    if (bridge.error) std::rethrow_exception(bridge.error);
+    return from_c(r);
 }
 
-// Device-resident solves on device vectors.
-inline owtk_solver_result bicgstab(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
-                                   DeviceVector<double>& x, const owtk_solver_options& o)
+} // namespace detail
+
+// bicgstab(op, b, x, options, pc, reduction, workspace)
+// (krylov_solvers.hpp:542-553) on device vectors, found by argument-dependent
+// lookup on the owtk::hip types: a caller's unqualified
+// `bicgstab(op, b, x, options, pc)` keeps its argument order.  The device
+// keeps its own workspace, so `workspace` is accepted and not used.
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t>
+SolverResult<double> bicgstab(DeviceMatrix& op, const DeviceVector<double>& rhs, DeviceVector<double>& solution,
+                              const Options& options = Options{}, Pc&& preconditioner = Pc{},
+                              Reduction = Reduction{}, Workspace = nullptr)
 {
-    owtk_solver_result r;
-    check(owtk_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_bicgstab(op.get(), pc, rhs.data(), solution.data(), &c, &r);
+                              });
 }
 
-// pipelined_bicgstab (pipelined_bicgstab.hpp:20-244)
-inline owtk_solver_result pipelined_bicgstab(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
-                                             DeviceVector<double>& x, const owtk_solver_options& o)
+// bicgstab_stable / bicgstab_mixed_precision (krylov_solvers.hpp:766-809):
+// bicgstab with MixedPrecisionSerialReduction (true residual verified).
+template<class Options, class Pc>
+SolverResult<double> bicgstab_stable(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                     DeviceVector<double>& solution, const Options& options, Pc&& preconditioner)
 {
-    owtk_solver_result r;
-    check(owtk_pipelined_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
+    return bicgstab(op, rhs, solution, options, std::forward<Pc>(preconditioner), MixedPrecisionSerialReduction{});
+}
+
+template<class Options, class Pc>
+SolverResult<double> bicgstab_mixed_precision(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                              DeviceVector<double>& solution, const Options& options,
+                                              Pc&& preconditioner)
+{
+    return bicgstab(op, rhs, solution, options, std::forward<Pc>(preconditioner), MixedPrecisionSerialReduction{});
+}
+
+// fgmres(op, b, x, options, pc, reduction, workspace, arnoldi_snapshot)
+// (krylov_solvers.hpp:159-171).  The device FGMRES reduces with the serial
+// reduction (another reduction is rejected); `arnoldi_snapshot` is null or an
+// owtk::hip::ArnoldiSnapshot*.
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t, class Snapshot = std::nullptr_t>
+SolverResult<double> fgmres(DeviceMatrix& op, const DeviceVector<double>& rhs, DeviceVector<double>& solution,
+                            const Options& options = Options{}, Pc&& preconditioner = Pc{},
+                            Reduction = Reduction{}, Workspace = nullptr, Snapshot arnoldi_snapshot = nullptr)
+{
+    static_assert(std::is_same_v<Snapshot, std::nullptr_t> || std::is_same_v<Snapshot, ArnoldiSnapshot*>,
+                  "the device FGMRES records into an owtk::hip::ArnoldiSnapshot");
+    owtk_arnoldi_snapshot snap = nullptr;
+    if constexpr (std::is_same_v<Snapshot, ArnoldiSnapshot*>) snap = arnoldi_snapshot ? arnoldi_snapshot->get() : nullptr;
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_fgmres_with_snapshot(op.get(), pc, rhs.data(), solution.data(), &c, snap,
+                                                                   &r);
+                              });
+}
+
+// pipelined_bicgstab(op, b, x, options, pc, reduction, workspace)
+// (pipelined_bicgstab.hpp:20-35)
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t>
+SolverResult<double> pipelined_bicgstab(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                        DeviceVector<double>& solution, const Options& options = Options{},
+                                        Pc&& preconditioner = Pc{}, Reduction = Reduction{}, Workspace = nullptr)
+{
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_pipelined_bicgstab(op.get(), pc, rhs.data(), solution.data(), &c, &r);
+                              });
 }
 
 // communication_hiding_bicgstab (pipelined_bicgstab.hpp:317-597)
-inline owtk_solver_result communication_hiding_bicgstab(DeviceMatrix& a, const Preconditioner* pc,
-                                                        const DeviceVector<double>& b, DeviceVector<double>& x,
-                                                        const owtk_solver_options& o)
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t>
+SolverResult<double> communication_hiding_bicgstab(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                                   DeviceVector<double>& solution,
+                                                   const Options& options = Options{}, Pc&& preconditioner = Pc{},
+                                                   Reduction = Reduction{}, Workspace = nullptr)
 {
-    owtk_solver_result r;
-    check(owtk_communication_hiding_bicgstab(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_communication_hiding_bicgstab(op.get(), pc, rhs.data(),
+                                                                            solution.data(), &c, &r);
+                              });
 }
 
-// idrs (idrs.hpp:128-505), s = o.idr_shadow_space
-inline owtk_solver_result idrs(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
-                               DeviceVector<double>& x, const owtk_solver_options& o)
+// idrs(op, b, x, options, pc, reduction, workspace) (idrs.hpp:128-140),
+// s = options.idr_shadow_space
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t>
+SolverResult<double> idrs(DeviceMatrix& op, const DeviceVector<double>& rhs, DeviceVector<double>& solution,
+                          const Options& options = Options{}, Pc&& preconditioner = Pc{},
+                          Reduction = Reduction{}, Workspace = nullptr)
 {
-    owtk_solver_result r;
-    check(owtk_idrs(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_idrs(op.get(), pc, rhs.data(), solution.data(), &c, &r);
+                              });
 }
 
-inline owtk_solver_result fgmres(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
-                                 DeviceVector<double>& x, const owtk_solver_options& o)
+// FP64 iterative refinement around an FP32/BF16 inner FGMRES (no reference
+// counterpart): the outer tolerances and limits come from `options` (a
+// SolverOptions), the inner solve from `inner` (owtk_refinement_options
+// without its outer part); `fallback` (may be null) takes over when a
+// refinement step with `preconditioner` fails.
+template<class Options, class Pc>
+SolverResult<double> fgmres_refined(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                    DeviceVector<double>& solution, const Options& options, Pc&& preconditioner,
+                                    owtk_refinement_options inner, const Preconditioner* fallback = nullptr)
 {
-    owtk_solver_result r;
-    check(owtk_fgmres(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
-}
-
-inline owtk_solver_result fgmres_refined(DeviceMatrix& a, const Preconditioner* pc, const DeviceVector<double>& b,
-                                         DeviceVector<double>& x, const owtk_refinement_options& o,
-                                         const Preconditioner* fallback = nullptr)
-{
+    if (rhs.size() < op.owned_size() || solution.size() < op.local_size())
+        throw std::invalid_argument("solver vector layout mismatch");
+    inner.outer = detail::to_c(options);
+    const owtk_precond pc = detail::pc_handle(preconditioner);
     owtk_solver_result r;
     if (fallback)
-        check(owtk_fgmres_refined_fallback(a.get(), pc ? pc->get() : nullptr, fallback->get(), b.data(), x.data(), &o, &r));
+        check(owtk_fgmres_refined_fallback(op.get(), pc, fallback->get(), rhs.data(), solution.data(), &inner, &r));
     else
-        check(owtk_fgmres_refined(a.get(), pc ? pc->get() : nullptr, b.data(), x.data(), &o, &r));
-    return r;
+        check(owtk_fgmres_refined(op.get(), pc, rhs.data(), solution.data(), &inner, &r));
+    return from_c(r);
 }
 
-// Host-vector convenience: the reference's bicgstab(op, b, x, options, pc)
-// shape for a matrix solved on the device.  b and x are host vectors with
-// data()/owned_size()/local_size(); x holds the initial guess and receives
-// the solution; Options/Result are the reference's SolverOptions /
-// SolverResult (or any type with the same fields).
-template<class Result, class Matrix, class Vector, class Options>
-Result solve_bicgstab(HipExecutionPolicy& policy, const Matrix& matrix, const Vector& b, Vector& x,
-                      const Options& options, owtk_precond_kind kind = OWTK_PC_JACOBI, std::int64_t block_nodes = 0)
+inline owtk_refinement_options refinement_options()
+{
+    owtk_refinement_options o;
+    owtk_refinement_options_default(&o);
+    return o;
+}
+
+// Host-vector routes: the reference's bicgstab / fgmres on a matrix and
+// host vectors (data(), owned_size(), local_size()), with the whole solve on
+// the device.  The preconditioner is described by owtk_precond_params (built
+// on the device copy of the matrix); x holds the initial guess and receives
+// the solution.
+template<class Matrix, class Vector, class Options>
+SolverResult<double> solve_bicgstab(HipExecutionPolicy& policy, const Matrix& matrix, const Vector& b, Vector& x,
+                                    const Options& options, const owtk_precond_params& pc_params)
 {
     Context& ctx = policy.context();
     DeviceMatrix a(ctx, matrix);
-    auto p = Preconditioner::params(kind);
-    p.block_nodes = block_nodes;
-    Preconditioner pc(a, p);
+    Preconditioner pc(a, pc_params);
     DeviceVector<double> db(ctx, a.local_size()), dx(ctx, a.local_size());
     db.upload(b.data(), a.owned_size());
     dx.upload(x.data(), a.local_size());
-    const owtk_solver_result c = bicgstab(a, &pc, db, dx, to_c(options));
+    SolverResult<double> r = bicgstab(a, db, dx, options, pc);
     dx.download(x.data(), a.owned_size());
-    Result r{};
-    from_c(c, r);
+    return r;
+}
+
+template<class Matrix, class Vector, class Options>
+SolverResult<double> solve_fgmres(HipExecutionPolicy& policy, const Matrix& matrix, const Vector& b, Vector& x,
+                                  const Options& options, const owtk_precond_params& pc_params)
+{
+    Context& ctx = policy.context();
+    DeviceMatrix a(ctx, matrix);
+    Preconditioner pc(a, pc_params);
+    DeviceVector<double> db(ctx, a.local_size()), dx(ctx, a.local_size());
+    db.upload(b.data(), a.owned_size());
+    dx.upload(x.data(), a.local_size());
+    SolverResult<double> r = fgmres(a, db, dx, options, pc);
+    dx.download(x.data(), a.owned_size());
     return r;
 }
 

@@ -214,9 +214,35 @@ int owtk_precond_ilu_factors(owtk_precond pc, int64_t* entries, int64_t* row_off
 int owtk_precond_destroy(owtk_precond pc);
 
 /* ---------------------------------------------------------------- solvers -- */
+/* How a solve reduces its inner products and norms: the device counterparts
+ * of the reference's Reduction template argument (reduction.hpp:103-299).
+ * SERIAL: FP64 sums over a fixed tree (SerialReduction).  COMPENSATED: every
+ * product and partial sum carried as a double-double pair (TwoProd/TwoSum at
+ * every addition, in the threads, the block trees and the finisher), the
+ * counterpart of CompensatedSerialReduction's Neumaier sums.
+ * MIXED_PRECISION: MixedPrecisionSerialReduction, whose local sums are wider
+ * than the vectors: for FP64 vectors the same double-double accumulation (at
+ * least the 64-bit significand of x86 long double), for FP32 vectors FP64. */
+typedef enum owtk_reduction {
+    OWTK_REDUCTION_SERIAL = 0,
+    OWTK_REDUCTION_COMPENSATED = 1,
+    OWTK_REDUCTION_MIXED_PRECISION = 2
+} owtk_reduction;
+
+/* SolverOptions::convergence_test (core.hpp:100-108): called with the
+ * iteration count and the current solution (a DEVICE pointer to `count`
+ * owned values, FP64) at iteration zero and after every complete iterate;
+ * a non-zero return stops the solve, which then measures the true residual
+ * and reports converged with converged_by_application set.  While a test is
+ * set, the residual tolerance is not tested (the reference's
+ * convergence_threshold returns 0).  The solution must not be modified; the
+ * test runs with the context's stream idle and may read the solution with
+ * owtk_memcpy_dtoh on the same context. */
+typedef int (*owtk_convergence_test)(void* user_data, uint64_t iteration, const double* device_solution,
+                                     size_t count);
+
 /* Mirrors SolverOptions<T> (core.hpp:88-111); enum values mirror
- * GmresOrthogonalization (core.hpp:32-39).  The application callback
- * convergence_test is not part of the device path. */
+ * GmresOrthogonalization (core.hpp:32-39). */
 typedef struct owtk_solver_options {
     double relative_tolerance;
     double absolute_tolerance;
@@ -228,6 +254,14 @@ typedef struct owtk_solver_options {
     int32_t gmres_orthogonalization; /* 0 iterated CGS, 1 adaptive CGS, 2 one-sync CGS */
     int32_t collect_timings;
     uint64_t idr_shadow_space;       /* IDR(s): s (SolverOptions::idr_shadow_space, core.hpp:97) */
+    double relaxation;               /* SolverOptions::relaxation (core.hpp:98): the stationary
+                                        solvers' weight; the Krylov solvers do not read it */
+    int32_t verify_true_residual;    /* SolverOptions::verify_true_residual (core.hpp:99): the
+                                        device solvers always verify the true residual before
+                                        reporting convergence, as the reference's do */
+    int32_t reduction;               /* owtk_reduction; non-SERIAL: owtk_bicgstab only */
+    owtk_convergence_test convergence_test;  /* NULL: tolerance test (owtk_bicgstab only) */
+    void* convergence_test_data;
 } owtk_solver_options;
 
 /* Mirrors SolverResult<T> (core.hpp:118-146); status and breakdown_reason use
@@ -247,6 +281,7 @@ typedef struct owtk_solver_result {
     double true_residual_norm;
     double relative_residual_norm;
     double solve_seconds;
+    int32_t converged_by_application; /* SolverResult::converged_by_application (core.hpp:127) */
     /* iterative refinement (owtk_fgmres_refined) only */
     uint64_t refinement_steps;
     uint64_t inner_iterations;
@@ -256,8 +291,12 @@ typedef struct owtk_solver_result {
 /* Defaults of SolverOptions<double>. */
 void owtk_solver_options_default(owtk_solver_options* options);
 
-/* bicgstab(op, b, x, options, pc) (krylov_solvers.hpp:542-745) with every
- * vector resident on the device.  pc == NULL means IdentityPreconditioner. */
+/* bicgstab(op, b, x, options, pc, reduction) (krylov_solvers.hpp:542-745) with
+ * every vector resident on the device.  pc == NULL means
+ * IdentityPreconditioner.  options->reduction selects the reduction
+ * (bicgstab_stable and bicgstab_mixed_precision, krylov_solvers.hpp:766-809,
+ * are this call with OWTK_REDUCTION_MIXED_PRECISION); options->convergence_test
+ * is honoured. */
 int owtk_bicgstab(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                   const owtk_solver_options* options, owtk_solver_result* result);
 /* pipelined_bicgstab(op, b, x, options, pc) (pipelined_bicgstab.hpp:20-244):
@@ -282,9 +321,38 @@ int owtk_communication_hiding_bicgstab(owtk_matrix m, owtk_precond pc, const dou
  * global reduction per inner product or norm it takes. */
 int owtk_idrs(owtk_matrix m, owtk_precond pc, const double* b, double* x,
               const owtk_solver_options* options, owtk_solver_result* result);
-/* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident. */
+/* fgmres(op, b, x, options, pc) (krylov_solvers.hpp:159-486), device-resident.
+ * An application convergence test (options->convergence_test) sees every
+ * iterate x + Z y, formed without touching x (krylov_solvers.hpp:398-427);
+ * it costs one host round trip per Arnoldi column.  Serial reduction only. */
 int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x,
                 const owtk_solver_options* options, owtk_solver_result* result);
+
+/* ArnoldiSnapshot<double> (core.hpp:462-534): the last restart cycle's
+ * unrotated Hessenberg columns and preconditioned basis vectors Z, as the
+ * fgmres(..., workspace, arnoldi_snapshot) argument fills it
+ * (krylov_solvers.hpp:171, 252-253, 361-366, 420-436).  The basis vectors
+ * stay on the device in the solve's owned + ghost layout, ghosts zero. */
+typedef struct owtk_arnoldi_snapshot_s* owtk_arnoldi_snapshot;
+int owtk_arnoldi_snapshot_create(owtk_context ctx, owtk_arnoldi_snapshot* out);
+int owtk_arnoldi_snapshot_destroy(owtk_arnoldi_snapshot snapshot);
+/* ArnoldiSnapshot::clear (core.hpp:504-510) */
+int owtk_arnoldi_snapshot_clear(owtk_arnoldi_snapshot snapshot);
+/* ArnoldiSnapshot::columns (core.hpp:512) */
+int owtk_arnoldi_snapshot_columns(owtk_arnoldi_snapshot snapshot, uint64_t* columns);
+/* ArnoldiSnapshot::h(row, column) (core.hpp:513-519): INVALID_ARGUMENT where
+ * the reference throws std::out_of_range (column >= columns or row > columns). */
+int owtk_arnoldi_snapshot_h(owtk_arnoldi_snapshot snapshot, uint64_t row, uint64_t column, double* value);
+/* ArnoldiSnapshot::preconditioned_basis(column) (core.hpp:520-527): the
+ * device address of Z_column (owned + ghost values), valid until the next
+ * solve with this snapshot or its destruction. */
+int owtk_arnoldi_snapshot_basis(owtk_arnoldi_snapshot snapshot, uint64_t column, const double** device_vector);
+/* fgmres(op, b, x, options, pc, SerialReduction, nullptr, arnoldi_snapshot)
+ * (krylov_solvers.hpp:159-171): owtk_fgmres that also records the last
+ * cycle into `snapshot` (null: owtk_fgmres). */
+int owtk_fgmres_with_snapshot(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                              const owtk_solver_options* options, owtk_arnoldi_snapshot snapshot,
+                              owtk_solver_result* result);
 
 /* Mixed precision: FP64 iterative refinement around an inner FGMRES whose
  * matrix and preconditioner values are stored in `inner_storage` (F32/BF16)
@@ -414,13 +482,17 @@ int owtk_read_state(const char* const* paths, int32_t files, int64_t ns, int64_t
  * precision of the preconditioner's own copy of the off-diagonal values,
  * built from the FP64 values at set-up and at owtk_precond_update_values).
  * The apply evaluates the same series through w_k = D u_k over column-scaled
- * values, so no step reads D^-1.  A fresh model has a zero network and
- * gamma = 1: the plain truncated Neumann series.
+ * values, so no step reads D^-1.  Network: features per vertex and component
+ * rho = sum|a|/d, log(1 - rho)/8 and the edge anisotropy
+ * log(deg max|a| / sum|a|)/2, plus deg/8; decoder gamma_kic = g_kc + o_kci
+ * (o = W_out h3 + b_out).  A fresh model has Glorot-uniform encoder and
+ * layer weights (a fixed seed), a zero decoder and g = 1: the plain
+ * truncated Neumann series.
  * Training: each problem holds a split matrix and P (residual, error) pairs;
  * the loss is (1/P) sum_p |D^-1/2 A (M r_p - e_p)|^2 / |D^-1/2 A e_p|^2.
  * Gradients accumulate on the device (owtk_gnn_accumulate) and an Adam step
  * applies and clears them.  Parameters are a flat FP32 vector in the order
- * W_in[32][2b+1], b_in[32], then per layer W_s[32][32], W_n[32][32], b[32],
+ * W_in[32][3b+1], b_in[32], then per layer W_s[32][32], W_n[32][32], b[32],
  * then W_out[(degree+1)b][32], b_out, g[(degree+1)b]. */
 typedef struct owtk_gnn_s* owtk_gnn;
 typedef struct owtk_gnn_problem_s* owtk_gnn_problem;
@@ -434,7 +506,9 @@ int owtk_gnn_get_parameters(owtk_gnn model, float* params);
 /* Network inference on a split matrix: gamma[k][vertex*b + c] to host. */
 int owtk_gnn_coefficients(owtk_gnn model, owtk_matrix m, float* gamma);
 /* residuals/errors: device FP64, pairs x (owned*b).  The pattern must be
- * structurally symmetric (the backward pass uses the transposed matrix). */
+ * structurally symmetric (the backward pass uses the transposed matrix).
+ * Every model of the same context, block size and degree can train on the
+ * problem (owtk_gnn_accumulate). */
 int owtk_gnn_problem_create(owtk_gnn model, owtk_matrix m, int32_t pairs, const double* residuals,
                             const double* errors, owtk_gnn_problem* out);
 int owtk_gnn_problem_destroy(owtk_gnn_problem problem);
@@ -494,6 +568,13 @@ int owtk_dot(owtk_context ctx, owtk_dtype dtype, size_t n, const void* a, const 
              double* out);
 /* overflow-safe 2-norm (detail::scaled_norm, reduction.hpp:64-89). */
 int owtk_norm(owtk_context ctx, owtk_dtype dtype, size_t n, const void* a, double* out);
+/* Reduction::dot / Reduction::norm with the reduction selected
+ * (owtk_reduction: SerialReduction, CompensatedSerialReduction,
+ * MixedPrecisionSerialReduction; reduction.hpp:114-168, 176-299). */
+int owtk_reduction_dot(owtk_context ctx, int32_t reduction, owtk_dtype dtype, size_t n, const void* a,
+                       const void* b, double* out);
+int owtk_reduction_norm(owtk_context ctx, int32_t reduction, owtk_dtype dtype, size_t n, const void* a,
+                        double* out);
 /* y += alpha x (simd::axpy, simd.hpp:110-202). */
 int owtk_axpy(owtk_context ctx, size_t n, double alpha, const double* x, double* y);
 /* y += alpha x, then out = <w, y> in the same pass. */

@@ -966,6 +966,7 @@ int owtk_batch_bicgstab(owtk_batch bt, const double* b, double* x, const owtk_so
         if (!bt || !b || !x || !options) throw std::invalid_argument("null argument");
         if (graph_iterations < 1 || graph_iterations > 1024) throw std::invalid_argument("graph_iterations must be 1..1024");
         if (!valid_options(options)) throw std::invalid_argument("invalid solver options");
+        require_plain_options(options, "batched bicgstab");
         bt->ctx->activate();
         std::vector<double> ms;
         std::vector<int> its;

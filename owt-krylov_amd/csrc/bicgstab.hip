@@ -48,6 +48,7 @@ struct BicgState {
     int reason;
     long long stop_iteration;
     long long extra_reductions;  // pipelined: fused squared norms that fell back to a scaled norm
+    int app_test;                // an application convergence test replaces the cadence test
 };
 
 // norm_from_squared (krylov_solvers.hpp:95-108) charges one more reduction
@@ -79,15 +80,19 @@ __device__ void fin_rho(BicgState* st, double rho, long long it)
     st->rho = rho;
 }
 
-// <shadow, r> at (re)start
+// <shadow, r> at (re)start.  CMP: compensated (double-double) accumulation.
+template<bool CMP>
 __global__ __launch_bounds__(256) void k_rho(const double* __restrict__ shadow, const double* __restrict__ r,
                                              std::int64_t n, BicgState* st, ReduceScratch scratch)
 {
-    double red[1] = {0.0};
+    double red[1] = {0.0}, lo[1] = {0.0};
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
-        red[0] = __builtin_fma(shadow[i], r[i], red[0]);
-    dev::store_partials<1>(red, 0, scratch);
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        if constexpr (CMP) dev::dd_fma(red[0], lo[0], shadow[i], r[i]);
+        else red[0] = __builtin_fma(shadow[i], r[i], red[0]);
+    }
+    if constexpr (CMP) dev::store_partials_dd<1>(red, lo, 0, scratch);
+    else dev::store_partials<1>(red, 0, scratch);
 }
 
 struct FinRho {
@@ -111,21 +116,24 @@ __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const 
 }
 
 // s = r - alpha v, ||s||
+template<bool CMP>
 __global__ __launch_bounds__(256) void k_s_update(double* __restrict__ s, const double* __restrict__ r,
                                                   const double* __restrict__ v, std::int64_t n, BicgState* st,
                                                   ReduceScratch scratch)
 {
     if (st->stop) return;
     const double alpha = st->alpha;
-    double red[2] = {0.0, 0.0};
+    double red[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
         const double si = __builtin_fma(-alpha, v[i], r[i]);
         s[i] = si;
-        red[0] = __builtin_fma(si, si, red[0]);
+        if constexpr (CMP) dev::dd_fma(red[0], lo[0], si, si);
+        else red[0] = __builtin_fma(si, si, red[0]);
         red[1] = dev::combine<dev::Combine::max>(red[1], __builtin_fabs(si));
     }
-    dev::store_partials<2>(red, 2, scratch);
+    if constexpr (CMP) dev::store_partials_dd<2>(red, lo, 2, scratch);
+    else dev::store_partials<2>(red, 2, scratch);
 }
 
 struct FinSnorm {
@@ -146,6 +154,7 @@ struct FinSnorm {
 };
 
 // x += alpha z; x += omega y; r = s - omega t; <shadow, r>, ||r||
+template<bool CMP>
 __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, double* __restrict__ r,
                                                    const double* __restrict__ z, const double* __restrict__ y,
                                                    const double* __restrict__ s, const double* __restrict__ t,
@@ -154,7 +163,7 @@ __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, doubl
 {
     if (st->stop) return;
     const double alpha = st->alpha, omega = st->omega;
-    double red[3] = {0.0, 0.0, 0.0};
+    double red[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
         double xi = __builtin_fma(alpha, z[i], x[i]);
@@ -162,12 +171,86 @@ __global__ __launch_bounds__(256) void k_xr_update(double* __restrict__ x, doubl
         x[i] = xi;
         const double ri = __builtin_fma(-omega, t[i], s[i]);
         r[i] = ri;
-        red[0] = __builtin_fma(shadow[i], ri, red[0]);
-        red[1] = __builtin_fma(ri, ri, red[1]);
+        if constexpr (CMP) {
+            dev::dd_fma(red[0], lo[0], shadow[i], ri);
+            dev::dd_fma(red[1], lo[1], ri, ri);
+        } else {
+            red[0] = __builtin_fma(shadow[i], ri, red[0]);
+            red[1] = __builtin_fma(ri, ri, red[1]);
+        }
         red[2] = dev::combine<dev::Combine::max>(red[2], __builtin_fabs(ri));
     }
-    dev::store_partials<3>(red, 4, scratch);
+    if constexpr (CMP) dev::store_partials_dd<3>(red, lo, 4, scratch);
+    else dev::store_partials<3>(red, 4, scratch);
 }
+
+// Compensated-mode reductions that the FP64 path fuses into SpMV epilogues:
+// the SpMV stores its output, then one of these passes reduces it.
+//   k_dot_dd<1>: <a, w>;  k_dot_dd<2>: <a, w>, <a, a>
+template<int K>
+__global__ __launch_bounds__(256) void k_dot_dd(const double* __restrict__ a, const double* __restrict__ w,
+                                                std::int64_t n, const int* stop, ReduceScratch scratch)
+{
+    if (stop != nullptr && *stop != 0) return;
+    double hi[K], lo[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) hi[k] = lo[k] = 0.0;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        const double ai = a[i];
+        dev::dd_fma(hi[0], lo[0], ai, w[i]);
+        if constexpr (K > 1) dev::dd_fma(hi[1], lo[1], ai, ai);
+    }
+    dev::store_partials_dd<K>(hi, lo, 0, scratch);
+}
+
+// r = b - r (r holds A x), compensated sum of squares and max |r|; with b
+// null: sum of squares and max of r itself (a norm).
+__global__ __launch_bounds__(256) void k_residual_dd(double* __restrict__ r, const double* __restrict__ b,
+                                                     std::int64_t n, ReduceScratch scratch)
+{
+    double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double ri = r[i];
+        if (b != nullptr) {
+            ri = b[i] - ri;
+            r[i] = ri;
+        }
+        dev::dd_fma(hi[0], lo[0], ri, ri);
+        hi[1] = dev::combine<dev::Combine::max>(hi[1], __builtin_fabs(ri));
+    }
+    dev::store_partials_dd<2>(hi, lo, 2, scratch);
+}
+
+struct FinNormTo {
+    const double* v;
+    std::int64_t n;
+    double* out;
+    __device__ void operator()(const double* t) const
+    {
+        const double nrm = dev::block_norm(t[0], t[1], v, n);
+        if (threadIdx.x == 0) *out = nrm;
+    }
+};
+
+template<class Fin>
+struct FinDot1 {
+    Fin fin;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x == 0) fin(t[0]);
+    }
+};
+
+template<class Fin>
+struct FinDot2 {
+    Fin fin;
+    __device__ void operator()(const double* t) const
+    {
+        if (threadIdx.x == 0) fin(t[0], t[1]);
+    }
+};
 
 struct FinXr {
     BicgState* st;
@@ -177,7 +260,7 @@ struct FinXr {
     {
         const long long it = st->iteration;
         const bool replace = st->replace_interval > 0 && it % st->replace_interval == 0;
-        const bool check = it % st->check_interval == 0;
+        const bool check = !st->app_test && it % st->check_interval == 0;
         double rn = 0.0;
         if (replace || check) rn = dev::block_norm(tot[1], tot[2], r, n);
         __syncthreads();  // every thread has read the state before thread 0 updates it
@@ -641,6 +724,12 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     const auto t0 = std::chrono::steady_clock::now();
     init_result(res);
     if (!valid_options(o)) return OWTK_OK;
+    if (o->reduction < OWTK_REDUCTION_SERIAL || o->reduction > OWTK_REDUCTION_MIXED_PRECISION)
+        throw std::invalid_argument("unknown reduction");
+    // compensated and mixed-precision reductions: double-double sums (the
+    // vectors are FP64, so both widen the accumulation beyond FP64)
+    const bool cmp = o->reduction != OWTK_REDUCTION_SERIAL;
+    const bool app = o->convergence_test != nullptr;
     if (pc && pc->m->owned * pc->m->bs != m->owned * m->bs)
         throw std::invalid_argument("preconditioner layout mismatch");
     if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
@@ -669,13 +758,27 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     hst = static_cast<BicgState*>(ctx->pinned.ptr);
     std::uint64_t ops = 0, pcs = 0, reds = 0;
 
+    const unsigned int g = dev::stream_grid(n, 4);
+    // ||v|| into *out (compensated mode; the FP64 path fuses its norms)
+    auto norm_dd = [&](double* v_in, const double* b_in, double* out) {
+        k_residual_dd<<<g, kBlock, 0, st>>>(v_in, b_in, n, scratch);
+        dev::launch_finish_dd<2>(st, FinNormTo{v_in, n, out}, g, 2, scratch, nullptr);
+        OWTK_LAUNCH_CHECK();
+    };
     auto true_residual = [&](double* norm_slot) {
-        dev::EpiResidual e;
-        e.y = r;
-        e.b = b;
-        e.n = n;
-        e.norm_out = norm_slot;
-        spmv_f64(m, x, e);
+        if (cmp) {
+            dev::EpiStore<double> e;
+            e.y = r;
+            spmv_f64(m, x, e);
+            norm_dd(r, b, norm_slot);
+        } else {
+            dev::EpiResidual e;
+            e.y = r;
+            e.b = b;
+            e.n = n;
+            e.norm_out = norm_slot;
+            spmv_f64(m, x, e);
+        }
         ++ops;
         ++reds;
     };
@@ -685,21 +788,16 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     };
     auto write_state = [&] { OWTK_HIP(hipMemcpyAsync(dst, hst, sizeof(BicgState), hipMemcpyHostToDevice, st)); };
 
-    // r = b - A x; shadow = r; ||b||, ||r||
-    std::memset(hst, 0, sizeof(BicgState));
-    write_state();
     double* d_norms = ctx->scalars.get();
-    true_residual(&d_norms[1]);
-    OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    kern::norm_to(st, scratch, b, n, &d_norms[0]);
-    ++reds;
     double norms[2];
-    OWTK_HIP(hipMemcpyAsync(norms, d_norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    ctx->sync();
-    const double rhs_norm = norms[0], initial = norms[1];
-    const double thr = threshold_of(rhs_norm, o);
-    res->initial_residual_norm = initial;
-    set_residual(res, initial, initial, rhs_norm);
+    auto rhs_norm_to = [&](double* slot) {
+        if (cmp) {
+            norm_dd(const_cast<double*>(b), nullptr, slot);  // read only: no b_in
+        } else {
+            kern::norm_to(st, scratch, b, n, slot);
+        }
+        ++reds;
+    };
     auto finish = [&](int status) {
         res->status = status;
         res->operator_applications = ops;
@@ -709,13 +807,53 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return OWTK_OK;
     };
+    // application_convergence (krylov_solvers.hpp:122-143): the test sees the
+    // current solution; on success the true residual is measured
+    auto app_converged = [&](long long iteration) {
+        ctx->sync();
+        if (!o->convergence_test(o->convergence_test_data, static_cast<std::uint64_t>(iteration), x,
+                                 static_cast<std::size_t>(n)))
+            return false;
+        true_residual(&d_norms[2]);
+        rhs_norm_to(&d_norms[3]);
+        OWTK_HIP(hipMemcpyAsync(norms, &d_norms[2], 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        set_residual(res, res->recursive_residual_norm, norms[0], norms[1]);
+        if (!std::isfinite(norms[0]) || !std::isfinite(norms[1])) {
+            res->status = status_breakdown;
+            res->breakdown_reason = reason_non_finite_scalar;
+        } else {
+            res->status = status_converged;
+            res->converged_by_application = 1;
+        }
+        return true;
+    };
+    if (app && app_converged(0)) {
+        finish(res->status);
+        return OWTK_OK;
+    }
+
+    // r = b - A x; shadow = r; ||b||, ||r||
+    std::memset(hst, 0, sizeof(BicgState));
+    write_state();
+    true_residual(&d_norms[1]);
+    OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    rhs_norm_to(&d_norms[0]);
+    OWTK_HIP(hipMemcpyAsync(norms, d_norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double rhs_norm = norms[0], initial = norms[1];
+    // with an application test only an exactly solved system stops on the
+    // residual (convergence_threshold, core.hpp:549-560)
+    double thr = threshold_of(rhs_norm, o);
+    if (app && std::isfinite(thr)) thr = 0.0;
+    res->initial_residual_norm = initial;
+    set_residual(res, initial, initial, rhs_norm);
     if (!std::isfinite(initial) || !std::isfinite(thr)) {
         res->breakdown_reason = reason_non_finite_scalar;
         return finish(status_breakdown);
     }
     if (initial <= thr) return finish(status_converged);
 
-    const unsigned int g = dev::stream_grid(n, 4);
     auto restart_state = [&](long long it) {
         hst->rho_prev = 1;
         hst->alpha = 1;
@@ -727,8 +865,13 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
         OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         kern::fill_zero_bytes(st, p, n * sizeof(double));
         kern::fill_zero_bytes(st, v, n * sizeof(double));
-        k_rho<<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
-        dev::launch_finish<1>(st, FinRho{dst}, g, 0, scratch, nullptr);
+        if (cmp) {
+            k_rho<true><<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+            dev::launch_finish_dd<1>(st, FinRho{dst}, g, 0, scratch, nullptr);
+        } else {
+            k_rho<false><<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+            dev::launch_finish<1>(st, FinRho{dst}, g, 0, scratch, nullptr);
+        }
         OWTK_LAUNCH_CHECK();
     };
     std::memset(hst, 0, sizeof(BicgState));
@@ -739,6 +882,7 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
     hst->max_iterations = static_cast<long long>(o->maximum_iterations);
     hst->check_interval = static_cast<long long>(o->convergence_check_interval);
     hst->replace_interval = static_cast<long long>(o->residual_replacement_interval);
+    hst->app_test = app ? 1 : 0;
     restart_state(1);
 
     const int* stop = &dst->stop;
@@ -746,28 +890,46 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
         k_p_update<<<g, kBlock, 0, st>>>(p, r, v, n, dst);
         OWTK_LAUNCH_CHECK();
         apply_pc_f64(pc, p, z, n, st, stop);
-        {
+        if (cmp) {
+            dev::EpiStore<double> e;
+            e.y = v;
+            e.stop = stop;
+            spmv_f64(m, z, e);
+            k_dot_dd<1><<<g, kBlock, 0, st>>>(v, shadow, n, stop, scratch);
+            dev::launch_finish_dd<1>(st, FinDot1<FinAlpha>{FinAlpha{dst}}, g, 0, scratch, stop);
+            k_s_update<true><<<g, kBlock, 0, st>>>(s, r, v, n, dst, scratch);
+            dev::launch_finish_dd<2>(st, FinSnorm{dst, s, n}, g, 2, scratch, stop);
+        } else {
             dev::EpiDot1<FinAlpha> e;
             e.y = v;
             e.w = shadow;
             e.stop = stop;
             e.fin = FinAlpha{dst};
             spmv_f64(m, z, e);
+            k_s_update<false><<<g, kBlock, 0, st>>>(s, r, v, n, dst, scratch);
+            dev::launch_finish<2>(st, FinSnorm{dst, s, n}, g, 2, scratch, stop);
         }
-        k_s_update<<<g, kBlock, 0, st>>>(s, r, v, n, dst, scratch);
-        dev::launch_finish<2>(st, FinSnorm{dst, s, n}, g, 2, scratch, stop);
         OWTK_LAUNCH_CHECK();
         apply_pc_f64(pc, s, yv, n, st, stop);
-        {
+        if (cmp) {
+            dev::EpiStore<double> e;
+            e.y = t;
+            e.stop = stop;
+            spmv_f64(m, yv, e);
+            k_dot_dd<2><<<g, kBlock, 0, st>>>(t, s, n, stop, scratch);
+            dev::launch_finish_dd<2>(st, FinDot2<FinOmega>{FinOmega{dst}}, g, 0, scratch, stop);
+            k_xr_update<true><<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+            dev::launch_finish_dd<3>(st, FinXr{dst, r, n}, g, 4, scratch, stop);
+        } else {
             dev::EpiDot2<FinOmega> e;
             e.y = t;
             e.w = s;
             e.stop = stop;
             e.fin = FinOmega{dst};
             spmv_f64(m, yv, e);
+            k_xr_update<false><<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+            dev::launch_finish<3>(st, FinXr{dst, r, n}, g, 4, scratch, stop);
         }
-        k_xr_update<<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
-        dev::launch_finish<3>(st, FinXr{dst, r, n}, g, 4, scratch, stop);
         OWTK_LAUNCH_CHECK();
     };
     const long long maxit = static_cast<long long>(o->maximum_iterations);
@@ -782,10 +944,14 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
         }
     };
 
+    // with an application test: one iteration per host round trip, the test
+    // after every complete iterate (krylov_solvers.hpp:658, 710)
+    auto app_stop = [&]() { return finish(res->status); };
     long long it = 1;
     while (true) {
         long long end = it;
-        while (end < maxit && !is_cadence(end)) ++end;
+        if (!app)
+            while (end < maxit && !is_cadence(end)) ++end;
         for (long long k = it; k <= end; ++k) enqueue_iteration();
         read_state();
         const int code = hst->stop;
@@ -795,10 +961,19 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             count_full(it, end);
             res->iterations = static_cast<std::uint64_t>(end);
             res->recursive_residual_norm = hst->recursive;
+            if (app && app_converged(end)) return app_stop();
             it = end + 1;
             continue;
         }
         if (code == stop_breakdown) {
+            // a rho breakdown (stage 0) follows the complete iterate j - 1
+            if (app && hst->stage == 0 && j == end + 1) {
+                count_full(it, j - 1);
+                res->iterations = static_cast<std::uint64_t>(j - 1);
+                res->recursive_residual_norm = hst->recursive;
+                if (app_converged(j - 1)) return app_stop();
+                it = j;  // counted already
+            }
             count_full(it, j - 1);
             if (hst->stage == 0) reds += 1;
             else if (hst->stage == 1) { pcs += 1; ops += 1; reds += 2; }
@@ -812,6 +987,7 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             count_full(it, j);
             res->iterations = static_cast<std::uint64_t>(j);
             res->recursive_residual_norm = hst->recursive;
+            if (app && app_converged(j)) return app_stop();
             break;
         }
         // early (s small) or cadence: x/true residual then converge or restart
@@ -823,9 +999,14 @@ int bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, con
             reds += 3;
             kern::axpy(st, n, hst->alpha, z, x);
             rec = hst->snorm;
+            res->iterations = static_cast<std::uint64_t>(j);
+            if (app && app_converged(j)) return app_stop();
         } else {
             count_full(it, j);
             rec = hst->rnorm;
+            res->iterations = static_cast<std::uint64_t>(j);
+            res->recursive_residual_norm = hst->recursive;
+            if (app && app_converged(j)) return app_stop();
         }
         res->iterations = static_cast<std::uint64_t>(j);
         true_residual(&d_norms[2]);
@@ -952,7 +1133,7 @@ int pipelined_bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, doub
         hst->iteration = it;
         write_state();
         OWTK_HIP(hipMemcpyAsync(shadow, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        k_rho<<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
+        k_rho<false><<<g, kBlock, 0, st>>>(shadow, r, n, dst, scratch);
         dev::launch_finish<1>(st, FinRhoP{dst}, g, 0, scratch, nullptr);
         OWTK_LAUNCH_CHECK();
         ++reds;
@@ -992,7 +1173,7 @@ int pipelined_bicgstab_run(owtk_matrix m, owtk_precond pc, const double* b, doub
             e.stop = stop;
             spmv_f64(m, yv, e);
         }
-        k_xr_update<<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
+        k_xr_update<false><<<g, kBlock, 0, st>>>(x, r, z, yv, s, t, shadow, n, dst, scratch);
         dev::launch_finish<3>(st, FinNextP{dst, r, n}, g, 4, scratch, stop);
         OWTK_LAUNCH_CHECK();
     };
@@ -1339,6 +1520,7 @@ extern "C" int owtk_pipelined_bicgstab(owtk_matrix m, owtk_precond pc, const dou
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        require_plain_options(options, "pipelined_bicgstab");
         m->ctx->activate();
         pipelined_bicgstab_run(m, pc, b, x, options, result);
     });
@@ -1349,6 +1531,7 @@ extern "C" int owtk_communication_hiding_bicgstab(owtk_matrix m, owtk_precond pc
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        require_plain_options(options, "communication_hiding_bicgstab");
         m->ctx->activate();
         communication_hiding_bicgstab_run(m, pc, b, x, options, result);
     });

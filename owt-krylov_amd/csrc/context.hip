@@ -367,6 +367,62 @@ int owtk_norm(owtk_context ctx, owtk_dtype dtype, size_t n, const void* a, doubl
     });
 }
 
+// Reduction::dot and Reduction::norm (reduction.hpp:114-168, 176-299) with
+// the reduction selected: serial, compensated or mixed precision (FP64
+// vectors: double-double sums; FP32 vectors: FP64 sums, as serial).
+int owtk_reduction_dot(owtk_context ctx, int32_t reduction, owtk_dtype dtype, size_t n, const void* a,
+                       const void* b, double* out)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->activate();
+        if (reduction < OWTK_REDUCTION_SERIAL || reduction > OWTK_REDUCTION_MIXED_PRECISION)
+            throw std::invalid_argument("unknown reduction");
+        if (!out) throw std::invalid_argument("null output");
+        if (n == 0) {
+            *out = 0.0;
+            return;
+        }
+        if (!a || !b) throw std::invalid_argument("null vector");
+        double* d_out = ctx->scalars.get();
+        if (dtype == OWTK_F64)
+            kern::dot_to(ctx->stream, ctx->scratch(), as<double>(a), as<double>(b), static_cast<std::int64_t>(n), d_out,
+                         reduction != OWTK_REDUCTION_SERIAL);
+        else
+            kern::dot_to(ctx->stream, ctx->scratch(), as<float>(a), as<float>(b), static_cast<std::int64_t>(n), d_out,
+                         reduction == OWTK_REDUCTION_COMPENSATED);
+        OWTK_HIP(hipMemcpyAsync(ctx->pinned.ptr, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        *out = *static_cast<double*>(ctx->pinned.ptr);
+    });
+}
+
+int owtk_reduction_norm(owtk_context ctx, int32_t reduction, owtk_dtype dtype, size_t n, const void* a, double* out)
+{
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->activate();
+        if (reduction < OWTK_REDUCTION_SERIAL || reduction > OWTK_REDUCTION_MIXED_PRECISION)
+            throw std::invalid_argument("unknown reduction");
+        if (!out) throw std::invalid_argument("null output");
+        if (n == 0) {
+            *out = 0.0;
+            return;
+        }
+        if (!a) throw std::invalid_argument("null vector");
+        double* d_out = ctx->scalars.get();
+        if (dtype == OWTK_F64)
+            kern::norm_to(ctx->stream, ctx->scratch(), as<double>(a), static_cast<std::int64_t>(n), d_out,
+                          reduction != OWTK_REDUCTION_SERIAL);
+        else
+            kern::norm_to(ctx->stream, ctx->scratch(), as<float>(a), static_cast<std::int64_t>(n), d_out,
+                          reduction == OWTK_REDUCTION_COMPENSATED);
+        OWTK_HIP(hipMemcpyAsync(ctx->pinned.ptr, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        *out = *static_cast<double*>(ctx->pinned.ptr);
+    });
+}
+
 int owtk_axpy(owtk_context ctx, size_t n, double alpha, const double* x, double* y)
 {
     return guarded([&] {

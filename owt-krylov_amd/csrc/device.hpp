@@ -43,6 +43,14 @@ __host__ __device__ inline std::uint16_t float_to_bf16(float f)
     return static_cast<std::uint16_t>((v.u + rounding) >> 16);
 }
 
+// Dense B x B blocks in the SELL-64 layout: element j = r*B + c of slot s at
+// (s & ~63) * B*B + 64 j + (s & 63), so the j-th elements of a wave's 64
+// slots are one contiguous run (one coalesced load per element).
+__host__ __device__ constexpr std::int64_t dense_index(std::int64_t slot, int j, int bb)
+{
+    return (slot & ~std::int64_t(63)) * bb + 64 * j + (slot & 63);
+}
+
 template<class V> __device__ inline float load_as_float(const V* p, std::int64_t i);
 template<> __device__ inline float load_as_float<float>(const float* p, std::int64_t i) { return p[i]; }
 template<> __device__ inline float load_as_float<std::uint16_t>(const std::uint16_t* p, std::int64_t i)
@@ -197,6 +205,137 @@ inline void launch_finish(hipStream_t st, const Fin& fin, unsigned int nblocks, 
                           ReduceScratch scratch, const int* stop, const int* need = nullptr)
 {
     k_finish<K, Fin><<<1, kBlock, 0, st>>>(fin, nblocks, max_mask, scratch, stop, need);
+}
+
+// ------------------------------------------- double-double (compensated) --
+// Compensated reductions carry each sum as an unevaluated pair hi + lo.
+// TwoSum (Knuth) gives s + e == a + b exactly; TwoProd by FMA gives the
+// product's rounding error.  The kernels are built with -ffp-contract=off,
+// so none of these operations is fused or reassociated.
+__device__ inline void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (hi, lo) += a * b
+__device__ inline void dd_fma(double& hi, double& lo, double a, double b)
+{
+    const double p = a * b;
+    const double pe = __builtin_fma(a, b, -p);
+    double s, e;
+    two_sum(hi, p, s, e);
+    hi = s;
+    lo += e + pe;
+}
+
+// (h1, l1) += (h2, l2)
+__device__ inline void dd_combine(double& h1, double& l1, double h2, double l2)
+{
+    double s, e;
+    two_sum(h1, h2, s, e);
+    h1 = s;
+    l1 = (l1 + l2) + e;
+}
+
+// Block reduction of K pairs (hi[k], lo[k]); bit k of max_mask: value k is a
+// max of magnitudes (carried in hi, lo stays 0).  Thread 0 gets the totals.
+template<int K>
+__device__ inline void block_reduce_dd(double (&hi)[K], double (&lo)[K], std::uint64_t max_mask)
+{
+    __shared__ double lds[kWavesPerBlock][2 * (K > 0 ? K : 1)];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int offset = kWave / 2; offset > 0; offset >>= 1) {
+            const double h2 = __shfl_xor(hi[k], offset, kWave);
+            const double l2 = __shfl_xor(lo[k], offset, kWave);
+            if ((max_mask >> k) & 1) hi[k] = combine<Combine::max>(hi[k], h2);
+            else dd_combine(hi[k], lo[k], h2, l2);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            lds[wave][k] = hi[k];
+            lds[wave][K + k] = lo[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double h = lds[0][k], l = lds[0][K + k];
+            for (int w = 1; w < kWavesPerBlock; ++w) {
+                if ((max_mask >> k) & 1) h = combine<Combine::max>(h, lds[w][k]);
+                else dd_combine(h, l, lds[w][k], lds[w][K + k]);
+            }
+            hi[k] = h;
+            lo[k] = l;
+        }
+    }
+    __syncthreads();
+}
+
+// Compensated partials: value k of block b as hi at row k, lo at row K + k
+// (2K <= kMaxReduceValues rows).
+template<int K>
+__device__ inline void store_partials_dd(double (&hi)[K], double (&lo)[K], std::uint64_t max_mask,
+                                         ReduceScratch scratch)
+{
+    static_assert(2 * K <= kMaxReduceValues, "too many compensated values");
+    block_reduce_dd<K>(hi, lo, max_mask);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            scratch.partials[k * scratch.stride + blockIdx.x] = hi[k];
+            scratch.partials[(K + k) * scratch.stride + blockIdx.x] = lo[k];
+        }
+    }
+}
+
+// Finisher of compensated partials: pairs combined in a fixed order (block
+// b = t, t + 256, ... per thread, then the block tree); the finalizer gets
+// hi + lo rounded once.
+template<int K, class Fin>
+__global__ __launch_bounds__(256) void k_finish_dd(Fin fin, unsigned int nblocks, std::uint64_t max_mask,
+                                                   ReduceScratch scratch, const int* stop)
+{
+    if (stop != nullptr && *stop != 0) return;
+    __shared__ double totals[K > 0 ? K : 1];
+    double hi[K], lo[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) hi[k] = lo[k] = 0.0;
+    for (unsigned int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        double ph[K], pl[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ph[k] = scratch.partials[k * scratch.stride + b];
+            pl[k] = scratch.partials[(K + k) * scratch.stride + b];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if ((max_mask >> k) & 1) hi[k] = combine<Combine::max>(hi[k], ph[k]);
+            else dd_combine(hi[k], lo[k], ph[k], pl[k]);
+        }
+    }
+    block_reduce_dd<K>(hi, lo, max_mask);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) totals[k] = ((max_mask >> k) & 1) ? hi[k] : hi[k] + lo[k];
+    }
+    __syncthreads();
+    fin(static_cast<const double*>(totals));
+}
+
+template<int K, class Fin>
+inline void launch_finish_dd(hipStream_t st, const Fin& fin, unsigned int nblocks, std::uint64_t max_mask,
+                             ReduceScratch scratch, const int* stop)
+{
+    k_finish_dd<K, Fin><<<1, kBlock, 0, st>>>(fin, nblocks, max_mask, scratch, stop);
 }
 
 // Sum over the block, result on every thread.

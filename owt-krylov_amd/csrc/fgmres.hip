@@ -27,6 +27,22 @@
 
 using namespace owtk;
 
+// ArnoldiSnapshot (core.hpp:463-534) on the device: the last cycle's
+// unrotated Hessenberg columns and its preconditioned basis vectors.  The
+// columns are written by the Givens finisher of each Arnoldi step (FP64,
+// column stride kStride) and read back once when the cycle ends; the basis
+// vectors keep the solver's owned + ghost layout with zero ghosts, so each
+// one is a valid SpMV input.
+struct owtk_arnoldi_snapshot_s {
+    static constexpr int kStride = 61;   // restart limit + 1
+    owtk_context ctx = nullptr;
+    std::int64_t maximum_columns = 0, columns = 0;
+    std::int64_t ld = -1, owned = -1;    // layout of the basis vectors
+    owtk::DeviceArray<double> h;         // kStride * (kStride - 1)
+    owtk::DeviceArray<double> basis;     // maximum_columns * ld
+    std::vector<double> host_h;          // h as of the last finished cycle
+};
+
 namespace owtk {
 
 namespace {
@@ -54,6 +70,43 @@ struct FgState {
 };
 
 __device__ inline double& Hd(FgState* s, int row, int col) { return s->H[col * kLd + row]; }
+
+static_assert(owtk_arnoldi_snapshot_s::kStride == kLd, "snapshot stride follows the Hessenberg stride");
+
+// ArnoldiSnapshot::clear / begin_cycle / finish_cycle (core.hpp:465-508).
+void snapshot_clear(owtk_arnoldi_snapshot sn)
+{
+    if (!sn) return;
+    sn->columns = 0;
+    sn->maximum_columns = 0;
+}
+
+void snapshot_begin(owtk_arnoldi_snapshot sn, int m, std::int64_t ld, std::int64_t owned, hipStream_t st)
+{
+    sn->maximum_columns = m;
+    sn->columns = 0;
+    if (!sn->h.get()) sn->h.allocate(static_cast<std::size_t>(kLd) * kMaxRestart);
+    const std::size_t need = static_cast<std::size_t>(m) * static_cast<std::size_t>(ld);
+    if (sn->basis.count < need || sn->ld != ld || sn->owned != owned) {
+        if (sn->basis.count < need) sn->basis.allocate(need);
+        // ghost entries are never written afterwards
+        kern::fill_zero_bytes(st, sn->basis.get(), sn->basis.count * sizeof(double));
+        sn->ld = ld;
+        sn->owned = owned;
+    }
+}
+
+void snapshot_finish(owtk_arnoldi_snapshot sn, const double* Z, int columns, owtk_context ctx)
+{
+    const hipStream_t st = ctx->stream;
+    if (columns > 0)
+        OWTK_HIP(hipMemcpyAsync(sn->basis.get(), Z, static_cast<std::size_t>(columns) * sn->ld * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+    sn->host_h.resize(sn->h.count);
+    OWTK_HIP(hipMemcpyAsync(sn->host_h.data(), sn->h.get(), sn->h.count * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    sn->columns = columns;
+}
 
 // Post-process h(k+1,k): finiteness, breakdown test, reciprocal
 // (krylov_solvers.hpp:346-359).
@@ -355,6 +408,7 @@ struct FinC {
     int k;
     const T* w;
     std::int64_t n;
+    double* snap_h;  // Arnoldi snapshot columns (stride kLd), or null
     __device__ void operator()(const double* t) const
     {
         double verified = 0.0;
@@ -366,6 +420,10 @@ struct FinC {
             settle_h(s, k, verified, false);
             if (s->stop) return;
         }
+        // the unrotated column, as ArnoldiSnapshot::record_column stores it
+        // before the Givens rotations (krylov_solvers.hpp:361-366)
+        if (snap_h)
+            for (int row = 0; row <= k + 1; ++row) snap_h[k * kLd + row] = Hd(s, row, k);
         for (int row = 0; row < k; ++row) {
             const double first = s->cs[row] * Hd(s, row, k) + s->sn[row] * Hd(s, row + 1, k);
             Hd(s, row + 1, k) = -s->sn[row] * Hd(s, row, k) + s->cs[row] * Hd(s, row + 1, k);
@@ -558,13 +616,25 @@ struct FgEngine {
     std::function<void(const T*, T*, const int*)> op;
     std::function<void(const T*, const T*, T*, double*)> residual;
     std::function<void(const T*, T*, const int*)> pc;
+    double* snap_h = nullptr;           // snapshot Hessenberg columns, when one is recorded
 
-    int run(const T* b, T* x, const owtk_solver_options* o, owtk_solver_result* res, Counters& cnt)
+    // `snap` (optional) receives the last cycle's Arnoldi relation; an
+    // application convergence test in `o` sees every iterate (one host round
+    // trip per column).  Both need FP64 vectors.
+    int run(const T* b, T* x, const owtk_solver_options* o, owtk_solver_result* res, Counters& cnt,
+            owtk_arnoldi_snapshot snap = nullptr)
     {
         init_result(res);
-        if (!valid_options(o) || o->restart == 0) return OWTK_OK;
+        snap_h = nullptr;
+        if (!valid_options(o) || o->restart == 0) {
+            snapshot_clear(snap);
+            return OWTK_OK;
+        }
         if (o->restart > static_cast<std::uint64_t>(kMaxRestart))
             throw unsupported_error("restart above 60 is not supported on the device path");
+        const bool app = o->convergence_test != nullptr;
+        if ((app || snap) && sizeof(T) != 8)
+            throw unsupported_error("Arnoldi snapshots and application tests need FP64 vectors");
         const hipStream_t st = ctx->stream;
         const auto scratch = ctx->scratch();
         const int m = static_cast<int>(o->restart);
@@ -614,6 +684,33 @@ struct FgEngine {
             res->arnoldi_norm_verifications = cnt.verif;
             return OWTK_OK;
         };
+        // application_convergence (krylov_solvers.hpp:122-143): the test sees
+        // `cand`; when it accepts, the true residual and ||b|| are measured
+        auto app_check = [&](const T* cand, std::uint64_t iteration) {
+            ctx->sync();
+            if (!o->convergence_test(o->convergence_test_data, iteration, reinterpret_cast<const double*>(cand),
+                                     static_cast<std::size_t>(n)))
+                return false;
+            residual(b, cand, r, d_norm);
+            ++cnt.ops;
+            if constexpr (sizeof(T) == 8) kern::norm_to(st, scratch, reinterpret_cast<const double*>(b), n, d_norm + 1);
+            double two[2];
+            OWTK_HIP(hipMemcpyAsync(two, d_norm, sizeof(two), hipMemcpyDeviceToHost, st));
+            ctx->sync();
+            cnt.reds += 2;
+            set_residual(res, res->recursive_residual_norm, two[0], two[1]);
+            if (!std::isfinite(two[0]) || !std::isfinite(two[1])) {
+                mark(reason_non_finite_scalar);
+            } else {
+                res->status = status_converged;
+                res->converged_by_application = 1;
+            }
+            return true;
+        };
+        if (app && app_check(x, 0)) {
+            snapshot_clear(snap);
+            return finish(-1);
+        }
         // ||b - A x|| and ||b|| in two slots, read back together
         residual(b, x, r, d_norm);
         ++cnt.ops;
@@ -624,14 +721,30 @@ struct FgEngine {
         ctx->sync();
         const double initial = pair[0], rhs_norm = pair[1];
         cnt.reds += 2;
-        const double thr = threshold_of(rhs_norm, o);
+        // with an application test only an exactly solved system stops on the
+        // residual (convergence_threshold, core.hpp:549-560)
+        double thr = threshold_of(rhs_norm, o);
+        if (app && std::isfinite(thr)) thr = 0.0;
         res->initial_residual_norm = initial;
         set_residual(res, initial, initial, rhs_norm);
         if (!std::isfinite(initial) || !std::isfinite(thr)) {
             mark(reason_non_finite_scalar);
             return finish(-1);
         }
-        if (initial <= thr) return finish(status_converged);
+        if (initial <= thr) {
+            snapshot_clear(snap);
+            return finish(status_converged);
+        }
+        // the candidate iterate x + Z y formed for the application test
+        DeviceArray<T> cand;
+        if (app) cand.allocate(static_cast<std::size_t>(ld));
+        auto account = [&](int ran) {
+            cnt.pcs += ran;
+            cnt.ops += ran;
+            cnt.reds += ran + hs->reorth_count + hs->verif_count;
+            cnt.reorth += hs->reorth_count;
+            cnt.verif += hs->verif_count;
+        };
         double beta = initial;
         const unsigned int g = dev::stream_grid(n, 4);
         // 1024 blocks: more would multiply the per-block K-value reductions and
@@ -641,6 +754,10 @@ struct FgEngine {
         if (red_blocks < 1) red_blocks = 1;
         const unsigned int gr = static_cast<unsigned int>(red_blocks);
         while (res->iterations < o->maximum_iterations) {
+            if (snap) {
+                snapshot_begin(snap, m, ld, n, st);
+                snap_h = snap->h.get();
+            }
             // beta = ||r|| (the reference re-reduces it; same vector, same value)
             ++cnt.reds;
             k_fg_v0<T><<<g, kBlock, 0, st>>>(V, r, n, beta);
@@ -662,6 +779,43 @@ struct FgEngine {
                 pc(vk, zk, halt);
                 op(zk, w, halt);
                 launch_passes(k, V, ld, w, vnext, dstate.get(), gr, st, scratch);
+                if (app) {
+                    // the iterate after column k, formed without touching x
+                    // (krylov_solvers.hpp:398-427)
+                    OWTK_HIP(hipMemcpyAsync(hs, dstate.get(), sizeof(FgState), hipMemcpyDeviceToHost, st));
+                    ctx->sync();
+                    if (hs->stop) break;
+                    res->recursive_residual_norm = hs->recursive;
+                    Coefficients cc{};
+                    bool nonsingular = true;
+                    for (int row = k + 1; row-- > 0;) {
+                        double value = hs->g[row];
+                        for (int j = row + 1; j <= k; ++j) value -= hs->H[j * kLd + row] * cc.y[j];
+                        if (hs->H[row * kLd + row] == 0.0) {
+                            nonsingular = false;
+                            break;
+                        }
+                        cc.y[row] = value / hs->H[row * kLd + row];
+                    }
+                    if (nonsingular) {
+                        OWTK_HIP(hipMemcpyAsync(cand.get(), x, static_cast<std::size_t>(ld) * sizeof(T),
+                                                hipMemcpyDeviceToDevice, st));
+                        k_fg_update<T><<<g, kBlock, 0, st>>>(cand.get(), Z, ld, k + 1, cc, n);
+                        OWTK_LAUNCH_CHECK();
+                        const std::uint64_t iteration = res->iterations + static_cast<std::uint64_t>(k) + 1;
+                        if (app_check(cand.get(), iteration)) {
+                            account(k + 1);
+                            res->iterations = iteration;
+                            if constexpr (sizeof(T) == 8)
+                                if (snap) snapshot_finish(snap, Z, k + 1, ctx);
+                            OWTK_HIP(hipMemcpyAsync(x, cand.get(), static_cast<std::size_t>(n) * sizeof(T),
+                                                    hipMemcpyDeviceToDevice, st));
+                            return finish(-1);
+                        }
+                    }
+                    if (hs->cycle_done) break;
+                    continue;
+                }
                 if ((k + 1) % kCheckEvery == 0 && k + 1 < cols) {
                     // the launches after a stop are no-ops: stop issuing them
                     const int slot = ((k + 1) / kCheckEvery) & 1;
@@ -680,11 +834,7 @@ struct FgEngine {
             // columns whose launches ran: all up to `done`, and on a hard stop
             // the failing column too
             const int ran = hs->stop ? std::max(done + 1, 1) : done;
-            cnt.pcs += ran;
-            cnt.ops += ran;
-            cnt.reds += ran + hs->reorth_count + hs->verif_count;
-            cnt.reorth += hs->reorth_count;
-            cnt.verif += hs->verif_count;
+            account(ran);
             res->iterations += static_cast<std::uint64_t>(hs->stop ? done : done);
             res->recursive_residual_norm = hs->recursive;
             if (hs->stop) {
@@ -692,6 +842,8 @@ struct FgEngine {
                 return finish(-1);
             }
             const bool abreak = hs->abreak != 0;
+            if constexpr (sizeof(T) == 8)
+                if (snap) snapshot_finish(snap, Z, done, ctx);
             // back-substitution (krylov_solvers.hpp:439-457)
             Coefficients coef{};
             auto h = [&](int row, int col) { return hs->H[col * kLd + row]; };
@@ -749,7 +901,7 @@ struct FgEngine {
         // mode never does
         if (may_reorth) dev::launch_finish<K>(st, FinB<K>{s, k, count}, gr, 0, scratch, halt, &s->do_reorth);
         k_fg_pass_c<K, T, L><<<gr, kBlock, 0, st>>>(V, ld, count, w, vnext, n, s, k, scratch);
-        dev::launch_finish<2>(st, FinC<T>{s, k, w, n}, gr, 2, scratch, halt);
+        dev::launch_finish<2>(st, FinC<T>{s, k, w, n, snap_h}, gr, 2, scratch, halt);
         k_fg_pass_d<T><<<dev::stream_grid(n, 4), kBlock, 0, st>>>(w, vnext, n, s);
         OWTK_LAUNCH_CHECK();
     }
@@ -835,14 +987,14 @@ void copy_stop_f32(hipStream_t st, const float* in, float* out, std::int64_t n, 
 }
 
 int fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, const owtk_solver_options* o,
-               owtk_solver_result* res)
+               owtk_solver_result* res, owtk_arnoldi_snapshot snap = nullptr)
 {
     const auto t0 = std::chrono::steady_clock::now();
     if (pc && pc->kind != OWTK_PC_IDENTITY && pc->storage != OWTK_F64)
         throw std::invalid_argument("FP64 solve needs an FP64 preconditioner");
     auto e = fp64_engine(m, pc);
     Counters cnt;
-    e.run(b, x, o, res, cnt);
+    e.run(b, x, o, res, cnt, snap);
     if (o->collect_timings)
         res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return OWTK_OK;
@@ -1256,6 +1408,11 @@ void owtk_solver_options_default(owtk_solver_options* o)
     o->gmres_orthogonalization = 0;
     o->collect_timings = 0;
     o->idr_shadow_space = 4;
+    o->relaxation = 1.0;
+    o->verify_true_residual = 1;
+    o->reduction = OWTK_REDUCTION_SERIAL;
+    o->convergence_test = nullptr;
+    o->convergence_test_data = nullptr;
 }
 
 void owtk_refinement_options_default(owtk_refinement_options* o)
@@ -1276,8 +1433,78 @@ int owtk_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, cons
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        require_serial_reduction(options, "fgmres");
         m->ctx->activate();
         fgmres_run(m, pc, b, x, options, result);
+    });
+}
+
+int owtk_fgmres_with_snapshot(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                              const owtk_solver_options* options, owtk_arnoldi_snapshot snapshot,
+                              owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        if (snapshot && snapshot->ctx != m->ctx) throw std::invalid_argument("snapshot belongs to another context");
+        require_serial_reduction(options, "fgmres");
+        m->ctx->activate();
+        fgmres_run(m, pc, b, x, options, result, snapshot);
+    });
+}
+
+int owtk_arnoldi_snapshot_create(owtk_context ctx, owtk_arnoldi_snapshot* out)
+{
+    return guarded([&] {
+        if (!ctx || !out) throw std::invalid_argument("null argument");
+        auto sn = std::make_unique<owtk_arnoldi_snapshot_s>();
+        sn->ctx = ctx;
+        *out = sn.release();
+    });
+}
+
+int owtk_arnoldi_snapshot_destroy(owtk_arnoldi_snapshot snapshot)
+{
+    return guarded([&] {
+        if (!snapshot) return;
+        snapshot->ctx->activate();
+        snapshot->ctx->sync();
+        delete snapshot;
+    });
+}
+
+int owtk_arnoldi_snapshot_clear(owtk_arnoldi_snapshot snapshot)
+{
+    return guarded([&] {
+        if (!snapshot) throw std::invalid_argument("null argument");
+        snapshot_clear(snapshot);
+    });
+}
+
+int owtk_arnoldi_snapshot_columns(owtk_arnoldi_snapshot snapshot, uint64_t* columns)
+{
+    return guarded([&] {
+        if (!snapshot || !columns) throw std::invalid_argument("null argument");
+        *columns = static_cast<uint64_t>(snapshot->columns);
+    });
+}
+
+int owtk_arnoldi_snapshot_h(owtk_arnoldi_snapshot snapshot, uint64_t row, uint64_t column, double* value)
+{
+    return guarded([&] {
+        if (!snapshot || !value) throw std::invalid_argument("null argument");
+        const auto cols = static_cast<uint64_t>(snapshot->columns);
+        if (column >= cols || row > cols) throw std::out_of_range("Arnoldi snapshot entry");
+        // rows below the recorded subdiagonal are the zeros begin_cycle wrote
+        *value = row > column + 1 ? 0.0 : snapshot->host_h[column * owtk_arnoldi_snapshot_s::kStride + row];
+    });
+}
+
+int owtk_arnoldi_snapshot_basis(owtk_arnoldi_snapshot snapshot, uint64_t column, const double** device_vector)
+{
+    return guarded([&] {
+        if (!snapshot || !device_vector) throw std::invalid_argument("null argument");
+        if (column >= static_cast<uint64_t>(snapshot->columns)) throw std::out_of_range("Arnoldi snapshot basis");
+        *device_vector = snapshot->basis.get() + static_cast<std::int64_t>(column) * snapshot->ld;
     });
 }
 
@@ -1286,6 +1513,7 @@ int owtk_fgmres_refined(owtk_matrix m, owtk_precond inner_pc, const double* b, d
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        require_plain_options(&options->outer, "fgmres_refined");
         m->ctx->activate();
         fgmres_refined_run(m, inner_pc, nullptr, b, x, options, result);
     });
@@ -1296,6 +1524,7 @@ int owtk_fgmres_refined_fallback(owtk_matrix m, owtk_precond inner_pc, owtk_prec
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result || !fallback_pc) throw std::invalid_argument("null argument");
+        require_plain_options(&options->outer, "fgmres_refined_fallback");
         m->ctx->activate();
         fgmres_refined_run(m, inner_pc, fallback_pc, b, x, options, result);
     });
@@ -1403,6 +1632,7 @@ int owtk_recycled_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double
 {
     return guarded([&] {
         if (!m || !b || !x || !space || !options || !result) throw std::invalid_argument("null argument");
+        require_plain_options(options, "recycled_fgmres");
         m->ctx->activate();
         owtk::recycled_fgmres_run(m, pc, b, x, space, options, update_recycle_space != 0, result);
     });

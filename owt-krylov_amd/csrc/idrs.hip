@@ -435,6 +435,7 @@ extern "C" int owtk_idrs(owtk_matrix m, owtk_precond pc, const double* b, double
 {
     return guarded([&] {
         if (!m || !b || !x || !options || !result) throw std::invalid_argument("null argument");
+        if (options->idr_shadow_space != 1) require_plain_options(options, "idrs");  // IDR(1) runs bicgstab
         m->ctx->activate();
         idrs_run(m, pc, b, x, options, result);
     });

@@ -65,15 +65,19 @@ __global__ __launch_bounds__(256) void k_to_bf16(std::uint16_t* dst, const doubl
         dst[i] = float_to_bf16(static_cast<float>(src[i]));
 }
 
-template<class X>
+// Dot product with FP64 accumulation; CMP: compensated (double-double).
+template<class X, bool CMP = false>
 __global__ __launch_bounds__(256) void k_dot(const X* __restrict__ a, const X* __restrict__ b, std::int64_t n,
                                              ReduceScratch scratch, double* out)
 {
-    double red[1] = {0.0};
+    double red[1] = {0.0}, lo[1] = {0.0};
     const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-        red[0] = __builtin_fma(static_cast<double>(a[i]), static_cast<double>(b[i]), red[0]);
-    store_partials<1>(red, 0, scratch);
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if constexpr (CMP) dd_fma(red[0], lo[0], static_cast<double>(a[i]), static_cast<double>(b[i]));
+        else red[0] = __builtin_fma(static_cast<double>(a[i]), static_cast<double>(b[i]), red[0]);
+    }
+    if constexpr (CMP) store_partials_dd<1>(red, lo, 0, scratch);
+    else store_partials<1>(red, 0, scratch);
 }
 
 struct FinStore {
@@ -86,26 +90,34 @@ struct FinStore {
 };
 
 template<class X>
-inline void dot_to(hipStream_t st, ReduceScratch scratch, const X* a, const X* b, std::int64_t n, double* out)
+inline void dot_to(hipStream_t st, ReduceScratch scratch, const X* a, const X* b, std::int64_t n, double* out,
+                   bool compensated = false)
 {
     const unsigned int g = stream_grid(n, 4);
-    k_dot<X><<<g, kBlock, 0, st>>>(a, b, n, scratch, out);
-    launch_finish<1>(st, FinStore{out, 1}, g, 0, scratch, nullptr);
+    if (compensated) {
+        k_dot<X, true><<<g, kBlock, 0, st>>>(a, b, n, scratch, out);
+        launch_finish_dd<1>(st, FinStore{out, 1}, g, 0, scratch, nullptr);
+    } else {
+        k_dot<X, false><<<g, kBlock, 0, st>>>(a, b, n, scratch, out);
+        launch_finish<1>(st, FinStore{out, 1}, g, 0, scratch, nullptr);
+    }
     OWTK_LAUNCH_CHECK();
 }
 
-template<class X>
+template<class X, bool CMP = false>
 __global__ __launch_bounds__(256) void k_norm(const X* __restrict__ a, std::int64_t n, ReduceScratch scratch,
                                               double* out)
 {
-    double red[2] = {0.0, 0.0};
+    double red[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const double v = static_cast<double>(a[i]);
-        red[0] = __builtin_fma(v, v, red[0]);
+        if constexpr (CMP) dd_fma(red[0], lo[0], v, v);
+        else red[0] = __builtin_fma(v, v, red[0]);
         red[1] = combine<Combine::max>(red[1], __builtin_fabs(v));
     }
-    store_partials<2>(red, 2, scratch);
+    if constexpr (CMP) store_partials_dd<2>(red, lo, 2, scratch);
+    else store_partials<2>(red, 2, scratch);
 }
 
 template<class X>
@@ -121,11 +133,17 @@ struct FinNorm {
 };
 
 template<class X>
-inline void norm_to(hipStream_t st, ReduceScratch scratch, const X* a, std::int64_t n, double* out)
+inline void norm_to(hipStream_t st, ReduceScratch scratch, const X* a, std::int64_t n, double* out,
+                    bool compensated = false)
 {
     const unsigned int g = stream_grid(n, 4);
-    k_norm<X><<<g, kBlock, 0, st>>>(a, n, scratch, out);
-    launch_finish<2>(st, FinNorm<X>{a, n, out}, g, 2, scratch, nullptr);
+    if (compensated) {
+        k_norm<X, true><<<g, kBlock, 0, st>>>(a, n, scratch, out);
+        launch_finish_dd<2>(st, FinNorm<X>{a, n, out}, g, 2, scratch, nullptr);
+    } else {
+        k_norm<X, false><<<g, kBlock, 0, st>>>(a, n, scratch, out);
+        launch_finish<2>(st, FinNorm<X>{a, n, out}, g, 2, scratch, nullptr);
+    }
     OWTK_LAUNCH_CHECK();
 }
 

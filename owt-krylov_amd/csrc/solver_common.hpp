@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <string>
 
 namespace owtk {
 
@@ -48,6 +49,24 @@ inline bool valid_options(const owtk_solver_options* o)
     return std::isfinite(o->relative_tolerance) && std::isfinite(o->absolute_tolerance)
         && std::isfinite(o->breakdown_tolerance) && o->relative_tolerance >= 0 && o->absolute_tolerance >= 0
         && o->breakdown_tolerance > 0 && o->maximum_iterations > 0 && o->convergence_check_interval > 0;
+}
+
+// Options only some solvers implement on the device: a non-serial reduction
+// (owtk_bicgstab) and the application convergence test (owtk_bicgstab,
+// owtk_fgmres).  Other solvers reject them instead of silently ignoring them.
+inline void require_serial_reduction(const owtk_solver_options* o, const char* solver)
+{
+    if (o->reduction != OWTK_REDUCTION_SERIAL)
+        throw unsupported_error(std::string(solver)
+                                + ": compensated and mixed-precision reductions are implemented by owtk_bicgstab only");
+}
+
+inline void require_plain_options(const owtk_solver_options* o, const char* solver)
+{
+    require_serial_reduction(o, solver);
+    if (o->convergence_test != nullptr)
+        throw unsupported_error(std::string(solver) + ": the application convergence test is implemented by "
+                                "owtk_bicgstab and owtk_fgmres only");
 }
 
 // convergence_threshold (core.hpp:549-560), without the application callback

@@ -15,13 +15,14 @@ import numpy as np
 from . import _lib
 from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_UNIFORM, MATRIX_BSR,
                    MATRIX_CSR, MATRIX_SPLIT, ORDER_HILBERT, ORDER_IDENTITY, ORDER_RCM, PC_IDENTITY, PC_ILU,
-                   PC_JACOBI, PC_LEARNED, PC_SSOR, InvalidArgumentError, NoDeviceError, OwtkError,
+                   PC_JACOBI, PC_LEARNED, PC_SSOR, REDUCTION_COMPENSATED, REDUCTION_MIXED_PRECISION,
+                   REDUCTION_SERIAL, InvalidArgumentError, NoDeviceError, OwtkError,
                    UnsupportedError, ZeroPivotError, check, lib)
 
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
     "communication_hiding_bicgstab", "idrs", "Multigrid", "RecycleSpace", "recycled_fgmres", "fgmres",
-    "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
+    "ArnoldiSnapshot", "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
 ]
@@ -100,16 +101,23 @@ class Context:
         buf.upload(a)
         return buf
 
-    def dot(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int | None = None) -> float:
+    def dot(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int | None = None, reduction: int | None = None) -> float:
+        """<a, b>; reduction (REDUCTION_*) selects owtk_reduction_dot."""
         out = C.c_double()
         n = a.count if count is None else count
-        check(lib().owtk_dot(self.handle, a.owtk_dtype, n, a.ptr, b.ptr, C.byref(out)))
+        if reduction is None:
+            check(lib().owtk_dot(self.handle, a.owtk_dtype, n, a.ptr, b.ptr, C.byref(out)))
+        else:
+            check(lib().owtk_reduction_dot(self.handle, reduction, a.owtk_dtype, n, a.ptr, b.ptr, C.byref(out)))
         return out.value
 
-    def norm(self, a: "DeviceBuffer", count: int | None = None) -> float:
+    def norm(self, a: "DeviceBuffer", count: int | None = None, reduction: int | None = None) -> float:
         out = C.c_double()
         n = a.count if count is None else count
-        check(lib().owtk_norm(self.handle, a.owtk_dtype, n, a.ptr, C.byref(out)))
+        if reduction is None:
+            check(lib().owtk_norm(self.handle, a.owtk_dtype, n, a.ptr, C.byref(out)))
+        else:
+            check(lib().owtk_reduction_norm(self.handle, reduction, a.owtk_dtype, n, a.ptr, C.byref(out)))
         return out.value
 
     def axpy(self, alpha: float, x: "DeviceBuffer", y: "DeviceBuffer") -> None:
@@ -425,7 +433,8 @@ class Batch:
 
 class GnnModel:
     """Vertex-block graph network of the learned preconditioner
-    (owtk_gnn_*).  A fresh model is the plain truncated Neumann series."""
+    (owtk_gnn_*).  A fresh model (Glorot-initialised network, zero decoder,
+    g = 1) is the plain truncated Neumann series."""
 
     def __init__(self, ctx: Context, block: int, degree: int):
         self.ctx = ctx
@@ -473,6 +482,35 @@ class GnnModel:
 
     def save(self, path) -> None:
         np.savez(path, block=self.block, degree=self.degree, params=self.get_parameters())
+
+    def decoder_slice(self) -> slice:
+        """Position of W_out and b_out in the flat parameter vector (the
+        vector ends with W_out[(degree+1)b][32], b_out, g)."""
+        outs = (self.degree + 1) * self.block
+        end = self.parameter_count - outs
+        return slice(end - outs * 33, end)
+
+    def constant_polynomial(self, matrix: "Matrix") -> "GnnModel":
+        """A model whose coefficients are this model's gamma on `matrix`
+        averaged over the vertices (per degree and component): the best
+        single polynomial the network's output implies on that matrix, the
+        ablation that isolates the per-vertex modulation."""
+        gam = self.coefficients(matrix).astype(np.float64).mean(axis=1).ravel()
+        p = self.get_parameters()
+        p[self.decoder_slice()] = 0.0
+        p[-gam.size:] = gam.astype(np.float32)
+        out = GnnModel(self.ctx, self.block, self.degree)
+        out.set_parameters(p)
+        return out
+
+    def without_network(self) -> "GnnModel":
+        """The same model with its decoder zeroed: gamma = g at every vertex,
+        the learned global polynomial alone (the ablation of the network)."""
+        p = self.get_parameters()
+        p[self.decoder_slice()] = 0.0
+        out = GnnModel(self.ctx, self.block, self.degree)
+        out.set_parameters(p)
+        return out
 
     @classmethod
     def load(cls, ctx: Context, path) -> "GnnModel":
@@ -578,9 +616,21 @@ class Preconditioner:
 
 
 def SolverOptions(**kw) -> _lib.SolverOptions:
+    """owtk_solver_options with the reference's defaults.  convergence_test
+    may be a Python callable test(iteration, device_pointer, count) -> bool
+    (SolverOptions::convergence_test); the ctypes callback is kept on the
+    options object."""
     o = _lib.SolverOptions()
     lib().owtk_solver_options_default(C.byref(o))
     for k, v in kw.items():
+        if k == "convergence_test" and v is not None and not isinstance(v, _lib.CONVERGENCE_TEST):
+            test = v
+
+            def bridge(_user, iteration, ptr, count, _test=test):
+                return 1 if _test(int(iteration), ptr, int(count)) else 0
+
+            v = _lib.CONVERGENCE_TEST(bridge)
+            o._keep_test = v
         setattr(o, k, v)
     return o
 
@@ -616,8 +666,71 @@ def communication_hiding_bicgstab(matrix: Matrix, pc: Preconditioner | None, b: 
     return _solve(lib().owtk_communication_hiding_bicgstab, matrix, pc, b, x, options)
 
 
-def fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None) -> dict:
-    return _solve(lib().owtk_fgmres, matrix, pc, b, x, options)
+def fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, options=None,
+           snapshot: "ArnoldiSnapshot | None" = None) -> dict:
+    """fgmres (krylov_solvers.hpp:159-486) on the device; `snapshot` receives
+    the last cycle's Arnoldi relation (the arnoldi_snapshot argument)."""
+    if snapshot is None:
+        return _solve(lib().owtk_fgmres, matrix, pc, b, x, options)
+    snapshot.local = matrix.local_size
+
+    def with_snapshot(m, p, bp, xp, o, r):
+        return lib().owtk_fgmres_with_snapshot(m, p, bp, xp, o, snapshot.handle, r)
+
+    return _solve(with_snapshot, matrix, pc, b, x, options)
+
+
+class ArnoldiSnapshot:
+    """ArnoldiSnapshot<double> (core.hpp:462-534): the last FGMRES cycle's
+    unrotated Hessenberg columns and preconditioned basis vectors, kept on
+    the device."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.local = 0
+        self.handle = C.c_void_p()
+        check(lib().owtk_arnoldi_snapshot_create(ctx.handle, C.byref(self.handle)))
+
+    def columns(self) -> int:
+        n = C.c_uint64()
+        check(lib().owtk_arnoldi_snapshot_columns(self.handle, C.byref(n)))
+        return n.value
+
+    def h(self, row: int, column: int) -> float:
+        v = C.c_double()
+        check(lib().owtk_arnoldi_snapshot_h(self.handle, row, column, C.byref(v)))
+        return v.value
+
+    def hessenberg(self) -> np.ndarray:
+        """The (columns + 1) x columns unrotated Hessenberg matrix."""
+        k = self.columns()
+        return np.array([[self.h(i, j) for j in range(k)] for i in range(k + 1)]).reshape(k + 1, k)
+
+    def basis_pointer(self, column: int) -> int:
+        p = C.c_void_p()
+        check(lib().owtk_arnoldi_snapshot_basis(self.handle, column, C.byref(p)))
+        return p.value
+
+    def preconditioned_basis(self, column: int) -> np.ndarray:
+        """Z_column (owned + ghost values) copied to the host."""
+        out = np.empty(self.local)
+        check(lib().owtk_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(self.basis_pointer(column)), out.nbytes))
+        return out
+
+    def clear(self) -> None:
+        check(lib().owtk_arnoldi_snapshot_clear(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().owtk_arnoldi_snapshot_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def refinement_options(**kw) -> _lib.RefinementOptions:

@@ -77,6 +77,11 @@ class PrecondInfo(C.Structure):
     ]
 
 
+# int (*)(void* user, uint64_t iteration, const double* device_solution, size_t count)
+CONVERGENCE_TEST = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t)
+REDUCTION_SERIAL, REDUCTION_COMPENSATED, REDUCTION_MIXED_PRECISION = 0, 1, 2
+
+
 class SolverOptions(C.Structure):
     _fields_ = [
         ("relative_tolerance", C.c_double), ("absolute_tolerance", C.c_double),
@@ -84,6 +89,8 @@ class SolverOptions(C.Structure):
         ("convergence_check_interval", C.c_uint64), ("restart", C.c_uint64),
         ("residual_replacement_interval", C.c_uint64), ("gmres_orthogonalization", C.c_int32),
         ("collect_timings", C.c_int32), ("idr_shadow_space", C.c_uint64),
+        ("relaxation", C.c_double), ("verify_true_residual", C.c_int32), ("reduction", C.c_int32),
+        ("convergence_test", CONVERGENCE_TEST), ("convergence_test_data", C.c_void_p),
     ]
 
 
@@ -103,7 +110,7 @@ class SolverResult(C.Structure):
         ("reorthogonalizations", C.c_uint64), ("arnoldi_norm_verifications", C.c_uint64),
         ("initial_residual_norm", C.c_double), ("recursive_residual_norm", C.c_double),
         ("true_residual_norm", C.c_double), ("relative_residual_norm", C.c_double),
-        ("solve_seconds", C.c_double), ("refinement_steps", C.c_uint64),
+        ("solve_seconds", C.c_double), ("converged_by_application", C.c_int32), ("refinement_steps", C.c_uint64),
         ("inner_iterations", C.c_uint64), ("fallback_refinements", C.c_uint64),
     ]
 
@@ -184,6 +191,14 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_communication_hiding_bicgstab": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions),
                                                      C.POINTER(SolverResult)]),
     "owtk_fgmres": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), C.POINTER(SolverResult)]),
+    "owtk_fgmres_with_snapshot": (C.c_int, [_P, _P, _P, _P, C.POINTER(SolverOptions), _P,
+                                            C.POINTER(SolverResult)]),
+    "owtk_arnoldi_snapshot_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "owtk_arnoldi_snapshot_destroy": (C.c_int, [_P]),
+    "owtk_arnoldi_snapshot_clear": (C.c_int, [_P]),
+    "owtk_arnoldi_snapshot_columns": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "owtk_arnoldi_snapshot_h": (C.c_int, [_P, C.c_uint64, C.c_uint64, _DP]),
+    "owtk_arnoldi_snapshot_basis": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "owtk_refinement_options_default": (None, [C.POINTER(RefinementOptions)]),
     "owtk_fgmres_refined": (C.c_int, [_P, _P, _P, _P, C.POINTER(RefinementOptions), C.POINTER(SolverResult)]),
     "owtk_fgmres_refined_fallback": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(RefinementOptions),
@@ -194,6 +209,8 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_axpy_dot": (C.c_int, [_P, C.c_size_t, C.c_double, _P, _P, _P, _DP]),
     "owtk_multidot": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(_P), _P, _DP]),
     "owtk_multidot_noalloc": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(_P), _P, _P]),
+    "owtk_reduction_dot": (C.c_int, [_P, C.c_int32, C.c_int, C.c_size_t, _P, _P, _DP]),
+    "owtk_reduction_norm": (C.c_int, [_P, C.c_int32, C.c_int, C.c_size_t, _P, _DP]),
     "owtk_batch_create": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, C.POINTER(_P)]),
     "owtk_batch_destroy": (C.c_int, [_P]),
     "owtk_batch_set_tile": (C.c_int, [_P, C.c_int32]),

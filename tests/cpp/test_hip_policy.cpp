@@ -1,26 +1,93 @@
-// Drives include/owtk/hip_policy.hpp with stand-in types shaped like the
-// reference's BlockVector and SplitBlockCsrMatrixView accessors
-// (core.hpp:197-395, split_block_csr.hpp:38-124): a split SpMV through
-// HipExecutionPolicy::apply checked against a host loop, and a BiCGSTAB +
-// Jacobi solve through solve_bicgstab checked by its true residual.
+// Drives include/owtk/hip_policy.hpp with stand-in types that carry the
+// reference's names and accessors (core.hpp:24-146, 197-395;
+// split_block_csr.hpp:38-124), in namespace owt::krylov as a caller's code
+// would see them:
+//   - HipExecutionPolicy::apply bitwise against a host loop, a different
+//     matrix at a reused address and an in-place value change;
+//   - device solves called unqualified in the reference's argument order
+//     (krylov_solvers.hpp:159-171, 542-553; pipelined_bicgstab.hpp:20-35):
+//     FGMRES + block-Jacobi ILU(0), FP64-refined FGMRES with FP32 storage,
+//     BiCGSTAB with the compensated reduction, an application convergence
+//     test over a host BlockVector, pipelined BiCGSTAB + SSOR, and the
+//     host-vector route;
+//   each checked by its true residual recomputed on the host.
 #include <owtk/hip_policy.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <functional>
+#include <memory>
 #include <optional>
 #include <vector>
 
+namespace owt::krylov {
+
+enum class SolverStatus { converged, maximum_iterations, breakdown, diverged, invalid_input };
+enum class GmresOrthogonalization {
+    iterated_classical_gram_schmidt,
+    adaptive_classical_gram_schmidt,
+    one_synchronization_classical_gram_schmidt,
+};
+enum class BreakdownReason { none, arnoldi_invariant_subspace, singular_projected_system, biorthogonality_loss };
+
+template<class T>
+class BlockVector {
+public:
+    BlockVector(std::size_t owned, std::size_t ghost, std::size_t block)
+        : owned_(owned), local_(owned + ghost), block_(block), v_((owned + ghost) * block, T(0))
+    {
+    }
+    T* data() { return v_.data(); }
+    const T* data() const { return v_.data(); }
+    std::size_t owned_size() const { return owned_ * block_; }
+    std::size_t local_size() const { return local_ * block_; }
+    std::vector<T>& values() { return v_; }
+    const std::vector<T>& values() const { return v_; }
+
+private:
+    std::size_t owned_, local_, block_;
+    std::vector<T> v_;
+};
+
+template<class T>
+struct SolverOptions {
+    T relative_tolerance = T(1e-8);
+    T absolute_tolerance = T(0);
+    T breakdown_tolerance = T(64) * std::numeric_limits<T>::epsilon();
+    std::size_t maximum_iterations = 1000;
+    std::size_t convergence_check_interval = 10;
+    std::size_t restart = 30;
+    std::size_t residual_replacement_interval = 50;
+    std::size_t idr_shadow_space = 4;
+    T relaxation = T(1);
+    bool verify_true_residual = true;
+    bool collect_timings = false;
+    std::function<bool(std::size_t, const BlockVector<T>&)> convergence_test;
+    GmresOrthogonalization gmres_orthogonalization = GmresOrthogonalization::iterated_classical_gram_schmidt;
+};
+
+template<class T>
+struct SolverTimings {
+    double solve_seconds = 0;
+};
+
+template<class T>
+struct SolverResult {
+    SolverStatus status = SolverStatus::invalid_input;
+    BreakdownReason breakdown_reason = BreakdownReason::none;
+    std::size_t iterations = 0, operator_applications = 0, preconditioner_applications = 0, global_reductions = 0,
+                reorthogonalizations = 0, arnoldi_norm_verifications = 0;
+    T initial_residual_norm = 0, recursive_residual_norm = 0, true_residual_norm = 0, relative_residual_norm = 0;
+    bool converged_by_application = false;
+    std::shared_ptr<SolverTimings<T>> timings;
+    bool converged() const { return status == SolverStatus::converged; }
+};
+
+} // namespace owt::krylov
+
 namespace {
 
-struct Vec {
-    std::vector<double> v;
-    std::size_t owned, local;
-    Vec(std::size_t o, std::size_t l) : v(l, 0.0), owned(o), local(l) {}
-    double* data() { return v.data(); }
-    const double* data() const { return v.data(); }
-    std::size_t owned_size() const { return owned; }
-    std::size_t local_size() const { return local; }
-};
+using Vec = owt::krylov::BlockVector<double>;
 
 struct Split {
     std::size_t n, b;
@@ -34,24 +101,6 @@ struct Split {
     const std::vector<std::int64_t>& off_diagonal_columns() const { return col; }
     const std::vector<double>& diagonal_values() const { return diag; }
     const std::vector<double>& off_diagonal_values() const { return vals; }
-};
-
-struct Options {
-    double relative_tolerance = 1e-10, absolute_tolerance = 0, breakdown_tolerance = 1e-14;
-    std::size_t maximum_iterations = 1000, convergence_check_interval = 10, restart = 30,
-                residual_replacement_interval = 50;
-    int gmres_orthogonalization = 0;
-    bool collect_timings = false;
-};
-
-enum class Status { converged, maximum_iterations, breakdown, diverged, invalid_input };
-enum class Reason { none };
-struct Result {
-    Status status;
-    Reason breakdown_reason;
-    std::size_t iterations, operator_applications, preconditioner_applications, global_reductions,
-        reorthogonalizations, arnoldi_norm_verifications;
-    double initial_residual_norm, recursive_residual_norm, true_residual_norm, relative_residual_norm;
 };
 
 // 1-D chain of n vertices, b components: d = 4 + c, neighbours -1 - 0.1c
@@ -70,35 +119,62 @@ Split chain(std::size_t n, std::size_t b)
     return s;
 }
 
-void host_apply(const Split& s, const Vec& x, Vec& y)
+void host_apply(const Split& s, const double* x, double* y)
 {
     for (std::size_t i = 0; i < s.n; ++i)
         for (std::size_t c = 0; c < s.b; ++c) {
-            double acc = s.diag[i * s.b + c] * x.v[i * s.b + c];
+            double acc = s.diag[i * s.b + c] * x[i * s.b + c];
             for (auto e = s.off[i]; e < s.off[i + 1]; ++e)
-                acc = std::fma(s.vals[e * s.b + c], x.v[s.col[e] * s.b + c], acc);
-            y.v[i * s.b + c] = acc;
+                acc = std::fma(s.vals[e * s.b + c], x[s.col[e] * s.b + c], acc);
+            y[i * s.b + c] = acc;
         }
+}
+
+double true_rel(const Split& s, const std::vector<double>& b, const std::vector<double>& x)
+{
+    std::vector<double> ax(b.size());
+    host_apply(s, x.data(), ax.data());
+    double rn = 0, bn = 0;
+    for (std::size_t i = 0; i < b.size(); ++i) {
+        rn += (b[i] - ax[i]) * (b[i] - ax[i]);
+        bn += b[i] * b[i];
+    }
+    return std::sqrt(rn / bn);
+}
+
+int failures = 0;
+void expect(bool ok, const char* what)
+{
+    if (!ok) {
+        std::printf("FAILED: %s\n", what);
+        ++failures;
+    }
+}
+
+void check_hip_copy(owtk::hip::Context& ctx, double* host, const double* device, std::size_t count)
+{
+    owtk::hip::check(owtk_memcpy_dtoh(ctx.get(), host, device, count * sizeof(double)));
 }
 
 } // namespace
 
 int main()
 {
-    using namespace owtk::hip;
+    using owtk::hip::HipExecutionPolicy;
     if (!HipExecutionPolicy::accelerator_available()) {
         std::printf("no device\n");
         return 2;
     }
     const Split s = chain(1000, 3);
+    const std::size_t n = 3000;
     HipExecutionPolicy policy;
-    Vec x(3000, 3000), y(3000, 3000), yref(3000, 3000);
-    for (std::size_t i = 0; i < 3000; ++i) x.v[i] = std::sin(0.1 * i);
+    Vec x(1000, 0, 3), y(1000, 0, 3), yref(1000, 0, 3);
+    for (std::size_t i = 0; i < n; ++i) x.data()[i] = std::sin(0.1 * i);
     policy.apply(s, x, y);
-    host_apply(s, x, yref);
-    for (std::size_t i = 0; i < 3000; ++i)
-        if (y.v[i] != yref.v[i]) {
-            std::printf("apply mismatch at %zu: %.17g vs %.17g\n", i, y.v[i], yref.v[i]);
+    host_apply(s, x.data(), yref.data());
+    for (std::size_t i = 0; i < n; ++i)
+        if (y.data()[i] != yref.data()[i]) {
+            std::printf("apply mismatch at %zu: %.17g vs %.17g\n", i, y.data()[i], yref.data()[i]);
             return 1;
         }
     // A different matrix at a reused address (smaller), then a value change
@@ -106,8 +182,8 @@ int main()
     {
         std::optional<Split> slot;
         slot.emplace(chain(700, 3));
-        Vec x1(2100, 2100), y1(2100, 2100), r1(2100, 2100);
-        for (std::size_t i = 0; i < 2100; ++i) x1.v[i] = std::cos(0.05 * i);
+        Vec x1(700, 0, 3), y1(700, 0, 3);
+        for (std::size_t i = 0; i < 2100; ++i) x1.data()[i] = std::cos(0.05 * i);
         policy.apply(*slot, x1, y1);
         const void* first = &*slot;
         slot.reset();
@@ -116,69 +192,181 @@ int main()
             std::printf("address not reused\n");
             return 1;
         }
-        Vec x2(1200, 1200), y2(1200, 1200), r2(1200, 1200);
-        for (std::size_t i = 0; i < 1200; ++i) x2.v[i] = std::sin(0.3 * i);
+        Vec x2(400, 0, 3), y2(400, 0, 3), r2(400, 0, 3);
+        for (std::size_t i = 0; i < 1200; ++i) x2.data()[i] = std::sin(0.3 * i);
         policy.apply(*slot, x2, y2);
-        host_apply(*slot, x2, r2);
+        host_apply(*slot, x2.data(), r2.data());
         for (std::size_t i = 0; i < 1200; ++i)
-            if (y2.v[i] != r2.v[i]) {
+            if (y2.data()[i] != r2.data()[i]) {
                 std::printf("reused-address apply mismatch at %zu\n", i);
                 return 1;
             }
         for (auto& v : slot->vals) v *= 1.5;
         slot->diag[7] = 11.0;
         policy.apply(*slot, x2, y2);
-        host_apply(*slot, x2, r2);
+        host_apply(*slot, x2.data(), r2.data());
         for (std::size_t i = 0; i < 1200; ++i)
-            if (y2.v[i] != r2.v[i]) {
+            if (y2.data()[i] != r2.data()[i]) {
                 std::printf("value change not seen at %zu\n", i);
                 return 1;
             }
         std::printf("reused address and in-place value change ok\n");
     }
-    Vec b(3000, 3000), sol(3000, 3000), r(3000, 3000);
-    for (std::size_t i = 0; i < 3000; ++i) b.v[i] = std::cos(0.37 * i);
-    const Result res = solve_bicgstab<Result>(policy, s, b, sol, Options{});
-    host_apply(s, sol, r);
-    double rn = 0, bn = 0;
-    for (std::size_t i = 0; i < 3000; ++i) {
-        rn += (b.v[i] - r.v[i]) * (b.v[i] - r.v[i]);
-        bn += b.v[i] * b.v[i];
-    }
-    const double rel = std::sqrt(rn / bn);
-    std::printf("policy %s: apply bitwise ok; bicgstab status %d iterations %zu true rel %.3e\n",
-                HipExecutionPolicy::name(), static_cast<int>(res.status), res.iterations, rel);
     bool bad_layout_caught = false;
     try {
-        Vec shortv(10, 10);
+        Vec shortv(10, 0, 1);
         policy.apply(s, shortv, y);
     } catch (const std::invalid_argument&) {
         bad_layout_caught = true;
     }
-    if (!bad_layout_caught) {
-        std::printf("layout mismatch not rejected\n");
-        return 1;
+    expect(bad_layout_caught, "layout mismatch rejected");
+    std::printf("policy %s: apply bitwise ok\n", HipExecutionPolicy::name());
+
+    std::vector<double> b(n);
+    for (std::size_t i = 0; i < n; ++i) b[i] = std::cos(0.37 * i);
+
+    // device solves in the reference's argument order, unqualified
+    owtk::hip::Context& ctx = policy.context();
+    owtk::hip::DeviceMatrix dm(ctx, s);
+    auto ilu_params = owtk::hip::Preconditioner::params(OWTK_PC_ILU);
+    ilu_params.block_nodes = 256;
+    owtk::hip::Preconditioner ilu(dm, ilu_params);
+    owtk::hip::DeviceVector<double> db(ctx, n), dx(ctx, n);
+    db.upload(b.data(), n);
+    std::vector<double> hx(n);
+    owt::krylov::SolverOptions<double> opts;
+    opts.relative_tolerance = 1e-10;
+    opts.collect_timings = true;
+
+    owt::krylov::SolverResult<double> r1 = fgmres(dm, db, dx, opts, ilu);
+    dx.download(hx.data(), n);
+    const double rel1 = true_rel(s, b, hx);
+    std::printf("fgmres + block-Jacobi ILU(0): status %d iterations %zu true rel %.3e timed %d\n",
+                static_cast<int>(r1.status), r1.iterations, rel1, r1.timings != nullptr);
+    expect(r1.converged() && rel1 <= 1e-10 && r1.timings != nullptr, "fgmres + ILU(0)");
+
+    dm.add_storage(OWTK_F32);
+    auto ilu32_params = ilu_params;
+    ilu32_params.storage = OWTK_F32;
+    owtk::hip::Preconditioner ilu32(dm, ilu32_params);
+    dx.upload(std::vector<double>(n, 0.0).data(), n);
+    owt::krylov::SolverResult<double> r2 =
+        fgmres_refined(dm, db, dx, opts, ilu32, owtk::hip::refinement_options());
+    dx.download(hx.data(), n);
+    const double rel2 = true_rel(s, b, hx);
+    std::printf("fgmres_refined (FP32 storage) + ILU(0): status %d iterations %zu true rel %.3e\n",
+                static_cast<int>(r2.status), r2.iterations, rel2);
+    expect(r2.converged() && rel2 <= 1e-10, "refined fgmres");
+
+    dx.upload(std::vector<double>(n, 0.0).data(), n);
+    owt::krylov::SolverResult<double> r3s = bicgstab(dm, db, dx, opts, ilu);
+    dx.upload(std::vector<double>(n, 0.0).data(), n);
+    owt::krylov::SolverResult<double> r3 = bicgstab(dm, db, dx, opts, ilu, owtk::hip::CompensatedSerialReduction{});
+    dx.download(hx.data(), n);
+    const double rel3 = true_rel(s, b, hx);
+    std::printf("bicgstab + ILU(0), compensated reduction: status %d iterations %zu (serial %zu) reductions %zu "
+                "(serial %zu) true rel %.3e\n",
+                static_cast<int>(r3.status), r3.iterations, r3s.iterations, r3.global_reductions,
+                r3s.global_reductions, rel3);
+    expect(r3.converged() && rel3 <= 1e-10, "compensated bicgstab");
+    expect(r3.global_reductions == r3s.global_reductions || r3.iterations != r3s.iterations,
+           "compensated bicgstab counts the same reductions");
+
+    // application convergence test over a host BlockVector: stop after the
+    // third iterate; the test sees iterations 0, 1, 2, 3
+    std::vector<std::size_t> seen;
+    double seen_norm = 0;
+    owt::krylov::SolverOptions<double> app = opts;
+    app.convergence_test = [&](std::size_t it, const Vec& sol) {
+        seen.push_back(it);
+        double a = 0;
+        for (std::size_t i = 0; i < sol.owned_size(); ++i) a += sol.data()[i] * sol.data()[i];
+        seen_norm = std::sqrt(a);
+        return it >= 3;
+    };
+    dx.upload(std::vector<double>(n, 0.0).data(), n);
+    owt::krylov::SolverResult<double> r4 = bicgstab(dm, db, dx, app, ilu);
+    dx.download(hx.data(), n);
+    double xn = 0;
+    for (double v : hx) xn += v * v;
+    std::printf("bicgstab + application test: status %d iterations %zu by application %d, test saw %zu calls, "
+                "last |x| %.6e (solution %.6e)\n",
+                static_cast<int>(r4.status), r4.iterations, r4.converged_by_application, seen.size(), seen_norm,
+                std::sqrt(xn));
+    expect(r4.converged() && r4.converged_by_application && r4.iterations == 3, "application test stops at 3");
+    expect(seen == std::vector<std::size_t>({0, 1, 2, 3}), "application test sees iterations 0..3");
+    expect(std::abs(seen_norm - std::sqrt(xn)) <= 1e-12 * std::sqrt(xn), "application test sees the solution");
+    expect(std::abs(r4.relative_residual_norm - true_rel(s, b, hx)) <= 1e-9 * true_rel(s, b, hx),
+           "application stop reports the true residual");
+
+    // fgmres with an application test and an Arnoldi snapshot, unqualified
+    // with the reference's trailing arguments: a stop at iteration 1 keeps a
+    // one-column snapshot, a stop at 0 clears it
+    {
+        owtk::hip::ArnoldiSnapshot snap(ctx);
+        owt::krylov::SolverOptions<double> at1 = opts;
+        at1.convergence_test = [](std::size_t it, const Vec&) { return it == 1; };
+        dx.upload(std::vector<double>(n, 0.0).data(), n);
+        owt::krylov::SolverResult<double> r7 =
+            fgmres(dm, db, dx, at1, owtk::hip::IdentityPreconditioner{}, owtk::hip::SerialReduction{}, nullptr, &snap);
+        std::printf("fgmres + application test + snapshot: status %d iterations %zu by application %d, "
+                    "snapshot columns %zu, h(0,0) %.6e h(1,0) %.6e\n",
+                    static_cast<int>(r7.status), r7.iterations, r7.converged_by_application, snap.columns(),
+                    snap.h(0, 0), snap.h(1, 0));
+        expect(r7.converged_by_application && r7.iterations == 1 && snap.columns() == 1,
+               "application stop keeps its Arnoldi snapshot");
+        // Z_0 = b / |b| for the identity preconditioner and x0 = 0
+        std::vector<double> z0(n);
+        check_hip_copy(ctx, z0.data(), snap.preconditioned_basis(0), n);
+        double bn = 0;
+        for (double v : b) bn += v * v;
+        bn = std::sqrt(bn);
+        double dz = 0;
+        for (std::size_t i = 0; i < n; ++i) dz = std::max(dz, std::abs(z0[i] - b[i] / bn));
+        expect(dz <= 1e-15, "snapshot basis column 0 is b / |b|");
+        bool out_of_range = false;
+        try {
+            (void)snap.h(0, 1);
+        } catch (const std::out_of_range&) {
+            out_of_range = true;
+        }
+        expect(out_of_range, "snapshot entry beyond its columns is out of range");
+        owt::krylov::SolverOptions<double> at0 = opts;
+        at0.convergence_test = [](std::size_t, const Vec&) { return true; };
+        owt::krylov::SolverResult<double> r8 =
+            fgmres(dm, db, dx, at0, owtk::hip::IdentityPreconditioner{}, owtk::hip::SerialReduction{}, nullptr, &snap);
+        expect(r8.converged_by_application && r8.iterations == 0 && snap.columns() == 0,
+               "initial application stop clears a stale snapshot");
     }
-    // the production selection on device vectors: split SSOR + pipelined BiCGSTAB
-    Context ctx(0);
-    DeviceMatrix dm(ctx, s);
-    auto pp = Preconditioner::params(OWTK_PC_SSOR);
-    Preconditioner ssor(dm, pp);
-    DeviceVector<double> db(ctx, 3000), dx(ctx, 3000);
-    db.upload(b.v.data(), 3000);
-    owtk_solver_options o;
-    owtk_solver_options_default(&o);
-    o.relative_tolerance = 1e-10;
-    const owtk_solver_result pr = pipelined_bicgstab(dm, &ssor, db, dx, o);
-    Vec px(3000, 3000), pr_ax(3000, 3000);
-    dx.download(px.v.data(), 3000);
-    host_apply(s, px, pr_ax);
-    double prn = 0;
-    for (std::size_t i = 0; i < 3000; ++i) prn += (b.v[i] - pr_ax.v[i]) * (b.v[i] - pr_ax.v[i]);
-    const double prel = std::sqrt(prn / bn);
-    std::printf("pipelined bicgstab + ssor: status %d iterations %llu true rel %.3e reductions %llu\n", pr.status,
-                static_cast<unsigned long long>(pr.iterations), prel,
-                static_cast<unsigned long long>(pr.global_reductions));
-    if (pr.status != 0 || prel > 1e-10) return 1;
-    return (res.status == Status::converged && rel <= 1e-10) ? 0 : 1;
+
+    // the production selection: split SSOR + pipelined BiCGSTAB
+    owtk::hip::Preconditioner ssor(dm, owtk::hip::Preconditioner::params(OWTK_PC_SSOR));
+    dx.upload(std::vector<double>(n, 0.0).data(), n);
+    owt::krylov::SolverResult<double> r5 = pipelined_bicgstab(dm, db, dx, opts, ssor);
+    dx.download(hx.data(), n);
+    const double rel5 = true_rel(s, b, hx);
+    std::printf("pipelined bicgstab + ssor: status %d iterations %zu true rel %.3e reductions %zu\n",
+                static_cast<int>(r5.status), r5.iterations, rel5, r5.global_reductions);
+    expect(r5.converged() && rel5 <= 1e-10, "pipelined bicgstab + ssor");
+
+    // a reduction or test the device solver does not implement is refused
+    bool refused = false;
+    try {
+        (void)fgmres(dm, db, dx, opts, ilu, owtk::hip::CompensatedSerialReduction{});
+    } catch (const std::runtime_error&) {
+        refused = true;
+    }
+    expect(refused, "fgmres refuses a compensated reduction");
+
+    // host vectors, whole solve on the device
+    Vec hb(1000, 0, 3), hsol(1000, 0, 3);
+    std::copy(b.begin(), b.end(), hb.data());
+    owt::krylov::SolverResult<double> r6 = owtk::hip::solve_bicgstab(
+        policy, s, hb, hsol, opts, owtk::hip::Preconditioner::params(OWTK_PC_JACOBI));
+    const double rel6 = true_rel(s, b, hsol.values());
+    std::printf("host-vector bicgstab + Jacobi: status %d iterations %zu true rel %.3e\n",
+                static_cast<int>(r6.status), r6.iterations, rel6);
+    expect(r6.converged() && rel6 <= 1e-10, "host-vector bicgstab");
+    std::printf("%s\n", failures == 0 ? "all C++ checks passed" : "C++ checks FAILED");
+    return failures == 0 ? 0 : 1;
 }

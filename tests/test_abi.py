@@ -50,6 +50,8 @@ def test_options_defaults_mirror_reference():
     assert o.restart == 30 and o.residual_replacement_interval == 50
     assert o.gmres_orthogonalization == 0
     assert o.breakdown_tolerance == 64 * 2.220446049250313e-16
+    assert o.idr_shadow_space == 4 and o.relaxation == 1.0 and o.verify_true_residual == 1
+    assert o.reduction == owtk.REDUCTION_SERIAL and not o.convergence_test
 
 
 def test_cpp_header_compiles():

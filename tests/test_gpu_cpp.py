@@ -1,6 +1,9 @@
 """The header-only C++ host layer (include/owtk/hip_policy.hpp) on the GPU:
-HipExecutionPolicy::apply bitwise against a host loop and a BiCGSTAB solve,
-through stand-in types with the reference's accessor names."""
+HipExecutionPolicy::apply bitwise against a host loop, and the device solvers
+called unqualified in the reference's argument order (FGMRES + block-Jacobi
+ILU(0), refined FGMRES, compensated BiCGSTAB, an application convergence
+test, pipelined BiCGSTAB + SSOR), through stand-in types with the reference's
+names (tests/cpp/test_hip_policy.cpp)."""
 import pathlib
 import subprocess
 
@@ -18,3 +21,4 @@ def test_cpp_policy_layer():
     assert out.returncode == 0
     assert "apply bitwise ok" in out.stdout
     assert "reused address and in-place value change ok" in out.stdout
+    assert "all C++ checks passed" in out.stdout
