@@ -12,6 +12,40 @@ import pathlib
 import numpy as np
 
 LIB = pathlib.Path(__file__).resolve().parent / "liboracle.so"
+# The same restatement built for x86-64-v4 (AVX-512), used by the CPU
+# baseline on hosts whose CPU has the v4 instructions (select_build).
+LIB_V4 = pathlib.Path(__file__).resolve().parent / "liboracle_v4.so"
+V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def host_has_v4() -> bool:
+    try:
+        flags = set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+        return all(f in flags for f in V4_FLAGS)
+    except OSError:
+        return False
+
+
+def select_build(prefer_v4: bool = True) -> str:
+    """Choose the library before the first call: the x86-64-v4 build when
+    the host has AVX-512 and it exists, else the x86-64-v3 one.  Returns the
+    -march level used."""
+    global LIB
+    if _lib is not None:
+        return "x86-64-v4" if LIB == LIB_V4 else "x86-64-v3"
+    if prefer_v4 and LIB_V4.exists() and host_has_v4():
+        LIB = LIB_V4
+        return "x86-64-v4"
+    return "x86-64-v3"
+
+
+def set_threads(threads: int) -> None:
+    """Threads of the all-cores baseline (1: the serial restatement)."""
+    lib().oracle_set_threads(int(threads))
 
 KIND_CSR, KIND_SPLIT, KIND_BSR = 0, 1, 2
 
@@ -65,6 +99,9 @@ def lib():
         L = C.CDLL(str(LIB))
         P = C.c_void_p
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_set_threads.restype = None
+        L.oracle_get_threads.restype = C.c_int
         L.oracle_spmv.argtypes = [C.POINTER(Matrix), P, P]
         L.oracle_spmv_f32.argtypes = [C.POINTER(Matrix), P, P, P, P]
         L.oracle_dot.restype = C.c_double
@@ -79,6 +116,8 @@ def lib():
         L.oracle_ilu_factors.argtypes = [P, C.POINTER(C.c_int64), P, P, P, P]
         L.oracle_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_fgmres.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
+        L.oracle_fgmres_snapshot.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
+                                             C.POINTER(C.c_uint64), P, P, C.POINTER(Result)]
         L.oracle_pipelined_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                                            C.POINTER(Result)]
@@ -266,6 +305,26 @@ def fgmres(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
     r = Result()
     _check(lib().oracle_fgmres(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o), C.byref(r)))
     return x, r.as_dict()
+
+
+def fgmres_snapshot(matrix: OMatrix, pc: OPreconditioner | None, b, x0=None, **kw):
+    """fgmres with an ArnoldiSnapshot (core.hpp:462-534): returns x, the
+    result, and the snapshot as (H, Z) with H the (columns + 1) x columns
+    unrotated Hessenberg matrix and Z the columns preconditioned basis
+    vectors (local length each)."""
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    m = int(o.restart)
+    h = np.zeros((m, m + 1))  # column-major storage: row j is column j
+    z = np.zeros((m, n_local))
+    cols = C.c_uint64()
+    r = Result()
+    _check(lib().oracle_fgmres_snapshot(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), C.byref(o),
+                                        C.byref(cols), _p(h), _p(z), C.byref(r)))
+    k = cols.value
+    return x, r.as_dict(), (h[:k, :k + 1].T.copy(), z[:k].copy())
 
 
 class OMultigrid:

@@ -34,9 +34,19 @@
 #include <utility>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 thread_local std::string g_error;
+
+// Threads of the all-cores CPU baseline (oracle_set_threads).  With the
+// default of 1 every loop below runs serially in the reference's order, which
+// is what the tests check; with more threads the row, block and vector loops
+// run under OpenMP (results per row unchanged) and the dot products and
+// norms sum per-thread partials (a different summation order).
+int g_threads = 1;
+#define OMP_ROWS _Pragma("omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)")
 
 // ------------------------------------------------------------------ matrix --
 enum Kind { kCsr = 0, kSplit = 1, kBsr = 2 };
@@ -56,6 +66,7 @@ struct Matrix {
 void spmv_csr(const Matrix& a, const double* x, double* y)
 {
     const int bs = a.bs;
+    OMP_ROWS
     for (std::int64_t row = 0; row < a.owned; ++row) {
         double* out = y + row * bs;
         std::fill(out, out + bs, 0.0);
@@ -71,6 +82,7 @@ void spmv_csr(const Matrix& a, const double* x, double* y)
 void spmv_split(const Matrix& a, const double* x, double* y)
 {
     const int bs = a.bs;
+    OMP_ROWS
     for (std::int64_t row = 0; row < a.owned; ++row) {
         double* out = y + row * bs;
         for (int c = 0; c < bs; ++c) out[c] = a.diag[row * bs + c] * x[row * bs + c];
@@ -86,6 +98,7 @@ void spmv_split(const Matrix& a, const double* x, double* y)
 void spmv_bsr(const Matrix& a, const double* x, double* y)
 {
     const int bs = a.bs;
+    OMP_ROWS
     for (std::int64_t row = 0; row < a.owned; ++row) {
         double* dst = y + row * bs;
         std::fill(dst, dst + bs, 0.0);
@@ -156,7 +169,7 @@ Csr local_csr(const Matrix& a, std::int64_t block_nodes, bool sorted = true)
 
 // -------------------------------------------------------------- reductions --
 // SerialReduction::dot (reduction.hpp:114-145): eight lane accumulators.
-double dot(std::int64_t n, const double* a, const double* b)
+double dot_serial(std::int64_t n, const double* a, const double* b)
 {
     double partial[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::int64_t i = 0;
@@ -168,8 +181,73 @@ double dot(std::int64_t n, const double* a, const double* b)
     return sum;
 }
 
-// detail::scaled_norm (reduction.hpp:64-89)
+// All-cores baseline: contiguous per-thread chunks, each summed as above,
+// the chunk sums added in thread order.
+double dot(std::int64_t n, const double* a, const double* b)
+{
+    if (g_threads <= 1 || n < 65536) return dot_serial(n, a, b);
+    std::vector<double> part(static_cast<std::size_t>(g_threads), 0.0);
+#pragma omp parallel num_threads(g_threads)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        const std::int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        part[t] = dot_serial(hi - lo, a + lo, b + lo);
+    }
+    double sum = 0;
+    for (double v : part) sum += v;
+    return sum;
+}
+
+double norm_serial(std::int64_t n, const double* v);
+
+// detail::scaled_norm (reduction.hpp:64-89); all-cores baseline: the maximum
+// and the scaled compensated sums per thread chunk, then combined.
 double norm(std::int64_t n, const double* v)
+{
+    if (g_threads <= 1 || n < 65536) return norm_serial(n, v);
+    const int nt = g_threads;
+    std::vector<double> scale(nt, 0.0), sum(nt, 0.0), corr(nt, 0.0);
+    std::vector<int> nan(nt, 0);
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num(), nth = omp_get_num_threads();
+        const std::int64_t lo = n * t / nth, hi = n * (t + 1) / nth;
+        for (std::int64_t i = lo; i < hi; ++i) {
+            const double m = std::abs(v[i]);
+            if (std::isnan(m)) nan[t] = 1;
+            scale[t] = std::max(scale[t], m);
+        }
+    }
+    double s = 0;
+    for (int t = 0; t < nt; ++t) {
+        if (nan[t]) return std::numeric_limits<double>::quiet_NaN();
+        s = std::max(s, scale[t]);
+    }
+    if (s == 0.0 || std::isinf(s)) return s;
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num(), nth = omp_get_num_threads();
+        const std::int64_t lo = n * t / nth, hi = n * (t + 1) / nth;
+        double acc = 0, c = 0;
+        for (std::int64_t i = lo; i < hi; ++i) {
+            const double ratio = v[i] / s;
+            const double term = ratio * ratio;
+            const double updated = acc + term;
+            c += acc >= term ? (acc - updated) + term : (term - updated) + acc;
+            acc = updated;
+        }
+        sum[t] = acc;
+        corr[t] = c;
+    }
+    double total = 0, c = 0;
+    for (int t = 0; t < nt; ++t) {
+        total += sum[t];
+        c += corr[t];
+    }
+    return s * std::sqrt(total + c);
+}
+
+double norm_serial(std::int64_t n, const double* v)
 {
     double scale = 0;
     for (std::int64_t i = 0; i < n; ++i) {
@@ -559,10 +637,23 @@ struct Ilu : Pc {
         factorize();
         ++updates;
     }
-    // apply (preconditioner.hpp:320-363); in == out allowed.
+    // apply (preconditioner.hpp:320-363); in == out allowed.  All-cores
+    // baseline: the independent block-Jacobi blocks in parallel, each swept
+    // forward then backward in the same row order.
     void apply(const double* in, double* out) override
     {
-        for (std::int64_t row = 0; row < n; ++row) {
+        if (g_threads > 1 && block_nodes > 0 && n > block_nodes) {
+            const std::int64_t blocks = (n + block_nodes - 1) / block_nodes;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+            for (std::int64_t k = 0; k < blocks; ++k)
+                sweep(in, out, k * block_nodes, std::min(n, (k + 1) * block_nodes));
+            return;
+        }
+        sweep(in, out, 0, n);
+    }
+    void sweep(const double* in, double* out, std::int64_t first, std::int64_t last) const
+    {
+        for (std::int64_t row = first; row < last; ++row) {
             double* y = out + row * bs;
             if (in != out) std::copy_n(in + row * bs, bs, y);
             for (std::size_t e = row_offsets[row]; e < diagonal_positions[row]; ++e) {
@@ -571,7 +662,7 @@ struct Ilu : Pc {
                 for (int k = 0; k < bs; ++k) y[k] = std::fma(-f[k], prev[k], y[k]);
             }
         }
-        for (std::int64_t row = n; row-- > 0;) {
+        for (std::int64_t row = last; row-- > first;) {
             double* x = out + row * bs;
             for (std::size_t e = diagonal_positions[row] + 1; e < row_offsets[row + 1]; ++e) {
                 const double* f = values.data() + e * bs;
@@ -670,6 +761,7 @@ struct Solve {
     void true_residual(const double* b, const double* x, double* res, double* work)
     {
         op(x, work);
+        OMP_ROWS
         for (std::int64_t i = 0; i < n; ++i) res[i] = b[i] - work[i];
     }
     void set_residual(double recursive, double truen, double rhs_norm)
@@ -1061,6 +1153,7 @@ void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_
                 return;
             }
             const double beta = (rho / rho_prev) * (alpha / omega);
+            OMP_ROWS
             for (std::int64_t i = 0; i < s.n; ++i)
                 search[i] = std::fma(beta, std::fma(-omega, osearch[i], search[i]), residual[i]);
         }
@@ -1072,9 +1165,11 @@ void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_
             return;
         }
         alpha = rho / sv;
+        OMP_ROWS
         for (std::int64_t i = 0; i < s.n; ++i) inter[i] = std::fma(-alpha, osearch[i], residual[i]);
         const double inorm = s.nrm(inter.data());
         if (inorm <= thr) {
+            OMP_ROWS
             for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(alpha, psearch[i], x[i]);
             s.true_residual(b, x, residual.data(), work.data());
             const double tn = s.nrm(residual.data());
@@ -1100,8 +1195,11 @@ void bicgstab(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_
             mark(std::isfinite(omega) ? omega_zero : non_finite_scalar);
             return;
         }
+        OMP_ROWS
         for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(alpha, psearch[i], x[i]);
+        OMP_ROWS
         for (std::int64_t i = 0; i < s.n; ++i) x[i] = std::fma(omega, pinter[i], x[i]);
+        OMP_ROWS
         for (std::int64_t i = 0; i < s.n; ++i) residual[i] = std::fma(-omega, ointer[i], inter[i]);
         const bool replace = o.residual_replacement_interval > 0 && it % o.residual_replacement_interval == 0;
         const bool check = it % o.convergence_check_interval == 0;
@@ -1694,12 +1792,29 @@ struct Multigrid {
     }
 };
 
-// fgmres (krylov_solvers.hpp:159-486)
+// ArnoldiSnapshot<double> (core.hpp:462-534): unrotated Hessenberg columns
+// and the preconditioned basis of the last cycle.
+struct Snapshot {
+    std::size_t maximum_columns = 0, columns = 0;
+    std::vector<double> H;                 // column stride maximum_columns + 1
+    std::vector<std::vector<double>> Z;    // owned values, ghosts zero
+    void clear()
+    {
+        columns = maximum_columns = 0;
+        H.clear();
+        Z.clear();
+    }
+};
+
+// fgmres (krylov_solvers.hpp:159-486); `snap` as the arnoldi_snapshot argument
 void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_options& o, oracle_result& r,
-            double* cached = nullptr, bool* cache_ready = nullptr)
+            double* cached = nullptr, bool* cache_ready = nullptr, Snapshot* snap = nullptr)
 {
     init_result(r);
-    if (!valid_options(o) || o.restart == 0) return;
+    if (!valid_options(o) || o.restart == 0) {
+        if (snap) snap->clear();
+        return;
+    }
     Solve s(a, pc, r);
     s.cached = cached;
     s.cache_ready = cache_ready;
@@ -1724,12 +1839,19 @@ void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_op
         return;
     }
     if (initial <= thr) {
+        if (snap) snap->clear();
         r.status = converged;
         return;
     }
     const double eps = std::numeric_limits<double>::epsilon();
     const double dmin = std::numeric_limits<double>::min();
     while (r.iterations < o.maximum_iterations) {
+        if (snap) {  // begin_cycle (core.hpp:465-480)
+            snap->maximum_columns = m;
+            snap->columns = 0;
+            snap->H.assign((m + 1) * m, 0.0);
+            snap->Z.assign(m, std::vector<double>(L, 0.0));
+        }
         const double beta = s.nrm(residual.data());
         for (std::int64_t i = 0; i < s.n; ++i) V[0][i] = residual[i] / beta;
         std::fill(H.begin(), H.end(), 0.0);
@@ -1786,6 +1908,8 @@ void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_op
             } else {
                 abreak = true;
             }
+            if (snap)  // record_column (krylov_solvers.hpp:361-366)
+                for (std::size_t row = 0; row <= cols + 1; ++row) snap->H[cols * (m + 1) + row] = h(row, cols);
             for (std::size_t row = 0; row < cols; ++row) {
                 const double first = cs[row] * h(row, cols) + sn[row] * h(row + 1, cols);
                 h(row + 1, cols) = -sn[row] * h(row, cols) + cs[row] * h(row + 1, cols);
@@ -1815,6 +1939,10 @@ void fgmres(const Matrix& a, Pc& pc, const double* b, double* x, const oracle_op
                 ++r.iterations;
                 break;
             }
+        }
+        if (snap) {  // finish_cycle (krylov_solvers.hpp:435-437): owned values
+            for (std::size_t c = 0; c < cols; ++c) std::copy(Z[c].begin(), Z[c].begin() + s.n, snap->Z[c].begin());
+            snap->columns = cols;
         }
         std::fill(coef.begin(), coef.end(), 0.0);
         for (std::size_t rev = cols; rev-- > 0;) {
@@ -2015,6 +2143,10 @@ static Matrix to_matrix(const oracle_matrix* m)
 }
 
 const char* oracle_last_error(void) { return g_error.c_str(); }
+
+// Threads of the all-cores CPU baseline (1: the serial restatement).
+void oracle_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+int oracle_get_threads(void) { return g_threads; }
 
 int oracle_spmv(const oracle_matrix* m, const double* x, double* y)
 {
@@ -2235,6 +2367,29 @@ int oracle_fgmres(const oracle_matrix* m, void* pc_handle, const double* b, doub
         Identity id(a.owned * a.bs);
         Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
         fgmres(a, pc, b, x, *o, *r);
+    });
+}
+
+// fgmres with an ArnoldiSnapshot: *columns receives its column count, h the
+// (restart + 1) x restart unrotated Hessenberg (column-major, zeros beyond),
+// z the restart preconditioned basis vectors (local length each, ghosts 0).
+int oracle_fgmres_snapshot(const oracle_matrix* m, void* pc_handle, const double* b, double* x,
+                           const oracle_options* o, std::uint64_t* columns, double* h, double* z, oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        Snapshot snap;
+        fgmres(a, pc, b, x, *o, *r, nullptr, nullptr, &snap);
+        *columns = snap.columns;
+        const std::size_t mm = o->restart;
+        const std::size_t L = static_cast<std::size_t>((a.owned + a.ghost) * a.bs);
+        std::fill(h, h + (mm + 1) * mm, 0.0);
+        std::fill(z, z + mm * L, 0.0);
+        if (snap.columns == 0) return;
+        std::copy(snap.H.begin(), snap.H.end(), h);
+        for (std::size_t c = 0; c < snap.columns; ++c) std::copy(snap.Z[c].begin(), snap.Z[c].end(), z + c * L);
     });
 }
 

@@ -634,3 +634,35 @@ def test_recycled_fgmres_properties(c0):
     # a candidate already in the space is rejected
     added, _ = space.add_candidate(om, space.get(1, n)[0])
     assert not added
+
+
+def test_fgmres_snapshot_arnoldi_relation(c0):
+    """fgmres's ArnoldiSnapshot (core.hpp:462-534; krylov_solvers.hpp:252-253,
+    361-366, 435-437), pinned by the Arnoldi relation A Z = V H with V
+    orthonormal:
+    - without a preconditioner Z = V, so Z^T Z = I and Z^T A Z is the square
+      part of H (a stored rotated H or a transposed H fails this);
+    - with Jacobi, V = D Z (z = D^-1 v per component), so (D Z)^T (D Z) = I
+      and (A Z)^T (A Z) = H^T H;
+    - Z_0 = b / |b| from x0 = 0, and H's subdiagonal holds the norms (> 0);
+    - the column count is the last cycle's, and an empty run clears it."""
+    s = c0
+    om = orc.OMatrix.split(s.nodes, 0, 3, s.offsets, s.columns, s.diagonal, s.off_diagonal)
+    _, res, (H, Z) = orc.fgmres_snapshot(om, None, s.rhs, restart=20, maximum_iterations=20,
+                                         relative_tolerance=1e-15)
+    k = H.shape[1]
+    assert k == 20 and H.shape == (21, 20) and Z.shape[0] == 20
+    assert np.max(np.abs(Z @ Z.T - np.eye(k))) < 1e-12
+    AZ = np.array([om.spmv(z) for z in Z])
+    assert np.max(np.abs(Z @ AZ.T - H[:k, :])) < 1e-12 * np.max(np.abs(H))
+    assert np.max(np.abs(Z[0] - s.rhs / np.linalg.norm(s.rhs))) < 1e-15
+    assert np.all(np.diag(H, -1) > 0) and np.all(np.tril(H, -2) == 0)
+    jac = orc.OPreconditioner(om, orc.OPreconditioner.JACOBI)
+    _, res, (H, Z) = orc.fgmres_snapshot(om, jac, s.rhs, restart=5, maximum_iterations=12, relative_tolerance=1e-15)
+    assert res["status"] == 1 and H.shape == (3, 2)  # cycles of 5, 5, 2
+    V = Z * s.diagonal.reshape(-1)[None, :]
+    assert np.max(np.abs(V @ V.T - np.eye(2))) < 1e-12
+    AZ = np.array([om.spmv(z) for z in Z])
+    assert np.max(np.abs(AZ @ AZ.T - H.T @ H)) < 1e-12 * np.max(np.abs(H)) ** 2
+    _, _, (H, Z) = orc.fgmres_snapshot(om, None, np.zeros_like(s.rhs), restart=5)
+    assert H.shape == (1, 0) and Z.shape[0] == 0
