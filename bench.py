@@ -698,6 +698,7 @@ def main():
     ap.add_argument("--orth", type=int, default=2,
                     help="Gram-Schmidt of the inner FGMRES: 0 iterated CGS, 1 adaptive, 2 one-synchronization "
                          "(GmresOrthogonalization, core.hpp:32-39); inner products stay FP64")
+    ap.add_argument("--model", default=str(MODEL), help="learned preconditioner parameters (.npz)")
     ap.add_argument("--workload", choices=["headline", "batched", "bsr", "pipelined", "multigrid", "recycling"],
                     default="headline")
     args = ap.parse_args()
@@ -738,7 +739,8 @@ def main():
         ilu_low = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=BLOCK_NODES, storage=storage)
         ctx.synchronize()
         setup["ilu0_low_s"] = time.perf_counter() - t0
-        model = owtk.GnnModel.load(ctx, MODEL)
+        model_path = pathlib.Path(args.model)
+        model = owtk.GnnModel.load(ctx, model_path)
         t0 = time.perf_counter()
         learned = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=pc_storage)
         ctx.synchronize()
@@ -863,7 +865,7 @@ def main():
                                    "Hilbert order, seed 0; rtol 1e-8 (true FP64 residual)",
                        "headline": f"FP64-refined FGMRES(30) (Gram-Schmidt mode {args.orth}, FP64 inner products), "
                                    f"{args.storage} matrix storage, learned vertex-block preconditioner on "
-                                   f"{args.pc_storage} matrix storage (degree {model.degree}, {MODEL.name}), "
+                                   f"{args.pc_storage} matrix storage (degree {model.degree}, {model_path.name}), "
                                    f"ILU(0) fallback",
                        "baseline": "FP64 BiCGSTAB + FP64 block-Jacobi ILU(0), 4096-vertex blocks"},
             "fp64_bicgstab_ilu0_ms": round(ms64, 3), "speedup_vs_fp64": round(ms64 / headline, 3),

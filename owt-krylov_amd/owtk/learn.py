@@ -92,12 +92,31 @@ def build_problems(ctx, model, count, nx, ny, sigma, seed0, correlation=16.0, la
     return problems
 
 
-def train(model, problems, epochs=30, lr=3e-3, batch=4, seed=0, lr_final=3e-4):
+def train(model, problems, epochs=30, lr=3e-3, batch=4, seed=0, lr_final=3e-4, select=None, select_every=5):
+    """Adam with a cosine learning-rate decay.  With `select` (a function of
+    the model returning a score, lower is better, e.g. the mean FGMRES
+    iteration count on held-out systems), the parameters are scored every
+    `select_every` epochs and at the end, and the best scored parameters are
+    restored on return (the training loss is only a proxy for the solver's
+    convergence).  Returns the loss history and the selection record."""
     rng = np.random.default_rng(seed)
-    history = []
+    history, record = [], []
+    best = None
     steps = epochs * ((len(problems) + batch - 1) // batch)
     step = 0
     t0 = time.perf_counter()
+
+    def score(epoch):
+        nonlocal best
+        if select is None:
+            return
+        val = float(select(model))
+        record.append({"epoch": epoch, "score": val})
+        log(f"[learn] selection score after epoch {epoch}: {val:.2f}")
+        if best is None or val < best[0]:
+            best = (val, model.get_parameters().copy(), epoch)
+
+    score(0)
     for epoch in range(epochs):
         order = rng.permutation(len(problems))
         total = 0.0
@@ -112,7 +131,28 @@ def train(model, problems, epochs=30, lr=3e-3, batch=4, seed=0, lr_final=3e-4):
             step += 1
         history.append(total / len(problems))
         log(f"[learn] epoch {epoch + 1}/{epochs}: loss {history[-1]:.5f} ({time.perf_counter() - t0:.1f}s)")
-    return history
+        if (epoch + 1) % select_every == 0 or epoch + 1 == epochs:
+            score(epoch + 1)
+    if best is not None:
+        model.set_parameters(best[1])
+        record.append({"selected_epoch": best[2], "score": best[0]})
+    return history, record
+
+
+def iteration_score(ctx, systems, restart=30, rtol=1e-8):
+    """Mean FGMRES(restart) iteration count to rtol over (system, matrix)
+    pairs with the model's preconditioner: the selection score of train()."""
+    def score(model):
+        its = []
+        for s, m in systems:
+            b = ctx.upload(s.rhs)
+            x = ctx.empty(s.nodes * s.block)
+            pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model)
+            res, _ = fgmres_run(ctx, m, pc, b, x, restart, rtol)
+            its.append(res["iterations"] if res["status_name"] == "converged" else 10 * res["iterations"])
+            pc.close()
+        return float(np.mean(its))
+    return score
 
 
 def evaluation_loss(model, problems):
@@ -189,6 +229,11 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=3e-3)
     t.add_argument("--seed", type=int, default=1000)
     t.add_argument("--init", default=None, help="start from these parameters")
+    t.add_argument("--select-fields", type=int, default=0,
+                   help="score checkpoints by FGMRES iterations on this many held-out systems")
+    t.add_argument("--select-nx", type=int, default=1024)
+    t.add_argument("--select-ny", type=int, default=512)
+    t.add_argument("--select-every", type=int, default=5)
     e = sub.add_parser("eval")
     e.add_argument("--model", required=True)
     e.add_argument("--fields", type=int, default=20)
@@ -208,12 +253,21 @@ def main(argv=None):
             held = problems[-max(1, len(problems) // 10):]
             train_set = problems[:len(problems) - len(held)] or problems
             before = evaluation_loss(model, held)
-            hist = train(model, train_set, args.epochs, args.lr, args.batch, seed=args.seed)
+            select = None
+            if args.select_fields > 0:
+                # held-out systems at a larger size, seeds apart from training and evaluation
+                systems = [instance(ctx, args.select_nx, args.select_ny, args.seed + 50_000 + k, args.sigma, law=law)
+                           for k in range(args.select_fields)]
+                select = iteration_score(ctx, systems)
+            hist, record = train(model, train_set, args.epochs, args.lr, args.batch, seed=args.seed, select=select,
+                                 select_every=args.select_every)
             after = evaluation_loss(model, held)
             model.save(args.out)
             print(json.dumps({"trained": args.out, "instances": args.instances, "grid": [args.nx, args.ny],
                               "law": args.law, "sigma": args.sigma, "degree": args.degree, "epochs": args.epochs,
-                              "batch": args.batch, "lr": args.lr, "seed": args.seed,
+                              "batch": args.batch, "lr": args.lr, "seed": args.seed, "init": args.init,
+                              "selection": {"fields": args.select_fields, "grid": [args.select_nx, args.select_ny],
+                                            "record": record},
                               "held_out_loss_before": before, "held_out_loss_after": after,
                               "held_out_loss_without_network": evaluation_loss(model.without_network(), held),
                               "train_loss": hist}))

@@ -134,11 +134,29 @@ def test_training_reduces_loss(ctx):
     problems = learn.build_problems(ctx, model, 4, 48, 24, 1.5, 100, correlation=4.0)
     held = learn.build_problems(ctx, model, 1, 48, 24, 1.5, 200, correlation=4.0)
     before = learn.evaluation_loss(model, held)
-    hist = learn.train(model, problems, epochs=40, lr=1e-2, batch=2, lr_final=1e-3)
+    hist, record = learn.train(model, problems, epochs=40, lr=1e-2, batch=2, lr_final=1e-3)
     after = learn.evaluation_loss(model, held)
     print(f"train loss {hist[0]:.4f} -> {hist[-1]:.4f}; held-out {before:.4f} -> {after:.4f}")
     assert hist[-1] < 0.95 * hist[0]
     assert after < 0.95 * before
+    assert record == []
+
+
+def test_training_selects_best_scored_checkpoint(ctx):
+    """train(select=...) restores the parameters of the best score: with a
+    score that is the held-out loss itself, the returned model's loss is the
+    minimum of the recorded scores."""
+    from owtk import learn
+    model = owtk.GnnModel(ctx, 3, 3)
+    problems = learn.build_problems(ctx, model, 3, 48, 24, 1.5, 300, correlation=4.0)
+    held = learn.build_problems(ctx, model, 1, 48, 24, 1.5, 400, correlation=4.0)
+    _, record = learn.train(model, problems, epochs=12, lr=1e-2, batch=2, lr_final=1e-3,
+                            select=lambda mdl: learn.evaluation_loss(mdl, held), select_every=3)
+    scores = [r["score"] for r in record if "epoch" in r]
+    chosen = record[-1]
+    print(record)
+    assert len(scores) == 5 and chosen["score"] == min(scores)
+    assert learn.evaluation_loss(model, held) == pytest.approx(chosen["score"], rel=1e-6)
 
 
 def test_learned_preconditioned_fgmres(ctx, lognormal):
