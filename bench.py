@@ -284,10 +284,34 @@ def cpu_baseline(s, x_ref, samples=10, sample_iterations=3):
     return out
 
 
+def bsr_kernel_loop(steps):
+    """The dense 4x4 BSR SpMV of the 1024x1024 grid (FP64 values, then BF16
+    values with FP32 vectors), `steps` launches each, for rocprofv3 runs."""
+    with owtk.Context(0) as ctx:
+        n, off, col, val, _ = owtk.generate_bsr(BSR_GRID, BSR_GRID, 4, SEED, ordering=owtk.ORDER_HILBERT)
+        m = owtk.Matrix.bsr(ctx, n, 0, 4, off, col, val)
+        m.add_storage(owtk.BF16)
+        x = ctx.upload(np.random.default_rng(0).uniform(-1, 1, n * 4))
+        y = ctx.empty(n * 4)
+        xf = ctx.upload(np.random.default_rng(0).uniform(-1, 1, n * 4).astype(np.float32))
+        yf = ctx.empty(n * 4, np.float32)
+        for _ in range(steps):
+            m.spmv(x, y)
+        for _ in range(steps):
+            m.spmv(xf, yf, owtk.BF16)
+        ctx.synchronize()
+        info = m.info()
+        print(json.dumps({"kernel": "bsr", "launches": steps, "vertices": n,
+                          "spmv_bytes_f64": info.spmv_bytes_f64, "spmv_bytes_bf16": info.spmv_bytes_bf16}))
+
+
 def kernel_loop(kind, steps, pc_storage="f32"):
     """Kernel loops for rocprofv3 counter runs: FP64 split SpMV, the learned
     preconditioner apply (FP32 storage), or a copy kernel of known size
     (FP64 -> FP32 conversion: 8 B read, 4 B written)."""
+    if kind == "bsr":
+        bsr_kernel_loop(steps)
+        return
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
         n = s.nodes * BS
@@ -690,7 +714,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["spmv", "copy", "learned"], default=None)
+    ap.add_argument("--kernel", choices=["spmv", "copy", "learned", "bsr"], default=None)
     ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
                     help="matrix storage of the inner FGMRES (SpMV and ILU fallback)")
     ap.add_argument("--pc-storage", choices=["f32", "bf16"], default="bf16",

@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void spmv_csr_cd(const std::int32_t* __restric
 }
 
 // Dense BSR, SELL-64 layout, one lane per row, BS x BS blocks.
-template<int BS, class V, class X, class E>
+template<int BS, class V, class X, class E, int U>
 __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restrict__ slice_ptr,
                                                      const std::int32_t* __restrict__ row_len,
                                                      const std::int32_t* __restrict__ cols,
@@ -386,20 +386,38 @@ __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restr
 #pragma unroll
         for (int r = 0; r < BS; ++r) acc[r] = A(0);
         const std::int64_t base = slice_ptr[slice];
+        const int width = static_cast<int>((slice_ptr[slice + 1] - base) >> 6);
         const int len = row_len[row];
-        for (int k = 0; k < len; ++k) {
-            const std::int64_t slot = base + 64 * k + lane;
-            const std::int64_t col = cols[slot];
-            const V* blk = vals + (slot - lane) * (BS * BS) + lane;  // element j at blk[64 j]
-            A xc[BS];
+        // U slots per pass, all loads issued before the arithmetic; a slot
+        // past the slice's width is clamped to its last run (in bounds) and
+        // a slot past the row's length is dropped by a select, so each
+        // block row still adds its blocks in slot order
+        for (int k0 = 0; k0 < len; k0 += U) {
+            std::int32_t colk[U];
 #pragma unroll
-            for (int c = 0; c < BS; ++c) xc[c] = static_cast<A>(x[col * BS + c]);
+            for (int u = 0; u < U; ++u) {
+                const int kk = k0 + u < width ? k0 + u : width - 1;
+                colk[u] = cols[base + 64 * kk + lane];
+            }
+            A v[U][BS * BS], xc[U][BS];
 #pragma unroll
-            for (int r = 0; r < BS; ++r) {
-                A t = A(0);
+            for (int u = 0; u < U; ++u) {
+                const int kk = k0 + u < width ? k0 + u : width - 1;
+                const V* blk = vals + (base + 64 * kk) * (BS * BS) + lane;  // element j at blk[64 j]
 #pragma unroll
-                for (int c = 0; c < BS; ++c) t = dfma(static_cast<A>(load_value(blk, 64 * (r * BS + c))), xc[c], t);
-                acc[r] = acc[r] + t;
+                for (int j = 0; j < BS * BS; ++j) v[u][j] = static_cast<A>(load_value(blk, 64 * j));
+#pragma unroll
+                for (int c = 0; c < BS; ++c) xc[u][c] = static_cast<A>(x[static_cast<std::int64_t>(colk[u]) * BS + c]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+#pragma unroll
+                for (int r = 0; r < BS; ++r) {
+                    A t = A(0);
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) t = dfma(v[u][r * BS + c], xc[u][c], t);
+                    acc[r] = k0 + u < len ? acc[r] + t : acc[r];
+                }
             }
         }
 #pragma unroll
@@ -467,11 +485,15 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
         const unsigned int grid = static_cast<unsigned int>(blocks);
         launched = grid;
         if (m->dense()) {
+            // slots per pass: 2 for FP32/BF16 values (c2 BF16: 54.6 -> 50.7 us),
+            // 1 for FP64, whose 16 values per block already keep enough loads
+            // in flight (2 and 4 measured slower)
+            constexpr int kBsrU = sizeof(V) == 8 ? 1 : 2;
             switch (m->bs) {
-            case 2: spmv_sell_bsr<2, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
-            case 3: spmv_sell_bsr<3, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
-            case 4: spmv_sell_bsr<4, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
-            default: spmv_sell_bsr<1, V, X, E><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            case 2: spmv_sell_bsr<2, V, X, E, kBsrU><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            case 3: spmv_sell_bsr<3, V, X, E, kBsrU><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            case 4: spmv_sell_bsr<4, V, X, E, kBsrU><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
+            default: spmv_sell_bsr<1, V, X, E, kBsrU><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;
             }
         } else if (m->has_diag_array()) {
             launch_sell_cd_bs<V, X, true, E>(m->bs, s.max_width, grid, st, s.slice_ptr.get(), s.row_len.get(),
