@@ -352,6 +352,21 @@ inline std::size_t ilu_chunk_lds(const IluChunkProgram& P)
                                            + 2 * (P.max_a * P.max_nua + P.max_b * P.max_nub));
 }
 
+// count values load(j) -> store(j, v), eight loads in flight before their
+// stores; load() gets only indices below count
+template<class T, class Load, class Store>
+__device__ inline void copy_batched(int count, Load load, Store store)
+{
+    for (int j0 = 0; j0 < count; j0 += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = load(j0 + u < count ? j0 + u : count - 1);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u < count) store(j0 + u, v[u]);
+    }
+}
+
 __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const std::int64_t* __restrict__ offsets,
                                                           const std::int64_t* __restrict__ diag_pos,
                                                           const std::int32_t* __restrict__ row0,
@@ -384,16 +399,35 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
             base = offsets[row_first + lane];
             len = static_cast<int>(offsets[row_first + lane + 1] - base);
         }
-        // 1. row values and the chunk's program, all loads independent
-        for (int q = 0; q < P.kmax; ++q) rv[q * 64 + lane] = q < len ? Fc[base + q] : 0.0;
-        for (int j = 0; j < na; ++j) opa[j * 64 + lane] = P.op_a[opa_off + j * 64 + lane];
-        for (int j = 0; j < na * nua; ++j) upa[j * 64 + lane] = P.upd_a[upa_off + j * 64 + lane];
-        for (int j = 0; j < nb; ++j) opb[j * 64 + lane] = P.op_b[opb_off + j * 64 + lane];
-        for (int j = 0; j < nb * nub; ++j) upb[j * 64 + lane] = P.upd_b[upb_off + j * 64 + lane];
-        // 2. earlier chunks: gather, then the lower entries in column order
-        for (int t = 0; t < nsrc; ++t) {
-            const std::int32_t idx = P.src_a[src_off + t * 64 + lane];
-            ga[t * 64 + lane] = idx >= 0 ? Fc[idx] : 0.0;
+        // 1. row values and the chunk's program, all loads independent and
+        //    issued in batches (a plain loop would wait for each load before
+        //    its LDS store)
+        const int lastq = len > 0 ? len - 1 : 0;
+        copy_batched<double>(P.kmax, [&](int q) { return Fc[base + (q < lastq ? q : lastq)]; },
+                             [&](int q, double v) { rv[q * 64 + lane] = q < len ? v : 0.0; });
+        copy_batched<std::uint32_t>(na, [&](int j) { return P.op_a[opa_off + j * 64 + lane]; },
+                                    [&](int j, std::uint32_t v) { opa[j * 64 + lane] = v; });
+        copy_batched<std::uint16_t>(na * nua, [&](int j) { return P.upd_a[upa_off + j * 64 + lane]; },
+                                    [&](int j, std::uint16_t v) { upa[j * 64 + lane] = v; });
+        copy_batched<std::uint32_t>(nb, [&](int j) { return P.op_b[opb_off + j * 64 + lane]; },
+                                    [&](int j, std::uint32_t v) { opb[j * 64 + lane] = v; });
+        copy_batched<std::uint16_t>(nb * nub, [&](int j) { return P.upd_b[upb_off + j * 64 + lane]; },
+                                    [&](int j, std::uint16_t v) { upb[j * 64 + lane] = v; });
+        // 2. earlier chunks: gather (source indices, then values, each batch
+        //    in flight together), then the lower entries in column order
+        for (int t0 = 0; t0 < nsrc; t0 += 8) {
+            std::int32_t idx[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u < nsrc ? t0 + u : nsrc - 1;
+                idx[u] = P.src_a[src_off + t * 64 + lane];
+            }
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = Fc[idx[u] >= 0 ? idx[u] : 0];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (t0 + u < nsrc) ga[(t0 + u) * 64 + lane] = idx[u] >= 0 ? v[u] : 0.0;
         }
         for (int j = 0; j < na; ++j) {
             const std::uint32_t d = opa[j * 64 + lane];
