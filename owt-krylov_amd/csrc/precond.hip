@@ -188,21 +188,20 @@ __global__ __launch_bounds__(256) void k_block_jacobi_apply(const V* __restrict_
 // the colour-permuted order.  Ghost columns are dropped (rank-local, like the
 // component-diagonal SSOR).  Entries are visited in the row's stored order.
 template<int BS, class V, class X, class I, bool FWD>
-__global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict__ rows, std::int64_t count,
-                                                 const std::int32_t* __restrict__ slice_ptr,
+__global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict__ prow, std::int64_t seg0,
+                                                 std::int64_t count, const std::int32_t* __restrict__ slice_ptr,
                                                  const std::int32_t* __restrict__ row_len,
                                                  const std::int32_t* __restrict__ cols, const V* __restrict__ vals,
-                                                 const std::int64_t* __restrict__ dslot,
-                                                 const std::int32_t* __restrict__ color, int c, std::int64_t owned,
-                                                 double omega, double scale, const X* __restrict__ in,
-                                                 X* __restrict__ y, X* __restrict__ out, const I* __restrict__ inv,
-                                                 const int* stop)
+                                                 const V* __restrict__ dvals, double omega, double scale,
+                                                 const X* __restrict__ in, X* __restrict__ y, X* __restrict__ out,
+                                                 const I* __restrict__ inv, const int* stop)
 {
     using A = typename dev::acc_type<X>::type;
     if (stop && *stop) return;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    const std::int64_t i = rows[t];
+    const std::int64_t q = seg0 + t;  // copy row: consecutive threads read consecutive slots
+    const std::int64_t i = prow[q];
     const A w = static_cast<A>(omega);
     A acc[BS];
     if constexpr (FWD) {
@@ -212,46 +211,59 @@ __global__ __launch_bounds__(256) void k_mc_ssor(const std::int32_t* __restrict_
         A yi[BS];
 #pragma unroll
         for (int r = 0; r < BS; ++r) yi[r] = static_cast<A>(y[i * BS + r]);
-        const std::int64_t ds = dslot[i];
 #pragma unroll
         for (int r = 0; r < BS; ++r) {
             A d = A(0);
 #pragma unroll
-            for (int q = 0; q < BS; ++q)
-                d = dev::dfma(static_cast<A>(dev::load_value(vals, dev::dense_index(ds, r * BS + q, BS * BS))), yi[q],
-                              d);
+            for (int c = 0; c < BS; ++c)
+                d = dev::dfma(static_cast<A>(dev::load_value(dvals, dev::dense_index(q, r * BS + c, BS * BS))),
+                              yi[c], d);
             acc[r] = d * static_cast<A>(scale);
         }
     }
-    const std::int64_t base = slice_ptr[i >> 6];
-    const int lane = static_cast<int>(i & 63);
-    const int len = row_len[i];
+    const std::int64_t base = slice_ptr[q >> 6];
+    const int lane = static_cast<int>(q & 63);
+    const int len = row_len[q];
     const X* src = FWD ? y : out;
     for (int k = 0; k < len; ++k) {
         const std::int64_t slot = base + 64 * k + lane;
         const std::int64_t j = cols[slot];
-        if (j >= owned) continue;
-        const int cj = color[j];
-        if (FWD ? !(cj < c) : !(cj > c)) continue;
         A xj[BS];
 #pragma unroll
-        for (int q = 0; q < BS; ++q) xj[q] = static_cast<A>(src[j * BS + q]);
+        for (int c = 0; c < BS; ++c) xj[c] = static_cast<A>(src[j * BS + c]);
 #pragma unroll
         for (int r = 0; r < BS; ++r)
 #pragma unroll
-            for (int q = 0; q < BS; ++q)
-                acc[r] = dev::dfma(-(w * static_cast<A>(dev::load_value(vals, dev::dense_index(slot, r * BS + q,
+            for (int c = 0; c < BS; ++c)
+                acc[r] = dev::dfma(-(w * static_cast<A>(dev::load_value(vals, dev::dense_index(slot, r * BS + c,
                                                                                               BS * BS)))),
-                                   xj[q], acc[r]);
+                                   xj[c], acc[r]);
     }
     X* dst = FWD ? y : out;
     const I* blk = inv + i * (BS * BS);
 #pragma unroll
     for (int r = 0; r < BS; ++r) {
-        A s = A(0);
+        A sum = A(0);
 #pragma unroll
-        for (int q = 0; q < BS; ++q) s = dev::dfma(static_cast<A>(blk[r * BS + q]), acc[q], s);
-        dst[i * BS + r] = static_cast<X>(s);
+        for (int c = 0; c < BS; ++c) sum = dev::dfma(static_cast<A>(blk[r * BS + c]), acc[c], sum);
+        dst[i * BS + r] = static_cast<X>(sum);
+    }
+}
+
+// Copy slots' block values from the matrix: dst[dense_index(s, j, bb)] =
+// src[dense_index(map[s], j, bb)] (0 for padding).  Thread t writes dst[t].
+template<class V>
+__global__ __launch_bounds__(256) void k_mc_gather(V* __restrict__ dst, const V* __restrict__ src,
+                                                   const std::int64_t* __restrict__ map, std::int64_t slots, int bb)
+{
+    const std::int64_t total = slots * bb;
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
+        const std::int64_t slice = t / (64 * bb);
+        const int j = static_cast<int>((t / 64) % bb);
+        const std::int64_t sl = slice * 64 + (t & 63);
+        const std::int64_t from = map[sl];
+        dst[t] = from >= 0 ? src[dev::dense_index(from, j, bb)] : V(0);
     }
 }
 
@@ -1480,32 +1492,130 @@ void dense_ssor_setup(owtk_precond pc)
     std::vector<std::int32_t> rows(colour.size());
     std::vector<std::int64_t> fill(pc->h_mc_ptr.begin(), pc->h_mc_ptr.end() - 1);
     for (std::int64_t i = 0; i < static_cast<std::int64_t>(colour.size()); ++i) rows[fill[colour[i]]++] = static_cast<std::int32_t>(i);
-    upload(pc->mc_color, colour);
-    upload(pc->mc_rows, rows);
+    // Colour-ordered copies: colour c's rows (ascending) fill copy rows
+    // h_mc_seg[c] .. h_mc_seg[c] + count - 1, padded to whole slices.  The
+    // forward copy keeps each row's blocks on earlier colours, the backward
+    // copy those on later colours, both in the row's slot order (the FMA
+    // order of the sweeps); the diagonal copy has one block per copy row.
+    pc->h_mc_seg.assign(colours + 1, 0);
+    for (std::int32_t c = 0; c < colours; ++c) {
+        const std::int64_t cnt = pc->h_mc_ptr[c + 1] - pc->h_mc_ptr[c];
+        pc->h_mc_seg[c + 1] = pc->h_mc_seg[c] + (cnt + 63) / 64 * 64;
+    }
+    const std::int64_t padded = pc->h_mc_seg[colours];
+    std::vector<std::int32_t> prow(static_cast<std::size_t>(std::max<std::int64_t>(padded, 1)), -1);
+    for (std::int32_t c = 0; c < colours; ++c)
+        for (std::int64_t t = 0; t < pc->h_mc_ptr[c + 1] - pc->h_mc_ptr[c]; ++t)
+            prow[pc->h_mc_seg[c] + t] = rows[pc->h_mc_ptr[c] + t];
+    upload(pc->mc_prow, prow);
+    std::vector<std::int64_t> dslot(m->owned);
+    OWTK_HIP(hipMemcpy(dslot.data(), pc->tri.dsrc.get(), dslot.size() * sizeof(std::int64_t), hipMemcpyDeviceToHost));
+    const int bb = m->bs * m->bs;
+    const std::size_t sv = pc->storage == OWTK_F64 ? 8 : (pc->storage == OWTK_F32 ? 4 : 2);
+    auto build = [&](McSell& out, int dir) {  // dir -1 forward, +1 backward, 0 diagonal
+        std::vector<std::int32_t> len(static_cast<std::size_t>(std::max<std::int64_t>(padded, 1)), 0);
+        std::vector<std::vector<std::pair<std::int32_t, std::int64_t>>> ent(static_cast<std::size_t>(padded));
+        for (std::int64_t q = 0; q < padded; ++q) {
+            const std::int32_t i = prow[q];
+            if (i < 0) continue;
+            if (dir == 0) {
+                ent[q].push_back({i, dslot[i]});
+            } else {
+                for (std::int32_t e = m->h_offsets[i]; e < m->h_offsets[i + 1]; ++e) {
+                    const std::int32_t j = m->h_columns[e];
+                    if (j >= m->owned || j == i) continue;
+                    if ((dir < 0) == (colour[j] < colour[i])) ent[q].push_back({j, m->s.slot_of_entry[e]});
+                }
+            }
+            len[q] = static_cast<std::int32_t>(ent[q].size());
+        }
+        const std::int64_t slices = padded / 64;
+        std::vector<std::int32_t> sp(static_cast<std::size_t>(slices + 1), 0);
+        for (std::int64_t sl = 0; sl < slices; ++sl) {
+            int wd = 0;
+            for (int l = 0; l < 64; ++l) wd = std::max(wd, len[sl * 64 + l]);
+            sp[sl + 1] = sp[sl] + 64 * wd;
+        }
+        const std::int64_t slots = sp[slices];
+        std::vector<std::int32_t> cols(static_cast<std::size_t>(std::max<std::int64_t>(slots, 1)), 0);
+        std::vector<std::int64_t> map(static_cast<std::size_t>(std::max<std::int64_t>(slots, 1)), -1);
+        for (std::int64_t q = 0; q < padded; ++q)
+            for (std::size_t k = 0; k < ent[q].size(); ++k) {
+                const std::int64_t slot = sp[q / 64] + 64 * static_cast<std::int64_t>(k) + (q % 64);
+                cols[slot] = ent[q][k].first;
+                map[slot] = ent[q][k].second;
+            }
+        if (dir == 0) {  // one block per copy row: slot q
+            map.assign(static_cast<std::size_t>(std::max<std::int64_t>(padded, 1)), -1);
+            for (std::int64_t q = 0; q < padded; ++q)
+                if (prow[q] >= 0) map[q] = dslot[prow[q]];
+            out.slots = padded;
+        } else {
+            upload(out.slice_ptr, sp);
+            upload(out.row_len, len);
+            upload(out.cols, cols);
+            out.slots = slots;
+        }
+        upload(out.map, map);
+        out.vals.allocate(static_cast<std::size_t>(std::max<std::int64_t>(out.slots, 1)) * bb * sv);
+    };
+    build(pc->mc_fwd, -1);
+    build(pc->mc_bwd, 1);
+    build(pc->mc_diag, 0);
     const std::size_t nb = static_cast<std::size_t>(m->owned * m->bs);
     if (pc->storage == OWTK_F64) pc->work64.allocate(std::max<std::size_t>(nb, 1));
     else pc->work32.allocate(std::max<std::size_t>(nb, 1));
 }
 
-template<class V, class X, class I>
-void dense_ssor_apply(owtk_precond pc, const V* vals, const X* in, X* out, const I* inv, X* y, const int* stop)
+// The copies' values from the matrix, in the preconditioner's storage type.
+void dense_ssor_gather(owtk_precond pc)
 {
     owtk_matrix m = pc->m;
     const hipStream_t st = m->ctx->stream;
-    const auto& s = m->s;
+    const int bb = m->bs * m->bs;
+    for (McSell* c : {&pc->mc_fwd, &pc->mc_bwd, &pc->mc_diag}) {
+        if (c->slots == 0) continue;
+        const unsigned int g = grid_for(c->slots * bb);
+        switch (pc->storage) {
+        case OWTK_F64:
+            k_mc_gather<double><<<g, kBlock, 0, st>>>(reinterpret_cast<double*>(c->vals.get()), m->v64.get(),
+                                                      c->map.get(), c->slots, bb);
+            break;
+        case OWTK_F32:
+            k_mc_gather<float><<<g, kBlock, 0, st>>>(reinterpret_cast<float*>(c->vals.get()), m->v32.get(),
+                                                     c->map.get(), c->slots, bb);
+            break;
+        default:
+            k_mc_gather<std::uint16_t><<<g, kBlock, 0, st>>>(reinterpret_cast<std::uint16_t*>(c->vals.get()),
+                                                             m->v16.get(), c->map.get(), c->slots, bb);
+            break;
+        }
+        OWTK_LAUNCH_CHECK();
+    }
+}
+
+template<class V, class X, class I>
+void dense_ssor_apply(owtk_precond pc, const X* in, X* out, const I* inv, X* y, const int* stop)
+{
+    owtk_matrix m = pc->m;
+    const hipStream_t st = m->ctx->stream;
     const double scale = pc->omega * (2.0 - pc->omega);
     const int colours = static_cast<int>(pc->h_mc_ptr.size()) - 1;
+    const V* dvals = reinterpret_cast<const V*>(pc->mc_diag.vals.get());
     auto launch = [&](int c, auto fwd) {
         constexpr bool FWD = decltype(fwd)::value;
-        const std::int64_t first = pc->h_mc_ptr[c], count = pc->h_mc_ptr[c + 1] - first;
+        const std::int64_t count = pc->h_mc_ptr[c + 1] - pc->h_mc_ptr[c];
         if (count == 0) return;
         const unsigned int g = static_cast<unsigned int>((count + kBlock - 1) / kBlock);
-        const std::int32_t* rows = pc->mc_rows.get() + first;
+        const McSell& cp = FWD ? pc->mc_fwd : pc->mc_bwd;
+        const V* vals = reinterpret_cast<const V*>(cp.vals.get());
+        const std::int64_t seg0 = pc->h_mc_seg[c];
+        const std::int32_t* prow = pc->mc_prow.get();
         switch (m->bs) {
-        case 1: k_mc_ssor<1, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
-        case 2: k_mc_ssor<2, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
-        case 3: k_mc_ssor<3, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
-        default: k_mc_ssor<4, V, X, I, FWD><<<g, kBlock, 0, st>>>(rows, count, s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, pc->tri.dsrc.get(), pc->mc_color.get(), c, m->owned, pc->omega, scale, in, y, out, inv, stop); break;
+        case 1: k_mc_ssor<1, V, X, I, FWD><<<g, kBlock, 0, st>>>(prow, seg0, count, cp.slice_ptr.get(), cp.row_len.get(), cp.cols.get(), vals, dvals, pc->omega, scale, in, y, out, inv, stop); break;
+        case 2: k_mc_ssor<2, V, X, I, FWD><<<g, kBlock, 0, st>>>(prow, seg0, count, cp.slice_ptr.get(), cp.row_len.get(), cp.cols.get(), vals, dvals, pc->omega, scale, in, y, out, inv, stop); break;
+        case 3: k_mc_ssor<3, V, X, I, FWD><<<g, kBlock, 0, st>>>(prow, seg0, count, cp.slice_ptr.get(), cp.row_len.get(), cp.cols.get(), vals, dvals, pc->omega, scale, in, y, out, inv, stop); break;
+        default: k_mc_ssor<4, V, X, I, FWD><<<g, kBlock, 0, st>>>(prow, seg0, count, cp.slice_ptr.get(), cp.row_len.get(), cp.cols.get(), vals, dvals, pc->omega, scale, in, y, out, inv, stop); break;
         }
         OWTK_LAUNCH_CHECK();
     };
@@ -1561,6 +1671,7 @@ void ssor_numeric(owtk_precond pc)
         } catch (const std::invalid_argument&) {
             throw std::invalid_argument("SSOR preconditioner has a singular diagonal block");
         }
+        dense_ssor_gather(pc);
         return;
     }
     auto& t = pc->tri;
@@ -1943,7 +2054,7 @@ void precond_apply_f64(owtk_precond pc, const double* in, double* out, const int
         return;
     case OWTK_PC_SSOR:
         if (m->dense()) {
-            dense_ssor_apply<double, double, double>(pc, m->v64.get(), in, out, pc->inv64.get(), pc->work64.get(), stop);
+            dense_ssor_apply<double, double, double>(pc, in, out, pc->inv64.get(), pc->work64.get(), stop);
             return;
         }
         tri_launch<TriMode::ssor, double, double>(pc, pc->tri.fval64.get(), pc->tri.bval64.get(),
@@ -1991,11 +2102,10 @@ void precond_apply_f32(owtk_precond pc, const float* in, float* out, const int* 
         if (m->dense()) {  // dense-block SSOR (ILU is rejected at creation)
             if (pc->storage == OWTK_F32) {
                 if (!m->v32.get()) throw std::invalid_argument("matrix has no FP32 storage");
-                dense_ssor_apply<float, float, float>(pc, m->v32.get(), in, out, pc->inv32.get(), pc->work32.get(), stop);
+                dense_ssor_apply<float, float, float>(pc, in, out, pc->inv32.get(), pc->work32.get(), stop);
             } else {
                 if (!m->v16.get()) throw std::invalid_argument("matrix has no BF16 storage");
-                dense_ssor_apply<std::uint16_t, float, float>(pc, m->v16.get(), in, out, pc->inv32.get(),
-                                                              pc->work32.get(), stop);
+                dense_ssor_apply<std::uint16_t, float, float>(pc, in, out, pc->inv32.get(), pc->work32.get(), stop);
             }
             return;
         }

@@ -98,6 +98,19 @@ struct IluFactor {
 
 } // namespace owtk
 
+namespace owtk {
+// A SELL-64 copy of some of a dense-block matrix's blocks, rows in a
+// different order: slot (s, k, lane) at slice_ptr[s] + 64k + lane, block
+// values at dense_index(slot, j, bb) in the preconditioner's storage type
+// (vals), map = matrix slot of each copy slot (-1 for padding).
+struct McSell {
+    DeviceArray<std::int32_t> slice_ptr, row_len, cols;
+    DeviceArray<std::int64_t> map;
+    DeviceArray<unsigned char> vals;
+    std::int64_t slots = 0;
+};
+} // namespace owtk
+
 struct owtk_precond_s {
     owtk_matrix m = nullptr;
     // the context, kept apart from the matrix: destroy may run after the
@@ -117,9 +130,14 @@ struct owtk_precond_s {
     owtk::TriSchedule tri;
     owtk::IluFactor ilu;
     owtk::DeviceArray<int> flag;   // numeric failure flag (zero pivot / singular)
-    // dense-block SSOR: multicolour order (colour per row, rows grouped by colour)
-    owtk::DeviceArray<std::int32_t> mc_color, mc_rows;
+    // dense-block SSOR: rows of colour c are h_mc_ptr[c] .. h_mc_ptr[c+1]-1
+    // in colour order; colour-ordered SELL-64 copies of the blocks each
+    // sweep reads: colour c's rows occupy copy rows h_mc_seg[c].. (padded to
+    // whole slices), mc_prow maps a copy row to its matrix row (-1 padding)
     std::vector<std::int64_t> h_mc_ptr;
+    std::vector<std::int64_t> h_mc_seg;
+    owtk::DeviceArray<std::int32_t> mc_prow;
+    owtk::McSell mc_fwd, mc_bwd, mc_diag;
     owtk::DeviceArray<double> work64;
     owtk::DeviceArray<float> work32;
     void* learned = nullptr;       // owtk_gnn model for OWTK_PC_LEARNED
