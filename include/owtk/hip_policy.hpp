@@ -21,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <span>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -530,9 +531,73 @@ public:
         check(owtk_arnoldi_snapshot_basis(snap_, column, &p));
         return p;
     }
+    // begin_cycle / record_column / finish_cycle (core.hpp:465-502) for a
+    // snapshot an application assembles itself; `layout` fixes the vector
+    // layout and `basis` holds device vectors.
+    void begin_cycle(const DeviceMatrix& layout, std::size_t maximum_columns)
+    {
+        check(owtk_arnoldi_snapshot_begin_cycle(snap_, layout.get(), maximum_columns));
+    }
+    void record_column(std::size_t column, std::span<const double> values)
+    {
+        check(owtk_arnoldi_snapshot_record_column(snap_, column, values.data(), values.size()));
+    }
+    void finish_cycle(std::span<const DeviceVector<double>> basis, std::size_t columns)
+    {
+        if (basis.size() < columns) throw std::invalid_argument("invalid Arnoldi snapshot basis");
+        std::vector<const double*> p(columns);
+        for (std::size_t c = 0; c < columns; ++c) p[c] = basis[c].data();
+        check(owtk_arnoldi_snapshot_finish_cycle(snap_, p.data(), columns));
+    }
 
 private:
     owtk_arnoldi_snapshot snap_ = nullptr;
+};
+
+// RecycleSpace<double> (recycling.hpp:24-187) kept on the device: up to
+// `capacity` pairs (U_i, C_i = A U_i), C orthonormal, in the vector layout of
+// `layout`.  The device reduces with the serial reduction.
+class RecycleSpace {
+public:
+    RecycleSpace(const DeviceMatrix& layout, std::size_t capacity) : capacity_(capacity)
+    {
+        check(owtk_recycle_space_create(layout.get(), capacity, &space_));
+    }
+    ~RecycleSpace()
+    {
+        if (space_) (void)owtk_recycle_space_destroy(space_);
+    }
+    RecycleSpace(const RecycleSpace&) = delete;
+    RecycleSpace& operator=(const RecycleSpace&) = delete;
+    owtk_recycle_space get() const noexcept { return space_; }
+    std::size_t capacity() const noexcept { return capacity_; }
+    std::size_t size() const
+    {
+        std::int64_t n = 0;
+        check(owtk_recycle_space_size(space_, &n));
+        return static_cast<std::size_t>(n);
+    }
+    bool empty() const { return size() == 0; }
+    void clear() { check(owtk_recycle_space_clear(space_)); }
+    // add_candidate (recycling.hpp:76-135): false when the candidate is
+    // numerically contained in the space.
+    template<class Telemetry = std::nullptr_t>
+    bool add_candidate(DeviceMatrix& op, const DeviceVector<double>& candidate, Telemetry* telemetry = nullptr,
+                       double dependence_tolerance = 128 * std::numeric_limits<double>::epsilon())
+    {
+        owtk_solver_result t{};
+        std::int32_t added = 0;
+        check(owtk_recycle_space_add_candidate(space_, op.get(), candidate.data(), dependence_tolerance, &added, &t));
+        if (telemetry) {
+            telemetry->operator_applications += t.operator_applications;
+            telemetry->global_reductions += t.global_reductions;
+        }
+        return added != 0;
+    }
+
+private:
+    owtk_recycle_space space_ = nullptr;
+    std::size_t capacity_;
 };
 
 namespace detail {
@@ -541,6 +606,15 @@ inline owtk_precond pc_handle(const Preconditioner& p) noexcept { return p.get()
 inline owtk_precond pc_handle(const Preconditioner* p) noexcept { return p ? p->get() : nullptr; }
 inline owtk_precond pc_handle(IdentityPreconditioner) noexcept { return nullptr; }
 inline owtk_precond pc_handle(std::nullptr_t) noexcept { return nullptr; }
+
+template<class Snapshot>
+owtk_arnoldi_snapshot snapshot_handle(Snapshot s)
+{
+    static_assert(std::is_same_v<Snapshot, std::nullptr_t> || std::is_same_v<Snapshot, ArnoldiSnapshot*>,
+                  "the device solvers record into an owtk::hip::ArnoldiSnapshot");
+    if constexpr (std::is_same_v<Snapshot, ArnoldiSnapshot*>) return s ? s->get() : nullptr;
+    else return nullptr;
+}
 
 template<class R>
 constexpr owtk_reduction reduction_of()
@@ -684,15 +758,72 @@ SolverResult<double> fgmres(DeviceMatrix& op, const DeviceVector<double>& rhs, D
                             const Options& options = Options{}, Pc&& preconditioner = Pc{},
                             Reduction = Reduction{}, Workspace = nullptr, Snapshot arnoldi_snapshot = nullptr)
 {
-    static_assert(std::is_same_v<Snapshot, std::nullptr_t> || std::is_same_v<Snapshot, ArnoldiSnapshot*>,
-                  "the device FGMRES records into an owtk::hip::ArnoldiSnapshot");
-    owtk_arnoldi_snapshot snap = nullptr;
-    if constexpr (std::is_same_v<Snapshot, ArnoldiSnapshot*>) snap = arnoldi_snapshot ? arnoldi_snapshot->get() : nullptr;
+    const owtk_arnoldi_snapshot snap = detail::snapshot_handle(arnoldi_snapshot);
     const owtk_precond pc = detail::pc_handle(preconditioner);
     return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
                               [&](const owtk_solver_options& c, owtk_solver_result& r) {
                                   return owtk_fgmres_with_snapshot(op.get(), pc, rhs.data(), solution.data(), &c, snap,
                                                                    &r);
+                              });
+}
+
+// recycled_fgmres(op, b, x, space, options, pc, reduction, update,
+// workspace, arnoldi_snapshot) (recycling.hpp:291-310): FGMRES augmented
+// with the recycle space; with update_recycle_space the converged correction
+// becomes the next candidate.  Serial reduction only.
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t, class Snapshot = std::nullptr_t>
+SolverResult<double> recycled_fgmres(DeviceMatrix& op, const DeviceVector<double>& rhs,
+                                     DeviceVector<double>& solution, RecycleSpace& recycle_space,
+                                     const Options& options = Options{}, Pc&& preconditioner = Pc{},
+                                     Reduction = Reduction{}, bool update_recycle_space = true,
+                                     Workspace = nullptr, Snapshot arnoldi_snapshot = nullptr)
+{
+    const owtk_arnoldi_snapshot snap = detail::snapshot_handle(arnoldi_snapshot);
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_recycled_fgmres_with_snapshot(op.get(), pc, rhs.data(),
+                                                                            solution.data(), recycle_space.get(), &c,
+                                                                            update_recycle_space ? 1 : 0, snap, &r);
+                              });
+}
+
+// extract_harmonic_ritz(op, snapshot, space, reduction, telemetry)
+// (gcrodr.hpp:86-181): replaces the space with the harmonic Ritz vectors of
+// the snapshot's basis, smallest |theta| first; returns the new size.
+template<class Reduction = SerialReduction, class Telemetry = std::nullptr_t>
+std::size_t extract_harmonic_ritz(DeviceMatrix& op, const ArnoldiSnapshot& snapshot, RecycleSpace& recycle_space,
+                                  Reduction = Reduction{}, Telemetry* telemetry = nullptr)
+{
+    static_assert(detail::reduction_of<Reduction>() == OWTK_REDUCTION_SERIAL,
+                  "the device harmonic-Ritz extraction reduces with owtk::hip::SerialReduction");
+    owtk_solver_result t{};
+    std::uint64_t size = 0;
+    check(owtk_extract_harmonic_ritz(op.get(), snapshot.get(), recycle_space.get(), &t, &size));
+    if (telemetry) {
+        telemetry->operator_applications += t.operator_applications;
+        telemetry->global_reductions += t.global_reductions;
+    }
+    return static_cast<std::size_t>(size);
+}
+
+// gcrodr(op, b, x, space, options, pc, reduction, workspace, snapshot)
+// (gcrodr.hpp:185-212): recycled_fgmres without the correction update,
+// recording its last cycle, then extract_harmonic_ritz into the space.
+template<class Options = SolverOptions<double>, class Pc = IdentityPreconditioner, class Reduction = SerialReduction,
+         class Workspace = std::nullptr_t, class Snapshot = std::nullptr_t>
+SolverResult<double> gcrodr(DeviceMatrix& op, const DeviceVector<double>& rhs, DeviceVector<double>& solution,
+                            RecycleSpace& recycle_space, const Options& options = Options{},
+                            Pc&& preconditioner = Pc{}, Reduction = Reduction{}, Workspace = nullptr,
+                            Snapshot supplied_snapshot = nullptr)
+{
+    const owtk_arnoldi_snapshot snap = detail::snapshot_handle(supplied_snapshot);
+    const owtk_precond pc = detail::pc_handle(preconditioner);
+    return detail::run_solver(op, rhs, solution, options, detail::reduction_of<Reduction>(),
+                              [&](const owtk_solver_options& c, owtk_solver_result& r) {
+                                  return owtk_gcrodr(op.get(), pc, rhs.data(), solution.data(), recycle_space.get(),
+                                                     &c, snap, &r);
                               });
 }
 

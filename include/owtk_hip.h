@@ -410,6 +410,48 @@ int owtk_recycle_space_project(owtk_recycle_space space, const double* residual,
 int owtk_recycled_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space space,
                          const owtk_solver_options* options, int32_t update_recycle_space,
                          owtk_solver_result* result);
+/* recycled_fgmres with its trailing ArnoldiSnapshot* argument
+ * (recycling.hpp:300-310): the last FGMRES cycle is recorded into `snapshot`
+ * (may be null) */
+int owtk_recycled_fgmres_with_snapshot(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                       owtk_recycle_space space, const owtk_solver_options* options,
+                                       int32_t update_recycle_space, owtk_arnoldi_snapshot snapshot,
+                                       owtk_solver_result* result);
+
+/* ---------------------------------------------------------------- GCRO-DR -- */
+/* ArnoldiSnapshot::begin_cycle / record_column / finish_cycle
+ * (core.hpp:465-500), for snapshots an application assembles itself:
+ * begin_cycle takes the vector layout of `layout` (owned + ghost of that
+ * matrix); record_column takes column+2 host values; finish_cycle copies the
+ * owned values of `columns` device vectors. */
+int owtk_arnoldi_snapshot_begin_cycle(owtk_arnoldi_snapshot snapshot, owtk_matrix layout,
+                                      uint64_t maximum_columns);
+int owtk_arnoldi_snapshot_record_column(owtk_arnoldi_snapshot snapshot, uint64_t column, const double* values,
+                                        size_t count);
+int owtk_arnoldi_snapshot_finish_cycle(owtk_arnoldi_snapshot snapshot, const double* const* basis,
+                                       uint64_t columns);
+/* The harmonic-Ritz eigenproblem harmonic y = theta metric y (n x n,
+ * column-major; harmonic symmetric positive definite) with the output of
+ * LAPACK dggev as gcrodr.hpp:30-78 calls it: theta_j = (real_j + i imag_j)
+ * / beta_j (beta_j = 0: infinite), complex pairs consecutive with imag > 0
+ * first and their right eigenvector as (real, imaginary) columns of `right`
+ * (n x n).  Host computation: Cholesky of `harmonic`, then Hessenberg QR and
+ * inverse iteration on the equivalent standard problem. */
+int owtk_harmonic_ggev(int n, const double* harmonic, const double* metric, double* real, double* imag,
+                       double* beta, double* right);
+/* extract_harmonic_ritz(op, snapshot, space, reduction, telemetry)
+ * (gcrodr.hpp:86-181): replace `space` with the harmonic Ritz vectors of the
+ * snapshot's basis Z for A = m (smallest |theta| first, a complex pair only
+ * when both directions fit), each added with add_candidate.  *size receives
+ * the new size; telemetry (may be null) is incremented. */
+int owtk_extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot snapshot, owtk_recycle_space space,
+                               owtk_solver_result* telemetry, uint64_t* size);
+/* gcrodr(op, b, x, space, options, pc, reduction, workspace, snapshot)
+ * (gcrodr.hpp:185-212): recycled_fgmres without the correction update,
+ * recording the last cycle, then extract_harmonic_ritz.  `snapshot` may be
+ * null (a temporary one is used). */
+int owtk_gcrodr(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space space,
+                const owtk_solver_options* options, owtk_arnoldi_snapshot snapshot, owtk_solver_result* result);
 
 /* -------------------------------------------------------------- multigrid -- */
 /* AggregationMultigrid (multigrid.hpp:41-421): two-level aggregation solver

@@ -118,6 +118,8 @@ def lib():
         L.oracle_fgmres.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_fgmres_snapshot.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                              C.POINTER(C.c_uint64), P, P, C.POINTER(Result)]
+        L.oracle_recycled_fgmres_snapshot.argtypes = [C.POINTER(Matrix), P, P, P, P, C.POINTER(Options), C.c_int,
+                                                      C.POINTER(C.c_uint64), P, P, C.POINTER(Result)]
         L.oracle_pipelined_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options), C.POINTER(Result)]
         L.oracle_communication_hiding_bicgstab.argtypes = [C.POINTER(Matrix), P, P, P, C.POINTER(Options),
                                                            C.POINTER(Result)]
@@ -408,3 +410,94 @@ def recycled_fgmres(matrix: OMatrix, pc: OPreconditioner | None, b, space: ORecy
     _check(lib().oracle_recycled_fgmres(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), space.h, C.byref(o),
                                         1 if update else 0, C.byref(r)))
     return x, r.as_dict()
+
+
+def extract_harmonic_ritz(matrix: OMatrix, Z, capacity: int, eig=None):
+    """extract_harmonic_ritz (gcrodr.hpp:86-181) over the snapshot basis Z
+    (columns x local): images A Z, G = (AZ)^T AZ, M = (AZ)^T Z, the pencil
+    G y = theta M y by LAPACK ggev (scipy.linalg.eig with two matrices, the
+    routine the reference calls; `eig` may substitute another solver with the
+    same (alpha, beta, vr) output), groups in ascending |theta|, each a real
+    vector or a conjugate pair's (real, imaginary) parts of Z y, committed to
+    a new ORecycleSpace(capacity) only if all its directions pass
+    add_candidate.  Returns the new space, its candidates and the telemetry
+    the reference adds (operator applications and global reductions of the
+    images, the projected matrices and every trial add_candidate)."""
+    import scipy.linalg as sla
+
+    n_owned = matrix.owned * matrix.block
+    Z = np.asarray(Z, np.float64)
+    k = Z.shape[0]
+    space = ORecycleSpace(capacity)
+    tel = {"operator_applications": 0, "global_reductions": 0}
+    if k == 0:
+        return space, [], tel
+    AZ = np.array([matrix.spmv(z)[:n_owned] for z in Z])
+    Zo = Z[:, :n_owned]
+    G = AZ @ AZ.T
+    M = AZ @ Zo.T
+    tel["operator_applications"] += k
+    tel["global_reductions"] += 1
+    if eig is None:
+        w, vr = sla.eig(G, M, homogeneous_eigvals=True)
+        alpha, beta = w[0], w[1]
+    else:
+        alpha, beta, vr = eig(G, M)
+    # gcrodr.hpp:137-155 over dggev's (alphar + i alphai, beta)
+    groups = []
+    j = 0
+    while j < k:
+        width = 2 if alpha[j].imag > 0 else 1
+        bj = abs(beta[j])
+        mag = abs(alpha[j]) / bj if bj != 0 else np.inf
+        if alpha[j].imag >= 0 and j + width <= k and bj != 0 and np.isfinite(mag):
+            groups.append((mag, j, width))
+        j += width
+    groups.sort(key=lambda g: g[0])  # stable
+    accepted = []
+    for _, first, width in groups:
+        if len(accepted) + width > capacity:
+            continue
+        y = vr[:, first]
+        parts = [np.real(y)] if width == 1 else [np.real(y), np.imag(y)]
+        cands = [np.asarray(part @ Z, np.float64) for part in parts]
+        trial = ORecycleSpace(capacity)
+        for c in accepted:
+            trial.add_candidate(matrix, c)
+        ok = True
+        for c in cands:  # every part is tried, as gcrodr.hpp:169-170
+            added, t = trial.add_candidate(matrix, c)
+            ok = added and ok
+            for key in tel:
+                tel[key] += t[key]
+        if ok:
+            accepted.extend(cands)
+    for c in accepted:
+        space.add_candidate(matrix, c)
+    return space, accepted, tel
+
+
+def gcrodr(matrix: OMatrix, pc: OPreconditioner | None, b, space: ORecycleSpace, capacity: int, x0=None, **kw):
+    """gcrodr (gcrodr.hpp:185-212): recycled_fgmres without the update,
+    recording the last cycle, then extract_harmonic_ritz.  Returns x, the
+    result, the new space and the snapshot (H, Z)."""
+    b = np.ascontiguousarray(b, np.float64)
+    n_local = (matrix.owned + matrix.ghost) * matrix.block
+    x = np.zeros(n_local) if x0 is None else np.array(x0, np.float64)
+    o = options(**kw)
+    m = int(o.restart)
+    h = np.zeros((m, m + 1))
+    z = np.zeros((m, n_local))
+    cols = C.c_uint64()
+    r = Result()
+    _check(lib().oracle_recycled_fgmres_snapshot(C.byref(matrix.s), pc.h if pc else None, _p(b), _p(x), space.h,
+                                                 C.byref(o), 0, C.byref(cols), _p(h), _p(z), C.byref(r)))
+    kc = cols.value
+    new_space = space
+    res = r.as_dict()
+    if kc:
+        new_space, _, tel = extract_harmonic_ritz(matrix, z[:kc], capacity)
+        for key in tel:
+            res[key] += tel[key]
+    return x, res, new_space, (h[:kc, :kc + 1].T.copy(), z[:kc].copy())
+

@@ -2071,7 +2071,7 @@ struct RecycledPc : Pc {
 
 // recycled_fgmres (recycling.hpp:291-360)
 void recycled_fgmres(const Matrix& a, Pc& pc, const double* b, double* x, RecycleSpace& space,
-                     const oracle_options& o, bool update, oracle_result& r)
+                     const oracle_options& o, bool update, oracle_result& r, Snapshot* snap = nullptr)
 {
     oracle_result tel;
     std::memset(&tel, 0, sizeof(tel));
@@ -2092,7 +2092,7 @@ void recycled_fgmres(const Matrix& a, Pc& pc, const double* b, double* x, Recycl
     rpc.inner = &pc;
     rpc.space = &space;
     rpc.cached.assign(a.local() * a.bs, 0.0);
-    fgmres(a, rpc, b, x, o, r, rpc.cached.data(), &rpc.ready);
+    fgmres(a, rpc, b, x, o, r, rpc.cached.data(), &rpc.ready, snap);
     r.operator_applications += tel.operator_applications;
     r.global_reductions += tel.global_reductions + rpc.extra_reductions;
     if (update && r.status == converged) {
@@ -2318,6 +2318,29 @@ int oracle_recycle_add(void* h, const oracle_matrix* m, const double* candidate,
     return guard([&] {
         const Matrix a = to_matrix(m);
         *added = static_cast<RecycleSpace*>(h)->add_candidate(a, candidate, telemetry, tol) ? 1 : 0;
+    });
+}
+
+// recycled_fgmres with an ArnoldiSnapshot (recycling.hpp:300-342), outputs as
+// oracle_fgmres_snapshot's
+int oracle_recycled_fgmres_snapshot(const oracle_matrix* m, void* pc_handle, const double* b, double* x, void* space,
+                                    const oracle_options* o, int update, std::uint64_t* columns, double* h, double* z,
+                                    oracle_result* r)
+{
+    return guard([&] {
+        const Matrix a = to_matrix(m);
+        Identity id(a.owned * a.bs);
+        Pc& pc = pc_handle ? *static_cast<OraclePcHandle*>(pc_handle)->pc : static_cast<Pc&>(id);
+        Snapshot snap;
+        recycled_fgmres(a, pc, b, x, *static_cast<RecycleSpace*>(space), *o, update != 0, *r, &snap);
+        *columns = snap.columns;
+        const std::size_t mm = o->restart;
+        const std::size_t L = static_cast<std::size_t>((a.owned + a.ghost) * a.bs);
+        std::fill(h, h + (mm + 1) * mm, 0.0);
+        std::fill(z, z + mm * L, 0.0);
+        if (snap.columns == 0) return;
+        std::copy(snap.H.begin(), snap.H.end(), h);
+        for (std::size_t c = 0; c < snap.columns; ++c) std::copy(snap.Z[c].begin(), snap.Z[c].end(), z + c * L);
     });
 }
 

@@ -14,6 +14,7 @@
 // solve of the refinement driver (inner products and the Hessenberg stay FP64).
 #include "precond.hpp"
 #include "solver_common.hpp"
+#include "host/host.hpp"
 #include "kernels/blas1.hpp"
 #include "kernels/spmv.hpp"
 
@@ -1326,7 +1327,8 @@ __global__ __launch_bounds__(256) void k_rc_diff(const double* __restrict__ x, c
 
 // recycled_fgmres (recycling.hpp:291-360)
 int recycled_fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space sp,
-                        const owtk_solver_options* o, bool update, owtk_solver_result* res)
+                        const owtk_solver_options* o, bool update, owtk_solver_result* res,
+                        owtk_arnoldi_snapshot snap = nullptr)
 {
     const auto t0 = std::chrono::steady_clock::now();
     init_result(res);
@@ -1376,7 +1378,7 @@ int recycled_fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double*
     // RecycledOperator::apply (recycling.hpp:206-214): the cached projected image
     eng.op = [&](const double*, double* out, const int* stop) { copy_stop_f64(st, cached.get(), out, n, stop); };
     Counters cnt;
-    eng.run(b, x, o, res, cnt);
+    eng.run(b, x, o, res, cnt, snap);
     res->operator_applications += tel.operator_applications;
     res->global_reductions += tel.global_reductions + (k > 0 ? res->preconditioner_applications : 0);
     if (update && res->status == status_converged) {
@@ -1388,6 +1390,145 @@ int recycled_fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double*
     ctx->sync();
     if (o->collect_timings)
         res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return OWTK_OK;
+}
+
+namespace {
+
+// A copy of a recycle space (its own device vectors).
+std::unique_ptr<owtk_recycle_space_s> clone_space(const owtk_recycle_space_s& sp)
+{
+    auto out = std::make_unique<owtk_recycle_space_s>();
+    out->ctx = sp.ctx;
+    out->n = sp.n;
+    out->local = sp.local;
+    out->capacity = sp.capacity;
+    out->coef.allocate(sp.capacity + 1);
+    const hipStream_t st = sp.ctx->stream;
+    auto copy = [&](const std::deque<DeviceArray<double>>& from, std::deque<DeviceArray<double>>& to) {
+        for (const auto& v : from) {
+            DeviceArray<double> c(static_cast<std::size_t>(sp.local));
+            OWTK_HIP(hipMemcpyAsync(c.get(), v.get(), static_cast<std::size_t>(sp.local) * sizeof(double),
+                                    hipMemcpyDeviceToDevice, st));
+            to.push_back(std::move(c));
+        }
+    };
+    copy(sp.v.U, out->v.U);
+    copy(sp.v.C, out->v.C);
+    return out;
+}
+
+} // namespace
+
+// extract_harmonic_ritz (gcrodr.hpp:86-181): images A Z, the projected
+// matrices (AZ)^T AZ and (AZ)^T Z in one batch of multi-dots, the
+// eigenproblem on the host, then the candidates Z y in ascending |theta|,
+// each group committed only when all its directions pass add_candidate.
+std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owtk_recycle_space sp,
+                                    owtk_solver_result* tel)
+{
+    owtk_context ctx = m->ctx;
+    const hipStream_t st = ctx->stream;
+    const int n = static_cast<int>(sn->columns);
+    if (sp->n != m->owned * m->bs || sp->local != m->local() * m->bs)
+        throw std::invalid_argument("recycle space vector layout mismatch");
+    if (n == 0) {
+        ctx->sync();
+        sp->v = RecycleVectors{};
+        return 0;
+    }
+    if (sn->ld != sp->local || sn->owned != sp->n) throw std::invalid_argument("snapshot vector layout mismatch");
+    const std::int64_t local = sp->local, owned = sp->n;
+    std::vector<const double*> Z(n), AZ(n);
+    for (int c = 0; c < n; ++c) Z[c] = sn->basis.get() + static_cast<std::int64_t>(c) * local;
+    DeviceArray<double> images(static_cast<std::size_t>(n) * local);
+    kern::fill_zero_bytes(st, images.get(), images.bytes());
+    for (int c = 0; c < n; ++c) {
+        AZ[c] = images.get() + static_cast<std::int64_t>(c) * local;
+        dev::EpiStore<double> e;
+        e.y = const_cast<double*>(AZ[c]);
+        spmv_f64(m, Z[c], e);
+    }
+    count(tel, static_cast<std::uint64_t>(n), 1);
+    // column c: <AZ_r, AZ_c> and <AZ_r, Z_c> for every r (n + 1 values each)
+    const std::size_t per = static_cast<std::size_t>(n) + 1;
+    DeviceArray<double> dots(2 * static_cast<std::size_t>(n) * per);
+    const auto scratch = ctx->scratch();
+    for (int c = 0; c < n; ++c) {
+        kern::multidot_to(st, scratch, owned, n, AZ.data(), AZ[c], dots.get() + static_cast<std::size_t>(c) * per);
+        kern::multidot_to(st, scratch, owned, n, AZ.data(), Z[c],
+                          dots.get() + static_cast<std::size_t>(n + c) * per);
+    }
+    std::vector<double> h_dots(dots.count);
+    OWTK_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), dots.bytes(), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    std::vector<double> harmonic(static_cast<std::size_t>(n) * n), metric(harmonic.size());
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r) {
+            harmonic[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(c) * per + r];
+            metric[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(n + c) * per + r];
+        }
+    for (double v : harmonic)
+        if (!std::isfinite(v)) throw std::runtime_error("nonfinite harmonic-Ritz projected matrix");
+    for (double v : metric)
+        if (!std::isfinite(v)) throw std::runtime_error("nonfinite harmonic-Ritz projected matrix");
+    std::vector<double> re(n), im(n), be(n), right(static_cast<std::size_t>(n) * n);
+    host::harmonic_ggev(n, harmonic.data(), metric.data(), re.data(), im.data(), be.data(), right.data());
+    struct Group {
+        double magnitude;
+        int first, width;
+    };
+    std::vector<Group> groups;
+    for (int e = 0; e < n;) {
+        const int width = im[e] > 0.0 ? 2 : 1;
+        const double magnitude = std::hypot(re[e], im[e]) / std::abs(be[e]);
+        if (im[e] >= 0.0 && e + width <= n && be[e] != 0.0 && std::isfinite(magnitude))
+            groups.push_back({magnitude, e, width});
+        e += width;
+    }
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const Group& a, const Group& b) { return a.magnitude < b.magnitude; });
+    auto replacement = std::make_unique<owtk_recycle_space_s>();
+    replacement->ctx = ctx;
+    replacement->n = owned;
+    replacement->local = local;
+    replacement->capacity = sp->capacity;
+    replacement->coef.allocate(sp->capacity + 1);
+    kern::BasisPointers zp{};
+    for (int c = 0; c < n; ++c) zp.v[c] = Z[c];
+    DeviceArray<double> coef(static_cast<std::size_t>(n)), cand(static_cast<std::size_t>(owned));
+    const unsigned int g = dev::stream_grid(owned, 4);
+    for (const Group& grp : groups) {
+        if (replacement->v.C.size() + static_cast<std::size_t>(grp.width) > replacement->capacity) continue;
+        auto trial = clone_space(*replacement);
+        bool accepted = true;
+        for (int part = 0; part < grp.width; ++part) {
+            // candidate = sum_b y_b Z_b, added in basis order from zero
+            OWTK_HIP(hipMemcpyAsync(coef.get(), right.data() + static_cast<std::size_t>(grp.first + part) * n,
+                                    static_cast<std::size_t>(n) * sizeof(double), hipMemcpyHostToDevice, st));
+            kern::fill_zero_bytes(st, cand.get(), cand.bytes());
+            k_rc_project<<<g, kBlock, 0, st>>>(cand.get(), nullptr, zp, kern::BasisPointers{}, coef.get(), n, owned,
+                                               1.0, nullptr);
+            OWTK_LAUNCH_CHECK();
+            accepted = rc_add(trial.get(), m, cand.get(), 128 * std::numeric_limits<double>::epsilon(), tel) && accepted;
+        }
+        ctx->sync();  // the candidate and coefficients are reused by the next group
+        if (accepted) replacement = std::move(trial);
+    }
+    ctx->sync();
+    sp->v = std::move(replacement->v);
+    return static_cast<std::uint64_t>(sp->v.C.size());
+}
+
+// gcrodr (gcrodr.hpp:185-212)
+int gcrodr_run(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space sp,
+               const owtk_solver_options* o, owtk_arnoldi_snapshot snap, owtk_solver_result* res)
+{
+    owtk_arnoldi_snapshot_s local_snap;
+    local_snap.ctx = m->ctx;
+    owtk_arnoldi_snapshot s = snap ? snap : &local_snap;
+    recycled_fgmres_run(m, pc, b, x, sp, o, false, res, s);
+    if (s->columns != 0) extract_harmonic_ritz(m, s, sp, res);
     return OWTK_OK;
 }
 
@@ -1496,6 +1637,79 @@ int owtk_arnoldi_snapshot_h(owtk_arnoldi_snapshot snapshot, uint64_t row, uint64
         if (column >= cols || row > cols) throw std::out_of_range("Arnoldi snapshot entry");
         // rows below the recorded subdiagonal are the zeros begin_cycle wrote
         *value = row > column + 1 ? 0.0 : snapshot->host_h[column * owtk_arnoldi_snapshot_s::kStride + row];
+    });
+}
+
+int owtk_arnoldi_snapshot_begin_cycle(owtk_arnoldi_snapshot snapshot, owtk_matrix layout, uint64_t maximum_columns)
+{
+    return guarded([&] {
+        if (!snapshot || !layout) throw std::invalid_argument("null argument");
+        if (snapshot->ctx != layout->ctx) throw std::invalid_argument("snapshot belongs to another context");
+        if (maximum_columns > static_cast<uint64_t>(owtk_arnoldi_snapshot_s::kStride - 1))
+            throw unsupported_error("Arnoldi snapshots hold at most 60 columns on the device path");
+        layout->ctx->activate();
+        owtk::snapshot_begin(snapshot, static_cast<int>(maximum_columns), layout->local() * layout->bs,
+                             layout->owned * layout->bs, layout->ctx->stream);
+        snapshot->host_h.assign(static_cast<std::size_t>(owtk_arnoldi_snapshot_s::kStride)
+                                    * (owtk_arnoldi_snapshot_s::kStride - 1),
+                                0.0);
+        layout->ctx->sync();
+    });
+}
+
+int owtk_arnoldi_snapshot_record_column(owtk_arnoldi_snapshot snapshot, uint64_t column, const double* values,
+                                        size_t count)
+{
+    return guarded([&] {
+        if (!snapshot || (count && !values)) throw std::invalid_argument("null argument");
+        if (column >= static_cast<uint64_t>(snapshot->maximum_columns) || count != column + 2)
+            throw std::invalid_argument("invalid Arnoldi snapshot column");
+        for (std::size_t row = 0; row < count; ++row)
+            snapshot->host_h[column * owtk_arnoldi_snapshot_s::kStride + row] = values[row];
+    });
+}
+
+int owtk_arnoldi_snapshot_finish_cycle(owtk_arnoldi_snapshot snapshot, const double* const* basis, uint64_t columns)
+{
+    return guarded([&] {
+        if (!snapshot || (columns && !basis)) throw std::invalid_argument("null argument");
+        if (columns > static_cast<uint64_t>(snapshot->maximum_columns))
+            throw std::invalid_argument("invalid Arnoldi snapshot basis");
+        snapshot->ctx->activate();
+        const hipStream_t st = snapshot->ctx->stream;
+        for (uint64_t c = 0; c < columns; ++c) {
+            if (!basis[c]) throw std::invalid_argument("invalid Arnoldi snapshot basis");
+            OWTK_HIP(hipMemcpyAsync(snapshot->basis.get() + static_cast<std::int64_t>(c) * snapshot->ld, basis[c],
+                                    static_cast<std::size_t>(snapshot->owned) * sizeof(double),
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        snapshot->ctx->sync();
+        snapshot->columns = static_cast<std::int64_t>(columns);
+    });
+}
+
+int owtk_extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot snapshot, owtk_recycle_space space,
+                               owtk_solver_result* telemetry, uint64_t* size)
+{
+    return guarded([&] {
+        if (!m || !snapshot || !space) throw std::invalid_argument("null argument");
+        if (snapshot->ctx != m->ctx || space->ctx != m->ctx) throw std::invalid_argument("objects of another context");
+        m->ctx->activate();
+        const uint64_t k = owtk::extract_harmonic_ritz(m, snapshot, space, telemetry);
+        if (size) *size = k;
+    });
+}
+
+int owtk_gcrodr(owtk_matrix m, owtk_precond pc, const double* b, double* x, owtk_recycle_space space,
+                const owtk_solver_options* options, owtk_arnoldi_snapshot snapshot, owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !space || !options || !result) throw std::invalid_argument("null argument");
+        if (space->ctx != m->ctx || (snapshot && snapshot->ctx != m->ctx))
+            throw std::invalid_argument("objects of another context");
+        require_plain_options(options, "gcrodr");
+        m->ctx->activate();
+        owtk::gcrodr_run(m, pc, b, x, space, options, snapshot, result);
     });
 }
 
@@ -1635,6 +1849,21 @@ int owtk_recycled_fgmres(owtk_matrix m, owtk_precond pc, const double* b, double
         require_plain_options(options, "recycled_fgmres");
         m->ctx->activate();
         owtk::recycled_fgmres_run(m, pc, b, x, space, options, update_recycle_space != 0, result);
+    });
+}
+
+int owtk_recycled_fgmres_with_snapshot(owtk_matrix m, owtk_precond pc, const double* b, double* x,
+                                       owtk_recycle_space space, const owtk_solver_options* options,
+                                       int32_t update_recycle_space, owtk_arnoldi_snapshot snapshot,
+                                       owtk_solver_result* result)
+{
+    return guarded([&] {
+        if (!m || !b || !x || !space || !options || !result) throw std::invalid_argument("null argument");
+        if (space->ctx != m->ctx || (snapshot && snapshot->ctx != m->ctx))
+            throw std::invalid_argument("objects of another context");
+        require_plain_options(options, "recycled_fgmres");
+        m->ctx->activate();
+        owtk::recycled_fgmres_run(m, pc, b, x, space, options, update_recycle_space != 0, result, snapshot);
     });
 }
 

@@ -79,4 +79,10 @@ struct LevelSchedule {
     std::vector<std::int32_t> rows;
 };
 
+// GCRO-DR's harmonic-Ritz eigenproblem harmonic y = theta metric y
+// (harmonic SPD, n x n column-major) in LAPACK dggev's output convention
+// (eig.cpp).
+void harmonic_ggev(int n, const double* harmonic, const double* metric, double* real, double* imag, double* beta,
+                   double* right);
+
 } // namespace owtk::host

@@ -22,7 +22,7 @@ from ._lib import (BF16, F32, F64, LAW_EDGE_CONTRAST, LAW_LOGNORMAL_FIELD, LAW_U
 __all__ = [
     "Context", "DeviceBuffer", "Matrix", "Preconditioner", "SolverOptions", "bicgstab", "pipelined_bicgstab",
     "communication_hiding_bicgstab", "idrs", "Multigrid", "RecycleSpace", "recycled_fgmres", "fgmres",
-    "ArnoldiSnapshot", "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
+    "ArnoldiSnapshot", "gcrodr", "extract_harmonic_ritz", "harmonic_ggev", "fgmres_refined", "SplitSystem", "generate_split", "generate_bsr", "split_to_csr", "read_state", "device_count",
     "F64", "F32", "BF16", "OwtkError", "ZeroPivotError", "InvalidArgumentError", "NoDeviceError",
     "UnsupportedError", "lib", "STATUS", "REASON",
 ]
@@ -309,13 +309,55 @@ class RecycleSpace:
             pass
 
 
-def recycled_fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
-                    space: RecycleSpace, options=None, update: bool = True) -> dict:
-    """recycled_fgmres (recycling.hpp:291-360) on the device."""
+def harmonic_ggev(harmonic, metric):
+    """harmonic y = theta metric y (harmonic SPD) in LAPACK dggev's output
+    convention: (real, imag, beta, right) with theta = (real + i imag) / beta
+    and conjugate pairs as consecutive (real, imaginary) columns of right."""
+    g = np.asfortranarray(harmonic, np.float64)
+    mm = np.asfortranarray(metric, np.float64)
+    n = g.shape[0]
+    re, im, be = np.zeros(n), np.zeros(n), np.zeros(n)
+    right = np.zeros((n, n), order="F")
+    dp = lambda a: a.ctypes.data_as(_lib._DP)
+    check(lib().owtk_harmonic_ggev(n, dp(g), dp(mm), dp(re), dp(im), dp(be), dp(right)))
+    return re, im, be, right
+
+
+def extract_harmonic_ritz(matrix: Matrix, snapshot: "ArnoldiSnapshot", space: "RecycleSpace") -> tuple:
+    """extract_harmonic_ritz (gcrodr.hpp:86-181); returns (size, telemetry)."""
+    tel = _lib.SolverResult()
+    k = C.c_uint64()
+    check(lib().owtk_extract_harmonic_ritz(matrix.handle, snapshot.handle, space.handle, C.byref(tel), C.byref(k)))
+    return k.value, tel.as_dict()
+
+
+def gcrodr(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer, space: "RecycleSpace",
+           options=None, snapshot: "ArnoldiSnapshot | None" = None) -> dict:
+    """gcrodr (gcrodr.hpp:185-212) on the device."""
     opts = options if options is not None else SolverOptions()
     res = _lib.SolverResult()
-    check(lib().owtk_recycled_fgmres(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
-                                     space.handle, C.byref(opts), 1 if update else 0, C.byref(res)))
+    if snapshot is not None:
+        snapshot.local = matrix.local_size
+    check(lib().owtk_gcrodr(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr, space.handle,
+                            C.byref(opts), snapshot.handle if snapshot is not None else None, C.byref(res)))
+    d = res.as_dict()
+    d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
+    d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
+    return d
+
+
+def recycled_fgmres(matrix: Matrix, pc: Preconditioner | None, b: DeviceBuffer, x: DeviceBuffer,
+                    space: RecycleSpace, options=None, update: bool = True,
+                    snapshot: "ArnoldiSnapshot | None" = None) -> dict:
+    """recycled_fgmres (recycling.hpp:291-360) on the device; `snapshot`
+    receives the last FGMRES cycle."""
+    opts = options if options is not None else SolverOptions()
+    res = _lib.SolverResult()
+    if snapshot is not None:
+        snapshot.local = matrix.local_size
+    check(lib().owtk_recycled_fgmres_with_snapshot(matrix.handle, pc.handle if pc is not None else None, b.ptr, x.ptr,
+                                                   space.handle, C.byref(opts), 1 if update else 0,
+                                                   snapshot.handle if snapshot is not None else None, C.byref(res)))
     d = res.as_dict()
     d["status_name"] = STATUS[d["status"]] if 0 <= d["status"] < len(STATUS) else str(d["status"])
     d["reason_name"] = REASON[d["breakdown_reason"]] if 0 <= d["breakdown_reason"] < len(REASON) else "?"
@@ -720,6 +762,21 @@ class ArnoldiSnapshot:
 
     def clear(self) -> None:
         check(lib().owtk_arnoldi_snapshot_clear(self.handle))
+
+    def begin_cycle(self, layout: "Matrix", maximum_columns: int) -> None:
+        """ArnoldiSnapshot::begin_cycle (core.hpp:465-480) for the vector
+        layout of `layout`."""
+        self.local = layout.local_size
+        check(lib().owtk_arnoldi_snapshot_begin_cycle(self.handle, layout.handle, maximum_columns))
+
+    def record_column(self, column: int, values) -> None:
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib().owtk_arnoldi_snapshot_record_column(self.handle, column, v.ctypes.data_as(_lib._DP), v.size))
+
+    def finish_cycle(self, basis: list["DeviceBuffer"], columns: int | None = None) -> None:
+        k = len(basis) if columns is None else columns
+        arr = (C.c_void_p * max(k, 1))(*[b.ptr for b in basis[:k]])
+        check(lib().owtk_arnoldi_snapshot_finish_cycle(self.handle, arr, k))
 
     def close(self) -> None:
         if self.handle:

@@ -199,6 +199,12 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_arnoldi_snapshot_columns": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "owtk_arnoldi_snapshot_h": (C.c_int, [_P, C.c_uint64, C.c_uint64, _DP]),
     "owtk_arnoldi_snapshot_basis": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "owtk_arnoldi_snapshot_begin_cycle": (C.c_int, [_P, _P, C.c_uint64]),
+    "owtk_arnoldi_snapshot_record_column": (C.c_int, [_P, C.c_uint64, _DP, C.c_size_t]),
+    "owtk_arnoldi_snapshot_finish_cycle": (C.c_int, [_P, C.POINTER(_P), C.c_uint64]),
+    "owtk_harmonic_ggev": (C.c_int, [C.c_int, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "owtk_extract_harmonic_ritz": (C.c_int, [_P, _P, _P, C.POINTER(SolverResult), C.POINTER(C.c_uint64)]),
+    "owtk_gcrodr": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(SolverOptions), _P, C.POINTER(SolverResult)]),
     "owtk_refinement_options_default": (None, [C.POINTER(RefinementOptions)]),
     "owtk_fgmres_refined": (C.c_int, [_P, _P, _P, _P, C.POINTER(RefinementOptions), C.POINTER(SolverResult)]),
     "owtk_fgmres_refined_fallback": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(RefinementOptions),
@@ -231,6 +237,8 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_recycle_space_project": (C.c_int, [_P, _P, _P, C.POINTER(SolverResult)]),
     "owtk_recycled_fgmres": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(SolverOptions), C.c_int32,
                                        C.POINTER(SolverResult)]),
+    "owtk_recycled_fgmres_with_snapshot": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(SolverOptions), C.c_int32, _P,
+                                                     C.POINTER(SolverResult)]),
     "owtk_multigrid_options_default": (None, [C.POINTER(MultigridOptions)]),
     "owtk_multigrid_create": (C.c_int, [_P, C.POINTER(MultigridOptions), C.POINTER(_P)]),
     "owtk_multigrid_destroy": (C.c_int, [_P]),

@@ -9,7 +9,8 @@
 //     FGMRES + block-Jacobi ILU(0), FP64-refined FGMRES with FP32 storage,
 //     BiCGSTAB with the compensated reduction, an application convergence
 //     test over a host BlockVector, pipelined BiCGSTAB + SSOR, and the
-//     host-vector route;
+//     host-vector route, GCRO-DR with its recycle space and Arnoldi
+//     snapshot (gcrodr.hpp:86-212; recycling.hpp:291-310);
 //   each checked by its true residual recomputed on the host.
 #include <owtk/hip_policy.hpp>
 
@@ -337,6 +338,77 @@ int main()
             fgmres(dm, db, dx, at0, owtk::hip::IdentityPreconditioner{}, owtk::hip::SerialReduction{}, nullptr, &snap);
         expect(r8.converged_by_application && r8.iterations == 0 && snap.columns() == 0,
                "initial application stop clears a stale snapshot");
+    }
+
+    // GCRO-DR (gcrodr.hpp:185-212) called unqualified with the reference's
+    // trailing arguments, as test_core.cpp:611-634: two harmonic Ritz
+    // vectors kept with a recorded snapshot, then a related system converges
+    // keeping two
+    {
+        owtk::hip::Preconditioner jacobi(dm, owtk::hip::Preconditioner::params(OWTK_PC_JACOBI));
+        owt::krylov::SolverOptions<double> gopts;
+        gopts.relative_tolerance = 1e-10;
+        gopts.restart = 20;
+        owtk::hip::RecycleSpace space(dm, 2);
+        owtk::hip::ArnoldiSnapshot snap(ctx);
+        dx.upload(std::vector<double>(n, 0.0).data(), n);
+        owt::krylov::SolverResult<double> g1 =
+            gcrodr(dm, db, dx, space, gopts, jacobi, owtk::hip::SerialReduction{}, nullptr, &snap);
+        dx.download(hx.data(), n);
+        const double relg1 = true_rel(s, b, hx);
+        std::vector<double> b2(b);
+        for (std::size_t i = 0; i < n; i += 97) b2[i] += 0.25;
+        owtk::hip::DeviceVector<double> db2(ctx, n);
+        db2.upload(b2.data(), n);
+        dx.upload(std::vector<double>(n, 0.0).data(), n);
+        owt::krylov::SolverResult<double> g2 = gcrodr(dm, db2, dx, space, gopts, jacobi);
+        dx.download(hx.data(), n);
+        const double relg2 = true_rel(s, b2, hx);
+        std::printf("gcrodr + Jacobi: first %zu iterations (true rel %.3e, snapshot columns %zu), related %zu "
+                    "iterations (true rel %.3e), space %zu\n",
+                    g1.iterations, relg1, snap.columns(), g2.iterations, relg2, space.size());
+        expect(g1.converged() && relg1 <= 1e-10 && space.size() == 2 && snap.columns() > 0,
+               "GCRO-DR keeps two harmonic Ritz vectors");
+        expect(g2.converged() && relg2 <= 1e-10 && space.size() == 2, "GCRO-DR on a related system");
+
+        // recycled_fgmres recording its last cycle, then the extraction by
+        // hand: the same space as gcrodr's replacement
+        owtk::hip::RecycleSpace manual(dm, 2);
+        owtk::hip::ArnoldiSnapshot rec(ctx);
+        dx.upload(std::vector<double>(n, 0.0).data(), n);
+        owt::krylov::SolverResult<double> rr = recycled_fgmres(dm, db, dx, manual, gopts, jacobi,
+                                                               owtk::hip::SerialReduction{}, false, nullptr, &rec);
+        const std::size_t kept = extract_harmonic_ritz(dm, rec, manual, owtk::hip::SerialReduction{}, &rr);
+        expect(rr.converged() && rec.columns() == snap.columns() && kept == 2 && rr.iterations == g1.iterations
+                   && rr.operator_applications == g1.operator_applications
+                   && rr.global_reductions == g1.global_reductions,
+               "recycled_fgmres + extract_harmonic_ritz is gcrodr");
+
+        // an application-assembled snapshot (core.hpp:465-502): Z = e_0, the
+        // diagonal-dominant chain gives one real harmonic value
+        owtk::hip::ArnoldiSnapshot one(ctx);
+        one.begin_cycle(dm, 1);
+        const double col0[2] = {4.0, 1.0};
+        one.record_column(0, col0);
+        std::vector<owtk::hip::DeviceVector<double>> basis;
+        basis.emplace_back(ctx, n);
+        std::vector<double> e0(n, 0.0);
+        e0[0] = 1.0;
+        basis[0].upload(e0.data(), n);
+        one.finish_cycle(basis, 1);
+        owtk::hip::RecycleSpace single(dm, 1);
+        owt::krylov::SolverResult<double> tel;
+        const std::size_t got = extract_harmonic_ritz(dm, one, single, owtk::hip::SerialReduction{}, &tel);
+        expect(got == 1 && one.columns() == 1 && one.h(1, 0) == 1.0 && tel.operator_applications == 2
+                   && tel.global_reductions == 3,
+               "extraction from an assembled snapshot");
+        bool refused_record = false;
+        try {
+            one.record_column(0, std::span<const double>(col0, 1));
+        } catch (const std::invalid_argument&) {
+            refused_record = true;
+        }
+        expect(refused_record, "record_column takes column + 2 values");
     }
 
     // the production selection: split SSOR + pipelined BiCGSTAB
