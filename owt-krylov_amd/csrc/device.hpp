@@ -43,12 +43,31 @@ __host__ __device__ inline std::uint16_t float_to_bf16(float f)
     return static_cast<std::uint16_t>((v.u + rounding) >> 16);
 }
 
-// Dense B x B blocks in the SELL-64 layout: element j = r*B + c of slot s at
-// (s & ~63) * B*B + 64 j + (s & 63), so the j-th elements of a wave's 64
-// slots are one contiguous run (one coalesced load per element).
+// Dense B x B blocks in the SELL-64 layout, elements in lane-interleaved
+// pairs: elements 2g and 2g+1 (j = r*B + c) of the 64 slots of a wave sit in
+// one contiguous run of 128 values, each lane's pair adjacent, so one load
+// of a wave reads both elements of a pair (16 bytes a lane in FP64).  With
+// B*B odd the last element is a run of 64 single values.  Element j of slot
+// s is at (s & ~63) * B*B + dense_offset(s & 63, j, B*B).
+__host__ __device__ constexpr std::int64_t dense_offset(std::int64_t lane, int j, int bb)
+{
+    return j < (bb & ~1) ? 128 * (j >> 1) + 2 * lane + (j & 1) : 64 * j + lane;
+}
 __host__ __device__ constexpr std::int64_t dense_index(std::int64_t slot, int j, int bb)
 {
-    return (slot & ~std::int64_t(63)) * bb + 64 * j + (slot & 63);
+    return (slot & ~std::int64_t(63)) * bb + dense_offset(slot & 63, j, bb);
+}
+// The inverse within one wave's run of 64 * bb values: offset o holds
+// element j of lane `lane`.
+__host__ __device__ constexpr void dense_element(std::int64_t o, int bb, int& lane, int& j)
+{
+    if (o < 64 * (bb & ~1)) {
+        lane = static_cast<int>((o & 127) >> 1);
+        j = static_cast<int>(2 * (o >> 7) + (o & 1));
+    } else {
+        lane = static_cast<int>(o & 63);
+        j = bb - 1;
+    }
 }
 
 template<class V> __device__ inline float load_as_float(const V* p, std::int64_t i);

@@ -163,8 +163,8 @@ namespace owtk {
 // the slice; slot (s, k, lane) lives at slice_ptr[s] + 64k + lane, so the k-th
 // entry of 64 consecutive rows is one contiguous 64-wide load.  Component-
 // diagonal values sit at [slot & ~63] * B + 64c + (slot & 63) (per entry k and
-// component c a contiguous 64-lane run); dense blocks the same way, element
-// j = r*B + c of slot s at (s & ~63) * B*B + 64 j + (s & 63) (dense_index).
+// component c a contiguous 64-lane run); dense blocks likewise, with elements
+// 2g and 2g+1 of a slot adjacent in a 128-value run per pair (dense_index).
 // Row entries keep the CSR order, so the
 // per-row accumulation order is the reference's.
 //

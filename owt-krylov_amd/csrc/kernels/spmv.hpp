@@ -391,21 +391,38 @@ __global__ __launch_bounds__(256) void spmv_sell_bsr(const std::int32_t* __restr
         // U slots per pass, all loads issued before the arithmetic; a slot
         // past the slice's width is clamped to its last run (in bounds) and
         // a slot past the row's length is dropped by a select, so each
-        // block row still adds its blocks in slot order
+        // block row still adds its blocks in slot order.  The next pass's
+        // columns are loaded one pass ahead, so a pass's x gathers issue with
+        // its value loads (one memory round trip per pass, not two).
+        std::int32_t coln[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) coln[u] = cols[base + 64 * (u < width ? u : width - 1) + lane];
         for (int k0 = 0; k0 < len; k0 += U) {
             std::int32_t colk[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int kk = k0 + u < width ? k0 + u : width - 1;
-                colk[u] = cols[base + 64 * kk + lane];
+                colk[u] = coln[u];
+                const int kn = k0 + U + u < width ? k0 + U + u : width - 1;
+                coln[u] = cols[base + 64 * kn + lane];
             }
+            // keeps the prefetch in this pass: left to itself the compiler
+            // sinks it to its use in the next pass, after this pass's wait
+            __asm__ volatile("" ::: "memory");
             A v[U][BS * BS], xc[U][BS];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int kk = k0 + u < width ? k0 + u : width - 1;
-                const V* blk = vals + (base + 64 * kk) * (BS * BS) + lane;  // element j at blk[64 j]
+                const V* blk = vals + (base + 64 * kk) * (BS * BS);  // element j at blk[dense_offset(lane, j)]
+                // FP64 values stream past the caches (4x4 blocks of a 1M-vertex
+                // grid are 1 GB), so the gathered x stays resident: 191 -> 174
+                // us there.  BF16 values measured the same either way.
 #pragma unroll
-                for (int j = 0; j < BS * BS; ++j) v[u][j] = static_cast<A>(load_value(blk, 64 * j));
+                for (int j = 0; j < BS * BS; ++j) {
+                    if constexpr (sizeof(V) == 8)
+                        v[u][j] = static_cast<A>(load_value_nt(blk, dense_offset(lane, j, BS * BS)));
+                    else
+                        v[u][j] = static_cast<A>(load_value(blk, dense_offset(lane, j, BS * BS)));
+                }
 #pragma unroll
                 for (int c = 0; c < BS; ++c) xc[u][c] = static_cast<A>(x[static_cast<std::int64_t>(colk[u]) * BS + c]);
             }
@@ -485,9 +502,10 @@ void spmv_dispatch(owtk_matrix m, const V* vals, const V* diag, const X* x, E e)
         const unsigned int grid = static_cast<unsigned int>(blocks);
         launched = grid;
         if (m->dense()) {
-            // slots per pass: 2 for FP32/BF16 values (c2 BF16: 54.6 -> 50.7 us),
-            // 1 for FP64, whose 16 values per block already keep enough loads
-            // in flight (2 and 4 measured slower)
+            // slots per pass: 2 for FP32/BF16 values (4x4 BF16 blocks of a
+            // 1M-vertex grid: 54.6 -> 50.7 us), 1 for FP64, whose 16 values
+            // per block already keep enough loads in flight (2 and 4 measured
+            // slower)
             constexpr int kBsrU = sizeof(V) == 8 ? 1 : 2;
             switch (m->bs) {
             case 2: spmv_sell_bsr<2, V, X, E, kBsrU><<<grid, kBlock, 0, st>>>(s.slice_ptr.get(), s.row_len.get(), s.columns.get(), vals, x, rows, e, scratch); break;

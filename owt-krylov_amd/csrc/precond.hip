@@ -260,8 +260,9 @@ __global__ __launch_bounds__(256) void k_mc_gather(V* __restrict__ dst, const V*
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += step) {
         const std::int64_t slice = t / (64 * bb);
-        const int j = static_cast<int>((t / 64) % bb);
-        const std::int64_t sl = slice * 64 + (t & 63);
+        int lane = 0, j = 0;
+        dev::dense_element(t - slice * 64 * bb, bb, lane, j);
+        const std::int64_t sl = slice * 64 + lane;
         const std::int64_t from = map[sl];
         dst[t] = from >= 0 ? src[dev::dense_index(from, j, bb)] : V(0);
     }
@@ -900,8 +901,9 @@ struct EllArgs {
     int fk, bk;                   // slots per row of each sweep (both the kernel's E)
 };
 
-// Two waves per SIMD at least (at most 256 VGPRs), so all of a c1 apply's
-// 1,536 one-wave workgroups are resident at once.
+// Two waves per SIMD at least (at most 256 VGPRs), so all 1,536 one-wave
+// workgroups of a 2M-vertex, 3-component apply in 4096-vertex blocks are
+// resident at once.
 template<TriMode MODE, class X, class V, int E, bool GLOBAL_Y>
 __global__ __launch_bounds__(64, 2) void k_tri_ell(TriArgs a, EllArgs l, const V* __restrict__ fval,
                                                 const V* __restrict__ bval, const X* __restrict__ dval, const X* in,
