@@ -3,7 +3,7 @@ block-3 grid, learned-preconditioner + mixed-precision FGMRES vs plain FP64
 BiCGSTAB + block-Jacobi ILU(0), plus the split SpMV's HBM bandwidth.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32|bf16]
-    python bench.py --kernel spmv|copy|learned --steps K   (kernel loops for rocprofv3 counters)
+    python bench.py --kernel spmv|copy|learned|bsr|learned-update --steps K   (kernel loops for rocprofv3)
     python bench.py --workload batched                   (512 uneven systems, batched BiCGSTAB)
     python bench.py --workload bsr                       (dense 4x4 BSR, BF16 FGMRES(40)-IR + SSOR)
 
@@ -305,12 +305,37 @@ def bsr_kernel_loop(steps):
                           "spmv_bytes_f64": info.spmv_bytes_f64, "spmv_bytes_bf16": info.spmv_bytes_bf16}))
 
 
+def learned_update_loop(steps, pc_storage="bf16"):
+    """The learned preconditioner's numeric update on the headline system
+    (network inference, coefficient and column scaling), `steps` times after
+    two untimed ones; prints the mean host-timed update."""
+    with owtk.Context(0) as ctx:
+        s, m = build_system(ctx)
+        st = {"f32": owtk.F32, "bf16": owtk.BF16}[pc_storage]
+        m.add_storage(st)
+        model = owtk.GnnModel.load(ctx, MODEL)
+        pc = owtk.Preconditioner(m, owtk.PC_LEARNED, model=model, storage=st)
+        for _ in range(2):
+            pc.update_values()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pc.update_values()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
+        print(json.dumps({"kernel": "learned-update", "launches": steps, "vertices": s.nodes,
+                          "storage": pc_storage, "update_ms": round(ms, 4)}))
+
+
 def kernel_loop(kind, steps, pc_storage="f32"):
     """Kernel loops for rocprofv3 counter runs: FP64 split SpMV, the learned
     preconditioner apply (FP32 storage), or a copy kernel of known size
     (FP64 -> FP32 conversion: 8 B read, 4 B written)."""
     if kind == "bsr":
         bsr_kernel_loop(steps)
+        return
+    if kind == "learned-update":
+        learned_update_loop(steps, pc_storage)
         return
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
@@ -714,7 +739,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["spmv", "copy", "learned", "bsr"], default=None)
+    ap.add_argument("--kernel", choices=["spmv", "copy", "learned", "bsr", "learned-update"], default=None)
     ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
                     help="matrix storage of the inner FGMRES (SpMV and ILU fallback)")
     ap.add_argument("--pc-storage", choices=["f32", "bf16"], default="bf16",

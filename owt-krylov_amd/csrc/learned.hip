@@ -114,6 +114,66 @@ __global__ __launch_bounds__(256) void k_gnn_features(std::int64_t n, int b, int
     x[i * f0 + 3 * b] = deg * 0.125f;
 }
 
+// k_gnn_gather + k_gnn_features for a SELL-layout matrix, read in its own
+// slot order: row i's k-th off-diagonal entry sits in slot
+// slice_ptr[i / 64] + 64 k + i % 64, so each k is one coalesced run, and the
+// entries come in the CSR order of the row.  Same arithmetic, same values;
+// the edge weights are stored per slot and the features feature-major,
+// x[f n + i], so every store is coalesced.  Four slots per pass, all loads
+// issued first (a slot past the slice's width is clamped to its last run,
+// in bounds; one past the row's length is dropped).
+__global__ __launch_bounds__(256) void k_gnn_features_sell(std::int64_t n, int b, int f0,
+                                                           const std::int32_t* __restrict__ slice_ptr,
+                                                           const std::int32_t* __restrict__ row_len,
+                                                           const double* __restrict__ v64,
+                                                           const double* __restrict__ d64, float* __restrict__ x,
+                                                           float* __restrict__ w)
+{
+    constexpr int U = 4;
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const std::int64_t lane = i & 63;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    float mx[4] = {0.f, 0.f, 0.f, 0.f};
+    float inv[4];
+    for (int c = 0; c < b; ++c) inv[c] = 1.0f / static_cast<float>(d64[(i - lane) * b + 64 * c + lane]);
+    const std::int64_t base = slice_ptr[i >> 6];
+    const int width = static_cast<int>((slice_ptr[(i >> 6) + 1] - base) >> 6);
+    const int len = row_len[i];
+    for (int k0 = 0; k0 < len; k0 += U) {
+        double va[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k0 + u < width ? k0 + u : width - 1;
+            const std::int64_t run = base + 64 * kk;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) va[u][c] = c < b ? v64[run * b + 64 * c + lane] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u >= len) break;
+            float we = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (c >= b) break;
+                const float a = __builtin_fabsf(static_cast<float>(va[u][c]));
+                s[c] += a;
+                mx[c] = fmaxf(mx[c], a);
+                we += a * inv[c];
+            }
+            w[base + 64 * (k0 + u) + lane] = we / static_cast<float>(b);
+        }
+    }
+    const float deg = static_cast<float>(len);
+    for (int c = 0; c < b; ++c) {
+        const float rho = s[c] * inv[c];
+        x[c * n + i] = rho;
+        x[(b + c) * n + i] = logf(fmaxf(1.0f - rho, 1e-6f)) * 0.125f;
+        x[(2 * b + c) * n + i] = s[c] > 0.f ? logf(fmaxf(fmaxf(deg, 1.f) * mx[c] / s[c], 1.0f)) * 0.5f : 0.f;
+    }
+    x[3 * b * n + i] = deg * 0.125f;
+}
+
 // BF16 parameters -> LDS floats.
 __device__ inline void load_weights(float* dst, const std::uint16_t* __restrict__ p16, std::int64_t off, int count)
 {
@@ -219,6 +279,160 @@ __global__ __launch_bounds__(256) void k_gnn_layer(std::int64_t n, const std::in
             r[j] = tanhf(acc);
         }
         out[q] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+using f32x16 = float __attribute__((ext_vector_type(16)));
+constexpr int kMfmaEdges = 2;  // edges gathered per pass of the aggregation
+
+// One message-passing layer over a SELL-layout graph on the matrix cores
+// (inference).  A wave computes a tile of 32 vertices:
+//     out = tanh([h_v | a_v] Wcat^T + b),  Wcat = [Ws | Wn] (32 x 64),
+// as 32 steps of v_mfma_f32_32x32x2_f32.  Lane l supplies row v = l % 32 of
+// the 32 x 2 A slice and column o = l % 32 of the 2 x 32 B slice; the two
+// half-waves (h = l / 32) split the 64 inputs, step t taking
+//     t < 16: self feature 16 h + t,    t >= 16: aggregate 16 h + t - 16,
+// so each lane gathers and sums only its half of its vertex's features, and
+// its 32 weights stay in registers for the whole launch.  The aggregate is
+// summed in edge order exactly as k_gnn_layer; the products are accumulated
+// in the matrix cores' order.  D: lane l holds column o, rows
+// 8 (r / 4) + 4 h + r % 4 of register r.
+// The input encoding on the matrix cores (inference): a wave computes a
+// tile of 32 vertices, h = tanh(x W_in^T + b_in) with x feature-major
+// (x[f n + v], as k_gnn_features_sell writes it), the f0 <= 16 features
+// padded with zeros to 16 and split over the half-waves (step t takes
+// feature t + 8 h), 8 steps of v_mfma_f32_32x32x2_f32; output layout as in
+// k_gnn_layer_mfma, so each store instruction writes two whole rows of h.
+static_assert(kGnnMaxFeatures <= 16, "the matrix-core encoding takes at most 16 features");
+__global__ __launch_bounds__(256) void k_gnn_encode_mfma(std::int64_t n, int f0, const float* __restrict__ x,
+                                                         const std::uint16_t* __restrict__ p16, std::int64_t off_w,
+                                                         std::int64_t off_b, float* __restrict__ h)
+{
+    const int l = static_cast<int>(threadIdx.x & 63);
+    const int o = l & 31, half = l >> 5;
+    float bw[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int f = 8 * half + t;
+        bw[t] = f < f0 ? dev::bf16_to_float(p16[off_w + o * f0 + f]) : 0.f;
+    }
+    const float bias = dev::bf16_to_float(p16[off_b + o]);
+    const std::int64_t tiles = (n + 31) >> 5;
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         tile < tiles; tile += waves) {
+        const std::int64_t v = tile * 32 + o;
+        const bool valid = v < n;
+        float av[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int f = 8 * half + t;
+            av[t] = valid && f < f0 ? x[f * n + v] : 0.f;  // feature-major
+        }
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = bias;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], bw[t], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const std::int64_t vv = tile * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            if (vv < n) h[vv * W + o] = tanhf(acc[r]);
+        }
+    }
+}
+
+// Three waves per SIMD (at most 168 registers): a tile waits on its gathers,
+// and 2 edges per pass with 3 waves measured faster than 4 edges with 2.
+__global__ __launch_bounds__(256, 3) void k_gnn_layer_mfma(std::int64_t n, const std::int32_t* __restrict__ slice_ptr,
+                                                        const std::int32_t* __restrict__ row_len,
+                                                        const std::int32_t* __restrict__ cols,
+                                                        const float* __restrict__ w, const float* __restrict__ hin,
+                                                        const std::uint16_t* __restrict__ p16, std::int64_t off_ws,
+                                                        std::int64_t off_wn, std::int64_t off_b,
+                                                        float* __restrict__ hout)
+{
+    const int l = static_cast<int>(threadIdx.x & 63);
+    const int o = l & 31, half = l >> 5;
+    float bw[32];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        bw[t] = dev::bf16_to_float(p16[off_ws + o * W + 16 * half + t]);
+        bw[16 + t] = dev::bf16_to_float(p16[off_wn + o * W + 16 * half + t]);
+    }
+    const float bias = dev::bf16_to_float(p16[off_b + o]);
+    const std::int64_t tiles = (n + 31) >> 5;
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         tile < tiles; tile += waves) {
+        const std::int64_t v = tile * 32 + o;
+        const bool valid = v < n;
+        const std::int64_t vc = valid ? v : n - 1;  // in-bounds stand-in for a tail lane
+        float av[32];
+        {
+            const float4* hs = reinterpret_cast<const float4*>(hin + vc * W + 16 * half);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 t = hs[q];
+                av[4 * q] = t.x;
+                av[4 * q + 1] = t.y;
+                av[4 * q + 2] = t.z;
+                av[4 * q + 3] = t.w;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) av[16 + j] = 0.f;
+        const std::int64_t base = slice_ptr[vc >> 6];
+        const int width = static_cast<int>((slice_ptr[(vc >> 6) + 1] - base) >> 6);
+        const int len = valid ? row_len[vc] : 0;
+        // kMfmaEdges edges per pass, every load issued before the sums: a
+        // slot past the slice's width is clamped to its last run (in
+        // bounds) and an edge past the row's length is dropped by a select
+        for (int k0 = 0; k0 < len; k0 += kMfmaEdges) {
+            int ck[kMfmaEdges];
+            float wk[kMfmaEdges];
+#pragma unroll
+            for (int u = 0; u < kMfmaEdges; ++u) {
+                const int kk = k0 + u < width ? k0 + u : width - 1;
+                const std::int64_t slot = base + 64 * kk + (vc & 63);
+                ck[u] = cols[slot];
+                wk[u] = w[slot];
+            }
+            float4 hk[kMfmaEdges][4];
+#pragma unroll
+            for (int u = 0; u < kMfmaEdges; ++u) {
+                const float4* hj = reinterpret_cast<const float4*>(hin + static_cast<std::int64_t>(ck[u]) * W + 16 * half);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) hk[u][q] = hj[q];
+            }
+#pragma unroll
+            for (int u = 0; u < kMfmaEdges; ++u) {
+                const bool on = k0 + u < len;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 t = hk[u][q];
+                    float* a = av + 16 + 4 * q;
+                    const float f0 = __builtin_fmaf(wk[u], t.x, a[0]);
+                    const float f1 = __builtin_fmaf(wk[u], t.y, a[1]);
+                    const float f2 = __builtin_fmaf(wk[u], t.z, a[2]);
+                    const float f3 = __builtin_fmaf(wk[u], t.w, a[3]);
+                    a[0] = on ? f0 : a[0];
+                    a[1] = on ? f1 : a[1];
+                    a[2] = on ? f2 : a[2];
+                    a[3] = on ? f3 : a[3];
+                }
+            }
+        }
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = bias;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], bw[t], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const std::int64_t vv = tile * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            if (vv < n) hout[vv * W + o] = tanhf(acc[r]);
+        }
     }
 }
 
@@ -589,47 +803,56 @@ __global__ __launch_bounds__(256) void k_series_init(std::int64_t nb, const X* _
     }
 }
 
-// Column-scaled off-diagonal values a_ec / d_(col e, c) in the matrix's value
-// layout (SELL-64 or CSR), rounded once from the FP64 product.
-template<class V>
-__global__ __launch_bounds__(256) void k_scale_columns(std::int64_t count, int b, int sell,
-                                                       const std::int32_t* __restrict__ cols,
-                                                       const double* __restrict__ v64, const double* __restrict__ d64,
-                                                       V* __restrict__ out)
+// 1 / d for every diagonal entry, in the diagonal's own layout: the IEEE
+// quotient the column and coefficient scalings below multiply by, formed once
+// per entry instead of once per off-diagonal value.
+__global__ __launch_bounds__(256) void k_inv_diag(std::int64_t count, const double* __restrict__ d64,
+                                                  double* __restrict__ dinv)
 {
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count; t += step) {
-        std::int64_t slot;
-        int c;
-        if (sell) {
-            const std::int64_t run = t / (64 * b);
-            const std::int64_t rem = t - run * 64 * b;
-            c = static_cast<int>(rem / 64);
-            slot = run * 64 + (rem & 63);
-        } else {
-            slot = t / b;
-            c = static_cast<int>(t - slot * b);
+    for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count; t += step)
+        dinv[t] = 1.0 / d64[t];
+}
+
+// Column-scaled off-diagonal values a_ec * (1 / d_(col e, c)) in the matrix's
+// value layout (SELL-64 or CSR), rounded once from the FP64 product.  A
+// thread takes one slot (its column loaded once) and all its components.
+template<class V>
+__global__ __launch_bounds__(256) void k_scale_columns(std::int64_t slots, int b, int sell,
+                                                       const std::int32_t* __restrict__ cols,
+                                                       const double* __restrict__ v64,
+                                                       const double* __restrict__ dinv, V* __restrict__ out)
+{
+    const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t sl = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; sl < slots; sl += step) {
+        const std::int64_t j = cols[sl];
+        for (int c = 0; c < b; ++c) {
+            // SELL: value (slot, c) at (slot & ~63) b + 64 c + slot % 64, diagonal (j, c) likewise
+            const std::int64_t t = sell ? (sl & ~std::int64_t(63)) * b + 64 * c + (sl & 63) : sl * b + c;
+            const std::int64_t dj = sell ? (j & ~std::int64_t(63)) * b + 64 * c + (j & 63) : j * b + c;
+            const double a = v64[t] * dinv[dj];
+            if constexpr (sizeof(V) == 2) out[t] = dev::float_to_bf16(static_cast<float>(a));
+            else out[t] = static_cast<V>(a);
         }
-        const std::int64_t j = cols[slot];
-        const std::int64_t dj = sell ? (j & ~std::int64_t(63)) * b + 64 * c + (j & 63) : j * b + c;
-        const double a = v64[t] * (1.0 / d64[dj]);
-        if constexpr (sizeof(V) == 2) out[t] = dev::float_to_bf16(static_cast<float>(a));
-        else out[t] = static_cast<V>(a);
     }
 }
 
-// gamma'_k = gamma_k / d, rounded once from the FP64 product, k = 0..K.
+// gamma'_k = gamma_k * (1 / d), rounded once from the FP64 product, k = 0..K.
+// One thread per (vertex, component), consecutive threads on consecutive
+// entries; the vertex index by a 32-bit division when n b fits.
 template<class X>
 __global__ __launch_bounds__(256) void k_scale_gamma(std::int64_t nb, int b, int sell, int k1,
-                                                     const float* __restrict__ gamma, const double* __restrict__ d64,
+                                                     const float* __restrict__ gamma, const double* __restrict__ dinv,
                                                      X* __restrict__ out)
 {
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    const bool narrow = nb < (std::int64_t(1) << 31);
     for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nb; t += step) {
-        const std::int64_t i = t / b;
+        const std::int64_t i = narrow ? static_cast<std::int64_t>(static_cast<std::uint32_t>(t) / static_cast<std::uint32_t>(b))
+                                      : t / b;
         const int c = static_cast<int>(t - i * b);
         const std::int64_t di = sell ? (i & ~std::int64_t(63)) * b + 64 * c + (i & 63) : t;
-        const double inv = 1.0 / d64[di];
+        const double inv = dinv[di];
         for (int k = 0; k < k1; ++k)
             out[k * nb + t] = static_cast<X>(static_cast<double>(gamma[k * nb + t]) * inv);
     }
@@ -786,6 +1009,15 @@ void build_graph_structure(owtk_gnn model, owtk_matrix m, GnnGraph& g, bool with
     g.n = m->owned;
     g.b = m->bs;
     g.nnz = m->h_offsets[g.n];
+    if (!with_mirror && m->s.layout == Layout::sell) {
+        g.sell = true;
+        g.slice_ptr = m->s.slice_ptr.get();
+        g.row_len = m->s.row_len.get();
+        g.slot_cols = m->s.columns.get();
+        g.slot_w.allocate(std::max<std::int64_t>(m->s.slots, 1));
+        g.x.allocate(g.n * model->f0);
+        return;
+    }
     upload_vec(g.offsets, m->h_offsets);
     upload_vec(g.cols, m->h_columns);
     if (with_mirror) upload_vec(g.mirror, mirror_index(g.n, m->h_offsets, m->h_columns));
@@ -806,6 +1038,12 @@ void graph_values(owtk_gnn model, owtk_matrix m, GnnGraph& g)
 {
     const hipStream_t st = m->ctx->stream;
     const unsigned int grid = rows_grid(g.n);
+    if (g.sell) {
+        k_gnn_features_sell<<<grid, kBlock, 0, st>>>(g.n, g.b, model->f0, g.slice_ptr, g.row_len, m->v64.get(),
+                                                     m->d64.get(), g.x.get(), g.slot_w.get());
+        OWTK_LAUNCH_CHECK();
+        return;
+    }
     k_gnn_gather<<<grid, kBlock, 0, st>>>(g.n, g.b, g.offsets.get(), g.slot.get(), m->s.layout == Layout::sell ? 1 : 0,
                                           m->v64.get(), m->d64.get(), g.vals.get(), g.diag.get());
     OWTK_LAUNCH_CHECK();
@@ -828,11 +1066,26 @@ void forward(owtk_gnn model, const GnnGraph& g, float* h, float* agg, float* gam
     const std::int64_t n = g.n;
     const unsigned int grid = rows_grid(n);
     const std::uint16_t* p16 = model->params16.get();
-    k_gnn_encode<<<grid, kBlock, 0, st>>>(n, model->f0, g.x.get(), p16, model->off_w_in, model->off_b_in, h);
+    if (g.sell && train) throw std::logic_error("training runs over the CSR graph");
+    // the matrix-core kernels keep each wave's weights in registers across a
+    // grid-stride loop over the 32-vertex tiles
+    const std::int64_t tiles = (n + 31) / 32;
+    const unsigned int mgrid = static_cast<unsigned int>(std::max<std::int64_t>(1, std::min<std::int64_t>((tiles + 3) / 4, 4096)));
+    if (g.sell)
+        k_gnn_encode_mfma<<<mgrid, kBlock, 0, st>>>(n, model->f0, g.x.get(), p16, model->off_w_in, model->off_b_in, h);
+    else
+        k_gnn_encode<<<grid, kBlock, 0, st>>>(n, model->f0, g.x.get(), p16, model->off_w_in, model->off_b_in, h);
     OWTK_LAUNCH_CHECK();
     float* cur = h;
     for (int l = 0; l < kGnnLayers; ++l) {
         float* next = train ? h + static_cast<std::int64_t>(l + 1) * n * W : h + ((l + 1) & 1) * n * W;
+        if (g.sell) {
+            k_gnn_layer_mfma<<<mgrid, kBlock, 0, st>>>(n, g.slice_ptr, g.row_len, g.slot_cols, g.slot_w.get(), cur, p16,
+                                                       model->off_ws[l], model->off_wn[l], model->off_b[l], next);
+            OWTK_LAUNCH_CHECK();
+            cur = next;
+            continue;
+        }
         k_gnn_layer<<<grid, kBlock, 0, st>>>(n, g.offsets.get(), g.cols.get(), g.w.get(), cur, p16, model->off_ws[l],
                                              model->off_wn[l], model->off_b[l], next,
                                              train ? agg + static_cast<std::int64_t>(l) * n * W : nullptr);
@@ -996,30 +1249,34 @@ void learned_numeric(owtk_precond pc)
     GnnGraph& g = st->graph;
     graph_values(model, m, g);
     forward(model, g, st->h.get(), nullptr, st->gamma.get(), false);
-    const std::int64_t nb = pc->n();
     const int sell = m->s.layout == Layout::sell ? 1 : 0;
     const int k1 = model->degree + 1;
-    const std::int64_t count = m->value_count;
-    const unsigned int gv = dev::stream_grid(count, 4);
+    const std::int64_t slots = m->s.slots;
+    const std::int64_t n = m->owned;
+    const std::int64_t dcount = m->diag_count;
+    if (st->dinv.count != static_cast<std::size_t>(std::max<std::int64_t>(dcount, 1)))
+        st->dinv.allocate(std::max<std::int64_t>(dcount, 1));
+    k_inv_diag<<<dev::stream_grid(dcount, 4), kBlock, 0, s>>>(dcount, m->d64.get(), st->dinv.get());
+    OWTK_LAUNCH_CHECK();
+    const std::int64_t nb = n * m->bs;
+    const unsigned int gv = dev::stream_grid(slots, 4);
+    const unsigned int gg = dev::stream_grid(nb, 4);
+    const double* dinv = st->dinv.get();
     switch (pc->storage) {
     case OWTK_F64:
-        k_scale_columns<double><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(), m->d64.get(),
+        k_scale_columns<double><<<gv, kBlock, 0, s>>>(slots, m->bs, sell, m->s.columns.get(), m->v64.get(), dinv,
                                                       reinterpret_cast<double*>(st->scaled.get()));
-        k_scale_gamma<double><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
-                                                                         m->d64.get(), st->sgamma64.get());
+        k_scale_gamma<double><<<gg, kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(), dinv, st->sgamma64.get());
         break;
     case OWTK_F32:
-        k_scale_columns<float><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(), m->d64.get(),
+        k_scale_columns<float><<<gv, kBlock, 0, s>>>(slots, m->bs, sell, m->s.columns.get(), m->v64.get(), dinv,
                                                      reinterpret_cast<float*>(st->scaled.get()));
-        k_scale_gamma<float><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
-                                                                        m->d64.get(), st->sgamma32.get());
+        k_scale_gamma<float><<<gg, kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(), dinv, st->sgamma32.get());
         break;
     default:
-        k_scale_columns<std::uint16_t><<<gv, kBlock, 0, s>>>(count, m->bs, sell, m->s.columns.get(), m->v64.get(),
-                                                             m->d64.get(),
-                                                             reinterpret_cast<std::uint16_t*>(st->scaled.get()));
-        k_scale_gamma<float><<<dev::stream_grid(nb, 4), kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(),
-                                                                        m->d64.get(), st->sgamma32.get());
+        k_scale_columns<std::uint16_t><<<gv, kBlock, 0, s>>>(slots, m->bs, sell, m->s.columns.get(), m->v64.get(),
+                                                             dinv, reinterpret_cast<std::uint16_t*>(st->scaled.get()));
+        k_scale_gamma<float><<<gg, kBlock, 0, s>>>(nb, m->bs, sell, k1, st->gamma.get(), dinv, st->sgamma32.get());
         break;
     }
     OWTK_LAUNCH_CHECK();

@@ -22,8 +22,16 @@ struct GnnGraph {
     int b = 1;
     DeviceArray<std::int32_t> offsets, cols, mirror;  // mirror: entry of (col, row); training only
     DeviceArray<float> vals, diag;                     // [e][c], [i][c]
-    DeviceArray<float> x, w;                           // features [i][f0], weights [e]
+    DeviceArray<float> x, w;                           // features [i][f0] ([f][i] when sell), weights [e]
     DeviceArray<std::int64_t> slot;                    // matrix value slot of each CSR entry
+    // Inference over a SELL-layout matrix reads the matrix's own structure
+    // (edges in slot order, the CSR order of each row) and keeps the edge
+    // weights per slot; the CSR arrays above stay empty.
+    bool sell = false;
+    const std::int32_t* slice_ptr = nullptr;
+    const std::int32_t* row_len = nullptr;
+    const std::int32_t* slot_cols = nullptr;
+    DeviceArray<float> slot_w;                         // weight of the edge in each slot
 };
 
 } // namespace owtk
@@ -78,6 +86,7 @@ struct LearnedPc {
     DeviceArray<unsigned char> scaled;    // A_off D^-1 values, value_count entries
     DeviceArray<double> sgamma64;         // [k][n*b] gamma_k D^-1 (FP64 storage)
     DeviceArray<float> sgamma32;          // [k][n*b] gamma_k D^-1 (FP32 and BF16 storage)
+    DeviceArray<double> dinv;             // 1 / d in the diagonal's layout
     DeviceArray<double> u64a, u64b;       // w_k work vectors
     DeviceArray<float> u32a, u32b;
 };
