@@ -325,6 +325,45 @@ private:
 };
 
 // Device preconditioner (Jacobi, SSOR, ILU(k), learned).
+// The learned preconditioner's graph network (owtk_gnn, no reference
+// counterpart): block size and series degree fixed at creation, parameters
+// one flat FP32 vector of parameter_count() values in the order
+// owtk_gnn_set_parameters documents.  A fresh network gives gamma = 1, the
+// plain truncated Neumann series.  Preconditioner::params(OWTK_PC_LEARNED)
+// with learned_model = model.get() builds the preconditioner from it.
+class GnnModel {
+public:
+    GnnModel(Context& ctx, int block_size, int degree) { check(owtk_gnn_create(ctx.get(), block_size, degree, &g_)); }
+    ~GnnModel()
+    {
+        if (g_) (void)owtk_gnn_destroy(g_);
+    }
+    GnnModel(const GnnModel&) = delete;
+    GnnModel& operator=(const GnnModel&) = delete;
+    owtk_gnn get() const noexcept { return g_; }
+    std::size_t parameter_count() const
+    {
+        std::int32_t b = 0, d = 0;
+        std::int64_t n = 0;
+        check(owtk_gnn_info(g_, &b, &d, &n));
+        return static_cast<std::size_t>(n);
+    }
+    std::vector<float> parameters() const
+    {
+        std::vector<float> p(parameter_count());
+        check(owtk_gnn_get_parameters(g_, p.data()));
+        return p;
+    }
+    void set_parameters(const std::vector<float>& p)
+    {
+        if (p.size() != parameter_count()) throw std::invalid_argument("network parameter count mismatch");
+        check(owtk_gnn_set_parameters(g_, p.data()));
+    }
+
+private:
+    owtk_gnn g_ = nullptr;
+};
+
 class Preconditioner {
 public:
     Preconditioner(DeviceMatrix& m, const owtk_precond_params& params) { check(owtk_precond_create(m.get(), &params, &pc_)); }

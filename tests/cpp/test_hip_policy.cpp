@@ -10,7 +10,8 @@
 //     BiCGSTAB with the compensated reduction, an application convergence
 //     test over a host BlockVector, pipelined BiCGSTAB + SSOR, and the
 //     host-vector route, GCRO-DR with its recycle space and Arnoldi
-//     snapshot (gcrodr.hpp:86-212; recycling.hpp:291-310);
+//     snapshot (gcrodr.hpp:86-212; recycling.hpp:291-310), and the learned
+//     preconditioner (no reference counterpart) in refined FGMRES;
 //   each checked by its true residual recomputed on the host.
 #include <owtk/hip_policy.hpp>
 
@@ -409,6 +410,37 @@ int main()
             refused_record = true;
         }
         expect(refused_record, "record_column takes column + 2 values");
+    }
+
+    // the learned preconditioner through C++: a fresh network (gamma = 1,
+    // the truncated Neumann series of degree 3) in BF16 storage inside the
+    // FP64-refined FP32 FGMRES, then the same network's parameters round trip
+    {
+        owtk::hip::GnnModel model(ctx, 3, 3);
+        auto lp = owtk::hip::Preconditioner::params(OWTK_PC_LEARNED);
+        lp.learned_model = model.get();
+        lp.storage = OWTK_BF16;
+        dm.add_storage(OWTK_BF16);
+        owtk::hip::Preconditioner learned(dm, lp);
+        dx.upload(std::vector<double>(n, 0.0).data(), n);
+        owt::krylov::SolverResult<double> rl =
+            fgmres_refined(dm, db, dx, opts, learned, owtk::hip::refinement_options());
+        dx.download(hx.data(), n);
+        const double rell = true_rel(s, b, hx);
+        std::printf("fgmres_refined + learned (fresh network, BF16): status %d iterations %zu true rel %.3e\n",
+                    static_cast<int>(rl.status), rl.iterations, rell);
+        expect(rl.converged() && rell <= 1e-10, "refined fgmres + learned preconditioner");
+        std::vector<float> params = model.parameters();
+        params.back() = 0.5f;  // one series coefficient g: a changed model
+        model.set_parameters(params);
+        expect(model.parameters() == params, "network parameters round trip");
+        bool wrong_size = false;
+        try {
+            model.set_parameters(std::vector<float>(3));
+        } catch (const std::invalid_argument&) {
+            wrong_size = true;
+        }
+        expect(wrong_size, "network parameter count is checked");
     }
 
     // the production selection: split SSOR + pipelined BiCGSTAB
