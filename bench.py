@@ -680,7 +680,9 @@ def recycling_bench(args):
     """A sequence of related systems (recycling.hpp): the 2048x1024 grid, five
     right-hand sides (the generated one and four perturbed by 1% noise),
     FP64 FGMRES(30) + 4096-block ILU(0) to rtol 1e-8; plain FGMRES for each
-    against recycled FGMRES through one recycle space of 4 vectors."""
+    against recycled FGMRES and GCRO-DR (gcrodr.hpp:185-212: harmonic Ritz
+    vectors of each solve's last cycle), each through one recycle space of 4
+    vectors."""
     with owtk.Context(0) as ctx:
         s, m = build_system(ctx)
         n = s.nodes * BS
@@ -691,14 +693,19 @@ def recycling_bench(args):
         pc = owtk.Preconditioner(m, owtk.PC_ILU, block_nodes=BLOCK_NODES)
         opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=5000, restart=30)
 
-        def sequence(recycle):
-            space = owtk.RecycleSpace(m, 4) if recycle else None
+        def sequence(method):
+            space = owtk.RecycleSpace(m, 4) if method != "fgmres" else None
             its, ms = [], []
             for b in bs_dev:
                 x.zero()
                 ctx.synchronize()
                 t0 = time.perf_counter()
-                res = (owtk.recycled_fgmres(m, pc, b, x, space, opts) if recycle else owtk.fgmres(m, pc, b, x, opts))
+                if method == "fgmres":
+                    res = owtk.fgmres(m, pc, b, x, opts)
+                elif method == "recycled_fgmres":
+                    res = owtk.recycled_fgmres(m, pc, b, x, space, opts)
+                else:
+                    res = owtk.gcrodr(m, pc, b, x, space, opts)
                 ctx.synchronize()
                 ms.append((time.perf_counter() - t0) * 1e3)
                 its.append(res["iterations"])
@@ -708,10 +715,10 @@ def recycling_bench(args):
             return its, ms
 
         out_methods = {}
-        for name, rec in (("fgmres", False), ("recycled_fgmres", True)):
+        for name in ("fgmres", "recycled_fgmres", "gcrodr"):
             for _ in range(args.warmup):
-                sequence(rec)
-            runs = [sequence(rec) for _ in range(args.steps)]
+                sequence(name)
+            runs = [sequence(name) for _ in range(args.steps)]
             its = runs[-1][0]
             totals = [sum(ms) for _, ms in runs]
             out_methods[name] = {"total_ms": round(float(np.median(totals)), 2), "iterations": its,

@@ -1217,6 +1217,11 @@ struct owtk_recycle_space_s {
     std::size_t capacity = 0;
     owtk::RecycleVectors v;
     owtk::DeviceArray<double> coef;  // capacity + 1 coefficients
+    // vectors of `local` doubles kept for reuse: add_candidate takes its two
+    // work vectors from here and rejected, dropped or replaced pairs return
+    // here, so a steady sequence of solves allocates (and frees, which
+    // synchronises the device) nothing
+    std::vector<owtk::DeviceArray<double>> spare;
 };
 
 namespace owtk {
@@ -1237,6 +1242,23 @@ double read_scalar(owtk_context ctx, const double* d)
     return v;
 }
 
+// A vector of the space's layout from its spares, or a new one.
+DeviceArray<double> rc_take(owtk_recycle_space sp)
+{
+    if (sp->spare.empty()) return DeviceArray<double>(static_cast<std::size_t>(sp->local));
+    DeviceArray<double> v = std::move(sp->spare.back());
+    sp->spare.pop_back();
+    return v;
+}
+
+// Back to the spares (at most two per recycled pair plus two).  Every use is
+// on the context's one stream, so work already launched on a vector stays
+// ordered before its next user's.
+void rc_give(owtk_recycle_space sp, DeviceArray<double>&& v)
+{
+    if (v.get() && sp->spare.size() < 2 * sp->capacity + 2) sp->spare.push_back(std::move(v));
+}
+
 // add_candidate (recycling.hpp:79-135)
 bool rc_add(owtk_recycle_space sp, owtk_matrix m, const double* cand, double tol, owtk_solver_result* tel)
 {
@@ -1244,7 +1266,12 @@ bool rc_add(owtk_recycle_space sp, owtk_matrix m, const double* cand, double tol
     const hipStream_t st = ctx->stream;
     const auto scratch = ctx->scratch();
     const std::int64_t n = sp->n;
-    DeviceArray<double> vec(static_cast<std::size_t>(sp->local)), img(static_cast<std::size_t>(sp->local));
+    DeviceArray<double> vec = rc_take(sp), img = rc_take(sp);
+    auto reject = [&] {
+        rc_give(sp, std::move(vec));
+        rc_give(sp, std::move(img));
+        return false;
+    };
     kern::fill_zero_bytes(st, vec.get(), vec.bytes());
     kern::fill_zero_bytes(st, img.get(), img.bytes());
     OWTK_HIP(hipMemcpyAsync(vec.get(), cand, n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1257,7 +1284,7 @@ bool rc_add(owtk_recycle_space sp, owtk_matrix m, const double* cand, double tol
     kern::norm_to(st, scratch, img.get(), n, d_norm);
     const double orig = read_scalar(ctx, d_norm);
     count(tel, 1, 1);
-    if (!std::isfinite(orig) || orig == 0.0) return false;
+    if (!std::isfinite(orig) || orig == 0.0) return reject();
     const int k = static_cast<int>(sp->v.C.size());
     const unsigned int g = dev::stream_grid(n, 4);
     for (int pass = 0; pass < 2 && k > 0; ++pass) {
@@ -1271,11 +1298,12 @@ bool rc_add(owtk_recycle_space sp, owtk_matrix m, const double* cand, double tol
     kern::norm_to(st, scratch, img.get(), n, d_norm);
     const double nv = read_scalar(ctx, d_norm);
     count(tel, 0, 1);
-    if (!std::isfinite(nv) || nv == 0.0 || nv / orig <= tol) return false;
+    if (!std::isfinite(nv) || nv == 0.0 || nv / orig <= tol) return reject();
     k_rc_scale2<<<g, kBlock, 0, st>>>(img.get(), vec.get(), 1.0 / nv, n);
     OWTK_LAUNCH_CHECK();
     if (sp->v.C.size() == sp->capacity) {
-        ctx->sync();  // earlier kernels may still read the dropped pair
+        rc_give(sp, std::move(sp->v.C.front()));
+        rc_give(sp, std::move(sp->v.U.front()));
         sp->v.C.pop_front();
         sp->v.U.pop_front();
     }
@@ -1290,7 +1318,8 @@ void rc_refresh(owtk_recycle_space sp, owtk_matrix m, owtk_solver_result* tel)
     RecycleVectors old = std::move(sp->v);
     sp->v = RecycleVectors{};
     for (const auto& u : old.U) rc_add(sp, m, u.get(), 128 * std::numeric_limits<double>::epsilon(), tel);
-    sp->ctx->sync();  // the old vectors are read by the launches above
+    for (auto& u : old.U) rc_give(sp, std::move(u));
+    for (auto& c : old.C) rc_give(sp, std::move(c));
 }
 
 // project_initial_guess (recycling.hpp:138-163): x += U C^T r
@@ -1321,6 +1350,110 @@ __global__ __launch_bounds__(256) void k_rc_diff(const double* __restrict__ x, c
     const std::int64_t step = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
     for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += step)
         out[t] = x[t] - x0[t];
+}
+
+// The projected matrices of the harmonic-Ritz extraction in one pass:
+// P[i][j] = <A_i, B_j> for i < na, j < nb, with A_i = a + i lda and
+// B_j = j < nb1 ? b1 + j ldb1 : b2 + (j - nb1) ldb2 (here A = AZ, B = [AZ | Z],
+// so P = [(AZ)^T AZ | (AZ)^T Z]).  A multi-dot per column would read every
+// vector once per column; here each is read once.  A block stages `rows`
+// rows of all na + nb vectors in LDS ([row][vector]), then accumulates 4 x 4
+// tiles of P over them: with fewer tiles than threads, the threads of a tile
+// split the staged rows and are summed in thread order at the end; with
+// more, each thread takes several tiles.  Per-block partials are summed in
+// block order by k_gram_finish, so the result does not depend on scheduling.
+constexpr int kGramBlocks = 2048;
+constexpr int kGramLds = 48 * 1024;                         // staging bytes per block
+constexpr int kGramMaxTilesPerThread = 2;                   // (60 / 4) x (120 / 4) = 450 tiles, 256 threads
+
+int gram_rows(int vecs) { return std::max(1, std::min(128, kGramLds / (8 * vecs))); }
+
+__global__ __launch_bounds__(256) void k_gram(const double* __restrict__ a, std::int64_t lda, int na,
+                                              const double* __restrict__ b1, std::int64_t ldb1,
+                                              const double* __restrict__ b2, std::int64_t ldb2, int nb1, int nb,
+                                              std::int64_t n, int rows, double* __restrict__ partials)
+{
+    extern __shared__ double stage[];  // [rows][na + nb], later the per-thread tiles
+    const int vecs = na + nb;
+    const int ta = (na + 3) / 4, tb = (nb + 3) / 4, tiles = ta * tb;
+    const int nt = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+    const int per = tiles >= nt ? 1 : nt / tiles;    // threads sharing a tile
+    const int sub = tiles >= nt ? 0 : tid % per;     // this thread's share of the staged rows
+    const int first = tiles >= nt ? tid : tid / per; // this thread's first tile
+    const bool active = tiles >= nt || tid < tiles * per;
+    double acc[kGramMaxTilesPerThread][16];
+#pragma unroll
+    for (int u = 0; u < kGramMaxTilesPerThread; ++u)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[u][e] = 0.0;
+    for (std::int64_t r0 = static_cast<std::int64_t>(blockIdx.x) * rows; r0 < n;
+         r0 += static_cast<std::int64_t>(gridDim.x) * rows) {
+        // rows r0 .. r0 + rows - 1 of every vector (coalesced along the rows), zero past n
+        for (int e = tid; e < rows * vecs; e += nt) {
+            const int v = e / rows, r = e - v * rows;
+            const std::int64_t row = r0 + r;
+            const double* src = v < na ? a + v * lda
+                                       : (v - na < nb1 ? b1 + (v - na) * ldb1 : b2 + (v - na - nb1) * ldb2);
+            stage[r * vecs + v] = row < n ? src[row] : 0.0;
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < kGramMaxTilesPerThread; ++u) {
+                const int tile = first + u * nt;
+                if (tile >= tiles || (u > 0 && tiles < nt)) break;
+                const int i0 = 4 * (tile / tb), j0 = 4 * (tile % tb);
+                for (int r = sub; r < rows; r += per) {
+                    const double* row = stage + r * vecs;
+                    double av[4], bv[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        av[q] = i0 + q < na ? row[i0 + q] : 0.0;
+                        bv[q] = j0 + q < nb ? row[na + j0 + q] : 0.0;
+                    }
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[u][4 * p + q] = __builtin_fma(av[p], bv[q], acc[u][4 * p + q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* out = partials + static_cast<std::int64_t>(blockIdx.x) * na * nb;
+    if (tiles >= nt) {
+#pragma unroll
+        for (int u = 0; u < kGramMaxTilesPerThread; ++u) {
+            const int tile = first + u * nt;
+            if (tile >= tiles) break;
+            const int i0 = 4 * (tile / tb), j0 = 4 * (tile % tb);
+            for (int p = 0; p < 4; ++p)
+                for (int q = 0; q < 4; ++q)
+                    if (i0 + p < na && j0 + q < nb) out[(i0 + p) * nb + j0 + q] = acc[u][4 * p + q];
+        }
+        return;
+    }
+    // the threads of a tile, summed in thread order (256 x 16 doubles fit the staging area)
+    for (int e = 0; e < 16; ++e) stage[tid * 16 + e] = acc[0][e];
+    __syncthreads();
+    for (int t = tid; t < tiles * 16; t += nt) {
+        const int tile = t >> 4, e = t & 15;
+        double s = 0.0;
+        for (int k = 0; k < per; ++k) s += stage[(tile * per + k) * 16 + e];
+        const int i = 4 * (tile / tb) + (e >> 2), j = 4 * (tile % tb) + (e & 3);
+        if (i < na && j < nb) out[i * nb + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish(const double* __restrict__ partials, int blocks, int count,
+                                                     double* __restrict__ out)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= count) return;
+    double s = 0.0;
+    for (int blk = 0; blk < blocks; ++blk) s += partials[static_cast<std::int64_t>(blk) * count + t];
+    out[t] = s;
 }
 
 } // namespace
@@ -1393,35 +1526,9 @@ int recycled_fgmres_run(owtk_matrix m, owtk_precond pc, const double* b, double*
     return OWTK_OK;
 }
 
-namespace {
-
-// A copy of a recycle space (its own device vectors).
-std::unique_ptr<owtk_recycle_space_s> clone_space(const owtk_recycle_space_s& sp)
-{
-    auto out = std::make_unique<owtk_recycle_space_s>();
-    out->ctx = sp.ctx;
-    out->n = sp.n;
-    out->local = sp.local;
-    out->capacity = sp.capacity;
-    out->coef.allocate(sp.capacity + 1);
-    const hipStream_t st = sp.ctx->stream;
-    auto copy = [&](const std::deque<DeviceArray<double>>& from, std::deque<DeviceArray<double>>& to) {
-        for (const auto& v : from) {
-            DeviceArray<double> c(static_cast<std::size_t>(sp.local));
-            OWTK_HIP(hipMemcpyAsync(c.get(), v.get(), static_cast<std::size_t>(sp.local) * sizeof(double),
-                                    hipMemcpyDeviceToDevice, st));
-            to.push_back(std::move(c));
-        }
-    };
-    copy(sp.v.U, out->v.U);
-    copy(sp.v.C, out->v.C);
-    return out;
-}
-
-} // namespace
 
 // extract_harmonic_ritz (gcrodr.hpp:86-181): images A Z, the projected
-// matrices (AZ)^T AZ and (AZ)^T Z in one batch of multi-dots, the
+// matrices (AZ)^T AZ and (AZ)^T Z in one pass over the vectors (k_gram), the
 // eigenproblem on the host, then the candidates Z y in ascending |theta|,
 // each group committed only when all its directions pass add_candidate.
 std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owtk_recycle_space sp,
@@ -1438,6 +1545,7 @@ std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owt
         return 0;
     }
     if (sn->ld != sp->local || sn->owned != sp->n) throw std::invalid_argument("snapshot vector layout mismatch");
+    if (n > kMaxRestart) throw std::invalid_argument("Arnoldi snapshot holds too many columns");
     const std::int64_t local = sp->local, owned = sp->n;
     std::vector<const double*> Z(n), AZ(n);
     for (int c = 0; c < n; ++c) Z[c] = sn->basis.get() + static_cast<std::int64_t>(c) * local;
@@ -1450,23 +1558,25 @@ std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owt
         spmv_f64(m, Z[c], e);
     }
     count(tel, static_cast<std::uint64_t>(n), 1);
-    // column c: <AZ_r, AZ_c> and <AZ_r, Z_c> for every r (n + 1 values each)
-    const std::size_t per = static_cast<std::size_t>(n) + 1;
-    DeviceArray<double> dots(2 * static_cast<std::size_t>(n) * per);
-    const auto scratch = ctx->scratch();
-    for (int c = 0; c < n; ++c) {
-        kern::multidot_to(st, scratch, owned, n, AZ.data(), AZ[c], dots.get() + static_cast<std::size_t>(c) * per);
-        kern::multidot_to(st, scratch, owned, n, AZ.data(), Z[c],
-                          dots.get() + static_cast<std::size_t>(n + c) * per);
-    }
+    // P = [(AZ)^T AZ | (AZ)^T Z] (n x 2n, row-major) in one pass over the vectors
+    const int grows = gram_rows(3 * n);
+    const int blocks = static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(
+        (owned + grows - 1) / grows, kGramBlocks)));
+    const int pairs = 2 * n * n;
+    DeviceArray<double> partials(static_cast<std::size_t>(blocks) * pairs), dots(static_cast<std::size_t>(pairs));
+    k_gram<<<blocks, kBlock, kGramLds, st>>>(images.get(), local, n, images.get(), local, sn->basis.get(), sn->ld, n,
+                                             2 * n, owned, grows, partials.get());
+    OWTK_LAUNCH_CHECK();
+    k_gram_finish<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(partials.get(), blocks, pairs, dots.get());
+    OWTK_LAUNCH_CHECK();
     std::vector<double> h_dots(dots.count);
     OWTK_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), dots.bytes(), hipMemcpyDeviceToHost, st));
     ctx->sync();
     std::vector<double> harmonic(static_cast<std::size_t>(n) * n), metric(harmonic.size());
     for (int c = 0; c < n; ++c)
         for (int r = 0; r < n; ++r) {
-            harmonic[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(c) * per + r];
-            metric[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(n + c) * per + r];
+            harmonic[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(r) * 2 * n + c];
+            metric[static_cast<std::size_t>(c) * n + r] = h_dots[static_cast<std::size_t>(r) * 2 * n + n + c];
         }
     for (double v : harmonic)
         if (!std::isfinite(v)) throw std::runtime_error("nonfinite harmonic-Ritz projected matrix");
@@ -1494,13 +1604,18 @@ std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owt
     replacement->local = local;
     replacement->capacity = sp->capacity;
     replacement->coef.allocate(sp->capacity + 1);
+    replacement->spare = std::move(sp->spare);  // the replacement draws on the space's spares
     kern::BasisPointers zp{};
     for (int c = 0; c < n; ++c) zp.v[c] = Z[c];
     DeviceArray<double> coef(static_cast<std::size_t>(n)), cand(static_cast<std::size_t>(owned));
     const unsigned int g = dev::stream_grid(owned, 4);
     for (const Group& grp : groups) {
         if (replacement->v.C.size() + static_cast<std::size_t>(grp.width) > replacement->capacity) continue;
-        auto trial = clone_space(*replacement);
+        // The group is tried in place: within the capacity add_candidate only
+        // appends (a rejected candidate leaves the space as it was), so
+        // taking back what a rejected group appended restores the space, as
+        // discarding the reference's trial copy does.
+        const std::size_t before = replacement->v.C.size();
         bool accepted = true;
         for (int part = 0; part < grp.width; ++part) {
             // candidate = sum_b y_b Z_b, added in basis order from zero
@@ -1510,13 +1625,24 @@ std::uint64_t extract_harmonic_ritz(owtk_matrix m, owtk_arnoldi_snapshot sn, owt
             k_rc_project<<<g, kBlock, 0, st>>>(cand.get(), nullptr, zp, kern::BasisPointers{}, coef.get(), n, owned,
                                                1.0, nullptr);
             OWTK_LAUNCH_CHECK();
-            accepted = rc_add(trial.get(), m, cand.get(), 128 * std::numeric_limits<double>::epsilon(), tel) && accepted;
+            accepted = rc_add(replacement.get(), m, cand.get(), 128 * std::numeric_limits<double>::epsilon(), tel)
+                && accepted;
         }
         ctx->sync();  // the candidate and coefficients are reused by the next group
-        if (accepted) replacement = std::move(trial);
+        if (!accepted) {
+            while (replacement->v.C.size() > before) {
+                rc_give(replacement.get(), std::move(replacement->v.C.back()));
+                rc_give(replacement.get(), std::move(replacement->v.U.back()));
+                replacement->v.C.pop_back();
+                replacement->v.U.pop_back();
+            }
+        }
     }
     ctx->sync();
+    for (auto& u : sp->v.U) rc_give(replacement.get(), std::move(u));
+    for (auto& c : sp->v.C) rc_give(replacement.get(), std::move(c));
     sp->v = std::move(replacement->v);
+    sp->spare = std::move(replacement->spare);
     return static_cast<std::uint64_t>(sp->v.C.size());
 }
 
@@ -1791,6 +1917,7 @@ int owtk_recycle_space_clear(owtk_recycle_space sp)
         sp->ctx->activate();
         sp->ctx->sync();
         sp->v = owtk::RecycleVectors{};
+        sp->spare.clear();
     });
 }
 
