@@ -381,8 +381,10 @@ def batched_roofline(systems, vertices, ms_it):
 
 
 def batched_bench(args):
-    """512 independent block-3 systems, BiCGSTAB + Jacobi to rtol 1e-8 in one
-    graph-replayed batch; systems/s and per-system stop times."""
+    """512 independent block-3 systems, BiCGSTAB + Jacobi to rtol 1e-8: the
+    small systems each solved whole by one workgroup in one launch, the others
+    in graph-replayed iterations beside it (split autotuned; the all-graph
+    time is reported too); systems/s and per-system stop times."""
     from owtk import workloads
 
     t0 = time.perf_counter()
@@ -399,19 +401,30 @@ def batched_bench(args):
         x = ctx.empty(rhs.size)
         tile, ms_it = bt.autotune(b, iterations=8)
         log(f"[bench] autotuner: tile {tile} rows, {ms_it:.3f} ms per all-active iteration")
-        for _ in range(args.warmup):
-            x.zero()
-            bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
-        times, last = [], None
-        for k in range(args.steps):
-            x.zero()
-            ctx.synchronize()
-            t1 = time.perf_counter()
-            last = bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
-            ctx.synchronize()
-            times.append((time.perf_counter() - t1) * 1e3)
-            log(f"[bench] batched solve {k + 1}/{args.steps}: {times[-1]:.1f} ms")
-        ms = float(np.median(times))
+
+        def timed(label):
+            for _ in range(args.warmup):
+                x.zero()
+                bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+            times, res = [], None
+            for k in range(args.steps):
+                x.zero()
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                res = bt.bicgstab(b, x, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+                ctx.synchronize()
+                times.append((time.perf_counter() - t1) * 1e3)
+                log(f"[bench] batched solve ({label}) {k + 1}/{args.steps}: {times[-1]:.1f} ms")
+            return float(np.median(times)), res
+
+        # every system on the graph replays, then the autotuned split between
+        # one-workgroup solves of the small systems and the graph replays
+        graph_ms, _ = timed("graph replays only")
+        limit, threads, tune_ms = bt.autotune_groups(b, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+        group_systems = sum(s[0] <= limit for s in systems)
+        log(f"[bench] group autotuner: systems of <= {limit} vertices on one workgroup of {threads} threads "
+            f"({group_systems} systems), {tune_ms:.1f} ms per solve")
+        ms, last = timed(f"group limit {limit}")
         stream = stream_ceiling(ctx)
         stop = np.array([r["stop_ms"] for r in last])
         its = np.array([r["iterations"] for r in last])
@@ -455,7 +468,10 @@ def batched_bench(args):
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"{len(systems)} systems, grids log-uniform 32x16..512x256, block 3, "
                                    f"edge-contrast sigma U(0,1.5), seeds 3000+, {vertices} vertices in all",
-                       "tile_rows": tile, "graph_iterations": BATCH_GRAPH_ITERATIONS},
+                       "tile_rows": tile, "graph_iterations": BATCH_GRAPH_ITERATIONS,
+                       "group_limit_vertices": limit, "group_threads": threads, "group_systems": group_systems},
+            "graph_only_ms": round(graph_ms, 3),
+            "graph_only_systems_per_s": round(len(systems) / (graph_ms / 1e3), 2),
             "per_system_ms": {"p50": float(np.percentile(stop, 50)), "p99": float(np.percentile(stop, 99)),
                               "max": float(stop.max())},
             "iterations": {"p50": float(np.median(its)), "max": int(its.max())},

@@ -584,7 +584,9 @@ typedef struct owtk_batch_result {
     double rhs_norm;
     double relative_residual_norm;
     double stop_ms;            /* ms from the solve's start to the graph replay
-                                  in which the system stopped */
+                                  in which the system stopped (a workgroup-
+                                  solved system: to the device clock's reading
+                                  when it stopped) */
 } owtk_batch_result;
 
 int owtk_batch_create(owtk_context ctx, int32_t systems, int32_t block_size, const int64_t* vertex_offsets,
@@ -597,6 +599,18 @@ int owtk_batch_set_tile(owtk_batch batch, int32_t tile_rows);
  * and keeps the fastest; returns it and its ms per iteration. */
 int owtk_batch_autotune(owtk_batch batch, const double* b, int32_t iterations, int32_t* tile_rows,
                         double* ms_per_iteration);
+/* Systems of at most max_vertices vertices are solved whole by one workgroup
+ * of `threads` threads (256, 512 or 1024) each, in one launch that loops over
+ * the iterations on a second stream while the graph replays run the others;
+ * 0 (the default) puts every system on the graph path.  Same recurrence and
+ * stages; the reductions are summed in another (fixed) order. */
+int owtk_batch_set_group_limit(owtk_batch batch, int64_t max_vertices, int32_t threads);
+/* Times whole solves of b (x0 = 0, `options`) for limits 0, 1024, 2048, ...
+ * (up to the largest system, at most 65,536) with 1024 threads, then 256 and
+ * 512 threads at the best limit, and keeps the fastest; returns its limit,
+ * threads and ms per solve. */
+int owtk_batch_autotune_groups(owtk_batch batch, const double* b, const owtk_solver_options* options,
+                               int32_t graph_iterations, int64_t* max_vertices, int32_t* threads, double* ms);
 /* b, x: device FP64 over all systems' unknowns (x holds the initial guesses).
  * graph_iterations: iterations per captured graph (the host checks the
  * number of running systems once per replay).  results: host array of

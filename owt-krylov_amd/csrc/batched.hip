@@ -20,6 +20,13 @@
 // replacement checks.  G iterations are captured once into a hipGraph and the
 // graph is replayed until every system has stopped; the host reads one
 // counter per replay.  The tile size is chosen by owtk_batch_autotune.
+//
+// Small systems.  Systems of at most a set number of vertices can instead be
+// solved whole by one workgroup each (kb_solve_group): one launch on a second
+// stream, beside the graph replays, loops over the iterations with the same
+// passes and scalar stages, its reductions summed inside the workgroup.
+// owtk_batch_autotune_groups picks the limit and the workgroup size by timing
+// whole solves.
 #include "internal.hpp"
 #include "solver_common.hpp"
 #include "kernels/blas1.hpp"
@@ -53,6 +60,7 @@ struct SysState {
     int final;        // iteration limit reached
     int iters, status, reason;
     int check_interval, replace_interval, max_iterations;
+    int owner;        // 0: graph path; 1: one workgroup (kb_solve_group)
 };
 
 struct Tiling {
@@ -94,6 +102,23 @@ struct owtk_batch_s {
     DeviceArray<int> active_count;
     std::unique_ptr<Tiling> tiling;
     int tile = 256;
+    // Systems of at most group_limit vertices are solved by one workgroup of
+    // group_threads threads each (kb_solve_group) on `side`, beside the graph
+    // replays of the others.
+    std::int64_t group_limit = 0;
+    int group_threads = 512;
+    std::vector<int> h_owner;              // per system: 0 graph, 1 workgroup
+    int graph_systems = 0, group_systems = 0;
+    DeviceArray<std::int32_t> group_order; // workgroup systems, largest first
+    DeviceArray<unsigned long long> clocks;  // [systems] stop times, [systems] start
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    ~owtk_batch_s()
+    {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
+    }
 };
 
 namespace {
@@ -134,7 +159,7 @@ __device__ inline RowRef row_of(const Ptrs& P)
 }
 
 // y_c = sum over the row's entries (diagonal included) of a_c x_col,c
-template<int BS>
+template<int BS, int U = 8>
 __device__ inline void row_spmv(const Ptrs& P, std::int64_t prow, const double* __restrict__ x, double (&y)[BS])
 {
     const std::int64_t slice = prow >> 6;
@@ -146,7 +171,6 @@ __device__ inline void row_spmv(const Ptrs& P, std::int64_t prow, const double* 
     for (int c = 0; c < BS; ++c) y[c] = 0.0;
     // The first U slots load with no branch (index clamped into the slice,
     // product kept for k < len), so all of them are in flight at once.
-    constexpr int U = 8;
     if (width > 0) {
         std::int32_t colk[U];
 #pragma unroll
@@ -456,21 +480,30 @@ __device__ inline void restart_recurrence(SysState& st)
     st.omega = 1.0;
 }
 
+// Whether stage STAGE needs the system's sums; when it does not, stage_skip
+// makes the one state change a skipped stage makes.
 template<int STAGE>
-__global__ __launch_bounds__(64) void kb_finish(Ptrs P, const std::int32_t* __restrict__ sys_blk0)
+__device__ inline bool stage_needs_sums(const SysState& st)
 {
-    const int sys = blockIdx.x;
-    SysState& st = P.st[sys];
-    if (STAGE != fin_init && st.phase != 1) return;
-    if (STAGE == fin_true && !st.need_true) return;
-    constexpr int K = (STAGE == fin_init || STAGE == fin_omega || STAGE == fin_rho) ? 2 : 1;
-    if (STAGE == fin_omega && st.early) return;
-    if (STAGE == fin_rho && st.early) {
-        if (threadIdx.x == 0) st.need_true = 1;
-        return;
-    }
-    const Sums sm = system_sums<K>(P, sys_blk0, sys);
-    if (threadIdx.x != 0) return;
+    if (STAGE != fin_init && st.phase != 1) return false;
+    if (STAGE == fin_true && !st.need_true) return false;
+    if ((STAGE == fin_omega || STAGE == fin_rho) && st.early) return false;
+    return true;
+}
+
+template<int STAGE>
+__device__ inline void stage_skip(SysState& st)
+{
+    if (STAGE == fin_rho && st.phase == 1 && st.early) st.need_true = 1;
+}
+
+template<int STAGE>
+constexpr int stage_sums = (STAGE == fin_init || STAGE == fin_omega || STAGE == fin_rho) ? 2 : 1;
+
+// The scalar step of stage STAGE from the system's sums.
+template<int STAGE>
+__device__ inline void finish_stage(SysState& st, const Sums& sm)
+{
     if constexpr (STAGE == fin_init) {
         const double rn = __builtin_sqrt(sm.v[0]);
         st.bnorm = __builtin_sqrt(sm.v[1]);
@@ -567,12 +600,240 @@ __global__ __launch_bounds__(64) void kb_finish(Ptrs P, const std::int32_t* __re
     }
 }
 
+template<int STAGE>
+__global__ __launch_bounds__(64) void kb_finish(Ptrs P, const std::int32_t* __restrict__ sys_blk0)
+{
+    const int sys = blockIdx.x;
+    SysState& st = P.st[sys];
+    if (st.owner != 0) return;
+    if (!stage_needs_sums<STAGE>(st)) {
+        if (threadIdx.x == 0) stage_skip<STAGE>(st);
+        return;
+    }
+    const Sums sm = system_sums<stage_sums<STAGE>>(P, sys_blk0, sys);
+    if (threadIdx.x != 0) return;
+    finish_stage<STAGE>(st, sm);
+}
+
 __global__ void kb_count_active(const SysState* __restrict__ st, int systems, int* __restrict__ count)
 {
     int c = 0;
-    for (int s = threadIdx.x; s < systems; s += blockDim.x) c += st[s].phase == 1 ? 1 : 0;
+    for (int s = threadIdx.x; s < systems; s += blockDim.x) c += (st[s].owner == 0 && st[s].phase == 1) ? 1 : 0;
     const double total = dev::block_sum_all(static_cast<double>(c));
     if (threadIdx.x == 0) *count = static_cast<int>(total);
+}
+
+// ------------------------------------------------- one-workgroup solves --
+// Systems at or below the group limit are solved whole by one workgroup each,
+// in one launch that loops over the iterations: the same streaming passes and
+// the same scalar stages as the graph path, with the five reductions summed
+// inside the workgroup (per wave, then the waves in order: deterministic) and
+// workgroup barriers where the graph path has kernel boundaries.  No
+// workgroup waits on another, and the stages stop every system by its
+// iteration limit, so every workgroup reaches the end.
+
+__global__ void kb_stamp(unsigned long long* __restrict__ out)
+{
+    *out = wall_clock64();
+}
+
+// Sum of K values over the workgroup, valid in thread 0.  Ends with the
+// barrier that makes the wave sums visible; the caller's next barrier
+// separates this use of `lds` from the next.
+template<int K, int NT>
+__device__ inline Sums group_sums(double (&v)[K], double (*lds)[kPartials])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = dev::wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave][k] = v[k];
+    }
+    __syncthreads();
+    Sums out{};
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double acc = lds[0][k];
+            for (int w = 1; w < NT / 64; ++w) acc += lds[w][k];
+            out.v[k] = acc;
+        }
+    }
+    return out;
+}
+
+struct GroupVectors {
+    const double* b;
+    double *x, *r, *rhat, *p, *v, *s, *t, *phat, *shat;
+};
+
+template<int BS, int NT>
+__global__ __launch_bounds__(NT) void kb_solve_group(Ptrs P, const std::int32_t* __restrict__ order, GroupVectors V,
+                                                     unsigned long long* __restrict__ stop_clock)
+{
+    // 1024 threads leave 128 registers a lane: four slots in flight then
+    constexpr int kU = NT >= 1024 ? 4 : 8;
+    __shared__ SysState sst;
+    __shared__ double lds[NT / 64][kPartials];
+    const int sys = order[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (tid == 0) sst = P.st[sys];
+    const std::int64_t v0 = P.vprefix[sys];
+    const std::int64_t n = P.vprefix[sys + 1] - v0;
+    const std::int64_t p0 = P.pprefix[sys];
+    __syncthreads();
+
+    // r = b - A x, r^ = r, p = v = 0; |r|^2, |b|^2
+    {
+        double red[2] = {0.0, 0.0};
+        for (std::int64_t l = tid; l < n; l += NT) {
+            double ax[BS];
+            row_spmv<BS, kU>(P, p0 + l, V.x, ax);
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const std::int64_t i = (v0 + l) * BS + c;
+                const double ri = V.b[i] - ax[c];
+                V.r[i] = ri;
+                V.rhat[i] = ri;
+                V.p[i] = 0.0;
+                V.v[i] = 0.0;
+                red[0] = __builtin_fma(ri, ri, red[0]);
+                red[1] = __builtin_fma(V.b[i], V.b[i], red[1]);
+            }
+        }
+        const Sums sm = group_sums<2, NT>(red, lds);
+        if (tid == 0) finish_stage<fin_init>(sst, sm);
+        __syncthreads();
+    }
+
+    // Every stage stops the system by max_iterations + 1 iterations; the
+    // bound below only guards the loop itself.
+    const int cap = sst.max_iterations + 2;
+    for (int guard = 0; guard < cap && sst.phase == 1; ++guard) {
+        // p = r or r + beta (p - omega v); r^ = r on restart; phat = M p
+        {
+            const bool first = sst.first != 0, restart = sst.restart != 0;
+            const double beta = sst.beta, omega = sst.omega;
+            for (std::int64_t l = tid; l < n; l += NT) {
+                const std::int64_t i0 = (v0 + l) * BS;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const double rv = V.r[i0 + c];
+                    const double pn = first ? rv : rv + beta * (V.p[i0 + c] - omega * V.v[i0 + c]);
+                    V.p[i0 + c] = pn;
+                    V.phat[i0 + c] = P.dinv[i0 + c] * pn;
+                    if (restart) V.rhat[i0 + c] = rv;
+                }
+            }
+            __syncthreads();
+        }
+        // v = A phat; <r^, v> -> alpha
+        {
+            double red[1] = {0.0};
+            for (std::int64_t l = tid; l < n; l += NT) {
+                double y[BS];
+                row_spmv<BS, kU>(P, p0 + l, V.phat, y);
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    V.v[i] = y[c];
+                    red[0] = __builtin_fma(V.rhat[i], y[c], red[0]);
+                }
+            }
+            const Sums sm = group_sums<1, NT>(red, lds);
+            if (tid == 0) finish_stage<fin_alpha>(sst, sm);
+            __syncthreads();
+        }
+        if (sst.phase != 1) break;
+        // s = r - alpha v; shat = M s; |s|^2 -> early exit test
+        {
+            const double alpha = sst.alpha;
+            double red[1] = {0.0};
+            for (std::int64_t l = tid; l < n; l += NT) {
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    const double si = V.r[i] - alpha * V.v[i];
+                    V.s[i] = si;
+                    V.shat[i] = P.dinv[i] * si;
+                    red[0] = __builtin_fma(si, si, red[0]);
+                }
+            }
+            const Sums sm = group_sums<1, NT>(red, lds);
+            if (tid == 0) finish_stage<fin_s>(sst, sm);
+            __syncthreads();
+        }
+        // early exit: x += alpha phat; else t = A shat; <t, s>, <t, t> -> omega
+        if (sst.early) {
+            const double alpha = sst.alpha;
+            for (std::int64_t l = tid; l < n; l += NT) {
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    V.x[i] = __builtin_fma(alpha, V.phat[i], V.x[i]);
+                }
+            }
+            if (tid == 0) stage_skip<fin_rho>(sst);  // fin_omega skips with no change
+            __syncthreads();
+        } else {
+            double red[2] = {0.0, 0.0};
+            for (std::int64_t l = tid; l < n; l += NT) {
+                double y[BS];
+                row_spmv<BS, kU>(P, p0 + l, V.shat, y);
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    V.t[i] = y[c];
+                    red[0] = __builtin_fma(y[c], V.s[i], red[0]);
+                    red[1] = __builtin_fma(y[c], y[c], red[1]);
+                }
+            }
+            const Sums sm = group_sums<2, NT>(red, lds);
+            if (tid == 0) finish_stage<fin_omega>(sst, sm);
+            __syncthreads();
+            if (sst.phase != 1) break;
+            // x += alpha phat + omega shat; r = s - omega t; <r^, r>, |r|^2
+            const double alpha = sst.alpha, omega = sst.omega;
+            double red2[2] = {0.0, 0.0};
+            for (std::int64_t l = tid; l < n; l += NT) {
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    V.x[i] = __builtin_fma(omega, V.shat[i], __builtin_fma(alpha, V.phat[i], V.x[i]));
+                    const double ri = V.s[i] - omega * V.t[i];
+                    V.r[i] = ri;
+                    red2[0] = __builtin_fma(V.rhat[i], ri, red2[0]);
+                    red2[1] = __builtin_fma(ri, ri, red2[1]);
+                }
+            }
+            const Sums sm2 = group_sums<2, NT>(red2, lds);
+            if (tid == 0) finish_stage<fin_rho>(sst, sm2);
+            __syncthreads();
+        }
+        // true residual of the check, replacement or final iteration
+        if (sst.phase == 1 && sst.need_true) {
+            double red[1] = {0.0};
+            for (std::int64_t l = tid; l < n; l += NT) {
+                double ax[BS];
+                row_spmv<BS, kU>(P, p0 + l, V.x, ax);
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const std::int64_t i = (v0 + l) * BS + c;
+                    const double ri = V.b[i] - ax[c];
+                    V.r[i] = ri;
+                    red[0] = __builtin_fma(ri, ri, red[0]);
+                }
+            }
+            const Sums sm = group_sums<1, NT>(red, lds);
+            if (tid == 0) finish_stage<fin_true>(sst, sm);
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        P.st[sys] = sst;
+        stop_clock[sys] = wall_clock64();
+    }
 }
 
 // ------------------------------------------------------------------ host --
@@ -604,6 +865,7 @@ std::unique_ptr<Tiling> make_tiling(owtk_batch bt, int tile)
     std::vector<std::int64_t> brow;
     for (int s = 0; s < bt->systems; ++s) {
         sb0[s] = static_cast<std::int32_t>(bsys.size());
+        if (bt->h_owner[s] != 0) continue;  // solved by one workgroup
         const std::int64_t n = bt->h_vprefix[s + 1] - bt->h_vprefix[s];
         for (std::int64_t r0 = 0; r0 < n; r0 += tile) {
             bsys.push_back(s);
@@ -611,16 +873,16 @@ std::unique_ptr<Tiling> make_tiling(owtk_batch bt, int tile)
         }
     }
     sb0[bt->systems] = static_cast<std::int32_t>(bsys.size());
-    if (bsys.empty()) throw std::invalid_argument("empty batch");
     if (bsys.size() > 0x7fffffffULL) throw std::invalid_argument("batch too large");
     tl->blocks = static_cast<std::int64_t>(bsys.size());
+    tl->sys_blk0.allocate(sb0.size());
+    OWTK_HIP(hipMemcpy(tl->sys_blk0.get(), sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
+    if (bsys.empty()) return tl;  // every system is solved by a workgroup
     tl->blk_sys.allocate(bsys.size());
     tl->blk_row0.allocate(brow.size());
-    tl->sys_blk0.allocate(sb0.size());
     tl->partials.allocate(kPartials * bsys.size());
     OWTK_HIP(hipMemcpy(tl->blk_sys.get(), bsys.data(), bsys.size() * 4, hipMemcpyHostToDevice));
     OWTK_HIP(hipMemcpy(tl->blk_row0.get(), brow.data(), brow.size() * 8, hipMemcpyHostToDevice));
-    OWTK_HIP(hipMemcpy(tl->sys_blk0.get(), sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
     return tl;
 }
 
@@ -722,6 +984,7 @@ void set_options(owtk_batch bt, const owtk_solver_options* o)
         s.max_iterations = static_cast<int>(std::min<std::uint64_t>(o->maximum_iterations, 1u << 30));
         s.status = status_invalid_input;
     }
+    for (int k = 0; k < bt->systems; ++k) h[k].owner = bt->h_owner[k];
     OWTK_HIP(hipMemcpyAsync(bt->state.get(), h.data(), h.size() * sizeof(SysState), hipMemcpyHostToDevice,
                             bt->ctx->stream));
     OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));  // h is a host temporary
@@ -738,19 +1001,62 @@ int read_active(owtk_batch bt)
     return n;
 }
 
-// One batched solve; times[s] (may be null): ms from start to the end of the
-// graph replay in which system s stopped.
+template<int BS, int NT>
+void launch_group_bs(owtk_batch bt, const Ptrs& P, const GroupVectors& V)
+{
+    kb_solve_group<BS, NT><<<static_cast<unsigned int>(bt->group_systems), NT, 0, bt->side>>>(
+        P, bt->group_order.get(), V, bt->clocks.get());
+}
+
+template<int NT>
+void launch_group_nt(owtk_batch bt, const Ptrs& P, const GroupVectors& V)
+{
+    switch (bt->bs) {
+    case 1: launch_group_bs<1, NT>(bt, P, V); break;
+    case 2: launch_group_bs<2, NT>(bt, P, V); break;
+    case 3: launch_group_bs<3, NT>(bt, P, V); break;
+    default: launch_group_bs<4, NT>(bt, P, V); break;
+    }
+}
+
+// The workgroup solves, on the side stream after everything queued so far
+// on the context's stream; stamps the start time first.
+void launch_groups(owtk_batch bt, const double* b, double* x)
+{
+    const hipStream_t st = bt->ctx->stream;
+    kb_stamp<<<1, 1, 0, st>>>(bt->clocks.get() + bt->systems);
+    OWTK_HIP(hipEventRecord(bt->fork, st));
+    OWTK_HIP(hipStreamWaitEvent(bt->side, bt->fork, 0));
+    Ptrs P = make_ptrs(bt, *bt->tiling);
+    const GroupVectors V{b, x, bt->r.get(), bt->rhat.get(), bt->p.get(), bt->v.get(), bt->s.get(), bt->t.get(),
+                         bt->phat.get(), bt->shat.get()};
+    switch (bt->group_threads) {
+    case 256: launch_group_nt<256>(bt, P, V); break;
+    case 512: launch_group_nt<512>(bt, P, V); break;
+    default: launch_group_nt<1024>(bt, P, V); break;
+    }
+    OWTK_LAUNCH_CHECK();
+    OWTK_HIP(hipEventRecord(bt->join, bt->side));
+}
+
+// One batched solve; stop_ms[s] (may be null): ms from the start to the end
+// of the graph replay in which system s stopped, or, for a system solved by a
+// workgroup, to the device clock's reading when it stopped.
 void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options* o, int g,
            std::vector<double>* stop_ms, std::vector<int>* stop_iters)
 {
     Tiling& tl = *bt->tiling;
     set_options(bt, o);
+    const bool graph = bt->graph_systems > 0, groups = bt->group_systems > 0;
+    if (graph) ensure_graph(bt, tl, b, x, g);
     const auto t0 = std::chrono::steady_clock::now();
-    init(bt, tl, b, x);
-    ensure_graph(bt, tl, b, x, g);
+    if (groups) launch_groups(bt, b, x);
     std::vector<SysState> h(bt->systems);
     if (stop_ms) stop_ms->assign(bt->systems, 0.0);
     std::vector<char> done(bt->systems, 0);
+    // Reads every state; counts the running graph-path systems (the
+    // workgroup systems' entries may be mid-update and are read again after
+    // the join).
     auto snapshot = [&]() {
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         OWTK_HIP(hipMemcpyAsync(h.data(), bt->state.get(), h.size() * sizeof(SysState), hipMemcpyDeviceToHost,
@@ -758,6 +1064,7 @@ void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options*
         OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
         int active = 0;
         for (int s = 0; s < bt->systems; ++s) {
+            if (bt->h_owner[s] != 0) continue;
             if (h[s].phase == 1) {
                 ++active;
             } else if (!done[s]) {
@@ -767,7 +1074,11 @@ void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options*
         }
         return active;
     };
-    int active = stop_ms ? snapshot() : read_active(bt);
+    int active = 0;
+    if (graph) {
+        init(bt, tl, b, x);
+        active = stop_ms ? snapshot() : read_active(bt);
+    }
     const std::uint64_t cap = o->maximum_iterations + 2;
     std::uint64_t replays = 0;
     while (active > 0 && replays * static_cast<std::uint64_t>(g) <= cap) {
@@ -775,12 +1086,64 @@ void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options*
         ++replays;
         active = stop_ms ? snapshot() : read_active(bt);
     }
+    if (groups) {
+        OWTK_HIP(hipStreamWaitEvent(bt->ctx->stream, bt->join, 0));
+        OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
+    }
     if (active > 0) throw std::runtime_error("batched BiCGSTAB did not stop within its iteration limit");
+    if (groups || stop_iters) {
+        OWTK_HIP(hipMemcpy(h.data(), bt->state.get(), h.size() * sizeof(SysState), hipMemcpyDeviceToHost));
+        for (int s = 0; s < bt->systems; ++s)
+            if (bt->h_owner[s] != 0 && h[s].phase == 1)
+                throw std::runtime_error("batched BiCGSTAB did not stop within its iteration limit");
+    }
+    if (groups && stop_ms) {
+        std::vector<unsigned long long> clk(bt->systems + 1);
+        OWTK_HIP(hipMemcpy(clk.data(), bt->clocks.get(), clk.size() * sizeof(clk[0]), hipMemcpyDeviceToHost));
+        int khz = 0;
+        OWTK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, bt->ctx->device));
+        if (khz <= 0) throw std::runtime_error("device wall clock rate unavailable");
+        for (int s = 0; s < bt->systems; ++s)
+            if (bt->h_owner[s] != 0)
+                (*stop_ms)[s] = static_cast<double>(clk[s] - clk[bt->systems]) / khz;
+    }
     if (stop_iters) {
-        if (!stop_ms) snapshot();
         stop_iters->resize(bt->systems);
         for (int s = 0; s < bt->systems; ++s) (*stop_iters)[s] = h[s].iters;
     }
+}
+
+// Partition the systems at `limit` vertices (0: all on the graph path) and
+// rebuild the tiling of the graph systems.
+void set_groups(owtk_batch bt, std::int64_t limit, int threads)
+{
+    bt->ctx->sync();
+    std::vector<int> owner(bt->systems, 0);
+    std::vector<std::int32_t> order;
+    for (int s = 0; s < bt->systems; ++s) {
+        if (bt->h_vprefix[s + 1] - bt->h_vprefix[s] <= limit) {
+            owner[s] = 1;
+            order.push_back(s);
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [&](std::int32_t a, std::int32_t c) {
+        return bt->h_vprefix[a + 1] - bt->h_vprefix[a] > bt->h_vprefix[c + 1] - bt->h_vprefix[c];
+    });
+    bt->h_owner = owner;
+    bt->group_systems = static_cast<int>(order.size());
+    bt->graph_systems = bt->systems - bt->group_systems;
+    bt->group_limit = limit;
+    bt->group_threads = threads;
+    if (!order.empty()) {
+        bt->group_order.allocate(order.size());
+        OWTK_HIP(hipMemcpy(bt->group_order.get(), order.data(), order.size() * 4, hipMemcpyHostToDevice));
+        if (!bt->side) {
+            OWTK_HIP(hipStreamCreateWithFlags(&bt->side, hipStreamNonBlocking));
+            OWTK_HIP(hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming));
+            OWTK_HIP(hipEventCreateWithFlags(&bt->join, hipEventDisableTiming));
+        }
+    }
+    bt->tiling = make_tiling(bt, bt->tile);
 }
 
 } // namespace
@@ -885,6 +1248,9 @@ int owtk_batch_create(owtk_context ctx, int32_t systems, int32_t block_size, con
         }
         bt->state.allocate(systems);
         bt->active_count.allocate(1);
+        bt->clocks.allocate(static_cast<std::size_t>(systems) + 1);
+        bt->h_owner.assign(systems, 0);
+        bt->graph_systems = systems;
         bt->tiling = make_tiling(bt.get(), bt->tile);
         ctx->sync();
         *out = bt.release();
@@ -931,6 +1297,11 @@ int owtk_batch_autotune(owtk_batch bt, const double* b, int32_t iterations, int3
         o.maximum_iterations = 1u << 30;
         int best = bt->tile;
         double best_ms = std::numeric_limits<double>::infinity();
+        if (bt->graph_systems == 0) {  // no system on the graph path: nothing to tune
+            if (tile_rows) *tile_rows = best;
+            if (ms_per_iteration) *ms_per_iteration = 0.0;
+            return;
+        }
         for (int tile : {64, 128, 256, 512}) {
             owtk_batch_set_tile(bt, tile);
             Tiling& tl = *bt->tiling;
@@ -956,6 +1327,69 @@ int owtk_batch_autotune(owtk_batch bt, const double* b, int32_t iterations, int3
         bt->tiling->graph_b = nullptr;  // the trial graphs bound the scratch x
         if (tile_rows) *tile_rows = best;
         if (ms_per_iteration) *ms_per_iteration = best_ms;
+    });
+}
+
+int owtk_batch_set_group_limit(owtk_batch bt, int64_t max_vertices, int32_t threads)
+{
+    return guarded([&] {
+        if (!bt) throw std::invalid_argument("null batch");
+        if (max_vertices < 0) throw std::invalid_argument("group limit must be >= 0");
+        if (threads != 256 && threads != 512 && threads != 1024)
+            throw std::invalid_argument("group threads must be 256, 512 or 1024");
+        bt->ctx->activate();
+        set_groups(bt, max_vertices, threads);
+    });
+}
+
+int owtk_batch_autotune_groups(owtk_batch bt, const double* b, const owtk_solver_options* options,
+                               int32_t graph_iterations, int64_t* max_vertices, int32_t* threads, double* ms)
+{
+    return guarded([&] {
+        if (!bt || !b || !options) throw std::invalid_argument("null argument");
+        if (graph_iterations < 1 || graph_iterations > 1024) throw std::invalid_argument("graph_iterations must be 1..1024");
+        if (!valid_options(options)) throw std::invalid_argument("invalid solver options");
+        require_plain_options(options, "batched bicgstab");
+        bt->ctx->activate();
+        DeviceArray<double> x(static_cast<std::size_t>(bt->vertices * bt->bs));
+        auto trial = [&](std::int64_t limit, int nt) {
+            set_groups(bt, limit, nt);
+            double best = std::numeric_limits<double>::infinity();
+            for (int k = 0; k < 2; ++k) {
+                kern::fill_zero_bytes(bt->ctx->stream, x.get(), x.bytes());
+                OWTK_HIP(hipStreamSynchronize(bt->ctx->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                solve(bt, b, x.get(), options, graph_iterations, nullptr, nullptr);
+                best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            }
+            return best;
+        };
+        std::int64_t largest = 0;
+        for (int s = 0; s < bt->systems; ++s) largest = std::max(largest, bt->h_vprefix[s + 1] - bt->h_vprefix[s]);
+        std::int64_t best_limit = 0;
+        int best_threads = 1024;
+        double best_ms = trial(0, 1024);
+        for (std::int64_t limit = 1024; limit <= std::min<std::int64_t>(largest, 65536); limit *= 2) {
+            const double t = trial(limit, 1024);
+            if (t < best_ms) {
+                best_ms = t;
+                best_limit = limit;
+            }
+        }
+        if (best_limit > 0) {
+            for (int nt : {256, 512}) {
+                const double t = trial(best_limit, nt);
+                if (t < best_ms) {
+                    best_ms = t;
+                    best_threads = nt;
+                }
+            }
+        }
+        set_groups(bt, best_limit, best_threads);
+        bt->tiling->graph_b = nullptr;  // the trial graphs bound the scratch x
+        if (max_vertices) *max_vertices = best_limit;
+        if (threads) *threads = best_threads;
+        if (ms) *ms = best_ms;
     });
 }
 

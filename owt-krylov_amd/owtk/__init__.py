@@ -450,6 +450,20 @@ class Batch:
         check(lib().owtk_batch_autotune(self.handle, b.ptr, iterations, C.byref(tile), C.byref(ms)))
         return tile.value, ms.value
 
+    def set_group_limit(self, max_vertices: int, threads: int = 512) -> None:
+        """Systems of at most max_vertices vertices are solved whole by one
+        workgroup each, beside the graph replays of the others (0: none)."""
+        check(lib().owtk_batch_set_group_limit(self.handle, max_vertices, threads))
+
+    def autotune_groups(self, b: DeviceBuffer, options=None, graph_iterations: int = 8):
+        """Times whole solves over group limits and workgroup sizes and keeps
+        the fastest; returns (max_vertices, threads, ms)."""
+        opts = options if options is not None else SolverOptions()
+        limit, threads, ms = C.c_int64(), C.c_int32(), C.c_double()
+        check(lib().owtk_batch_autotune_groups(self.handle, b.ptr, C.byref(opts), graph_iterations, C.byref(limit),
+                                               C.byref(threads), C.byref(ms)))
+        return limit.value, threads.value, ms.value
+
     def bicgstab(self, b: DeviceBuffer, x: DeviceBuffer, options=None, graph_iterations: int = 8) -> list[dict]:
         opts = options if options is not None else SolverOptions()
         res = (_lib.BatchResult * self.systems)()

@@ -221,6 +221,9 @@ PROTOTYPES: dict[str, tuple] = {
     "owtk_batch_destroy": (C.c_int, [_P]),
     "owtk_batch_set_tile": (C.c_int, [_P, C.c_int32]),
     "owtk_batch_autotune": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), _DP]),
+    "owtk_batch_set_group_limit": (C.c_int, [_P, C.c_int64, C.c_int32]),
+    "owtk_batch_autotune_groups": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                             _DP]),
     "owtk_batch_bicgstab": (C.c_int, [_P, _P, _P, C.POINTER(SolverOptions), C.c_int32, C.POINTER(BatchResult)]),
     "owtk_read_block_csr": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64), _P, _P, _P]),

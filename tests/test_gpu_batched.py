@@ -119,3 +119,97 @@ def test_autotune_picks_a_valid_tile(ctx, small_batch):
     x = ctx.empty(rhs.size)
     results = bt.bicgstab(b, x, owtk.SolverOptions(relative_tolerance=1e-8))
     assert all(r["status_name"] == "converged" for r in results)
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_group_solves_match_oracle(ctx, small_batch, threads):
+    """Every system solved whole by one workgroup (limit above the largest
+    system): same recurrence and stages as the graph path, so each count stays
+    inside the oracle's rounding envelope and the solutions match the
+    single-system solver's."""
+    systems, rhs, grids = small_batch
+    bt = owtk.Batch(ctx, 3, systems)
+    bt.set_group_limit(1 << 20, threads)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    opts = owtk.SolverOptions(relative_tolerance=1e-8)
+    results = bt.bicgstab(b, x, opts, graph_iterations=4)
+    xb = x.download()
+    off = 0
+    for k, (sysk, res) in enumerate(zip(systems, results)):
+        n = sysk[0] * 3
+        assert res["status_name"] == "converged", (k, res)
+        assert res["relative_residual_norm"] <= 1e-8
+        assert res["stop_ms"] > 0.0
+        lo, hi = oracle_envelope(sysk, rhs[off:off + n])
+        assert lo - 3 <= res["iterations"] <= hi + 3, (k, res["iterations"], lo, hi)
+        single, xs = single_solve(ctx, sysk, rhs[off:off + n], opts)
+        if grids[k][2] < 0.6:
+            assert abs(res["iterations"] - single["iterations"]) <= 1, (k, res["iterations"], single["iterations"])
+        assert np.linalg.norm(xb[off:off + n] - xs) / np.linalg.norm(xs) < 1e-6, k
+        # the true residual, recomputed on the host
+        nn, o, c, v = sysk
+        om = orc.OMatrix.csr(nn, 0, 3, o, c, v)
+        r = rhs[off:off + n] - om.spmv(xb[off:off + n])[:n]
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(rhs[off:off + n]), k
+        off += n
+    print(f"group {threads}: iterations", [r["iterations"] for r in results])
+
+
+def test_split_between_groups_and_graph(ctx, small_batch):
+    """Half the systems on workgroups, half on the graph replays, at the same
+    time: every system converges, and each path gives the iteration counts it
+    gives alone."""
+    systems, rhs, grids = small_batch
+    sizes = sorted(s[0] for s in systems)
+    limit = sizes[len(sizes) // 2]
+    opts = owtk.SolverOptions(relative_tolerance=1e-8)
+    b = ctx.upload(rhs)
+    runs = {}
+    for name, lim in (("graph", 0), ("groups", 1 << 20), ("split", limit)):
+        bt = owtk.Batch(ctx, 3, systems)
+        bt.set_group_limit(lim, 512)
+        x = ctx.empty(rhs.size)
+        runs[name] = (bt.bicgstab(b, x, opts, graph_iterations=4), x.download())
+    for k, sysk in enumerate(systems):
+        res, _ = runs["split"]
+        ref, _ = runs["groups" if sysk[0] <= limit else "graph"]
+        assert res[k]["status_name"] == "converged", (k, res[k])
+        assert res[k]["iterations"] == ref[k]["iterations"], (k, res[k]["iterations"], ref[k]["iterations"])
+        assert res[k]["true_residual_norm"] == ref[k]["true_residual_norm"], k
+    assert np.array_equal(runs["split"][1], np.where(
+        np.repeat([s[0] <= limit for s in systems], [s[0] * 3 for s in systems]), runs["groups"][1], runs["graph"][1]))
+
+
+def test_group_masks_zero_rhs_and_iteration_cap(ctx, small_batch):
+    """The stages of the graph path on a workgroup: b = 0 stops at iteration 0
+    with x untouched; a cap of 5 stops the rest after exactly 5."""
+    systems, rhs, grids = small_batch
+    rhs = rhs.copy()
+    n0 = systems[0][0] * 3
+    rhs[:n0] = 0.0
+    bt = owtk.Batch(ctx, 3, systems)
+    bt.set_group_limit(1 << 20, 256)
+    b = ctx.upload(rhs)
+    x = ctx.empty(rhs.size)
+    results = bt.bicgstab(b, x, owtk.SolverOptions(relative_tolerance=1e-8, maximum_iterations=5))
+    assert results[0]["status_name"] == "converged" and results[0]["iterations"] == 0
+    assert np.all(x.download()[:n0] == 0.0)
+    for r in results[1:]:
+        assert r["status_name"] in ("maximum_iterations", "converged")
+        if r["status_name"] == "maximum_iterations":
+            assert r["iterations"] == 5
+
+
+def test_group_autotune(ctx, small_batch):
+    systems, rhs, grids = small_batch
+    bt = owtk.Batch(ctx, 3, systems)
+    b = ctx.upload(rhs)
+    opts = owtk.SolverOptions(relative_tolerance=1e-8)
+    limit, threads, ms = bt.autotune_groups(b, opts, graph_iterations=4)
+    assert limit == 0 or limit >= 1024
+    assert threads in (256, 512, 1024) and ms > 0
+    x = ctx.empty(rhs.size)
+    results = bt.bicgstab(b, x, opts)
+    assert all(r["status_name"] == "converged" for r in results)
+    print("group limit", limit, "threads", threads, f"{ms:.2f} ms")
