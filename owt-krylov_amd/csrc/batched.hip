@@ -12,14 +12,16 @@
 // never straddles two systems either: tile -> system is a table, so a block
 // reads its system's mask first and returns at once once that system stopped.
 //
-// Iteration.  Eleven launches: 5 streaming kernels (update p, v = A M p with
+// Iteration.  Twelve launches: 5 streaming kernels (update p, v = A M p with
 // <r^,v>, s = r - alpha v with |s|^2, t = A M s with <t,s> and <t,t>, update
 // x and r with <r^,r> and |r|^2) each followed by a per-system finisher (one
 // wave per system summing that system's tile partials in a fixed order:
 // deterministic), plus the masked true-residual kernel of the convergence and
-// replacement checks.  G iterations are captured once into a hipGraph and the
-// graph is replayed until every system has stopped; the host reads one
-// counter per replay.  The tile size is chosen by owtk_batch_autotune.
+// replacement checks, and kb_live, which lists the tiles of the systems still
+// running for the next iteration's kernels (their blocks past the list return
+// at once).  G iterations are captured once into a hipGraph and the graph is
+// replayed until every system has stopped; the host reads one counter per
+// replay.  The tile size is chosen by owtk_batch_autotune.
 //
 // Small systems.  Systems of at most a set number of vertices can instead be
 // solved whole by one workgroup each (kb_solve_group): one launch on a second
@@ -63,13 +65,26 @@ struct SysState {
     int owner;        // 0: graph path; 1: one workgroup (kb_solve_group)
 };
 
+// A tile of the graph path: its system, first padded row, the offset from
+// padded row to vertex (vertex = prow + vshift) and its number of real rows.
+// One 16-byte load.
+struct alignas(16) TileRef {
+    std::int32_t sys, prow0, vshift, valid;
+};
+
 struct Tiling {
     int tile = 0;
     std::int64_t blocks = 0;
-    DeviceArray<std::int32_t> blk_sys;     // tile -> system
-    DeviceArray<std::int64_t> blk_row0;    // tile -> first padded row
+    DeviceArray<TileRef> refs;             // tile -> system, rows
     DeviceArray<std::int32_t> sys_blk0;    // system -> first tile (size systems + 1)
     DeviceArray<double> partials;          // [kPartials][blocks]
+    DeviceArray<TileRef> live;             // [blocks] tiles of the running systems (kb_live)
+    DeviceArray<std::int32_t> live_count;
+    DeviceArray<std::int32_t> live_first;  // [systems] first entry of each running system in `live`
+    int ranks = 0;                         // graph-path systems
+    DeviceArray<std::int32_t> blk_rank;    // tile -> rank of its system among the graph-path systems
+    DeviceArray<std::int32_t> rank_sys;    // rank -> system
+    DeviceArray<std::int32_t> rank_blk0;   // rank -> first tile (size ranks + 1)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_iterations = 0;
@@ -124,8 +139,6 @@ struct owtk_batch_s {
 namespace {
 
 struct Ptrs {
-    const std::int32_t* blk_sys;
-    const std::int64_t* blk_row0;
     const std::int64_t* vprefix;
     const std::int64_t* pprefix;
     const std::int32_t* slice_ptr;
@@ -135,26 +148,28 @@ struct Ptrs {
     const double* dinv;
     SysState* st;
     double* partials;
+    const TileRef* refs;             // every graph-path tile, in tile order
+    const TileRef* live;             // the running systems' tiles, in tile order
+    const std::int32_t* live_count;
+    const std::int32_t* live_first;  // system -> its first entry in `live`
     std::int64_t blocks;
     int bs;
 };
 
 // Row of this thread: system, unpadded vertex (or -1 past the system's end)
-// and padded row.  Returns false when the system has stopped.
+// and padded row.
 struct RowRef {
     int sys;
     std::int64_t vertex;  // -1: padding
     std::int64_t prow;
 };
 
-__device__ inline RowRef row_of(const Ptrs& P)
+__device__ inline RowRef row_of(const TileRef& T)
 {
     RowRef rr;
-    rr.sys = P.blk_sys[blockIdx.x];
-    rr.prow = P.blk_row0[blockIdx.x] + threadIdx.x;
-    const std::int64_t local = rr.prow - P.pprefix[rr.sys];
-    const std::int64_t n = P.vprefix[rr.sys + 1] - P.vprefix[rr.sys];
-    rr.vertex = local < n ? P.vprefix[rr.sys] + local : -1;
+    rr.sys = T.sys;
+    rr.prow = static_cast<std::int64_t>(T.prow0) + threadIdx.x;
+    rr.vertex = static_cast<int>(threadIdx.x) < T.valid ? rr.prow + T.vshift : -1;
     return rr;
 }
 
@@ -203,9 +218,11 @@ __device__ inline void row_spmv(const Ptrs& P, std::int64_t prow, const double* 
     }
 }
 
-// Block sum of K values; thread 0 writes partials[k][blockIdx.x].
+// Block sum of K values; thread 0 writes partials[k][slot].  An iteration
+// kernel's slot is its block's place in the running systems' tile list, the
+// initial kernel's its tile.
 template<int K>
-__device__ inline void store_tile_partials(double (&v)[K], const Ptrs& P)
+__device__ inline void store_tile_partials(double (&v)[K], const Ptrs& P, int slot)
 {
     __shared__ double lds[kMaxTile / 64][K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -222,12 +239,24 @@ __device__ inline void store_tile_partials(double (&v)[K], const Ptrs& P)
         for (int k = 0; k < K; ++k) {
             double acc = lds[0][k];
             for (int w = 1; w < waves; ++w) acc += lds[w][k];
-            P.partials[k * P.blocks + blockIdx.x] = acc;
+            P.partials[k * P.blocks + slot] = acc;
         }
     }
 }
 
 // ------------------------------------------------------ streaming kernels --
+// The iteration kernels' block q takes entry q of the running systems' tile
+// list (blocks past its end return at once).  Each loads its system's state
+// beside its first data loads, not before them, and tests the stop flag once
+// the data are in registers: the empty asm statements keep the compiler from
+// sinking the loads below that test, which would put a second round trip on
+// every row.
+
+template<typename T>
+__device__ inline void keep(const T& v)
+{
+    asm volatile("" ::"v"(v));
+}
 
 // Initial residual: r = b - A x, r^ = r, p = v = 0; partials |r|^2, |b|^2.
 template<int BS>
@@ -235,7 +264,8 @@ __global__ __launch_bounds__(kMaxTile) void kb_init(Ptrs P, const double* __rest
                                                     double* __restrict__ r, double* __restrict__ rhat,
                                                     double* __restrict__ p, double* __restrict__ v)
 {
-    const RowRef rr = row_of(P);
+    const int tile = static_cast<int>(blockIdx.x);
+    const RowRef rr = row_of(P.refs[tile]);
     double red[2] = {0.0, 0.0};
     if (rr.vertex >= 0) {
         double ax[BS];
@@ -252,7 +282,7 @@ __global__ __launch_bounds__(kMaxTile) void kb_init(Ptrs P, const double* __rest
             red[1] = __builtin_fma(b[i], b[i], red[1]);
         }
     }
-    store_tile_partials<2>(red, P);
+    store_tile_partials<2>(red, P, tile);
 }
 
 // p = r (first) or r + beta (p - omega v); r^ = r on restart; phat = M p.
@@ -261,16 +291,17 @@ __global__ __launch_bounds__(kMaxTile) void kb_update_p(Ptrs P, const double* __
                                                         double* __restrict__ p, const double* __restrict__ v,
                                                         double* __restrict__ phat)
 {
-    const RowRef rr = row_of(P);
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
+    if (rr.vertex < 0) return;
     const SysState& st = P.st[rr.sys];
-    if (st.phase != 1 || rr.vertex < 0) return;
+    const int phase = st.phase;
     const bool first = st.first != 0;
     const bool restart = st.restart != 0;
     const double beta = st.beta, omega = st.omega;
-    // All four inputs load before any branch (on a first iteration, after the
-    // initial kernel or a restart, the p and v values loaded are discarded by
-    // the select), so the row's loads overlap instead of waiting for each
-    // other around the restart/first branches.
+    // On a first iteration (after the initial kernel or a restart) the p and
+    // v values loaded are discarded by the select.
     const std::int64_t i0 = rr.vertex * BS;
     double rv[BS], pv[BS], vv[BS], dv[BS];
 #pragma unroll
@@ -280,10 +311,14 @@ __global__ __launch_bounds__(kMaxTile) void kb_update_p(Ptrs P, const double* __
         vv[c] = v[i0 + c];
         dv[c] = P.dinv[i0 + c];
     }
-    // Without this use the compiler sinks part of the p and v loads into the
-    // uniform not-first branch, a second round trip per row.
 #pragma unroll
-    for (int c = 0; c < BS; ++c) asm volatile("" ::"v"(pv[c]), "v"(vv[c]));
+    for (int c = 0; c < BS; ++c) {
+        keep(rv[c]);
+        keep(pv[c]);
+        keep(vv[c]);
+        keep(dv[c]);
+    }
+    if (phase != 1) return;
     double pn[BS], ph[BS];
 #pragma unroll
     for (int c = 0; c < BS; ++c) {
@@ -307,20 +342,31 @@ template<int BS>
 __global__ __launch_bounds__(kMaxTile) void kb_spmv_v(Ptrs P, const double* __restrict__ phat,
                                                       const double* __restrict__ rhat, double* __restrict__ v)
 {
-    const RowRef rr = row_of(P);
-    if (P.st[rr.sys].phase != 1) return;
-    double red[1] = {0.0};
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
+    const int phase = P.st[rr.sys].phase;
+    double y[BS], rh[BS];
     if (rr.vertex >= 0) {
-        double y[BS];
         row_spmv<BS>(P, rr.prow, phat, y);
 #pragma unroll
+        for (int c = 0; c < BS; ++c) rh[c] = rhat[rr.vertex * BS + c];
+#pragma unroll
         for (int c = 0; c < BS; ++c) {
-            const std::int64_t i = rr.vertex * BS + c;
-            v[i] = y[c];
-            red[0] = __builtin_fma(rhat[i], y[c], red[0]);
+            keep(y[c]);
+            keep(rh[c]);
         }
     }
-    store_tile_partials<1>(red, P);
+    if (phase != 1) return;
+    double red[1] = {0.0};
+    if (rr.vertex >= 0) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            v[rr.vertex * BS + c] = y[c];
+            red[0] = __builtin_fma(rh[c], y[c], red[0]);
+        }
+    }
+    store_tile_partials<1>(red, P, q);
 }
 
 // s = r - alpha v; shat = M s; partial |s|^2
@@ -328,22 +374,41 @@ template<int BS>
 __global__ __launch_bounds__(kMaxTile) void kb_update_s(Ptrs P, const double* __restrict__ r, const double* __restrict__ v,
                                                         double* __restrict__ s, double* __restrict__ shat)
 {
-    const RowRef rr = row_of(P);
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
     const SysState& st = P.st[rr.sys];
-    if (st.phase != 1) return;
+    const int phase = st.phase;
     const double alpha = st.alpha;
+    double rv[BS], vv[BS], dv[BS];
+    if (rr.vertex >= 0) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            rv[c] = r[i];
+            vv[c] = v[i];
+            dv[c] = P.dinv[i];
+        }
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            keep(rv[c]);
+            keep(vv[c]);
+            keep(dv[c]);
+        }
+    }
+    if (phase != 1) return;
     double red[1] = {0.0};
     if (rr.vertex >= 0) {
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const std::int64_t i = rr.vertex * BS + c;
-            const double si = r[i] - alpha * v[i];
+            const double si = rv[c] - alpha * vv[c];
             s[i] = si;
-            shat[i] = P.dinv[i] * si;
+            shat[i] = dv[c] * si;
             red[0] = __builtin_fma(si, si, red[0]);
         }
     }
-    store_tile_partials<1>(red, P);
+    store_tile_partials<1>(red, P, q);
 }
 
 // t = A shat; partials <t, s>, <t, t>.  Early exit: x += alpha phat only.
@@ -352,32 +417,46 @@ __global__ __launch_bounds__(kMaxTile) void kb_spmv_t(Ptrs P, const double* __re
                                                       const double* __restrict__ s, double* __restrict__ t,
                                                       const double* __restrict__ phat, double* __restrict__ x)
 {
-    const RowRef rr = row_of(P);
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
     const SysState& st = P.st[rr.sys];
-    if (st.phase != 1) return;
-    if (st.early) {
+    const int phase = st.phase, early = st.early;
+    const double alpha = st.alpha;
+    // The product is formed before the flags are known: an early exit (at
+    // most once per system) discards it.
+    double y[BS], sv[BS];
+    if (rr.vertex >= 0) {
+        row_spmv<BS>(P, rr.prow, shat, y);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) sv[c] = s[rr.vertex * BS + c];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            keep(y[c]);
+            keep(sv[c]);
+        }
+    }
+    if (phase != 1) return;
+    if (early) {
         if (rr.vertex >= 0) {
 #pragma unroll
             for (int c = 0; c < BS; ++c) {
                 const std::int64_t i = rr.vertex * BS + c;
-                x[i] = __builtin_fma(st.alpha, phat[i], x[i]);
+                x[i] = __builtin_fma(alpha, phat[i], x[i]);
             }
         }
         return;
     }
     double red[2] = {0.0, 0.0};
     if (rr.vertex >= 0) {
-        double y[BS];
-        row_spmv<BS>(P, rr.prow, shat, y);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
-            const std::int64_t i = rr.vertex * BS + c;
-            t[i] = y[c];
-            red[0] = __builtin_fma(y[c], s[i], red[0]);
+            t[rr.vertex * BS + c] = y[c];
+            red[0] = __builtin_fma(y[c], sv[c], red[0]);
             red[1] = __builtin_fma(y[c], y[c], red[1]);
         }
     }
-    store_tile_partials<2>(red, P);
+    store_tile_partials<2>(red, P, q);
 }
 
 // x += alpha phat + omega shat; r = s - omega t; partials <r^, r>, |r|^2
@@ -387,31 +466,59 @@ __global__ __launch_bounds__(kMaxTile) void kb_update_x(Ptrs P, const double* __
                                                         const double* __restrict__ t, const double* __restrict__ rhat,
                                                         double* __restrict__ x, double* __restrict__ r)
 {
-    const RowRef rr = row_of(P);
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
     const SysState& st = P.st[rr.sys];
-    if (st.phase != 1 || st.early) return;
+    const int phase = st.phase, early = st.early;
     const double alpha = st.alpha, omega = st.omega;
+    double ph[BS], sh[BS], sv[BS], tv[BS], rh[BS], xv[BS];
+    if (rr.vertex >= 0) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const std::int64_t i = rr.vertex * BS + c;
+            ph[c] = phat[i];
+            sh[c] = shat[i];
+            sv[c] = s[i];
+            tv[c] = t[i];
+            rh[c] = rhat[i];
+            xv[c] = x[i];
+        }
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            keep(ph[c]);
+            keep(sh[c]);
+            keep(sv[c]);
+            keep(tv[c]);
+            keep(rh[c]);
+            keep(xv[c]);
+        }
+    }
+    if (phase != 1 || early) return;
     double red[2] = {0.0, 0.0};
     if (rr.vertex >= 0) {
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const std::int64_t i = rr.vertex * BS + c;
-            x[i] = __builtin_fma(omega, shat[i], __builtin_fma(alpha, phat[i], x[i]));
-            const double ri = s[i] - omega * t[i];
+            x[i] = __builtin_fma(omega, sh[c], __builtin_fma(alpha, ph[c], xv[c]));
+            const double ri = sv[c] - omega * tv[c];
             r[i] = ri;
-            red[0] = __builtin_fma(rhat[i], ri, red[0]);
+            red[0] = __builtin_fma(rh[c], ri, red[0]);
             red[1] = __builtin_fma(ri, ri, red[1]);
         }
     }
-    store_tile_partials<2>(red, P);
+    store_tile_partials<2>(red, P, q);
 }
 
 // True residual of the systems that asked for it: r = b - A x; partial |r|^2.
+// Most iterations ask for none, so the flags are read first here.
 template<int BS>
 __global__ __launch_bounds__(kMaxTile) void kb_true_residual(Ptrs P, const double* __restrict__ b,
                                                              const double* __restrict__ x, double* __restrict__ r)
 {
-    const RowRef rr = row_of(P);
+    const int q = static_cast<int>(blockIdx.x);
+    if (q >= *P.live_count) return;
+    const RowRef rr = row_of(P.live[q]);
     const SysState& st = P.st[rr.sys];
     if (st.phase != 1 || !st.need_true) return;
     double red[1] = {0.0};
@@ -426,7 +533,7 @@ __global__ __launch_bounds__(kMaxTile) void kb_true_residual(Ptrs P, const doubl
             red[0] = __builtin_fma(ri, ri, red[0]);
         }
     }
-    store_tile_partials<1>(red, P);
+    store_tile_partials<1>(red, P, q);
 }
 
 // ------------------------------------------------------------- finishers --
@@ -436,12 +543,16 @@ struct Sums {
     double v[kPartials];
 };
 
-template<int K>
+// The initial kernel's partials sit at the system's tiles, the iteration
+// kernels' at the system's entries of the running tiles' list: the same
+// tiles in the same order.
+template<int K, bool INIT>
 __device__ inline Sums system_sums(const Ptrs& P, const std::int32_t* sys_blk0, int sys)
 {
     Sums out;
     const int lane = threadIdx.x;
-    const std::int32_t b0 = sys_blk0[sys], b1 = sys_blk0[sys + 1];
+    const std::int32_t tiles = sys_blk0[sys + 1] - sys_blk0[sys];
+    const std::int32_t b0 = INIT ? sys_blk0[sys] : P.live_first[sys], b1 = b0 + tiles;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double acc = 0.0;
@@ -610,9 +721,84 @@ __global__ __launch_bounds__(64) void kb_finish(Ptrs P, const std::int32_t* __re
         if (threadIdx.x == 0) stage_skip<STAGE>(st);
         return;
     }
-    const Sums sm = system_sums<stage_sums<STAGE>>(P, sys_blk0, sys);
+    const Sums sm = system_sums<stage_sums<STAGE>, STAGE == fin_init>(P, sys_blk0, sys);
     if (threadIdx.x != 0) return;
     finish_stage<STAGE>(st, sm);
+}
+
+// Exclusive prefix sum of one int per thread over a 256-thread block; `total`
+// receives the block's sum.  wave_tot: 4 ints of LDS.
+__device__ inline int block_exclusive_scan256(int v, int* wave_tot, int& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += u;
+    }
+    if (lane == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int t = wave_tot[w];
+        before += w < wave ? t : 0;
+        total += t;
+    }
+    __syncthreads();  // wave_tot is rewritten by the next call
+    return before + inc - v;
+}
+
+__device__ inline bool graph_running(const SysState& st)
+{
+    return st.owner == 0 && st.phase == 1;
+}
+
+// The tiles of the running graph-path systems, in tile order, and their
+// number (and each running system's first entry): the iteration kernels'
+// blocks take live[blockIdx.x], and the blocks
+// past the count return at once, so a system that stopped costs its tiles
+// nothing in later iterations.  The graph-path systems are ranked 0..G-1 in
+// system order (every rank has at least one tile).  Block b writes the
+// entries of tiles [256 b, 256 b + 256), which belong to at most 256
+// consecutive ranks: a tile's place is the running tiles of the ranks before
+// its own (each block scans the ranks' counts) plus its index in its system.
+// Launched with 256 threads and ceil(tiles / 256) blocks.
+__global__ __launch_bounds__(256) void kb_live(Ptrs P, const std::int32_t* __restrict__ blk_rank,
+                                               const std::int32_t* __restrict__ rank_sys,
+                                               const std::int32_t* __restrict__ rank_blk0, int ranks,
+                                               TileRef* __restrict__ live, std::int32_t* __restrict__ live_count,
+                                               std::int32_t* __restrict__ live_first)
+{
+    __shared__ int wave_tot[4];
+    __shared__ int first_place[256];  // running tiles before each rank of this block
+    const std::int64_t t0 = static_cast<std::int64_t>(blockIdx.x) * 256;
+    const std::int64_t t = t0 + threadIdx.x;
+    const std::int64_t t_end = t0 + 256 < P.blocks ? t0 + 256 : P.blocks;
+    const int r_lo = blk_rank[t0], r_hi = blk_rank[t_end - 1];  // r_hi - r_lo < 256
+    int carry = 0;
+    for (int c = 0; c < ranks; c += 256) {
+        const int r = c + static_cast<int>(threadIdx.x);
+        int cnt = 0;
+        if (r < ranks && graph_running(P.st[rank_sys[r]])) cnt = rank_blk0[r + 1] - rank_blk0[r];
+        int total = 0;
+        const int before = block_exclusive_scan256(cnt, wave_tot, total);
+        if (r >= r_lo && r <= r_hi) first_place[r - r_lo] = carry + before;
+        carry += total;
+    }
+    __syncthreads();
+    if (t < P.blocks) {
+        const int r = blk_rank[t];
+        const int sy = rank_sys[r];
+        if (graph_running(P.st[sy])) {
+            const int place = first_place[r - r_lo];
+            live[place + static_cast<int>(t - rank_blk0[r])] = P.refs[t];
+            if (t == rank_blk0[r]) live_first[sy] = place;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *live_count = carry;
 }
 
 __global__ void kb_count_active(const SysState* __restrict__ st, int systems, int* __restrict__ count)
@@ -841,8 +1027,7 @@ __global__ __launch_bounds__(NT) void kb_solve_group(Ptrs P, const std::int32_t*
 Ptrs make_ptrs(owtk_batch bt, const Tiling& tl)
 {
     Ptrs P{};
-    P.blk_sys = tl.blk_sys.get();
-    P.blk_row0 = tl.blk_row0.get();
+    P.refs = tl.refs.get();
     P.vprefix = bt->vprefix.get();
     P.pprefix = bt->pprefix.get();
     P.slice_ptr = bt->slice_ptr.get();
@@ -852,6 +1037,9 @@ Ptrs make_ptrs(owtk_batch bt, const Tiling& tl)
     P.dinv = bt->dinv.get();
     P.st = bt->state.get();
     P.partials = tl.partials.get();
+    P.live = tl.live.get();
+    P.live_count = tl.live_count.get();
+    P.live_first = tl.live_first.get();
     P.blocks = tl.blocks;
     P.bs = bt->bs;
     return P;
@@ -861,29 +1049,54 @@ std::unique_ptr<Tiling> make_tiling(owtk_batch bt, int tile)
 {
     auto tl = std::make_unique<Tiling>();
     tl->tile = tile;
-    std::vector<std::int32_t> bsys, sb0(bt->systems + 1, 0);
-    std::vector<std::int64_t> brow;
+    std::vector<std::int32_t> brank, rsys, rb0, sb0(bt->systems + 1, 0);
+    std::vector<TileRef> bref;
     for (int s = 0; s < bt->systems; ++s) {
-        sb0[s] = static_cast<std::int32_t>(bsys.size());
+        sb0[s] = static_cast<std::int32_t>(bref.size());
         if (bt->h_owner[s] != 0) continue;  // solved by one workgroup
+        const int rank = static_cast<int>(rsys.size());
+        rsys.push_back(s);
+        rb0.push_back(static_cast<std::int32_t>(bref.size()));
         const std::int64_t n = bt->h_vprefix[s + 1] - bt->h_vprefix[s];
         for (std::int64_t r0 = 0; r0 < n; r0 += tile) {
-            bsys.push_back(s);
-            brow.push_back(bt->h_pprefix[s] + r0);
+            TileRef t{};
+            t.sys = s;
+            t.prow0 = static_cast<std::int32_t>(bt->h_pprefix[s] + r0);
+            t.vshift = static_cast<std::int32_t>(bt->h_vprefix[s] - bt->h_pprefix[s]);
+            t.valid = static_cast<std::int32_t>(std::min<std::int64_t>(tile, n - r0));
+            bref.push_back(t);
+            brank.push_back(rank);
         }
     }
-    sb0[bt->systems] = static_cast<std::int32_t>(bsys.size());
-    if (bsys.size() > 0x7fffffffULL) throw std::invalid_argument("batch too large");
-    tl->blocks = static_cast<std::int64_t>(bsys.size());
+    sb0[bt->systems] = static_cast<std::int32_t>(bref.size());
+    rb0.push_back(static_cast<std::int32_t>(bref.size()));
+    tl->blocks = static_cast<std::int64_t>(bref.size());
     tl->sys_blk0.allocate(sb0.size());
     OWTK_HIP(hipMemcpy(tl->sys_blk0.get(), sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
-    if (bsys.empty()) return tl;  // every system is solved by a workgroup
-    tl->blk_sys.allocate(bsys.size());
-    tl->blk_row0.allocate(brow.size());
-    tl->partials.allocate(kPartials * bsys.size());
-    OWTK_HIP(hipMemcpy(tl->blk_sys.get(), bsys.data(), bsys.size() * 4, hipMemcpyHostToDevice));
-    OWTK_HIP(hipMemcpy(tl->blk_row0.get(), brow.data(), brow.size() * 8, hipMemcpyHostToDevice));
+    if (bref.empty()) return tl;  // every system is solved by a workgroup
+    tl->refs.allocate(bref.size());
+    tl->partials.allocate(kPartials * bref.size());
+    tl->live.allocate(bref.size());
+    tl->live_count.allocate(1);
+    tl->live_first.allocate(bt->systems);
+    tl->ranks = static_cast<int>(rsys.size());
+    tl->blk_rank.allocate(brank.size());
+    tl->rank_sys.allocate(rsys.size());
+    tl->rank_blk0.allocate(rb0.size());
+    OWTK_HIP(hipMemcpy(tl->blk_rank.get(), brank.data(), brank.size() * 4, hipMemcpyHostToDevice));
+    OWTK_HIP(hipMemcpy(tl->rank_sys.get(), rsys.data(), rsys.size() * 4, hipMemcpyHostToDevice));
+    OWTK_HIP(hipMemcpy(tl->rank_blk0.get(), rb0.data(), rb0.size() * 4, hipMemcpyHostToDevice));
+    OWTK_HIP(hipMemcpy(tl->refs.get(), bref.data(), bref.size() * sizeof(TileRef), hipMemcpyHostToDevice));
     return tl;
+}
+
+// Rebuild the running systems' tile list from the current states.
+void launch_live(owtk_batch bt, Tiling& tl)
+{
+    const unsigned int grid = static_cast<unsigned int>((tl.blocks + 255) / 256);
+    kb_live<<<grid, 256, 0, bt->ctx->stream>>>(make_ptrs(bt, tl), tl.blk_rank.get(), tl.rank_sys.get(),
+                                               tl.rank_blk0.get(), tl.ranks, tl.live.get(), tl.live_count.get(),
+                                               tl.live_first.get());
 }
 
 template<int BS>
@@ -907,6 +1120,7 @@ void launch_iteration(owtk_batch bt, Tiling& tl, const double* b, double* x)
     kb_finish<fin_rho><<<S, 64, 0, st>>>(P, sb0);
     kb_true_residual<BS><<<grid, T, 0, st>>>(P, b, x, bt->r.get());
     kb_finish<fin_true><<<S, 64, 0, st>>>(P, sb0);
+    launch_live(bt, tl);
     OWTK_LAUNCH_CHECK();
 }
 
@@ -928,6 +1142,7 @@ void launch_init(owtk_batch bt, Tiling& tl, const double* b, const double* x)
     kb_init<BS><<<static_cast<unsigned int>(tl.blocks), static_cast<unsigned int>(tl.tile), 0, st>>>(
         P, b, x, bt->r.get(), bt->rhat.get(), bt->p.get(), bt->v.get());
     kb_finish<fin_init><<<static_cast<unsigned int>(bt->systems), 64, 0, st>>>(P, tl.sys_blk0.get());
+    launch_live(bt, tl);
     OWTK_LAUNCH_CHECK();
 }
 
