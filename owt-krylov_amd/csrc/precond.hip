@@ -344,7 +344,9 @@ __global__ __launch_bounds__(256) void k_ilu_factor(TriArgs a, const std::int32_
 //   3. the chunk's own lower entries run in 64 steps: at step s row s is
 //      final, and every lane whose row has an entry in column s divides by
 //      row s's pivot and updates its targets from row s's entries (read from
-//      LDS), which keeps each row's column order;
+//      LDS), which keeps each row's column order.  With a bound NUB on the
+//      updates per op, a step's reads go out together and its updates take
+//      no branch (an ILU(0) factor of a triangulated grid has NUB = 3);
 //   4. the chunk's rows are stored, fenced for the next chunks' gathers.
 // Each row sees the same operations in the same order as the reference's
 // loop, so the factors are bitwise those of the level-scheduled kernel.
@@ -380,6 +382,7 @@ __device__ inline void copy_batched(int count, Load load, Store store)
     }
 }
 
+template<int NUB>
 __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const std::int64_t* __restrict__ offsets,
                                                           const std::int64_t* __restrict__ diag_pos,
                                                           const std::int32_t* __restrict__ row0,
@@ -458,24 +461,75 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
             }
         }
         // 3. the chunk's own lower entries, one pivot row per step
-        int jb = 0;
-        std::uint32_t d = nb > 0 ? opb[lane] : 0xffffffffu;
-        for (int s = 0; s < cnt; ++s) {
-            __builtin_amdgcn_wave_barrier();
-            if (d != 0xffffffffu && static_cast<int>(d & 255u) == s) {
-                const int es = static_cast<int>((d >> 8) & 255u), ps = static_cast<int>((d >> 16) & 255u);
-                const int nu = static_cast<int>(d >> 24);
-                const double pivot = rv[ps * 64 + s];
-                if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
-                const double lv = rv[es * 64 + lane] / pivot;
-                rv[es * 64 + lane] = lv;
-                for (int u = 0; u < nu; ++u) {
-                    const std::uint16_t w = upb[(jb * nub + u) * 64 + lane];
-                    const int ts = w & 255, qs = w >> 8;
-                    rv[ts * 64 + lane] = __builtin_fma(-lv, rv[qs * 64 + s], rv[ts * 64 + lane]);
+        if constexpr (NUB > 0) {
+            // At most NUB updates per op: an op's update words load with the
+            // op, and a step issues all its reads (pivot, the lower entry,
+            // row s's sources, its own targets) before its division, then
+            // its stores, with no branch per update (an update past the op's
+            // count reads a valid slot and writes the lane's trash slot).
+            // The targets are distinct slots of the lane's own row, neither
+            // the lower entry nor row s, so the values are those of the
+            // one-at-a-time order.
+            __shared__ double trash[64];
+            int jb = 0;
+            std::uint32_t d = nb > 0 ? opb[lane] : 0xffffffffu;
+            std::uint16_t w[NUB];
+#pragma unroll
+            for (int u = 0; u < NUB; ++u) w[u] = 0;
+            if (nb > 0 && nub > 0) {
+#pragma unroll
+                for (int u = 0; u < NUB; ++u) w[u] = upb[(u < nub ? u : nub - 1) * 64 + lane];
+            }
+            for (int s = 0; s < cnt; ++s) {
+                __builtin_amdgcn_wave_barrier();
+                if (d != 0xffffffffu && static_cast<int>(d & 255u) == s) {
+                    const int es = static_cast<int>((d >> 8) & 255u), ps = static_cast<int>((d >> 16) & 255u);
+                    const int nu = static_cast<int>(d >> 24);
+                    const double pivot = rv[ps * 64 + s];
+                    const double own = rv[es * 64 + lane];
+                    double src[NUB], tgt[NUB];
+                    double* dst[NUB];
+#pragma unroll
+                    for (int u = 0; u < NUB; ++u) {
+                        const bool on = u < nu;
+                        dst[u] = on ? &rv[(w[u] & 255) * 64 + lane] : &trash[lane];
+                        src[u] = rv[(on ? (w[u] >> 8) : 0) * 64 + s];
+                        tgt[u] = *dst[u];
+                    }
+                    if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+                    const double lv = own / pivot;
+                    rv[es * 64 + lane] = lv;
+#pragma unroll
+                    for (int u = 0; u < NUB; ++u) *dst[u] = __builtin_fma(-lv, src[u], tgt[u]);
+                    ++jb;
+                    const int jn = jb < nb ? jb : nb - 1;
+                    d = jb < nb ? opb[jn * 64 + lane] : 0xffffffffu;
+                    if (nub > 0) {
+#pragma unroll
+                        for (int u = 0; u < NUB; ++u) w[u] = upb[(jn * nub + (u < nub ? u : nub - 1)) * 64 + lane];
+                    }
                 }
-                ++jb;
-                d = jb < nb ? opb[jb * 64 + lane] : 0xffffffffu;
+            }
+        } else {
+            int jb = 0;
+            std::uint32_t d = nb > 0 ? opb[lane] : 0xffffffffu;
+            for (int s = 0; s < cnt; ++s) {
+                __builtin_amdgcn_wave_barrier();
+                if (d != 0xffffffffu && static_cast<int>(d & 255u) == s) {
+                    const int es = static_cast<int>((d >> 8) & 255u), ps = static_cast<int>((d >> 16) & 255u);
+                    const int nu = static_cast<int>(d >> 24);
+                    const double pivot = rv[ps * 64 + s];
+                    if (__builtin_fabs(pivot) <= kTol) atomicExch(flag, static_cast<int>(flag_zero_pivot));
+                    const double lv = rv[es * 64 + lane] / pivot;
+                    rv[es * 64 + lane] = lv;
+                    for (int u = 0; u < nu; ++u) {
+                        const std::uint16_t w = upb[(jb * nub + u) * 64 + lane];
+                        const int ts = w & 255, qs = w >> 8;
+                        rv[ts * 64 + lane] = __builtin_fma(-lv, rv[qs * 64 + s], rv[ts * 64 + lane]);
+                    }
+                    ++jb;
+                    d = jb < nb ? opb[jb * 64 + lane] : 0xffffffffu;
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1938,9 +1992,15 @@ void ilu_numeric(owtk_precond pc)
         const IluChunkArgs args{P.chunk_first.get(), P.hdr.get(), P.src_a.get(), P.op_a.get(), P.op_b.get(),
                                 P.upd_a.get(), P.upd_b.get(), P.kmax, P.max_src, P.max_a, P.max_nua, P.max_b,
                                 P.max_nub};
-        k_ilu_factor_chunks<<<groups, 64, ilu_chunk_lds(P), st>>>(args, f.offsets.get(), f.diag_pos.get(),
-                                                                  t.row0.get(), t.nrows.get(), bs, f.values.get(),
-                                                                  f.nnz, t.rows, t.dval64.get(), pc->flag.get());
+        // ILU(0) on a triangulated grid has at most 3 updates per op
+        auto launch = [&](auto kernel) {
+            kernel<<<groups, 64, ilu_chunk_lds(P), st>>>(args, f.offsets.get(), f.diag_pos.get(), t.row0.get(),
+                                                         t.nrows.get(), bs, f.values.get(), f.nnz, t.rows,
+                                                         t.dval64.get(), pc->flag.get());
+        };
+        if (P.max_nub <= 3) launch(k_ilu_factor_chunks<3>);
+        else if (P.max_nub <= 6) launch(k_ilu_factor_chunks<6>);
+        else launch(k_ilu_factor_chunks<0>);
     } else {
         k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), t.frow_lvl.get(), f.offsets.get(), f.diag_pos.get(),
                                             f.upd_ptr.get(), f.upd_target.get(), f.upd_upper.get(), f.columns.get(),
