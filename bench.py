@@ -420,10 +420,14 @@ def batched_bench(args):
         # every system on the graph replays, then the autotuned split between
         # one-workgroup solves of the small systems and the graph replays
         graph_ms, _ = timed("graph replays only")
-        limit, threads, tune_ms = bt.autotune_groups(b, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+        if args.group_limit >= 0:
+            limit, threads = args.group_limit, 1024
+            bt.set_group_limit(limit, threads)
+        else:
+            limit, threads, tune_ms = bt.autotune_groups(b, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
+            log(f"[bench] group autotuner: {tune_ms:.1f} ms per solve")
         group_systems = sum(s[0] <= limit for s in systems)
-        log(f"[bench] group autotuner: systems of <= {limit} vertices on one workgroup of {threads} threads "
-            f"({group_systems} systems), {tune_ms:.1f} ms per solve")
+        log(f"[bench] systems of <= {limit} vertices on one workgroup of {threads} threads ({group_systems} systems)")
         ms, last = timed(f"group limit {limit}")
         stream = stream_ceiling(ctx)
         stop = np.array([r["stop_ms"] for r in last])
@@ -771,6 +775,9 @@ def main():
                     help="Gram-Schmidt of the inner FGMRES: 0 iterated CGS, 1 adaptive, 2 one-synchronization "
                          "(GmresOrthogonalization, core.hpp:32-39); inner products stay FP64")
     ap.add_argument("--model", default=str(MODEL), help="learned preconditioner parameters (.npz)")
+    ap.add_argument("--group-limit", type=int, default=-1,
+                    help="batched workload: systems of at most this many vertices on one workgroup each "
+                         "(0: none; default: autotuned)")
     ap.add_argument("--workload", choices=["headline", "batched", "bsr", "pipelined", "multigrid", "recycling"],
                     default="headline")
     args = ap.parse_args()

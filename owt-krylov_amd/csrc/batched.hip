@@ -1332,6 +1332,8 @@ void solve(owtk_batch bt, const double* b, double* x, const owtk_solver_options*
 // rebuild the tiling of the graph systems.
 void set_groups(owtk_batch bt, std::int64_t limit, int threads)
 {
+    // the same split: the tiling (and its captured graph) stays
+    if (bt->tiling && limit == bt->group_limit && (limit == 0 || threads == bt->group_threads)) return;
     bt->ctx->sync();
     std::vector<int> owner(bt->systems, 0);
     std::vector<std::int32_t> order;
