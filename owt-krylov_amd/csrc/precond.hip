@@ -382,7 +382,63 @@ __device__ inline void copy_batched(int count, Load load, Store store)
     }
 }
 
-template<int NUB>
+// A chunk's header, row extents, program words and row values held in
+// registers (the prefetching kernel loads chunk ch + 1's during chunk ch).
+// Its caps bound the chunk's row length and program sizes.
+constexpr int kPfRows = 16, kPfOps = 16, kPfUpdates = 48;
+struct ChunkPrefetch {
+    std::int64_t row_first, src_off, base;
+    int cnt, nsrc, na, nua, nb, nub, len;
+    double vd[kPfRows];
+    std::uint32_t v32[kPfOps];      // op words: earlier-chunk ops, then this chunk's
+    std::uint16_t v16[kPfUpdates];  // update words, likewise
+};
+
+// Header, row extents and program words of chunk ch (no row values).
+__device__ inline void prefetch_program(const IluChunkArgs& P, const std::int64_t* __restrict__ offsets, int ch,
+                                        int lane, ChunkPrefetch& n)
+{
+    const std::int64_t* h = P.hdr + static_cast<std::int64_t>(ch) * 12;
+    n.row_first = h[0];
+    n.cnt = static_cast<int>(h[1]);
+    n.src_off = h[2];
+    n.nsrc = static_cast<int>(h[3]);
+    const std::int64_t opa_off = h[4], upa_off = h[6], opb_off = h[8], upb_off = h[10];
+    n.na = static_cast<int>(h[5]);
+    n.nua = static_cast<int>(h[7]);
+    n.nb = static_cast<int>(h[9]);
+    n.nub = static_cast<int>(h[11]);
+    n.base = 0;
+    n.len = 0;
+    if (lane < n.cnt) {
+        n.base = offsets[n.row_first + lane];
+        n.len = static_cast<int>(offsets[n.row_first + lane + 1] - n.base);
+    }
+    const int n32 = n.na + n.nb, na16 = n.na * n.nua, n16 = na16 + n.nb * n.nub;
+#pragma unroll
+    for (int j = 0; j < kPfOps; ++j)
+        if (j < n32) n.v32[j] = j < n.na ? P.op_a[opa_off + j * 64 + lane] : P.op_b[opb_off + (j - n.na) * 64 + lane];
+#pragma unroll
+    for (int j = 0; j < kPfUpdates; ++j)
+        if (j < n16) n.v16[j] = j < na16 ? P.upd_a[upa_off + j * 64 + lane] : P.upd_b[upb_off + (j - na16) * 64 + lane];
+}
+
+// Row values of a prefetched chunk (after its row extents have arrived).
+__device__ inline void prefetch_rows(const double* __restrict__ Fc, int kmax, ChunkPrefetch& n)
+{
+    const int lastq = n.len > 0 ? n.len - 1 : 0;
+#pragma unroll
+    for (int q = 0; q < kPfRows; ++q)
+        if (q < kmax) n.vd[q] = Fc[n.base + (q < lastq ? q : lastq)];
+}
+
+inline bool ilu_chunks_prefetchable(const IluChunkProgram& P)
+{
+    return P.kmax <= kPfRows && P.max_a + P.max_b <= kPfOps
+           && P.max_a * P.max_nua + P.max_b * P.max_nub <= kPfUpdates;
+}
+
+template<int NUB, bool PF>
 __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const std::int64_t* __restrict__ offsets,
                                                           const std::int64_t* __restrict__ diag_pos,
                                                           const std::int32_t* __restrict__ row0,
@@ -401,17 +457,47 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
     std::uint32_t* opb = opa + P.max_a * 64;              // [max_b][64]
     auto* upa = reinterpret_cast<std::uint16_t*>(opb + P.max_b * 64);  // [max_a*max_nua][64]
     std::uint16_t* upb = upa + P.max_a * P.max_nua * 64;  // [max_b*max_nub][64]
-    for (int ch = P.chunk_first[b]; ch < P.chunk_first[b + 1]; ++ch) {
+    const int ch_end = P.chunk_first[b + 1];
+    ChunkPrefetch cur, nxt;
+    if constexpr (PF) {
+        if (P.chunk_first[b] < ch_end) {
+            prefetch_program(P, offsets, P.chunk_first[b], lane, cur);
+            prefetch_rows(Fc, P.kmax, cur);
+        }
+    }
+    for (int ch = P.chunk_first[b]; ch < ch_end; ++ch) {
         const std::int64_t* h = P.hdr + static_cast<std::int64_t>(ch) * 12;
-        const std::int64_t row_first = h[0];
-        const int cnt = static_cast<int>(h[1]);
-        const std::int64_t src_off = h[2], opa_off = h[4], upa_off = h[6], opb_off = h[8], upb_off = h[10];
-        const int nsrc = static_cast<int>(h[3]), na = static_cast<int>(h[5]), nua = static_cast<int>(h[7]);
-        const int nb = static_cast<int>(h[9]), nub = static_cast<int>(h[11]);
-        const bool active = lane < cnt;
+        const int cnt = PF ? cur.cnt : static_cast<int>(h[1]);
+        const std::int64_t src_off = PF ? cur.src_off : h[2];
+        const int nsrc = PF ? cur.nsrc : static_cast<int>(h[3]);
+        const int na = PF ? cur.na : static_cast<int>(h[5]), nua = PF ? cur.nua : static_cast<int>(h[7]);
+        const int nb = PF ? cur.nb : static_cast<int>(h[9]), nub = PF ? cur.nub : static_cast<int>(h[11]);
         std::int64_t base = 0;
         int len = 0;
-        if (active) {
+        if constexpr (PF) {
+            // 1. the prefetched row values and program into LDS; then chunk
+            //    ch + 1's header, row extents and program start loading
+            base = cur.base;
+            len = cur.len;
+#pragma unroll
+            for (int q = 0; q < kPfRows; ++q)
+                if (q < P.kmax) rv[q * 64 + lane] = q < len ? cur.vd[q] : 0.0;
+            const int na16 = na * nua;
+#pragma unroll
+            for (int j = 0; j < kPfOps; ++j) {
+                if (j < na) opa[j * 64 + lane] = cur.v32[j];
+                else if (j < na + nb) opb[(j - na) * 64 + lane] = cur.v32[j];
+            }
+#pragma unroll
+            for (int j = 0; j < kPfUpdates; ++j) {
+                if (j < na16) upa[j * 64 + lane] = cur.v16[j];
+                else if (j < na16 + nb * nub) upb[(j - na16) * 64 + lane] = cur.v16[j];
+            }
+            if (ch + 1 < ch_end) prefetch_program(P, offsets, ch + 1, lane, nxt);
+        } else {
+        const std::int64_t row_first = h[0];
+        const std::int64_t opa_off = h[4], upa_off = h[6], opb_off = h[8], upb_off = h[10];
+        if (lane < cnt) {
             base = offsets[row_first + lane];
             len = static_cast<int>(offsets[row_first + lane + 1] - base);
         }
@@ -429,6 +515,7 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
                                     [&](int j, std::uint32_t v) { opb[j * 64 + lane] = v; });
         copy_batched<std::uint16_t>(nb * nub, [&](int j) { return P.upd_b[upb_off + j * 64 + lane]; },
                                     [&](int j, std::uint16_t v) { upb[j * 64 + lane] = v; });
+        }
         // 2. earlier chunks: gather (source indices, then values, each batch
         //    in flight together), then the lower entries in column order
         for (int t0 = 0; t0 < nsrc; t0 += 8) {
@@ -444,6 +531,12 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (t0 + u < nsrc) ga[(t0 + u) * 64 + lane] = idx[u] >= 0 ? v[u] : 0.0;
+        }
+        // chunk ch + 1's row values (their extents have arrived by now) load
+        // during this chunk's LDS work; no row of chunk ch + 1 is written
+        // before its own store
+        if constexpr (PF) {
+            if (ch + 1 < ch_end) prefetch_rows(Fc, P.kmax, nxt);
         }
         for (int j = 0; j < na; ++j) {
             const std::uint32_t d = opa[j * 64 + lane];
@@ -537,6 +630,7 @@ __global__ __launch_bounds__(64) void k_ilu_factor_chunks(IluChunkArgs P, const 
         for (int q = 0; q < len; ++q) Fc[base + q] = rv[q * 64 + lane];
         __threadfence_block();
         __builtin_amdgcn_wave_barrier();
+        if constexpr (PF) cur = nxt;
     }
     const std::int64_t r0 = row0[b], n = nrows[b];
     for (std::int64_t i = lane; i < n; i += 64) {
@@ -1998,9 +2092,10 @@ void ilu_numeric(owtk_precond pc)
                                                          t.nrows.get(), bs, f.values.get(), f.nnz, t.rows,
                                                          t.dval64.get(), pc->flag.get());
         };
-        if (P.max_nub <= 3) launch(k_ilu_factor_chunks<3>);
-        else if (P.max_nub <= 6) launch(k_ilu_factor_chunks<6>);
-        else launch(k_ilu_factor_chunks<0>);
+        const bool pf = ilu_chunks_prefetchable(P);
+        if (P.max_nub <= 3) pf ? launch(k_ilu_factor_chunks<3, true>) : launch(k_ilu_factor_chunks<3, false>);
+        else if (P.max_nub <= 6) pf ? launch(k_ilu_factor_chunks<6, true>) : launch(k_ilu_factor_chunks<6, false>);
+        else pf ? launch(k_ilu_factor_chunks<0, true>) : launch(k_ilu_factor_chunks<0, false>);
     } else {
         k_ilu_factor<<<groups, 64, 0, st>>>(tri_args(t), t.frow_lvl.get(), f.offsets.get(), f.diag_pos.get(),
                                             f.upd_ptr.get(), f.upd_target.get(), f.upd_upper.get(), f.columns.get(),
