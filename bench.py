@@ -40,7 +40,8 @@ RTOL = 1e-8
 MODEL = ROOT / "owt-krylov_amd" / "owtk" / "models" / "gnn_uniform.npz"
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # `--kernel` loops (FETCH_SIZE scaled by 2: it reads half of a plain copy
-# kernel's known input; WRITE_SIZE by 1), c1 workload.
+# kernel's known input; WRITE_SIZE by 1), on the 2048x1024 grid (2,097,152
+# vertices, block 3).
 TRAFFIC_BYTES = {("spmv", "f64"): 513_325_466, ("learned", "bf16"): 261_427_405}
 KERNEL_NAMES = {
     "learned_bf16": "learned series step, BF16 column-scaled off-diagonal values: SpMV + fused update "

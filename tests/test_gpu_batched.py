@@ -213,3 +213,28 @@ def test_group_autotune(ctx, small_batch):
     results = bt.bicgstab(b, x, opts)
     assert all(r["status_name"] == "converged" for r in results)
     print("group limit", limit, "threads", threads, f"{ms:.2f} ms")
+
+
+@pytest.mark.parametrize("block", [1, 2, 4])
+def test_group_and_graph_paths_other_block_sizes(ctx, block):
+    """Block sizes 1, 2 and 4: the one-workgroup solves and the graph path
+    both converge every system to 1e-8 (true residual recomputed on the host)
+    and agree on the solutions."""
+    systems, rhs, grids = workloads.batched_systems(count=6, nx_min=8, nx_max=48, block=block, seed0=7100)
+    b = ctx.upload(rhs)
+    opts = owtk.SolverOptions(relative_tolerance=1e-8)
+    xs = {}
+    for name, lim in (("graph", 0), ("groups", 1 << 20)):
+        bt = owtk.Batch(ctx, block, systems)
+        bt.set_group_limit(lim, 256)
+        x = ctx.empty(rhs.size)
+        res = bt.bicgstab(b, x, opts, graph_iterations=4)
+        assert all(r["status_name"] == "converged" for r in res), (name, [r["status_name"] for r in res])
+        xs[name] = x.download()
+        off = 0
+        for (n, o, c, v) in systems:
+            om = orc.OMatrix.csr(n, 0, block, o, c, v)
+            r = rhs[off:off + n * block] - om.spmv(xs[name][off:off + n * block])[:n * block]
+            assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(rhs[off:off + n * block]), (name, n)
+            off += n * block
+    assert np.linalg.norm(xs["groups"] - xs["graph"]) <= 1e-6 * np.linalg.norm(xs["graph"])
