@@ -390,6 +390,12 @@ def batched_bench(args):
 
     t0 = time.perf_counter()
     systems, rhs, grids = workloads.batched_systems(BATCH_SYSTEMS)
+    if args.batch_max_vertices > 0:  # only the systems of at most this many vertices
+        keep = [k for k, s in enumerate(systems) if s[0] <= args.batch_max_vertices]
+        offs = np.concatenate([[0], np.cumsum([s[0] * BS for s in systems])])
+        rhs = np.concatenate([rhs[offs[k]:offs[k + 1]] for k in keep])
+        systems = [systems[k] for k in keep]
+        grids = [grids[k] for k in keep]
     vertices = sum(s[0] for s in systems)
     log(f"[bench] batched: {len(systems)} systems, {vertices} vertices, generated in {time.perf_counter() - t0:.1f}s")
     opts = owtk.SolverOptions(relative_tolerance=RTOL, maximum_iterations=20000)
@@ -420,16 +426,20 @@ def batched_bench(args):
 
         # every system on the graph replays, then the autotuned split between
         # one-workgroup solves of the small systems and the graph replays
-        graph_ms, _ = timed("graph replays only")
         if args.group_limit >= 0:
+            # a fixed split, set before any solve
             limit, threads = args.group_limit, 1024
             bt.set_group_limit(limit, threads)
+            graph_ms = None
         else:
+            graph_ms, _ = timed("graph replays only")
             limit, threads, tune_ms = bt.autotune_groups(b, opts, graph_iterations=BATCH_GRAPH_ITERATIONS)
             log(f"[bench] group autotuner: {tune_ms:.1f} ms per solve")
         group_systems = sum(s[0] <= limit for s in systems)
         log(f"[bench] systems of <= {limit} vertices on one workgroup of {threads} threads ({group_systems} systems)")
         ms, last = timed(f"group limit {limit}")
+        if graph_ms is None and group_systems == 0:
+            graph_ms = ms
         stream = stream_ceiling(ctx)
         stop = np.array([r["stop_ms"] for r in last])
         its = np.array([r["iterations"] for r in last])
@@ -472,11 +482,13 @@ def batched_bench(args):
             "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"{len(systems)} systems, grids log-uniform 32x16..512x256, block 3, "
-                                   f"edge-contrast sigma U(0,1.5), seeds 3000+, {vertices} vertices in all",
+                                   f"edge-contrast sigma U(0,1.5), seeds 3000+, {vertices} vertices in all"
+                                   + (f" (the systems of at most {args.batch_max_vertices} vertices of the 512)"
+                                      if args.batch_max_vertices > 0 else ""),
                        "tile_rows": tile, "graph_iterations": BATCH_GRAPH_ITERATIONS,
                        "group_limit_vertices": limit, "group_threads": threads, "group_systems": group_systems},
-            "graph_only_ms": round(graph_ms, 3),
-            "graph_only_systems_per_s": round(len(systems) / (graph_ms / 1e3), 2),
+            "graph_only_ms": round(graph_ms, 3) if graph_ms is not None else None,
+            "graph_only_systems_per_s": round(len(systems) / (graph_ms / 1e3), 2) if graph_ms is not None else None,
             "per_system_ms": {"p50": float(np.percentile(stop, 50)), "p99": float(np.percentile(stop, 99)),
                               "max": float(stop.max())},
             "iterations": {"p50": float(np.median(its)), "max": int(its.max())},
@@ -776,6 +788,8 @@ def main():
                     help="Gram-Schmidt of the inner FGMRES: 0 iterated CGS, 1 adaptive, 2 one-synchronization "
                          "(GmresOrthogonalization, core.hpp:32-39); inner products stay FP64")
     ap.add_argument("--model", default=str(MODEL), help="learned preconditioner parameters (.npz)")
+    ap.add_argument("--batch-max-vertices", type=int, default=0,
+                    help="batched workload: keep only the systems of at most this many vertices (0: all 512)")
     ap.add_argument("--group-limit", type=int, default=-1,
                     help="batched workload: systems of at most this many vertices on one workgroup each "
                          "(0: none; default: autotuned)")
