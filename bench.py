@@ -5,6 +5,9 @@ BiCGSTAB + block-Jacobi ILU(0), plus the split SpMV's HBM bandwidth.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32|bf16]
     python bench.py --kernel spmv|copy|learned|bsr|learned-update --steps K   (kernel loops for rocprofv3)
     python bench.py --workload batched                   (512 uneven systems, batched BiCGSTAB)
+    python bench.py --workload batched [--batch-max-vertices N] [--group-limit L]
+                                                         (only the systems of <= N vertices; systems of
+                                                          <= L vertices on one workgroup each, default autotuned)
     python bench.py --workload bsr                       (dense 4x4 BSR, BF16 FGMRES(40)-IR + SSOR)
 
 One "step" is one full solve of the headline method from x0 = 0 on the
